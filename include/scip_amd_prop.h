@@ -1,0 +1,146 @@
+/* scip_amd_prop.h -- C ABI of the MI355X linear-constraint propagation engine.
+ *
+ * The engine replaces the activity and bound-tightening path of SCIP's linear constraint
+ * handler (src/scip/cons_linear.c: consdataCalcActivities :2283, getMinActivity/
+ * getMaxActivity :2345/:2440, consdataGetActivityResiduals :2661, tightenVarBoundsEasy
+ * :5380, tightenVarBounds :6700, tightenBounds :6980, propagateCons :7621, consPropLinear
+ * :16126) and the round loop of src/scip/solve.c propagateDomains (:723-795) with a
+ * device-resident, round-synchronous fixpoint.  It is called from the SCIP propagator
+ * plugin scip_amd/plugin/prop_gpulinear.c through SCIPincludePropBasic's PROPEXEC and
+ * SCIPsetPropPresol's PROPPRESOL callbacks (src/scip/scip_prop.c:118, :283).
+ *
+ * Conventions
+ *   - Return codes have the values of SCIP_RETCODE (src/scip/type_retcode.h:40-60):
+ *     SAP_OKAY = 1, errors <= 0.  No function aborts or throws across this ABI.
+ *   - All array arguments are plain host pointers owned by the caller unless the
+ *     function name says "device".  Nothing here allocates memory the caller must free,
+ *     except the opaque handles, which have matching *_free functions.
+ *   - One sap_context per SCIP instance (SCIP* is single-threaded); a context owns one
+ *     HIP stream.  Calls on different contexts may run concurrently.
+ *   - Arithmetic is IEEE FP64 throughout.
+ */
+#ifndef SCIP_AMD_PROP_H
+#define SCIP_AMD_PROP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes: same values as SCIP_RETCODE (type_retcode.h:42-60) */
+#define SAP_OKAY            1
+#define SAP_ERROR           0
+#define SAP_NOMEMORY       -1
+#define SAP_INVALIDCALL    -8
+#define SAP_INVALIDDATA    -9
+#define SAP_NOTIMPLEMENTED -18
+
+/* propagation verdicts: same values as SCIP_RESULT (type_result.h:42-59) */
+#define SAP_DIDNOTFIND      3
+#define SAP_CUTOFF          7
+#define SAP_REDUCEDDOM     10
+
+/* numerics and propagation parameters.  Defaults (sap_params_default) are SCIP's:
+ * numerics/{infinity,epsilon,sumepsilon,feastol,boundstreps,hugeval,recomputefac}
+ * (def.h:172-184), constraints/linear/maxeasyactivitydelta (cons_linear.c), and
+ * propagating/maxrounds = -1 (unlimited). */
+typedef struct sap_params
+{
+   double  infinity;
+   double  epsilon;
+   double  sumepsilon;
+   double  feastol;
+   double  boundstreps;
+   double  hugeval;
+   double  recompfac;
+   double  maxeasyactivitydelta;
+   int32_t maxrounds;             /* -1: until fixpoint */
+   int32_t flags;                 /* SAP_FLAG_* */
+} sap_params;
+
+/* the input bound vector(s) are already a propagation fixpoint except for the columns an
+ * overlay changes: only rows containing such columns are swept in the first round */
+#define SAP_FLAG_BASE_IS_FIXPOINT 0x1
+
+/* per-call statistics */
+typedef struct sap_stats
+{
+   int32_t result;       /* SAP_CUTOFF, SAP_REDUCEDDOM or SAP_DIDNOTFIND (single vector) */
+   int32_t rounds;       /* propagation rounds executed (max over vectors for batches) */
+   int64_t nchgbds;      /* accepted bound changes */
+   int64_t rows_swept;   /* rows swept over all rounds (and vectors) */
+   int64_t nnz_swept;    /* nonzeros swept over all rounds (and vectors) */
+   double  device_ms;    /* device time of the propagation kernels (HIP events) */
+   double  sweep_ms;     /* device time of the row-sweep kernels alone */
+   int64_t sweep_launches;
+} sap_stats;
+
+typedef struct sap_context sap_context;
+typedef struct sap_matrix  sap_matrix;
+typedef struct sap_bounds  sap_bounds;
+
+void sap_params_default(sap_params* params);
+const char* sap_error_string(int rc);
+
+/* number of visible HIP devices (0 if none); never fails */
+int sap_device_count(int32_t* ndevices);
+
+/* context = device + stream; one per SCIP instance (created in PROPINITSOL/INITPRE) */
+int sap_context_create(int32_t device, sap_context** ctx);
+int sap_context_free(sap_context* ctx);
+
+/* Upload a linear system lhs <= A x <= rhs in CSR form (rows = linear constraints of
+ * SCIPgetVarsLinear/SCIPgetValsLinear/SCIPgetLhsLinear/SCIPgetRhsLinear,
+ * cons_linear.c:18326-18436; columns = SCIPvarGetProbindex).  Column indices must be
+ * distinct within a row and coefficients nonzero.  integral[j] = SCIPvarIsIntegral;
+ * glb/gub are the global bounds, which decide SCIPvarIsBinary (var.c:23801) and the
+ * binary maxactdelta rule.  The matrix stays resident on the device until freed. */
+int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int64_t* rowptr,
+   const int32_t* colidx, const double* vals, const double* lhs, const double* rhs,
+   const uint8_t* integral, const double* glb, const double* gub, sap_matrix** mat);
+int sap_matrix_free(sap_matrix* mat);
+int sap_matrix_info(const sap_matrix* mat, int64_t* nrows, int64_t* ncols, int64_t* nnz,
+   int64_t* device_bytes);
+
+/* Propagate one bound vector to fixpoint (PROPEXEC path): lb/ub are read and overwritten
+ * with the propagated local bounds.  stats->result is SAP_CUTOFF when infeasibility was
+ * detected (bounds are then unspecified), else SAP_REDUCEDDOM or SAP_DIDNOTFIND. */
+int sap_propagate(sap_matrix* mat, const sap_params* params, double* lb, double* ub, sap_stats* stats);
+
+/* Device-resident bound vector(s), for callers that keep bounds in HBM across calls. */
+int sap_bounds_create(sap_matrix* mat, int32_t nvec, sap_bounds** bnd);
+int sap_bounds_free(sap_bounds* bnd);
+/* set vector v (or all vectors if v < 0) from host arrays */
+int sap_bounds_upload(sap_bounds* bnd, int32_t v, const double* lb, const double* ub);
+int sap_bounds_download(sap_bounds* bnd, int32_t v, double* lb, double* ub);
+/* device-to-device copy of vector sv of src into vector dv of dst (same matrix) */
+int sap_bounds_copy(sap_bounds* dst, int32_t dv, const sap_bounds* src, int32_t sv);
+/* propagate vector 0 of bnd in place; no host transfer of bounds */
+int sap_propagate_resident(sap_bounds* bnd, const sap_params* params, sap_stats* stats);
+
+/* Batched node propagation (many B&B node bound vectors on one matrix, one launch per
+ * round for all vectors).  Vector v starts from base_lb/base_ub with the overlay entries
+ * ovptr[v]..ovptr[v+1]-1 applied (column ovcol[q] gets bounds [ovlb[q], ovub[q]]).
+ * Per-vector verdicts go to result[v] (SAP_CUTOFF/SAP_REDUCEDDOM/SAP_DIDNOTFIND) and
+ * rounds[v]; if out_lb/out_ub are non-NULL they receive nvec*ncols propagated bounds
+ * (vector-major).  bnd must have been created with at least nvec vectors; it holds the
+ * propagated bounds on the device afterwards. */
+int sap_propagate_batch(sap_bounds* bnd, const sap_params* params, int32_t nvec, const double* base_lb,
+   const double* base_ub, const int64_t* ovptr, const int32_t* ovcol, const double* ovlb,
+   const double* ovub, int32_t* result, int32_t* rounds, double* out_lb, double* out_ub,
+   sap_stats* stats);
+
+/* Row activities on the given bounds with the engine's canonical double-double sums
+ * (diagnostic / parity): minact/maxact finite parts (hi+lo), counts[8*i..8*i+7] =
+ * {minposinf, minneginf, minposhuge, minneghuge, maxposinf, maxneginf, maxposhuge,
+ * maxneghuge}, maxactdelta as in consdataRecomputeMaxActivityDelta (cons_linear.c:1542).
+ * Any output pointer may be NULL. */
+int sap_row_activities(sap_matrix* mat, const sap_params* params, const double* lb, const double* ub,
+   double* minact, double* maxact, int32_t* counts, double* maxactdelta);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

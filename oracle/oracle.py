@@ -1,0 +1,174 @@
+"""ctypes binding of the oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REFLIB = os.path.join(HERE, "_ref", "libdbldbl_ref.so")
+
+
+class orc_params(C.Structure):
+    _fields_ = [
+        ("infinity", C.c_double), ("epsilon", C.c_double), ("sumepsilon", C.c_double), ("feastol", C.c_double),
+        ("boundstreps", C.c_double), ("hugeval", C.c_double), ("recompfac", C.c_double),
+        ("maxeasyactivitydelta", C.c_double), ("maxrounds", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class orc_matrix(C.Structure):
+    _fields_ = [
+        ("nrows", C.c_int64), ("ncols", C.c_int64), ("rowptr", C.POINTER(C.c_int64)),
+        ("colidx", C.POINTER(C.c_int32)), ("vals", C.POINTER(C.c_double)), ("lhs", C.POINTER(C.c_double)),
+        ("rhs", C.POINTER(C.c_double)), ("integral", C.POINTER(C.c_uint8)), ("binary", C.POINTER(C.c_uint8)),
+    ]
+
+
+class orc_result(C.Structure):
+    _fields_ = [("cutoff", C.c_int32), ("rounds", C.c_int32), ("nchgbds", C.c_int64), ("rows_swept", C.c_int64),
+                ("nnz_swept", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class orc_rowact(C.Structure):
+    _fields_ = [("minact", C.c_double), ("maxact", C.c_double), ("minact_hi", C.c_double), ("minact_lo", C.c_double),
+                ("maxact_hi", C.c_double), ("maxact_lo", C.c_double)] + \
+               [(n, C.c_int32) for n in ("minposinf", "minneginf", "minposhuge", "minneghuge", "maxposinf",
+                                         "maxneginf", "maxposhuge", "maxneghuge")] + [("maxactdelta", C.c_double)]
+
+
+_DP = C.POINTER(C.c_double)
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = C.CDLL(LIB)
+        L.orc_params_default.argtypes = [C.POINTER(orc_params)]
+        L.orc_propagate_jacobi.argtypes = [C.POINTER(orc_matrix), C.POINTER(orc_params), _DP, _DP, C.c_int,
+                                           C.POINTER(orc_result)]
+        L.orc_propagate_scip.argtypes = [C.POINTER(orc_matrix), C.POINTER(orc_params), _DP, _DP, C.POINTER(orc_result)]
+        L.orc_row_activities.argtypes = [C.POINTER(orc_matrix), C.POINTER(orc_params), _DP, _DP, C.c_int,
+                                         C.POINTER(orc_rowact)]
+        L.orc_compute_binary.argtypes = [C.c_int64, C.POINTER(C.c_uint8), _DP, _DP, C.POINTER(C.c_uint8)]
+        for n in ("orc_dd_sum", "orc_dd_prod"):
+            getattr(L, n).argtypes = [C.c_double, C.c_double, _DP, _DP]
+        for n in ("orc_dd_sum21", "orc_dd_prod21"):
+            getattr(L, n).argtypes = [C.c_double, C.c_double, C.c_double, _DP, _DP]
+        L.orc_dd_pairadd.argtypes = [C.c_double] * 4 + [_DP, _DP]
+        L.orc_canonical_sum.argtypes = [_DP, C.c_int64, _DP, _DP]
+        _lib = L
+    return _lib
+
+
+def reflib():
+    """The reference's own dbldblarith.h macros (None when the reference is absent)."""
+    global _ref
+    if _ref is None and os.path.exists(REFLIB):
+        R = C.CDLL(REFLIB)
+        for n in ("ref_dd_sum", "ref_dd_prod"):
+            getattr(R, n).argtypes = [C.c_double, C.c_double, _DP, _DP]
+        for n in ("ref_dd_sum21", "ref_dd_prod21"):
+            getattr(R, n).argtypes = [C.c_double, C.c_double, C.c_double, _DP, _DP]
+        R.ref_dd_sum22.argtypes = [C.c_double] * 4 + [_DP, _DP]
+        R.ref_activity_sum.argtypes = [_DP, C.c_long, _DP, _DP]
+        _ref = R
+    return _ref
+
+
+def params(**kw) -> orc_params:
+    p = orc_params()
+    lib().orc_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class OMatrix:
+    """Keeps the numpy arrays alive for the orc_matrix view."""
+
+    def __init__(self, rowptr, colidx, vals, lhs, rhs, integral, glb, gub):
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+        self.colidx = np.ascontiguousarray(colidx, dtype=np.int32)
+        self.vals = np.ascontiguousarray(vals, dtype=np.float64)
+        self.lhs = np.ascontiguousarray(lhs, dtype=np.float64)
+        self.rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        self.integral = np.ascontiguousarray(integral, dtype=np.uint8)
+        self.glb = np.ascontiguousarray(glb, dtype=np.float64)
+        self.gub = np.ascontiguousarray(gub, dtype=np.float64)
+        self.binary = np.zeros(len(self.glb), dtype=np.uint8)
+        lib().orc_compute_binary(len(self.glb), self.integral.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                 self.glb.ctypes.data_as(_DP), self.gub.ctypes.data_as(_DP),
+                                 self.binary.ctypes.data_as(C.POINTER(C.c_uint8)))
+        m = orc_matrix()
+        m.nrows = len(self.lhs)
+        m.ncols = len(self.glb)
+        m.rowptr = self.rowptr.ctypes.data_as(C.POINTER(C.c_int64))
+        m.colidx = self.colidx.ctypes.data_as(C.POINTER(C.c_int32))
+        m.vals = self.vals.ctypes.data_as(_DP)
+        m.lhs = self.lhs.ctypes.data_as(_DP)
+        m.rhs = self.rhs.ctypes.data_as(_DP)
+        m.integral = self.integral.ctypes.data_as(C.POINTER(C.c_uint8))
+        m.binary = self.binary.ctypes.data_as(C.POINTER(C.c_uint8))
+        self.m = m
+
+    @classmethod
+    def from_instance(cls, inst):
+        return cls(inst.rowptr, inst.colidx, inst.vals, inst.lhs, inst.rhs, inst.integral, inst.lb, inst.ub)
+
+
+def propagate_jacobi(om: OMatrix, lb, ub, prm: orc_params | None = None, full_sweep: bool = False):
+    lb = np.array(lb, dtype=np.float64, copy=True)
+    ub = np.array(ub, dtype=np.float64, copy=True)
+    r = orc_result()
+    lib().orc_propagate_jacobi(C.byref(om.m), C.byref(prm or params()), lb.ctypes.data_as(_DP),
+                               ub.ctypes.data_as(_DP), 1 if full_sweep else 0, C.byref(r))
+    return lb, ub, r.as_dict()
+
+
+def propagate_scip(om: OMatrix, lb, ub, prm: orc_params | None = None):
+    lb = np.array(lb, dtype=np.float64, copy=True)
+    ub = np.array(ub, dtype=np.float64, copy=True)
+    r = orc_result()
+    lib().orc_propagate_scip(C.byref(om.m), C.byref(prm or params()), lb.ctypes.data_as(_DP),
+                             ub.ctypes.data_as(_DP), C.byref(r))
+    return lb, ub, r.as_dict()
+
+
+def row_activities(om: OMatrix, lb, ub, canonical: bool = True, prm: orc_params | None = None):
+    lb = np.ascontiguousarray(lb, dtype=np.float64)
+    ub = np.ascontiguousarray(ub, dtype=np.float64)
+    nr = om.m.nrows
+    out = (orc_rowact * max(nr, 1))()
+    lib().orc_row_activities(C.byref(om.m), C.byref(prm or params()), lb.ctypes.data_as(_DP),
+                             ub.ctypes.data_as(_DP), 1 if canonical else 0, out)
+    mn = np.array([out[i].minact for i in range(nr)])
+    mx = np.array([out[i].maxact for i in range(nr)])
+    cnt = np.array([[out[i].minposinf, out[i].minneginf, out[i].minposhuge, out[i].minneghuge, out[i].maxposinf,
+                     out[i].maxneginf, out[i].maxposhuge, out[i].maxneghuge] for i in range(nr)], dtype=np.int32)
+    mad = np.array([out[i].maxactdelta for i in range(nr)])
+    return mn, mx, cnt.reshape(nr, 8), mad
+
+
+def dd2(fn, *args):
+    hi, lo = C.c_double(), C.c_double()
+    fn(*args, C.byref(hi), C.byref(lo))
+    return hi.value, lo.value
+
+
+def canonical_sum(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    hi, lo = C.c_double(), C.c_double()
+    lib().orc_canonical_sum(x.ctypes.data_as(_DP), len(x), C.byref(hi), C.byref(lo))
+    return hi.value, lo.value

@@ -1,0 +1,393 @@
+"""scip_amd -- MI355X linear-constraint domain propagation for SCIP.
+
+Thin ctypes binding of the C ABI declared in include/scip_amd_prop.h and
+include/scip_amd_gen.h (library: scip_amd/libscipamd.so, built by
+scip_amd/build.py).  This module is a convenience for tests and benchmarks; the
+product boundary is the C ABI, which the SCIP propagator plugin
+(scip_amd/plugin/prop_gpulinear.c) calls directly.
+
+There is no fallback: if the HIP library is missing, importing the binding
+raises, and every propagation call goes through the device engine.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libscipamd.so")
+
+SAP_OKAY = 1
+SAP_ERROR = 0
+SAP_NOMEMORY = -1
+SAP_INVALIDCALL = -8
+SAP_INVALIDDATA = -9
+SAP_NOTIMPLEMENTED = -18
+
+SAP_DIDNOTFIND = 3
+SAP_CUTOFF = 7
+SAP_REDUCEDDOM = 10
+
+SAP_FLAG_BASE_IS_FIXPOINT = 0x1
+
+INFINITY = 1e20
+
+
+class SapError(RuntimeError):
+    def __init__(self, rc: int, what: str):
+        self.rc = rc
+        super().__init__(f"{what} failed with code {rc}")
+
+
+class sap_params(C.Structure):
+    _fields_ = [
+        ("infinity", C.c_double),
+        ("epsilon", C.c_double),
+        ("sumepsilon", C.c_double),
+        ("feastol", C.c_double),
+        ("boundstreps", C.c_double),
+        ("hugeval", C.c_double),
+        ("recompfac", C.c_double),
+        ("maxeasyactivitydelta", C.c_double),
+        ("maxrounds", C.c_int32),
+        ("flags", C.c_int32),
+    ]
+
+
+class sap_stats(C.Structure):
+    _fields_ = [
+        ("result", C.c_int32),
+        ("rounds", C.c_int32),
+        ("nchgbds", C.c_int64),
+        ("rows_swept", C.c_int64),
+        ("nnz_swept", C.c_int64),
+        ("device_ms", C.c_double),
+        ("sweep_ms", C.c_double),
+        ("sweep_launches", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class sap_instance(C.Structure):
+    _fields_ = [
+        ("nrows", C.c_int64),
+        ("ncols", C.c_int64),
+        ("nnz", C.c_int64),
+        ("rowptr", C.POINTER(C.c_int64)),
+        ("colidx", C.POINTER(C.c_int32)),
+        ("vals", C.POINTER(C.c_double)),
+        ("lhs", C.POINTER(C.c_double)),
+        ("rhs", C.POINTER(C.c_double)),
+        ("integral", C.POINTER(C.c_uint8)),
+        ("lb", C.POINTER(C.c_double)),
+        ("ub", C.POINTER(C.c_double)),
+    ]
+
+
+_P = C.c_void_p
+_I64P = C.POINTER(C.c_int64)
+_I32P = C.POINTER(C.c_int32)
+_U8P = C.POINTER(C.c_uint8)
+_DP = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); every symbol of include/scip_amd_prop.h and scip_amd_gen.h
+EXPORTS = {
+    "sap_params_default": (None, [C.POINTER(sap_params)]),
+    "sap_error_string": (C.c_char_p, [C.c_int]),
+    "sap_device_count": (C.c_int, [_I32P]),
+    "sap_context_create": (C.c_int, [C.c_int32, C.POINTER(_P)]),
+    "sap_context_free": (C.c_int, [_P]),
+    "sap_matrix_create": (C.c_int, [_P, C.c_int64, C.c_int64, _I64P, _I32P, _DP, _DP, _DP, _U8P, _DP, _DP,
+                                    C.POINTER(_P)]),
+    "sap_matrix_free": (C.c_int, [_P]),
+    "sap_matrix_info": (C.c_int, [_P, _I64P, _I64P, _I64P, _I64P]),
+    "sap_propagate": (C.c_int, [_P, C.POINTER(sap_params), _DP, _DP, C.POINTER(sap_stats)]),
+    "sap_bounds_create": (C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
+    "sap_bounds_free": (C.c_int, [_P]),
+    "sap_bounds_upload": (C.c_int, [_P, C.c_int32, _DP, _DP]),
+    "sap_bounds_download": (C.c_int, [_P, C.c_int32, _DP, _DP]),
+    "sap_bounds_copy": (C.c_int, [_P, C.c_int32, _P, C.c_int32]),
+    "sap_propagate_resident": (C.c_int, [_P, C.POINTER(sap_params), C.POINTER(sap_stats)]),
+    "sap_propagate_batch": (C.c_int, [_P, C.POINTER(sap_params), C.c_int32, _DP, _DP, _I64P, _I32P, _DP, _DP,
+                                      _I32P, _I32P, _DP, _DP, C.POINTER(sap_stats)]),
+    "sap_row_activities": (C.c_int, [_P, C.POINTER(sap_params), _DP, _DP, _DP, _DP, _I32P, _DP]),
+    "sap_gen_covering_packing": (C.c_int, [C.c_int64, C.c_int64, C.c_int32, C.c_uint64, C.POINTER(sap_instance)]),
+    "sap_gen_mixed": (C.c_int, [C.c_int64, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                C.c_int32, C.c_uint64, C.POINTER(sap_instance)]),
+    "sap_gen_node_fixings": (C.c_int, [C.POINTER(sap_instance), C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
+                                       _I64P, _I32P, _DP]),
+    "sap_instance_free": (None, [C.POINTER(sap_instance)]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libscipamd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError(f"{LIBPATH} is missing: run scip_amd/build.py (or __graft_entry__.build())")
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in EXPORTS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _chk(rc: int, what: str):
+    if rc != SAP_OKAY:
+        raise SapError(rc, what)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP) if a is not None else None
+
+
+def _i64p(a):
+    return a.ctypes.data_as(_I64P) if a is not None else None
+
+
+def _i32p(a):
+    return a.ctypes.data_as(_I32P) if a is not None else None
+
+
+def _u8p(a):
+    return a.ctypes.data_as(_U8P) if a is not None else None
+
+
+def default_params(**kw) -> sap_params:
+    p = sap_params()
+    lib().sap_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    lib().sap_device_count(C.byref(n))
+    return n.value
+
+
+# ------------------------------------------------------------------------------------------
+# instances
+# ------------------------------------------------------------------------------------------
+@dataclass
+class Instance:
+    """Linear system lhs <= A x <= rhs (CSR) with column integrality and global bounds."""
+    rowptr: np.ndarray
+    colidx: np.ndarray
+    vals: np.ndarray
+    lhs: np.ndarray
+    rhs: np.ndarray
+    integral: np.ndarray
+    lb: np.ndarray
+    ub: np.ndarray
+
+    @property
+    def nrows(self):
+        return len(self.lhs)
+
+    @property
+    def ncols(self):
+        return len(self.lb)
+
+    @property
+    def nnz(self):
+        return int(self.rowptr[-1])
+
+
+def _from_c(inst: sap_instance) -> Instance:
+    nr, nc, nnz = inst.nrows, inst.ncols, inst.nnz
+
+    def arr(ptr, n, dt):
+        if n == 0:
+            return np.zeros(0, dtype=dt)
+        return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt, copy=True)
+
+    out = Instance(
+        rowptr=arr(inst.rowptr, nr + 1, np.int64),
+        colidx=arr(inst.colidx, nnz, np.int32),
+        vals=arr(inst.vals, nnz, np.float64),
+        lhs=arr(inst.lhs, nr, np.float64),
+        rhs=arr(inst.rhs, nr, np.float64),
+        integral=arr(inst.integral, nc, np.uint8),
+        lb=arr(inst.lb, nc, np.float64),
+        ub=arr(inst.ub, nc, np.float64),
+    )
+    lib().sap_instance_free(C.byref(inst))
+    return out
+
+
+def gen_covering_packing(nrows: int, ncols: int, nnz_per_row: int = 10, seed: int = 1) -> Instance:
+    inst = sap_instance()
+    _chk(lib().sap_gen_covering_packing(nrows, ncols, nnz_per_row, seed, C.byref(inst)), "sap_gen_covering_packing")
+    return _from_c(inst)
+
+
+def gen_mixed(nrows: int, ncols: int, seed: int = 2, lenmode: int = 0, median: float = 8.0, sigma: float = 1.0,
+              minlen: int = 1, maxlen: int = 10000, planted: bool = False) -> Instance:
+    """planted=False: sides centred on the activity span (the stated workload); planted=True:
+    sides centred on the activity of a hidden feasible point."""
+    inst = sap_instance()
+    _chk(lib().sap_gen_mixed(nrows, ncols, lenmode, median, sigma, minlen, maxlen, 1 if planted else 0, seed,
+                             C.byref(inst)), "sap_gen_mixed")
+    return _from_c(inst)
+
+
+def gen_node_fixings(inst: Instance, nnodes: int, minfix: int = 1, maxfix: int = 50, seed: int = 3):
+    """Returns (ovptr, ovcol, ovval) with lb = ub = ovval for each fixed column."""
+    ci = sap_instance()
+    ci.nrows, ci.ncols, ci.nnz = inst.nrows, inst.ncols, inst.nnz
+    ci.integral = _u8p(inst.integral)
+    ptr = np.zeros(nnodes + 1, dtype=np.int64)
+    col = np.zeros(max(nnodes * maxfix, 1), dtype=np.int32)
+    val = np.zeros(max(nnodes * maxfix, 1), dtype=np.float64)
+    _chk(lib().sap_gen_node_fixings(C.byref(ci), nnodes, minfix, maxfix, seed, _i64p(ptr), _i32p(col), _dp(val)),
+         "sap_gen_node_fixings")
+    n = int(ptr[-1])
+    return ptr, col[:n].copy(), val[:n].copy()
+
+
+# ------------------------------------------------------------------------------------------
+# engine
+# ------------------------------------------------------------------------------------------
+class Context:
+    def __init__(self, device: int = 0):
+        self.h = _P()
+        _chk(lib().sap_context_create(device, C.byref(self.h)), "sap_context_create")
+
+    def close(self):
+        if self.h:
+            lib().sap_context_free(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Matrix:
+    """Device-resident CSR matrix (sap_matrix)."""
+
+    def __init__(self, ctx: Context, inst: Instance):
+        self.ctx = ctx
+        self.inst = inst
+        self.h = _P()
+        self._keep = [np.ascontiguousarray(a) for a in (inst.rowptr, inst.colidx, inst.vals, inst.lhs, inst.rhs,
+                                                         inst.integral, inst.lb, inst.ub)]
+        rp, ci, va, lh, rh, it, lb, ub = self._keep
+        _chk(lib().sap_matrix_create(ctx.h, inst.nrows, inst.ncols, _i64p(rp), _i32p(ci), _dp(va), _dp(lh), _dp(rh),
+                                     _u8p(it), _dp(lb), _dp(ub), C.byref(self.h)), "sap_matrix_create")
+
+    def info(self):
+        a, b, c, d = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        _chk(lib().sap_matrix_info(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)), "sap_matrix_info")
+        return dict(nrows=a.value, ncols=b.value, nnz=c.value, device_bytes=d.value)
+
+    def propagate(self, lb=None, ub=None, params: sap_params | None = None):
+        """Propagate one bound vector; returns (lb, ub, stats dict)."""
+        lb = np.array(self.inst.lb if lb is None else lb, dtype=np.float64, copy=True)
+        ub = np.array(self.inst.ub if ub is None else ub, dtype=np.float64, copy=True)
+        st = sap_stats()
+        p = params or default_params()
+        _chk(lib().sap_propagate(self.h, C.byref(p), _dp(lb), _dp(ub), C.byref(st)), "sap_propagate")
+        return lb, ub, st.as_dict()
+
+    def row_activities(self, lb=None, ub=None, params: sap_params | None = None):
+        nr = self.inst.nrows
+        lb = np.ascontiguousarray(self.inst.lb if lb is None else lb, dtype=np.float64)
+        ub = np.ascontiguousarray(self.inst.ub if ub is None else ub, dtype=np.float64)
+        mn, mx, mad = np.zeros(nr), np.zeros(nr), np.zeros(nr)
+        cnt = np.zeros(8 * nr, dtype=np.int32)
+        p = params or default_params()
+        _chk(lib().sap_row_activities(self.h, C.byref(p), _dp(lb), _dp(ub), _dp(mn), _dp(mx), _i32p(cnt), _dp(mad)),
+             "sap_row_activities")
+        return mn, mx, cnt.reshape(nr, 8), mad
+
+    def close(self):
+        if self.h:
+            lib().sap_matrix_free(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Bounds:
+    """Device-resident bound vectors (sap_bounds)."""
+
+    def __init__(self, mat: Matrix, nvec: int = 1):
+        self.mat = mat
+        self.nvec = nvec
+        self.h = _P()
+        _chk(lib().sap_bounds_create(mat.h, nvec, C.byref(self.h)), "sap_bounds_create")
+
+    def upload(self, lb, ub, v: int = -1):
+        lb = np.ascontiguousarray(lb, dtype=np.float64)
+        ub = np.ascontiguousarray(ub, dtype=np.float64)
+        _chk(lib().sap_bounds_upload(self.h, v, _dp(lb), _dp(ub)), "sap_bounds_upload")
+
+    def download(self, v: int = 0):
+        n = self.mat.inst.ncols
+        lb, ub = np.zeros(n), np.zeros(n)
+        _chk(lib().sap_bounds_download(self.h, v, _dp(lb), _dp(ub)), "sap_bounds_download")
+        return lb, ub
+
+    def copy_from(self, src: "Bounds", dv: int = 0, sv: int = 0):
+        _chk(lib().sap_bounds_copy(self.h, dv, src.h, sv), "sap_bounds_copy")
+
+    def propagate(self, params: sap_params | None = None):
+        st = sap_stats()
+        p = params or default_params()
+        _chk(lib().sap_propagate_resident(self.h, C.byref(p), C.byref(st)), "sap_propagate_resident")
+        return st.as_dict()
+
+    def propagate_batch(self, base_lb, base_ub, ovptr, ovcol, ovlb, ovub, params: sap_params | None = None,
+                        want_bounds: bool = True, want_result: bool = True):
+        nvec = len(ovptr) - 1
+        n = self.mat.inst.ncols
+        base_lb = np.ascontiguousarray(base_lb, dtype=np.float64)
+        base_ub = np.ascontiguousarray(base_ub, dtype=np.float64)
+        ovptr = np.ascontiguousarray(ovptr, dtype=np.int64)
+        ovcol = np.ascontiguousarray(ovcol, dtype=np.int32)
+        ovlb = np.ascontiguousarray(ovlb, dtype=np.float64)
+        ovub = np.ascontiguousarray(ovub, dtype=np.float64)
+        res = np.zeros(nvec, dtype=np.int32) if want_result else None
+        rnd = np.zeros(nvec, dtype=np.int32)
+        olb = np.zeros(nvec * n) if want_bounds else None
+        oub = np.zeros(nvec * n) if want_bounds else None
+        st = sap_stats()
+        p = params or default_params()
+        _chk(lib().sap_propagate_batch(self.h, C.byref(p), nvec, _dp(base_lb), _dp(base_ub), _i64p(ovptr),
+                                       _i32p(ovcol), _dp(ovlb), _dp(ovub), _i32p(res), _i32p(rnd), _dp(olb), _dp(oub),
+                                       C.byref(st)), "sap_propagate_batch")
+        out = dict(result=res, rounds=rnd, stats=st.as_dict())
+        if want_bounds:
+            out["lb"] = olb.reshape(nvec, n)
+            out["ub"] = oub.reshape(nvec, n)
+        return out
+
+    def close(self):
+        if self.h:
+            lib().sap_bounds_free(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,57 @@
+"""Build scip_amd/libscipamd.so (HIP engine + C generators) for gfx950.
+
+Usage: python scip_amd/build.py
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+BUILD = os.path.join(ROOT, "_build")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# -ffp-contract=off: no FMA contraction, every FP64 operation is the written IEEE operation
+# (bit-identical to a host build); packed FP32 instructions are disabled for all HIP code.
+HIPFLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    f"-I{os.path.join(ROOT, 'include')}", f"-I{os.path.join(HERE, 'csrc')}",
+]
+CFLAGS = ["-O2", "-fPIC", "-ffp-contract=off", "-Wall", f"-I{os.path.join(ROOT, 'include')}"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l and l.strip())
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}\n{out}")
+    return out
+
+
+def build(verbose: bool = True) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    eng_o = os.path.join(BUILD, "engine.o")
+    gen_o = os.path.join(BUILD, "gen.o")
+    lib = os.path.join(HERE, "libscipamd.so")
+    steps = [
+        [HIPCC, *HIPFLAGS, "-c", os.path.join(HERE, "csrc", "engine.hip"), "-o", eng_o],
+        ["gcc", *CFLAGS, "-c", os.path.join(HERE, "csrc", "gen.c"), "-o", gen_o],
+        [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", eng_o, gen_o, "-o", lib + ".tmp"],
+    ]
+    for cmd in steps:
+        out = _run(cmd)
+        if verbose and out:
+            print(out)
+    os.replace(lib + ".tmp", lib)
+    if verbose:
+        print(f"built {lib}")
+    return lib
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

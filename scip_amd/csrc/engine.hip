@@ -1,0 +1,2128 @@
+// engine.hip -- MI355X (gfx950) linear-constraint domain propagation engine.
+//
+// Device algorithm (one propagation round, all bound vectors of a sap_bounds at once):
+//   k_round_begin : advance the round id, stop when the worklists are empty / cutoff /
+//                   maxrounds, reset the next round's counters.
+//   k_sweep       : one launch over all row-length bins.  Every worklist item (row, vector)
+//                   computes the row's min/max activity (canonical pairwise double-double
+//                   sum over row positions), infinity/huge counters and maxactdelta from the
+//                   round's bounds, then each nonzero computes its bound candidates exactly
+//                   as tightenVarBoundsEasy / tightenVarBounds (cons_linear.c:5380, :6700)
+//                   and reduces them into per-column candidate slots with 64-bit atomic
+//                   min/max on the order-preserving integer image of the FP64 value.  The
+//                   row's infeasibility / redundancy test (propagateCons, :7721-7753) runs
+//                   on the same state.
+//   k_columns     : every touched column applies SCIPinferVarLbCons / SCIPinferVarUbCons
+//                   (scip_var.c:6965-7174) to its tightest regular and forced candidates,
+//                   writes the new bounds and appends the rows of changed columns (CSC) to
+//                   the next round's per-bin worklists.
+//
+// Bins: rows of length <= 4/8/16/32/64 are swept by groups of that many lanes (one nonzero
+// per lane, several rows per wavefront); longer rows by one wavefront (64-element chunks);
+// the longest by a whole workgroup.  All tilings compute the same canonical tree, so the
+// result does not depend on the binning.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "devnum.h"
+#include "scip_amd_prop.h"
+
+using namespace sap;
+
+#define SAP_NBINS 7
+#define SAP_BLOCK 256
+#define SAP_WAVE 64
+#define SAP_MAXLEVELS 40
+#define SAP_ROUND_CAP 10000000
+
+// bins: 0..4 -> group of 4,8,16,32,64 lanes; 5 -> wavefront per row; 6 -> workgroup per row
+static const int32_t kBinMaxLen[SAP_NBINS - 1] = { 4, 8, 16, 32, 64, 4096 };
+
+struct Status
+{
+   int32_t round;       // global round id, monotone over the life of a sap_bounds
+   int32_t rlocal;      // rounds executed in the current call
+   int32_t done;
+   int32_t cutoff;      // single-vector mode: infeasibility detected
+   uint32_t wlcount[2][SAP_NBINS];
+   uint32_t touched;
+   uint32_t pad;
+   unsigned long long rows_swept;
+   unsigned long long nnz_swept;
+   unsigned long long nchg;
+};
+
+struct Mat
+{
+   int64_t nrows;
+   int64_t ncols;
+   const int64_t* rowptr;
+   const int32_t* colidx;
+   const double* vals;
+   const double2* sides;    // (lhs, rhs)
+   const uint8_t* colflags; // bit0 integral, bit1 binary
+   const uint8_t* rowflags; // bit0 all-binary row
+   const uint8_t* rowbin;
+   const int64_t* colptr;   // CSC for row marking
+   const int32_t* rowidx;
+};
+
+struct Vec
+{
+   int32_t nvec;
+   int32_t single;          // 1: one vector, a cutoff stops the whole call
+   int32_t hasforced;
+   int32_t pad;
+   double2* bnd;            // nvec*ncols (lb, ub)
+   long long* clb;          // regular lb candidates (max)
+   long long* cub;          // regular ub candidates (min)
+   long long* flb;          // forced lb candidates (single-variable rows)
+   long long* fub;
+   int32_t* coltag;         // last round a column got a candidate
+   int32_t* rowmark;        // last round a row was queued for
+   uint8_t* redundant;      // nvec*nrows
+   int32_t* vcutoff;        // nvec
+   int32_t* vrounds;        // nvec
+   int2* wl[2][SAP_NBINS];  // (row, vector) worklists, double-buffered by round parity
+   int2* touched;           // (column, vector)
+   Status* st;
+};
+
+struct Layout
+{
+   int32_t blockstart[SAP_NBINS + 1];
+};
+
+#define KEY_NEG_INF ((long long)0x800FFFFFFFFFFFFFLL)   // dkey(-inf)
+#define KEY_POS_INF ((long long)0x7FF0000000000000LL)   // dkey(+inf)
+
+// ------------------------------------------------------------------------------------------
+// row state
+// ------------------------------------------------------------------------------------------
+struct RowState
+{
+   DD minact;
+   DD maxact;
+   int32_t c[8];   // minposinf, minneginf, minposhuge, minneghuge, maxposinf, maxneginf, maxposhuge, maxneghuge
+   double mad;     // maxactdelta
+};
+
+// classification of a contribution a*b (consdataUpdateActivities from bound 0,
+// cons_linear.c:1773-1945): 0 finite, 1 +inf, 2 -inf, 3 +huge, 4 -huge
+__device__ __forceinline__ int classify(const Params& p, double a, double b, double& c)
+{
+   if( isInf(p, fabs(b)) )
+   {
+      c = 0.0;
+      return ((b > 0.0) == (a > 0.0)) ? 1 : 2;
+   }
+   double x = a * b;
+   if( isHuge(p, fabs(x)) )
+   {
+      c = 0.0;
+      return x > 0.0 ? 3 : 4;
+   }
+   c = x;
+   return 0;
+}
+
+// per-element contribution to the row state
+struct Elem
+{
+   double a;
+   double lb;
+   double ub;
+   int32_t j;
+   uint8_t cf;
+};
+
+__device__ __forceinline__ void elemContrib(const Params& p, const Elem& e, bool allbin, double& cmin, double& cmax,
+   int& kmin, int& kmax, double& mad, bool& madinf)
+{
+   kmin = classify(p, e.a, e.a > 0.0 ? e.lb : e.ub, cmin);
+   kmax = classify(p, e.a, e.a > 0.0 ? e.ub : e.lb, cmax);
+   if( allbin )
+   {
+      mad = (e.lb < 0.5 && e.ub > 0.5) ? fabs(e.a) : 0.0;
+      madinf = false;
+   }
+   else
+   {
+      madinf = isInf(p, -e.lb) || isInf(p, e.ub);
+      mad = madinf ? 0.0 : fabs(e.a) * (e.ub - e.lb);
+   }
+}
+
+// packed counters: 8 fields of 8 bits (rows of at most 64 nonzeros per group)
+__device__ __forceinline__ unsigned long long packKinds(int kmin, int kmax)
+{
+   unsigned long long v = 0;
+   if( kmin > 0 )
+      v |= 1ULL << (8 * (kmin - 1));
+   if( kmax > 0 )
+      v |= 1ULL << (8 * (kmax + 3));
+   return v;
+}
+
+__device__ __forceinline__ double shflXorD(double x, int m) { return __shfl_xor(x, m, SAP_WAVE); }
+__device__ __forceinline__ DD shflXorDD(DD x, int m)
+{
+   DD r;
+   r.hi = __shfl_xor(x.hi, m, SAP_WAVE);
+   r.lo = __shfl_xor(x.lo, m, SAP_WAVE);
+   return r;
+}
+__device__ __forceinline__ DD shflDD(DD x, int src)
+{
+   DD r;
+   r.hi = __shfl(x.hi, src, SAP_WAVE);
+   r.lo = __shfl(x.lo, src, SAP_WAVE);
+   return r;
+}
+
+// ------------------------------------------------------------------------------------------
+// activities (getMinActivity / getMaxActivity, cons_linear.c:2345-2529)
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void getMinAct(const Params& p, DD act, int pinf, int ninf, int phuge, int nhuge,
+   double delta, bool goodrelax, double& res, bool& tight, bool& setinf)
+{
+   if( ninf > 0 )
+   {
+      res = -p.infinity;
+      setinf = true;
+      tight = (pinf == 0);
+   }
+   else if( pinf > 0 )
+   {
+      res = p.infinity;
+      setinf = true;
+      tight = true;
+   }
+   else if( nhuge > 0 || (phuge > 0 && !goodrelax) )
+   {
+      res = -p.infinity;
+      setinf = true;
+      tight = false;
+   }
+   else
+   {
+      DD t = sum21(act, -delta);
+      if( phuge > 0 )
+      {
+         t = sum21(t, phuge * p.hugeval);
+         tight = false;
+      }
+      else
+         tight = true;
+      res = ddVal(t);
+      setinf = false;
+   }
+}
+
+__device__ __forceinline__ void getMaxAct(const Params& p, DD act, int pinf, int ninf, int phuge, int nhuge,
+   double delta, bool goodrelax, double& res, bool& tight, bool& setinf)
+{
+   if( pinf > 0 )
+   {
+      res = p.infinity;
+      setinf = true;
+      tight = (ninf == 0);
+   }
+   else if( ninf > 0 )
+   {
+      res = -p.infinity;
+      setinf = true;
+      tight = true;
+   }
+   else if( phuge > 0 || (nhuge > 0 && !goodrelax) )
+   {
+      res = p.infinity;
+      setinf = true;
+      tight = false;
+   }
+   else
+   {
+      DD t = sum21(act, -delta);
+      if( nhuge > 0 )
+      {
+         t = sum21(t, -nhuge * p.hugeval);
+         tight = false;
+      }
+      else
+         tight = true;
+      res = ddVal(t);
+      setinf = false;
+   }
+}
+
+// consdataGetActivityResiduals (cons_linear.c:2661-2806), goodrelax = FALSE
+__device__ __forceinline__ void residuals(const Params& p, const RowState& rs, double a, double lb, double ub,
+   double& minres, double& maxres, bool& mintight, bool& maxtight, bool& minsetinf, bool& maxsetinf)
+{
+   double minb, maxb, absval;
+   if( a > 0.0 )
+   {
+      minb = lb;
+      maxb = ub;
+      absval = a;
+   }
+   else
+   {
+      minb = -ub;
+      maxb = -lb;
+      absval = -a;
+   }
+   int c0 = rs.c[0], c1 = rs.c[1], c2 = rs.c[2], c3 = rs.c[3];
+   double delta = 0.0;
+   if( isInf(p, minb) )
+      c0 -= 1;
+   else if( isInf(p, -minb) )
+      c1 -= 1;
+   else if( isHuge(p, minb * absval) )
+      c2 -= 1;
+   else if( isHuge(p, -minb * absval) )
+      c3 -= 1;
+   else
+      delta = absval * minb;
+   getMinAct(p, rs.minact, c0, c1, c2, c3, delta, false, minres, mintight, minsetinf);
+
+   int d0 = rs.c[4], d1 = rs.c[5], d2 = rs.c[6], d3 = rs.c[7];
+   delta = 0.0;
+   if( isInf(p, -maxb) )
+      d1 -= 1;
+   else if( isInf(p, maxb) )
+      d0 -= 1;
+   else if( isHuge(p, absval * maxb) )
+      d2 -= 1;
+   else if( isHuge(p, -absval * maxb) )
+      d3 -= 1;
+   else
+      delta = absval * maxb;
+   getMaxAct(p, rs.maxact, d0, d1, d2, d3, delta, false, maxres, maxtight, maxsetinf);
+}
+
+// ------------------------------------------------------------------------------------------
+// candidates
+// ------------------------------------------------------------------------------------------
+struct Ctx
+{
+   const Mat* M;
+   const Vec* V;
+   const Params* P;
+   int32_t v;
+   int32_t round;
+};
+
+__device__ __forceinline__ void touch(const Ctx& C, size_t flat, int32_t j)
+{
+   if( atomicMax(&C.V->coltag[flat], C.round) < C.round )
+   {
+      uint32_t q = atomicAdd(&C.V->st->touched, 1u);
+      C.V->touched[q] = make_int2(j, C.v);
+   }
+}
+
+__device__ __forceinline__ void submitUb(const Ctx& C, int32_t j, double val, bool force)
+{
+   size_t flat = (size_t)C.v * (size_t)C.M->ncols + (size_t)j;
+   long long k = dkey(val);
+   long long* slot = force ? &C.V->fub[flat] : &C.V->cub[flat];
+   if( k < *slot )
+      atomicMin(slot, k);
+   touch(C, flat, j);
+}
+
+__device__ __forceinline__ void submitLb(const Ctx& C, int32_t j, double val, bool force)
+{
+   size_t flat = (size_t)C.v * (size_t)C.M->ncols + (size_t)j;
+   long long k = dkey(val);
+   long long* slot = force ? &C.V->flb[flat] : &C.V->clb[flat];
+   if( k > *slot )
+      atomicMax(slot, k);
+   touch(C, flat, j);
+}
+
+__device__ __forceinline__ void setCutoff(const Ctx& C)
+{
+   C.V->vcutoff[C.v] = 1;
+   if( C.V->single )
+      C.V->st->cutoff = 1;
+}
+
+// tightenVarBoundsEasy (cons_linear.c:5380-5653) on start-of-round values
+__device__ __forceinline__ void candEasy(const Ctx& C, const RowState& rs, double lhs, double rhs, const Elem& e, bool force)
+{
+   const Params& p = *C.P;
+   const bool integral = (e.cf & 1) != 0;
+   const double a = e.a, lb = e.lb, ub = e.ub;
+   const double minact = ddVal(rs.minact), maxact = ddVal(rs.maxact);
+
+   if( a > 0.0 )
+   {
+      if( !isInf(p, rhs) )
+      {
+         if( isFeasLT(p, rhs, minact) )
+         {
+            setCutoff(C);
+            return;
+         }
+         double slack = rhs - minact;
+         if( !isPositive(p, slack) )
+            slack = 0.0;
+         double alpha = a * (ub - lb);
+         if( isSumGT(p, alpha, slack) || (force && isGT(p, alpha, slack)) )
+         {
+            double newub = adjustedUb(p, integral, lb + (slack / a));
+            if( force || isUbBetter(p, newub, lb, ub) )
+               submitUb(C, e.j, newub, force);
+         }
+      }
+      if( !isInf(p, -lhs) )
+      {
+         if( isFeasLT(p, maxact, lhs) )
+         {
+            setCutoff(C);
+            return;
+         }
+         double slack = maxact - lhs;
+         if( !isPositive(p, slack) )
+            slack = 0.0;
+         double alpha = a * (ub - lb);
+         if( isSumGT(p, alpha, slack) || (force && isGT(p, alpha, slack)) )
+         {
+            double newlb = adjustedLb(p, integral, ub - (slack / a));
+            if( force || isLbBetter(p, newlb, lb, ub) )
+               submitLb(C, e.j, newlb, force);
+         }
+      }
+   }
+   else
+   {
+      if( !isInf(p, rhs) )
+      {
+         if( isFeasLT(p, rhs, minact) )
+         {
+            setCutoff(C);
+            return;
+         }
+         double slack = rhs - minact;
+         if( !isPositive(p, slack) )
+            slack = 0.0;
+         double alpha = a * (lb - ub);
+         if( isSumGT(p, alpha, slack) || (force && isGT(p, alpha, slack)) )
+         {
+            double newlb = adjustedLb(p, integral, ub + slack / a);
+            if( force || isLbBetter(p, newlb, lb, ub) )
+               submitLb(C, e.j, newlb, force);
+         }
+      }
+      if( !isInf(p, -lhs) )
+      {
+         if( isFeasLT(p, maxact, lhs) )
+         {
+            setCutoff(C);
+            return;
+         }
+         double slack = maxact - lhs;
+         if( !isPositive(p, slack) )
+            slack = 0.0;
+         double alpha = a * (lb - ub);
+         if( isSumGT(p, alpha, slack) || (force && isGT(p, alpha, slack)) )
+         {
+            double newub = adjustedUb(p, integral, lb - (slack / a));
+            if( force || isUbBetter(p, newub, lb, ub) )
+               submitUb(C, e.j, newub, force);
+         }
+      }
+   }
+}
+
+// consdataGetReliableResidualActivity (cons_linear.c:2580-2657): plain double sum in
+// position order over the row, skipping position skip
+__device__ __forceinline__ double reliableResidual(const Ctx& C, int64_t s, int64_t n, int64_t skip, bool isminres)
+{
+   const double2* bnd = C.V->bnd + (size_t)C.v * (size_t)C.M->ncols;
+   double r = 0.0;
+   for( int64_t q = 0; q < n; ++q )
+   {
+      if( q == skip )
+         continue;
+      double a = C.M->vals[s + q];
+      double2 b = bnd[C.M->colidx[s + q]];
+      if( a > 0.0 )
+         r += a * (isminres ? b.x : b.y);
+      else
+         r += a * (isminres ? b.y : b.x);
+   }
+   return r;
+}
+
+// tightenVarBounds (cons_linear.c:6700-6974) on start-of-round values
+__device__ __forceinline__ void candHard(const Ctx& C, const RowState& rs, double lhs, double rhs, const Elem& e, bool force,
+   int64_t s, int64_t n, int64_t pos)
+{
+   const Params& p = *C.P;
+   const bool integral = (e.cf & 1) != 0;
+   const double a = e.a, lb = e.lb, ub = e.ub;
+   const double lastmin = ddVal(rs.minact), lastmax = ddVal(rs.maxact);
+   double minres, maxres;
+   bool mintight, maxtight, minsetinf, maxsetinf;
+   residuals(p, rs, a, lb, ub, minres, maxres, mintight, maxtight, minsetinf, maxsetinf);
+
+   if( a > 0.0 )
+   {
+      if( !minsetinf && !isInf(p, rhs) && mintight && (isLT(p, fabs(rhs), 1.0) || !isEQ(p, minres / rhs, 1.0)) )
+      {
+         double newub = (rhs - minres) / a;
+         if( !isInf(p, newub)
+            && ((force && isLT(p, newub, ub)) || (integral && isFeasLT(p, newub, ub)) || isUbBetter(p, newub, lb, ub)) )
+         {
+            bool unrel = isUpdateUnreliable(p, minres, lastmin);
+            if( unrel )
+            {
+               minres = reliableResidual(C, s, n, pos, true);
+               newub = (rhs - minres) / a;
+               unrel = isInf(p, -minres)
+                  || (!isUbBetter(p, newub, lb, ub) && (!isFeasLT(p, newub, ub) || !integral)
+                     && (!force || !isLT(p, newub, ub)));
+            }
+            if( !unrel )
+               submitUb(C, e.j, newub, force);
+         }
+      }
+      if( !maxsetinf && !isInf(p, -lhs) && maxtight && (isLT(p, fabs(lhs), 1.0) || !isEQ(p, maxres / lhs, 1.0)) )
+      {
+         double newlb = (lhs - maxres) / a;
+         if( !isInf(p, -newlb)
+            && ((force && isGT(p, newlb, lb)) || (integral && isFeasGT(p, newlb, lb)) || isLbBetter(p, newlb, lb, ub)) )
+         {
+            if( isUpdateUnreliable(p, maxres, lastmax) )
+            {
+               maxres = reliableResidual(C, s, n, pos, false);
+               newlb = (lhs - maxres) / a;
+               if( isInf(p, maxres) || (!isLbBetter(p, newlb, lb, ub) && (!isFeasGT(p, newlb, lb) || !integral)
+                     && (!force || !isGT(p, newlb, lb))) )
+                  return;
+            }
+            submitLb(C, e.j, newlb, force);
+         }
+      }
+   }
+   else
+   {
+      if( !minsetinf && !isInf(p, rhs) && mintight && (isLT(p, fabs(rhs), 1.0) || !isEQ(p, minres / rhs, 1.0)) )
+      {
+         double newlb = (rhs - minres) / a;
+         if( !isInf(p, -newlb)
+            && ((force && isGT(p, newlb, lb)) || (integral && isFeasGT(p, newlb, lb)) || isLbBetter(p, newlb, lb, ub)) )
+         {
+            bool unrel = isUpdateUnreliable(p, minres, lastmin);
+            if( unrel )
+            {
+               minres = reliableResidual(C, s, n, pos, true);
+               newlb = (rhs - minres) / a;
+               unrel = isInf(p, -minres)
+                  || (!isLbBetter(p, newlb, lb, ub) && (!isFeasGT(p, newlb, lb) || !integral)
+                     && (!force || !isGT(p, newlb, lb)));
+            }
+            if( !unrel )
+               submitLb(C, e.j, newlb, force);
+         }
+      }
+      if( !maxsetinf && !isInf(p, -lhs) && maxtight && (isLT(p, fabs(lhs), 1.0) || !isEQ(p, maxres / lhs, 1.0)) )
+      {
+         double newub = (lhs - maxres) / a;
+         if( !isInf(p, newub)
+            && ((force && isLT(p, newub, ub)) || (integral && isFeasLT(p, newub, ub)) || isUbBetter(p, newub, lb, ub)) )
+         {
+            if( isUpdateUnreliable(p, maxres, lastmax) )
+            {
+               maxres = reliableResidual(C, s, n, pos, false);
+               newub = (lhs - maxres) / a;
+               // the reference joins the last two tests with && here (cons_linear.c:6942-6944)
+               if( isInf(p, maxres) || (!isUbBetter(p, newub, lb, ub) && (!isFeasLT(p, newub, ub) && !integral)
+                     && (!force || !isLT(p, newub, ub))) )
+                  return;
+            }
+            submitUb(C, e.j, newub, force);
+         }
+      }
+   }
+}
+
+// row-level decisions of tightenBounds (cons_linear.c:7014-7086)
+struct RowDecision
+{
+   bool tighten;
+   bool easy;
+   bool force;
+};
+
+__device__ __forceinline__ RowDecision decideRow(const Params& p, const RowState& rs, double lhs, double rhs, int64_t n)
+{
+   RowDecision d;
+   int infmin = rs.c[0] + rs.c[1] + rs.c[2] + rs.c[3];
+   int infmax = rs.c[4] + rs.c[5] + rs.c[6] + rs.c[7];
+   d.force = (n == 1);
+   d.tighten = n > 0 && !(infmin > 1 && infmax > 1);
+   if( d.tighten && isFeasZero(p, rs.mad) )
+      d.tighten = false;
+   if( d.tighten && !isInf(p, rs.mad) )
+   {
+      double minact, maxact;
+      bool t1, t2, f1, f2;
+      getMinAct(p, rs.minact, rs.c[0], rs.c[1], rs.c[2], rs.c[3], 0.0, false, minact, t1, f1);
+      getMaxAct(p, rs.maxact, rs.c[4], rs.c[5], rs.c[6], rs.c[7], 0.0, false, maxact, t2, f2);
+      double slack = (isInf(p, rhs) || f1) ? p.infinity : (rhs - minact);
+      double surplus = (isInf(p, -lhs) || f2) ? p.infinity : (maxact - lhs);
+      if( isLE(p, rs.mad, dmin(slack, surplus)) )
+         d.tighten = false;
+   }
+   d.easy = isLT(p, rs.mad, p.maxeasy);
+   return d;
+}
+
+// infeasibility / redundancy test at the end of propagateCons (cons_linear.c:7717-7753)
+__device__ __forceinline__ void rowVerdict(const Ctx& C, const RowState& rs, double lhs, double rhs, int32_t row)
+{
+   const Params& p = *C.P;
+   double minact, maxact;
+   bool t1, t2, f1, f2;
+   getMinAct(p, rs.minact, rs.c[0], rs.c[1], rs.c[2], rs.c[3], 0.0, true, minact, t1, f1);
+   getMaxAct(p, rs.maxact, rs.c[4], rs.c[5], rs.c[6], rs.c[7], 0.0, true, maxact, t2, f2);
+   if( isFeasGT(p, minact, rhs) || isFeasLT(p, maxact, lhs) )
+      setCutoff(C);
+   else if( isGE(p, minact, lhs) && isLE(p, maxact, rhs) )
+      C.V->redundant[(size_t)C.v * (size_t)C.M->nrows + (size_t)row] = 1;
+}
+
+__device__ __forceinline__ Elem loadElem(const Mat& M, const double2* bnd, int64_t k)
+{
+   Elem e;
+   e.a = M.vals[k];
+   e.j = M.colidx[k];
+   double2 b = bnd[e.j];
+   e.lb = b.x;
+   e.ub = b.y;
+   e.cf = M.colflags[e.j];
+   return e;
+}
+
+// ------------------------------------------------------------------------------------------
+// sweep: groups of G lanes, one row per group, one nonzero per lane (rows of <= G nonzeros)
+// ------------------------------------------------------------------------------------------
+template <int G>
+__device__ __forceinline__ void sweepGroups(const Mat& M, const Vec& V, const Params& P, const int2* items, uint32_t nitems,
+   int32_t round, int32_t rlocal, int lblock, int nblocks, unsigned long long& nnzacc)
+{
+   const int tid = lblock * SAP_BLOCK + (int)threadIdx.x;
+   const int glane = (int)threadIdx.x & (G - 1);
+   const uint32_t ngroups = (uint32_t)nblocks * (SAP_BLOCK / G);
+   for( uint32_t it = (uint32_t)(tid / G); it < nitems; it += ngroups )
+   {
+      const int2 item = items[it];
+      const int32_t row = item.x;
+      const int32_t v = item.y;
+      if( V.vcutoff[v] )
+         continue;
+      Ctx C;
+      C.M = &M;
+      C.V = &V;
+      C.P = &P;
+      C.v = v;
+      C.round = round;
+      const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
+      const int64_t s = M.rowptr[row];
+      const int64_t n = M.rowptr[row + 1] - s;
+      const double2 sides = M.sides[row];
+      const bool allbin = (M.rowflags[row] & 1) != 0;
+      const bool has = glane < n;
+      Elem e;
+      e.a = 0.0;
+      e.lb = 0.0;
+      e.ub = 0.0;
+      e.j = 0;
+      e.cf = 0;
+      if( has )
+         e = loadElem(M, bnd, s + glane);
+
+      double cmin = 0.0, cmax = 0.0, mad = 0.0;
+      int kmin = 0, kmax = 0;
+      bool madinf = false;
+      if( has )
+         elemContrib(P, e, allbin, cmin, cmax, kmin, kmax, mad, madinf);
+      unsigned long long cnt = packKinds(kmin, kmax);
+      DD dmn = { cmin, 0.0 };
+      DD dmx = { cmax, 0.0 };
+      int inf = madinf ? 1 : 0;
+#pragma unroll
+      for( int off = 1; off < G; off <<= 1 )
+      {
+         dmn = pairAdd(dmn, shflXorDD(dmn, off));
+         dmx = pairAdd(dmx, shflXorDD(dmx, off));
+         cnt += (unsigned long long)__shfl_xor((long long)cnt, off, SAP_WAVE);
+         mad = dmax(mad, shflXorD(mad, off));
+         inf |= __shfl_xor(inf, off, SAP_WAVE);
+      }
+      RowState rs;
+      rs.minact = dmn;
+      rs.maxact = dmx;
+#pragma unroll
+      for( int q = 0; q < 8; ++q )
+         rs.c[q] = (int)((cnt >> (8 * q)) & 0xFF);
+      rs.mad = inf ? P.infinity : mad;
+
+      RowDecision d = decideRow(P, rs, sides.x, sides.y, n);
+      if( d.tighten && has )
+      {
+         if( d.easy )
+            candEasy(C, rs, sides.x, sides.y, e, d.force);
+         else
+            candHard(C, rs, sides.x, sides.y, e, d.force, s, n, glane);
+      }
+      if( glane == 0 )
+      {
+         rowVerdict(C, rs, sides.x, sides.y, row);
+         nnzacc += (unsigned long long)n;
+         if( V.vrounds[v] < rlocal )
+            atomicMax(&V.vrounds[v], rlocal);
+      }
+   }
+}
+
+// ------------------------------------------------------------------------------------------
+// sweep of one long row by one wavefront: the chunks [c0, c0+nch) of 64 positions form an
+// aligned subtree of the canonical tree; the wave returns its root (all lanes) and the
+// per-lane counters / maxactdelta contributions.
+// ------------------------------------------------------------------------------------------
+struct WavePart
+{
+   DD minact;
+   DD maxact;
+   int32_t c[8];
+   double mad;
+   int inf;
+};
+
+__device__ __forceinline__ WavePart waveSubtree(const Mat& M, const Params& P, const double2* bnd, int64_t s, int64_t n, int64_t c0,
+   int64_t nch, bool allbin)
+{
+   const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
+   WavePart w;
+   DD stmin = { 0.0, 0.0 }, stmax = { 0.0, 0.0 };   // level `lane` of the binary-counter stack
+   int32_t k0 = 0, k1 = 0, k2 = 0, k3 = 0, k4 = 0, k5 = 0, k6 = 0, k7 = 0;
+   double mad = 0.0;
+   int inf = 0;
+   int64_t cnt = 0;
+   for( int64_t c = c0; c < c0 + nch; ++c )
+   {
+      const int64_t pos = c * SAP_WAVE + lane;
+      double cmin = 0.0, cmax = 0.0;
+      if( pos < n )
+      {
+         Elem e = loadElem(M, bnd, s + pos);
+         int kmin, kmax;
+         double md;
+         bool mi;
+         elemContrib(P, e, allbin, cmin, cmax, kmin, kmax, md, mi);
+         k0 += (kmin == 1);
+         k1 += (kmin == 2);
+         k2 += (kmin == 3);
+         k3 += (kmin == 4);
+         k4 += (kmax == 1);
+         k5 += (kmax == 2);
+         k6 += (kmax == 3);
+         k7 += (kmax == 4);
+         mad = dmax(mad, md);
+         inf |= mi ? 1 : 0;
+      }
+      DD rmin = { cmin, 0.0 }, rmax = { cmax, 0.0 };
+#pragma unroll
+      for( int off = 1; off < SAP_WAVE; off <<= 1 )
+      {
+         rmin = pairAdd(rmin, shflXorDD(rmin, off));
+         rmax = pairAdd(rmax, shflXorDD(rmax, off));
+      }
+      // push the chunk root on the binary-counter stack (level l lives in lane l)
+      int64_t cc = cnt;
+      int lvl = 0;
+      while( cc & 1 )
+      {
+         rmin = pairAdd(shflDD(stmin, lvl), rmin);
+         rmax = pairAdd(shflDD(stmax, lvl), rmax);
+         cc >>= 1;
+         ++lvl;
+      }
+      if( lane == lvl )
+      {
+         stmin = rmin;
+         stmax = rmax;
+      }
+      ++cnt;
+   }
+   // fold the stack from the lowest set level upwards
+   bool first = true;
+   DD amin = { 0.0, 0.0 }, amax = { 0.0, 0.0 };
+   for( int lvl = 0; lvl < SAP_MAXLEVELS && (cnt >> lvl) != 0; ++lvl )
+   {
+      if( (cnt >> lvl) & 1 )
+      {
+         DD emin = shflDD(stmin, lvl), emax = shflDD(stmax, lvl);
+         if( first )
+         {
+            amin = emin;
+            amax = emax;
+            first = false;
+         }
+         else
+         {
+            amin = pairAdd(emin, amin);
+            amax = pairAdd(emax, amax);
+         }
+      }
+   }
+   w.minact = amin;
+   w.maxact = amax;
+   // counters and maxactdelta reduced over the wave (pairs of 32-bit counters per 64-bit lane word)
+   long long p01 = ((long long)k1 << 32) | (unsigned)k0, p23 = ((long long)k3 << 32) | (unsigned)k2;
+   long long p45 = ((long long)k5 << 32) | (unsigned)k4, p67 = ((long long)k7 << 32) | (unsigned)k6;
+#pragma unroll
+   for( int off = 1; off < SAP_WAVE; off <<= 1 )
+   {
+      p01 += __shfl_xor(p01, off, SAP_WAVE);
+      p23 += __shfl_xor(p23, off, SAP_WAVE);
+      p45 += __shfl_xor(p45, off, SAP_WAVE);
+      p67 += __shfl_xor(p67, off, SAP_WAVE);
+   }
+   w.c[0] = (int32_t)(p01 & 0xFFFFFFFF);
+   w.c[1] = (int32_t)(p01 >> 32);
+   w.c[2] = (int32_t)(p23 & 0xFFFFFFFF);
+   w.c[3] = (int32_t)(p23 >> 32);
+   w.c[4] = (int32_t)(p45 & 0xFFFFFFFF);
+   w.c[5] = (int32_t)(p45 >> 32);
+   w.c[6] = (int32_t)(p67 & 0xFFFFFFFF);
+   w.c[7] = (int32_t)(p67 >> 32);
+#pragma unroll
+   for( int off = 1; off < SAP_WAVE; off <<= 1 )
+   {
+      mad = dmax(mad, shflXorD(mad, off));
+      inf |= __shfl_xor(inf, off, SAP_WAVE);
+   }
+   w.mad = mad;
+   w.inf = inf;
+   return w;
+}
+
+__device__ __forceinline__ void candidatesRange(const Ctx& C, const RowState& rs, const RowDecision& d, double lhs, double rhs,
+   const double2* bnd, int64_t s, int64_t n, int64_t c0, int64_t nch)
+{
+   const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
+   for( int64_t c = c0; c < c0 + nch; ++c )
+   {
+      const int64_t pos = c * SAP_WAVE + lane;
+      if( pos >= n )
+         break;
+      Elem e = loadElem(*C.M, bnd, s + pos);
+      if( d.easy )
+         candEasy(C, rs, lhs, rhs, e, d.force);
+      else
+         candHard(C, rs, lhs, rhs, e, d.force, s, n, pos);
+   }
+}
+
+__device__ __forceinline__ void sweepWaves(const Mat& M, const Vec& V, const Params& P, const int2* items, uint32_t nitems,
+   int32_t round, int32_t rlocal, int lblock, int nblocks, unsigned long long& nnzacc)
+{
+   const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
+   const int wid = lblock * (SAP_BLOCK / SAP_WAVE) + (int)threadIdx.x / SAP_WAVE;
+   const uint32_t nw = (uint32_t)nblocks * (SAP_BLOCK / SAP_WAVE);
+   for( uint32_t it = (uint32_t)wid; it < nitems; it += nw )
+   {
+      const int2 item = items[it];
+      const int32_t row = item.x;
+      const int32_t v = item.y;
+      if( V.vcutoff[v] )
+         continue;
+      Ctx C;
+      C.M = &M;
+      C.V = &V;
+      C.P = &P;
+      C.v = v;
+      C.round = round;
+      const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
+      const int64_t s = M.rowptr[row];
+      const int64_t n = M.rowptr[row + 1] - s;
+      const int64_t nch = (n + SAP_WAVE - 1) / SAP_WAVE;
+      const double2 sides = M.sides[row];
+      const bool allbin = (M.rowflags[row] & 1) != 0;
+      WavePart w = waveSubtree(M, P, bnd, s, n, 0, nch, allbin);
+      RowState rs;
+      rs.minact = w.minact;
+      rs.maxact = w.maxact;
+#pragma unroll
+      for( int q = 0; q < 8; ++q )
+         rs.c[q] = w.c[q];
+      rs.mad = w.inf ? P.infinity : w.mad;
+      RowDecision d = decideRow(P, rs, sides.x, sides.y, n);
+      if( d.tighten )
+         candidatesRange(C, rs, d, sides.x, sides.y, bnd, s, n, 0, nch);
+      if( lane == 0 )
+      {
+         rowVerdict(C, rs, sides.x, sides.y, row);
+         nnzacc += (unsigned long long)n;
+         if( V.vrounds[v] < rlocal )
+            atomicMax(&V.vrounds[v], rlocal);
+      }
+   }
+}
+
+// workgroup per row: wave w reduces the aligned chunk subtree [w*span, (w+1)*span)
+__device__ __forceinline__ void sweepBlocks(const Mat& M, const Vec& V, const Params& P, const int2* items, uint32_t nitems,
+   int32_t round, int32_t rlocal, int lblock, int nblocks, unsigned long long& nnzacc)
+{
+   constexpr int NW = SAP_BLOCK / SAP_WAVE;
+   __shared__ WavePart parts[NW];
+   __shared__ RowState srs;
+   const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
+   const int wave = (int)threadIdx.x / SAP_WAVE;
+   for( uint32_t it = (uint32_t)lblock; it < nitems; it += (uint32_t)nblocks )
+   {
+      const int2 item = items[it];
+      const int32_t row = item.x;
+      const int32_t v = item.y;
+      if( V.vcutoff[v] )
+         continue;   // block-uniform
+      Ctx C;
+      C.M = &M;
+      C.V = &V;
+      C.P = &P;
+      C.v = v;
+      C.round = round;
+      const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
+      const int64_t s = M.rowptr[row];
+      const int64_t n = M.rowptr[row + 1] - s;
+      const int64_t nch = (n + SAP_WAVE - 1) / SAP_WAVE;
+      int64_t P2 = 1;
+      while( P2 < nch )
+         P2 <<= 1;
+      const int64_t span = P2 >= NW ? P2 / NW : 1;
+      const int64_t c0 = (int64_t)wave * span;
+      const int64_t mine = c0 < nch ? (c0 + span <= nch ? span : nch - c0) : 0;
+      const double2 sides = M.sides[row];
+      const bool allbin = (M.rowflags[row] & 1) != 0;
+      WavePart w = waveSubtree(M, P, bnd, s, n, c0, mine, allbin);
+      if( lane == 0 )
+         parts[wave] = w;
+      __syncthreads();
+      if( threadIdx.x == 0 )
+      {
+         // pairwise combination of the four subtree roots (wave order = position order)
+         static_assert(NW == 4, "workgroup combine is written for four wavefronts");
+         RowState rs;
+         rs.minact = pairAdd(pairAdd(parts[0].minact, parts[1].minact), pairAdd(parts[2].minact, parts[3].minact));
+         rs.maxact = pairAdd(pairAdd(parts[0].maxact, parts[1].maxact), pairAdd(parts[2].maxact, parts[3].maxact));
+#pragma unroll
+         for( int k = 0; k < 8; ++k )
+            rs.c[k] = parts[0].c[k] + parts[1].c[k] + parts[2].c[k] + parts[3].c[k];
+         const double mad = dmax(dmax(parts[0].mad, parts[1].mad), dmax(parts[2].mad, parts[3].mad));
+         const int inf = parts[0].inf | parts[1].inf | parts[2].inf | parts[3].inf;
+         rs.mad = inf ? P.infinity : mad;
+         srs = rs;
+      }
+      __syncthreads();
+      RowState rs = srs;
+      RowDecision d = decideRow(P, rs, sides.x, sides.y, n);
+      if( d.tighten )
+         candidatesRange(C, rs, d, sides.x, sides.y, bnd, s, n, c0, mine);
+      if( threadIdx.x == 0 )
+      {
+         rowVerdict(C, rs, sides.x, sides.y, row);
+         nnzacc += (unsigned long long)n;
+         if( V.vrounds[v] < rlocal )
+            atomicMax(&V.vrounds[v], rlocal);
+      }
+      __syncthreads();
+   }
+}
+
+__global__ __launch_bounds__(SAP_BLOCK) void k_sweep(Mat M, Vec V, Params P, Layout L)
+{
+   Status* st = V.st;
+   if( st->done )
+      return;
+   const int32_t round = st->round;
+   const int32_t rlocal = st->rlocal;
+   const int par = round & 1;
+   const int blk = (int)blockIdx.x;
+   int b = 0;
+   while( b < SAP_NBINS - 1 && blk >= L.blockstart[b + 1] )
+      ++b;
+   const int lblock = blk - L.blockstart[b];
+   const int nblocks = L.blockstart[b + 1] - L.blockstart[b];
+   const uint32_t nitems = st->wlcount[par][b];
+   const int2* items = V.wl[par][b];
+   unsigned long long nnzacc = 0;
+   switch( b )
+   {
+   case 0: sweepGroups<4>(M, V, P, items, nitems, round, rlocal, lblock, nblocks, nnzacc); break;
+   case 1: sweepGroups<8>(M, V, P, items, nitems, round, rlocal, lblock, nblocks, nnzacc); break;
+   case 2: sweepGroups<16>(M, V, P, items, nitems, round, rlocal, lblock, nblocks, nnzacc); break;
+   case 3: sweepGroups<32>(M, V, P, items, nitems, round, rlocal, lblock, nblocks, nnzacc); break;
+   case 4: sweepGroups<64>(M, V, P, items, nitems, round, rlocal, lblock, nblocks, nnzacc); break;
+   case 5: sweepWaves(M, V, P, items, nitems, round, rlocal, lblock, nblocks, nnzacc); break;
+   default: sweepBlocks(M, V, P, items, nitems, round, rlocal, lblock, nblocks, nnzacc); break;
+   }
+   // one atomic per wavefront for the swept-nonzero statistic
+#pragma unroll
+   for( int off = 1; off < SAP_WAVE; off <<= 1 )
+      nnzacc += (unsigned long long)__shfl_xor((long long)nnzacc, off, SAP_WAVE);
+   if( ((int)threadIdx.x & (SAP_WAVE - 1)) == 0 && nnzacc != 0 )
+      atomicAdd(&st->nnz_swept, nnzacc);
+}
+
+// ------------------------------------------------------------------------------------------
+// column step
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(SAP_BLOCK) void k_columns(Mat M, Vec V, Params P)
+{
+   Status* st = V.st;
+   if( st->done )
+      return;
+   const int32_t round = st->round;
+   const int32_t next = round + 1;
+   const int npar = next & 1;
+   const uint32_t n = st->touched;
+   unsigned long long nchg = 0;
+   for( uint32_t idx = blockIdx.x * SAP_BLOCK + threadIdx.x; idx < n; idx += gridDim.x * SAP_BLOCK )
+   {
+      const int2 t = V.touched[idx];
+      const int32_t j = t.x;
+      const int32_t v = t.y;
+      const size_t flat = (size_t)v * (size_t)M.ncols + (size_t)j;
+      const long long kcl = V.clb[flat], kcu = V.cub[flat];
+      V.clb[flat] = KEY_NEG_INF;
+      V.cub[flat] = KEY_POS_INF;
+      long long kfl = KEY_NEG_INF, kfu = KEY_POS_INF;
+      if( V.hasforced )
+      {
+         kfl = V.flb[flat];
+         kfu = V.fub[flat];
+         V.flb[flat] = KEY_NEG_INF;
+         V.fub[flat] = KEY_POS_INF;
+      }
+      if( V.vcutoff[v] )
+         continue;
+      const bool integral = (M.colflags[j] & 1) != 0;
+      const double2 b0 = V.bnd[flat];
+      const double lb0 = b0.x, ub0 = b0.y;
+      double nlb = lb0, nub = ub0;
+      bool cut = false;
+      // SCIPinferVarLbCons (scip_var.c:6965-7070): forced, then regular candidate
+      if( kfl != KEY_NEG_INF )
+      {
+         double tt = adjustedLb(P, integral, keyToDouble(kfl));
+         if( isInf(P, tt) || isFeasGT(P, tt, ub0) )
+            cut = true;
+         else
+         {
+            tt = dmin(tt, ub0);
+            if( !isLE(P, tt, lb0) && dkey(tt) > dkey(nlb) )
+               nlb = tt;
+         }
+      }
+      if( !cut && kcl != KEY_NEG_INF )
+      {
+         double tt = adjustedLb(P, integral, keyToDouble(kcl));
+         if( isInf(P, tt) || isFeasGT(P, tt, ub0) )
+            cut = true;
+         else
+         {
+            tt = dmin(tt, ub0);
+            if( isLbBetter(P, tt, lb0, ub0) && dkey(tt) > dkey(nlb) )
+               nlb = tt;
+         }
+      }
+      // SCIPinferVarUbCons (scip_var.c:7071-7174) against the new lower bound
+      if( !cut && kfu != KEY_POS_INF )
+      {
+         double tt = adjustedUb(P, integral, keyToDouble(kfu));
+         if( isInf(P, -tt) || isFeasLT(P, tt, nlb) )
+            cut = true;
+         else
+         {
+            tt = dmax(tt, nlb);
+            if( !isGE(P, tt, ub0) && dkey(tt) < dkey(nub) )
+               nub = tt;
+         }
+      }
+      if( !cut && kcu != KEY_POS_INF )
+      {
+         double tt = adjustedUb(P, integral, keyToDouble(kcu));
+         if( isInf(P, -tt) || isFeasLT(P, tt, nlb) )
+            cut = true;
+         else
+         {
+            tt = dmax(tt, nlb);
+            if( isUbBetter(P, tt, nlb, ub0) && dkey(tt) < dkey(nub) )
+               nub = tt;
+         }
+      }
+      if( cut )
+      {
+         V.vcutoff[v] = 1;
+         if( V.single )
+            st->cutoff = 1;
+         continue;
+      }
+      const bool chl = dkey(nlb) != dkey(lb0);
+      const bool chu = dkey(nub) != dkey(ub0);
+      if( !chl && !chu )
+         continue;
+      V.bnd[flat] = make_double2(nlb, nub);
+      nchg += (chl ? 1 : 0) + (chu ? 1 : 0);
+      // queue the rows of the column for the next round (eventExecLinear marks them)
+      for( int64_t q = M.colptr[j]; q < M.colptr[j + 1]; ++q )
+      {
+         const int32_t row = M.rowidx[q];
+         const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)row;
+         if( V.redundant[rflat] )
+            continue;
+         if( atomicMax(&V.rowmark[rflat], next) < next )
+         {
+            const int bin = M.rowbin[row];
+            const uint32_t pos = atomicAdd(&st->wlcount[npar][bin], 1u);
+            V.wl[npar][bin][pos] = make_int2(row, v);
+         }
+      }
+   }
+#pragma unroll
+   for( int off = 1; off < SAP_WAVE; off <<= 1 )
+      nchg += (unsigned long long)__shfl_xor((long long)nchg, off, SAP_WAVE);
+   if( ((int)threadIdx.x & (SAP_WAVE - 1)) == 0 && nchg != 0 )
+      atomicAdd(&st->nchg, nchg);
+}
+
+// ------------------------------------------------------------------------------------------
+// round control and set-up kernels (kernels, not memsets, so a round can be graph-captured)
+// ------------------------------------------------------------------------------------------
+__global__ void k_round_begin(Vec V, int32_t maxrounds)
+{
+   Status* st = V.st;
+   if( st->done )
+      return;
+   const int32_t r = st->round + 1;
+   uint32_t in = 0;
+   for( int b = 0; b < SAP_NBINS; ++b )
+      in += st->wlcount[r & 1][b];
+   if( in == 0 || st->cutoff || (maxrounds >= 0 && st->rlocal >= maxrounds) )
+   {
+      st->done = 1;
+      return;
+   }
+   st->round = r;
+   st->rlocal += 1;
+   for( int b = 0; b < SAP_NBINS; ++b )
+      st->wlcount[(r + 1) & 1][b] = 0;
+   st->touched = 0;
+   st->rows_swept += in;
+}
+
+// per-call reset of the vector state; status counters for the call
+__global__ void k_call_reset(Vec V, int64_t nrows, int32_t nvec)
+{
+   const size_t tot = (size_t)nvec * (size_t)nrows;
+   for( size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x )
+      V.redundant[i] = 0;
+   const size_t gi = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+   if( gi < (size_t)nvec )
+   {
+      V.vcutoff[gi] = 0;
+      V.vrounds[gi] = 0;
+   }
+   if( gi == 0 )
+   {
+      Status* st = V.st;
+      st->rlocal = 0;
+      st->done = 0;
+      st->cutoff = 0;
+      st->touched = 0;
+      st->rows_swept = 0;
+      st->nnz_swept = 0;
+      st->nchg = 0;
+      for( int b = 0; b < SAP_NBINS; ++b )
+      {
+         st->wlcount[0][b] = 0;
+         st->wlcount[1][b] = 0;
+      }
+   }
+}
+
+// initial worklists: every row of every vector (static bin lists replicated per vector)
+__global__ void k_init_all(Vec V, const int32_t* binrows, int32_t nbin, int32_t bin, int32_t nvec)
+{
+   Status* st = V.st;
+   const int par = (st->round + 1) & 1;
+   const size_t tot = (size_t)nvec * (size_t)nbin;
+   int2* wl = V.wl[par][bin];
+   for( size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x )
+      wl[i] = make_int2(binrows[i % (size_t)nbin], (int32_t)(i / (size_t)nbin));
+   if( blockIdx.x == 0 && threadIdx.x == 0 )
+      st->wlcount[par][bin] = (uint32_t)tot;
+}
+
+// initial worklists from overlay columns (base bounds already a fixpoint)
+__global__ void k_init_overlay(Mat M, Vec V, const int64_t* ovptr, const int32_t* ovcol, int32_t nvec)
+{
+   Status* st = V.st;
+   const int32_t first = st->round + 1;
+   const int par = first & 1;
+   for( int32_t v = blockIdx.y; v < nvec; v += gridDim.y )
+   {
+      for( int64_t q = ovptr[v] + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ovptr[v + 1];
+           q += (int64_t)gridDim.x * blockDim.x )
+      {
+         const int32_t j = ovcol[q];
+         for( int64_t r = M.colptr[j]; r < M.colptr[j + 1]; ++r )
+         {
+            const int32_t row = M.rowidx[r];
+            const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)row;
+            if( atomicMax(&V.rowmark[rflat], first) < first )
+            {
+               const int bin = M.rowbin[row];
+               const uint32_t pos = atomicAdd(&st->wlcount[par][bin], 1u);
+               V.wl[par][bin][pos] = make_int2(row, v);
+            }
+         }
+      }
+   }
+}
+
+// bounds of all vectors from a base vector plus overlays
+__global__ void k_broadcast_base(double2* bnd, const double2* base, int64_t ncols, int32_t nvec)
+{
+   const size_t tot = (size_t)nvec * (size_t)ncols;
+   for( size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x )
+      bnd[i] = base[i % (size_t)ncols];
+}
+
+__global__ void k_apply_overlay(double2* bnd, int64_t ncols, const int64_t* ovptr, const int32_t* ovcol,
+   const double* ovlb, const double* ovub, int32_t nvec)
+{
+   for( int32_t v = blockIdx.y; v < nvec; v += gridDim.y )
+      for( int64_t q = ovptr[v] + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ovptr[v + 1];
+           q += (int64_t)gridDim.x * blockDim.x )
+         bnd[(size_t)v * (size_t)ncols + (size_t)ovcol[q]] = make_double2(ovlb[q], ovub[q]);
+}
+
+__global__ void k_fill_i64(long long* p, long long val, size_t n)
+{
+   for( size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x )
+      p[i] = val;
+}
+
+__global__ void k_fill_i32(int32_t* p, int32_t val, size_t n)
+{
+   for( size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x )
+      p[i] = val;
+}
+
+__global__ void k_interleave(double2* dst, const double* lb, const double* ub, int64_t n)
+{
+   for( int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x )
+      dst[i] = make_double2(lb[i], ub[i]);
+}
+
+__global__ void k_deinterleave(const double2* src, double* lb, double* ub, int64_t n)
+{
+   for( int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x )
+   {
+      double2 b = src[i];
+      lb[i] = b.x;
+      ub[i] = b.y;
+   }
+}
+
+// row activities for diagnostics: one wavefront per row (chunked canonical tree)
+__global__ __launch_bounds__(SAP_BLOCK) void k_row_activities(Mat M, Params P, const double2* bnd, double* minact,
+   double* maxact, int32_t* counts, double* mad)
+{
+   const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
+   const int64_t wid = (blockIdx.x * (int64_t)SAP_BLOCK + threadIdx.x) / SAP_WAVE;
+   const int64_t nw = (int64_t)gridDim.x * (SAP_BLOCK / SAP_WAVE);
+   for( int64_t row = wid; row < M.nrows; row += nw )
+   {
+      const int64_t s = M.rowptr[row];
+      const int64_t n = M.rowptr[row + 1] - s;
+      const int64_t nch = (n + SAP_WAVE - 1) / SAP_WAVE;
+      WavePart w = waveSubtree(M, P, bnd, s, n, 0, nch, (M.rowflags[row] & 1) != 0);
+      if( lane == 0 )
+      {
+         minact[row] = ddVal(w.minact);
+         maxact[row] = ddVal(w.maxact);
+         for( int q = 0; q < 8; ++q )
+            counts[8 * row + q] = w.c[q];
+         mad[row] = w.inf ? P.infinity : w.mad;
+      }
+   }
+}
+
+// ------------------------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------------------------
+struct sap_context
+{
+   int32_t device;
+   hipStream_t stream;
+   hipEvent_t ev[4];
+   Status* hstatus;   // pinned
+};
+
+struct sap_matrix
+{
+   sap_context* ctx;
+   int64_t nrows, ncols, nnz;
+   int64_t device_bytes;
+   int32_t hasforced;
+   int64_t* rowptr;
+   int32_t* colidx;
+   double* vals;
+   double2* sides;
+   uint8_t* colflags;
+   uint8_t* rowflags;
+   uint8_t* rowbin;
+   int64_t* colptr;
+   int32_t* rowidx;
+   int32_t* binrows[SAP_NBINS];
+   int32_t nbin[SAP_NBINS];
+   sap_bounds* single;   // cached one-vector state for sap_propagate
+};
+
+struct sap_bounds
+{
+   sap_matrix* mat;
+   int32_t nvec;
+   Vec V;
+   Status* dstatus;
+   int64_t device_bytes;
+   double* stage;   // device staging for host lb/ub (2*ncols)
+   int32_t roundbase;
+};
+
+static int hipToRc(hipError_t e)
+{
+   if( e == hipSuccess )
+      return SAP_OKAY;
+   if( e == hipErrorOutOfMemory )
+      return SAP_NOMEMORY;
+   if( e == hipErrorInvalidValue || e == hipErrorInvalidDevice )
+      return SAP_INVALIDDATA;
+   return SAP_ERROR;
+}
+
+#define HIPCHK(x)                                                                                    \
+   do                                                                                                \
+   {                                                                                                 \
+      hipError_t _e = (x);                                                                           \
+      if( _e != hipSuccess )                                                                         \
+      {                                                                                              \
+         fprintf(stderr, "[scip_amd] %s failed: %s (%s:%d)\n", #x, hipGetErrorString(_e), __FILE__, \
+            __LINE__);                                                                               \
+         return hipToRc(_e);                                                                         \
+      }                                                                                              \
+   } while( 0 )
+
+template <typename T>
+static int devAlloc(T** p, size_t count, int64_t* acc)
+{
+   size_t bytes = sizeof(T) * (count > 0 ? count : 1);
+   hipError_t e = hipMalloc((void**)p, bytes);
+   if( e != hipSuccess )
+   {
+      *p = NULL;
+      fprintf(stderr, "[scip_amd] hipMalloc(%zu) failed: %s\n", bytes, hipGetErrorString(e));
+      return hipToRc(e);
+   }
+   if( acc )
+      *acc += (int64_t)bytes;
+   return SAP_OKAY;
+}
+
+#define RCCHK(x)                \
+   do                           \
+   {                            \
+      int _rc = (x);            \
+      if( _rc != SAP_OKAY )     \
+         return _rc;            \
+   } while( 0 )
+
+static inline unsigned gridFor(size_t n, unsigned cap = 8192)
+{
+   size_t g = (n + SAP_BLOCK - 1) / SAP_BLOCK;
+   if( g < 1 )
+      g = 1;
+   if( g > cap )
+      g = cap;
+   return (unsigned)g;
+}
+
+static Params toParams(const sap_params* p)
+{
+   Params q;
+   q.infinity = p->infinity;
+   q.epsilon = p->epsilon;
+   q.sumepsilon = p->sumepsilon;
+   q.feastol = p->feastol;
+   q.boundstreps = p->boundstreps;
+   q.hugeval = p->hugeval;
+   q.recompfac = p->recompfac;
+   q.maxeasy = p->maxeasyactivitydelta;
+   q.maxrounds = p->maxrounds;
+   q.flags = p->flags;
+   return q;
+}
+
+static Mat toMat(const sap_matrix* m)
+{
+   Mat M;
+   M.nrows = m->nrows;
+   M.ncols = m->ncols;
+   M.rowptr = m->rowptr;
+   M.colidx = m->colidx;
+   M.vals = m->vals;
+   M.sides = m->sides;
+   M.colflags = m->colflags;
+   M.rowflags = m->rowflags;
+   M.rowbin = m->rowbin;
+   M.colptr = m->colptr;
+   M.rowidx = m->rowidx;
+   return M;
+}
+
+extern "C" {
+
+void sap_params_default(sap_params* p)
+{
+   if( p == NULL )
+      return;
+   p->infinity = 1e20;
+   p->epsilon = 1e-9;
+   p->sumepsilon = 1e-6;
+   p->feastol = 1e-6;
+   p->boundstreps = 0.05;
+   p->hugeval = 1e15;
+   p->recompfac = 1e6;
+   p->maxeasyactivitydelta = 1e6;
+   p->maxrounds = -1;
+   p->flags = 0;
+}
+
+const char* sap_error_string(int rc)
+{
+   switch( rc )
+   {
+   case SAP_OKAY: return "okay";
+   case SAP_ERROR: return "unspecified error (HIP runtime)";
+   case SAP_NOMEMORY: return "insufficient device or host memory";
+   case SAP_INVALIDCALL: return "invalid call";
+   case SAP_INVALIDDATA: return "invalid input data";
+   case SAP_NOTIMPLEMENTED: return "not implemented";
+   default: return "unknown return code";
+   }
+}
+
+int sap_device_count(int32_t* n)
+{
+   int c = 0;
+   if( n == NULL )
+      return SAP_INVALIDDATA;
+   if( hipGetDeviceCount(&c) != hipSuccess )
+      c = 0;
+   *n = c;
+   return SAP_OKAY;
+}
+
+int sap_context_create(int32_t device, sap_context** out)
+{
+   if( out == NULL )
+      return SAP_INVALIDDATA;
+   *out = NULL;
+   int nd = 0;
+   if( hipGetDeviceCount(&nd) != hipSuccess || nd == 0 )
+   {
+      fprintf(stderr, "[scip_amd] no HIP device available\n");
+      return SAP_INVALIDCALL;
+   }
+   if( device < 0 || device >= nd )
+      return SAP_INVALIDDATA;
+   HIPCHK(hipSetDevice(device));
+   sap_context* c = (sap_context*)calloc(1, sizeof(sap_context));
+   if( c == NULL )
+      return SAP_NOMEMORY;
+   c->device = device;
+   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+   for( int i = 0; i < 4; ++i )
+      HIPCHK(hipEventCreate(&c->ev[i]));
+   HIPCHK(hipHostMalloc((void**)&c->hstatus, sizeof(Status), hipHostMallocDefault));
+   *out = c;
+   return SAP_OKAY;
+}
+
+int sap_context_free(sap_context* c)
+{
+   if( c == NULL )
+      return SAP_OKAY;
+   (void)hipSetDevice(c->device);
+   (void)hipStreamSynchronize(c->stream);
+   for( int i = 0; i < 4; ++i )
+      (void)hipEventDestroy(c->ev[i]);
+   (void)hipHostFree(c->hstatus);
+   (void)hipStreamDestroy(c->stream);
+   free(c);
+   return SAP_OKAY;
+}
+
+static void matrixRelease(sap_matrix* m)
+{
+   (void)hipFree(m->rowptr);
+   (void)hipFree(m->colidx);
+   (void)hipFree(m->vals);
+   (void)hipFree(m->sides);
+   (void)hipFree(m->colflags);
+   (void)hipFree(m->rowflags);
+   (void)hipFree(m->rowbin);
+   (void)hipFree(m->colptr);
+   (void)hipFree(m->rowidx);
+   for( int b = 0; b < SAP_NBINS; ++b )
+      (void)hipFree(m->binrows[b]);
+}
+
+int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int64_t* rowptr, const int32_t* colidx,
+   const double* vals, const double* lhs, const double* rhs, const uint8_t* integral, const double* glb,
+   const double* gub, sap_matrix** out)
+{
+   if( ctx == NULL || out == NULL || rowptr == NULL || lhs == NULL || rhs == NULL || integral == NULL || glb == NULL
+      || gub == NULL || nrows < 0 || ncols < 0 || nrows > INT32_MAX || ncols > INT32_MAX )
+      return SAP_INVALIDDATA;
+   *out = NULL;
+   const int64_t nnz = rowptr[nrows];
+   if( rowptr[0] != 0 || nnz < 0 || (nnz > 0 && (colidx == NULL || vals == NULL)) )
+      return SAP_INVALIDDATA;
+   // host-side validation and derived arrays
+   std::vector<uint8_t> colflags((size_t)ncols), rowflags((size_t)nrows), rowbin((size_t)nrows);
+   std::vector<int64_t> colptr((size_t)ncols + 1, 0);
+   std::vector<int32_t> rowidx((size_t)nnz);
+   std::vector<std::vector<int32_t>> bins(SAP_NBINS);
+   std::vector<double2> sides((size_t)nrows);
+   int32_t hasforced = 0;
+   for( int64_t j = 0; j < ncols; ++j )
+   {
+      if( glb[j] != glb[j] || gub[j] != gub[j] )
+         return SAP_INVALIDDATA;
+      bool isint = integral[j] != 0;
+      bool isbin = isint && glb[j] >= 0.0 && gub[j] <= 1.0;
+      colflags[(size_t)j] = (uint8_t)((isint ? 1 : 0) | (isbin ? 2 : 0));
+   }
+   for( int64_t i = 0; i < nrows; ++i )
+   {
+      int64_t s = rowptr[i], e = rowptr[i + 1];
+      if( e < s || e > nnz )
+         return SAP_INVALIDDATA;
+      if( lhs[i] != lhs[i] || rhs[i] != rhs[i] )
+         return SAP_INVALIDDATA;
+      sides[(size_t)i] = make_double2(lhs[i], rhs[i]);
+      bool allbin = true;
+      for( int64_t k = s; k < e; ++k )
+      {
+         int32_t c = colidx[k];
+         if( c < 0 || c >= ncols || vals[k] == 0.0 || vals[k] != vals[k] )
+            return SAP_INVALIDDATA;
+         if( !(colflags[(size_t)c] & 2) )
+            allbin = false;
+         colptr[(size_t)c + 1]++;
+      }
+      rowflags[(size_t)i] = allbin ? 1 : 0;
+      int64_t L = e - s;
+      if( L == 1 )
+         hasforced = 1;
+      int b = SAP_NBINS - 1;
+      for( int q = 0; q < SAP_NBINS - 1; ++q )
+         if( L <= kBinMaxLen[q] )
+         {
+            b = q;
+            break;
+         }
+      rowbin[(size_t)i] = (uint8_t)b;
+      bins[(size_t)b].push_back((int32_t)i);
+   }
+   for( int64_t j = 0; j < ncols; ++j )
+      colptr[(size_t)j + 1] += colptr[(size_t)j];
+   {
+      std::vector<int64_t> fill(colptr.begin(), colptr.end() - 1);
+      for( int64_t i = 0; i < nrows; ++i )
+         for( int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k )
+            rowidx[(size_t)fill[(size_t)colidx[k]]++] = (int32_t)i;
+   }
+   // duplicate columns within a row are not allowed (SCIP merges them)
+   {
+      std::vector<int64_t> seen((size_t)ncols, -1);
+      for( int64_t i = 0; i < nrows; ++i )
+         for( int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k )
+         {
+            if( seen[(size_t)colidx[k]] == i )
+               return SAP_INVALIDDATA;
+            seen[(size_t)colidx[k]] = i;
+         }
+   }
+
+   HIPCHK(hipSetDevice(ctx->device));
+   sap_matrix* m = (sap_matrix*)calloc(1, sizeof(sap_matrix));
+   if( m == NULL )
+      return SAP_NOMEMORY;
+   m->ctx = ctx;
+   m->nrows = nrows;
+   m->ncols = ncols;
+   m->nnz = nnz;
+   m->hasforced = hasforced;
+   int rc = SAP_OKAY;
+#define MALLOC_OR_FAIL(ptr, cnt)                           \
+   do                                                      \
+   {                                                       \
+      rc = devAlloc(&(ptr), (size_t)(cnt), &m->device_bytes); \
+      if( rc != SAP_OKAY )                                 \
+      {                                                    \
+         matrixRelease(m);                                 \
+         free(m);                                          \
+         return rc;                                        \
+      }                                                    \
+   } while( 0 )
+   MALLOC_OR_FAIL(m->rowptr, nrows + 1);
+   MALLOC_OR_FAIL(m->colidx, nnz);
+   MALLOC_OR_FAIL(m->vals, nnz);
+   MALLOC_OR_FAIL(m->sides, nrows);
+   MALLOC_OR_FAIL(m->colflags, ncols);
+   MALLOC_OR_FAIL(m->rowflags, nrows);
+   MALLOC_OR_FAIL(m->rowbin, nrows);
+   MALLOC_OR_FAIL(m->colptr, ncols + 1);
+   MALLOC_OR_FAIL(m->rowidx, nnz);
+   for( int b = 0; b < SAP_NBINS; ++b )
+   {
+      m->nbin[b] = (int32_t)bins[(size_t)b].size();
+      MALLOC_OR_FAIL(m->binrows[b], m->nbin[b]);
+   }
+#undef MALLOC_OR_FAIL
+   hipStream_t s = ctx->stream;
+   HIPCHK(hipMemcpyAsync(m->rowptr, rowptr, sizeof(int64_t) * (size_t)(nrows + 1), hipMemcpyHostToDevice, s));
+   if( nnz > 0 )
+   {
+      HIPCHK(hipMemcpyAsync(m->colidx, colidx, sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(m->vals, vals, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(m->rowidx, rowidx.data(), sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice, s));
+   }
+   if( nrows > 0 )
+   {
+      HIPCHK(hipMemcpyAsync(m->sides, sides.data(), sizeof(double2) * (size_t)nrows, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(m->rowflags, rowflags.data(), (size_t)nrows, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(m->rowbin, rowbin.data(), (size_t)nrows, hipMemcpyHostToDevice, s));
+   }
+   if( ncols > 0 )
+      HIPCHK(hipMemcpyAsync(m->colflags, colflags.data(), (size_t)ncols, hipMemcpyHostToDevice, s));
+   HIPCHK(hipMemcpyAsync(m->colptr, colptr.data(), sizeof(int64_t) * (size_t)(ncols + 1), hipMemcpyHostToDevice, s));
+   for( int b = 0; b < SAP_NBINS; ++b )
+      if( m->nbin[b] > 0 )
+         HIPCHK(hipMemcpyAsync(m->binrows[b], bins[(size_t)b].data(), sizeof(int32_t) * (size_t)m->nbin[b],
+            hipMemcpyHostToDevice, s));
+   HIPCHK(hipStreamSynchronize(s));
+   *out = m;
+   return SAP_OKAY;
+}
+
+int sap_matrix_free(sap_matrix* m)
+{
+   if( m == NULL )
+      return SAP_OKAY;
+   (void)hipSetDevice(m->ctx->device);
+   (void)hipStreamSynchronize(m->ctx->stream);
+   if( m->single )
+      sap_bounds_free(m->single);
+   matrixRelease(m);
+   free(m);
+   return SAP_OKAY;
+}
+
+int sap_matrix_info(const sap_matrix* m, int64_t* nrows, int64_t* ncols, int64_t* nnz, int64_t* device_bytes)
+{
+   if( m == NULL )
+      return SAP_INVALIDDATA;
+   if( nrows )
+      *nrows = m->nrows;
+   if( ncols )
+      *ncols = m->ncols;
+   if( nnz )
+      *nnz = m->nnz;
+   if( device_bytes )
+      *device_bytes = m->device_bytes;
+   return SAP_OKAY;
+}
+
+static void boundsRelease(sap_bounds* b)
+{
+   Vec& V = b->V;
+   (void)hipFree(V.bnd);
+   (void)hipFree(V.clb);
+   (void)hipFree(V.cub);
+   (void)hipFree(V.flb);
+   (void)hipFree(V.fub);
+   (void)hipFree(V.coltag);
+   (void)hipFree(V.rowmark);
+   (void)hipFree(V.redundant);
+   (void)hipFree(V.vcutoff);
+   (void)hipFree(V.vrounds);
+   for( int p = 0; p < 2; ++p )
+      for( int q = 0; q < SAP_NBINS; ++q )
+         (void)hipFree(V.wl[p][q]);
+   (void)hipFree(V.touched);
+   (void)hipFree(b->dstatus);
+   (void)hipFree(b->stage);
+}
+
+int sap_bounds_create(sap_matrix* m, int32_t nvec, sap_bounds** out)
+{
+   if( m == NULL || out == NULL || nvec < 1 )
+      return SAP_INVALIDDATA;
+   *out = NULL;
+   if( (int64_t)nvec * m->nrows > (int64_t)UINT32_MAX || (int64_t)nvec * m->ncols > (int64_t)UINT32_MAX )
+      return SAP_INVALIDDATA;
+   HIPCHK(hipSetDevice(m->ctx->device));
+   sap_bounds* b = (sap_bounds*)calloc(1, sizeof(sap_bounds));
+   if( b == NULL )
+      return SAP_NOMEMORY;
+   b->mat = m;
+   b->nvec = nvec;
+   Vec& V = b->V;
+   V.nvec = nvec;
+   V.single = (nvec == 1);
+   V.hasforced = m->hasforced;
+   const size_t NC = (size_t)nvec * (size_t)m->ncols;
+   const size_t NR = (size_t)nvec * (size_t)m->nrows;
+   int rc = SAP_OKAY;
+#define BALLOC(ptr, cnt)                                       \
+   do                                                          \
+   {                                                           \
+      rc = devAlloc(&(ptr), (size_t)(cnt), &b->device_bytes);  \
+      if( rc != SAP_OKAY )                                     \
+      {                                                        \
+         boundsRelease(b);                                     \
+         free(b);                                              \
+         return rc;                                            \
+      }                                                        \
+   } while( 0 )
+   BALLOC(V.bnd, NC);
+   BALLOC(V.clb, NC);
+   BALLOC(V.cub, NC);
+   if( m->hasforced )
+   {
+      BALLOC(V.flb, NC);
+      BALLOC(V.fub, NC);
+   }
+   BALLOC(V.coltag, NC);
+   BALLOC(V.rowmark, NR);
+   BALLOC(V.redundant, NR);
+   BALLOC(V.vcutoff, nvec);
+   BALLOC(V.vrounds, nvec);
+   for( int p = 0; p < 2; ++p )
+      for( int q = 0; q < SAP_NBINS; ++q )
+         BALLOC(V.wl[p][q], (size_t)nvec * (size_t)m->nbin[q]);
+   BALLOC(V.touched, NC);
+   BALLOC(b->dstatus, 1);
+   BALLOC(b->stage, 2 * (size_t)m->ncols);
+#undef BALLOC
+   V.st = b->dstatus;
+   hipStream_t s = m->ctx->stream;
+   k_fill_i64<<<gridFor(NC), SAP_BLOCK, 0, s>>>(V.clb, KEY_NEG_INF, NC);
+   k_fill_i64<<<gridFor(NC), SAP_BLOCK, 0, s>>>(V.cub, KEY_POS_INF, NC);
+   if( m->hasforced )
+   {
+      k_fill_i64<<<gridFor(NC), SAP_BLOCK, 0, s>>>(V.flb, KEY_NEG_INF, NC);
+      k_fill_i64<<<gridFor(NC), SAP_BLOCK, 0, s>>>(V.fub, KEY_POS_INF, NC);
+   }
+   k_fill_i32<<<gridFor(NC), SAP_BLOCK, 0, s>>>(V.coltag, 0, NC);
+   k_fill_i32<<<gridFor(NR), SAP_BLOCK, 0, s>>>(V.rowmark, 0, NR);
+   HIPCHK(hipGetLastError());
+   HIPCHK(hipMemsetAsync(b->dstatus, 0, sizeof(Status), s));
+   HIPCHK(hipStreamSynchronize(s));
+   *out = b;
+   return SAP_OKAY;
+}
+
+int sap_bounds_free(sap_bounds* b)
+{
+   if( b == NULL )
+      return SAP_OKAY;
+   (void)hipSetDevice(b->mat->ctx->device);
+   (void)hipStreamSynchronize(b->mat->ctx->stream);
+   boundsRelease(b);
+   free(b);
+   return SAP_OKAY;
+}
+
+int sap_bounds_upload(sap_bounds* b, int32_t v, const double* lb, const double* ub)
+{
+   if( b == NULL || lb == NULL || ub == NULL || v >= b->nvec )
+      return SAP_INVALIDDATA;
+   sap_matrix* m = b->mat;
+   hipStream_t s = m->ctx->stream;
+   HIPCHK(hipSetDevice(m->ctx->device));
+   const int64_t nc = m->ncols;
+   for( int64_t j = 0; j < nc; ++j )
+      if( lb[j] != lb[j] || ub[j] != ub[j] )
+         return SAP_INVALIDDATA;
+   HIPCHK(hipMemcpyAsync(b->stage, lb, sizeof(double) * (size_t)nc, hipMemcpyHostToDevice, s));
+   HIPCHK(hipMemcpyAsync(b->stage + nc, ub, sizeof(double) * (size_t)nc, hipMemcpyHostToDevice, s));
+   int32_t v0 = v < 0 ? 0 : v, v1 = v < 0 ? b->nvec : v + 1;
+   for( int32_t q = v0; q < v1; ++q )
+      k_interleave<<<gridFor((size_t)nc), SAP_BLOCK, 0, s>>>(b->V.bnd + (size_t)q * (size_t)nc, b->stage,
+         b->stage + nc, nc);
+   HIPCHK(hipGetLastError());
+   HIPCHK(hipStreamSynchronize(s));
+   return SAP_OKAY;
+}
+
+int sap_bounds_download(sap_bounds* b, int32_t v, double* lb, double* ub)
+{
+   if( b == NULL || lb == NULL || ub == NULL || v < 0 || v >= b->nvec )
+      return SAP_INVALIDDATA;
+   sap_matrix* m = b->mat;
+   hipStream_t s = m->ctx->stream;
+   HIPCHK(hipSetDevice(m->ctx->device));
+   const int64_t nc = m->ncols;
+   k_deinterleave<<<gridFor((size_t)nc), SAP_BLOCK, 0, s>>>(b->V.bnd + (size_t)v * (size_t)nc, b->stage,
+      b->stage + nc, nc);
+   HIPCHK(hipGetLastError());
+   HIPCHK(hipMemcpyAsync(lb, b->stage, sizeof(double) * (size_t)nc, hipMemcpyDeviceToHost, s));
+   HIPCHK(hipMemcpyAsync(ub, b->stage + nc, sizeof(double) * (size_t)nc, hipMemcpyDeviceToHost, s));
+   HIPCHK(hipStreamSynchronize(s));
+   return SAP_OKAY;
+}
+
+int sap_bounds_copy(sap_bounds* dst, int32_t dv, const sap_bounds* src, int32_t sv)
+{
+   if( dst == NULL || src == NULL || dst->mat != src->mat || dv < 0 || dv >= dst->nvec || sv < 0 || sv >= src->nvec )
+      return SAP_INVALIDDATA;
+   sap_matrix* m = dst->mat;
+   HIPCHK(hipSetDevice(m->ctx->device));
+   const size_t nc = (size_t)m->ncols;
+   HIPCHK(hipMemcpyAsync(dst->V.bnd + (size_t)dv * nc, src->V.bnd + (size_t)sv * nc, sizeof(double2) * nc,
+      hipMemcpyDeviceToDevice, m->ctx->stream));
+   return SAP_OKAY;
+}
+
+// grid partition of the sweep launch over the bins, by worklist capacity
+static Layout makeLayout(const sap_bounds* b)
+{
+   Layout L;
+   const sap_matrix* m = b->mat;
+   static const int kRowsPerBlock[SAP_NBINS] = { SAP_BLOCK / 4, SAP_BLOCK / 8, SAP_BLOCK / 16, SAP_BLOCK / 32,
+      SAP_BLOCK / 64, SAP_BLOCK / 64, 1 };
+   static const int kCap[SAP_NBINS] = { 4096, 4096, 4096, 4096, 4096, 4096, 2048 };
+   int acc = 0;
+   for( int q = 0; q < SAP_NBINS; ++q )
+   {
+      L.blockstart[q] = acc;
+      size_t items = (size_t)b->nvec * (size_t)m->nbin[q];
+      size_t blocks = (items + kRowsPerBlock[q] - 1) / kRowsPerBlock[q];
+      if( blocks > (size_t)kCap[q] )
+         blocks = (size_t)kCap[q];
+      acc += (int)blocks;
+   }
+   L.blockstart[SAP_NBINS] = acc;
+   return L;
+}
+
+// run rounds until the device reports done; host polls the status every `poll` rounds
+static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats)
+{
+   sap_matrix* m = b->mat;
+   sap_context* c = m->ctx;
+   hipStream_t s = c->stream;
+   const Mat M = toMat(m);
+   const Layout L = makeLayout(b);
+   const int nsweep = L.blockstart[SAP_NBINS];
+   const unsigned colgrid = gridFor((size_t)b->nvec * (size_t)m->ncols, 4096);
+   double sweepms = 0.0;
+   int64_t launches = 0;
+   HIPCHK(hipEventRecord(c->ev[0], s));
+   for( int it = 0;; ++it )
+   {
+      // every round tightens some bound strictly (boundstreps, forced rows, sign fixing), so
+      // the loop terminates; the cap guards against a defect turning into an endless loop
+      if( it > SAP_ROUND_CAP )
+      {
+         fprintf(stderr, "[scip_amd] round cap %d exceeded\n", SAP_ROUND_CAP);
+         return SAP_ERROR;
+      }
+      k_round_begin<<<1, 1, 0, s>>>(b->V, P.maxrounds);
+      HIPCHK(hipEventRecord(c->ev[2], s));
+      if( nsweep > 0 )
+         k_sweep<<<nsweep, SAP_BLOCK, 0, s>>>(M, b->V, P, L);
+      HIPCHK(hipEventRecord(c->ev[3], s));
+      k_columns<<<colgrid, SAP_BLOCK, 0, s>>>(M, b->V, P);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(c->hstatus, b->dstatus, sizeof(Status), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if( c->hstatus->done )
+         break;
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+      sweepms += ms;
+      ++launches;
+   }
+   HIPCHK(hipEventRecord(c->ev[1], s));
+   HIPCHK(hipEventSynchronize(c->ev[1]));
+   float tot = 0.0f;
+   HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[1]));
+   if( stats )
+   {
+      const Status* h = c->hstatus;
+      stats->rounds = h->rlocal;
+      stats->nchgbds = (int64_t)h->nchg;
+      stats->rows_swept = (int64_t)h->rows_swept;
+      stats->nnz_swept = (int64_t)h->nnz_swept;
+      stats->device_ms = tot;
+      stats->sweep_ms = sweepms;
+      stats->sweep_launches = launches;
+   }
+   return SAP_OKAY;
+}
+
+// prepare a call: reset per-call state and queue all rows of all vectors
+static int beginCall(sap_bounds* b)
+{
+   sap_matrix* m = b->mat;
+   hipStream_t s = m->ctx->stream;
+   const size_t NR = (size_t)b->nvec * (size_t)m->nrows;
+   k_call_reset<<<gridFor(NR > (size_t)b->nvec ? NR : (size_t)b->nvec), SAP_BLOCK, 0, s>>>(b->V, m->nrows, b->nvec);
+   HIPCHK(hipGetLastError());
+   return SAP_OKAY;
+}
+
+static int queueAllRows(sap_bounds* b, int32_t nvec)
+{
+   sap_matrix* m = b->mat;
+   hipStream_t s = m->ctx->stream;
+   for( int q = 0; q < SAP_NBINS; ++q )
+      if( m->nbin[q] > 0 )
+         k_init_all<<<gridFor((size_t)nvec * (size_t)m->nbin[q]), SAP_BLOCK, 0, s>>>(b->V, m->binrows[q], m->nbin[q],
+            q, nvec);
+   HIPCHK(hipGetLastError());
+   return SAP_OKAY;
+}
+
+static int checkParams(const sap_params* p)
+{
+   if( p == NULL )
+      return SAP_INVALIDDATA;
+   if( !(p->infinity > 0.0) || !(p->epsilon >= 0.0) || !(p->sumepsilon >= 0.0) || !(p->feastol >= 0.0)
+      || !(p->boundstreps >= 0.0) || !(p->hugeval > 0.0) || !(p->recompfac > 0.0) || p->maxrounds < -1 )
+      return SAP_INVALIDDATA;
+   return SAP_OKAY;
+}
+
+int sap_propagate_resident(sap_bounds* b, const sap_params* params, sap_stats* stats)
+{
+   if( b == NULL )
+      return SAP_INVALIDDATA;
+   RCCHK(checkParams(params));
+   sap_matrix* m = b->mat;
+   sap_context* c = m->ctx;
+   HIPCHK(hipSetDevice(c->device));
+   const Params P = toParams(params);
+   RCCHK(beginCall(b));
+   RCCHK(queueAllRows(b, 1));
+   RCCHK(runRounds(b, P, stats));
+   if( stats )
+      stats->result = c->hstatus->cutoff ? SAP_CUTOFF : (c->hstatus->nchg > 0 ? SAP_REDUCEDDOM : SAP_DIDNOTFIND);
+   return SAP_OKAY;
+}
+
+int sap_propagate(sap_matrix* m, const sap_params* params, double* lb, double* ub, sap_stats* stats)
+{
+   if( m == NULL || lb == NULL || ub == NULL )
+      return SAP_INVALIDDATA;
+   RCCHK(checkParams(params));
+   if( m->single == NULL )
+      RCCHK(sap_bounds_create(m, 1, &m->single));
+   RCCHK(sap_bounds_upload(m->single, 0, lb, ub));
+   sap_stats local;
+   memset(&local, 0, sizeof(local));
+   RCCHK(sap_propagate_resident(m->single, params, &local));
+   if( local.result != SAP_CUTOFF )
+      RCCHK(sap_bounds_download(m->single, 0, lb, ub));
+   if( stats )
+      *stats = local;
+   return SAP_OKAY;
+}
+
+int sap_propagate_batch(sap_bounds* b, const sap_params* params, int32_t nvec, const double* base_lb,
+   const double* base_ub, const int64_t* ovptr, const int32_t* ovcol, const double* ovlb, const double* ovub,
+   int32_t* result, int32_t* rounds, double* out_lb, double* out_ub, sap_stats* stats)
+{
+   if( b == NULL || base_lb == NULL || base_ub == NULL || nvec < 1 || nvec > b->nvec )
+      return SAP_INVALIDDATA;
+   RCCHK(checkParams(params));
+   sap_matrix* m = b->mat;
+   sap_context* c = m->ctx;
+   HIPCHK(hipSetDevice(c->device));
+   hipStream_t s = c->stream;
+   const int64_t nc = m->ncols;
+   const int64_t nov = ovptr ? ovptr[nvec] : 0;
+   if( ovptr != NULL && (ovptr[0] != 0 || nov < 0 || (nov > 0 && (ovcol == NULL || ovlb == NULL || ovub == NULL))) )
+      return SAP_INVALIDDATA;
+   for( int64_t q = 0; q < nov; ++q )
+      if( ovcol[q] < 0 || ovcol[q] >= nc || ovlb[q] != ovlb[q] || ovub[q] != ovub[q] )
+         return SAP_INVALIDDATA;
+   for( int32_t v = 0; ovptr && v < nvec; ++v )
+      if( ovptr[v + 1] < ovptr[v] )
+         return SAP_INVALIDDATA;
+   // stage the base vector and the overlays
+   int64_t* dptr = NULL;
+   int32_t* dcol = NULL;
+   double *dlb = NULL, *dub = NULL;
+   RCCHK(devAlloc(&dptr, (size_t)nvec + 1, NULL));
+   RCCHK(devAlloc(&dcol, (size_t)(nov > 0 ? nov : 1), NULL));
+   RCCHK(devAlloc(&dlb, (size_t)(nov > 0 ? nov : 1), NULL));
+   RCCHK(devAlloc(&dub, (size_t)(nov > 0 ? nov : 1), NULL));
+   std::vector<int64_t> zptr;
+   if( ovptr == NULL )
+      zptr.assign((size_t)nvec + 1, 0);
+   HIPCHK(hipMemcpyAsync(dptr, ovptr ? ovptr : zptr.data(), sizeof(int64_t) * ((size_t)nvec + 1),
+      hipMemcpyHostToDevice, s));
+   if( nov > 0 )
+   {
+      HIPCHK(hipMemcpyAsync(dcol, ovcol, sizeof(int32_t) * (size_t)nov, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(dlb, ovlb, sizeof(double) * (size_t)nov, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(dub, ovub, sizeof(double) * (size_t)nov, hipMemcpyHostToDevice, s));
+   }
+   HIPCHK(hipMemcpyAsync(b->stage, base_lb, sizeof(double) * (size_t)nc, hipMemcpyHostToDevice, s));
+   HIPCHK(hipMemcpyAsync(b->stage + nc, base_ub, sizeof(double) * (size_t)nc, hipMemcpyHostToDevice, s));
+   // base vector interleaved into the last vector slot's staging area then broadcast
+   double2* base2 = NULL;
+   RCCHK(devAlloc(&base2, (size_t)nc, NULL));
+   k_interleave<<<gridFor((size_t)nc), SAP_BLOCK, 0, s>>>(base2, b->stage, b->stage + nc, nc);
+   k_broadcast_base<<<gridFor((size_t)nvec * (size_t)nc), SAP_BLOCK, 0, s>>>(b->V.bnd, base2, nc, nvec);
+   dim3 og(gridFor(nov > 0 ? (size_t)(nov / nvec + 1) : 1, 64), (unsigned)std::min<int32_t>(nvec, 65535));
+   if( nov > 0 )
+      k_apply_overlay<<<og, SAP_BLOCK, 0, s>>>(b->V.bnd, nc, dptr, dcol, dlb, dub, nvec);
+   const Params P = toParams(params);
+   RCCHK(beginCall(b));
+   if( (params->flags & SAP_FLAG_BASE_IS_FIXPOINT) && nov > 0 )
+      k_init_overlay<<<og, SAP_BLOCK, 0, s>>>(toMat(m), b->V, dptr, dcol, nvec);
+   else if( !(params->flags & SAP_FLAG_BASE_IS_FIXPOINT) )
+      RCCHK(queueAllRows(b, nvec));
+   HIPCHK(hipGetLastError());
+   // the single-vector cutoff shortcut does not apply to batches
+   Vec saved = b->V;
+   b->V.single = (nvec == 1 && b->nvec == 1) ? 1 : 0;
+   int rc = runRounds(b, P, stats);
+   b->V = saved;
+   RCCHK(rc);
+   std::vector<int32_t> cut((size_t)nvec), rnd((size_t)nvec);
+   HIPCHK(hipMemcpyAsync(cut.data(), b->V.vcutoff, sizeof(int32_t) * (size_t)nvec, hipMemcpyDeviceToHost, s));
+   HIPCHK(hipMemcpyAsync(rnd.data(), b->V.vrounds, sizeof(int32_t) * (size_t)nvec, hipMemcpyDeviceToHost, s));
+   HIPCHK(hipStreamSynchronize(s));
+   // per-vector verdicts: changed bounds are detected by comparing with the start vector
+   if( result != NULL || out_lb != NULL )
+   {
+      std::vector<double> tlb((size_t)nc), tub((size_t)nc);
+      std::vector<double> slb((size_t)nc), sub((size_t)nc);
+      for( int32_t v = 0; v < nvec; ++v )
+      {
+         double* dstl = out_lb ? out_lb + (size_t)v * (size_t)nc : tlb.data();
+         double* dstu = out_ub ? out_ub + (size_t)v * (size_t)nc : tub.data();
+         RCCHK(sap_bounds_download(b, v, dstl, dstu));
+         if( result != NULL )
+         {
+            if( cut[(size_t)v] )
+               result[v] = SAP_CUTOFF;
+            else
+            {
+               memcpy(slb.data(), base_lb, sizeof(double) * (size_t)nc);
+               memcpy(sub.data(), base_ub, sizeof(double) * (size_t)nc);
+               if( ovptr )
+                  for( int64_t q = ovptr[v]; q < ovptr[v + 1]; ++q )
+                  {
+                     slb[(size_t)ovcol[q]] = ovlb[q];
+                     sub[(size_t)ovcol[q]] = ovub[q];
+                  }
+               bool changed = false;
+               for( int64_t j = 0; j < nc && !changed; ++j )
+                  changed = (dstl[j] != slb[(size_t)j]) || (dstu[j] != sub[(size_t)j]);
+               result[v] = changed ? SAP_REDUCEDDOM : SAP_DIDNOTFIND;
+            }
+         }
+      }
+   }
+   if( rounds != NULL )
+      memcpy(rounds, rnd.data(), sizeof(int32_t) * (size_t)nvec);
+   if( stats )
+   {
+      int32_t anycut = 0;
+      for( int32_t v = 0; v < nvec; ++v )
+         anycut |= cut[(size_t)v];
+      stats->result = anycut ? SAP_CUTOFF : (stats->nchgbds > 0 ? SAP_REDUCEDDOM : SAP_DIDNOTFIND);
+   }
+   (void)hipFree(dptr);
+   (void)hipFree(dcol);
+   (void)hipFree(dlb);
+   (void)hipFree(dub);
+   (void)hipFree(base2);
+   return SAP_OKAY;
+}
+
+int sap_row_activities(sap_matrix* m, const sap_params* params, const double* lb, const double* ub, double* minact,
+   double* maxact, int32_t* counts, double* maxactdelta)
+{
+   if( m == NULL || lb == NULL || ub == NULL )
+      return SAP_INVALIDDATA;
+   RCCHK(checkParams(params));
+   sap_context* c = m->ctx;
+   HIPCHK(hipSetDevice(c->device));
+   hipStream_t s = c->stream;
+   if( m->single == NULL )
+      RCCHK(sap_bounds_create(m, 1, &m->single));
+   RCCHK(sap_bounds_upload(m->single, 0, lb, ub));
+   const int64_t nr = m->nrows;
+   double *dmin = NULL, *dmax = NULL, *dmad = NULL;
+   int32_t* dcnt = NULL;
+   RCCHK(devAlloc(&dmin, (size_t)nr, NULL));
+   RCCHK(devAlloc(&dmax, (size_t)nr, NULL));
+   RCCHK(devAlloc(&dmad, (size_t)nr, NULL));
+   RCCHK(devAlloc(&dcnt, 8 * (size_t)nr, NULL));
+   const Params P = toParams(params);
+   if( nr > 0 )
+      k_row_activities<<<gridFor((size_t)nr * SAP_WAVE, 8192), SAP_BLOCK, 0, s>>>(toMat(m), P, m->single->V.bnd, dmin,
+         dmax, dcnt, dmad);
+   HIPCHK(hipGetLastError());
+   if( minact && nr > 0 )
+      HIPCHK(hipMemcpyAsync(minact, dmin, sizeof(double) * (size_t)nr, hipMemcpyDeviceToHost, s));
+   if( maxact && nr > 0 )
+      HIPCHK(hipMemcpyAsync(maxact, dmax, sizeof(double) * (size_t)nr, hipMemcpyDeviceToHost, s));
+   if( maxactdelta && nr > 0 )
+      HIPCHK(hipMemcpyAsync(maxactdelta, dmad, sizeof(double) * (size_t)nr, hipMemcpyDeviceToHost, s));
+   if( counts && nr > 0 )
+      HIPCHK(hipMemcpyAsync(counts, dcnt, sizeof(int32_t) * 8 * (size_t)nr, hipMemcpyDeviceToHost, s));
+   HIPCHK(hipStreamSynchronize(s));
+   (void)hipFree(dmin);
+   (void)hipFree(dmax);
+   (void)hipFree(dmad);
+   (void)hipFree(dcnt);
+   return SAP_OKAY;
+}
+
+}  // extern "C"

@@ -1,0 +1,231 @@
+"""GPU parity: the HIP engine against the oracle's Jacobi twin, through the C ABI.
+
+Integer/index/verdict work is compared exactly; bounds are compared with numeric equality
+(identical doubles, -0.0 == +0.0), i.e. bit-exact up to the sign of zero.
+"""
+import numpy as np
+import pytest
+
+import scip_amd as sa
+from oracle import oracle as orc
+from tests.util import (INF, apply_fixings, make_instance, random_binary_instance, random_fixings,
+                        random_integer_instance, same_bounds)
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def check_single(ctx, inst, lb=None, ub=None, name="", **kw):
+    say(f"[case] {name} rows={inst.nrows} cols={inst.ncols} nnz={inst.nnz} {kw}")
+    lb = inst.lb if lb is None else lb
+    ub = inst.ub if ub is None else ub
+    om = orc.OMatrix.from_instance(inst)
+    ol, ou, orr = orc.propagate_jacobi(om, lb, ub, orc.params(**kw))
+    mat = sa.Matrix(ctx, inst)
+    gl, gu, gs = mat.propagate(lb, ub, sa.default_params(**kw))
+    assert (gs["result"] == sa.SAP_CUTOFF) == bool(orr["cutoff"]), (gs, orr)
+    assert gs["rounds"] == orr["rounds"], (gs, orr)
+    if not orr["cutoff"]:
+        assert same_bounds(gl, ol) and same_bounds(gu, ou)
+        assert gs["nchgbds"] == orr["nchgbds"]
+        assert gs["rows_swept"] == orr["rows_swept"] and gs["nnz_swept"] == orr["nnz_swept"]
+        assert gs["result"] == (sa.SAP_REDUCEDDOM if orr["nchgbds"] > 0 else sa.SAP_DIDNOTFIND)
+    mat.close()
+    return gl, gu, gs
+
+
+# --- activities --------------------------------------------------------------------------------
+@pytest.mark.parametrize("kind", ["cover", "mixed", "long"])
+def test_row_activities_bit_exact(gpu_ctx, kind):
+    if kind == "cover":
+        inst = sa.gen_covering_packing(2000, 5000, 10, seed=1)
+    elif kind == "mixed":
+        inst = sa.gen_mixed(20000, 20000, seed=2)
+    else:
+        inst = sa.gen_mixed(300, 20000, seed=9, lenmode=1, minlen=60, maxlen=9000)
+    say(f"[case] activities {kind} nnz={inst.nnz}")
+    om = orc.OMatrix.from_instance(inst)
+    mn, mx, cnt, mad = orc.row_activities(om, inst.lb, inst.ub)
+    mat = sa.Matrix(gpu_ctx, inst)
+    gmn, gmx, gcnt, gmad = mat.row_activities()
+    assert np.array_equal(gcnt, cnt)
+    assert same_bounds(gmn, mn) and same_bounds(gmx, mx) and same_bounds(gmad, mad)
+    mat.close()
+
+
+# --- closed-form and edge cases ------------------------------------------------------------------
+EDGE = {
+    "packing": ([([(0, 1.0), (1, 1.0)], -INF, 1.0)], [1.0, 0.0], [1.0, 1.0], [1, 1]),
+    "cont_rhs": ([([(0, 1.0), (1, 1.0)], -INF, 10.0)], [0.0, 5.0], [100.0, 8.0], [0, 0]),
+    "boundstreps": ([([(0, 1.0), (1, 1.0)], -INF, 10.0)], [0.0, 0.0], [10.2, 10.0], [0, 0]),
+    "int_round": ([([(0, 2.0)], -INF, 5.0), ([(1, 3.0)], -INF, 5.9999999)], [0.0, 0.0], [10.0, 10.0], [1, 1]),
+    "neg_coef": ([([(0, -1.0), (1, 1.0)], 3.0, INF)], [0.0, 0.0], [10.0, 4.0], [1, 1]),
+    "singleton": ([([(0, 1.0)], -INF, 9.9)], [0.0], [10.0], [0]),
+    "infeasible": ([([(0, 1.0), (1, 1.0)], 3.0, INF)], [0.0, 0.0], [1.0, 1.0], [1, 1]),
+    "inf_bound": ([([(0, 1.0), (1, 1.0)], -INF, 10.0)], [0.0, -INF], [100.0, 5.0], [0, 0]),
+    "huge": ([([(0, 1.0), (1, 1e10)], -INF, 1e16)], [0.0, 0.0], [5.0, 1e6], [0, 0]),
+    "empty_ok": ([([], -1.0, 1.0), ([(0, 1.0)], -INF, 0.5)], [0.0], [1.0], [1]),
+    "empty_bad": ([([], 1.0, 2.0)], [0.0], [1.0], [1]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(EDGE))
+def test_edge_cases(gpu_ctx, name):
+    rows, lb, ub, it = EDGE[name]
+    check_single(gpu_ctx, make_instance(rows, lb, ub, it), name=name)
+
+
+def test_boundstreps_zero(gpu_ctx):
+    rows, lb, ub, it = EDGE["boundstreps"]
+    check_single(gpu_ctx, make_instance(rows, lb, ub, it), name="boundstreps0", boundstreps=0.0)
+
+
+@pytest.mark.parametrize("seed", range(25))
+def test_random_small_integer(gpu_ctx, seed):
+    rng = np.random.default_rng(seed)
+    inst = random_integer_instance(rng, nrows=int(rng.integers(1, 8)), ncols=int(rng.integers(2, 8)), maxlen=6)
+    check_single(gpu_ctx, inst, name=f"int{seed}")
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_binary_with_fixings(gpu_ctx, seed):
+    rng = np.random.default_rng(300 + seed)
+    inst = random_binary_instance(rng, nrows=400, ncols=300, maxlen=12)
+    check_single(gpu_ctx, inst, name=f"bin{seed}")
+
+
+# --- generated workloads -------------------------------------------------------------------------
+def test_config1_covering_packing(gpu_ctx):
+    inst = sa.gen_covering_packing(2000, 5000, 10, seed=1)
+    check_single(gpu_ctx, inst, name="cfg1")
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_mixed_lognormal(gpu_ctx, seed):
+    inst = sa.gen_mixed(20000, 20000, seed=seed)
+    check_single(gpu_ctx, inst, name=f"mixed{seed}")
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_mixed_planted_feasible(gpu_ctx, seed):
+    inst = sa.gen_mixed(60000, 60000, seed=seed, planted=True)
+    gl, gu, gs = check_single(gpu_ctx, inst, name=f"planted{seed}")
+    assert gs["result"] == sa.SAP_REDUCEDDOM and gs["rounds"] >= 3
+
+
+def test_mixed_long_rows_all_bins(gpu_ctx):
+    # rows 60..9000 long exercise the wavefront and workgroup bins
+    inst = sa.gen_mixed(400, 30000, seed=7, lenmode=1, minlen=60, maxlen=9000)
+    check_single(gpu_ctx, inst, name="longrows")
+
+
+def test_mixed_uneven_lengths(gpu_ctx):
+    inst = sa.gen_mixed(5000, 7500, seed=4004, lenmode=1, minlen=4, maxlen=64)
+    check_single(gpu_ctx, inst, name="uniform4-64")
+
+
+@pytest.mark.parametrize("maxrounds", [0, 1, 2])
+def test_maxrounds(gpu_ctx, maxrounds):
+    inst = sa.gen_mixed(5000, 5000, seed=3)
+    check_single(gpu_ctx, inst, name="maxrounds", maxrounds=maxrounds)
+
+
+def test_determinism_and_resident_path(gpu_ctx):
+    inst = sa.gen_mixed(50000, 50000, seed=5)
+    say("[case] determinism")
+    mat = sa.Matrix(gpu_ctx, inst)
+    a = mat.propagate()
+    b = mat.propagate()
+    assert a[0].tobytes() == b[0].tobytes() and a[1].tobytes() == b[1].tobytes()
+    bnd = sa.Bounds(mat, 1)
+    bnd.upload(inst.lb, inst.ub)
+    st = bnd.propagate()
+    lb, ub = bnd.download()
+    assert lb.tobytes() == a[0].tobytes() and ub.tobytes() == a[1].tobytes()
+    assert st["rounds"] == a[2]["rounds"]
+    # a fixpoint stays a fixpoint
+    c = mat.propagate(a[0], a[1])
+    assert c[2]["nchgbds"] == 0 and c[2]["rounds"] == 1
+    bnd.close()
+    mat.close()
+
+
+# --- batches -------------------------------------------------------------------------------------
+def test_batch_nodes_match_single_vector_oracle(gpu_ctx):
+    inst = sa.gen_covering_packing(2000, 3000, 10, seed=3)
+    rng = np.random.default_rng(3)
+    # tighten sides so fixings cascade: use the planted-point binary generator instead
+    inst = random_binary_instance(rng, nrows=3000, ncols=2000, maxlen=12)
+    om = orc.OMatrix.from_instance(inst)
+    rl, ru, rr = orc.propagate_jacobi(om, inst.lb, inst.ub)
+    free = np.nonzero(rl < ru)[0]
+    nvec = 48
+    ptr, cols, vals = random_fixings(rng, len(free), nvec, 6)
+    cols = free[cols].astype(np.int32)
+    say(f"[case] batch nvec={nvec} free={len(free)}")
+    mat = sa.Matrix(gpu_ctx, inst)
+    bnd = sa.Bounds(mat, nvec)
+    for flags in (0, sa.SAP_FLAG_BASE_IS_FIXPOINT):
+        out = bnd.propagate_batch(rl, ru, ptr, cols, vals, vals, sa.default_params(flags=flags))
+        for v in range(nvec):
+            lb, ub = apply_fixings(rl, ru, ptr, cols, vals, v)
+            ol, ou, orr = orc.propagate_jacobi(om, lb, ub)
+            assert (out["result"][v] == sa.SAP_CUTOFF) == bool(orr["cutoff"]), (v, flags)
+            if not orr["cutoff"]:
+                assert same_bounds(out["lb"][v], ol) and same_bounds(out["ub"][v], ou), (v, flags)
+                if flags == 0:
+                    assert out["rounds"][v] == orr["rounds"]
+    bnd.close()
+    mat.close()
+
+
+def test_batch_mixed_instance(gpu_ctx):
+    inst = sa.gen_mixed(8000, 8000, seed=11)
+    om = orc.OMatrix.from_instance(inst)
+    rng = np.random.default_rng(11)
+    nvec = 8
+    it = np.nonzero(inst.integral)[0]
+    ptr = np.arange(0, 3 * nvec + 1, 3, dtype=np.int64)
+    cols = rng.choice(it, size=3 * nvec, replace=False).astype(np.int32)
+    vals = np.zeros(3 * nvec)
+    say("[case] batch mixed")
+    mat = sa.Matrix(gpu_ctx, inst)
+    bnd = sa.Bounds(mat, nvec)
+    out = bnd.propagate_batch(inst.lb, inst.ub, ptr, cols, vals, vals)
+    for v in range(nvec):
+        lb, ub = apply_fixings(inst.lb, inst.ub, ptr, cols, vals, v)
+        ol, ou, orr = orc.propagate_jacobi(om, lb, ub)
+        assert (out["result"][v] == sa.SAP_CUTOFF) == bool(orr["cutoff"])
+        if not orr["cutoff"]:
+            assert same_bounds(out["lb"][v], ol) and same_bounds(out["ub"][v], ou)
+            assert out["rounds"][v] == orr["rounds"]
+    bnd.close()
+    mat.close()
+
+
+# --- input validation ------------------------------------------------------------------------------
+def test_invalid_inputs_rejected(gpu_ctx):
+    say("[case] invalid inputs")
+    good = make_instance([([(0, 1.0), (1, 2.0)], -INF, 3.0)], [0.0, 0.0], [1.0, 1.0], [1, 1])
+    dup = make_instance([([(0, 1.0), (1, 2.0)], -INF, 3.0)], [0.0, 0.0], [1.0, 1.0], [1, 1])
+    dup.colidx[1] = 0
+    with pytest.raises(sa.SapError) as e:
+        sa.Matrix(gpu_ctx, dup)
+    assert e.value.rc == sa.SAP_INVALIDDATA
+    bad = make_instance([([(0, 1.0), (1, 2.0)], -INF, 3.0)], [0.0, 0.0], [1.0, 1.0], [1, 1])
+    bad.colidx[1] = 7
+    with pytest.raises(sa.SapError):
+        sa.Matrix(gpu_ctx, bad)
+    nanv = make_instance([([(0, 1.0), (1, 2.0)], -INF, 3.0)], [0.0, 0.0], [1.0, 1.0], [1, 1])
+    nanv.vals[0] = np.nan
+    with pytest.raises(sa.SapError):
+        sa.Matrix(gpu_ctx, nanv)
+    mat = sa.Matrix(gpu_ctx, good)
+    with pytest.raises(sa.SapError):
+        mat.propagate(np.array([np.nan, 0.0]), np.array([1.0, 1.0]))
+    with pytest.raises(sa.SapError):
+        mat.propagate(params=sa.default_params(maxrounds=-5))
+    mat.close()
