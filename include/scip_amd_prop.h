@@ -110,6 +110,14 @@ int sap_propagate(sap_matrix* mat, const sap_params* params, double* lb, double*
 
 /* Device-resident bound vector(s), for callers that keep bounds in HBM across calls. */
 int sap_bounds_create(sap_matrix* mat, int32_t nvec, sap_bounds** bnd);
+/* as sap_bounds_create with an explicit work-tracking mode: SAP_MODE_DENSE scans round-tagged
+ * row/column marks (no list atomics; best for one or few vectors), SAP_MODE_SPARSE keeps
+ * per-round worklists (best for many sparse node vectors); SAP_MODE_AUTO chooses by size.
+ * Results are identical in every mode. */
+#define SAP_MODE_AUTO   0
+#define SAP_MODE_DENSE  1
+#define SAP_MODE_SPARSE 2
+int sap_bounds_create_mode(sap_matrix* mat, int32_t nvec, int32_t mode, sap_bounds** bnd);
 int sap_bounds_free(sap_bounds* bnd);
 /* set vector v (or all vectors if v < 0) from host arrays */
 int sap_bounds_upload(sap_bounds* bnd, int32_t v, const double* lb, const double* ub);

@@ -33,6 +33,10 @@ SAP_REDUCEDDOM = 10
 
 SAP_FLAG_BASE_IS_FIXPOINT = 0x1
 
+SAP_MODE_AUTO = 0
+SAP_MODE_DENSE = 1
+SAP_MODE_SPARSE = 2
+
 INFINITY = 1e20
 
 
@@ -108,6 +112,7 @@ EXPORTS = {
     "sap_matrix_info": (C.c_int, [_P, _I64P, _I64P, _I64P, _I64P]),
     "sap_propagate": (C.c_int, [_P, C.POINTER(sap_params), _DP, _DP, C.POINTER(sap_stats)]),
     "sap_bounds_create": (C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
+    "sap_bounds_create_mode": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(_P)]),
     "sap_bounds_free": (C.c_int, [_P]),
     "sap_bounds_upload": (C.c_int, [_P, C.c_int32, _DP, _DP]),
     "sap_bounds_download": (C.c_int, [_P, C.c_int32, _DP, _DP]),
@@ -330,11 +335,12 @@ class Matrix:
 class Bounds:
     """Device-resident bound vectors (sap_bounds)."""
 
-    def __init__(self, mat: Matrix, nvec: int = 1):
+    def __init__(self, mat: Matrix, nvec: int = 1, mode: int = 0):
+        """mode: SAP_MODE_AUTO (0), SAP_MODE_DENSE (1) or SAP_MODE_SPARSE (2)."""
         self.mat = mat
         self.nvec = nvec
         self.h = _P()
-        _chk(lib().sap_bounds_create(mat.h, nvec, C.byref(self.h)), "sap_bounds_create")
+        _chk(lib().sap_bounds_create_mode(mat.h, nvec, mode, C.byref(self.h)), "sap_bounds_create_mode")
 
     def upload(self, lb, ub, v: int = -1):
         lb = np.ascontiguousarray(lb, dtype=np.float64)

@@ -36,14 +36,17 @@
 
 using namespace sap;
 
-#define SAP_NBINS 7
+#define SAP_NBINS 8
+#define SAP_NKINDS (SAP_NBINS + 1)
 #define SAP_BLOCK 256
 #define SAP_WAVE 64
 #define SAP_MAXLEVELS 40
 #define SAP_ROUND_CAP 10000000
+#define SAP_STAT_SLOTS 256
 
-// bins: 0..4 -> group of 4,8,16,32,64 lanes; 5 -> wavefront per row; 6 -> workgroup per row
-static const int32_t kBinMaxLen[SAP_NBINS - 1] = { 4, 8, 16, 32, 64, 4096 };
+// row-length bins: 0 empty rows; 1..5 at most 4,8,16,32,64 nonzeros (64-nonzero tiles in dense
+// mode, groups of 4,8,16,32,64 lanes in sparse mode); 6 wavefront per row; 7 workgroup per row
+static const int32_t kBinMaxLen[SAP_NBINS - 1] = { 0, 4, 8, 16, 32, 64, 4096 };
 
 struct Status
 {
@@ -51,27 +54,45 @@ struct Status
    int32_t rlocal;      // rounds executed in the current call
    int32_t done;
    int32_t cutoff;      // single-vector mode: infeasibility detected
-   uint32_t wlcount[2][SAP_NBINS];
-   uint32_t touched;
-   uint32_t pad;
+   int32_t firstall;    // dense mode: the first round of the call sweeps every row
+   int32_t anymark[2];  // dense mode: some row is marked for the round of this parity
+   int32_t pad;
+   uint32_t wlcount[2][SAP_NBINS];   // sparse mode worklist lengths
+   uint32_t touched;                 // sparse mode touched-column count
+   uint32_t pad2;
    unsigned long long rows_swept;
    unsigned long long nnz_swept;
    unsigned long long nchg;
+   // per-workgroup statistics land in one of SAP_STAT_SLOTS slots (spread over cache lines
+   // so workgroups do not serialise on one address); the host sums them after the call
+   unsigned long long slot_nnz[SAP_STAT_SLOTS];
+   unsigned long long slot_rows[SAP_STAT_SLOTS];
+   unsigned long long slot_chg[SAP_STAT_SLOTS];
 };
 
 struct Mat
 {
    int64_t nrows;
    int64_t ncols;
-   const int64_t* rowptr;
+   const int64_t* rowptr;   // rows in bin order (stable): bin b is [binstart[b], binstart[b+1])
    const int32_t* colidx;
    const double* vals;
    const double2* sides;    // (lhs, rhs)
    const uint8_t* colflags; // bit0 integral, bit1 binary
    const uint8_t* rowflags; // bit0 all-binary row
    const uint8_t* rowbin;
-   const int64_t* colptr;   // CSC for row marking
+   const int64_t* colptr;   // CSC (bin-ordered row ids) for row marking
    const int32_t* rowidx;
+   int32_t binstart[SAP_NBINS + 1];
+   // tiles: the rows of bins 1..5 packed greedily into runs of at most 64 consecutive
+   // nonzeros; tile t holds nonzeros [tstart[t], tstart[t+1]) and rows trow0[t].., with bit i
+   // of tmask[t] set where nonzero tstart[t]+i starts a row
+   int32_t ntiles;
+   int32_t pad;
+   const int64_t* tstart;
+   const unsigned long long* tmask;
+   const int32_t* trow0;
+   const uint8_t* tlev;     // levels of the shift tree: ceil(log2(longest row of the tile))
 };
 
 struct Vec
@@ -79,25 +100,26 @@ struct Vec
    int32_t nvec;
    int32_t single;          // 1: one vector, a cutoff stops the whole call
    int32_t hasforced;
-   int32_t pad;
+   int32_t dense;           // 1: round-tagged marks scanned densely; 0: worklists
    double2* bnd;            // nvec*ncols (lb, ub)
    long long* clb;          // regular lb candidates (max)
    long long* cub;          // regular ub candidates (min)
    long long* flb;          // forced lb candidates (single-variable rows)
    long long* fub;
    int32_t* coltag;         // last round a column got a candidate
-   int32_t* rowmark;        // last round a row was queued for
+   int32_t* rowmark;        // round a row is marked for
    uint8_t* redundant;      // nvec*nrows
    int32_t* vcutoff;        // nvec
    int32_t* vrounds;        // nvec
-   int2* wl[2][SAP_NBINS];  // (row, vector) worklists, double-buffered by round parity
-   int2* touched;           // (column, vector)
+   int2* wl[2][SAP_NBINS];  // sparse mode: (row, vector) worklists, double-buffered by round parity
+   int2* touched;           // sparse mode: (column, vector)
    Status* st;
 };
 
+// block ranges of one sweep launch: kind 0 = tiles (dense mode), kind 1+b = bin b
 struct Layout
 {
-   int32_t blockstart[SAP_NBINS + 1];
+   int32_t blockstart[SAP_NKINDS + 1];
 };
 
 #define KEY_NEG_INF ((long long)0x800FFFFFFFFFFFFFLL)   // dkey(-inf)
@@ -309,6 +331,54 @@ __device__ __forceinline__ void residuals(const Params& p, const RowState& rs, d
 }
 
 // ------------------------------------------------------------------------------------------
+// wavefront-aggregated appends: one atomic per wavefront (and bin) instead of one per lane;
+// valid in divergent code, the currently active lanes cooperate
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t laneMaskLt()
+{
+   const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
+   return lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+}
+
+__device__ __forceinline__ void waveAppendTouched(Status* st, int2* list, bool pred, int2 item)
+{
+   const uint64_t m = __ballot(pred);
+   if( m == 0 )
+      return;
+   const int leader = __ffsll((unsigned long long)m) - 1;
+   const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
+   uint32_t base = 0;
+   if( lane == leader )
+      base = atomicAdd(&st->touched, (uint32_t)__popcll(m));
+   base = __shfl(base, leader, SAP_WAVE);
+   if( pred )
+      list[base + __popcll(m & laneMaskLt())] = item;
+}
+
+__device__ __forceinline__ void waveAppendRow(Status* st, int2* const* wl, int npar, bool pred, int bin, int2 item)
+{
+   uint64_t pend = __ballot(pred);
+   const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
+   while( pend )
+   {
+      const int leader = __ffsll((unsigned long long)pend) - 1;
+      const int lbin = __shfl(bin, leader, SAP_WAVE);
+      const bool mine = pred && bin == lbin;
+      const uint64_t grp = __ballot(mine);
+      uint32_t base = 0;
+      if( lane == leader )
+         base = atomicAdd(&st->wlcount[npar][lbin], (uint32_t)__popcll(grp));
+      base = __shfl(base, leader, SAP_WAVE);
+      if( mine )
+      {
+         wl[lbin][base + __popcll(grp & laneMaskLt())] = item;
+         pred = false;
+      }
+      pend &= ~grp;
+   }
+}
+
+// ------------------------------------------------------------------------------------------
 // candidates
 // ------------------------------------------------------------------------------------------
 struct Ctx
@@ -322,10 +392,16 @@ struct Ctx
 
 __device__ __forceinline__ void touch(const Ctx& C, size_t flat, int32_t j)
 {
-   if( atomicMax(&C.V->coltag[flat], C.round) < C.round )
+   if( C.V->dense )
    {
-      uint32_t q = atomicAdd(&C.V->st->touched, 1u);
-      C.V->touched[q] = make_int2(j, C.v);
+      // round tag scanned by k_columns; plain store, every writer stores the same value
+      if( C.V->coltag[flat] != C.round )
+         C.V->coltag[flat] = C.round;
+   }
+   else
+   {
+      const bool first = atomicMax(&C.V->coltag[flat], C.round) < C.round;
+      waveAppendTouched(C.V->st, C.V->touched, first, make_int2(j, C.v));
    }
 }
 
@@ -616,22 +692,114 @@ __device__ __forceinline__ Elem loadElem(const Mat& M, const double2* bnd, int64
 }
 
 // ------------------------------------------------------------------------------------------
-// sweep: groups of G lanes, one row per group, one nonzero per lane (rows of <= G nonzeros)
+// screening and the per-wavefront candidate queue
+//
+// Most nonzeros of a tightening row yield no candidate.  A cheap exact screen decides per
+// nonzero whether tightenVarBoundsEasy / tightenVarBounds could submit one; only those go to
+// a per-wavefront queue in LDS, which is drained 64 at a time through candEasy / candHard, so
+// the division-heavy exact code runs on full wavefronts of real work.
+//   easy rows: the alpha > slack tests of tightenVarBoundsEasy (cons_linear.c:5467, :5521, ...)
+//   hard rows: every acceptance test of a candidate ub = x/a requires ub_new < ub; with
+//     correctly rounded division, fl(x/a) >= ub whenever x/a >= ub exactly, which a single
+//     rounding fma(a, ub, -x) decides by its sign when it is strictly nonzero; symmetric for
+//     lower bounds.  Equality is left to the exact path.
 // ------------------------------------------------------------------------------------------
-template <int G>
-__device__ __forceinline__ void sweepGroups(const Mat& M, const Vec& V, const Params& P, const int2* items, uint32_t nitems,
-   int32_t round, int32_t rlocal, int lblock, int nblocks, unsigned long long& nnzacc)
+#define SAP_QCAP 128
+
+struct WaveQueue
 {
-   const int tid = lblock * SAP_BLOCK + (int)threadIdx.x;
-   const int glane = (int)threadIdx.x & (G - 1);
-   const uint32_t ngroups = (uint32_t)nblocks * (SAP_BLOCK / G);
-   for( uint32_t it = (uint32_t)(tid / G); it < nitems; it += ngroups )
+   int64_t* k;              // nonzero index
+   int32_t* row;
+   int32_t* v;
+   double* mnhi;
+   double* mnlo;
+   double* mxhi;
+   double* mxlo;
+   unsigned long long* cnt; // eight counters, 8 bits each, saturated at 255
+   uint32_t* flags;         // bit0 easy, bit1 force
+   int head;
+   int count;
+};
+
+__device__ __forceinline__ unsigned long long packCounts(const RowState& rs)
+{
+   unsigned long long v = 0;
+#pragma unroll
+   for( int q = 0; q < 8; ++q )
    {
-      const int2 item = items[it];
-      const int32_t row = item.x;
-      const int32_t v = item.y;
-      if( V.vcutoff[v] )
-         continue;
+      const unsigned long long c = (unsigned long long)(rs.c[q] > 255 ? 255 : rs.c[q]);
+      v |= c << (8 * q);
+   }
+   return v;
+}
+
+// true if the nonzero can produce a candidate (necessary condition, exact)
+__device__ __forceinline__ bool screenElem(const Params& p, const RowState& rs, const RowDecision& d, double lhs,
+   double rhs, const Elem& e)
+{
+   const double a = e.a, lb = e.lb, ub = e.ub;
+   if( d.easy )
+   {
+      const double minact = ddVal(rs.minact), maxact = ddVal(rs.maxact);
+      const double alpha = a > 0.0 ? a * (ub - lb) : a * (lb - ub);
+      bool pass = false;
+      if( !isInf(p, rhs) )
+      {
+         double slack = rhs - minact;
+         if( !isPositive(p, slack) )
+            slack = 0.0;
+         pass |= isSumGT(p, alpha, slack) || (d.force && isGT(p, alpha, slack));
+      }
+      if( !isInf(p, -lhs) )
+      {
+         double slack = maxact - lhs;
+         if( !isPositive(p, slack) )
+            slack = 0.0;
+         pass |= isSumGT(p, alpha, slack) || (d.force && isGT(p, alpha, slack));
+      }
+      return pass;
+   }
+   double minres, maxres;
+   bool mintight, maxtight, minsetinf, maxsetinf;
+   residuals(p, rs, a, lb, ub, minres, maxres, mintight, maxtight, minsetinf, maxsetinf);
+   bool pass = false;
+   if( !minsetinf && !isInf(p, rhs) && mintight )
+   {
+      const double x = rhs - minres;
+      // a > 0: ub candidate x/a must be < ub;  a < 0: lb candidate x/a must be > lb
+      pass |= a > 0.0 ? !(fma(a, ub, -x) < 0.0) : !(fma(a, lb, -x) < 0.0);
+   }
+   if( !maxsetinf && !isInf(p, -lhs) && maxtight )
+   {
+      const double y = lhs - maxres;
+      // a > 0: lb candidate y/a must be > lb;  a < 0: ub candidate y/a must be < ub
+      pass |= a > 0.0 ? !(fma(a, lb, -y) > 0.0) : !(fma(a, ub, -y) > 0.0);
+   }
+   return pass;
+}
+
+// exact candidates of queue entries [head, head+n) for lanes < n
+__device__ __forceinline__ void drainQueue(const Mat& M, const Vec& V, const Params& P, WaveQueue& Q, int n, int32_t round)
+{
+   const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
+   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+   if( lane < n )
+   {
+      const int q = (Q.head + lane) & (SAP_QCAP - 1);
+      const int64_t k = Q.k[q];
+      const int32_t row = Q.row[q];
+      const int32_t v = Q.v[q];
+      RowState rs;
+      rs.minact.hi = Q.mnhi[q];
+      rs.minact.lo = Q.mnlo[q];
+      rs.maxact.hi = Q.mxhi[q];
+      rs.maxact.lo = Q.mxlo[q];
+      const unsigned long long c = Q.cnt[q];
+#pragma unroll
+      for( int i = 0; i < 8; ++i )
+         rs.c[i] = (int)((c >> (8 * i)) & 0xFF);
+      rs.mad = 0.0;
+      const uint32_t fl = Q.flags[q];
       Ctx C;
       C.M = &M;
       C.V = &V;
@@ -639,11 +807,124 @@ __device__ __forceinline__ void sweepGroups(const Mat& M, const Vec& V, const Pa
       C.v = v;
       C.round = round;
       const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
-      const int64_t s = M.rowptr[row];
-      const int64_t n = M.rowptr[row + 1] - s;
+      const Elem e = loadElem(M, bnd, k);
       const double2 sides = M.sides[row];
-      const bool allbin = (M.rowflags[row] & 1) != 0;
-      const bool has = glane < n;
+      if( fl & 1 )
+         candEasy(C, rs, sides.x, sides.y, e, (fl & 2) != 0);
+      else
+      {
+         const int64_t s0 = M.rowptr[row];
+         const int64_t n0 = M.rowptr[row + 1] - s0;
+         candHard(C, rs, sides.x, sides.y, e, (fl & 2) != 0, s0, n0, k - s0);
+      }
+   }
+   Q.head = (Q.head + n) & (SAP_QCAP - 1);
+   Q.count -= n;
+}
+
+// append the lanes with pred; drain a full wavefront of work when one is available
+__device__ __forceinline__ void pushQueue(const Mat& M, const Vec& V, const Params& P, WaveQueue& Q, bool pred,
+   int64_t k, int32_t row, int32_t v, const RowState& rs, const RowDecision& d, int32_t round)
+{
+   const uint64_t m = __ballot(pred);
+   if( m == 0 )
+      return;
+   if( pred )
+   {
+      const int q = (Q.head + Q.count + __popcll(m & laneMaskLt())) & (SAP_QCAP - 1);
+      Q.k[q] = k;
+      Q.row[q] = row;
+      Q.v[q] = v;
+      Q.mnhi[q] = rs.minact.hi;
+      Q.mnlo[q] = rs.minact.lo;
+      Q.mxhi[q] = rs.maxact.hi;
+      Q.mxlo[q] = rs.maxact.lo;
+      Q.cnt[q] = packCounts(rs);
+      Q.flags[q] = (d.easy ? 1u : 0u) | (d.force ? 2u : 0u);
+   }
+   Q.count += __popcll(m);
+}
+
+// drain one full wavefront of queued work if available (end of a loop iteration, where few
+// values are live; the queue then holds at most 63 entries before the next push)
+__device__ __forceinline__ void maybeDrain(const Mat& M, const Vec& V, const Params& P, WaveQueue& Q, int32_t round)
+{
+   if( Q.count >= SAP_WAVE )
+      drainQueue(M, V, P, Q, SAP_WAVE, round);
+}
+
+// the row-level cutoff tests of tightenVarBoundsEasy (cons_linear.c:5450, :5504)
+__device__ __forceinline__ void easyRowCutoff(const Ctx& C, const RowState& rs, double lhs, double rhs)
+{
+   const Params& p = *C.P;
+   if( (!isInf(p, rhs) && isFeasLT(p, rhs, ddVal(rs.minact))) || (!isInf(p, -lhs) && isFeasLT(p, ddVal(rs.maxact), lhs)) )
+      setCutoff(C);
+}
+
+// ------------------------------------------------------------------------------------------
+// work items of one bin: dense mode enumerates (vector, row) over the bin's row range and
+// keeps the rows marked for this round; sparse mode reads the round's worklist
+// ------------------------------------------------------------------------------------------
+struct Items
+{
+   const int2* list;
+   uint32_t n;
+   int32_t binbase;
+   int32_t binlen;
+   int32_t round;
+   int32_t rlocal;
+   int32_t firstall;
+};
+
+__device__ __forceinline__ bool fetchItem(const Mat& M, const Vec& V, const Items& I, uint32_t it, int32_t& row,
+   int32_t& v)
+{
+   if( V.dense )
+   {
+      v = (int32_t)(it / (uint32_t)I.binlen);
+      row = I.binbase + (int32_t)(it - (uint32_t)v * (uint32_t)I.binlen);
+      if( !I.firstall && V.rowmark[(size_t)v * (size_t)M.nrows + (size_t)row] != I.round )
+         return false;
+   }
+   else
+   {
+      const int2 item = I.list[it];
+      row = item.x;
+      v = item.y;
+   }
+   return V.vcutoff[v] == 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// sweep: groups of G lanes, one row per group, one nonzero per lane (rows of <= G nonzeros)
+// ------------------------------------------------------------------------------------------
+template <int G>
+__device__ __forceinline__ void sweepGroups(const Mat& M, const Vec& V, const Params& P, const Items& I, int lblock,
+   int nblocks, unsigned long long& nnzacc, unsigned long long& rowsacc, WaveQueue& Q)
+{
+   const int tid = lblock * SAP_BLOCK + (int)threadIdx.x;
+   const int glane = (int)threadIdx.x & (G - 1);
+   const uint32_t ngroups = (uint32_t)nblocks * (SAP_BLOCK / G);
+   const int32_t rlocal = I.rlocal;
+   // the loop runs a wavefront-uniform number of times so that queue pushes stay convergent
+   const uint32_t base = (uint32_t)((tid & ~(SAP_WAVE - 1)) / G);
+   for( uint32_t it0 = base; it0 < I.n; it0 += ngroups )
+   {
+      const uint32_t it = it0 + (uint32_t)((tid & (SAP_WAVE - 1)) / G);
+      int32_t row = 0, v = 0;
+      const bool live = it < I.n && fetchItem(M, V, I, it, row, v);
+      Ctx C;
+      C.M = &M;
+      C.V = &V;
+      C.P = &P;
+      C.v = v;
+      C.round = I.round;
+      const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
+      const int64_t s = live ? M.rowptr[row] : 0;
+      const int64_t n = live ? M.rowptr[row + 1] - s : 0;
+      const double2 sides = live ? M.sides[row] : make_double2(0.0, 0.0);
+      const bool allbin = live && (M.rowflags[row] & 1) != 0;
+      const bool has = live && glane < n;
       Elem e;
       e.a = 0.0;
       e.lb = 0.0;
@@ -680,20 +961,141 @@ __device__ __forceinline__ void sweepGroups(const Mat& M, const Vec& V, const Pa
       rs.mad = inf ? P.infinity : mad;
 
       RowDecision d = decideRow(P, rs, sides.x, sides.y, n);
-      if( d.tighten && has )
+      const bool pass = has && d.tighten && screenElem(P, rs, d, sides.x, sides.y, e);
+      pushQueue(M, V, P, Q, pass, s + glane, row, v, rs, d, I.round);
+      if( live && glane == 0 )
       {
-         if( d.easy )
-            candEasy(C, rs, sides.x, sides.y, e, d.force);
-         else
-            candHard(C, rs, sides.x, sides.y, e, d.force, s, n, glane);
-      }
-      if( glane == 0 )
-      {
+         if( d.tighten && d.easy )
+            easyRowCutoff(C, rs, sides.x, sides.y);
          rowVerdict(C, rs, sides.x, sides.y, row);
          nnzacc += (unsigned long long)n;
-         if( V.vrounds[v] < rlocal )
+         rowsacc += 1;
+         if( !V.single && V.vrounds[v] < rlocal )
             atomicMax(&V.vrounds[v], rlocal);
       }
+      maybeDrain(M, V, P, Q, I.round);
+   }
+}
+
+// ------------------------------------------------------------------------------------------
+// sweep of 64-nonzero tiles (dense mode, rows of 1..64 nonzeros): a wavefront loads one tile,
+// lane i holds nonzero tstart+i at position p of its row; the canonical tree of every row is
+// formed by shifts: at width w a lane at a position aligned to 2w adds the partial held by the
+// lane w above it when that position is still inside the row
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ DD shflDownDD(DD x, int w)
+{
+   DD r;
+   r.hi = __shfl_down(x.hi, w, SAP_WAVE);
+   r.lo = __shfl_down(x.lo, w, SAP_WAVE);
+   return r;
+}
+
+__device__ __forceinline__ void sweepTiles(const Mat& M, const Vec& V, const Params& P, const Items& I, int lblock,
+   int nblocks, unsigned long long& nnzacc, unsigned long long& rowsacc, WaveQueue& Q)
+{
+   const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
+   const int wid = lblock * (SAP_BLOCK / SAP_WAVE) + (int)threadIdx.x / SAP_WAVE;
+   const uint32_t nw = (uint32_t)nblocks * (SAP_BLOCK / SAP_WAVE);
+   const uint32_t ntiles = (uint32_t)M.ntiles;
+   const uint32_t ntot = (uint32_t)V.nvec * ntiles;
+   const uint64_t lemask = (lane == 63) ? ~0ULL : ((2ULL << lane) - 1);
+   for( uint32_t t = (uint32_t)wid; t < ntot; t += nw )
+   {
+      const uint32_t tu = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+      const int32_t v = (int32_t)(tu / ntiles);
+      const int32_t tile = (int32_t)(tu - (uint32_t)v * ntiles);
+      if( V.vcutoff[v] )
+         continue;
+      const int64_t e0 = M.tstart[tile];
+      const int cnt = (int)(M.tstart[tile + 1] - e0);
+      const uint64_t mask = M.tmask[tile];
+      const int32_t row0 = M.trow0[tile];
+      const int lev = M.tlev[tile];
+      const bool has = lane < cnt;
+      // my row: start lane s, end lane ns (exclusive), position p
+      const uint64_t le = mask & lemask;
+      const int k = __popcll(le) - 1;
+      const int sl = 63 - __clzll((long long)le);
+      const uint64_t above = (sl >= 63) ? 0ULL : (mask >> (sl + 1));
+      const int ns = above ? (sl + 1 + __ffsll((unsigned long long)above) - 1) : cnt;
+      const int L = ns - sl;
+      const int p = lane - sl;
+      const int32_t row = row0 + k;
+      const bool marked = has
+         && (I.firstall || V.rowmark[(size_t)v * (size_t)M.nrows + (size_t)row] == I.round);
+      if( __ballot(marked) == 0 )
+         continue;
+
+      const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
+      Elem e;
+      e.a = 0.0;
+      e.lb = 0.0;
+      e.ub = 0.0;
+      e.j = 0;
+      e.cf = 0;
+      bool allbin = false;
+      double2 sides = make_double2(0.0, 0.0);
+      if( marked )
+      {
+         e = loadElem(M, bnd, e0 + lane);
+         allbin = (M.rowflags[row] & 1) != 0;
+         sides = M.sides[row];
+      }
+      double cmin = 0.0, cmax = 0.0, mad = 0.0;
+      int kmin = 0, kmax = 0;
+      bool madinf = false;
+      if( marked )
+         elemContrib(P, e, allbin, cmin, cmax, kmin, kmax, mad, madinf);
+      DD dmn = { cmin, 0.0 };
+      DD dmx = { cmax, 0.0 };
+      for( int w = 1; w < (1 << lev); w <<= 1 )
+      {
+         const DD omn = shflDownDD(dmn, w);
+         const DD omx = shflDownDD(dmx, w);
+         const double omad = __shfl_down(mad, w, SAP_WAVE);
+         if( (p & (2 * w - 1)) == 0 && p + w < L )
+         {
+            dmn = pairAdd(dmn, omn);
+            dmx = pairAdd(dmx, omx);
+            mad = dmax(mad, omad);
+         }
+      }
+      RowState rs;
+      rs.minact = shflDD(dmn, sl);
+      rs.maxact = shflDD(dmx, sl);
+      rs.mad = __shfl(mad, sl, SAP_WAVE);
+      const uint64_t rowbits = ((L >= 64) ? ~0ULL : ((1ULL << L) - 1)) << sl;
+      rs.c[0] = __popcll(__ballot(kmin == 1) & rowbits);
+      rs.c[1] = __popcll(__ballot(kmin == 2) & rowbits);
+      rs.c[2] = __popcll(__ballot(kmin == 3) & rowbits);
+      rs.c[3] = __popcll(__ballot(kmin == 4) & rowbits);
+      rs.c[4] = __popcll(__ballot(kmax == 1) & rowbits);
+      rs.c[5] = __popcll(__ballot(kmax == 2) & rowbits);
+      rs.c[6] = __popcll(__ballot(kmax == 3) & rowbits);
+      rs.c[7] = __popcll(__ballot(kmax == 4) & rowbits);
+      if( (__ballot(madinf) & rowbits) != 0 )
+         rs.mad = P.infinity;
+      Ctx C;
+      C.M = &M;
+      C.V = &V;
+      C.P = &P;
+      C.v = v;
+      C.round = I.round;
+      const RowDecision d = decideRow(P, rs, sides.x, sides.y, L);
+      const bool pass = marked && d.tighten && screenElem(P, rs, d, sides.x, sides.y, e);
+      pushQueue(M, V, P, Q, pass, e0 + lane, row, v, rs, d, I.round);
+      if( marked && p == 0 )
+      {
+         if( d.tighten && d.easy )
+            easyRowCutoff(C, rs, sides.x, sides.y);
+         rowVerdict(C, rs, sides.x, sides.y, row);
+         nnzacc += (unsigned long long)L;
+         rowsacc += 1;
+         if( !V.single && V.vrounds[v] < I.rlocal )
+            atomicMax(&V.vrounds[v], I.rlocal);
+      }
+      maybeDrain(M, V, P, Q, I.round);
    }
 }
 
@@ -820,42 +1222,47 @@ __device__ __forceinline__ WavePart waveSubtree(const Mat& M, const Params& P, c
    return w;
 }
 
-__device__ __forceinline__ void candidatesRange(const Ctx& C, const RowState& rs, const RowDecision& d, double lhs, double rhs,
-   const double2* bnd, int64_t s, int64_t n, int64_t c0, int64_t nch)
+__device__ __forceinline__ void candidatesRange(const Mat& M, const Vec& V, const Params& P, WaveQueue& Q,
+   const RowState& rs, const RowDecision& d, double lhs, double rhs, const double2* bnd, int64_t s, int64_t n,
+   int64_t c0, int64_t nch, int32_t row, int32_t v, int32_t round)
 {
    const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
    for( int64_t c = c0; c < c0 + nch; ++c )
    {
       const int64_t pos = c * SAP_WAVE + lane;
-      if( pos >= n )
-         break;
-      Elem e = loadElem(*C.M, bnd, s + pos);
-      if( d.easy )
-         candEasy(C, rs, lhs, rhs, e, d.force);
-      else
-         candHard(C, rs, lhs, rhs, e, d.force, s, n, pos);
+      const bool has = pos < n;
+      Elem e;
+      e.a = 1.0;
+      e.lb = 0.0;
+      e.ub = 0.0;
+      e.j = 0;
+      e.cf = 0;
+      if( has )
+         e = loadElem(M, bnd, s + pos);
+      const bool pass = has && screenElem(P, rs, d, lhs, rhs, e);
+      pushQueue(M, V, P, Q, pass, s + pos, row, v, rs, d, round);
+      maybeDrain(M, V, P, Q, round);
    }
 }
 
-__device__ __forceinline__ void sweepWaves(const Mat& M, const Vec& V, const Params& P, const int2* items, uint32_t nitems,
-   int32_t round, int32_t rlocal, int lblock, int nblocks, unsigned long long& nnzacc)
+__device__ __forceinline__ void sweepWaves(const Mat& M, const Vec& V, const Params& P, const Items& I, int lblock,
+   int nblocks, unsigned long long& nnzacc, unsigned long long& rowsacc, WaveQueue& Q)
 {
    const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
    const int wid = lblock * (SAP_BLOCK / SAP_WAVE) + (int)threadIdx.x / SAP_WAVE;
    const uint32_t nw = (uint32_t)nblocks * (SAP_BLOCK / SAP_WAVE);
-   for( uint32_t it = (uint32_t)wid; it < nitems; it += nw )
+   const int32_t rlocal = I.rlocal;
+   for( uint32_t it = (uint32_t)wid; it < I.n; it += nw )
    {
-      const int2 item = items[it];
-      const int32_t row = item.x;
-      const int32_t v = item.y;
-      if( V.vcutoff[v] )
+      int32_t row, v;
+      if( !fetchItem(M, V, I, it, row, v) )
          continue;
       Ctx C;
       C.M = &M;
       C.V = &V;
       C.P = &P;
       C.v = v;
-      C.round = round;
+      C.round = I.round;
       const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
       const int64_t s = M.rowptr[row];
       const int64_t n = M.rowptr[row + 1] - s;
@@ -872,39 +1279,41 @@ __device__ __forceinline__ void sweepWaves(const Mat& M, const Vec& V, const Par
       rs.mad = w.inf ? P.infinity : w.mad;
       RowDecision d = decideRow(P, rs, sides.x, sides.y, n);
       if( d.tighten )
-         candidatesRange(C, rs, d, sides.x, sides.y, bnd, s, n, 0, nch);
+         candidatesRange(M, V, P, Q, rs, d, sides.x, sides.y, bnd, s, n, 0, nch, row, v, I.round);
       if( lane == 0 )
       {
+         if( d.tighten && d.easy )
+            easyRowCutoff(C, rs, sides.x, sides.y);
          rowVerdict(C, rs, sides.x, sides.y, row);
          nnzacc += (unsigned long long)n;
-         if( V.vrounds[v] < rlocal )
+         rowsacc += 1;
+         if( !V.single && V.vrounds[v] < rlocal )
             atomicMax(&V.vrounds[v], rlocal);
       }
    }
 }
 
 // workgroup per row: wave w reduces the aligned chunk subtree [w*span, (w+1)*span)
-__device__ __forceinline__ void sweepBlocks(const Mat& M, const Vec& V, const Params& P, const int2* items, uint32_t nitems,
-   int32_t round, int32_t rlocal, int lblock, int nblocks, unsigned long long& nnzacc)
+__device__ __forceinline__ void sweepBlocks(const Mat& M, const Vec& V, const Params& P, const Items& I, int lblock,
+   int nblocks, unsigned long long& nnzacc, unsigned long long& rowsacc, WaveQueue& Q)
 {
    constexpr int NW = SAP_BLOCK / SAP_WAVE;
    __shared__ WavePart parts[NW];
    __shared__ RowState srs;
    const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
    const int wave = (int)threadIdx.x / SAP_WAVE;
-   for( uint32_t it = (uint32_t)lblock; it < nitems; it += (uint32_t)nblocks )
+   const int32_t rlocal = I.rlocal;
+   for( uint32_t it = (uint32_t)lblock; it < I.n; it += (uint32_t)nblocks )
    {
-      const int2 item = items[it];
-      const int32_t row = item.x;
-      const int32_t v = item.y;
-      if( V.vcutoff[v] )
+      int32_t row, v;
+      if( !fetchItem(M, V, I, it, row, v) )
          continue;   // block-uniform
       Ctx C;
       C.M = &M;
       C.V = &V;
       C.P = &P;
       C.v = v;
-      C.round = round;
+      C.round = I.round;
       const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
       const int64_t s = M.rowptr[row];
       const int64_t n = M.rowptr[row + 1] - s;
@@ -940,56 +1349,257 @@ __device__ __forceinline__ void sweepBlocks(const Mat& M, const Vec& V, const Pa
       RowState rs = srs;
       RowDecision d = decideRow(P, rs, sides.x, sides.y, n);
       if( d.tighten )
-         candidatesRange(C, rs, d, sides.x, sides.y, bnd, s, n, c0, mine);
+         candidatesRange(M, V, P, Q, rs, d, sides.x, sides.y, bnd, s, n, c0, mine, row, v, I.round);
       if( threadIdx.x == 0 )
       {
+         if( d.tighten && d.easy )
+            easyRowCutoff(C, rs, sides.x, sides.y);
          rowVerdict(C, rs, sides.x, sides.y, row);
          nnzacc += (unsigned long long)n;
-         if( V.vrounds[v] < rlocal )
+         rowsacc += 1;
+         if( !V.single && V.vrounds[v] < rlocal )
             atomicMax(&V.vrounds[v], rlocal);
       }
       __syncthreads();
    }
 }
 
-__global__ __launch_bounds__(SAP_BLOCK) void k_sweep(Mat M, Vec V, Params P, Layout L)
+// workgroup sum of two per-lane counters, added into the slot of this workgroup
+__device__ __forceinline__ void blockStats(unsigned long long a, unsigned long long b, unsigned long long* slota,
+   unsigned long long* slotb)
+{
+   __shared__ unsigned long long red[2][SAP_BLOCK / SAP_WAVE];
+#pragma unroll
+   for( int off = 1; off < SAP_WAVE; off <<= 1 )
+   {
+      a += (unsigned long long)__shfl_xor((long long)a, off, SAP_WAVE);
+      b += (unsigned long long)__shfl_xor((long long)b, off, SAP_WAVE);
+   }
+   const int wave = (int)threadIdx.x / SAP_WAVE;
+   if( ((int)threadIdx.x & (SAP_WAVE - 1)) == 0 )
+   {
+      red[0][wave] = a;
+      red[1][wave] = b;
+   }
+   __syncthreads();
+   if( threadIdx.x == 0 )
+   {
+      unsigned long long sa = 0, sb = 0;
+      for( int w = 0; w < SAP_BLOCK / SAP_WAVE; ++w )
+      {
+         sa += red[0][w];
+         sb += red[1][w];
+      }
+      const int slot = (int)(blockIdx.x & (SAP_STAT_SLOTS - 1));
+      if( sa != 0 )
+         atomicAdd(&slota[slot], sa);
+      if( sb != 0 && slotb != NULL )
+         atomicAdd(&slotb[slot], sb);
+   }
+}
+
+__global__ __launch_bounds__(SAP_BLOCK, 4) void k_sweep(Mat M, Vec V, Params P, Layout L)
 {
    Status* st = V.st;
    if( st->done )
       return;
-   const int32_t round = st->round;
-   const int32_t rlocal = st->rlocal;
-   const int par = round & 1;
    const int blk = (int)blockIdx.x;
-   int b = 0;
-   while( b < SAP_NBINS - 1 && blk >= L.blockstart[b + 1] )
-      ++b;
-   const int lblock = blk - L.blockstart[b];
-   const int nblocks = L.blockstart[b + 1] - L.blockstart[b];
-   const uint32_t nitems = st->wlcount[par][b];
-   const int2* items = V.wl[par][b];
-   unsigned long long nnzacc = 0;
-   switch( b )
+   int kind = 0;
+   while( kind < SAP_NKINDS - 1 && blk >= L.blockstart[kind + 1] )
+      ++kind;
+   const int lblock = blk - L.blockstart[kind];
+   const int nblocks = L.blockstart[kind + 1] - L.blockstart[kind];
+   const int b = kind - 1;   // row-length bin (kind 0: tiles)
+   Items I;
+   I.round = st->round;
+   I.rlocal = st->rlocal;
+   I.firstall = (V.dense && st->firstall && I.rlocal == 1) ? 1 : 0;
+   I.binbase = b >= 0 ? M.binstart[b] : 0;
+   I.binlen = b >= 0 ? M.binstart[b + 1] - M.binstart[b] : 0;
+   if( V.dense )
    {
-   case 0: sweepGroups<4>(M, V, P, items, nitems, round, rlocal, lblock, nblocks, nnzacc); break;
-   case 1: sweepGroups<8>(M, V, P, items, nitems, round, rlocal, lblock, nblocks, nnzacc); break;
-   case 2: sweepGroups<16>(M, V, P, items, nitems, round, rlocal, lblock, nblocks, nnzacc); break;
-   case 3: sweepGroups<32>(M, V, P, items, nitems, round, rlocal, lblock, nblocks, nnzacc); break;
-   case 4: sweepGroups<64>(M, V, P, items, nitems, round, rlocal, lblock, nblocks, nnzacc); break;
-   case 5: sweepWaves(M, V, P, items, nitems, round, rlocal, lblock, nblocks, nnzacc); break;
-   default: sweepBlocks(M, V, P, items, nitems, round, rlocal, lblock, nblocks, nnzacc); break;
+      I.list = NULL;
+      I.n = (uint32_t)V.nvec * (uint32_t)I.binlen;
    }
-   // one atomic per wavefront for the swept-nonzero statistic
-#pragma unroll
-   for( int off = 1; off < SAP_WAVE; off <<= 1 )
-      nnzacc += (unsigned long long)__shfl_xor((long long)nnzacc, off, SAP_WAVE);
-   if( ((int)threadIdx.x & (SAP_WAVE - 1)) == 0 && nnzacc != 0 )
-      atomicAdd(&st->nnz_swept, nnzacc);
+   else
+   {
+      I.list = b >= 0 ? V.wl[I.round & 1][b] : NULL;
+      I.n = b >= 0 ? st->wlcount[I.round & 1][b] : 0;
+   }
+   unsigned long long nnzacc = 0, rowsacc = 0;
+   constexpr int NW = SAP_BLOCK / SAP_WAVE;
+   __shared__ int64_t qk[NW][SAP_QCAP];
+   __shared__ int32_t qrow[NW][SAP_QCAP];
+   __shared__ int32_t qv[NW][SAP_QCAP];
+   __shared__ double qmnhi[NW][SAP_QCAP];
+   __shared__ double qmnlo[NW][SAP_QCAP];
+   __shared__ double qmxhi[NW][SAP_QCAP];
+   __shared__ double qmxlo[NW][SAP_QCAP];
+   __shared__ unsigned long long qcnt[NW][SAP_QCAP];
+   __shared__ uint32_t qflags[NW][SAP_QCAP];
+   const int wave = (int)threadIdx.x / SAP_WAVE;
+   WaveQueue Q;
+   Q.k = qk[wave];
+   Q.row = qrow[wave];
+   Q.v = qv[wave];
+   Q.mnhi = qmnhi[wave];
+   Q.mnlo = qmnlo[wave];
+   Q.mxhi = qmxhi[wave];
+   Q.mxlo = qmxlo[wave];
+   Q.cnt = qcnt[wave];
+   Q.flags = qflags[wave];
+   Q.head = 0;
+   Q.count = 0;
+   switch( kind )
+   {
+   case 0:
+      sweepTiles(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+      // empty rows (first round only; they are never marked again): activity 0 against the
+      // sides, propagateCons's infeasibility test (cons_linear.c:7721-7742)
+      if( I.firstall && lblock == 0 )
+      {
+         const int32_t nempty = M.binstart[1] - M.binstart[0];
+         bool bad = false;
+         for( int32_t r = M.binstart[0] + (int32_t)threadIdx.x; r < M.binstart[1]; r += SAP_BLOCK )
+         {
+            const double2 sd = M.sides[r];
+            bad |= isFeasGT(P, 0.0, sd.y) || isFeasLT(P, 0.0, sd.x);
+         }
+         if( bad )
+            for( int32_t v = 0; v < V.nvec; ++v )
+            {
+               V.vcutoff[v] = 1;
+               if( V.single )
+                  st->cutoff = 1;
+            }
+         if( threadIdx.x == 0 )
+            rowsacc += (unsigned long long)V.nvec * (unsigned long long)nempty;
+      }
+      break;
+   case 1: sweepGroups<4>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
+   case 2: sweepGroups<4>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
+   case 3: sweepGroups<8>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
+   case 4: sweepGroups<16>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
+   case 5: sweepGroups<32>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
+   case 6: sweepGroups<64>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
+   case 7: sweepWaves(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
+   default: sweepBlocks(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
+   }
+   // remaining queued candidates of this wavefront
+   if( Q.count > 0 )
+      drainQueue(M, V, P, Q, Q.count, I.round);
+   blockStats(nnzacc, rowsacc, st->slot_nnz, st->slot_rows);
 }
 
-// ------------------------------------------------------------------------------------------
-// column step
-// ------------------------------------------------------------------------------------------
+// the column step for one (column, vector) whose candidate slots may hold values;
+// returns the number of changed bounds
+__device__ __forceinline__ int columnStep(const Mat& M, const Vec& V, const Params& P, int32_t j, int32_t v,
+   int32_t next, int npar, bool& anymarked)
+{
+   Status* st = V.st;
+   const size_t flat = (size_t)v * (size_t)M.ncols + (size_t)j;
+   const long long kcl = V.clb[flat], kcu = V.cub[flat];
+   V.clb[flat] = KEY_NEG_INF;
+   V.cub[flat] = KEY_POS_INF;
+   long long kfl = KEY_NEG_INF, kfu = KEY_POS_INF;
+   if( V.hasforced )
+   {
+      kfl = V.flb[flat];
+      kfu = V.fub[flat];
+      V.flb[flat] = KEY_NEG_INF;
+      V.fub[flat] = KEY_POS_INF;
+   }
+   if( V.vcutoff[v] )
+      return 0;
+   const bool integral = (M.colflags[j] & 1) != 0;
+   const double2 b0 = V.bnd[flat];
+   const double lb0 = b0.x, ub0 = b0.y;
+   double nlb = lb0, nub = ub0;
+   bool cut = false;
+   // SCIPinferVarLbCons (scip_var.c:6965-7070): forced, then regular candidate
+   if( kfl != KEY_NEG_INF )
+   {
+      double tt = adjustedLb(P, integral, keyToDouble(kfl));
+      if( isInf(P, tt) || isFeasGT(P, tt, ub0) )
+         cut = true;
+      else
+      {
+         tt = dmin(tt, ub0);
+         if( !isLE(P, tt, lb0) && dkey(tt) > dkey(nlb) )
+            nlb = tt;
+      }
+   }
+   if( !cut && kcl != KEY_NEG_INF )
+   {
+      double tt = adjustedLb(P, integral, keyToDouble(kcl));
+      if( isInf(P, tt) || isFeasGT(P, tt, ub0) )
+         cut = true;
+      else
+      {
+         tt = dmin(tt, ub0);
+         if( isLbBetter(P, tt, lb0, ub0) && dkey(tt) > dkey(nlb) )
+            nlb = tt;
+      }
+   }
+   // SCIPinferVarUbCons (scip_var.c:7071-7174) against the new lower bound
+   if( !cut && kfu != KEY_POS_INF )
+   {
+      double tt = adjustedUb(P, integral, keyToDouble(kfu));
+      if( isInf(P, -tt) || isFeasLT(P, tt, nlb) )
+         cut = true;
+      else
+      {
+         tt = dmax(tt, nlb);
+         if( !isGE(P, tt, ub0) && dkey(tt) < dkey(nub) )
+            nub = tt;
+      }
+   }
+   if( !cut && kcu != KEY_POS_INF )
+   {
+      double tt = adjustedUb(P, integral, keyToDouble(kcu));
+      if( isInf(P, -tt) || isFeasLT(P, tt, nlb) )
+         cut = true;
+      else
+      {
+         tt = dmax(tt, nlb);
+         if( isUbBetter(P, tt, nlb, ub0) && dkey(tt) < dkey(nub) )
+            nub = tt;
+      }
+   }
+   if( cut )
+   {
+      V.vcutoff[v] = 1;
+      if( V.single )
+         st->cutoff = 1;
+      return 0;
+   }
+   const bool chl = dkey(nlb) != dkey(lb0);
+   const bool chu = dkey(nub) != dkey(ub0);
+   if( !chl && !chu )
+      return 0;
+   V.bnd[flat] = make_double2(nlb, nub);
+   // mark the rows of the column for the next round (eventExecLinear marks them)
+   for( int64_t q = M.colptr[j]; q < M.colptr[j + 1]; ++q )
+   {
+      const int32_t row = M.rowidx[q];
+      const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)row;
+      if( V.redundant[rflat] )
+         continue;
+      if( V.dense )
+      {
+         if( V.rowmark[rflat] != next )
+            V.rowmark[rflat] = next;
+         anymarked = true;
+      }
+      else
+      {
+         const bool first = atomicMax(&V.rowmark[rflat], next) < next;
+         waveAppendRow(st, V.wl[npar], npar, first, M.rowbin[row], make_int2(row, v));
+      }
+   }
+   return (chl ? 1 : 0) + (chu ? 1 : 0);
+}
+
 __global__ __launch_bounds__(SAP_BLOCK) void k_columns(Mat M, Vec V, Params P)
 {
    Status* st = V.st;
@@ -998,115 +1608,32 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_columns(Mat M, Vec V, Params P)
    const int32_t round = st->round;
    const int32_t next = round + 1;
    const int npar = next & 1;
-   const uint32_t n = st->touched;
    unsigned long long nchg = 0;
-   for( uint32_t idx = blockIdx.x * SAP_BLOCK + threadIdx.x; idx < n; idx += gridDim.x * SAP_BLOCK )
+   bool anymarked = false;
+   if( V.dense )
    {
-      const int2 t = V.touched[idx];
-      const int32_t j = t.x;
-      const int32_t v = t.y;
-      const size_t flat = (size_t)v * (size_t)M.ncols + (size_t)j;
-      const long long kcl = V.clb[flat], kcu = V.cub[flat];
-      V.clb[flat] = KEY_NEG_INF;
-      V.cub[flat] = KEY_POS_INF;
-      long long kfl = KEY_NEG_INF, kfu = KEY_POS_INF;
-      if( V.hasforced )
+      const size_t n = (size_t)V.nvec * (size_t)M.ncols;
+      for( size_t flat = blockIdx.x * (size_t)SAP_BLOCK + threadIdx.x; flat < n; flat += (size_t)gridDim.x * SAP_BLOCK )
       {
-         kfl = V.flb[flat];
-         kfu = V.fub[flat];
-         V.flb[flat] = KEY_NEG_INF;
-         V.fub[flat] = KEY_POS_INF;
-      }
-      if( V.vcutoff[v] )
-         continue;
-      const bool integral = (M.colflags[j] & 1) != 0;
-      const double2 b0 = V.bnd[flat];
-      const double lb0 = b0.x, ub0 = b0.y;
-      double nlb = lb0, nub = ub0;
-      bool cut = false;
-      // SCIPinferVarLbCons (scip_var.c:6965-7070): forced, then regular candidate
-      if( kfl != KEY_NEG_INF )
-      {
-         double tt = adjustedLb(P, integral, keyToDouble(kfl));
-         if( isInf(P, tt) || isFeasGT(P, tt, ub0) )
-            cut = true;
-         else
-         {
-            tt = dmin(tt, ub0);
-            if( !isLE(P, tt, lb0) && dkey(tt) > dkey(nlb) )
-               nlb = tt;
-         }
-      }
-      if( !cut && kcl != KEY_NEG_INF )
-      {
-         double tt = adjustedLb(P, integral, keyToDouble(kcl));
-         if( isInf(P, tt) || isFeasGT(P, tt, ub0) )
-            cut = true;
-         else
-         {
-            tt = dmin(tt, ub0);
-            if( isLbBetter(P, tt, lb0, ub0) && dkey(tt) > dkey(nlb) )
-               nlb = tt;
-         }
-      }
-      // SCIPinferVarUbCons (scip_var.c:7071-7174) against the new lower bound
-      if( !cut && kfu != KEY_POS_INF )
-      {
-         double tt = adjustedUb(P, integral, keyToDouble(kfu));
-         if( isInf(P, -tt) || isFeasLT(P, tt, nlb) )
-            cut = true;
-         else
-         {
-            tt = dmax(tt, nlb);
-            if( !isGE(P, tt, ub0) && dkey(tt) < dkey(nub) )
-               nub = tt;
-         }
-      }
-      if( !cut && kcu != KEY_POS_INF )
-      {
-         double tt = adjustedUb(P, integral, keyToDouble(kcu));
-         if( isInf(P, -tt) || isFeasLT(P, tt, nlb) )
-            cut = true;
-         else
-         {
-            tt = dmax(tt, nlb);
-            if( isUbBetter(P, tt, nlb, ub0) && dkey(tt) < dkey(nub) )
-               nub = tt;
-         }
-      }
-      if( cut )
-      {
-         V.vcutoff[v] = 1;
-         if( V.single )
-            st->cutoff = 1;
-         continue;
-      }
-      const bool chl = dkey(nlb) != dkey(lb0);
-      const bool chu = dkey(nub) != dkey(ub0);
-      if( !chl && !chu )
-         continue;
-      V.bnd[flat] = make_double2(nlb, nub);
-      nchg += (chl ? 1 : 0) + (chu ? 1 : 0);
-      // queue the rows of the column for the next round (eventExecLinear marks them)
-      for( int64_t q = M.colptr[j]; q < M.colptr[j + 1]; ++q )
-      {
-         const int32_t row = M.rowidx[q];
-         const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)row;
-         if( V.redundant[rflat] )
+         if( V.coltag[flat] != round )
             continue;
-         if( atomicMax(&V.rowmark[rflat], next) < next )
-         {
-            const int bin = M.rowbin[row];
-            const uint32_t pos = atomicAdd(&st->wlcount[npar][bin], 1u);
-            V.wl[npar][bin][pos] = make_int2(row, v);
-         }
+         const int32_t v = (int32_t)(flat / (size_t)M.ncols);
+         const int32_t j = (int32_t)(flat - (size_t)v * (size_t)M.ncols);
+         nchg += (unsigned long long)columnStep(M, V, P, j, v, next, npar, anymarked);
       }
    }
-#pragma unroll
-   for( int off = 1; off < SAP_WAVE; off <<= 1 )
-      nchg += (unsigned long long)__shfl_xor((long long)nchg, off, SAP_WAVE);
-   if( ((int)threadIdx.x & (SAP_WAVE - 1)) == 0 && nchg != 0 )
-      atomicAdd(&st->nchg, nchg);
+   else
+   {
+      const uint32_t n = st->touched;
+      for( uint32_t idx = blockIdx.x * SAP_BLOCK + threadIdx.x; idx < n; idx += gridDim.x * SAP_BLOCK )
+      {
+         const int2 t = V.touched[idx];
+         nchg += (unsigned long long)columnStep(M, V, P, t.x, t.y, next, npar, anymarked);
+      }
+   }
+   if( __ballot(anymarked) != 0 && ((int)threadIdx.x & (SAP_WAVE - 1)) == 0 && st->anymark[npar] == 0 )
+      st->anymark[npar] = 1;
+   blockStats(nchg, 0, st->slot_chg, NULL);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1118,24 +1645,31 @@ __global__ void k_round_begin(Vec V, int32_t maxrounds)
    if( st->done )
       return;
    const int32_t r = st->round + 1;
-   uint32_t in = 0;
-   for( int b = 0; b < SAP_NBINS; ++b )
-      in += st->wlcount[r & 1][b];
-   if( in == 0 || st->cutoff || (maxrounds >= 0 && st->rlocal >= maxrounds) )
+   bool more;
+   if( V.dense )
+      more = (st->rlocal == 0 && st->firstall) || st->anymark[r & 1];
+   else
+   {
+      uint32_t in = 0;
+      for( int b = 0; b < SAP_NBINS; ++b )
+         in += st->wlcount[r & 1][b];
+      more = in != 0;
+   }
+   if( !more || st->cutoff || (maxrounds >= 0 && st->rlocal >= maxrounds) )
    {
       st->done = 1;
       return;
    }
    st->round = r;
    st->rlocal += 1;
+   st->anymark[(r + 1) & 1] = 0;
    for( int b = 0; b < SAP_NBINS; ++b )
       st->wlcount[(r + 1) & 1][b] = 0;
    st->touched = 0;
-   st->rows_swept += in;
 }
 
 // per-call reset of the vector state; status counters for the call
-__global__ void k_call_reset(Vec V, int64_t nrows, int32_t nvec)
+__global__ void k_call_reset(Vec V, int64_t nrows, int32_t nvec, int32_t firstall)
 {
    const size_t tot = (size_t)nvec * (size_t)nrows;
    for( size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x )
@@ -1152,6 +1686,9 @@ __global__ void k_call_reset(Vec V, int64_t nrows, int32_t nvec)
       st->rlocal = 0;
       st->done = 0;
       st->cutoff = 0;
+      st->firstall = firstall;
+      st->anymark[0] = 0;
+      st->anymark[1] = 0;
       st->touched = 0;
       st->rows_swept = 0;
       st->nnz_swept = 0;
@@ -1162,27 +1699,36 @@ __global__ void k_call_reset(Vec V, int64_t nrows, int32_t nvec)
          st->wlcount[1][b] = 0;
       }
    }
+   if( gi < SAP_STAT_SLOTS )
+   {
+      V.st->slot_nnz[gi] = 0;
+      V.st->slot_rows[gi] = 0;
+      V.st->slot_chg[gi] = 0;
+   }
 }
 
-// initial worklists: every row of every vector (static bin lists replicated per vector)
-__global__ void k_init_all(Vec V, const int32_t* binrows, int32_t nbin, int32_t bin, int32_t nvec)
+// sparse mode: initial worklists hold every row of every vector
+__global__ void k_init_all(Mat M, Vec V, int32_t bin, int32_t nvec)
 {
    Status* st = V.st;
    const int par = (st->round + 1) & 1;
-   const size_t tot = (size_t)nvec * (size_t)nbin;
+   const int32_t base = M.binstart[bin];
+   const size_t len = (size_t)(M.binstart[bin + 1] - base);
+   const size_t tot = (size_t)nvec * len;
    int2* wl = V.wl[par][bin];
    for( size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x )
-      wl[i] = make_int2(binrows[i % (size_t)nbin], (int32_t)(i / (size_t)nbin));
+      wl[i] = make_int2(base + (int32_t)(i % len), (int32_t)(i / len));
    if( blockIdx.x == 0 && threadIdx.x == 0 )
       st->wlcount[par][bin] = (uint32_t)tot;
 }
 
-// initial worklists from overlay columns (base bounds already a fixpoint)
+// initial marks / worklists from overlay columns (base bounds already a fixpoint)
 __global__ void k_init_overlay(Mat M, Vec V, const int64_t* ovptr, const int32_t* ovcol, int32_t nvec)
 {
    Status* st = V.st;
    const int32_t first = st->round + 1;
    const int par = first & 1;
+   bool any = false;
    for( int32_t v = blockIdx.y; v < nvec; v += gridDim.y )
    {
       for( int64_t q = ovptr[v] + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ovptr[v + 1];
@@ -1193,15 +1739,21 @@ __global__ void k_init_overlay(Mat M, Vec V, const int64_t* ovptr, const int32_t
          {
             const int32_t row = M.rowidx[r];
             const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)row;
-            if( atomicMax(&V.rowmark[rflat], first) < first )
+            if( V.dense )
             {
-               const int bin = M.rowbin[row];
-               const uint32_t pos = atomicAdd(&st->wlcount[par][bin], 1u);
-               V.wl[par][bin][pos] = make_int2(row, v);
+               V.rowmark[rflat] = first;
+               any = true;
+            }
+            else
+            {
+               const bool fresh = atomicMax(&V.rowmark[rflat], first) < first;
+               waveAppendRow(st, V.wl[par], par, fresh, M.rowbin[row], make_int2(row, v));
             }
          }
       }
    }
+   if( __ballot(any) != 0 && ((int)threadIdx.x & (SAP_WAVE - 1)) == 0 )
+      st->anymark[par] = 1;
 }
 
 // bounds of all vectors from a base vector plus overlays
@@ -1250,8 +1802,8 @@ __global__ void k_deinterleave(const double2* src, double* lb, double* ub, int64
 }
 
 // row activities for diagnostics: one wavefront per row (chunked canonical tree)
-__global__ __launch_bounds__(SAP_BLOCK) void k_row_activities(Mat M, Params P, const double2* bnd, double* minact,
-   double* maxact, int32_t* counts, double* mad)
+__global__ __launch_bounds__(SAP_BLOCK) void k_row_activities(Mat M, Params P, const double2* bnd, const int32_t* perm,
+   double* minact, double* maxact, int32_t* counts, double* mad)
 {
    const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
    const int64_t wid = (blockIdx.x * (int64_t)SAP_BLOCK + threadIdx.x) / SAP_WAVE;
@@ -1264,11 +1816,12 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_row_activities(Mat M, Params P, c
       WavePart w = waveSubtree(M, P, bnd, s, n, 0, nch, (M.rowflags[row] & 1) != 0);
       if( lane == 0 )
       {
-         minact[row] = ddVal(w.minact);
-         maxact[row] = ddVal(w.maxact);
+         const int64_t orow = perm[row];
+         minact[orow] = ddVal(w.minact);
+         maxact[orow] = ddVal(w.maxact);
          for( int q = 0; q < 8; ++q )
-            counts[8 * row + q] = w.c[q];
-         mad[row] = w.inf ? P.infinity : w.mad;
+            counts[8 * orow + q] = w.c[q];
+         mad[orow] = w.inf ? P.infinity : w.mad;
       }
    }
 }
@@ -1299,8 +1852,13 @@ struct sap_matrix
    uint8_t* rowbin;
    int64_t* colptr;
    int32_t* rowidx;
-   int32_t* binrows[SAP_NBINS];
-   int32_t nbin[SAP_NBINS];
+   int32_t* perm;        // device row position -> caller's row index
+   int32_t binstart[SAP_NBINS + 1];
+   int32_t ntiles;
+   int64_t* tstart;
+   unsigned long long* tmask;
+   int32_t* trow0;
+   uint8_t* tlev;
    sap_bounds* single;   // cached one-vector state for sap_propagate
 };
 
@@ -1402,6 +1960,14 @@ static Mat toMat(const sap_matrix* m)
    M.rowbin = m->rowbin;
    M.colptr = m->colptr;
    M.rowidx = m->rowidx;
+   for( int b = 0; b <= SAP_NBINS; ++b )
+      M.binstart[b] = m->binstart[b];
+   M.ntiles = m->ntiles;
+   M.pad = 0;
+   M.tstart = m->tstart;
+   M.tmask = m->tmask;
+   M.trow0 = m->trow0;
+   M.tlev = m->tlev;
    return M;
 }
 
@@ -1499,8 +2065,11 @@ static void matrixRelease(sap_matrix* m)
    (void)hipFree(m->rowbin);
    (void)hipFree(m->colptr);
    (void)hipFree(m->rowidx);
-   for( int b = 0; b < SAP_NBINS; ++b )
-      (void)hipFree(m->binrows[b]);
+   (void)hipFree(m->perm);
+   (void)hipFree(m->tstart);
+   (void)hipFree(m->tmask);
+   (void)hipFree(m->trow0);
+   (void)hipFree(m->tlev);
 }
 
 int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int64_t* rowptr, const int32_t* colidx,
@@ -1514,13 +2083,9 @@ int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int6
    const int64_t nnz = rowptr[nrows];
    if( rowptr[0] != 0 || nnz < 0 || (nnz > 0 && (colidx == NULL || vals == NULL)) )
       return SAP_INVALIDDATA;
-   // host-side validation and derived arrays
-   std::vector<uint8_t> colflags((size_t)ncols), rowflags((size_t)nrows), rowbin((size_t)nrows);
-   std::vector<int64_t> colptr((size_t)ncols + 1, 0);
-   std::vector<int32_t> rowidx((size_t)nnz);
-   std::vector<std::vector<int32_t>> bins(SAP_NBINS);
-   std::vector<double2> sides((size_t)nrows);
-   int32_t hasforced = 0;
+   // host-side validation and derived arrays; rows are stored grouped by length bin
+   // (stable), so each bin is a contiguous range and sub-wavefront groups load contiguously
+   std::vector<uint8_t> colflags((size_t)ncols);
    for( int64_t j = 0; j < ncols; ++j )
    {
       if( glb[j] != glb[j] || gub[j] != gub[j] )
@@ -1529,56 +2094,133 @@ int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int6
       bool isbin = isint && glb[j] >= 0.0 && gub[j] <= 1.0;
       colflags[(size_t)j] = (uint8_t)((isint ? 1 : 0) | (isbin ? 2 : 0));
    }
-   for( int64_t i = 0; i < nrows; ++i )
+   std::vector<uint8_t> obin((size_t)nrows);
+   int32_t hasforced = 0;
+   int64_t bincount[SAP_NBINS] = { 0, 0, 0, 0, 0, 0, 0, 0 };
    {
-      int64_t s = rowptr[i], e = rowptr[i + 1];
-      if( e < s || e > nnz )
-         return SAP_INVALIDDATA;
-      if( lhs[i] != lhs[i] || rhs[i] != rhs[i] )
-         return SAP_INVALIDDATA;
-      sides[(size_t)i] = make_double2(lhs[i], rhs[i]);
-      bool allbin = true;
-      for( int64_t k = s; k < e; ++k )
+      std::vector<int64_t> seen((size_t)ncols, -1);
+      for( int64_t i = 0; i < nrows; ++i )
       {
-         int32_t c = colidx[k];
-         if( c < 0 || c >= ncols || vals[k] == 0.0 || vals[k] != vals[k] )
+         int64_t st0 = rowptr[i], e = rowptr[i + 1];
+         if( e < st0 || e > nnz )
             return SAP_INVALIDDATA;
-         if( !(colflags[(size_t)c] & 2) )
-            allbin = false;
-         colptr[(size_t)c + 1]++;
-      }
-      rowflags[(size_t)i] = allbin ? 1 : 0;
-      int64_t L = e - s;
-      if( L == 1 )
-         hasforced = 1;
-      int b = SAP_NBINS - 1;
-      for( int q = 0; q < SAP_NBINS - 1; ++q )
-         if( L <= kBinMaxLen[q] )
+         if( lhs[i] != lhs[i] || rhs[i] != rhs[i] )
+            return SAP_INVALIDDATA;
+         for( int64_t k = st0; k < e; ++k )
          {
-            b = q;
-            break;
+            int32_t c = colidx[k];
+            if( c < 0 || c >= ncols || vals[k] == 0.0 || vals[k] != vals[k] )
+               return SAP_INVALIDDATA;
+            // duplicate columns within a row are not allowed (SCIP merges them)
+            if( seen[(size_t)c] == i )
+               return SAP_INVALIDDATA;
+            seen[(size_t)c] = i;
          }
-      rowbin[(size_t)i] = (uint8_t)b;
-      bins[(size_t)b].push_back((int32_t)i);
+         int64_t L = e - st0;
+         if( L == 1 )
+            hasforced = 1;
+         int b = SAP_NBINS - 1;
+         for( int q = 0; q < SAP_NBINS - 1; ++q )
+            if( L <= kBinMaxLen[q] )
+            {
+               b = q;
+               break;
+            }
+         obin[(size_t)i] = (uint8_t)b;
+         bincount[b]++;
+      }
+   }
+   int32_t binstart[SAP_NBINS + 1];
+   binstart[0] = 0;
+   for( int b = 0; b < SAP_NBINS; ++b )
+      binstart[b + 1] = binstart[b] + (int32_t)bincount[b];
+   std::vector<int32_t> perm((size_t)nrows), iperm((size_t)nrows);
+   {
+      int32_t fill[SAP_NBINS];
+      for( int b = 0; b < SAP_NBINS; ++b )
+         fill[b] = binstart[b];
+      for( int64_t i = 0; i < nrows; ++i )
+      {
+         int32_t pos = fill[obin[(size_t)i]]++;
+         perm[(size_t)pos] = (int32_t)i;
+         iperm[(size_t)i] = pos;
+      }
+   }
+   std::vector<int64_t> prowptr((size_t)nrows + 1);
+   std::vector<int32_t> pcol((size_t)nnz);
+   std::vector<double> pval((size_t)nnz);
+   std::vector<double2> sides((size_t)nrows);
+   std::vector<uint8_t> rowflags((size_t)nrows), rowbin((size_t)nrows);
+   std::vector<int64_t> colptr((size_t)ncols + 1, 0);
+   std::vector<int32_t> rowidx((size_t)nnz);
+   prowptr[0] = 0;
+   for( int64_t r = 0; r < nrows; ++r )
+   {
+      const int64_t i = perm[(size_t)r];
+      const int64_t st0 = rowptr[i], L = rowptr[i + 1] - st0, o = prowptr[(size_t)r];
+      bool allbin = true;
+      for( int64_t k = 0; k < L; ++k )
+      {
+         pcol[(size_t)(o + k)] = colidx[st0 + k];
+         pval[(size_t)(o + k)] = vals[st0 + k];
+         if( !(colflags[(size_t)colidx[st0 + k]] & 2) )
+            allbin = false;
+         colptr[(size_t)colidx[st0 + k] + 1]++;
+      }
+      prowptr[(size_t)r + 1] = o + L;
+      sides[(size_t)r] = make_double2(lhs[i], rhs[i]);
+      rowflags[(size_t)r] = allbin ? 1 : 0;
+      rowbin[(size_t)r] = obin[(size_t)i];
    }
    for( int64_t j = 0; j < ncols; ++j )
       colptr[(size_t)j + 1] += colptr[(size_t)j];
    {
       std::vector<int64_t> fill(colptr.begin(), colptr.end() - 1);
-      for( int64_t i = 0; i < nrows; ++i )
-         for( int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k )
-            rowidx[(size_t)fill[(size_t)colidx[k]]++] = (int32_t)i;
+      for( int64_t r = 0; r < nrows; ++r )
+         for( int64_t k = prowptr[(size_t)r]; k < prowptr[(size_t)r + 1]; ++k )
+            rowidx[(size_t)fill[(size_t)pcol[(size_t)k]]++] = (int32_t)r;
    }
-   // duplicate columns within a row are not allowed (SCIP merges them)
+
+   // 64-nonzero tiles over the rows of bins 1..5 (consecutive, non-empty, <= 64 nonzeros)
+   std::vector<int64_t> tstart;
+   std::vector<unsigned long long> tmask;
+   std::vector<int32_t> trow0;
+   std::vector<uint8_t> tlev;
    {
-      std::vector<int64_t> seen((size_t)ncols, -1);
-      for( int64_t i = 0; i < nrows; ++i )
-         for( int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k )
+      int cnt = 0, maxl = 0;
+      unsigned long long mask = 0;
+      auto levels = [](int l) {
+         int v = 0;
+         while( (1 << v) < l )
+            ++v;
+         return (uint8_t)v;
+      };
+      for( int32_t r = binstart[1]; r < binstart[6]; ++r )
+      {
+         const int L = (int)(prowptr[(size_t)r + 1] - prowptr[(size_t)r]);
+         if( cnt == 0 || cnt + L > 64 )
          {
-            if( seen[(size_t)colidx[k]] == i )
-               return SAP_INVALIDDATA;
-            seen[(size_t)colidx[k]] = i;
+            if( cnt > 0 )
+            {
+               tmask.push_back(mask);
+               tlev.push_back(levels(maxl));
+            }
+            tstart.push_back(prowptr[(size_t)r]);
+            trow0.push_back(r);
+            cnt = 0;
+            maxl = 0;
+            mask = 0;
          }
+         mask |= 1ULL << cnt;
+         cnt += L;
+         maxl = L > maxl ? L : maxl;
+      }
+      if( cnt > 0 )
+      {
+         tmask.push_back(mask);
+         tlev.push_back(levels(maxl));
+      }
+      tstart.push_back(prowptr[(size_t)binstart[6]]);
    }
 
    HIPCHK(hipSetDevice(ctx->device));
@@ -1586,10 +2228,13 @@ int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int6
    if( m == NULL )
       return SAP_NOMEMORY;
    m->ctx = ctx;
+   m->ntiles = (int32_t)tmask.size();
    m->nrows = nrows;
    m->ncols = ncols;
    m->nnz = nnz;
    m->hasforced = hasforced;
+   for( int b = 0; b <= SAP_NBINS; ++b )
+      m->binstart[b] = binstart[b];
    int rc = SAP_OKAY;
 #define MALLOC_OR_FAIL(ptr, cnt)                           \
    do                                                      \
@@ -1611,18 +2256,18 @@ int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int6
    MALLOC_OR_FAIL(m->rowbin, nrows);
    MALLOC_OR_FAIL(m->colptr, ncols + 1);
    MALLOC_OR_FAIL(m->rowidx, nnz);
-   for( int b = 0; b < SAP_NBINS; ++b )
-   {
-      m->nbin[b] = (int32_t)bins[(size_t)b].size();
-      MALLOC_OR_FAIL(m->binrows[b], m->nbin[b]);
-   }
+   MALLOC_OR_FAIL(m->perm, nrows);
+   MALLOC_OR_FAIL(m->tstart, tstart.size());
+   MALLOC_OR_FAIL(m->tmask, tmask.size());
+   MALLOC_OR_FAIL(m->trow0, trow0.size());
+   MALLOC_OR_FAIL(m->tlev, tlev.size());
 #undef MALLOC_OR_FAIL
    hipStream_t s = ctx->stream;
-   HIPCHK(hipMemcpyAsync(m->rowptr, rowptr, sizeof(int64_t) * (size_t)(nrows + 1), hipMemcpyHostToDevice, s));
+   HIPCHK(hipMemcpyAsync(m->rowptr, prowptr.data(), sizeof(int64_t) * (size_t)(nrows + 1), hipMemcpyHostToDevice, s));
    if( nnz > 0 )
    {
-      HIPCHK(hipMemcpyAsync(m->colidx, colidx, sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice, s));
-      HIPCHK(hipMemcpyAsync(m->vals, vals, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(m->colidx, pcol.data(), sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(m->vals, pval.data(), sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice, s));
       HIPCHK(hipMemcpyAsync(m->rowidx, rowidx.data(), sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice, s));
    }
    if( nrows > 0 )
@@ -1630,14 +2275,18 @@ int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int6
       HIPCHK(hipMemcpyAsync(m->sides, sides.data(), sizeof(double2) * (size_t)nrows, hipMemcpyHostToDevice, s));
       HIPCHK(hipMemcpyAsync(m->rowflags, rowflags.data(), (size_t)nrows, hipMemcpyHostToDevice, s));
       HIPCHK(hipMemcpyAsync(m->rowbin, rowbin.data(), (size_t)nrows, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(m->perm, perm.data(), sizeof(int32_t) * (size_t)nrows, hipMemcpyHostToDevice, s));
    }
    if( ncols > 0 )
       HIPCHK(hipMemcpyAsync(m->colflags, colflags.data(), (size_t)ncols, hipMemcpyHostToDevice, s));
    HIPCHK(hipMemcpyAsync(m->colptr, colptr.data(), sizeof(int64_t) * (size_t)(ncols + 1), hipMemcpyHostToDevice, s));
-   for( int b = 0; b < SAP_NBINS; ++b )
-      if( m->nbin[b] > 0 )
-         HIPCHK(hipMemcpyAsync(m->binrows[b], bins[(size_t)b].data(), sizeof(int32_t) * (size_t)m->nbin[b],
-            hipMemcpyHostToDevice, s));
+   HIPCHK(hipMemcpyAsync(m->tstart, tstart.data(), sizeof(int64_t) * tstart.size(), hipMemcpyHostToDevice, s));
+   if( !tmask.empty() )
+   {
+      HIPCHK(hipMemcpyAsync(m->tmask, tmask.data(), sizeof(unsigned long long) * tmask.size(), hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(m->trow0, trow0.data(), sizeof(int32_t) * trow0.size(), hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(m->tlev, tlev.data(), tlev.size(), hipMemcpyHostToDevice, s));
+   }
    HIPCHK(hipStreamSynchronize(s));
    *out = m;
    return SAP_OKAY;
@@ -1692,9 +2341,18 @@ static void boundsRelease(sap_bounds* b)
    (void)hipFree(b->stage);
 }
 
+// dense marks are scanned every round: worthwhile while the scan (4 B per row and per column
+// and vector) stays small against the row sweep itself
+#define SAP_DENSE_LIMIT ((size_t)1 << 24)
+
 int sap_bounds_create(sap_matrix* m, int32_t nvec, sap_bounds** out)
 {
-   if( m == NULL || out == NULL || nvec < 1 )
+   return sap_bounds_create_mode(m, nvec, SAP_MODE_AUTO, out);
+}
+
+int sap_bounds_create_mode(sap_matrix* m, int32_t nvec, int32_t mode, sap_bounds** out)
+{
+   if( m == NULL || out == NULL || nvec < 1 || mode < SAP_MODE_AUTO || mode > SAP_MODE_SPARSE )
       return SAP_INVALIDDATA;
    *out = NULL;
    if( (int64_t)nvec * m->nrows > (int64_t)UINT32_MAX || (int64_t)nvec * m->ncols > (int64_t)UINT32_MAX )
@@ -1711,6 +2369,10 @@ int sap_bounds_create(sap_matrix* m, int32_t nvec, sap_bounds** out)
    V.hasforced = m->hasforced;
    const size_t NC = (size_t)nvec * (size_t)m->ncols;
    const size_t NR = (size_t)nvec * (size_t)m->nrows;
+   if( mode == SAP_MODE_AUTO )
+      V.dense = (nvec == 1 || NC + NR <= SAP_DENSE_LIMIT) ? 1 : 0;
+   else
+      V.dense = (mode == SAP_MODE_DENSE) ? 1 : 0;
    int rc = SAP_OKAY;
 #define BALLOC(ptr, cnt)                                       \
    do                                                          \
@@ -1736,10 +2398,13 @@ int sap_bounds_create(sap_matrix* m, int32_t nvec, sap_bounds** out)
    BALLOC(V.redundant, NR);
    BALLOC(V.vcutoff, nvec);
    BALLOC(V.vrounds, nvec);
-   for( int p = 0; p < 2; ++p )
-      for( int q = 0; q < SAP_NBINS; ++q )
-         BALLOC(V.wl[p][q], (size_t)nvec * (size_t)m->nbin[q]);
-   BALLOC(V.touched, NC);
+   if( !V.dense )
+   {
+      for( int p = 0; p < 2; ++p )
+         for( int q = 0; q < SAP_NBINS; ++q )
+            BALLOC(V.wl[p][q], (size_t)nvec * (size_t)(m->binstart[q + 1] - m->binstart[q]));
+      BALLOC(V.touched, NC);
+   }
    BALLOC(b->dstatus, 1);
    BALLOC(b->stage, 2 * (size_t)m->ncols);
 #undef BALLOC
@@ -1828,20 +2493,28 @@ static Layout makeLayout(const sap_bounds* b)
 {
    Layout L;
    const sap_matrix* m = b->mat;
-   static const int kRowsPerBlock[SAP_NBINS] = { SAP_BLOCK / 4, SAP_BLOCK / 8, SAP_BLOCK / 16, SAP_BLOCK / 32,
-      SAP_BLOCK / 64, SAP_BLOCK / 64, 1 };
-   static const int kCap[SAP_NBINS] = { 4096, 4096, 4096, 4096, 4096, 4096, 2048 };
+   const size_t nvec = (size_t)b->V.nvec;
+   // items per workgroup and block caps per kind (kind 0 tiles, kind 1+b bin b)
+   static const int kPerBlock[SAP_NKINDS] = { SAP_BLOCK / SAP_WAVE, SAP_BLOCK / 4, SAP_BLOCK / 4, SAP_BLOCK / 8,
+      SAP_BLOCK / 16, SAP_BLOCK / 32, SAP_BLOCK / 64, SAP_BLOCK / SAP_WAVE, 1 };
+   static const int kCap[SAP_NKINDS] = { 4096, 4096, 4096, 4096, 4096, 4096, 4096, 4096, 2048 };
    int acc = 0;
-   for( int q = 0; q < SAP_NBINS; ++q )
+   for( int q = 0; q < SAP_NKINDS; ++q )
    {
       L.blockstart[q] = acc;
-      size_t items = (size_t)b->nvec * (size_t)m->nbin[q];
-      size_t blocks = (items + kRowsPerBlock[q] - 1) / kRowsPerBlock[q];
+      size_t items;
+      if( q == 0 )
+         items = b->V.dense ? nvec * (size_t)(m->ntiles > 0 ? m->ntiles : (m->binstart[1] > m->binstart[0] ? 1 : 0)) : 0;
+      else if( b->V.dense && q <= 6 )
+         items = 0;   // bins 0..5 go through the tiles in dense mode
+      else
+         items = nvec * (size_t)(m->binstart[q] - m->binstart[q - 1]);
+      size_t blocks = (items + kPerBlock[q] - 1) / kPerBlock[q];
       if( blocks > (size_t)kCap[q] )
          blocks = (size_t)kCap[q];
       acc += (int)blocks;
    }
-   L.blockstart[SAP_NBINS] = acc;
+   L.blockstart[SAP_NKINDS] = acc;
    return L;
 }
 
@@ -1853,7 +2526,7 @@ static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats)
    hipStream_t s = c->stream;
    const Mat M = toMat(m);
    const Layout L = makeLayout(b);
-   const int nsweep = L.blockstart[SAP_NBINS];
+   const int nsweep = L.blockstart[SAP_NKINDS];
    const unsigned colgrid = gridFor((size_t)b->nvec * (size_t)m->ncols, 4096);
    double sweepms = 0.0;
    int64_t launches = 0;
@@ -1887,6 +2560,20 @@ static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats)
    HIPCHK(hipEventSynchronize(c->ev[1]));
    float tot = 0.0f;
    HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[1]));
+   {
+      // fold the per-workgroup statistic slots into the host copy of the status
+      Status* h = c->hstatus;
+      unsigned long long a = 0, r = 0, g = 0;
+      for( int q = 0; q < SAP_STAT_SLOTS; ++q )
+      {
+         a += h->slot_nnz[q];
+         r += h->slot_rows[q];
+         g += h->slot_chg[q];
+      }
+      h->nnz_swept = a;
+      h->rows_swept = r;
+      h->nchg = g;
+   }
    if( stats )
    {
       const Status* h = c->hstatus;
@@ -1901,25 +2588,30 @@ static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats)
    return SAP_OKAY;
 }
 
-// prepare a call: reset per-call state and queue all rows of all vectors
-static int beginCall(sap_bounds* b)
+// prepare a call on the first nvec vectors: reset per-call state; with firstall the first
+// round sweeps every row (dense mode), else rows are queued explicitly
+static int beginCall(sap_bounds* b, int32_t nvec, int32_t firstall)
 {
    sap_matrix* m = b->mat;
    hipStream_t s = m->ctx->stream;
-   const size_t NR = (size_t)b->nvec * (size_t)m->nrows;
-   k_call_reset<<<gridFor(NR > (size_t)b->nvec ? NR : (size_t)b->nvec), SAP_BLOCK, 0, s>>>(b->V, m->nrows, b->nvec);
+   const size_t NR = (size_t)nvec * (size_t)m->nrows;
+   k_call_reset<<<gridFor(NR > (size_t)nvec ? NR : (size_t)nvec), SAP_BLOCK, 0, s>>>(b->V, m->nrows, nvec, firstall);
    HIPCHK(hipGetLastError());
    return SAP_OKAY;
 }
 
+// sparse mode: all rows of the first nvec vectors form the first round's worklists
 static int queueAllRows(sap_bounds* b, int32_t nvec)
 {
    sap_matrix* m = b->mat;
    hipStream_t s = m->ctx->stream;
+   const Mat M = toMat(m);
    for( int q = 0; q < SAP_NBINS; ++q )
-      if( m->nbin[q] > 0 )
-         k_init_all<<<gridFor((size_t)nvec * (size_t)m->nbin[q]), SAP_BLOCK, 0, s>>>(b->V, m->binrows[q], m->nbin[q],
-            q, nvec);
+   {
+      const int32_t len = m->binstart[q + 1] - m->binstart[q];
+      if( len > 0 )
+         k_init_all<<<gridFor((size_t)nvec * (size_t)len), SAP_BLOCK, 0, s>>>(M, b->V, q, nvec);
+   }
    HIPCHK(hipGetLastError());
    return SAP_OKAY;
 }
@@ -1943,9 +2635,16 @@ int sap_propagate_resident(sap_bounds* b, const sap_params* params, sap_stats* s
    sap_context* c = m->ctx;
    HIPCHK(hipSetDevice(c->device));
    const Params P = toParams(params);
-   RCCHK(beginCall(b));
-   RCCHK(queueAllRows(b, 1));
-   RCCHK(runRounds(b, P, stats));
+   Vec saved = b->V;
+   b->V.nvec = 1;
+   b->V.single = 1;
+   int rc = beginCall(b, 1, b->V.dense);
+   if( rc == SAP_OKAY && !b->V.dense )
+      rc = queueAllRows(b, 1);
+   if( rc == SAP_OKAY )
+      rc = runRounds(b, P, stats);
+   b->V = saved;
+   RCCHK(rc);
    if( stats )
       stats->result = c->hstatus->cutoff ? SAP_CUTOFF : (c->hstatus->nchg > 0 ? SAP_REDUCEDDOM : SAP_DIDNOTFIND);
    return SAP_OKAY;
@@ -2020,16 +2719,22 @@ int sap_propagate_batch(sap_bounds* b, const sap_params* params, int32_t nvec, c
    if( nov > 0 )
       k_apply_overlay<<<og, SAP_BLOCK, 0, s>>>(b->V.bnd, nc, dptr, dcol, dlb, dub, nvec);
    const Params P = toParams(params);
-   RCCHK(beginCall(b));
-   if( (params->flags & SAP_FLAG_BASE_IS_FIXPOINT) && nov > 0 )
-      k_init_overlay<<<og, SAP_BLOCK, 0, s>>>(toMat(m), b->V, dptr, dcol, nvec);
-   else if( !(params->flags & SAP_FLAG_BASE_IS_FIXPOINT) )
-      RCCHK(queueAllRows(b, nvec));
-   HIPCHK(hipGetLastError());
-   // the single-vector cutoff shortcut does not apply to batches
+   // the call works on the first nvec vectors; the single-vector cutoff shortcut only
+   // applies when the batch is one vector
    Vec saved = b->V;
-   b->V.single = (nvec == 1 && b->nvec == 1) ? 1 : 0;
-   int rc = runRounds(b, P, stats);
+   b->V.nvec = nvec;
+   b->V.single = (nvec == 1) ? 1 : 0;
+   const bool fromfix = (params->flags & SAP_FLAG_BASE_IS_FIXPOINT) != 0;
+   int rc = beginCall(b, nvec, (!fromfix && b->V.dense) ? 1 : 0);
+   if( rc == SAP_OKAY && fromfix && nov > 0 )
+   {
+      k_init_overlay<<<og, SAP_BLOCK, 0, s>>>(toMat(m), b->V, dptr, dcol, nvec);
+      rc = hipToRc(hipGetLastError());
+   }
+   else if( rc == SAP_OKAY && !fromfix && !b->V.dense )
+      rc = queueAllRows(b, nvec);
+   if( rc == SAP_OKAY )
+      rc = runRounds(b, P, stats);
    b->V = saved;
    RCCHK(rc);
    std::vector<int32_t> cut((size_t)nvec), rnd((size_t)nvec);
@@ -2106,8 +2811,8 @@ int sap_row_activities(sap_matrix* m, const sap_params* params, const double* lb
    RCCHK(devAlloc(&dcnt, 8 * (size_t)nr, NULL));
    const Params P = toParams(params);
    if( nr > 0 )
-      k_row_activities<<<gridFor((size_t)nr * SAP_WAVE, 8192), SAP_BLOCK, 0, s>>>(toMat(m), P, m->single->V.bnd, dmin,
-         dmax, dcnt, dmad);
+      k_row_activities<<<gridFor((size_t)nr * SAP_WAVE, 8192), SAP_BLOCK, 0, s>>>(toMat(m), P, m->single->V.bnd, m->perm,
+         dmin, dmax, dcnt, dmad);
    HIPCHK(hipGetLastError());
    if( minact && nr > 0 )
       HIPCHK(hipMemcpyAsync(minact, dmin, sizeof(double) * (size_t)nr, hipMemcpyDeviceToHost, s));

@@ -167,8 +167,10 @@ def test_batch_nodes_match_single_vector_oracle(gpu_ctx):
     cols = free[cols].astype(np.int32)
     say(f"[case] batch nvec={nvec} free={len(free)}")
     mat = sa.Matrix(gpu_ctx, inst)
-    bnd = sa.Bounds(mat, nvec)
-    for flags in (0, sa.SAP_FLAG_BASE_IS_FIXPOINT):
+    for mode, flags in [(m, f) for m in (sa.SAP_MODE_DENSE, sa.SAP_MODE_SPARSE)
+                        for f in (0, sa.SAP_FLAG_BASE_IS_FIXPOINT)]:
+        say(f"[case] batch mode={mode} flags={flags}")
+        bnd = sa.Bounds(mat, nvec, mode)
         out = bnd.propagate_batch(rl, ru, ptr, cols, vals, vals, sa.default_params(flags=flags))
         for v in range(nvec):
             lb, ub = apply_fixings(rl, ru, ptr, cols, vals, v)
@@ -178,6 +180,24 @@ def test_batch_nodes_match_single_vector_oracle(gpu_ctx):
                 assert same_bounds(out["lb"][v], ol) and same_bounds(out["ub"][v], ou), (v, flags)
                 if flags == 0:
                     assert out["rounds"][v] == orr["rounds"]
+        bnd.close()
+    mat.close()
+
+
+@pytest.mark.parametrize("mode", ["dense", "sparse"])
+def test_single_vector_both_modes(gpu_ctx, mode):
+    inst = sa.gen_mixed(30000, 30000, seed=21, planted=True)
+    say(f"[case] single vector mode={mode}")
+    om = orc.OMatrix.from_instance(inst)
+    ol, ou, orr = orc.propagate_jacobi(om, inst.lb, inst.ub)
+    mat = sa.Matrix(gpu_ctx, inst)
+    bnd = sa.Bounds(mat, 1, sa.SAP_MODE_DENSE if mode == "dense" else sa.SAP_MODE_SPARSE)
+    bnd.upload(inst.lb, inst.ub)
+    st = bnd.propagate()
+    lb, ub = bnd.download()
+    assert st["rounds"] == orr["rounds"] and st["nchgbds"] == orr["nchgbds"]
+    assert st["rows_swept"] == orr["rows_swept"] and st["nnz_swept"] == orr["nnz_swept"]
+    assert same_bounds(lb, ol) and same_bounds(ub, ou)
     bnd.close()
     mat.close()
 
@@ -193,16 +213,20 @@ def test_batch_mixed_instance(gpu_ctx):
     vals = np.zeros(3 * nvec)
     say("[case] batch mixed")
     mat = sa.Matrix(gpu_ctx, inst)
-    bnd = sa.Bounds(mat, nvec)
-    out = bnd.propagate_batch(inst.lb, inst.ub, ptr, cols, vals, vals)
-    for v in range(nvec):
-        lb, ub = apply_fixings(inst.lb, inst.ub, ptr, cols, vals, v)
-        ol, ou, orr = orc.propagate_jacobi(om, lb, ub)
-        assert (out["result"][v] == sa.SAP_CUTOFF) == bool(orr["cutoff"])
-        if not orr["cutoff"]:
-            assert same_bounds(out["lb"][v], ol) and same_bounds(out["ub"][v], ou)
-            assert out["rounds"][v] == orr["rounds"]
-    bnd.close()
+    for mode in (sa.SAP_MODE_DENSE, sa.SAP_MODE_SPARSE):
+        bnd = sa.Bounds(mat, nvec, mode)
+        out = bnd.propagate_batch(inst.lb, inst.ub, ptr, cols, vals, vals)
+        for v in range(nvec):
+            lb, ub = apply_fixings(inst.lb, inst.ub, ptr, cols, vals, v)
+            ol, ou, orr = orc.propagate_jacobi(om, lb, ub)
+            assert (out["result"][v] == sa.SAP_CUTOFF) == bool(orr["cutoff"])
+            if not orr["cutoff"]:
+                assert same_bounds(out["lb"][v], ol) and same_bounds(out["ub"][v], ou)
+                assert out["rounds"][v] == orr["rounds"]
+        # a smaller batch on the same (larger) bounds object
+        out2 = bnd.propagate_batch(inst.lb, inst.ub, ptr[:4], cols, vals, vals)
+        assert np.array_equal(out2["lb"], out["lb"][:3]) and np.array_equal(out2["ub"], out["ub"][:3])
+        bnd.close()
     mat.close()
 
 
