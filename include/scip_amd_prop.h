@@ -62,6 +62,9 @@ typedef struct sap_params
 /* the input bound vector(s) are already a propagation fixpoint except for the columns an
  * overlay changes: only rows containing such columns are swept in the first round */
 #define SAP_FLAG_BASE_IS_FIXPOINT 0x1
+/* run the round loop from the host with one synchronisation per round instead of launching
+ * hipGraph-captured groups of rounds (same results; for debugging and comparison) */
+#define SAP_FLAG_HOSTLOOP 0x2
 
 /* per-call statistics */
 typedef struct sap_stats

@@ -32,6 +32,7 @@ SAP_CUTOFF = 7
 SAP_REDUCEDDOM = 10
 
 SAP_FLAG_BASE_IS_FIXPOINT = 0x1
+SAP_FLAG_HOSTLOOP = 0x2
 
 SAP_MODE_AUTO = 0
 SAP_MODE_DENSE = 1

@@ -49,7 +49,27 @@ def build(verbose: bool = True) -> str:
     os.replace(lib + ".tmp", lib)
     if verbose:
         print(f"built {lib}")
+    build_plugin_testhost(verbose)
     return lib
+
+
+def build_plugin_testhost(verbose: bool = True) -> str:
+    """The SCIP propagator plugin (plugin/prop_gpulinear.c) linked against the test double of
+    the SCIP API it calls (plugin/testhost), giving plugin/libgpulinear_testhost.so for tests.
+    In a SCIP build the plugin source is compiled against SCIP's own headers instead."""
+    plug = os.path.join(HERE, "plugin")
+    out = os.path.join(plug, "libgpulinear_testhost.so")
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", "-Wno-unused-parameter",
+           f"-I{os.path.join(plug, 'testhost')}", f"-I{plug}", f"-I{os.path.join(ROOT, 'include')}",
+           os.path.join(plug, "prop_gpulinear.c"), os.path.join(plug, "testhost", "testhost.c"),
+           f"-L{HERE}", "-lscipamd", "-Wl,-rpath,$ORIGIN/..", "-lm", "-o", out + ".tmp"]
+    msg = _run(cmd)
+    if verbose and msg:
+        print(msg)
+    os.replace(out + ".tmp", out)
+    if verbose:
+        print(f"built {out}")
+    return out
 
 
 if __name__ == "__main__":

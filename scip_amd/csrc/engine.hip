@@ -68,6 +68,12 @@ struct Status
    unsigned long long slot_nnz[SAP_STAT_SLOTS];
    unsigned long long slot_rows[SAP_STAT_SLOTS];
    unsigned long long slot_chg[SAP_STAT_SLOTS];
+   // sweep launch duration from the constant-rate wall clock: earliest workgroup start and
+   // latest workgroup end per slot, folded by the next round's control kernel
+   unsigned long long tslot0[SAP_STAT_SLOTS];
+   unsigned long long tslot1[SAP_STAT_SLOTS];
+   unsigned long long sweep_ticks;
+   unsigned long long sweep_count;
 };
 
 struct Mat
@@ -1403,6 +1409,7 @@ __global__ __launch_bounds__(SAP_BLOCK, 4) void k_sweep(Mat M, Vec V, Params P, 
    Status* st = V.st;
    if( st->done )
       return;
+   const unsigned long long tstart = (unsigned long long)wall_clock64();
    const int blk = (int)blockIdx.x;
    int kind = 0;
    while( kind < SAP_NKINDS - 1 && blk >= L.blockstart[kind + 1] )
@@ -1489,6 +1496,12 @@ __global__ __launch_bounds__(SAP_BLOCK, 4) void k_sweep(Mat M, Vec V, Params P, 
    if( Q.count > 0 )
       drainQueue(M, V, P, Q, Q.count, I.round);
    blockStats(nnzacc, rowsacc, st->slot_nnz, st->slot_rows);
+   if( threadIdx.x == 0 )
+   {
+      const int slot = (int)(blockIdx.x & (SAP_STAT_SLOTS - 1));
+      atomicMin(&st->tslot0[slot], tstart);
+      atomicMax(&st->tslot1[slot], (unsigned long long)wall_clock64());
+   }
 }
 
 // the column step for one (column, vector) whose candidate slots may hold values;
@@ -1639,9 +1652,34 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_columns(Mat M, Vec V, Params P)
 // ------------------------------------------------------------------------------------------
 // round control and set-up kernels (kernels, not memsets, so a round can be graph-captured)
 // ------------------------------------------------------------------------------------------
-__global__ void k_round_begin(Vec V, int32_t maxrounds)
+// one workgroup of SAP_STAT_SLOTS threads: folds the previous sweep's timestamps, then
+// (thread 0) advances the round or declares the call done
+__global__ __launch_bounds__(SAP_STAT_SLOTS) void k_round_begin(Vec V, int32_t maxrounds)
 {
    Status* st = V.st;
+   __shared__ unsigned long long tmin[SAP_STAT_SLOTS], tmax[SAP_STAT_SLOTS];
+   const int t = (int)threadIdx.x;
+   tmin[t] = st->tslot0[t];
+   tmax[t] = st->tslot1[t];
+   st->tslot0[t] = ~0ULL;
+   st->tslot1[t] = 0;
+   __syncthreads();
+   for( int w = SAP_STAT_SLOTS / 2; w > 0; w >>= 1 )
+   {
+      if( t < w )
+      {
+         tmin[t] = tmin[t] < tmin[t + w] ? tmin[t] : tmin[t + w];
+         tmax[t] = tmax[t] > tmax[t + w] ? tmax[t] : tmax[t + w];
+      }
+      __syncthreads();
+   }
+   if( t != 0 )
+      return;
+   if( tmin[0] != ~0ULL && tmax[0] >= tmin[0] )
+   {
+      st->sweep_ticks += tmax[0] - tmin[0];
+      st->sweep_count += 1;
+   }
    if( st->done )
       return;
    const int32_t r = st->round + 1;
@@ -1704,6 +1742,13 @@ __global__ void k_call_reset(Vec V, int64_t nrows, int32_t nvec, int32_t firstal
       V.st->slot_nnz[gi] = 0;
       V.st->slot_rows[gi] = 0;
       V.st->slot_chg[gi] = 0;
+      V.st->tslot0[gi] = ~0ULL;
+      V.st->tslot1[gi] = 0;
+   }
+   if( gi == 0 )
+   {
+      V.st->sweep_ticks = 0;
+      V.st->sweep_count = 0;
    }
 }
 
@@ -1832,6 +1877,7 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_row_activities(Mat M, Params P, c
 struct sap_context
 {
    int32_t device;
+   int32_t wallclock_khz;   // rate of wall_clock64() on the device
    hipStream_t stream;
    hipEvent_t ev[4];
    Status* hstatus;   // pinned
@@ -1862,6 +1908,28 @@ struct sap_matrix
    sap_bounds* single;   // cached one-vector state for sap_propagate
 };
 
+// rounds captured into one hipGraph: a short graph for the first launch of a call (many
+// propagations end after one or two rounds) and a long one for the rest
+#define SAP_GRAPH_SHORT 2
+#define SAP_GRAPH_LONG 8
+
+struct RoundGraph
+{
+   hipGraphExec_t exec;
+   int32_t rounds;
+   int32_t valid;
+   hipEvent_t ev[2 * SAP_GRAPH_LONG];   // around each captured sweep launch
+};
+
+struct GraphKey
+{
+   Params P;
+   int32_t nvec;
+   int32_t single;
+   int32_t dense;
+   int32_t pad;
+};
+
 struct sap_bounds
 {
    sap_matrix* mat;
@@ -1871,6 +1939,9 @@ struct sap_bounds
    int64_t device_bytes;
    double* stage;   // device staging for host lb/ub (2*ncols)
    int32_t roundbase;
+   int32_t graphbroken;   // capture failed once: use the host-driven loop
+   RoundGraph graph[2];
+   GraphKey key;
 };
 
 static int hipToRc(hipError_t e)
@@ -2032,6 +2103,9 @@ int sap_context_create(int32_t device, sap_context** out)
    if( c == NULL )
       return SAP_NOMEMORY;
    c->device = device;
+   if( hipDeviceGetAttribute(&c->wallclock_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess
+      || c->wallclock_khz <= 0 )
+      c->wallclock_khz = 100000;   // 100 MHz reference clock
    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
    for( int i = 0; i < 4; ++i )
       HIPCHK(hipEventCreate(&c->ev[i]));
@@ -2339,6 +2413,15 @@ static void boundsRelease(sap_bounds* b)
    (void)hipFree(V.touched);
    (void)hipFree(b->dstatus);
    (void)hipFree(b->stage);
+   for( int g = 0; g < 2; ++g )
+   {
+      if( b->graph[g].valid )
+         (void)hipGraphExecDestroy(b->graph[g].exec);
+      b->graph[g].valid = 0;
+      for( int e = 0; e < 2 * SAP_GRAPH_LONG; ++e )
+         if( b->graph[g].ev[e] != NULL )
+            (void)hipEventDestroy(b->graph[g].ev[e]);
+   }
 }
 
 // dense marks are scanned every round: worthwhile while the scan (4 B per row and per column
@@ -2497,7 +2580,7 @@ static Layout makeLayout(const sap_bounds* b)
    // items per workgroup and block caps per kind (kind 0 tiles, kind 1+b bin b)
    static const int kPerBlock[SAP_NKINDS] = { SAP_BLOCK / SAP_WAVE, SAP_BLOCK / 4, SAP_BLOCK / 4, SAP_BLOCK / 8,
       SAP_BLOCK / 16, SAP_BLOCK / 32, SAP_BLOCK / 64, SAP_BLOCK / SAP_WAVE, 1 };
-   static const int kCap[SAP_NKINDS] = { 4096, 4096, 4096, 4096, 4096, 4096, 4096, 4096, 2048 };
+   static const int kCap[SAP_NKINDS] = { 4096, 2048, 2048, 2048, 2048, 2048, 2048, 2048, 1024 };
    int acc = 0;
    for( int q = 0; q < SAP_NKINDS; ++q )
    {
@@ -2518,16 +2601,61 @@ static Layout makeLayout(const sap_bounds* b)
    return L;
 }
 
-// run rounds until the device reports done; host polls the status every `poll` rounds
-static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats)
+static void foldStats(sap_context* c)
+{
+   // fold the per-workgroup statistic slots into the host copy of the status
+   Status* h = c->hstatus;
+   unsigned long long a = 0, r = 0, g = 0;
+   for( int q = 0; q < SAP_STAT_SLOTS; ++q )
+   {
+      a += h->slot_nnz[q];
+      r += h->slot_rows[q];
+      g += h->slot_chg[q];
+   }
+   h->nnz_swept = a;
+   h->rows_swept = r;
+   h->nchg = g;
+}
+
+static void fillStats(sap_context* c, double tot, double sweepms, int64_t launches, sap_stats* stats)
+{
+   if( stats == NULL )
+      return;
+   const Status* h = c->hstatus;
+   stats->rounds = h->rlocal;
+   stats->nchgbds = (int64_t)h->nchg;
+   stats->rows_swept = (int64_t)h->rows_swept;
+   stats->nnz_swept = (int64_t)h->nnz_swept;
+   stats->device_ms = tot;
+   stats->sweep_ms = sweepms;
+   stats->sweep_launches = launches;
+}
+
+// enqueue one round: control, sweep (between two events when given), column step
+static void enqueueRound(sap_bounds* b, const Mat& M, const Params& P, const Layout& L, hipEvent_t e0, hipEvent_t e1)
+{
+   sap_matrix* m = b->mat;
+   hipStream_t s = m->ctx->stream;
+   const int nsweep = L.blockstart[SAP_NKINDS];
+   const unsigned colgrid = gridFor((size_t)b->V.nvec * (size_t)m->ncols, 4096);
+   k_round_begin<<<1, SAP_STAT_SLOTS, 0, s>>>(b->V, P.maxrounds);
+   if( e0 != NULL )
+      (void)hipEventRecord(e0, s);
+   if( nsweep > 0 )
+      k_sweep<<<nsweep, SAP_BLOCK, 0, s>>>(M, b->V, P, L);
+   if( e1 != NULL )
+      (void)hipEventRecord(e1, s);
+   k_columns<<<colgrid, SAP_BLOCK, 0, s>>>(M, b->V, P);
+}
+
+// host-driven loop: one synchronisation per round
+static int runRoundsHost(sap_bounds* b, const Params& P, sap_stats* stats)
 {
    sap_matrix* m = b->mat;
    sap_context* c = m->ctx;
    hipStream_t s = c->stream;
    const Mat M = toMat(m);
    const Layout L = makeLayout(b);
-   const int nsweep = L.blockstart[SAP_NKINDS];
-   const unsigned colgrid = gridFor((size_t)b->nvec * (size_t)m->ncols, 4096);
    double sweepms = 0.0;
    int64_t launches = 0;
    HIPCHK(hipEventRecord(c->ev[0], s));
@@ -2540,12 +2668,7 @@ static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats)
          fprintf(stderr, "[scip_amd] round cap %d exceeded\n", SAP_ROUND_CAP);
          return SAP_ERROR;
       }
-      k_round_begin<<<1, 1, 0, s>>>(b->V, P.maxrounds);
-      HIPCHK(hipEventRecord(c->ev[2], s));
-      if( nsweep > 0 )
-         k_sweep<<<nsweep, SAP_BLOCK, 0, s>>>(M, b->V, P, L);
-      HIPCHK(hipEventRecord(c->ev[3], s));
-      k_columns<<<colgrid, SAP_BLOCK, 0, s>>>(M, b->V, P);
+      enqueueRound(b, M, P, L, c->ev[2], c->ev[3]);
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(c->hstatus, b->dstatus, sizeof(Status), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
@@ -2560,31 +2683,105 @@ static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats)
    HIPCHK(hipEventSynchronize(c->ev[1]));
    float tot = 0.0f;
    HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[1]));
+   foldStats(c);
+   fillStats(c, tot, sweepms, launches, stats);
+   return SAP_OKAY;
+}
+
+// capture `rounds` rounds into g; on failure the stream is left out of capture mode
+static bool captureGraph(sap_bounds* b, const Params& P, int rounds, RoundGraph* g)
+{
+   sap_matrix* m = b->mat;
+   hipStream_t s = m->ctx->stream;
+   const Mat M = toMat(m);
+   const Layout L = makeLayout(b);
+   if( hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess )
+      return false;
+   for( int r = 0; r < rounds; ++r )
+      enqueueRound(b, M, P, L, NULL, NULL);
+   hipGraph_t graph = NULL;
+   const hipError_t e1 = hipStreamEndCapture(s, &graph);
+   if( e1 != hipSuccess || graph == NULL || hipGetLastError() != hipSuccess )
    {
-      // fold the per-workgroup statistic slots into the host copy of the status
-      Status* h = c->hstatus;
-      unsigned long long a = 0, r = 0, g = 0;
-      for( int q = 0; q < SAP_STAT_SLOTS; ++q )
+      if( graph != NULL )
+         (void)hipGraphDestroy(graph);
+      return false;
+   }
+   const hipError_t e2 = hipGraphInstantiate(&g->exec, graph, NULL, NULL, 0);
+   (void)hipGraphDestroy(graph);
+   if( e2 != hipSuccess )
+      return false;
+   g->rounds = rounds;
+   g->valid = 1;
+   return true;
+}
+
+static bool ensureGraphs(sap_bounds* b, const Params& P)
+{
+   GraphKey k;
+   memset(&k, 0, sizeof(k));
+   k.P = P;
+   k.P.flags = 0;   // flags steer the host side only
+   k.nvec = b->V.nvec;
+   k.single = b->V.single;
+   k.dense = b->V.dense;
+   if( b->graph[0].valid && b->graph[1].valid && memcmp(&k, &b->key, sizeof(k)) == 0 )
+      return true;
+   for( int g = 0; g < 2; ++g )
+      if( b->graph[g].valid )
       {
-         a += h->slot_nnz[q];
-         r += h->slot_rows[q];
-         g += h->slot_chg[q];
+         (void)hipGraphExecDestroy(b->graph[g].exec);
+         b->graph[g].valid = 0;
       }
-      h->nnz_swept = a;
-      h->rows_swept = r;
-      h->nchg = g;
-   }
-   if( stats )
+   if( !captureGraph(b, P, SAP_GRAPH_SHORT, &b->graph[0]) || !captureGraph(b, P, SAP_GRAPH_LONG, &b->graph[1]) )
    {
-      const Status* h = c->hstatus;
-      stats->rounds = h->rlocal;
-      stats->nchgbds = (int64_t)h->nchg;
-      stats->rows_swept = (int64_t)h->rows_swept;
-      stats->nnz_swept = (int64_t)h->nnz_swept;
-      stats->device_ms = tot;
-      stats->sweep_ms = sweepms;
-      stats->sweep_launches = launches;
+      (void)hipGetLastError();
+      b->graphbroken = 1;
+      fprintf(stderr, "[scip_amd] round graph capture unavailable; using the host-driven loop\n");
+      return false;
    }
+   b->key = k;
+   return true;
+}
+
+// rounds until the device reports done: graph launches of several rounds each, one
+// synchronisation per launch (per round with SAP_FLAG_HOSTLOOP)
+static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats)
+{
+   if( (P.flags & SAP_FLAG_HOSTLOOP) || b->graphbroken || !ensureGraphs(b, P) )
+      return runRoundsHost(b, P, stats);
+   sap_context* c = b->mat->ctx;
+   hipStream_t s = c->stream;
+   double sweepms = 0.0;
+   int64_t launches = 0;
+   int32_t prev = 0;
+   HIPCHK(hipEventRecord(c->ev[0], s));
+   for( int it = 0;; ++it )
+   {
+      if( it > SAP_ROUND_CAP / SAP_GRAPH_SHORT )
+      {
+         fprintf(stderr, "[scip_amd] round cap %d exceeded\n", SAP_ROUND_CAP);
+         return SAP_ERROR;
+      }
+      RoundGraph* g = &b->graph[it == 0 ? 0 : 1];
+      HIPCHK(hipGraphLaunch(g->exec, s));
+      HIPCHK(hipMemcpyAsync(c->hstatus, b->dstatus, sizeof(Status), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      prev = c->hstatus->rlocal;
+      if( c->hstatus->done )
+         break;
+   }
+   // sweep launch durations from the in-kernel wall clock (events recorded inside a graph do
+   // not time on this runtime)
+   sweepms = (double)c->hstatus->sweep_ticks / (double)c->wallclock_khz;
+   launches = (int64_t)c->hstatus->sweep_count;
+   (void)prev;
+   HIPCHK(hipEventRecord(c->ev[1], s));
+   HIPCHK(hipEventSynchronize(c->ev[1]));
+   float tot = 0.0f;
+   HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[1]));
+   foldStats(c);
+   fillStats(c, tot, sweepms, launches, stats);
    return SAP_OKAY;
 }
 

@@ -1,0 +1,497 @@
+/* prop_gpulinear.c -- SCIP propagator: linear-constraint domain propagation on an AMD MI355X.
+ *
+ * The propagator collects the rows of all linear constraints (SCIPgetVarsLinear /
+ * SCIPgetValsLinear / SCIPgetLhsLinear / SCIPgetRhsLinear, cons_linear.c:18326-18436) into a
+ * CSR matrix over the active problem variables (SCIPvarGetProbindex), keeps it resident on
+ * the device, and in every PROPEXEC call propagates the node's local bounds to the device
+ * fixpoint with sap_propagate.  The device applies the acceptance rules of
+ * SCIPinferVarLbCons / SCIPinferVarUbCons (scip_var.c:6965-7174), so every returned bound is
+ * handed to SCIPinferVarLbProp / SCIPinferVarUbProp (scip_var.c:7589/7705) unchanged.
+ *
+ * Parameters mirror SCIP's (numerics/..., constraints/linear/maxeasyactivitydelta); the
+ * propagator's own:
+ *   propagating/gpulinear/device     HIP device ordinal
+ *   propagating/gpulinear/maxrounds  device rounds per call (-1: to fixpoint)
+ *
+ * Error convention: every HIP or engine failure becomes a SCIP_RETCODE (the engine's return
+ * codes have SCIP_RETCODE values); nothing aborts.
+ */
+#include <limits.h>
+#include <string.h>
+
+#include "scip/scip.h"
+#include "scip/cons_linear.h"
+#include "scip_amd_prop.h"
+#include "prop_gpulinear.h"
+
+#define PROP_NAME              "gpulinear"
+#define PROP_DESC              "linear constraint propagation on an AMD MI355X (scip_amd engine)"
+#define PROP_PRIORITY          -1000          /**< after the constraint handlers */
+#define PROP_FREQ                  1
+#define PROP_DELAY             FALSE
+#define PROP_TIMING            SCIP_PROPTIMING_BEFORELP
+#define PROP_PRESOL_PRIORITY   -1000
+#define PROP_PRESOL_MAXROUNDS     -1
+#define PROP_PRESOLTIMING      SCIP_PRESOLTIMING_MEDIUM
+
+#define DEFAULT_DEVICE             0
+#define DEFAULT_MAXROUNDS         -1
+
+/** propagator data */
+struct SCIP_PropData
+{
+   sap_context*          ctx;                /**< engine context (device + stream) */
+   sap_matrix*           mat;                /**< device-resident linear rows, or NULL */
+   SCIP_VAR**            vars;               /**< column -> variable (active problem variables) */
+   int                   nvars;              /**< number of columns */
+   int                   nrows;              /**< number of rows on the device */
+   int                   device;             /**< parameter: HIP device */
+   int                   maxrounds;          /**< parameter: device rounds per call */
+   SCIP_Longint          ncalls;             /**< PROPEXEC calls that ran on the device */
+   SCIP_Longint          ncutoffs;           /**< cutoffs found */
+   SCIP_Longint          nchgbds;            /**< bound changes handed to SCIP */
+};
+
+/*
+ * local methods
+ */
+
+/** maps an engine return code (SCIP_RETCODE values) to a SCIP_RETCODE */
+static
+SCIP_RETCODE engineCall(
+   int                   rc,                 /**< engine return code */
+   const char*           what                /**< engine function */
+   )
+{
+   if( rc != SAP_OKAY )
+   {
+      SCIPerrorMessage("%s failed: %s (%d)\n", what, sap_error_string(rc), rc);
+      return (SCIP_RETCODE)rc;
+   }
+   return SCIP_OKAY;
+}
+
+/** engine parameters from SCIP's numerics */
+static
+SCIP_RETCODE getEngineParams(
+   SCIP*                 scip,               /**< SCIP data structure */
+   SCIP_PROPDATA*        propdata,           /**< propagator data */
+   sap_params*           params              /**< parameters to fill */
+   )
+{
+   SCIP_Real val;
+
+   sap_params_default(params);
+   params->infinity = SCIPinfinity(scip);
+   params->epsilon = SCIPepsilon(scip);
+   params->sumepsilon = SCIPsumepsilon(scip);
+   params->feastol = SCIPfeastol(scip);
+   params->hugeval = SCIPgetHugeValue(scip);
+   SCIP_CALL( SCIPgetRealParam(scip, "numerics/boundstreps", &val) );
+   params->boundstreps = val;
+   SCIP_CALL( SCIPgetRealParam(scip, "numerics/recompfac", &val) );
+   params->recompfac = val;
+   SCIP_CALL( SCIPgetRealParam(scip, "constraints/linear/maxeasyactivitydelta", &val) );
+   params->maxeasyactivitydelta = val;
+   params->maxrounds = propdata->maxrounds;
+   return SCIP_OKAY;
+}
+
+/** releases the device matrix and the column map */
+static
+SCIP_RETCODE freeMatrix(
+   SCIP*                 scip,               /**< SCIP data structure */
+   SCIP_PROPDATA*        propdata            /**< propagator data */
+   )
+{
+   if( propdata->mat != NULL )
+   {
+      SCIP_CALL( engineCall(sap_matrix_free(propdata->mat), "sap_matrix_free") );
+      propdata->mat = NULL;
+   }
+   if( propdata->vars != NULL )
+   {
+      SCIPfreeBlockMemoryArray(scip, &propdata->vars, propdata->nvars);
+      propdata->vars = NULL;
+   }
+   propdata->nvars = 0;
+   propdata->nrows = 0;
+   return SCIP_OKAY;
+}
+
+/** builds the CSR matrix of all linear constraints over the active variables and uploads it */
+static
+SCIP_RETCODE buildMatrix(
+   SCIP*                 scip,               /**< SCIP data structure */
+   SCIP_PROPDATA*        propdata            /**< propagator data */
+   )
+{
+   SCIP_CONSHDLR* conshdlr;
+   SCIP_CONS** conss;
+   SCIP_VAR** probvars;
+   SCIP_VAR** rowvars;
+   SCIP_Real* rowvals;
+   int64_t* rowptr;
+   int32_t* colidx;
+   double* vals;
+   double* lhs;
+   double* rhs;
+   uint8_t* integral;
+   double* glb;
+   double* gub;
+   int nconss;
+   int nvars;
+   int nnzcap;
+   int nnz;
+   int nrows;
+   int c;
+   int j;
+
+   SCIP_CALL( freeMatrix(scip, propdata) );
+
+   conshdlr = SCIPfindConshdlr(scip, "linear");
+   if( conshdlr == NULL )
+      return SCIP_OKAY;
+   nconss = SCIPconshdlrGetNConss(conshdlr);
+   conss = SCIPconshdlrGetConss(conshdlr);
+   nvars = SCIPgetNVars(scip);
+   probvars = SCIPgetVars(scip);
+   if( nconss == 0 || nvars == 0 )
+      return SCIP_OKAY;
+
+   nnzcap = 0;
+   for( c = 0; c < nconss; ++c )
+      nnzcap += SCIPgetNVarsLinear(scip, conss[c]);
+
+   SCIP_CALL( SCIPallocBlockMemoryArray(scip, &propdata->vars, nvars) );
+   propdata->nvars = nvars;
+   for( j = 0; j < nvars; ++j )
+      propdata->vars[j] = probvars[j];
+
+   SCIP_CALL( SCIPallocBufferArray(scip, &rowptr, nconss + 1) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &colidx, nnzcap + 1) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &vals, nnzcap + 1) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &lhs, nconss) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &rhs, nconss) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &integral, nvars) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &glb, nvars) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &gub, nvars) );
+
+   nnz = 0;
+   nrows = 0;
+   rowptr[0] = 0;
+   for( c = 0; c < nconss; ++c )
+   {
+      SCIP_Real constant = 0.0;
+      int nrowvars;
+      int varssize;
+      int requiredsize;
+      int k;
+      SCIP_Bool ok = TRUE;
+
+      if( !SCIPconsIsActive(conss[c]) )
+         continue;
+      nrowvars = SCIPgetNVarsLinear(scip, conss[c]);
+      varssize = nrowvars + 1;
+      /* active representation of the row (aggregated / fixed variables resolved) */
+      SCIP_CALL( SCIPallocBufferArray(scip, &rowvars, varssize) );
+      SCIP_CALL( SCIPallocBufferArray(scip, &rowvals, varssize) );
+      for( k = 0; k < nrowvars; ++k )
+      {
+         rowvars[k] = SCIPgetVarsLinear(scip, conss[c])[k];
+         rowvals[k] = SCIPgetValsLinear(scip, conss[c])[k];
+      }
+      SCIP_CALL( SCIPgetProbvarLinearSum(scip, rowvars, rowvals, &nrowvars, varssize, &constant, &requiredsize) );
+      if( requiredsize > varssize )
+         ok = FALSE;   /* multi-aggregations expand the row; leave it to cons_linear */
+      for( k = 0; ok && k < nrowvars; ++k )
+         if( SCIPvarGetProbindex(rowvars[k]) < 0 || rowvals[k] == 0.0 )
+            ok = FALSE;
+      if( ok && nnz + nrowvars <= nnzcap )
+      {
+         SCIP_Real l = SCIPgetLhsLinear(scip, conss[c]);
+         SCIP_Real r = SCIPgetRhsLinear(scip, conss[c]);
+         for( k = 0; k < nrowvars; ++k )
+         {
+            colidx[nnz] = (int32_t)SCIPvarGetProbindex(rowvars[k]);
+            vals[nnz] = rowvals[k];
+            ++nnz;
+         }
+         lhs[nrows] = SCIPisInfinity(scip, -l) ? l : l - constant;
+         rhs[nrows] = SCIPisInfinity(scip, r) ? r : r - constant;
+         ++nrows;
+         rowptr[nrows] = nnz;
+      }
+      SCIPfreeBufferArray(scip, &rowvals);
+      SCIPfreeBufferArray(scip, &rowvars);
+   }
+
+   for( j = 0; j < nvars; ++j )
+   {
+      integral[j] = SCIPvarIsIntegral(propdata->vars[j]) ? 1 : 0;
+      glb[j] = SCIPvarGetLbGlobal(propdata->vars[j]);
+      gub[j] = SCIPvarGetUbGlobal(propdata->vars[j]);
+   }
+
+   if( nrows > 0 )
+   {
+      SCIP_CALL( engineCall(sap_matrix_create(propdata->ctx, nrows, nvars, rowptr, colidx, vals, lhs, rhs, integral,
+            glb, gub, &propdata->mat), "sap_matrix_create") );
+   }
+   propdata->nrows = nrows;
+
+   SCIPfreeBufferArray(scip, &gub);
+   SCIPfreeBufferArray(scip, &glb);
+   SCIPfreeBufferArray(scip, &integral);
+   SCIPfreeBufferArray(scip, &rhs);
+   SCIPfreeBufferArray(scip, &lhs);
+   SCIPfreeBufferArray(scip, &vals);
+   SCIPfreeBufferArray(scip, &colidx);
+   SCIPfreeBufferArray(scip, &rowptr);
+
+   return SCIP_OKAY;
+}
+
+/** propagates the given (local or global) bounds on the device; returns the new bounds */
+static
+SCIP_RETCODE propagateOnDevice(
+   SCIP*                 scip,               /**< SCIP data structure */
+   SCIP_PROPDATA*        propdata,           /**< propagator data */
+   SCIP_Bool             local,              /**< local (node) bounds, else global bounds */
+   double*               lb,                 /**< array of nvars: new lower bounds */
+   double*               ub,                 /**< array of nvars: new upper bounds */
+   SCIP_Bool*            cutoff              /**< pointer to store whether infeasibility was detected */
+   )
+{
+   sap_params params;
+   sap_stats stats;
+   int j;
+
+   for( j = 0; j < propdata->nvars; ++j )
+   {
+      SCIP_VAR* var = propdata->vars[j];
+      lb[j] = local ? SCIPvarGetLbLocal(var) : SCIPvarGetLbGlobal(var);
+      ub[j] = local ? SCIPvarGetUbLocal(var) : SCIPvarGetUbGlobal(var);
+   }
+   SCIP_CALL( getEngineParams(scip, propdata, &params) );
+   memset(&stats, 0, sizeof(stats));
+   SCIP_CALL( engineCall(sap_propagate(propdata->mat, &params, lb, ub, &stats), "sap_propagate") );
+   *cutoff = (stats.result == SAP_CUTOFF);
+   ++propdata->ncalls;
+   if( *cutoff )
+      ++propdata->ncutoffs;
+   return SCIP_OKAY;
+}
+
+/*
+ * callback methods of propagator
+ */
+
+/** destructor of propagator to free user data (called when SCIP is exiting) */
+static
+SCIP_DECL_PROPFREE(propFreeGpulinear)
+{  /*lint --e{715}*/
+   SCIP_PROPDATA* propdata = SCIPpropGetData(prop);
+
+   SCIP_CALL( freeMatrix(scip, propdata) );
+   if( propdata->ctx != NULL )
+   {
+      SCIP_CALL( engineCall(sap_context_free(propdata->ctx), "sap_context_free") );
+      propdata->ctx = NULL;
+   }
+   SCIPfreeBlockMemory(scip, &propdata);
+   SCIPpropSetData(prop, NULL);
+   return SCIP_OKAY;
+}
+
+/** creates the device context on first use */
+static
+SCIP_RETCODE ensureContext(
+   SCIP_PROPDATA*        propdata            /**< propagator data */
+   )
+{
+   if( propdata->ctx == NULL )
+   {
+      SCIP_CALL( engineCall(sap_context_create(propdata->device, &propdata->ctx), "sap_context_create") );
+   }
+   return SCIP_OKAY;
+}
+
+/** presolving initialization: matrix over the current (presolving) problem */
+static
+SCIP_DECL_PROPINITPRE(propInitpreGpulinear)
+{  /*lint --e{715}*/
+   SCIP_PROPDATA* propdata = SCIPpropGetData(prop);
+
+   SCIP_CALL( ensureContext(propdata) );
+   SCIP_CALL( buildMatrix(scip, propdata) );
+   return SCIP_OKAY;
+}
+
+/** presolving deinitialization: the presolved problem gets a new matrix in INITSOL */
+static
+SCIP_DECL_PROPEXITPRE(propExitpreGpulinear)
+{  /*lint --e{715}*/
+   SCIP_CALL( freeMatrix(scip, SCIPpropGetData(prop)) );
+   return SCIP_OKAY;
+}
+
+/** solving process initialization: matrix over the presolved problem */
+static
+SCIP_DECL_PROPINITSOL(propInitsolGpulinear)
+{  /*lint --e{715}*/
+   SCIP_PROPDATA* propdata = SCIPpropGetData(prop);
+
+   SCIP_CALL( ensureContext(propdata) );
+   SCIP_CALL( buildMatrix(scip, propdata) );
+   return SCIP_OKAY;
+}
+
+/** solving process deinitialization */
+static
+SCIP_DECL_PROPEXITSOL(propExitsolGpulinear)
+{  /*lint --e{715}*/
+   SCIP_CALL( freeMatrix(scip, SCIPpropGetData(prop)) );
+   return SCIP_OKAY;
+}
+
+/** presolving: global bounds to the device fixpoint (the bound tightening of consPresolLinear) */
+static
+SCIP_DECL_PROPPRESOL(propPresolGpulinear)
+{  /*lint --e{715}*/
+   SCIP_PROPDATA* propdata = SCIPpropGetData(prop);
+   double* lb;
+   double* ub;
+   SCIP_Bool cutoff;
+   int nchg = 0;
+   int j;
+
+   *result = SCIP_DIDNOTRUN;
+   if( propdata->mat == NULL || propdata->nvars != SCIPgetNVars(scip) )
+   {
+      /* presolving changed the variable set: rebuild over the current problem */
+      SCIP_CALL( ensureContext(propdata) );
+      SCIP_CALL( buildMatrix(scip, propdata) );
+      if( propdata->mat == NULL )
+         return SCIP_OKAY;
+   }
+
+   SCIP_CALL( SCIPallocBufferArray(scip, &lb, propdata->nvars) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &ub, propdata->nvars) );
+   SCIP_CALL( propagateOnDevice(scip, propdata, FALSE, lb, ub, &cutoff) );
+   if( cutoff )
+      *result = SCIP_CUTOFF;
+   else
+   {
+      for( j = 0; j < propdata->nvars && !cutoff; ++j )
+      {
+         SCIP_VAR* var = propdata->vars[j];
+         SCIP_Bool infeasible = FALSE;
+         SCIP_Bool tightened = FALSE;
+
+         if( lb[j] > SCIPvarGetLbGlobal(var) )
+         {
+            SCIP_CALL( SCIPtightenVarLb(scip, var, lb[j], TRUE, &infeasible, &tightened) );
+            nchg += tightened ? 1 : 0;
+            cutoff = cutoff || infeasible;
+         }
+         if( !cutoff && ub[j] < SCIPvarGetUbGlobal(var) )
+         {
+            SCIP_CALL( SCIPtightenVarUb(scip, var, ub[j], TRUE, &infeasible, &tightened) );
+            nchg += tightened ? 1 : 0;
+            cutoff = cutoff || infeasible;
+         }
+      }
+      *nchgbds += nchg;
+      propdata->nchgbds += nchg;
+      *result = cutoff ? SCIP_CUTOFF : (nchg > 0 ? SCIP_SUCCESS : SCIP_DIDNOTFIND);
+   }
+   SCIPfreeBufferArray(scip, &ub);
+   SCIPfreeBufferArray(scip, &lb);
+   return SCIP_OKAY;
+}
+
+/** execution method: local bounds of the node to the device fixpoint */
+static
+SCIP_DECL_PROPEXEC(propExecGpulinear)
+{  /*lint --e{715}*/
+   SCIP_PROPDATA* propdata = SCIPpropGetData(prop);
+   double* lb;
+   double* ub;
+   SCIP_Bool cutoff;
+   int nchg = 0;
+   int j;
+
+   *result = SCIP_DIDNOTRUN;
+   if( propdata->mat == NULL )
+      return SCIP_OKAY;
+
+   SCIP_CALL( SCIPallocBufferArray(scip, &lb, propdata->nvars) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &ub, propdata->nvars) );
+   SCIP_CALL( propagateOnDevice(scip, propdata, TRUE, lb, ub, &cutoff) );
+   if( cutoff )
+      *result = SCIP_CUTOFF;
+   else
+   {
+      for( j = 0; j < propdata->nvars && !cutoff; ++j )
+      {
+         SCIP_VAR* var = propdata->vars[j];
+         SCIP_Bool infeasible = FALSE;
+         SCIP_Bool tightened = FALSE;
+
+         /* the device accepted these changes under SCIP's rules (including forced changes of
+          * single-variable rows), so they are applied as forced */
+         if( lb[j] > SCIPvarGetLbLocal(var) )
+         {
+            SCIP_CALL( SCIPinferVarLbProp(scip, var, lb[j], prop, 0, TRUE, &infeasible, &tightened) );
+            nchg += tightened ? 1 : 0;
+            cutoff = cutoff || infeasible;
+         }
+         if( !cutoff && ub[j] < SCIPvarGetUbLocal(var) )
+         {
+            SCIP_CALL( SCIPinferVarUbProp(scip, var, ub[j], prop, 0, TRUE, &infeasible, &tightened) );
+            nchg += tightened ? 1 : 0;
+            cutoff = cutoff || infeasible;
+         }
+      }
+      propdata->nchgbds += nchg;
+      *result = cutoff ? SCIP_CUTOFF : (nchg > 0 ? SCIP_REDUCEDDOM : SCIP_DIDNOTFIND);
+   }
+   SCIPfreeBufferArray(scip, &ub);
+   SCIPfreeBufferArray(scip, &lb);
+   return SCIP_OKAY;
+}
+
+/*
+ * propagator specific interface methods
+ */
+
+/** creates the gpulinear propagator and includes it in SCIP */
+SCIP_RETCODE SCIPincludePropGpulinear(
+   SCIP*                 scip                /**< SCIP data structure */
+   )
+{
+   SCIP_PROPDATA* propdata;
+   SCIP_PROP* prop;
+
+   SCIP_CALL( SCIPallocBlockMemory(scip, &propdata) );
+   memset(propdata, 0, sizeof(*propdata));
+
+   SCIP_CALL( SCIPincludePropBasic(scip, &prop, PROP_NAME, PROP_DESC, PROP_PRIORITY, PROP_FREQ, PROP_DELAY,
+         PROP_TIMING, propExecGpulinear, propdata) );
+   SCIP_CALL( SCIPsetPropFree(scip, prop, propFreeGpulinear) );
+   SCIP_CALL( SCIPsetPropInitpre(scip, prop, propInitpreGpulinear) );
+   SCIP_CALL( SCIPsetPropExitpre(scip, prop, propExitpreGpulinear) );
+   SCIP_CALL( SCIPsetPropInitsol(scip, prop, propInitsolGpulinear) );
+   SCIP_CALL( SCIPsetPropExitsol(scip, prop, propExitsolGpulinear) );
+   SCIP_CALL( SCIPsetPropPresol(scip, prop, propPresolGpulinear, PROP_PRESOL_PRIORITY, PROP_PRESOL_MAXROUNDS,
+         PROP_PRESOLTIMING) );
+
+   SCIP_CALL( SCIPaddIntParam(scip, "propagating/" PROP_NAME "/device", "HIP device ordinal of the engine",
+         &propdata->device, TRUE, DEFAULT_DEVICE, 0, 1024, NULL, NULL) );
+   SCIP_CALL( SCIPaddIntParam(scip, "propagating/" PROP_NAME "/maxrounds",
+         "maximal number of device propagation rounds per call (-1: until fixpoint)",
+         &propdata->maxrounds, FALSE, DEFAULT_MAXROUNDS, -1, INT_MAX, NULL, NULL) );
+
+   return SCIP_OKAY;
+}

@@ -1,0 +1,438 @@
+/* testhost.c -- test double of the SCIP API subset used by prop_gpulinear.c.
+ *
+ * A problem is a set of variables with global and local bounds and a set of linear rows;
+ * the propagator registered through SCIPincludePropBasic is driven by the th_* entry points
+ * (the order of SCIP's solving stages: include, INITPRE/PRESOL/EXITPRE, INITSOL, PROPEXEC,
+ * EXITSOL, FREE).  Bound changes go through the acceptance logic of SCIPinferVarLbCons /
+ * SCIPinferVarUbCons (src/scip/scip_var.c:6965-7174, var.c:1957-2021, set.c:7711-7753), so a
+ * change the plugin hands over is accepted or rejected as SCIP would.
+ *
+ * Test infrastructure: built into scip_amd/plugin/libgpulinear_testhost.so for tests/ only.
+ */
+#include <math.h>
+#include <string.h>
+
+#include "scip/scip.h"
+#include "scip/cons_linear.h"
+#include "prop_gpulinear.h"
+
+struct SCIP_Var
+{
+   int probindex;
+   int integral;
+   double glb, gub, llb, lub;
+};
+
+struct SCIP_Cons
+{
+   int nvars;
+   SCIP_VAR** vars;
+   double* vals;
+   double lhs, rhs;
+};
+
+struct SCIP_Conshdlr
+{
+   int nconss;
+   int consssize;
+   SCIP_CONS** conss;
+};
+
+struct SCIP_Prop
+{
+   SCIP_PROPDATA* data;
+   SCIP_DECL_PROPEXEC((*exec));
+   SCIP_DECL_PROPFREE((*pfree));
+   SCIP_DECL_PROPINITPRE((*initpre));
+   SCIP_DECL_PROPEXITPRE((*exitpre));
+   SCIP_DECL_PROPINITSOL((*initsol));
+   SCIP_DECL_PROPEXITSOL((*exitsol));
+   SCIP_DECL_PROPPRESOL((*presol));
+   char name[64];
+};
+
+#define TH_MAXPARAMS 16
+
+struct Scip
+{
+   int nvars;
+   int varssize;
+   SCIP_VAR** vars;
+   SCIP_CONSHDLR linear;
+   SCIP_PROP* prop;
+   double infinity, epsilon, sumepsilon, feastol, hugeval, boundstreps, recompfac, maxeasy;
+   int nintparams;
+   char intname[TH_MAXPARAMS][96];
+   int* intptr[TH_MAXPARAMS];
+};
+
+/* ---- numerics (def.h, set.c) ---- */
+SCIP_Real SCIPinfinity(SCIP* scip) { return scip->infinity; }
+SCIP_Real SCIPepsilon(SCIP* scip) { return scip->epsilon; }
+SCIP_Real SCIPsumepsilon(SCIP* scip) { return scip->sumepsilon; }
+SCIP_Real SCIPfeastol(SCIP* scip) { return scip->feastol; }
+SCIP_Real SCIPgetHugeValue(SCIP* scip) { return scip->hugeval; }
+SCIP_Bool SCIPisInfinity(SCIP* scip, SCIP_Real val) { return val >= scip->infinity; }
+
+static double relDiff(double a, double b)
+{
+   double q = fabs(a) > fabs(b) ? fabs(a) : fabs(b);
+   q = q > 1.0 ? q : 1.0;
+   return (a - b) / q;
+}
+
+SCIP_RETCODE SCIPgetRealParam(SCIP* scip, const char* name, SCIP_Real* value)
+{
+   if( strcmp(name, "numerics/boundstreps") == 0 )
+      *value = scip->boundstreps;
+   else if( strcmp(name, "numerics/recompfac") == 0 )
+      *value = scip->recompfac;
+   else if( strcmp(name, "constraints/linear/maxeasyactivitydelta") == 0 )
+      *value = scip->maxeasy;
+   else
+      return SCIP_PARAMETERUNKNOWN;
+   return SCIP_OKAY;
+}
+
+SCIP_RETCODE SCIPaddIntParam(SCIP* scip, const char* name, const char* desc, int* valueptr, SCIP_Bool isadvanced,
+   int defaultvalue, int minvalue, int maxvalue, SCIP_DECL_PARAMCHGD((*paramchgd)), SCIP_PARAMDATA* paramdata)
+{
+   (void)desc; (void)isadvanced; (void)minvalue; (void)maxvalue; (void)paramchgd; (void)paramdata;
+   if( scip->nintparams >= TH_MAXPARAMS )
+      return SCIP_NOMEMORY;
+   strncpy(scip->intname[scip->nintparams], name, 95);
+   scip->intptr[scip->nintparams] = valueptr;
+   scip->nintparams++;
+   if( valueptr != NULL )
+      *valueptr = defaultvalue;
+   return SCIP_OKAY;
+}
+
+/* ---- propagator registration ---- */
+SCIP_RETCODE SCIPincludePropBasic(SCIP* scip, SCIP_PROP** propptr, const char* name, const char* desc, int priority,
+   int freq, SCIP_Bool delay, SCIP_PROPTIMING timingmask, SCIP_DECL_PROPEXEC((*propexec)), SCIP_PROPDATA* propdata)
+{
+   (void)desc; (void)priority; (void)freq; (void)delay; (void)timingmask;
+   if( scip->prop != NULL )
+      return SCIP_INVALIDDATA;   /* one propagator per test host */
+   scip->prop = (SCIP_PROP*)calloc(1, sizeof(SCIP_PROP));
+   if( scip->prop == NULL )
+      return SCIP_NOMEMORY;
+   strncpy(scip->prop->name, name, 63);
+   scip->prop->exec = propexec;
+   scip->prop->data = propdata;
+   if( propptr != NULL )
+      *propptr = scip->prop;
+   return SCIP_OKAY;
+}
+SCIP_RETCODE SCIPsetPropFree(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPFREE((*f))) { (void)scip; prop->pfree = f; return SCIP_OKAY; }
+SCIP_RETCODE SCIPsetPropInitpre(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPINITPRE((*f))) { (void)scip; prop->initpre = f; return SCIP_OKAY; }
+SCIP_RETCODE SCIPsetPropExitpre(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPEXITPRE((*f))) { (void)scip; prop->exitpre = f; return SCIP_OKAY; }
+SCIP_RETCODE SCIPsetPropInitsol(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPINITSOL((*f))) { (void)scip; prop->initsol = f; return SCIP_OKAY; }
+SCIP_RETCODE SCIPsetPropExitsol(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPEXITSOL((*f))) { (void)scip; prop->exitsol = f; return SCIP_OKAY; }
+SCIP_RETCODE SCIPsetPropPresol(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPPRESOL((*f)), int presolpriority,
+   int presolmaxrounds, SCIP_PRESOLTIMING presoltiming)
+{
+   (void)scip; (void)presolpriority; (void)presolmaxrounds; (void)presoltiming;
+   prop->presol = f;
+   return SCIP_OKAY;
+}
+SCIP_PROPDATA* SCIPpropGetData(SCIP_PROP* prop) { return prop->data; }
+void SCIPpropSetData(SCIP_PROP* prop, SCIP_PROPDATA* propdata) { prop->data = propdata; }
+
+/* ---- problem access ---- */
+SCIP_CONSHDLR* SCIPfindConshdlr(SCIP* scip, const char* name) { return strcmp(name, "linear") == 0 ? &scip->linear : NULL; }
+int SCIPconshdlrGetNConss(SCIP_CONSHDLR* h) { return h->nconss; }
+SCIP_CONS** SCIPconshdlrGetConss(SCIP_CONSHDLR* h) { return h->conss; }
+SCIP_Bool SCIPconsIsActive(SCIP_CONS* cons) { (void)cons; return TRUE; }
+int SCIPgetNVars(SCIP* scip) { return scip->nvars; }
+SCIP_VAR** SCIPgetVars(SCIP* scip) { return scip->vars; }
+int SCIPvarGetProbindex(SCIP_VAR* var) { return var->probindex; }
+SCIP_Bool SCIPvarIsIntegral(SCIP_VAR* var) { return var->integral != 0; }
+SCIP_Real SCIPvarGetLbGlobal(SCIP_VAR* var) { return var->glb; }
+SCIP_Real SCIPvarGetUbGlobal(SCIP_VAR* var) { return var->gub; }
+SCIP_Real SCIPvarGetLbLocal(SCIP_VAR* var) { return var->llb; }
+SCIP_Real SCIPvarGetUbLocal(SCIP_VAR* var) { return var->lub; }
+SCIP_Real SCIPgetLhsLinear(SCIP* scip, SCIP_CONS* cons) { (void)scip; return cons->lhs; }
+SCIP_Real SCIPgetRhsLinear(SCIP* scip, SCIP_CONS* cons) { (void)scip; return cons->rhs; }
+int SCIPgetNVarsLinear(SCIP* scip, SCIP_CONS* cons) { (void)scip; return cons->nvars; }
+SCIP_VAR** SCIPgetVarsLinear(SCIP* scip, SCIP_CONS* cons) { (void)scip; return cons->vars; }
+SCIP_Real* SCIPgetValsLinear(SCIP* scip, SCIP_CONS* cons) { (void)scip; return cons->vals; }
+
+/* all variables of the test host are active */
+SCIP_RETCODE SCIPgetProbvarLinearSum(SCIP* scip, SCIP_VAR** vars, SCIP_Real* scalars, int* nvars, int varssize,
+   SCIP_Real* constant, int* requiredsize)
+{
+   (void)scip; (void)vars; (void)scalars; (void)varssize; (void)constant;
+   *requiredsize = *nvars;
+   return SCIP_OKAY;
+}
+
+/* ---- bound changes: SCIPinferVarLbCons / SCIPinferVarUbCons semantics ---- */
+static double adjLb(SCIP* scip, int integral, double lb)
+{
+   if( lb < 0.0 && -lb >= scip->infinity )
+      return -scip->infinity;
+   if( lb > 0.0 && lb >= scip->infinity )
+      return scip->infinity;
+   if( integral )
+      return ceil(lb - scip->feastol);
+   if( lb > 0.0 && lb < scip->epsilon )
+      return 0.0;
+   return lb;
+}
+
+static double adjUb(SCIP* scip, int integral, double ub)
+{
+   if( ub > 0.0 && ub >= scip->infinity )
+      return scip->infinity;
+   if( ub < 0.0 && -ub >= scip->infinity )
+      return -scip->infinity;
+   if( integral )
+      return floor(ub + scip->feastol);
+   if( ub < 0.0 && ub > -scip->epsilon )
+      return 0.0;
+   return ub;
+}
+
+static int lbBetter(SCIP* scip, double newlb, double oldlb, double oldub)
+{
+   double t;
+   if( oldlb < 0.0 && newlb >= 0.0 )
+      return 1;
+   t = (oldub - oldlb) <= fabs(oldlb) ? (oldub - oldlb) : fabs(oldlb);
+   t = t >= 1e-3 ? t : 1e-3;
+   return newlb - oldlb > scip->boundstreps * t;
+}
+
+static int ubBetter(SCIP* scip, double newub, double oldlb, double oldub)
+{
+   double t;
+   if( oldub > 0.0 && newub <= 0.0 )
+      return 1;
+   t = (oldub - oldlb) <= fabs(oldub) ? (oldub - oldlb) : fabs(oldub);
+   t = t >= 1e-3 ? t : 1e-3;
+   return newub - oldub < -(scip->boundstreps * t);
+}
+
+static SCIP_RETCODE chgLb(SCIP* scip, SCIP_VAR* var, double nb, SCIP_Bool force, SCIP_Bool global,
+   SCIP_Bool* infeasible, SCIP_Bool* tightened)
+{
+   double lb = global ? var->glb : var->llb, ub = global ? var->gub : var->lub;
+   *infeasible = FALSE;
+   if( tightened )
+      *tightened = FALSE;
+   nb = adjLb(scip, var->integral, nb);
+   if( nb >= scip->infinity || relDiff(nb, ub) > scip->feastol )
+   {
+      *infeasible = TRUE;
+      return SCIP_OKAY;
+   }
+   nb = nb <= ub ? nb : ub;
+   if( (force && nb - lb <= scip->epsilon) || (!force && !lbBetter(scip, nb, lb, ub)) )
+      return SCIP_OKAY;
+   if( nb > lb )
+   {
+      if( global )
+         var->glb = nb;
+      var->llb = nb > var->llb ? nb : var->llb;
+      if( tightened )
+         *tightened = TRUE;
+   }
+   return SCIP_OKAY;
+}
+
+static SCIP_RETCODE chgUb(SCIP* scip, SCIP_VAR* var, double nb, SCIP_Bool force, SCIP_Bool global,
+   SCIP_Bool* infeasible, SCIP_Bool* tightened)
+{
+   double lb = global ? var->glb : var->llb, ub = global ? var->gub : var->lub;
+   *infeasible = FALSE;
+   if( tightened )
+      *tightened = FALSE;
+   nb = adjUb(scip, var->integral, nb);
+   if( -nb >= scip->infinity || relDiff(nb, lb) < -scip->feastol )
+   {
+      *infeasible = TRUE;
+      return SCIP_OKAY;
+   }
+   nb = nb >= lb ? nb : lb;
+   if( (force && nb - ub >= -scip->epsilon) || (!force && !ubBetter(scip, nb, lb, ub)) )
+      return SCIP_OKAY;
+   if( nb < ub )
+   {
+      if( global )
+         var->gub = nb;
+      var->lub = nb < var->lub ? nb : var->lub;
+      if( tightened )
+         *tightened = TRUE;
+   }
+   return SCIP_OKAY;
+}
+
+SCIP_RETCODE SCIPinferVarLbProp(SCIP* scip, SCIP_VAR* var, SCIP_Real newbound, SCIP_PROP* inferprop, int inferinfo,
+   SCIP_Bool force, SCIP_Bool* infeasible, SCIP_Bool* tightened)
+{
+   (void)inferprop; (void)inferinfo;
+   return chgLb(scip, var, newbound, force, FALSE, infeasible, tightened);
+}
+SCIP_RETCODE SCIPinferVarUbProp(SCIP* scip, SCIP_VAR* var, SCIP_Real newbound, SCIP_PROP* inferprop, int inferinfo,
+   SCIP_Bool force, SCIP_Bool* infeasible, SCIP_Bool* tightened)
+{
+   (void)inferprop; (void)inferinfo;
+   return chgUb(scip, var, newbound, force, FALSE, infeasible, tightened);
+}
+SCIP_RETCODE SCIPtightenVarLb(SCIP* scip, SCIP_VAR* var, SCIP_Real newbound, SCIP_Bool force, SCIP_Bool* infeasible,
+   SCIP_Bool* tightened)
+{
+   return chgLb(scip, var, newbound, force, TRUE, infeasible, tightened);
+}
+SCIP_RETCODE SCIPtightenVarUb(SCIP* scip, SCIP_VAR* var, SCIP_Real newbound, SCIP_Bool force, SCIP_Bool* infeasible,
+   SCIP_Bool* tightened)
+{
+   return chgUb(scip, var, newbound, force, TRUE, infeasible, tightened);
+}
+
+/* ---- driver (ctypes entry points for tests) ---- */
+SCIP_EXPORT SCIP* th_create(void)
+{
+   SCIP* s = (SCIP*)calloc(1, sizeof(SCIP));
+   if( s == NULL )
+      return NULL;
+   s->infinity = 1e20;
+   s->epsilon = 1e-9;
+   s->sumepsilon = 1e-6;
+   s->feastol = 1e-6;
+   s->hugeval = 1e15;
+   s->boundstreps = 0.05;
+   s->recompfac = 1e6;
+   s->maxeasy = 1e6;
+   return s;
+}
+
+SCIP_EXPORT int th_add_vars(SCIP* s, int n, const double* lb, const double* ub, const unsigned char* integral)
+{
+   int i;
+   s->vars = (SCIP_VAR**)malloc(sizeof(SCIP_VAR*) * (size_t)(n > 0 ? n : 1));
+   if( s->vars == NULL )
+      return SCIP_NOMEMORY;
+   for( i = 0; i < n; ++i )
+   {
+      SCIP_VAR* v = (SCIP_VAR*)calloc(1, sizeof(SCIP_VAR));
+      if( v == NULL )
+         return SCIP_NOMEMORY;
+      v->probindex = i;
+      v->integral = integral[i];
+      v->glb = v->llb = lb[i];
+      v->gub = v->lub = ub[i];
+      s->vars[i] = v;
+   }
+   s->nvars = n;
+   return SCIP_OKAY;
+}
+
+/* rows in CSR form */
+SCIP_EXPORT int th_add_rows(SCIP* s, int nrows, const long long* rowptr, const int* colidx, const double* vals,
+   const double* lhs, const double* rhs)
+{
+   int i;
+   s->linear.conss = (SCIP_CONS**)malloc(sizeof(SCIP_CONS*) * (size_t)(nrows > 0 ? nrows : 1));
+   if( s->linear.conss == NULL )
+      return SCIP_NOMEMORY;
+   for( i = 0; i < nrows; ++i )
+   {
+      long long k, b = rowptr[i], e = rowptr[i + 1];
+      SCIP_CONS* c = (SCIP_CONS*)calloc(1, sizeof(SCIP_CONS));
+      if( c == NULL )
+         return SCIP_NOMEMORY;
+      c->nvars = (int)(e - b);
+      c->vars = (SCIP_VAR**)malloc(sizeof(SCIP_VAR*) * (size_t)(c->nvars > 0 ? c->nvars : 1));
+      c->vals = (double*)malloc(sizeof(double) * (size_t)(c->nvars > 0 ? c->nvars : 1));
+      for( k = b; k < e; ++k )
+      {
+         c->vars[k - b] = s->vars[colidx[k]];
+         c->vals[k - b] = vals[k];
+      }
+      c->lhs = lhs[i];
+      c->rhs = rhs[i];
+      s->linear.conss[i] = c;
+   }
+   s->linear.nconss = nrows;
+   return SCIP_OKAY;
+}
+
+SCIP_EXPORT int th_include_gpulinear(SCIP* s) { return SCIPincludePropGpulinear(s); }
+
+SCIP_EXPORT int th_set_int_param(SCIP* s, const char* name, int value)
+{
+   int i;
+   for( i = 0; i < s->nintparams; ++i )
+      if( strcmp(s->intname[i], name) == 0 && s->intptr[i] != NULL )
+      {
+         *s->intptr[i] = value;
+         return SCIP_OKAY;
+      }
+   return SCIP_PARAMETERUNKNOWN;
+}
+
+SCIP_EXPORT int th_initpre(SCIP* s) { return s->prop->initpre ? s->prop->initpre(s, s->prop) : SCIP_OKAY; }
+SCIP_EXPORT int th_exitpre(SCIP* s) { return s->prop->exitpre ? s->prop->exitpre(s, s->prop) : SCIP_OKAY; }
+SCIP_EXPORT int th_initsol(SCIP* s) { return s->prop->initsol ? s->prop->initsol(s, s->prop) : SCIP_OKAY; }
+SCIP_EXPORT int th_exitsol(SCIP* s) { return s->prop->exitsol ? s->prop->exitsol(s, s->prop, FALSE) : SCIP_OKAY; }
+
+SCIP_EXPORT int th_propexec(SCIP* s, int* result)
+{
+   SCIP_RESULT r = SCIP_DIDNOTRUN;
+   SCIP_RETCODE rc = s->prop->exec(s, s->prop, SCIP_PROPTIMING_BEFORELP, &r);
+   *result = (int)r;
+   return rc;
+}
+
+SCIP_EXPORT int th_presol(SCIP* s, int* result, int* nchgbds)
+{
+   int z[10] = { 0 };
+   SCIP_RESULT r = SCIP_DIDNOTRUN;
+   SCIP_RETCODE rc = s->prop->presol(s, s->prop, 0, SCIP_PRESOLTIMING_MEDIUM, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
+      &z[0], &z[1], &z[2], nchgbds, &z[4], &z[5], &z[6], &z[7], &z[8], &z[9], &r);
+   *result = (int)r;
+   return rc;
+}
+
+SCIP_EXPORT void th_set_local_bounds(SCIP* s, const double* lb, const double* ub)
+{
+   int i;
+   for( i = 0; i < s->nvars; ++i )
+   {
+      s->vars[i]->llb = lb[i];
+      s->vars[i]->lub = ub[i];
+   }
+}
+
+SCIP_EXPORT void th_get_bounds(SCIP* s, int global, double* lb, double* ub)
+{
+   int i;
+   for( i = 0; i < s->nvars; ++i )
+   {
+      lb[i] = global ? s->vars[i]->glb : s->vars[i]->llb;
+      ub[i] = global ? s->vars[i]->gub : s->vars[i]->lub;
+   }
+}
+
+SCIP_EXPORT int th_free(SCIP* s)
+{
+   int i, rc = SCIP_OKAY;
+   if( s->prop != NULL && s->prop->pfree != NULL )
+      rc = s->prop->pfree(s, s->prop);
+   free(s->prop);
+   for( i = 0; i < s->linear.nconss; ++i )
+   {
+      free(s->linear.conss[i]->vars);
+      free(s->linear.conss[i]->vals);
+      free(s->linear.conss[i]);
+   }
+   free(s->linear.conss);
+   for( i = 0; i < s->nvars; ++i )
+      free(s->vars[i]);
+   free(s->vars);
+   free(s);
+   return rc;
+}

@@ -1,0 +1,94 @@
+"""ctypes driver for plugin/libgpulinear_testhost.so: the SCIP propagator plugin
+(scip_amd/plugin/prop_gpulinear.c) running inside the test double of the SCIP API subset
+it calls (scip_amd/plugin/testhost/testhost.c)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+import scip_amd as sa
+
+LIBPATH = os.path.join(os.path.dirname(sa.__file__), "plugin", "libgpulinear_testhost.so")
+
+# SCIP_RETCODE / SCIP_RESULT values (src/scip/type_retcode.h, type_result.h)
+SCIP_OKAY = 1
+SCIP_DIDNOTRUN = 1
+SCIP_DIDNOTFIND = 3
+SCIP_CUTOFF = 7
+SCIP_REDUCEDDOM = 10
+SCIP_SUCCESS = 17
+
+_lib = None
+_P = np.ctypeslib.ndpointer
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        sa.lib()  # the engine library first, so the plugin's dependency resolves
+        L = C.CDLL(LIBPATH)
+        L.th_create.restype = C.c_void_p
+        L.th_create.argtypes = []
+        L.th_add_vars.argtypes = [C.c_void_p, C.c_int, _P(np.float64), _P(np.float64), _P(np.uint8)]
+        L.th_add_rows.argtypes = [C.c_void_p, C.c_int, _P(np.int64), _P(np.int32), _P(np.float64),
+                                  _P(np.float64), _P(np.float64)]
+        for f in ("th_include_gpulinear", "th_initpre", "th_exitpre", "th_initsol", "th_exitsol", "th_free"):
+            getattr(L, f).argtypes = [C.c_void_p]
+        L.th_set_int_param.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        L.th_propexec.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.th_presol.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.th_set_local_bounds.argtypes = [C.c_void_p, _P(np.float64), _P(np.float64)]
+        L.th_set_local_bounds.restype = None
+        L.th_get_bounds.argtypes = [C.c_void_p, C.c_int, _P(np.float64), _P(np.float64)]
+        L.th_get_bounds.restype = None
+        _lib = L
+    return _lib
+
+
+class Host:
+    """One SCIP problem (variables + linear rows) with the gpulinear propagator included."""
+
+    def __init__(self, inst):
+        L = lib()
+        self.n = int(len(inst.lb))
+        self.h = L.th_create()
+        assert self.h
+        assert L.th_add_vars(self.h, self.n, np.ascontiguousarray(inst.lb, np.float64),
+                             np.ascontiguousarray(inst.ub, np.float64),
+                             np.ascontiguousarray(inst.integral, np.uint8)) == SCIP_OKAY
+        nrows = len(inst.rowptr) - 1
+        assert L.th_add_rows(self.h, nrows, np.ascontiguousarray(inst.rowptr, np.int64),
+                             np.ascontiguousarray(inst.colidx, np.int32), np.ascontiguousarray(inst.vals, np.float64),
+                             np.ascontiguousarray(inst.lhs, np.float64),
+                             np.ascontiguousarray(inst.rhs, np.float64)) == SCIP_OKAY
+        assert L.th_include_gpulinear(self.h) == SCIP_OKAY
+
+    def call(self, name, *args):
+        return getattr(lib(), name)(self.h, *args)
+
+    def propexec(self):
+        r = C.c_int(0)
+        rc = lib().th_propexec(self.h, C.byref(r))
+        return rc, r.value
+
+    def presol(self):
+        r = C.c_int(0)
+        n = C.c_int(0)
+        rc = lib().th_presol(self.h, C.byref(r), C.byref(n))
+        return rc, r.value, n.value
+
+    def set_local(self, lb, ub):
+        lib().th_set_local_bounds(self.h, np.ascontiguousarray(lb, np.float64), np.ascontiguousarray(ub, np.float64))
+
+    def bounds(self, glob=False):
+        lb = np.empty(self.n)
+        ub = np.empty(self.n)
+        lib().th_get_bounds(self.h, 1 if glob else 0, lb, ub)
+        return lb, ub
+
+    def close(self):
+        if self.h:
+            rc = lib().th_free(self.h)
+            self.h = None
+            return rc
+        return SCIP_OKAY

@@ -253,3 +253,21 @@ def test_invalid_inputs_rejected(gpu_ctx):
     with pytest.raises(sa.SapError):
         mat.propagate(params=sa.default_params(maxrounds=-5))
     mat.close()
+
+
+@pytest.mark.parametrize("planted", [False, True])
+def test_graph_rounds_equal_host_loop(gpu_ctx, planted):
+    inst = sa.gen_mixed(40000, 40000, seed=31, planted=planted)
+    say(f"[case] graph vs host loop planted={planted}")
+    mat = sa.Matrix(gpu_ctx, inst)
+    a = mat.propagate(params=sa.default_params())
+    b = mat.propagate(params=sa.default_params(flags=sa.SAP_FLAG_HOSTLOOP))
+    assert a[0].tobytes() == b[0].tobytes() and a[1].tobytes() == b[1].tobytes()
+    for k in ("result", "rounds", "nchgbds", "rows_swept", "nnz_swept"):
+        assert a[2][k] == b[2][k], k
+    assert a[2]["sweep_launches"] == b[2]["sweep_launches"] == a[2]["rounds"]
+    # different parameters on the same matrix re-capture the round graphs
+    c = mat.propagate(params=sa.default_params(maxrounds=2))
+    d = mat.propagate(params=sa.default_params(maxrounds=2, flags=sa.SAP_FLAG_HOSTLOOP))
+    assert c[0].tobytes() == d[0].tobytes() and c[2]["rounds"] == d[2]["rounds"] <= 2
+    mat.close()

@@ -1,0 +1,124 @@
+"""The SCIP propagator plugin (scip_amd/plugin/prop_gpulinear.c) driven through the test
+double of the SCIP API: registration and error handling on the CPU, and on the GPU the
+bounds it hands to SCIP (SCIPinferVarLbProp/UbProp in PROPEXEC, SCIPtightenVarLb/Ub in
+PRESOL) equal the engine's and the oracle's fixpoint."""
+import os
+
+import numpy as np
+import pytest
+
+import scip_amd as sa
+from oracle import oracle as orc
+from tests import plugin_host as ph
+from tests.util import random_binary_instance, random_fixings, apply_fixings
+
+
+def test_testhost_builds_and_links_the_plugin():
+    assert os.path.exists(ph.LIBPATH)
+    L = ph.lib()
+    for sym in ("th_create", "th_include_gpulinear", "th_propexec", "th_presol", "SCIPincludePropGpulinear"):
+        assert hasattr(L, sym)
+
+
+def test_plugin_registers_parameters():
+    inst = sa.gen_covering_packing(50, 40, 4, seed=3)
+    h = ph.Host(inst)
+    assert h.call("th_set_int_param", b"propagating/gpulinear/maxrounds", 5) == ph.SCIP_OKAY
+    assert h.call("th_set_int_param", b"propagating/gpulinear/device", 0) == ph.SCIP_OKAY
+    assert h.call("th_set_int_param", b"propagating/gpulinear/nosuch", 0) != ph.SCIP_OKAY
+    assert h.close() == ph.SCIP_OKAY
+
+
+@pytest.mark.skipif(sa.device_count() > 0, reason="checks the no-device error path")
+def test_plugin_without_device_returns_a_scip_error():
+    inst = sa.gen_covering_packing(50, 40, 4, seed=3)
+    h = ph.Host(inst)
+    assert h.call("th_initsol") == sa.SAP_INVALIDCALL  # SCIP_INVALIDCALL, no abort
+    rc, res = h.propexec()
+    assert rc == ph.SCIP_OKAY and res == ph.SCIP_DIDNOTRUN  # no matrix: the propagator does not run
+    assert h.close() == ph.SCIP_OKAY
+
+
+def _oracle(inst, lb, ub):
+    return orc.propagate_jacobi(orc.OMatrix.from_instance(inst), lb, ub)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_propexec_root_matches_engine_and_oracle(gpu_ctx, seed):
+    inst = sa.gen_mixed(3000, 3000, seed=seed, planted=True)
+    ol, ou, orr = _oracle(inst, inst.lb, inst.ub)
+    h = ph.Host(inst)
+    assert h.call("th_initsol") == ph.SCIP_OKAY
+    rc, res = h.propexec()
+    assert rc == ph.SCIP_OKAY
+    assert not orr["cutoff"]
+    changed = (ol != inst.lb).any() or (ou != inst.ub).any()
+    assert res == (ph.SCIP_REDUCEDDOM if changed else ph.SCIP_DIDNOTFIND)
+    lb, ub = h.bounds()
+    assert np.array_equal(lb, ol) and np.array_equal(ub, ou)
+    # global bounds are untouched by PROPEXEC
+    glb, gub = h.bounds(glob=True)
+    assert np.array_equal(glb, inst.lb) and np.array_equal(gub, inst.ub)
+    # a second call at the fixpoint finds nothing
+    rc, res = h.propexec()
+    assert rc == ph.SCIP_OKAY and res == ph.SCIP_DIDNOTFIND
+    assert h.call("th_exitsol") == ph.SCIP_OKAY
+    assert h.close() == ph.SCIP_OKAY
+
+
+@pytest.mark.gpu
+def test_propexec_nodes_match_oracle(gpu_ctx):
+    rng = np.random.default_rng(11)
+    inst = random_binary_instance(rng, 400, 300, 12)
+    rl, ru, _ = _oracle(inst, inst.lb, inst.ub)
+    h = ph.Host(inst)
+    assert h.call("th_initsol") == ph.SCIP_OKAY
+    free = np.nonzero(rl < ru)[0]
+    nvec = 12
+    ptr, cols, vals = random_fixings(rng, len(free), nvec, 20)
+    cols = free[cols].astype(np.int32)
+    ncut = 0
+    for k in range(nvec):
+        nl, nu = apply_fixings(rl, ru, ptr, cols, vals, k)
+        h.set_local(nl, nu)
+        rc, res = h.propexec()
+        assert rc == ph.SCIP_OKAY
+        ol, ou, orr = _oracle(inst, nl, nu)
+        if orr["cutoff"]:
+            assert res == ph.SCIP_CUTOFF
+            ncut += 1
+            continue
+        lb, ub = h.bounds()
+        assert np.array_equal(lb, ol) and np.array_equal(ub, ou), f"node {k}"
+    assert ncut < nvec
+    assert h.close() == ph.SCIP_OKAY
+
+
+@pytest.mark.gpu
+def test_presol_tightens_global_bounds(gpu_ctx):
+    inst = sa.gen_mixed(2000, 2000, seed=4, planted=True)
+    ol, ou, _ = _oracle(inst, inst.lb, inst.ub)
+    h = ph.Host(inst)
+    assert h.call("th_initpre") == ph.SCIP_OKAY
+    rc, res, nchg = h.presol()
+    assert rc == ph.SCIP_OKAY
+    expect = int((ol != inst.lb).sum() + (ou != inst.ub).sum())
+    assert nchg == expect
+    assert res == (ph.SCIP_SUCCESS if expect else ph.SCIP_DIDNOTFIND)
+    glb, gub = h.bounds(glob=True)
+    assert np.array_equal(glb, ol) and np.array_equal(gub, ou)
+    assert h.call("th_exitpre") == ph.SCIP_OKAY
+    assert h.close() == ph.SCIP_OKAY
+
+
+@pytest.mark.gpu
+def test_cutoff_reported(gpu_ctx):
+    inst = sa.gen_mixed(30000, 30000, seed=2, planted=False)  # literal sides: infeasible at this size
+    _, _, orr = _oracle(inst, inst.lb, inst.ub)
+    h = ph.Host(inst)
+    assert h.call("th_initsol") == ph.SCIP_OKAY
+    rc, res = h.propexec()
+    assert rc == ph.SCIP_OKAY
+    assert orr["cutoff"] and res == ph.SCIP_CUTOFF
+    assert h.close() == ph.SCIP_OKAY
