@@ -67,7 +67,7 @@ def cpu_baseline(budget_s: float = 12.0, planted: bool = False):
 
 def traffic_from_profiles():
     """HBM bytes per sweep launch from the committed rocprofv3 PMC summary, if present."""
-    path = os.path.join(ROOT, "profiles", "sweep_traffic_cfg2.json")
+    path = os.path.join(ROOT, "profiles", "k_sweep_mixed1M_traffic.json")
     if not os.path.exists(path):
         return None
     try:
@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--cols", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", action="store_true", help="only run the copy kernel used to scale PMC bytes")
+    ap.add_argument("--variant", choices=("both", "literal", "planted"), default="both",
+                    help="literal: only the headline instance (for profiling); planted: only the feasible variant")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -111,6 +113,11 @@ def main():
     import scip_amd as sa
 
     ctx = sa.Context(0)
+    if args.variant != "both":
+        out = run_workload(sa, ctx, args, planted=(args.variant == "planted"))
+        print(json.dumps(out), flush=True)
+        ctx.close()
+        return
     main_out = run_workload(sa, ctx, args, planted=False)
     feas = run_workload(sa, ctx, args, planted=True)
     main_out["config"]["feasible_variant"] = {

@@ -462,6 +462,17 @@ SCIP_DECL_PROPEXEC(propExecGpulinear)
    return SCIP_OKAY;
 }
 
+/** conflict resolution: the device keeps no per-change reason, so a bound change of this
+ *  propagator is reported as unexplained; conflict analysis then keeps the change itself in
+ *  the conflict (valid, possibly weaker conflicts) instead of failing with
+ *  SCIP_PLUGINNOTFOUND for a propagator without this callback (prop.c:780-783) */
+static
+SCIP_DECL_PROPRESPROP(propRespropGpulinear)
+{  /*lint --e{715}*/
+   *result = SCIP_DIDNOTFIND;
+   return SCIP_OKAY;
+}
+
 /*
  * propagator specific interface methods
  */
@@ -486,6 +497,7 @@ SCIP_RETCODE SCIPincludePropGpulinear(
    SCIP_CALL( SCIPsetPropExitsol(scip, prop, propExitsolGpulinear) );
    SCIP_CALL( SCIPsetPropPresol(scip, prop, propPresolGpulinear, PROP_PRESOL_PRIORITY, PROP_PRESOL_MAXROUNDS,
          PROP_PRESOLTIMING) );
+   SCIP_CALL( SCIPsetPropResprop(scip, prop, propRespropGpulinear) );
 
    SCIP_CALL( SCIPaddIntParam(scip, "propagating/" PROP_NAME "/device", "HIP device ordinal of the engine",
          &propdata->device, TRUE, DEFAULT_DEVICE, 0, 1024, NULL, NULL) );

@@ -58,6 +58,8 @@ typedef struct SCIP_Prop SCIP_PROP;
 typedef struct SCIP_PropData SCIP_PROPDATA;
 typedef struct SCIP_ParamData SCIP_PARAMDATA;
 typedef struct SCIP_Param SCIP_PARAM;
+typedef struct SCIP_BdChgIdx SCIP_BDCHGIDX;
+typedef enum SCIP_BoundType { SCIP_BOUNDTYPE_LOWER = 0, SCIP_BOUNDTYPE_UPPER = 1 } SCIP_BOUNDTYPE;
 
 #define SCIP_DECL_PARAMCHGD(x) SCIP_RETCODE x (SCIP* scip, SCIP_PARAM* param)
 #define SCIP_DECL_PROPFREE(x) SCIP_RETCODE x (SCIP* scip, SCIP_PROP* prop)
@@ -66,6 +68,8 @@ typedef struct SCIP_Param SCIP_PARAM;
 #define SCIP_DECL_PROPINITSOL(x) SCIP_RETCODE x (SCIP* scip, SCIP_PROP* prop)
 #define SCIP_DECL_PROPEXITSOL(x) SCIP_RETCODE x (SCIP* scip, SCIP_PROP* prop, SCIP_Bool restart)
 #define SCIP_DECL_PROPEXEC(x) SCIP_RETCODE x (SCIP* scip, SCIP_PROP* prop, SCIP_PROPTIMING proptiming, SCIP_RESULT* result)
+#define SCIP_DECL_PROPRESPROP(x) SCIP_RETCODE x (SCIP* scip, SCIP_PROP* prop, SCIP_VAR* infervar, int inferinfo, \
+      SCIP_BOUNDTYPE boundtype, SCIP_BDCHGIDX* bdchgidx, SCIP_Real relaxedbd, SCIP_RESULT* result)
 #define SCIP_DECL_PROPPRESOL(x) SCIP_RETCODE x (SCIP* scip, SCIP_PROP* prop, int nrounds, SCIP_PRESOLTIMING presoltiming, \
       int nnewfixedvars, int nnewaggrvars, int nnewchgvartypes, int nnewchgbds, int nnewholes, \
       int nnewdelconss, int nnewaddconss, int nnewupgdconss, int nnewchgcoefs, int nnewchgsides, \
@@ -94,6 +98,7 @@ SCIP_RETCODE SCIPsetPropInitsol(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPINITS
 SCIP_RETCODE SCIPsetPropExitsol(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPEXITSOL((*propexitsol)));
 SCIP_RETCODE SCIPsetPropPresol(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPPRESOL((*proppresol)), int presolpriority,
    int presolmaxrounds, SCIP_PRESOLTIMING presoltiming);
+SCIP_RETCODE SCIPsetPropResprop(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPRESPROP((*propresprop)));
 /* pub_prop.h */
 SCIP_PROPDATA* SCIPpropGetData(SCIP_PROP* prop);
 void SCIPpropSetData(SCIP_PROP* prop, SCIP_PROPDATA* propdata);

@@ -48,6 +48,7 @@ struct SCIP_Prop
    SCIP_DECL_PROPINITSOL((*initsol));
    SCIP_DECL_PROPEXITSOL((*exitsol));
    SCIP_DECL_PROPPRESOL((*presol));
+   SCIP_DECL_PROPRESPROP((*resprop));
    char name[64];
 };
 
@@ -137,6 +138,7 @@ SCIP_RETCODE SCIPsetPropPresol(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPPRESOL
    prop->presol = f;
    return SCIP_OKAY;
 }
+SCIP_RETCODE SCIPsetPropResprop(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPRESPROP((*f))) { (void)scip; prop->resprop = f; return SCIP_OKAY; }
 SCIP_PROPDATA* SCIPpropGetData(SCIP_PROP* prop) { return prop->data; }
 void SCIPpropSetData(SCIP_PROP* prop, SCIP_PROPDATA* propdata) { prop->data = propdata; }
 
@@ -393,6 +395,19 @@ SCIP_EXPORT int th_presol(SCIP* s, int* result, int* nchgbds)
    SCIP_RESULT r = SCIP_DIDNOTRUN;
    SCIP_RETCODE rc = s->prop->presol(s, s->prop, 0, SCIP_PRESOLTIMING_MEDIUM, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
       &z[0], &z[1], &z[2], nchgbds, &z[4], &z[5], &z[6], &z[7], &z[8], &z[9], &r);
+   *result = (int)r;
+   return rc;
+}
+
+/* conflict analysis asking for the reason of the bound change of column j (prop.c:739-788) */
+SCIP_EXPORT int th_resprop(SCIP* s, int j, int upper, int* result)
+{
+   SCIP_RESULT r = SCIP_DIDNOTRUN;
+   SCIP_RETCODE rc;
+   if( s->prop->resprop == NULL )
+      return SCIP_PLUGINNOTFOUND;   /* what SCIPpropResolvePropagation reports */
+   rc = s->prop->resprop(s, s->prop, s->vars[j], 0, upper ? SCIP_BOUNDTYPE_UPPER : SCIP_BOUNDTYPE_LOWER, NULL,
+      upper ? s->vars[j]->lub : s->vars[j]->llb, &r);
    *result = (int)r;
    return rc;
 }

@@ -37,6 +37,7 @@ def lib():
         L.th_set_int_param.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
         L.th_propexec.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.th_presol.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.th_resprop.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int)]
         L.th_set_local_bounds.argtypes = [C.c_void_p, _P(np.float64), _P(np.float64)]
         L.th_set_local_bounds.restype = None
         L.th_get_bounds.argtypes = [C.c_void_p, C.c_int, _P(np.float64), _P(np.float64)]
@@ -76,6 +77,11 @@ class Host:
         n = C.c_int(0)
         rc = lib().th_presol(self.h, C.byref(r), C.byref(n))
         return rc, r.value, n.value
+
+    def resprop(self, j, upper):
+        r = C.c_int(0)
+        rc = lib().th_resprop(self.h, j, 1 if upper else 0, C.byref(r))
+        return rc, r.value
 
     def set_local(self, lb, ub):
         lib().th_set_local_bounds(self.h, np.ascontiguousarray(lb, np.float64), np.ascontiguousarray(ub, np.float64))

@@ -29,6 +29,15 @@ def test_plugin_registers_parameters():
     assert h.close() == ph.SCIP_OKAY
 
 
+def test_conflict_resolution_reports_unexplained_change():
+    inst = sa.gen_covering_packing(50, 40, 4, seed=3)
+    h = ph.Host(inst)
+    for upper in (False, True):
+        rc, res = h.resprop(0, upper)
+        assert rc == ph.SCIP_OKAY and res == ph.SCIP_DIDNOTFIND
+    assert h.close() == ph.SCIP_OKAY
+
+
 @pytest.mark.skipif(sa.device_count() > 0, reason="checks the no-device error path")
 def test_plugin_without_device_returns_a_scip_error():
     inst = sa.gen_covering_packing(50, 40, 4, seed=3)
