@@ -142,6 +142,63 @@ int sap_propagate_batch(sap_bounds* bnd, const sap_params* params, int32_t nvec,
    const double* ovub, int32_t* result, int32_t* rounds, double* out_lb, double* out_ub,
    sap_stats* stats);
 
+/* Node propagation from a device-resident base vector (vector basevec of base, same
+ * matrix), with sparse results: vector v starts from the base with the overlay entries
+ * ovptr[v]..ovptr[v+1]-1 applied (replaces a loop of SCIPpropagateProbing calls,
+ * scip_probing.c, one per node).  result[v] and rounds[v] as in sap_propagate_batch.  The
+ * columns whose bounds differ from the node's start are returned per vector, sorted by
+ * column: entries chgptr[v]..chgptr[v+1]-1 of chgcol/chglb/chgub (empty for cut-off
+ * vectors).  chgptr (nvec+1) is always filled; if chgptr[nvec] > chgcap the lists are not
+ * written and SAP_NOMEMORY is returned (bounds stay on the device: sap_bounds_download).
+ * chgcol == NULL asks for verdicts only. */
+int sap_propagate_nodes(sap_bounds* bnd, const sap_params* params, int32_t nvec, const sap_bounds* base,
+   int32_t basevec, const int64_t* ovptr, const int32_t* ovcol, const double* ovlb, const double* ovub,
+   int32_t* result, int32_t* rounds, int64_t chgcap, int64_t* chgptr, int32_t* chgcol, double* chglb,
+   double* chgub, sap_stats* stats);
+
+/* Independent instances packed block-diagonally into one resident matrix (one launch per
+ * round for all of them; replaces one consPropLinear fixpoint per SCIP instance).  Instance
+ * s owns rows [rowoff[s], rowoff[s+1]) and columns [coloff[s], coloff[s+1]) of the
+ * concatenated CSR (global column indices); a row referencing a column of another
+ * instance is SAP_INVALIDDATA. */
+int sap_matrix_create_packed(sap_context* ctx, int32_t ninst, const int64_t* rowoff, const int64_t* coloff,
+   int64_t nrows, int64_t ncols, const int64_t* rowptr, const int32_t* colidx, const double* vals,
+   const double* lhs, const double* rhs, const uint8_t* integral, const double* glb, const double* gub,
+   sap_matrix** mat);
+/* Propagate all instances of a packed matrix (or the one instance of a plain matrix) from
+ * lb/ub (overwritten).  Per instance: result[s] (SAP_CUTOFF / SAP_REDUCEDDOM /
+ * SAP_DIDNOTFIND) and rounds[s]; a cutoff of one instance does not stop the others, and every
+ * instance gets exactly the result it gets when propagated alone. */
+int sap_propagate_packed(sap_matrix* mat, const sap_params* params, double* lb, double* ub, int32_t* result,
+   int32_t* rounds, sap_stats* stats);
+
+/* Continuous batching (serving).  The nvec vectors of a sap_bounds are slots.
+ *   sap_slots_admit   starts slots[0..n) from vector basevec of base (another sap_bounds of
+ *                     the same matrix) plus the overlays; with SAP_FLAG_BASE_IS_FIXPOINT in
+ *                     flags only rows of overlay columns are swept first, else every row.
+ *   sap_slots_launch  enqueues SAP_STEP_SHORT or SAP_STEP_LONG rounds for every running slot
+ *                     (asynchronous; admitted slots join at the next launch);
+ *   sap_slots_query   *ready = 1 when the launched step has finished;
+ *   sap_slots_wait    waits for the step and lists the slots that finished in it (fixpoint,
+ *                     cutoff or their own maxrounds); stats cover the step;
+ *   sap_slots_step    launch + wait;
+ *   sap_slots_collect verdicts, rounds and changed bounds of finished slots (as in
+ *                     sap_propagate_nodes), after which the slots are free.
+ * Every slot gets exactly the result it gets alone: slots share launches, not state.  One
+ * sap_bounds serves either as slots or for one-shot calls (batch/nodes), not both.  The
+ * admission, step and collection calls of one sap_bounds must not overlap. */
+#define SAP_STEP_SHORT 2
+#define SAP_STEP_LONG  8
+int sap_slots_admit(sap_bounds* bnd, int32_t n, const int32_t* slots, const sap_bounds* base, int32_t basevec,
+   const int64_t* ovptr, const int32_t* ovcol, const double* ovlb, const double* ovub, int32_t flags);
+int sap_slots_launch(sap_bounds* bnd, const sap_params* params, int32_t rounds);
+int sap_slots_query(sap_bounds* bnd, int32_t* ready);
+int sap_slots_wait(sap_bounds* bnd, int32_t* nfinished, int32_t* finished, sap_stats* stats);
+int sap_slots_step(sap_bounds* bnd, const sap_params* params, int32_t rounds, int32_t* nfinished, int32_t* finished,
+   sap_stats* stats);
+int sap_slots_collect(sap_bounds* bnd, int32_t n, const int32_t* slots, int32_t* result, int32_t* rounds,
+   int64_t chgcap, int64_t* chgptr, int32_t* chgcol, double* chglb, double* chgub);
+
 /* Row activities on the given bounds with the engine's canonical double-double sums
  * (diagnostic / parity): minact/maxact finite parts (hi+lo), counts[8*i..8*i+7] =
  * {minposinf, minneginf, minposhuge, minneghuge, maxposinf, maxneginf, maxposhuge,

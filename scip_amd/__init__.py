@@ -34,6 +34,9 @@ SAP_REDUCEDDOM = 10
 SAP_FLAG_BASE_IS_FIXPOINT = 0x1
 SAP_FLAG_HOSTLOOP = 0x2
 
+SAP_STEP_SHORT = 2
+SAP_STEP_LONG = 8
+
 SAP_MODE_AUTO = 0
 SAP_MODE_DENSE = 1
 SAP_MODE_SPARSE = 2
@@ -121,6 +124,17 @@ EXPORTS = {
     "sap_propagate_resident": (C.c_int, [_P, C.POINTER(sap_params), C.POINTER(sap_stats)]),
     "sap_propagate_batch": (C.c_int, [_P, C.POINTER(sap_params), C.c_int32, _DP, _DP, _I64P, _I32P, _DP, _DP,
                                       _I32P, _I32P, _DP, _DP, C.POINTER(sap_stats)]),
+    "sap_propagate_nodes": (C.c_int, [_P, C.POINTER(sap_params), C.c_int32, _P, C.c_int32, _I64P, _I32P, _DP, _DP,
+                                      _I32P, _I32P, C.c_int64, _I64P, _I32P, _DP, _DP, C.POINTER(sap_stats)]),
+    "sap_matrix_create_packed": (C.c_int, [_P, C.c_int32, _I64P, _I64P, C.c_int64, C.c_int64, _I64P, _I32P, _DP,
+                                           _DP, _DP, _U8P, _DP, _DP, C.POINTER(_P)]),
+    "sap_propagate_packed": (C.c_int, [_P, C.POINTER(sap_params), _DP, _DP, _I32P, _I32P, C.POINTER(sap_stats)]),
+    "sap_slots_admit": (C.c_int, [_P, C.c_int32, _I32P, _P, C.c_int32, _I64P, _I32P, _DP, _DP, C.c_int32]),
+    "sap_slots_launch": (C.c_int, [_P, C.POINTER(sap_params), C.c_int32]),
+    "sap_slots_query": (C.c_int, [_P, _I32P]),
+    "sap_slots_wait": (C.c_int, [_P, _I32P, _I32P, C.POINTER(sap_stats)]),
+    "sap_slots_step": (C.c_int, [_P, C.POINTER(sap_params), C.c_int32, _I32P, _I32P, C.POINTER(sap_stats)]),
+    "sap_slots_collect": (C.c_int, [_P, C.c_int32, _I32P, _I32P, _I32P, C.c_int64, _I64P, _I32P, _DP, _DP]),
     "sap_row_activities": (C.c_int, [_P, C.POINTER(sap_params), _DP, _DP, _DP, _DP, _I32P, _DP]),
     "sap_gen_covering_packing": (C.c_int, [C.c_int64, C.c_int64, C.c_int32, C.c_uint64, C.POINTER(sap_instance)]),
     "sap_gen_mixed": (C.c_int, [C.c_int64, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32,
@@ -333,6 +347,58 @@ class Matrix:
             pass
 
 
+def pack_instances(insts):
+    """Concatenate instances block-diagonally: returns (Instance, rowoff, coloff)."""
+    rowoff = np.zeros(len(insts) + 1, dtype=np.int64)
+    coloff = np.zeros(len(insts) + 1, dtype=np.int64)
+    for i, t in enumerate(insts):
+        rowoff[i + 1] = rowoff[i] + t.nrows
+        coloff[i + 1] = coloff[i] + t.ncols
+    nnzoff = np.cumsum([0] + [t.nnz for t in insts])
+    rowptr = np.concatenate([[0]] + [t.rowptr[1:] + nnzoff[i] for i, t in enumerate(insts)]).astype(np.int64)
+    colidx = np.concatenate([t.colidx.astype(np.int64) + coloff[i] for i, t in enumerate(insts)]).astype(np.int32)
+    cat = lambda f, dt: np.concatenate([getattr(t, f) for t in insts]).astype(dt)
+    inst = Instance(rowptr=rowptr, colidx=colidx, vals=cat("vals", np.float64), lhs=cat("lhs", np.float64),
+                    rhs=cat("rhs", np.float64), integral=cat("integral", np.uint8), lb=cat("lb", np.float64),
+                    ub=cat("ub", np.float64))
+    return inst, rowoff, coloff
+
+
+class PackedMatrix(Matrix):
+    """Several independent instances in one resident matrix (sap_matrix_create_packed)."""
+
+    def __init__(self, ctx: Context, insts):
+        self.ctx = ctx
+        self.parts = list(insts)
+        inst, self.rowoff, self.coloff = pack_instances(self.parts)
+        self.inst = inst
+        self.h = _P()
+        self._keep = [np.ascontiguousarray(a) for a in (inst.rowptr, inst.colidx, inst.vals, inst.lhs, inst.rhs,
+                                                         inst.integral, inst.lb, inst.ub)]
+        rp, ci, va, lh, rh, it, lb, ub = self._keep
+        _chk(lib().sap_matrix_create_packed(ctx.h, len(self.parts), _i64p(self.rowoff), _i64p(self.coloff),
+                                            inst.nrows, inst.ncols, _i64p(rp), _i32p(ci), _dp(va), _dp(lh), _dp(rh),
+                                            _u8p(it), _dp(lb), _dp(ub), C.byref(self.h)), "sap_matrix_create_packed")
+
+    def propagate_packed(self, lb=None, ub=None, params: sap_params | None = None):
+        """Returns (lb, ub, result[ninst], rounds[ninst], stats)."""
+        lb = np.array(self.inst.lb if lb is None else lb, dtype=np.float64, copy=True)
+        ub = np.array(self.inst.ub if ub is None else ub, dtype=np.float64, copy=True)
+        k = len(self.parts)
+        res = np.zeros(k, dtype=np.int32)
+        rnd = np.zeros(k, dtype=np.int32)
+        st = sap_stats()
+        p = params or default_params()
+        _chk(lib().sap_propagate_packed(self.h, C.byref(p), _dp(lb), _dp(ub), _i32p(res), _i32p(rnd), C.byref(st)),
+             "sap_propagate_packed")
+        return lb, ub, res, rnd, st.as_dict()
+
+
+def _overlay_arrays(ovptr, ovcol, ovlb, ovub):
+    return (np.ascontiguousarray(ovptr, dtype=np.int64), np.ascontiguousarray(ovcol, dtype=np.int32),
+            np.ascontiguousarray(ovlb, dtype=np.float64), np.ascontiguousarray(ovub, dtype=np.float64))
+
+
 class Bounds:
     """Device-resident bound vectors (sap_bounds)."""
 
@@ -387,6 +453,82 @@ class Bounds:
             out["lb"] = olb.reshape(nvec, n)
             out["ub"] = oub.reshape(nvec, n)
         return out
+
+    def propagate_nodes(self, base: "Bounds", basevec, ovptr, ovcol, ovlb, ovub, params: sap_params | None = None,
+                        chgcap: int | None = None, want_changes: bool = True):
+        """Node vectors from a resident base vector; sparse changed-bound lists per node."""
+        nvec = len(ovptr) - 1
+        ovptr, ovcol, ovlb, ovub = _overlay_arrays(ovptr, ovcol, ovlb, ovub)
+        res = np.zeros(nvec, dtype=np.int32)
+        rnd = np.zeros(nvec, dtype=np.int32)
+        cap = int(chgcap if chgcap is not None else max(1024, 64 * nvec))
+        while True:
+            cptr = np.zeros(nvec + 1, dtype=np.int64)
+            ccol = np.zeros(max(cap, 1), dtype=np.int32) if want_changes else None
+            clb = np.zeros(max(cap, 1)) if want_changes else None
+            cub = np.zeros(max(cap, 1)) if want_changes else None
+            st = sap_stats()
+            p = params or default_params()
+            rc = lib().sap_propagate_nodes(self.h, C.byref(p), nvec, base.h, basevec, _i64p(ovptr), _i32p(ovcol),
+                                           _dp(ovlb), _dp(ovub), _i32p(res), _i32p(rnd), cap, _i64p(cptr),
+                                           _i32p(ccol), _dp(clb), _dp(cub), C.byref(st))
+            if rc == SAP_NOMEMORY and chgcap is None and int(cptr[-1]) > cap:
+                cap = int(cptr[-1])   # run again with room for every change (same results)
+                continue
+            _chk(rc, "sap_propagate_nodes")
+            n = int(cptr[-1])
+            out = dict(result=res, rounds=rnd, chgptr=cptr, stats=st.as_dict())
+            if want_changes:
+                out.update(chgcol=ccol[:n], chglb=clb[:n], chgub=cub[:n])
+            return out
+
+    # ---- continuous batching ----
+    def admit(self, slots, base: "Bounds", basevec, ovptr, ovcol, ovlb, ovub, flags: int = SAP_FLAG_BASE_IS_FIXPOINT):
+        slots = np.ascontiguousarray(slots, dtype=np.int32)
+        ovptr, ovcol, ovlb, ovub = _overlay_arrays(ovptr, ovcol, ovlb, ovub)
+        _chk(lib().sap_slots_admit(self.h, len(slots), _i32p(slots), base.h, basevec, _i64p(ovptr), _i32p(ovcol),
+                                   _dp(ovlb), _dp(ovub), flags), "sap_slots_admit")
+
+    def launch(self, params: sap_params | None = None, rounds: int = SAP_STEP_LONG):
+        p = params or default_params()
+        _chk(lib().sap_slots_launch(self.h, C.byref(p), rounds), "sap_slots_launch")
+
+    def query(self) -> bool:
+        r = C.c_int32(0)
+        _chk(lib().sap_slots_query(self.h, C.byref(r)), "sap_slots_query")
+        return bool(r.value)
+
+    def wait(self):
+        nf = C.c_int32(0)
+        fin = np.zeros(self.nvec, dtype=np.int32)
+        st = sap_stats()
+        _chk(lib().sap_slots_wait(self.h, C.byref(nf), _i32p(fin), C.byref(st)), "sap_slots_wait")
+        return fin[:nf.value].copy(), st.as_dict()
+
+    def step(self, params: sap_params | None = None, rounds: int = SAP_STEP_LONG):
+        self.launch(params, rounds)
+        return self.wait()
+
+    def collect(self, slots, chgcap: int | None = None):
+        slots = np.ascontiguousarray(slots, dtype=np.int32)
+        n = len(slots)
+        res = np.zeros(n, dtype=np.int32)
+        rnd = np.zeros(n, dtype=np.int32)
+        cptr = np.zeros(n + 1, dtype=np.int64)
+        cap = int(chgcap if chgcap is not None else max(1024, 64 * n))
+        for _ in range(2):
+            ccol = np.zeros(max(cap, 1), dtype=np.int32)
+            clb = np.zeros(max(cap, 1))
+            cub = np.zeros(max(cap, 1))
+            rc = lib().sap_slots_collect(self.h, n, _i32p(slots), _i32p(res), _i32p(rnd), cap, _i64p(cptr),
+                                         _i32p(ccol), _dp(clb), _dp(cub))
+            if rc == SAP_NOMEMORY and chgcap is None:
+                cap = int(cptr[-1])
+                continue
+            _chk(rc, "sap_slots_collect")
+            m = int(cptr[-1])
+            return dict(result=res, rounds=rnd, chgptr=cptr.copy(), chgcol=ccol[:m], chglb=clb[:m], chgub=cub[:m])
+        raise SapError(SAP_NOMEMORY, "sap_slots_collect")
 
     def close(self):
         if self.h:

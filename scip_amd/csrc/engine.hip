@@ -74,6 +74,8 @@ struct Status
    unsigned long long tslot1[SAP_STAT_SLOTS];
    unsigned long long sweep_ticks;
    unsigned long long sweep_count;
+   int32_t cont;        // continuous batching: no global round cap (per-vector caps apply)
+   int32_t pad3;
 };
 
 struct Mat
@@ -99,6 +101,12 @@ struct Mat
    const unsigned long long* tmask;
    const int32_t* trow0;
    const uint8_t* tlev;     // levels of the shift tree: ceil(log2(longest row of the tile))
+   // independent instances packed block-diagonally: rows and columns carry their instance
+   // (segment); verdicts, cutoffs and round counts are kept per (vector, segment) unit
+   int32_t nseg;
+   int32_t pad2;
+   const int32_t* rowseg;   // NULL when nseg == 1
+   const int32_t* colseg;
 };
 
 struct Vec
@@ -115,8 +123,17 @@ struct Vec
    int32_t* coltag;         // last round a column got a candidate
    int32_t* rowmark;        // round a row is marked for
    uint8_t* redundant;      // nvec*nrows
-   int32_t* vcutoff;        // nvec
-   int32_t* vrounds;        // nvec
+   int32_t* vcutoff;        // per unit (vector, segment): infeasibility detected
+   int32_t* vrounds;        // per unit: last round id in which one of its rows was swept
+   int32_t* vstart;         // per vector: round id of its first round
+   int32_t* vpend;          // per vector: latest round id for which one of its rows is marked
+   int32_t* vlogn;          // per vector: columns with a changed bound (change log length)
+   int32_t* log;            // nvec*ncols: changed columns of vector v at [v*ncols, v*ncols+vlogn[v])
+   uint32_t* logbits;       // nvec*ncols bits: column already in the change log
+   int32_t* ovl;            // nvec*ncols: overlay columns of vector v at [v*ncols, v*ncols+ovln[v])
+   int32_t* ovln;           // per vector
+   int32_t logchg;          // record changed columns
+   int32_t nseg;            // segments of the matrix (units per vector)
    int2* wl[2][SAP_NBINS];  // sparse mode: (row, vector) worklists, double-buffered by round parity
    int2* touched;           // sparse mode: (column, vector)
    Status* st;
@@ -394,7 +411,18 @@ struct Ctx
    const Params* P;
    int32_t v;
    int32_t round;
+   int32_t row;
 };
+
+__device__ __forceinline__ int32_t unitOfRow(const Mat& M, int32_t v, int32_t row)
+{
+   return M.rowseg == NULL ? v : v * M.nseg + M.rowseg[row];
+}
+
+__device__ __forceinline__ int32_t unitOfCol(const Mat& M, int32_t v, int32_t j)
+{
+   return M.colseg == NULL ? v : v * M.nseg + M.colseg[j];
+}
 
 __device__ __forceinline__ void touch(const Ctx& C, size_t flat, int32_t j)
 {
@@ -433,7 +461,7 @@ __device__ __forceinline__ void submitLb(const Ctx& C, int32_t j, double val, bo
 
 __device__ __forceinline__ void setCutoff(const Ctx& C)
 {
-   C.V->vcutoff[C.v] = 1;
+   C.V->vcutoff[unitOfRow(*C.M, C.v, C.row)] = 1;
    if( C.V->single )
       C.V->st->cutoff = 1;
 }
@@ -812,6 +840,7 @@ __device__ __forceinline__ void drainQueue(const Mat& M, const Vec& V, const Par
       C.P = &P;
       C.v = v;
       C.round = round;
+      C.row = row;
       const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
       const Elem e = loadElem(M, bnd, k);
       const double2 sides = M.sides[row];
@@ -882,6 +911,14 @@ struct Items
    int32_t firstall;
 };
 
+// the unit of (v, row) swept a row in this round
+__device__ __forceinline__ void noteRound(const Mat& M, const Vec& V, int32_t v, int32_t row, int32_t round)
+{
+   const int32_t u = unitOfRow(M, v, row);
+   if( V.vrounds[u] < round )
+      atomicMax(&V.vrounds[u], round);
+}
+
 __device__ __forceinline__ bool fetchItem(const Mat& M, const Vec& V, const Items& I, uint32_t it, int32_t& row,
    int32_t& v)
 {
@@ -898,7 +935,7 @@ __device__ __forceinline__ bool fetchItem(const Mat& M, const Vec& V, const Item
       row = item.x;
       v = item.y;
    }
-   return V.vcutoff[v] == 0;
+   return V.vcutoff[unitOfRow(M, v, row)] == 0;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -925,6 +962,7 @@ __device__ __forceinline__ void sweepGroups(const Mat& M, const Vec& V, const Pa
       C.P = &P;
       C.v = v;
       C.round = I.round;
+      C.row = row;
       const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
       const int64_t s = live ? M.rowptr[row] : 0;
       const int64_t n = live ? M.rowptr[row + 1] - s : 0;
@@ -976,8 +1014,8 @@ __device__ __forceinline__ void sweepGroups(const Mat& M, const Vec& V, const Pa
          rowVerdict(C, rs, sides.x, sides.y, row);
          nnzacc += (unsigned long long)n;
          rowsacc += 1;
-         if( !V.single && V.vrounds[v] < rlocal )
-            atomicMax(&V.vrounds[v], rlocal);
+         if( !V.single )
+            noteRound(M, V, v, row, I.round);
       }
       maybeDrain(M, V, P, Q, I.round);
    }
@@ -1011,7 +1049,7 @@ __device__ __forceinline__ void sweepTiles(const Mat& M, const Vec& V, const Par
       const uint32_t tu = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
       const int32_t v = (int32_t)(tu / ntiles);
       const int32_t tile = (int32_t)(tu - (uint32_t)v * ntiles);
-      if( V.vcutoff[v] )
+      if( M.rowseg == NULL && V.vcutoff[v] )
          continue;
       const int64_t e0 = M.tstart[tile];
       const int cnt = (int)(M.tstart[tile + 1] - e0);
@@ -1029,7 +1067,8 @@ __device__ __forceinline__ void sweepTiles(const Mat& M, const Vec& V, const Par
       const int p = lane - sl;
       const int32_t row = row0 + k;
       const bool marked = has
-         && (I.firstall || V.rowmark[(size_t)v * (size_t)M.nrows + (size_t)row] == I.round);
+         && (I.firstall || V.rowmark[(size_t)v * (size_t)M.nrows + (size_t)row] == I.round)
+         && (M.rowseg == NULL || V.vcutoff[unitOfRow(M, v, row)] == 0);
       if( __ballot(marked) == 0 )
          continue;
 
@@ -1088,6 +1127,7 @@ __device__ __forceinline__ void sweepTiles(const Mat& M, const Vec& V, const Par
       C.P = &P;
       C.v = v;
       C.round = I.round;
+      C.row = row;
       const RowDecision d = decideRow(P, rs, sides.x, sides.y, L);
       const bool pass = marked && d.tighten && screenElem(P, rs, d, sides.x, sides.y, e);
       pushQueue(M, V, P, Q, pass, e0 + lane, row, v, rs, d, I.round);
@@ -1098,8 +1138,8 @@ __device__ __forceinline__ void sweepTiles(const Mat& M, const Vec& V, const Par
          rowVerdict(C, rs, sides.x, sides.y, row);
          nnzacc += (unsigned long long)L;
          rowsacc += 1;
-         if( !V.single && V.vrounds[v] < I.rlocal )
-            atomicMax(&V.vrounds[v], I.rlocal);
+         if( !V.single )
+            noteRound(M, V, v, row, I.round);
       }
       maybeDrain(M, V, P, Q, I.round);
    }
@@ -1269,6 +1309,7 @@ __device__ __forceinline__ void sweepWaves(const Mat& M, const Vec& V, const Par
       C.P = &P;
       C.v = v;
       C.round = I.round;
+      C.row = row;
       const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
       const int64_t s = M.rowptr[row];
       const int64_t n = M.rowptr[row + 1] - s;
@@ -1293,8 +1334,8 @@ __device__ __forceinline__ void sweepWaves(const Mat& M, const Vec& V, const Par
          rowVerdict(C, rs, sides.x, sides.y, row);
          nnzacc += (unsigned long long)n;
          rowsacc += 1;
-         if( !V.single && V.vrounds[v] < rlocal )
-            atomicMax(&V.vrounds[v], rlocal);
+         if( !V.single )
+            noteRound(M, V, v, row, I.round);
       }
    }
 }
@@ -1320,6 +1361,7 @@ __device__ __forceinline__ void sweepBlocks(const Mat& M, const Vec& V, const Pa
       C.P = &P;
       C.v = v;
       C.round = I.round;
+      C.row = row;
       const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
       const int64_t s = M.rowptr[row];
       const int64_t n = M.rowptr[row + 1] - s;
@@ -1363,8 +1405,8 @@ __device__ __forceinline__ void sweepBlocks(const Mat& M, const Vec& V, const Pa
          rowVerdict(C, rs, sides.x, sides.y, row);
          nnzacc += (unsigned long long)n;
          rowsacc += 1;
-         if( !V.single && V.vrounds[v] < rlocal )
-            atomicMax(&V.vrounds[v], rlocal);
+         if( !V.single )
+            noteRound(M, V, v, row, I.round);
       }
       __syncthreads();
    }
@@ -1466,19 +1508,17 @@ __global__ __launch_bounds__(SAP_BLOCK, 4) void k_sweep(Mat M, Vec V, Params P, 
       if( I.firstall && lblock == 0 )
       {
          const int32_t nempty = M.binstart[1] - M.binstart[0];
-         bool bad = false;
          for( int32_t r = M.binstart[0] + (int32_t)threadIdx.x; r < M.binstart[1]; r += SAP_BLOCK )
          {
             const double2 sd = M.sides[r];
-            bad |= isFeasGT(P, 0.0, sd.y) || isFeasLT(P, 0.0, sd.x);
+            if( isFeasGT(P, 0.0, sd.y) || isFeasLT(P, 0.0, sd.x) )
+               for( int32_t v = 0; v < V.nvec; ++v )
+               {
+                  V.vcutoff[unitOfRow(M, v, r)] = 1;
+                  if( V.single )
+                     st->cutoff = 1;
+               }
          }
-         if( bad )
-            for( int32_t v = 0; v < V.nvec; ++v )
-            {
-               V.vcutoff[v] = 1;
-               if( V.single )
-                  st->cutoff = 1;
-            }
          if( threadIdx.x == 0 )
             rowsacc += (unsigned long long)V.nvec * (unsigned long long)nempty;
       }
@@ -1522,7 +1562,8 @@ __device__ __forceinline__ int columnStep(const Mat& M, const Vec& V, const Para
       V.flb[flat] = KEY_NEG_INF;
       V.fub[flat] = KEY_POS_INF;
    }
-   if( V.vcutoff[v] )
+   const int32_t unit = unitOfCol(M, v, j);
+   if( V.vcutoff[unit] )
       return 0;
    const bool integral = (M.colflags[j] & 1) != 0;
    const double2 b0 = V.bnd[flat];
@@ -1581,7 +1622,7 @@ __device__ __forceinline__ int columnStep(const Mat& M, const Vec& V, const Para
    }
    if( cut )
    {
-      V.vcutoff[v] = 1;
+      V.vcutoff[unit] = 1;
       if( V.single )
          st->cutoff = 1;
       return 0;
@@ -1591,6 +1632,19 @@ __device__ __forceinline__ int columnStep(const Mat& M, const Vec& V, const Para
    if( !chl && !chu )
       return 0;
    V.bnd[flat] = make_double2(nlb, nub);
+   if( V.logchg )
+   {
+      // first change of this column in the vector's life: append it to the vector's log
+      const uint32_t bit = 1u << (flat & 31);
+      if( (atomicOr(&V.logbits[flat >> 5], bit) & bit) == 0 )
+      {
+         const int32_t at = atomicAdd(&V.vlogn[v], 1);
+         V.log[(size_t)v * (size_t)M.ncols + (size_t)at] = j;
+      }
+   }
+   // a vector stops after its own maxrounds rounds: rows are not marked beyond
+   if( P.maxrounds >= 0 && next - V.vstart[v] + 1 > P.maxrounds )
+      return (chl ? 1 : 0) + (chu ? 1 : 0);
    // mark the rows of the column for the next round (eventExecLinear marks them)
    for( int64_t q = M.colptr[j]; q < M.colptr[j + 1]; ++q )
    {
@@ -1598,6 +1652,8 @@ __device__ __forceinline__ int columnStep(const Mat& M, const Vec& V, const Para
       const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)row;
       if( V.redundant[rflat] )
          continue;
+      if( V.vpend[v] != next )
+         V.vpend[v] = next;   // every writer of this round stores the same value
       if( V.dense )
       {
          if( V.rowmark[rflat] != next )
@@ -1693,7 +1749,7 @@ __global__ __launch_bounds__(SAP_STAT_SLOTS) void k_round_begin(Vec V, int32_t m
          in += st->wlcount[r & 1][b];
       more = in != 0;
    }
-   if( !more || st->cutoff || (maxrounds >= 0 && st->rlocal >= maxrounds) )
+   if( !more || st->cutoff || (!st->cont && maxrounds >= 0 && st->rlocal >= maxrounds) )
    {
       st->done = 1;
       return;
@@ -1713,14 +1769,24 @@ __global__ void k_call_reset(Vec V, int64_t nrows, int32_t nvec, int32_t firstal
    for( size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x )
       V.redundant[i] = 0;
    const size_t gi = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+   const size_t nunits = (size_t)nvec * (size_t)V.nseg;
+   for( size_t u = gi; u < nunits; u += (size_t)gridDim.x * blockDim.x )
+   {
+      V.vcutoff[u] = 0;
+      V.vrounds[u] = 0;
+   }
    if( gi < (size_t)nvec )
    {
-      V.vcutoff[gi] = 0;
-      V.vrounds[gi] = 0;
+      V.vstart[gi] = V.st->round + 1;
+      V.vpend[gi] = firstall ? V.st->round + 1 : 0;
+      V.vlogn[gi] = 0;
+      if( V.ovln != NULL )
+         V.ovln[gi] = 0;
    }
    if( gi == 0 )
    {
       Status* st = V.st;
+      st->cont = 0;
       st->rlocal = 0;
       st->done = 0;
       st->cutoff = 0;
@@ -1763,6 +1829,8 @@ __global__ void k_init_all(Mat M, Vec V, int32_t bin, int32_t nvec)
    int2* wl = V.wl[par][bin];
    for( size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x )
       wl[i] = make_int2(base + (int32_t)(i % len), (int32_t)(i / len));
+   for( size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < (size_t)nvec; v += (size_t)gridDim.x * blockDim.x )
+      V.vpend[v] = st->round + 1;
    if( blockIdx.x == 0 && threadIdx.x == 0 )
       st->wlcount[par][bin] = (uint32_t)tot;
 }
@@ -1784,6 +1852,8 @@ __global__ void k_init_overlay(Mat M, Vec V, const int64_t* ovptr, const int32_t
          {
             const int32_t row = M.rowidx[r];
             const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)row;
+            if( V.vpend[v] != first )
+               V.vpend[v] = first;
             if( V.dense )
             {
                V.rowmark[rflat] = first;
@@ -1799,6 +1869,183 @@ __global__ void k_init_overlay(Mat M, Vec V, const int64_t* ovptr, const int32_t
    }
    if( __ballot(any) != 0 && ((int)threadIdx.x & (SAP_WAVE - 1)) == 0 )
       st->anymark[par] = 1;
+}
+
+// ---- slots: admission, step control, collection (vector indices through a slot list) ----
+
+// fresh state for admitted slots: redundancy flags, units, start round, change log length;
+// the bounds of each slot are the base vector plus the slot's overlay
+__global__ void k_slot_reset(Vec V, int64_t nrows, const int32_t* slots, int32_t n)
+{
+   const int32_t first = V.st->round + 1;
+   for( int32_t i = blockIdx.y; i < n; i += gridDim.y )
+   {
+      const int32_t v = slots[i];
+      uint8_t* red = V.redundant + (size_t)v * (size_t)nrows;
+      for( int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * blockDim.x )
+         red[r] = 0;
+      if( blockIdx.x == 0 )
+         for( int32_t q = (int32_t)threadIdx.x; q < V.nseg; q += (int32_t)blockDim.x )
+         {
+            V.vcutoff[(size_t)v * (size_t)V.nseg + (size_t)q] = 0;
+            V.vrounds[(size_t)v * (size_t)V.nseg + (size_t)q] = 0;
+         }
+      if( blockIdx.x == 0 && threadIdx.x == 0 )
+      {
+         V.vstart[v] = first;
+         V.vpend[v] = 0;
+         V.vlogn[v] = 0;
+         V.ovln[v] = 0;
+      }
+   }
+}
+
+__global__ void k_slot_base(double2* bnd, const double2* base, int64_t ncols, const int32_t* slots, int32_t n)
+{
+   for( int32_t i = blockIdx.y; i < n; i += gridDim.y )
+   {
+      double2* dst = bnd + (size_t)slots[i] * (size_t)ncols;
+      for( int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < ncols; j += (int64_t)gridDim.x * blockDim.x )
+         dst[j] = base[j];
+   }
+}
+
+__global__ void k_slot_overlay(Vec V, int64_t ncols, const int32_t* slots, int32_t n, const int64_t* ovptr,
+   const int32_t* ovcol, const double* ovlb, const double* ovub)
+{
+   for( int32_t i = blockIdx.y; i < n; i += gridDim.y )
+   {
+      const int32_t v = slots[i];
+      double2* dst = V.bnd + (size_t)v * (size_t)ncols;
+      int32_t* save = V.ovl + (size_t)v * (size_t)ncols;
+      for( int64_t q = ovptr[i] + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ovptr[i + 1];
+           q += (int64_t)gridDim.x * blockDim.x )
+      {
+         dst[ovcol[q]] = make_double2(ovlb[q], ovub[q]);
+         save[q - ovptr[i]] = ovcol[q];
+      }
+      if( blockIdx.x == 0 && threadIdx.x == 0 )
+         V.ovln[v] = (int32_t)(ovptr[i + 1] - ovptr[i]);
+   }
+}
+
+// back to the base vector: only the columns of the change log and of the overlay of the
+// slot's previous start differ from it
+__global__ void k_slot_restore(Vec V, const double2* base, int64_t ncols, const int32_t* slots, int32_t n)
+{
+   for( int32_t i = blockIdx.y; i < n; i += gridDim.y )
+   {
+      const int32_t v = slots[i];
+      double2* dst = V.bnd + (size_t)v * (size_t)ncols;
+      const int32_t* log = V.log + (size_t)v * (size_t)ncols;
+      const int32_t* ovl = V.ovl + (size_t)v * (size_t)ncols;
+      const int32_t nl = V.vlogn[v], no = V.ovln[v];
+      for( int32_t q = blockIdx.x * (int32_t)blockDim.x + (int32_t)threadIdx.x; q < nl + no;
+           q += (int32_t)(gridDim.x * blockDim.x) )
+      {
+         const int32_t j = q < nl ? log[q] : ovl[q - nl];
+         dst[j] = base[j];
+      }
+   }
+}
+
+// marks for the first round of the admitted slots: the rows of the overlay columns (base is a
+// fixpoint), or every row
+__global__ void k_slot_mark(Mat M, Vec V, const int32_t* slots, int32_t n, const int64_t* ovptr, const int32_t* ovcol,
+   int32_t allrows)
+{
+   Status* st = V.st;
+   const int32_t first = st->round + 1;
+   const int par = first & 1;
+   bool any = false;
+   for( int32_t i = blockIdx.y; i < n; i += gridDim.y )
+   {
+      const int32_t v = slots[i];
+      const int64_t cnt = allrows ? M.nrows : ovptr[i + 1] - ovptr[i];
+      for( int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < cnt; q += (int64_t)gridDim.x * blockDim.x )
+      {
+         int64_t r0, r1;
+         if( allrows )
+         {
+            r0 = q;
+            r1 = q + 1;
+         }
+         else
+         {
+            const int32_t j = ovcol[ovptr[i] + q];
+            r0 = M.colptr[j];
+            r1 = M.colptr[j + 1];
+         }
+         for( int64_t r = r0; r < r1; ++r )
+         {
+            const int32_t row = allrows ? (int32_t)r : M.rowidx[r];
+            const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)row;
+            if( V.vpend[v] != first )
+               V.vpend[v] = first;
+            if( V.dense )
+            {
+               V.rowmark[rflat] = first;
+               any = true;
+            }
+            else
+            {
+               const bool fresh = atomicMax(&V.rowmark[rflat], first) < first;
+               waveAppendRow(st, V.wl[par], par, fresh, M.rowbin[row], make_int2(row, v));
+            }
+         }
+      }
+   }
+   if( __ballot(any) != 0 && ((int)threadIdx.x & (SAP_WAVE - 1)) == 0 )
+      st->anymark[par] = 1;
+}
+
+// start of a step of continuous batching: clear the stop flag and the step's statistics
+__global__ void k_step_begin(Vec V)
+{
+   Status* st = V.st;
+   const int t = (int)threadIdx.x;
+   if( t < SAP_STAT_SLOTS )
+   {
+      st->slot_nnz[t] = 0;
+      st->slot_rows[t] = 0;
+      st->slot_chg[t] = 0;
+      st->tslot0[t] = ~0ULL;
+      st->tslot1[t] = 0;
+   }
+   if( t == 0 )
+   {
+      st->cont = 1;
+      st->done = 0;
+      st->cutoff = 0;
+      st->rlocal = 0;
+      st->firstall = 0;
+      st->sweep_ticks = 0;
+      st->sweep_count = 0;
+   }
+}
+
+// changed columns of the listed slots with their final bounds, at outoff[i] for slot i; the
+// log bits of the collected entries are cleared for the slot's next admission
+__global__ void k_collect(Vec V, int64_t ncols, const int32_t* slots, int32_t n, const int64_t* outoff,
+   int32_t* outcol, double2* outbnd)
+{
+   for( int32_t i = blockIdx.y; i < n; i += gridDim.y )
+   {
+      const int32_t v = slots[i];
+      const int32_t cnt = V.vlogn[v];
+      const int32_t* log = V.log + (size_t)v * (size_t)ncols;
+      for( int32_t q = blockIdx.x * (int32_t)blockDim.x + (int32_t)threadIdx.x; q < cnt; q += (int32_t)(gridDim.x * blockDim.x) )
+      {
+         const int32_t j = log[q];
+         const size_t flat = (size_t)v * (size_t)ncols + (size_t)j;
+         if( outcol != NULL )
+         {
+            outcol[outoff[i] + q] = j;
+            outbnd[outoff[i] + q] = V.bnd[flat];
+         }
+         atomicAnd(&V.logbits[flat >> 5], ~(1u << (flat & 31)));
+      }
+   }
 }
 
 // bounds of all vectors from a base vector plus overlays
@@ -1905,6 +2152,11 @@ struct sap_matrix
    unsigned long long* tmask;
    int32_t* trow0;
    uint8_t* tlev;
+   int32_t nseg;         // packed instances (1: a single instance)
+   int32_t* rowseg;      // device, by device row position (NULL when nseg == 1)
+   int32_t* colseg;      // device (NULL when nseg == 1)
+   std::vector<int64_t>* rowoff;   // host: segment row / column ranges (caller order)
+   std::vector<int64_t>* coloff;
    sap_bounds* single;   // cached one-vector state for sap_propagate
 };
 
@@ -1927,7 +2179,7 @@ struct GraphKey
    int32_t nvec;
    int32_t single;
    int32_t dense;
-   int32_t pad;
+   int32_t logchg;
 };
 
 struct sap_bounds
@@ -1942,6 +2194,32 @@ struct sap_bounds
    int32_t graphbroken;   // capture failed once: use the host-driven loop
    RoundGraph graph[2];
    GraphKey key;
+   // persistent staging for batches and slots (grown on demand, never per call)
+   int64_t* ovptr;       // nvec+1
+   int32_t* ovcol;
+   double* ovlb;
+   double* ovub;
+   int64_t ovcap;
+   double2* base2;       // ncols
+   int32_t* slotidx;     // nvec: slot list of an admission / collection
+   int32_t* slotidx2;    // nvec: second list (restored slots)
+   int64_t* outoff;      // nvec+1: output offsets of a collection
+   int32_t* outcol;      // gathered changes of a collection
+   double2* outbnd;
+   int64_t outcap;
+   int32_t logready;     // change log allocated and cleared
+   int32_t slotsmode;    // slots are in use (continuous batching)
+   int32_t inflight;     // a step was launched and not yet waited for
+   std::vector<int32_t>* active;   // host: 0 free, 1 running, 2 finished and not yet collected
+   Status* hstat;        // pinned: status after the last step
+   int32_t* hpend;       // pinned: vpend then vcutoff after the last step (2*nvec)
+   hipEvent_t evs[2];    // around a step
+   int64_t gen;          // bumped by every write to the vectors' bounds outside the change log
+   // per vector: the base vector it was last started from; while that base is unchanged, only
+   // the columns of the change log and of the overlay differ from it
+   std::vector<const sap_bounds*>* srcb;
+   std::vector<int32_t>* srcv;
+   std::vector<int64_t>* srcgen;
 };
 
 static int hipToRc(hipError_t e)
@@ -2039,6 +2317,10 @@ static Mat toMat(const sap_matrix* m)
    M.tmask = m->tmask;
    M.trow0 = m->trow0;
    M.tlev = m->tlev;
+   M.nseg = m->nseg;
+   M.pad2 = 0;
+   M.rowseg = m->nseg > 1 ? m->rowseg : NULL;
+   M.colseg = m->nseg > 1 ? m->colseg : NULL;
    return M;
 }
 
@@ -2144,19 +2426,49 @@ static void matrixRelease(sap_matrix* m)
    (void)hipFree(m->tmask);
    (void)hipFree(m->trow0);
    (void)hipFree(m->tlev);
+   (void)hipFree(m->rowseg);
+   (void)hipFree(m->colseg);
+   delete m->rowoff;
+   delete m->coloff;
 }
 
-int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int64_t* rowptr, const int32_t* colidx,
+// upload of one instance (nseg == 1) or of nseg instances packed block-diagonally, instance s
+// owning rows [rowoff[s], rowoff[s+1]) and columns [coloff[s], coloff[s+1])
+static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const int64_t* rowptr, const int32_t* colidx,
    const double* vals, const double* lhs, const double* rhs, const uint8_t* integral, const double* glb,
-   const double* gub, sap_matrix** out)
+   const double* gub, int32_t nseg, const int64_t* rowoff, const int64_t* coloff, sap_matrix** out)
 {
    if( ctx == NULL || out == NULL || rowptr == NULL || lhs == NULL || rhs == NULL || integral == NULL || glb == NULL
-      || gub == NULL || nrows < 0 || ncols < 0 || nrows > INT32_MAX || ncols > INT32_MAX )
+      || gub == NULL || nrows < 0 || ncols < 0 || nrows > INT32_MAX || ncols > INT32_MAX || nseg < 1 )
       return SAP_INVALIDDATA;
    *out = NULL;
    const int64_t nnz = rowptr[nrows];
    if( rowptr[0] != 0 || nnz < 0 || (nnz > 0 && (colidx == NULL || vals == NULL)) )
       return SAP_INVALIDDATA;
+   // segment of every row and column (instances must not share columns)
+   std::vector<int32_t> oseg, cseg;
+   if( nseg > 1 )
+   {
+      if( rowoff == NULL || coloff == NULL || rowoff[0] != 0 || coloff[0] != 0 || rowoff[nseg] != nrows
+         || coloff[nseg] != ncols )
+         return SAP_INVALIDDATA;
+      oseg.resize((size_t)nrows);
+      cseg.resize((size_t)ncols);
+      for( int32_t q = 0; q < nseg; ++q )
+      {
+         if( rowoff[q + 1] < rowoff[q] || coloff[q + 1] < coloff[q] )
+            return SAP_INVALIDDATA;
+         for( int64_t i = rowoff[q]; i < rowoff[q + 1]; ++i )
+         {
+            oseg[(size_t)i] = q;
+            for( int64_t k = rowptr[i]; k < rowptr[i + 1] && k >= 0 && k < nnz; ++k )
+               if( colidx[k] < coloff[q] || colidx[k] >= coloff[q + 1] )
+                  return SAP_INVALIDDATA;
+         }
+         for( int64_t j = coloff[q]; j < coloff[q + 1]; ++j )
+            cseg[(size_t)j] = q;
+      }
+   }
    // host-side validation and derived arrays; rows are stored grouped by length bin
    // (stable), so each bin is a contiguous range and sub-wavefront groups load contiguously
    std::vector<uint8_t> colflags((size_t)ncols);
@@ -2335,6 +2647,14 @@ int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int6
    MALLOC_OR_FAIL(m->tmask, tmask.size());
    MALLOC_OR_FAIL(m->trow0, trow0.size());
    MALLOC_OR_FAIL(m->tlev, tlev.size());
+   m->nseg = nseg;
+   if( nseg > 1 )
+   {
+      MALLOC_OR_FAIL(m->rowseg, nrows);
+      MALLOC_OR_FAIL(m->colseg, ncols);
+      m->rowoff = new std::vector<int64_t>(rowoff, rowoff + nseg + 1);
+      m->coloff = new std::vector<int64_t>(coloff, coloff + nseg + 1);
+   }
 #undef MALLOC_OR_FAIL
    hipStream_t s = ctx->stream;
    HIPCHK(hipMemcpyAsync(m->rowptr, prowptr.data(), sizeof(int64_t) * (size_t)(nrows + 1), hipMemcpyHostToDevice, s));
@@ -2361,9 +2681,37 @@ int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int6
       HIPCHK(hipMemcpyAsync(m->trow0, trow0.data(), sizeof(int32_t) * trow0.size(), hipMemcpyHostToDevice, s));
       HIPCHK(hipMemcpyAsync(m->tlev, tlev.data(), tlev.size(), hipMemcpyHostToDevice, s));
    }
+   if( nseg > 1 )
+   {
+      std::vector<int32_t> prseg((size_t)nrows);
+      for( int64_t r = 0; r < nrows; ++r )
+         prseg[(size_t)r] = oseg[(size_t)perm[(size_t)r]];
+      if( nrows > 0 )
+         HIPCHK(hipMemcpyAsync(m->rowseg, prseg.data(), sizeof(int32_t) * (size_t)nrows, hipMemcpyHostToDevice, s));
+      if( ncols > 0 )
+         HIPCHK(hipMemcpyAsync(m->colseg, cseg.data(), sizeof(int32_t) * (size_t)ncols, hipMemcpyHostToDevice, s));
+      HIPCHK(hipStreamSynchronize(s));
+   }
    HIPCHK(hipStreamSynchronize(s));
    *out = m;
    return SAP_OKAY;
+}
+
+int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int64_t* rowptr, const int32_t* colidx,
+   const double* vals, const double* lhs, const double* rhs, const uint8_t* integral, const double* glb,
+   const double* gub, sap_matrix** out)
+{
+   return matrixCreate(ctx, nrows, ncols, rowptr, colidx, vals, lhs, rhs, integral, glb, gub, 1, NULL, NULL, out);
+}
+
+int sap_matrix_create_packed(sap_context* ctx, int32_t ninst, const int64_t* rowoff, const int64_t* coloff,
+   int64_t nrows, int64_t ncols, const int64_t* rowptr, const int32_t* colidx, const double* vals, const double* lhs,
+   const double* rhs, const uint8_t* integral, const double* glb, const double* gub, sap_matrix** out)
+{
+   if( ninst < 1 )
+      return SAP_INVALIDDATA;
+   return matrixCreate(ctx, nrows, ncols, rowptr, colidx, vals, lhs, rhs, integral, glb, gub, ninst, rowoff, coloff,
+      out);
 }
 
 int sap_matrix_free(sap_matrix* m)
@@ -2407,6 +2755,40 @@ static void boundsRelease(sap_bounds* b)
    (void)hipFree(V.redundant);
    (void)hipFree(V.vcutoff);
    (void)hipFree(V.vrounds);
+   (void)hipFree(V.vstart);
+   (void)hipFree(V.vpend);
+   (void)hipFree(V.vlogn);
+   (void)hipFree(V.log);
+   (void)hipFree(V.logbits);
+   (void)hipFree(b->ovptr);
+   (void)hipFree(b->ovcol);
+   (void)hipFree(b->ovlb);
+   (void)hipFree(b->ovub);
+   (void)hipFree(b->base2);
+   (void)hipFree(b->slotidx);
+   (void)hipFree(b->slotidx2);
+   (void)hipFree(b->outoff);
+   (void)hipFree(b->outcol);
+   (void)hipFree(b->outbnd);
+   delete b->active;
+   b->active = NULL;
+   delete b->srcb;
+   delete b->srcv;
+   delete b->srcgen;
+   b->srcb = NULL;
+   b->srcv = NULL;
+   b->srcgen = NULL;
+   (void)hipFree(V.ovl);
+   (void)hipFree(V.ovln);
+   if( b->hstat )
+      (void)hipHostFree(b->hstat);
+   if( b->hpend )
+      (void)hipHostFree(b->hpend);
+   b->hstat = NULL;
+   b->hpend = NULL;
+   for( int e = 0; e < 2; ++e )
+      if( b->evs[e] != NULL )
+         (void)hipEventDestroy(b->evs[e]);
    for( int p = 0; p < 2; ++p )
       for( int q = 0; q < SAP_NBINS; ++q )
          (void)hipFree(V.wl[p][q]);
@@ -2448,7 +2830,8 @@ int sap_bounds_create_mode(sap_matrix* m, int32_t nvec, int32_t mode, sap_bounds
    b->nvec = nvec;
    Vec& V = b->V;
    V.nvec = nvec;
-   V.single = (nvec == 1);
+   V.nseg = m->nseg;
+   V.single = (nvec == 1 && m->nseg == 1);
    V.hasforced = m->hasforced;
    const size_t NC = (size_t)nvec * (size_t)m->ncols;
    const size_t NR = (size_t)nvec * (size_t)m->nrows;
@@ -2479,8 +2862,16 @@ int sap_bounds_create_mode(sap_matrix* m, int32_t nvec, int32_t mode, sap_bounds
    BALLOC(V.coltag, NC);
    BALLOC(V.rowmark, NR);
    BALLOC(V.redundant, NR);
-   BALLOC(V.vcutoff, nvec);
-   BALLOC(V.vrounds, nvec);
+   BALLOC(V.vcutoff, (size_t)nvec * (size_t)m->nseg);
+   BALLOC(V.vrounds, (size_t)nvec * (size_t)m->nseg);
+   BALLOC(V.vstart, nvec);
+   BALLOC(V.vpend, nvec);
+   BALLOC(V.vlogn, nvec);
+   BALLOC(b->ovptr, (size_t)nvec + 1);
+   BALLOC(b->base2, (size_t)(m->ncols > 0 ? m->ncols : 1));
+   BALLOC(b->slotidx, nvec);
+   BALLOC(b->slotidx2, nvec);
+   BALLOC(b->outoff, (size_t)nvec + 1);
    if( !V.dense )
    {
       for( int p = 0; p < 2; ++p )
@@ -2491,6 +2882,13 @@ int sap_bounds_create_mode(sap_matrix* m, int32_t nvec, int32_t mode, sap_bounds
    BALLOC(b->dstatus, 1);
    BALLOC(b->stage, 2 * (size_t)m->ncols);
 #undef BALLOC
+   for( int e = 0; e < 2; ++e )
+      if( hipEventCreate(&b->evs[e]) != hipSuccess )
+      {
+         boundsRelease(b);
+         free(b);
+         return SAP_ERROR;
+      }
    V.st = b->dstatus;
    hipStream_t s = m->ctx->stream;
    k_fill_i64<<<gridFor(NC), SAP_BLOCK, 0, s>>>(V.clb, KEY_NEG_INF, NC);
@@ -2502,6 +2900,9 @@ int sap_bounds_create_mode(sap_matrix* m, int32_t nvec, int32_t mode, sap_bounds
    }
    k_fill_i32<<<gridFor(NC), SAP_BLOCK, 0, s>>>(V.coltag, 0, NC);
    k_fill_i32<<<gridFor(NR), SAP_BLOCK, 0, s>>>(V.rowmark, 0, NR);
+   k_fill_i32<<<gridFor((size_t)nvec), SAP_BLOCK, 0, s>>>(V.vpend, 0, (size_t)nvec);
+   k_fill_i32<<<gridFor((size_t)nvec), SAP_BLOCK, 0, s>>>(V.vstart, 0, (size_t)nvec);
+   k_fill_i32<<<gridFor((size_t)nvec), SAP_BLOCK, 0, s>>>(V.vlogn, 0, (size_t)nvec);
    HIPCHK(hipGetLastError());
    HIPCHK(hipMemsetAsync(b->dstatus, 0, sizeof(Status), s));
    HIPCHK(hipStreamSynchronize(s));
@@ -2518,6 +2919,17 @@ int sap_bounds_free(sap_bounds* b)
    boundsRelease(b);
    free(b);
    return SAP_OKAY;
+}
+
+// vector v (all for v < 0) of b was written outside the change log
+static void touchedVectors(sap_bounds* b, int32_t v)
+{
+   b->gen++;
+   if( b->srcb == NULL )
+      return;
+   const int32_t v0 = v < 0 ? 0 : v, v1 = v < 0 ? b->nvec : v + 1;
+   for( int32_t q = v0; q < v1; ++q )
+      (*b->srcb)[(size_t)q] = NULL;
 }
 
 int sap_bounds_upload(sap_bounds* b, int32_t v, const double* lb, const double* ub)
@@ -2538,6 +2950,7 @@ int sap_bounds_upload(sap_bounds* b, int32_t v, const double* lb, const double* 
       k_interleave<<<gridFor((size_t)nc), SAP_BLOCK, 0, s>>>(b->V.bnd + (size_t)q * (size_t)nc, b->stage,
          b->stage + nc, nc);
    HIPCHK(hipGetLastError());
+   touchedVectors(b, v);
    HIPCHK(hipStreamSynchronize(s));
    return SAP_OKAY;
 }
@@ -2568,6 +2981,7 @@ int sap_bounds_copy(sap_bounds* dst, int32_t dv, const sap_bounds* src, int32_t 
    const size_t nc = (size_t)m->ncols;
    HIPCHK(hipMemcpyAsync(dst->V.bnd + (size_t)dv * nc, src->V.bnd + (size_t)sv * nc, sizeof(double2) * nc,
       hipMemcpyDeviceToDevice, m->ctx->stream));
+   touchedVectors(dst, dv);
    return SAP_OKAY;
 }
 
@@ -2725,6 +3139,7 @@ static bool ensureGraphs(sap_bounds* b, const Params& P)
    k.nvec = b->V.nvec;
    k.single = b->V.single;
    k.dense = b->V.dense;
+   k.logchg = b->V.logchg;
    if( b->graph[0].valid && b->graph[1].valid && memcmp(&k, &b->key, sizeof(k)) == 0 )
       return true;
    for( int g = 0; g < 2; ++g )
@@ -2832,15 +3247,18 @@ int sap_propagate_resident(sap_bounds* b, const sap_params* params, sap_stats* s
    sap_context* c = m->ctx;
    HIPCHK(hipSetDevice(c->device));
    const Params P = toParams(params);
+   if( b->slotsmode )
+      return SAP_INVALIDCALL;   // the vectors serve as slots of continuous batching
    Vec saved = b->V;
    b->V.nvec = 1;
-   b->V.single = 1;
+   b->V.single = (m->nseg == 1) ? 1 : 0;   // packed instances: a cutoff stops only its instance
    int rc = beginCall(b, 1, b->V.dense);
    if( rc == SAP_OKAY && !b->V.dense )
       rc = queueAllRows(b, 1);
    if( rc == SAP_OKAY )
       rc = runRounds(b, P, stats);
    b->V = saved;
+   touchedVectors(b, 0);
    RCCHK(rc);
    if( stats )
       stats->result = c->hstatus->cutoff ? SAP_CUTOFF : (c->hstatus->nchg > 0 ? SAP_REDUCEDDOM : SAP_DIDNOTFIND);
@@ -2865,11 +3283,330 @@ int sap_propagate(sap_matrix* m, const sap_params* params, double* lb, double* u
    return SAP_OKAY;
 }
 
+}   // extern "C"
+
+// ---- persistent staging, change log and slot bookkeeping ----
+
+template <typename T>
+static int growDev(T** p, int64_t have, int64_t need)
+{
+   if( need <= have && *p != NULL )
+      return SAP_OKAY;
+   (void)hipFree(*p);
+   *p = NULL;
+   return devAlloc(p, (size_t)need, NULL);
+}
+
+static int ensureOverlay(sap_bounds* b, int64_t nov)
+{
+   if( nov <= b->ovcap && b->ovcol != NULL )
+      return SAP_OKAY;
+   int64_t cap = nov > 2 * b->ovcap ? nov : 2 * b->ovcap;
+   if( cap < 1024 )
+      cap = 1024;
+   RCCHK(growDev(&b->ovcol, 0, cap));
+   RCCHK(growDev(&b->ovlb, 0, cap));
+   RCCHK(growDev(&b->ovub, 0, cap));
+   b->ovcap = cap;
+   return SAP_OKAY;
+}
+
+static int ensureOut(sap_bounds* b, int64_t n)
+{
+   if( n <= b->outcap && b->outcol != NULL )
+      return SAP_OKAY;
+   int64_t cap = n > 2 * b->outcap ? n : 2 * b->outcap;
+   if( cap < 4096 )
+      cap = 4096;
+   RCCHK(growDev(&b->outcol, 0, cap));
+   RCCHK(growDev(&b->outbnd, 0, cap));
+   b->outcap = cap;
+   return SAP_OKAY;
+}
+
+// the change log (nvec*ncols entries, one bit per column and vector) is allocated on first use
+static int ensureLog(sap_bounds* b)
+{
+   if( b->logready )
+      return SAP_OKAY;
+   sap_matrix* m = b->mat;
+   const size_t NC = (size_t)b->nvec * (size_t)m->ncols;
+   RCCHK(devAlloc(&b->V.log, NC, &b->device_bytes));
+   RCCHK(devAlloc(&b->V.logbits, NC / 32 + 1, &b->device_bytes));
+   RCCHK(devAlloc(&b->V.ovl, NC, &b->device_bytes));
+   RCCHK(devAlloc(&b->V.ovln, (size_t)b->nvec, &b->device_bytes));
+   k_fill_i32<<<gridFor(NC / 32 + 1), SAP_BLOCK, 0, m->ctx->stream>>>((int32_t*)b->V.logbits, 0, NC / 32 + 1);
+   k_fill_i32<<<gridFor((size_t)b->nvec), SAP_BLOCK, 0, m->ctx->stream>>>(b->V.ovln, 0, (size_t)b->nvec);
+   HIPCHK(hipGetLastError());
+   b->srcb = new std::vector<const sap_bounds*>((size_t)b->nvec, (const sap_bounds*)NULL);
+   b->srcv = new std::vector<int32_t>((size_t)b->nvec, 0);
+   b->srcgen = new std::vector<int64_t>((size_t)b->nvec, 0);
+   b->logready = 1;
+   return SAP_OKAY;
+}
+
+// validate overlays and copy them to the persistent device staging
+static int stageOverlay(sap_bounds* b, int32_t n, const int64_t* ovptr, const int32_t* ovcol, const double* ovlb,
+   const double* ovub, int64_t* novout)
+{
+   const int64_t nc = b->mat->ncols;
+   const int64_t nov = ovptr ? ovptr[n] : 0;
+   if( ovptr != NULL && (ovptr[0] != 0 || nov < 0 || (nov > 0 && (ovcol == NULL || ovlb == NULL || ovub == NULL))) )
+      return SAP_INVALIDDATA;
+   for( int32_t v = 0; ovptr && v < n; ++v )
+      if( ovptr[v + 1] < ovptr[v] || ovptr[v + 1] - ovptr[v] > nc )
+         return SAP_INVALIDDATA;
+   for( int64_t q = 0; q < nov; ++q )
+      if( ovcol[q] < 0 || ovcol[q] >= nc || ovlb[q] != ovlb[q] || ovub[q] != ovub[q] )
+         return SAP_INVALIDDATA;
+   RCCHK(ensureOverlay(b, nov));
+   hipStream_t s = b->mat->ctx->stream;
+   if( ovptr != NULL )
+      HIPCHK(hipMemcpyAsync(b->ovptr, ovptr, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyHostToDevice, s));
+   else
+   {
+      k_fill_i64<<<gridFor((size_t)n + 1), SAP_BLOCK, 0, s>>>((long long*)b->ovptr, 0, (size_t)n + 1);
+      HIPCHK(hipGetLastError());
+   }
+   if( nov > 0 )
+   {
+      HIPCHK(hipMemcpyAsync(b->ovcol, ovcol, sizeof(int32_t) * (size_t)nov, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(b->ovlb, ovlb, sizeof(double) * (size_t)nov, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(b->ovub, ovub, sizeof(double) * (size_t)nov, hipMemcpyHostToDevice, s));
+   }
+   *novout = nov;
+   return SAP_OKAY;
+}
+
+static dim3 slotGrid(size_t perslot, int32_t n)
+{
+   return dim3(gridFor(perslot, 64), (unsigned)std::min<int32_t>(n > 0 ? n : 1, 65535));
+}
+
+// bounds of slots hslots[0..n) from a base vector: a slot last started from the same, unchanged
+// base vector (B, bv) gets back only the columns of its change log and overlay; any other
+// slot gets a full copy (B == NULL: dbase is staged host data, always copied)
+static int initFromBase(sap_bounds* b, int32_t n, const int32_t* hslots, const sap_bounds* B, int32_t bv,
+   const double2* dbase)
+{
+   sap_matrix* m = b->mat;
+   hipStream_t s = m->ctx->stream;
+   std::vector<int32_t> full, rest;
+   for( int32_t i = 0; i < n; ++i )
+   {
+      const size_t v = (size_t)hslots[i];
+      const bool same = B != NULL && B != b && (*b->srcb)[v] == B && (*b->srcv)[v] == bv && (*b->srcgen)[v] == B->gen;
+      (same ? rest : full).push_back(hslots[i]);
+      (*b->srcb)[v] = B;
+      (*b->srcv)[v] = bv;
+      (*b->srcgen)[v] = B != NULL ? B->gen : 0;
+   }
+   if( !rest.empty() )
+   {
+      HIPCHK(hipMemcpyAsync(b->slotidx2, rest.data(), sizeof(int32_t) * rest.size(), hipMemcpyHostToDevice, s));
+      k_slot_restore<<<slotGrid(1024, (int32_t)rest.size()), SAP_BLOCK, 0, s>>>(b->V, dbase, m->ncols, b->slotidx2,
+         (int32_t)rest.size());
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(s));
+   }
+   if( !full.empty() )
+   {
+      HIPCHK(hipMemcpyAsync(b->slotidx2, full.data(), sizeof(int32_t) * full.size(), hipMemcpyHostToDevice, s));
+      k_slot_base<<<slotGrid((size_t)m->ncols, (int32_t)full.size()), SAP_BLOCK, 0, s>>>(b->V.bnd, dbase, m->ncols,
+         b->slotidx2, (int32_t)full.size());
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(s));
+   }
+   return SAP_OKAY;
+}
+
+// set up slots hslots[0..n) (device copy dslots) from the base vector and the staged overlays
+static int admitSlots(sap_bounds* b, int32_t n, const int32_t* hslots, const int32_t* dslots, const sap_bounds* B,
+   int32_t bv, const double2* dbase, int64_t nov, int32_t allrows)
+{
+   sap_matrix* m = b->mat;
+   hipStream_t s = m->ctx->stream;
+   const Mat M = toMat(m);
+   RCCHK(initFromBase(b, n, hslots, B, bv, dbase));
+   k_slot_reset<<<slotGrid((size_t)m->nrows, n), SAP_BLOCK, 0, s>>>(b->V, m->nrows, dslots, n);
+   if( nov > 0 )
+      k_slot_overlay<<<slotGrid((size_t)(nov / n + 1), n), SAP_BLOCK, 0, s>>>(b->V, m->ncols, dslots, n, b->ovptr,
+         b->ovcol, b->ovlb, b->ovub);
+   if( allrows || nov > 0 )
+      k_slot_mark<<<slotGrid(allrows ? (size_t)m->nrows : (size_t)(nov / n + 1), n), SAP_BLOCK, 0, s>>>(M, b->V,
+         dslots, n, b->ovptr, b->ovcol, allrows);
+   HIPCHK(hipGetLastError());
+   return SAP_OKAY;
+}
+
+// per-slot verdicts and sorted change lists of slots hslots[0..n); clears their change logs.
+// With chgcol == NULL only verdicts (and chgptr counts, when given) are produced.
+static int collectSlots(sap_bounds* b, int32_t n, const int32_t* hslots, int32_t* result, int32_t* rounds,
+   int64_t chgcap, int64_t* chgptr, int32_t* chgcol, double* chglb, double* chgub)
+{
+   sap_matrix* m = b->mat;
+   hipStream_t s = m->ctx->stream;
+   const size_t nv = (size_t)b->nvec;
+   std::vector<int32_t> logn(nv), cut(nv), vr(nv), vs(nv);
+   HIPCHK(hipMemcpyAsync(logn.data(), b->V.vlogn, sizeof(int32_t) * nv, hipMemcpyDeviceToHost, s));
+   HIPCHK(hipMemcpyAsync(cut.data(), b->V.vcutoff, sizeof(int32_t) * nv, hipMemcpyDeviceToHost, s));
+   HIPCHK(hipMemcpyAsync(vr.data(), b->V.vrounds, sizeof(int32_t) * nv, hipMemcpyDeviceToHost, s));
+   HIPCHK(hipMemcpyAsync(vs.data(), b->V.vstart, sizeof(int32_t) * nv, hipMemcpyDeviceToHost, s));
+   HIPCHK(hipStreamSynchronize(s));
+   // output offsets: a cut-off slot has an empty list
+   std::vector<int64_t> off((size_t)n + 1, 0);
+   int32_t maxlog = 0;
+   for( int32_t i = 0; i < n; ++i )
+   {
+      const size_t v = (size_t)hslots[i];
+      off[(size_t)i + 1] = off[(size_t)i] + (cut[v] ? 0 : logn[v]);
+      maxlog = std::max(maxlog, logn[v]);
+      if( result )
+         result[i] = cut[v] ? SAP_CUTOFF : (logn[v] > 0 ? SAP_REDUCEDDOM : SAP_DIDNOTFIND);
+      if( rounds )
+         rounds[i] = vr[v] > 0 ? vr[v] - vs[v] + 1 : 0;
+   }
+   const int64_t tot = off[(size_t)n];
+   if( chgptr )
+      memcpy(chgptr, off.data(), sizeof(int64_t) * ((size_t)n + 1));
+   const bool want = chgcol != NULL;
+   if( want && tot > chgcap )
+      return SAP_NOMEMORY;   // nothing cleared: the caller may collect again with room
+   if( maxlog == 0 )
+      return SAP_OKAY;
+   // slots whose logs are only cleared, and slots whose changes are gathered
+   std::vector<int32_t> clr, keep;
+   std::vector<int64_t> koff;
+   for( int32_t i = 0; i < n; ++i )
+   {
+      const int32_t v = hslots[i];
+      if( logn[(size_t)v] == 0 )
+         continue;
+      if( want && !cut[(size_t)v] )
+      {
+         keep.push_back(v);
+         koff.push_back(off[(size_t)i]);
+      }
+      else
+         clr.push_back(v);
+   }
+   if( !clr.empty() )
+   {
+      HIPCHK(hipMemcpyAsync(b->slotidx, clr.data(), sizeof(int32_t) * clr.size(), hipMemcpyHostToDevice, s));
+      k_collect<<<slotGrid((size_t)maxlog, (int32_t)clr.size()), SAP_BLOCK, 0, s>>>(b->V, m->ncols, b->slotidx,
+         (int32_t)clr.size(), b->outoff, NULL, NULL);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(s));
+   }
+   if( keep.empty() )
+      return SAP_OKAY;
+   RCCHK(ensureOut(b, tot));
+   HIPCHK(hipMemcpyAsync(b->slotidx, keep.data(), sizeof(int32_t) * keep.size(), hipMemcpyHostToDevice, s));
+   HIPCHK(hipMemcpyAsync(b->outoff, koff.data(), sizeof(int64_t) * koff.size(), hipMemcpyHostToDevice, s));
+   k_collect<<<slotGrid((size_t)maxlog, (int32_t)keep.size()), SAP_BLOCK, 0, s>>>(b->V, m->ncols, b->slotidx,
+      (int32_t)keep.size(), b->outoff, b->outcol, b->outbnd);
+   HIPCHK(hipGetLastError());
+   std::vector<int32_t> hc((size_t)tot);
+   std::vector<double2> hb((size_t)tot);
+   HIPCHK(hipMemcpyAsync(hc.data(), b->outcol, sizeof(int32_t) * (size_t)tot, hipMemcpyDeviceToHost, s));
+   HIPCHK(hipMemcpyAsync(hb.data(), b->outbnd, sizeof(double2) * (size_t)tot, hipMemcpyDeviceToHost, s));
+   HIPCHK(hipStreamSynchronize(s));
+   // the log order depends on atomics: each slot's list is sorted by column
+   std::vector<int64_t> idx;
+   for( int32_t i = 0; i < n; ++i )
+   {
+      const int64_t a = off[(size_t)i], e = off[(size_t)i + 1];
+      idx.resize((size_t)(e - a));
+      for( int64_t q = a; q < e; ++q )
+         idx[(size_t)(q - a)] = q;
+      std::sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) { return hc[(size_t)x] < hc[(size_t)y]; });
+      for( int64_t q = a; q < e; ++q )
+      {
+         const int64_t src = idx[(size_t)(q - a)];
+         chgcol[q] = hc[(size_t)src];
+         if( chglb )
+            chglb[q] = hb[(size_t)src].x;
+         if( chgub )
+            chgub[q] = hb[(size_t)src].y;
+      }
+   }
+   return SAP_OKAY;
+}
+
+// clear the change logs of all vectors (before a one-shot call reuses them)
+static int clearLogs(sap_bounds* b)
+{
+   sap_matrix* m = b->mat;
+   hipStream_t s = m->ctx->stream;
+   std::vector<int32_t> all((size_t)b->nvec), logn((size_t)b->nvec);
+   HIPCHK(hipMemcpyAsync(logn.data(), b->V.vlogn, sizeof(int32_t) * (size_t)b->nvec, hipMemcpyDeviceToHost, s));
+   HIPCHK(hipStreamSynchronize(s));
+   int32_t maxlog = 0;
+   for( int32_t v = 0; v < b->nvec; ++v )
+   {
+      all[(size_t)v] = v;
+      maxlog = std::max(maxlog, logn[(size_t)v]);
+   }
+   if( maxlog == 0 )
+      return SAP_OKAY;
+   HIPCHK(hipMemcpyAsync(b->slotidx, all.data(), sizeof(int32_t) * (size_t)b->nvec, hipMemcpyHostToDevice, s));
+   k_collect<<<slotGrid((size_t)maxlog, b->nvec), SAP_BLOCK, 0, s>>>(b->V, m->ncols, b->slotidx, b->nvec, b->outoff,
+      NULL, NULL);
+   HIPCHK(hipGetLastError());
+   HIPCHK(hipStreamSynchronize(s));
+   return SAP_OKAY;
+}
+
+// one-shot propagation of vectors 0..nvec-1 from a device base vector plus the staged overlays
+static int batchCall(sap_bounds* b, const sap_params* params, int32_t nvec, const sap_bounds* B, int32_t bv,
+   const double2* dbase, int64_t nov, sap_stats* stats)
+{
+   sap_matrix* m = b->mat;
+   hipStream_t s = m->ctx->stream;
+   if( b->slotsmode )
+      return SAP_INVALIDCALL;   // the vectors serve as slots of continuous batching
+   const Params P = toParams(params);
+   const bool fromfix = (params->flags & SAP_FLAG_BASE_IS_FIXPOINT) != 0;
+   RCCHK(ensureLog(b));
+   RCCHK(clearLogs(b));
+   std::vector<int32_t> ident((size_t)nvec);
+   for( int32_t v = 0; v < nvec; ++v )
+      ident[(size_t)v] = v;
+   RCCHK(initFromBase(b, nvec, ident.data(), B, bv, dbase));
+   b->gen++;
+   HIPCHK(hipMemcpyAsync(b->slotidx, ident.data(), sizeof(int32_t) * (size_t)nvec, hipMemcpyHostToDevice, s));
+   // the call works on the first nvec vectors; the single-vector cutoff shortcut only
+   // applies when the batch is one vector
+   Vec saved = b->V;
+   b->V.nvec = nvec;
+   b->V.single = (nvec == 1 && m->nseg == 1) ? 1 : 0;
+   b->V.logchg = 1;
+   int rc = beginCall(b, nvec, (!fromfix && b->V.dense) ? 1 : 0);
+   if( rc == SAP_OKAY )
+   {
+      if( nov > 0 )
+         k_slot_overlay<<<slotGrid((size_t)(nov / nvec + 1), nvec), SAP_BLOCK, 0, s>>>(b->V, m->ncols, b->slotidx,
+            nvec, b->ovptr, b->ovcol, b->ovlb, b->ovub);
+      if( fromfix && nov > 0 )
+         k_slot_mark<<<slotGrid((size_t)(nov / nvec + 1), nvec), SAP_BLOCK, 0, s>>>(toMat(m), b->V, b->slotidx, nvec,
+            b->ovptr, b->ovcol, 0);
+      rc = hipToRc(hipGetLastError());
+   }
+   if( rc == SAP_OKAY && !fromfix && !b->V.dense )
+      rc = queueAllRows(b, nvec);
+   if( rc == SAP_OKAY )
+      rc = runRounds(b, P, stats);
+   b->V = saved;
+   return rc;
+}
+
+extern "C" {
+
 int sap_propagate_batch(sap_bounds* b, const sap_params* params, int32_t nvec, const double* base_lb,
    const double* base_ub, const int64_t* ovptr, const int32_t* ovcol, const double* ovlb, const double* ovub,
    int32_t* result, int32_t* rounds, double* out_lb, double* out_ub, sap_stats* stats)
 {
-   if( b == NULL || base_lb == NULL || base_ub == NULL || nvec < 1 || nvec > b->nvec )
+   if( b == NULL || base_lb == NULL || base_ub == NULL || nvec < 1 || nvec > b->nvec || b->mat->nseg != 1 )
       return SAP_INVALIDDATA;
    RCCHK(checkParams(params));
    sap_matrix* m = b->mat;
@@ -2877,113 +3614,302 @@ int sap_propagate_batch(sap_bounds* b, const sap_params* params, int32_t nvec, c
    HIPCHK(hipSetDevice(c->device));
    hipStream_t s = c->stream;
    const int64_t nc = m->ncols;
-   const int64_t nov = ovptr ? ovptr[nvec] : 0;
-   if( ovptr != NULL && (ovptr[0] != 0 || nov < 0 || (nov > 0 && (ovcol == NULL || ovlb == NULL || ovub == NULL))) )
-      return SAP_INVALIDDATA;
-   for( int64_t q = 0; q < nov; ++q )
-      if( ovcol[q] < 0 || ovcol[q] >= nc || ovlb[q] != ovlb[q] || ovub[q] != ovub[q] )
-         return SAP_INVALIDDATA;
-   for( int32_t v = 0; ovptr && v < nvec; ++v )
-      if( ovptr[v + 1] < ovptr[v] )
-         return SAP_INVALIDDATA;
-   // stage the base vector and the overlays
-   int64_t* dptr = NULL;
-   int32_t* dcol = NULL;
-   double *dlb = NULL, *dub = NULL;
-   RCCHK(devAlloc(&dptr, (size_t)nvec + 1, NULL));
-   RCCHK(devAlloc(&dcol, (size_t)(nov > 0 ? nov : 1), NULL));
-   RCCHK(devAlloc(&dlb, (size_t)(nov > 0 ? nov : 1), NULL));
-   RCCHK(devAlloc(&dub, (size_t)(nov > 0 ? nov : 1), NULL));
-   std::vector<int64_t> zptr;
-   if( ovptr == NULL )
-      zptr.assign((size_t)nvec + 1, 0);
-   HIPCHK(hipMemcpyAsync(dptr, ovptr ? ovptr : zptr.data(), sizeof(int64_t) * ((size_t)nvec + 1),
-      hipMemcpyHostToDevice, s));
-   if( nov > 0 )
-   {
-      HIPCHK(hipMemcpyAsync(dcol, ovcol, sizeof(int32_t) * (size_t)nov, hipMemcpyHostToDevice, s));
-      HIPCHK(hipMemcpyAsync(dlb, ovlb, sizeof(double) * (size_t)nov, hipMemcpyHostToDevice, s));
-      HIPCHK(hipMemcpyAsync(dub, ovub, sizeof(double) * (size_t)nov, hipMemcpyHostToDevice, s));
-   }
+   int64_t nov = 0;
+   RCCHK(stageOverlay(b, nvec, ovptr, ovcol, ovlb, ovub, &nov));
    HIPCHK(hipMemcpyAsync(b->stage, base_lb, sizeof(double) * (size_t)nc, hipMemcpyHostToDevice, s));
    HIPCHK(hipMemcpyAsync(b->stage + nc, base_ub, sizeof(double) * (size_t)nc, hipMemcpyHostToDevice, s));
-   // base vector interleaved into the last vector slot's staging area then broadcast
-   double2* base2 = NULL;
-   RCCHK(devAlloc(&base2, (size_t)nc, NULL));
-   k_interleave<<<gridFor((size_t)nc), SAP_BLOCK, 0, s>>>(base2, b->stage, b->stage + nc, nc);
-   k_broadcast_base<<<gridFor((size_t)nvec * (size_t)nc), SAP_BLOCK, 0, s>>>(b->V.bnd, base2, nc, nvec);
-   dim3 og(gridFor(nov > 0 ? (size_t)(nov / nvec + 1) : 1, 64), (unsigned)std::min<int32_t>(nvec, 65535));
-   if( nov > 0 )
-      k_apply_overlay<<<og, SAP_BLOCK, 0, s>>>(b->V.bnd, nc, dptr, dcol, dlb, dub, nvec);
-   const Params P = toParams(params);
-   // the call works on the first nvec vectors; the single-vector cutoff shortcut only
-   // applies when the batch is one vector
-   Vec saved = b->V;
-   b->V.nvec = nvec;
-   b->V.single = (nvec == 1) ? 1 : 0;
-   const bool fromfix = (params->flags & SAP_FLAG_BASE_IS_FIXPOINT) != 0;
-   int rc = beginCall(b, nvec, (!fromfix && b->V.dense) ? 1 : 0);
-   if( rc == SAP_OKAY && fromfix && nov > 0 )
-   {
-      k_init_overlay<<<og, SAP_BLOCK, 0, s>>>(toMat(m), b->V, dptr, dcol, nvec);
-      rc = hipToRc(hipGetLastError());
-   }
-   else if( rc == SAP_OKAY && !fromfix && !b->V.dense )
-      rc = queueAllRows(b, nvec);
-   if( rc == SAP_OKAY )
-      rc = runRounds(b, P, stats);
-   b->V = saved;
-   RCCHK(rc);
-   std::vector<int32_t> cut((size_t)nvec), rnd((size_t)nvec);
-   HIPCHK(hipMemcpyAsync(cut.data(), b->V.vcutoff, sizeof(int32_t) * (size_t)nvec, hipMemcpyDeviceToHost, s));
-   HIPCHK(hipMemcpyAsync(rnd.data(), b->V.vrounds, sizeof(int32_t) * (size_t)nvec, hipMemcpyDeviceToHost, s));
-   HIPCHK(hipStreamSynchronize(s));
-   // per-vector verdicts: changed bounds are detected by comparing with the start vector
-   if( result != NULL || out_lb != NULL )
-   {
-      std::vector<double> tlb((size_t)nc), tub((size_t)nc);
-      std::vector<double> slb((size_t)nc), sub((size_t)nc);
+   k_interleave<<<gridFor((size_t)nc), SAP_BLOCK, 0, s>>>(b->base2, b->stage, b->stage + nc, nc);
+   HIPCHK(hipGetLastError());
+   RCCHK(batchCall(b, params, nvec, NULL, 0, b->base2, nov, stats));
+   std::vector<int32_t> ident((size_t)nvec), res((size_t)nvec);
+   for( int32_t v = 0; v < nvec; ++v )
+      ident[(size_t)v] = v;
+   RCCHK(collectSlots(b, nvec, ident.data(), res.data(), rounds, 0, NULL, NULL, NULL, NULL));
+   if( result )
+      memcpy(result, res.data(), sizeof(int32_t) * (size_t)nvec);
+   if( out_lb != NULL && out_ub != NULL )
       for( int32_t v = 0; v < nvec; ++v )
-      {
-         double* dstl = out_lb ? out_lb + (size_t)v * (size_t)nc : tlb.data();
-         double* dstu = out_ub ? out_ub + (size_t)v * (size_t)nc : tub.data();
-         RCCHK(sap_bounds_download(b, v, dstl, dstu));
-         if( result != NULL )
-         {
-            if( cut[(size_t)v] )
-               result[v] = SAP_CUTOFF;
-            else
-            {
-               memcpy(slb.data(), base_lb, sizeof(double) * (size_t)nc);
-               memcpy(sub.data(), base_ub, sizeof(double) * (size_t)nc);
-               if( ovptr )
-                  for( int64_t q = ovptr[v]; q < ovptr[v + 1]; ++q )
-                  {
-                     slb[(size_t)ovcol[q]] = ovlb[q];
-                     sub[(size_t)ovcol[q]] = ovub[q];
-                  }
-               bool changed = false;
-               for( int64_t j = 0; j < nc && !changed; ++j )
-                  changed = (dstl[j] != slb[(size_t)j]) || (dstu[j] != sub[(size_t)j]);
-               result[v] = changed ? SAP_REDUCEDDOM : SAP_DIDNOTFIND;
-            }
-         }
-      }
-   }
-   if( rounds != NULL )
-      memcpy(rounds, rnd.data(), sizeof(int32_t) * (size_t)nvec);
+         RCCHK(sap_bounds_download(b, v, out_lb + (size_t)v * (size_t)nc, out_ub + (size_t)v * (size_t)nc));
    if( stats )
    {
       int32_t anycut = 0;
       for( int32_t v = 0; v < nvec; ++v )
-         anycut |= cut[(size_t)v];
+         anycut |= res[(size_t)v] == SAP_CUTOFF;
       stats->result = anycut ? SAP_CUTOFF : (stats->nchgbds > 0 ? SAP_REDUCEDDOM : SAP_DIDNOTFIND);
    }
-   (void)hipFree(dptr);
-   (void)hipFree(dcol);
-   (void)hipFree(dlb);
-   (void)hipFree(dub);
-   (void)hipFree(base2);
+   return SAP_OKAY;
+}
+
+int sap_propagate_nodes(sap_bounds* b, const sap_params* params, int32_t nvec, const sap_bounds* base,
+   int32_t basevec, const int64_t* ovptr, const int32_t* ovcol, const double* ovlb, const double* ovub,
+   int32_t* result, int32_t* rounds, int64_t chgcap, int64_t* chgptr, int32_t* chgcol, double* chglb,
+   double* chgub, sap_stats* stats)
+{
+   if( b == NULL || base == NULL || base->mat != b->mat || basevec < 0 || basevec >= base->nvec || nvec < 1
+      || nvec > b->nvec || b->mat->nseg != 1 || (chgcol != NULL && chgptr == NULL) || chgcap < 0 )
+      return SAP_INVALIDDATA;
+   RCCHK(checkParams(params));
+   sap_matrix* m = b->mat;
+   HIPCHK(hipSetDevice(m->ctx->device));
+   int64_t nov = 0;
+   RCCHK(stageOverlay(b, nvec, ovptr, ovcol, ovlb, ovub, &nov));
+   const double2* dbase = base->V.bnd + (size_t)basevec * (size_t)m->ncols;
+   if( base == b && basevec < nvec )
+   {
+      // the base is one of the vectors being overwritten: keep a copy
+      HIPCHK(hipMemcpyAsync(b->base2, dbase, sizeof(double2) * (size_t)m->ncols, hipMemcpyDeviceToDevice,
+         m->ctx->stream));
+      dbase = b->base2;
+   }
+   RCCHK(batchCall(b, params, nvec, base == b ? NULL : base, basevec, dbase, nov, stats));
+   std::vector<int32_t> ident((size_t)nvec), res((size_t)nvec);
+   for( int32_t v = 0; v < nvec; ++v )
+      ident[(size_t)v] = v;
+   RCCHK(collectSlots(b, nvec, ident.data(), res.data(), rounds, chgcap, chgptr, chgcol, chglb, chgub));
+   if( result )
+      memcpy(result, res.data(), sizeof(int32_t) * (size_t)nvec);
+   if( stats )
+   {
+      int32_t anycut = 0;
+      for( int32_t v = 0; v < nvec; ++v )
+         anycut |= res[(size_t)v] == SAP_CUTOFF;
+      stats->result = anycut ? SAP_CUTOFF : (stats->nchgbds > 0 ? SAP_REDUCEDDOM : SAP_DIDNOTFIND);
+   }
+   return SAP_OKAY;
+}
+
+int sap_propagate_packed(sap_matrix* m, const sap_params* params, double* lb, double* ub, int32_t* result,
+   int32_t* rounds, sap_stats* stats)
+{
+   if( m == NULL || lb == NULL || ub == NULL )
+      return SAP_INVALIDDATA;
+   RCCHK(checkParams(params));
+   HIPCHK(hipSetDevice(m->ctx->device));
+   if( m->single == NULL )
+      RCCHK(sap_bounds_create(m, 1, &m->single));
+   sap_bounds* b = m->single;
+   hipStream_t s = m->ctx->stream;
+   const int32_t ns = m->nseg;
+   std::vector<double> in_lb(lb, lb + m->ncols), in_ub(ub, ub + m->ncols);
+   RCCHK(sap_bounds_upload(b, 0, lb, ub));
+   sap_stats local;
+   memset(&local, 0, sizeof(local));
+   RCCHK(sap_propagate_resident(b, params, &local));
+   std::vector<int32_t> cut((size_t)ns), vr((size_t)ns);
+   int32_t vs = 0;
+   HIPCHK(hipMemcpyAsync(cut.data(), b->V.vcutoff, sizeof(int32_t) * (size_t)ns, hipMemcpyDeviceToHost, s));
+   HIPCHK(hipMemcpyAsync(vr.data(), b->V.vrounds, sizeof(int32_t) * (size_t)ns, hipMemcpyDeviceToHost, s));
+   HIPCHK(hipMemcpyAsync(&vs, b->V.vstart, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+   HIPCHK(hipStreamSynchronize(s));
+   RCCHK(sap_bounds_download(b, 0, lb, ub));
+   int32_t anycut = 0;
+   for( int32_t q = 0; q < ns; ++q )
+   {
+      const int64_t c0 = ns > 1 ? (*m->coloff)[(size_t)q] : 0, c1 = ns > 1 ? (*m->coloff)[(size_t)q + 1] : m->ncols;
+      bool changed = false;
+      for( int64_t j = c0; j < c1 && !changed; ++j )
+         changed = lb[j] != in_lb[(size_t)j] || ub[j] != in_ub[(size_t)j];
+      const int32_t r = cut[(size_t)q] ? SAP_CUTOFF : (changed ? SAP_REDUCEDDOM : SAP_DIDNOTFIND);
+      anycut |= cut[(size_t)q];
+      if( result )
+         result[q] = r;
+      if( rounds )
+         rounds[q] = ns > 1 ? (vr[(size_t)q] > 0 ? vr[(size_t)q] - vs + 1 : 0) : local.rounds;
+   }
+   if( ns == 1 && result )
+      result[0] = local.result;
+   if( stats )
+   {
+      *stats = local;
+      if( ns > 1 )
+         stats->result = anycut ? SAP_CUTOFF : (local.nchgbds > 0 ? SAP_REDUCEDDOM : SAP_DIDNOTFIND);
+   }
+   return SAP_OKAY;
+}
+
+// ---- continuous batching ----
+
+static int slotsInit(sap_bounds* b)
+{
+   if( b->slotsmode )
+      return SAP_OKAY;
+   if( b->mat->nseg != 1 )
+      return SAP_INVALIDDATA;
+   RCCHK(ensureLog(b));
+   RCCHK(clearLogs(b));
+   b->active = new std::vector<int32_t>((size_t)b->nvec, 0);
+   if( b->hstat == NULL )
+   {
+      HIPCHK(hipHostMalloc((void**)&b->hstat, sizeof(Status), hipHostMallocDefault));
+      HIPCHK(hipHostMalloc((void**)&b->hpend, sizeof(int32_t) * 2 * (size_t)b->nvec, hipHostMallocDefault));
+   }
+   b->V.single = 0;
+   b->V.logchg = 1;
+   b->slotsmode = 1;
+   return SAP_OKAY;
+}
+
+int sap_slots_admit(sap_bounds* b, int32_t n, const int32_t* slots, const sap_bounds* base, int32_t basevec,
+   const int64_t* ovptr, const int32_t* ovcol, const double* ovlb, const double* ovub, int32_t flags)
+{
+   if( b == NULL || base == NULL || base->mat != b->mat || basevec < 0 || basevec >= base->nvec || n < 0
+      || n > b->nvec || (n > 0 && slots == NULL) )
+      return SAP_INVALIDDATA;
+   if( base == b )
+      return SAP_INVALIDDATA;   // the base must not be a slot
+   sap_matrix* m = b->mat;
+   HIPCHK(hipSetDevice(m->ctx->device));
+   RCCHK(slotsInit(b));
+   if( n == 0 )
+      return SAP_OKAY;
+   std::vector<char> seen((size_t)b->nvec, 0);
+   for( int32_t i = 0; i < n; ++i )
+   {
+      const int32_t v = slots[i];
+      if( v < 0 || v >= b->nvec || seen[(size_t)v] || (*b->active)[(size_t)v] != 0 )
+         return SAP_INVALIDDATA;
+      seen[(size_t)v] = 1;
+   }
+   int64_t nov = 0;
+   RCCHK(stageOverlay(b, n, ovptr, ovcol, ovlb, ovub, &nov));
+   HIPCHK(hipMemcpyAsync(b->slotidx, slots, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, m->ctx->stream));
+   RCCHK(admitSlots(b, n, slots, b->slotidx, base, basevec, base->V.bnd + (size_t)basevec * (size_t)m->ncols, nov,
+      (flags & SAP_FLAG_BASE_IS_FIXPOINT) ? 0 : 1));
+   b->gen++;
+   for( int32_t i = 0; i < n; ++i )
+      (*b->active)[(size_t)slots[i]] = 1;
+   // the staging buffers are reused by the next call: wait for the admission kernels
+   HIPCHK(hipStreamSynchronize(m->ctx->stream));
+   return SAP_OKAY;
+}
+
+int sap_slots_launch(sap_bounds* b, const sap_params* params, int32_t rounds)
+{
+   if( b == NULL || !b->slotsmode || (rounds != SAP_STEP_SHORT && rounds != SAP_STEP_LONG) )
+      return b != NULL && !b->slotsmode ? SAP_INVALIDCALL : SAP_INVALIDDATA;
+   RCCHK(checkParams(params));
+   sap_matrix* m = b->mat;
+   sap_context* c = m->ctx;
+   HIPCHK(hipSetDevice(c->device));
+   hipStream_t s = c->stream;
+   const Params P = toParams(params);
+   HIPCHK(hipEventRecord(b->evs[0], s));
+   k_step_begin<<<1, SAP_STAT_SLOTS, 0, s>>>(b->V);
+   HIPCHK(hipGetLastError());
+   if( !(P.flags & SAP_FLAG_HOSTLOOP) && !b->graphbroken && ensureGraphs(b, P) )
+      HIPCHK(hipGraphLaunch(b->graph[rounds == SAP_STEP_SHORT ? 0 : 1].exec, s));
+   else
+   {
+      const Mat M = toMat(m);
+      const Layout L = makeLayout(b);
+      for( int r = 0; r < rounds; ++r )
+         enqueueRound(b, M, P, L, NULL, NULL);
+      HIPCHK(hipGetLastError());
+   }
+   HIPCHK(hipMemcpyAsync(b->hstat, b->dstatus, sizeof(Status), hipMemcpyDeviceToHost, s));
+   HIPCHK(hipMemcpyAsync(b->hpend, b->V.vpend, sizeof(int32_t) * (size_t)b->nvec, hipMemcpyDeviceToHost, s));
+   HIPCHK(hipMemcpyAsync(b->hpend + b->nvec, b->V.vcutoff, sizeof(int32_t) * (size_t)b->nvec, hipMemcpyDeviceToHost,
+      s));
+   HIPCHK(hipEventRecord(b->evs[1], s));
+   b->inflight = 1;
+   return SAP_OKAY;
+}
+
+int sap_slots_query(sap_bounds* b, int32_t* ready)
+{
+   if( b == NULL || ready == NULL )
+      return SAP_INVALIDDATA;
+   if( !b->inflight )
+   {
+      *ready = 1;
+      return SAP_OKAY;
+   }
+   const hipError_t e = hipEventQuery(b->evs[1]);
+   if( e == hipErrorNotReady )
+   {
+      *ready = 0;
+      return SAP_OKAY;
+   }
+   HIPCHK(e);
+   *ready = 1;
+   return SAP_OKAY;
+}
+
+int sap_slots_wait(sap_bounds* b, int32_t* nfinished, int32_t* finished, sap_stats* stats)
+{
+   if( b == NULL || !b->slotsmode || nfinished == NULL )
+      return SAP_INVALIDDATA;
+   *nfinished = 0;
+   if( !b->inflight )
+      return SAP_INVALIDCALL;
+   HIPCHK(hipSetDevice(b->mat->ctx->device));
+   HIPCHK(hipEventSynchronize(b->evs[1]));
+   b->inflight = 0;
+   const Status* h = b->hstat;
+   int32_t nf = 0;
+   for( int32_t v = 0; v < b->nvec; ++v )
+   {
+      if( (*b->active)[(size_t)v] != 1 )
+         continue;
+      const bool cut = b->hpend[b->nvec + v] != 0;
+      const bool pending = b->hpend[v] > h->round;
+      if( cut || !pending )
+      {
+         (*b->active)[(size_t)v] = 2;
+         if( finished )
+            finished[nf] = v;
+         ++nf;
+      }
+   }
+   *nfinished = nf;
+   if( stats )
+   {
+      unsigned long long a = 0, r = 0, g = 0;
+      for( int q = 0; q < SAP_STAT_SLOTS; ++q )
+      {
+         a += h->slot_nnz[q];
+         r += h->slot_rows[q];
+         g += h->slot_chg[q];
+      }
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, b->evs[0], b->evs[1]));
+      memset(stats, 0, sizeof(*stats));
+      stats->result = SAP_DIDNOTFIND;
+      stats->rounds = h->rlocal;
+      stats->nchgbds = (int64_t)g;
+      stats->rows_swept = (int64_t)r;
+      stats->nnz_swept = (int64_t)a;
+      stats->device_ms = ms;
+      stats->sweep_ms = (double)h->sweep_ticks / (double)b->mat->ctx->wallclock_khz;
+      stats->sweep_launches = (int64_t)h->sweep_count;
+   }
+   return SAP_OKAY;
+}
+
+int sap_slots_step(sap_bounds* b, const sap_params* params, int32_t rounds, int32_t* nfinished, int32_t* finished,
+   sap_stats* stats)
+{
+   RCCHK(sap_slots_launch(b, params, rounds));
+   return sap_slots_wait(b, nfinished, finished, stats);
+}
+
+int sap_slots_collect(sap_bounds* b, int32_t n, const int32_t* slots, int32_t* result, int32_t* rounds,
+   int64_t chgcap, int64_t* chgptr, int32_t* chgcol, double* chglb, double* chgub)
+{
+   if( b == NULL || !b->slotsmode || n < 0 || (n > 0 && slots == NULL) || (chgcol != NULL && chgptr == NULL)
+      || chgcap < 0 )
+      return SAP_INVALIDDATA;
+   if( b->inflight )
+      return SAP_INVALIDCALL;
+   for( int32_t i = 0; i < n; ++i )
+      if( slots[i] < 0 || slots[i] >= b->nvec || (*b->active)[(size_t)slots[i]] != 2 )
+         return SAP_INVALIDDATA;
+   if( n == 0 )
+      return SAP_OKAY;
+   HIPCHK(hipSetDevice(b->mat->ctx->device));
+   RCCHK(collectSlots(b, n, slots, result, rounds, chgcap, chgptr, chgcol, chglb, chgub));
+   for( int32_t i = 0; i < n; ++i )
+      (*b->active)[(size_t)slots[i]] = 0;
    return SAP_OKAY;
 }
 
