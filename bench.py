@@ -17,6 +17,9 @@ import json
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -100,6 +103,12 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="only run the copy kernel used to scale PMC bytes")
     ap.add_argument("--variant", choices=("both", "literal", "planted"), default="both",
                     help="literal: only the headline instance (for profiling); planted: only the feasible variant")
+    ap.add_argument("--legs", default="nodes,packed,serve",
+                    help="batched legs to run after the headline: nodes, packed, serve (comma list, or none)")
+    ap.add_argument("--nodes", type=int, default=1024, help="node vectors of the batched-nodes leg")
+    ap.add_argument("--packed-instances", type=int, default=512, help="instances of the packed leg")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the served arrival trace")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate (requests/s)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -133,6 +142,16 @@ def main():
         fb = cpu_baseline(budget_s=6.0, planted=True)
         main_out["config"]["feasible_variant"]["cpu_baseline_value"] = fb["value"]
         main_out["config"]["feasible_variant"]["cpu_time_to_fixpoint_ms_100k_sample"] = fb["time_to_fixpoint_ms"]
+    legs = [x for x in args.legs.split(",") if x and x != "none"]
+    parts = None
+    if "nodes" in legs:
+        main_out["batched_nodes"] = run_batched_nodes(sa, ctx, args)
+    if "packed" in legs or "serve" in legs:
+        parts = gen_uneven_instances(sa, args.packed_instances)
+    if "packed" in legs:
+        main_out["packed_instances"] = run_packed(sa, ctx, args, parts)
+    if "serve" in legs:
+        main_out["serving"] = run_serving(sa, args, parts)
     print(json.dumps(main_out), flush=True)
     ctx.close()
 
@@ -223,3 +242,210 @@ def run_workload(sa, ctx, args, planted: bool):
 
 if __name__ == "__main__":
     main()
+
+
+# ------------------------------------------------------------------------------------------
+# batched legs: node vectors on one matrix, packed uneven instances, served requests
+# ------------------------------------------------------------------------------------------
+def _rate(nnz, seconds):
+    return nnz / seconds if seconds > 0 else 0.0
+
+
+def run_batched_nodes(sa, ctx, args):
+    """Covering/packing rows 200,000 x 300,000 (10 nonzeros per row, seed 3) and node vectors
+    fixing 1..50 random binaries to 0/1 (seed 3), propagated from the root fixpoint held on the
+    device.  Batched: all vectors in one call (one launch per round for all of them).  Plain:
+    one node per launch sequence, from the same resident root (C loop of the server's plain
+    mode), so both sweep exactly the same rows."""
+    nn = args.nodes
+    log(f"[bench] batched nodes: 200000x300000 covering/packing, {nn} nodes")
+    inst = sa.gen_covering_packing(200_000, 300_000, 10, seed=3)
+    mat = sa.Matrix(ctx, inst)
+    base = sa.Bounds(mat, 1)
+    base.upload(inst.lb, inst.ub)
+    rst = base.propagate()
+    ptr, col, val = sa.gen_node_fixings(inst, nn, 1, 50, seed=3)
+    p = sa.default_params(flags=sa.SAP_FLAG_BASE_IS_FIXPOINT)
+    nodes = sa.Bounds(mat, nn)
+    for _ in range(max(args.warmup, 1)):
+        out = nodes.propagate_nodes(base, 0, ptr, col, val, val, p)
+    steps = max(args.steps, 1)
+    nnz = rows = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = nodes.propagate_nodes(base, 0, ptr, col, val, val, p)
+        nnz += out["stats"]["nnz_swept"]
+        rows += out["stats"]["rows_swept"]
+    el = time.perf_counter() - t0
+    res = out["result"]
+    batched = {"value": _rate(nnz, el), "unit": "nnz-sweeps/s", "ms_per_batch": 1000 * el / steps,
+               "nodes_per_s": nn * steps / el, "nnz_swept_per_batch": nnz // steps,
+               "rows_swept_per_batch": rows // steps,
+               "cutoff_nodes": int((res == sa.SAP_CUTOFF).sum()),
+               "reduceddom_nodes": int((res == sa.SAP_REDUCEDDOM).sum()),
+               "max_rounds": int(out["rounds"].max()), "changed_bounds": int(out["chgptr"][-1])}
+    nodes.close()
+    # plain: the same nodes one at a time through the server's C loop (PLAIN mode, all
+    # arriving at once) -- admission, launch sequence and collection per node
+    srv = sa.Server(0, 1, 64 << 30, 1)
+    pid = srv.add_problem(inst)
+    srv.warm(pid)
+    arr = np.zeros(nn)
+    prob = np.full(nn, pid, dtype=np.int32)
+    srv.replay(arr[:8], prob[:8], ptr[:9], col[:ptr[8]], val[:ptr[8]], val[:ptr[8]], mode=sa.SAP_SERVE_PLAIN,
+               steprounds=sa.SAP_STEP_LONG)
+    t0 = time.perf_counter()
+    pl = srv.replay(arr, prob, ptr, col, val, val, mode=sa.SAP_SERVE_PLAIN, steprounds=sa.SAP_STEP_LONG)
+    elp = time.perf_counter() - t0
+    same = bool(np.array_equal(pl["result"], res))
+    srv.close()
+    plain = {"value": _rate(pl["stats"]["nnz_swept"], elp), "unit": "nnz-sweeps/s",
+             "ms_per_node": 1000 * elp / nn, "nodes_per_s": nn / elp, "nnz_swept": pl["stats"]["nnz_swept"],
+             "verdicts_equal_batched": same}
+    base.close()
+    mat.close()
+    return {"workload": "covering/packing 200000x300000, 10 nnz/row, seed 3; nodes fix 1..50 binaries, seed 3; "
+                        "from the root fixpoint (resident)",
+            "nodes": nn, "root_rounds": rst["rounds"], "batched": batched, "plain": plain,
+            "speedup_nnz_sweeps": batched["value"] / plain["value"] if plain["value"] > 0 else None}
+
+
+def gen_uneven_instances(sa, count, seed=4):
+    """Instances with rows log-uniform in [500, 500,000], columns 1.5 x rows, 4..64 nonzeros per
+    row and the mixed integer/continuous distributions of the large instance; instance i uses
+    seed 4000+i.  Sides are centred on a planted feasible point (sidemode 1)."""
+    rng = np.random.default_rng(seed)
+    m = np.rint(np.exp(rng.uniform(np.log(500), np.log(500_000), size=count))).astype(np.int64)
+    log(f"[bench] generating {count} uneven instances (rows {m.sum()})")
+    t = time.perf_counter()
+    with ThreadPoolExecutor(16) as ex:
+        parts = list(ex.map(lambda i: sa.gen_mixed(int(m[i]), int(round(1.5 * m[i])), seed=4000 + i, lenmode=1,
+                                                   minlen=4, maxlen=64, planted=True), range(count)))
+    log(f"[bench] generated nnz={sum(p.nnz for p in parts)} in {time.perf_counter() - t:.1f}s")
+    return parts
+
+
+def run_packed(sa, ctx, args, parts):
+    """All instances packed block-diagonally into one resident matrix, propagated to their
+    fixpoints from the root bounds in one launch per round (per-instance verdicts); against one
+    resident matrix per instance propagated one after another."""
+    t = time.perf_counter()
+    pm = sa.PackedMatrix(ctx, parts)
+    log(f"[bench] packed upload {time.perf_counter() - t:.1f}s, device bytes {pm.info()['device_bytes']}")
+    plb, pub, pres, prnd, pst = pm.propagate_packed()
+    init = sa.Bounds(pm, 1)
+    init.upload(pm.inst.lb, pm.inst.ub)
+    work = sa.Bounds(pm, 1)
+    for _ in range(max(args.warmup, 1)):
+        work.copy_from(init)
+        work.propagate()
+    steps = max(args.steps, 1)
+    nnz = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        work.copy_from(init)
+        st = work.propagate()
+        nnz += st["nnz_swept"]
+    el = time.perf_counter() - t0
+    packed = {"value": _rate(nnz, el), "unit": "nnz-sweeps/s", "ms_per_pass": 1000 * el / steps,
+              "nnz_swept_per_pass": nnz // steps, "rounds": st["rounds"], "sweep_launches_per_pass":
+              st["sweep_launches"] // 1, "cutoff_instances": int((pres == sa.SAP_CUTOFF).sum())}
+    work.close()
+    init.close()
+    pm.close()
+    # plain: one matrix per instance, one propagation after another
+    t = time.perf_counter()
+    mats = [sa.Matrix(ctx, p) for p in parts]
+    log(f"[bench] per-instance uploads {time.perf_counter() - t:.1f}s")
+    inits, works = [], []
+    for mt, p in zip(mats, parts):
+        a = sa.Bounds(mt, 1)
+        a.upload(p.lb, p.ub)
+        inits.append(a)
+        works.append(sa.Bounds(mt, 1))
+    for a, w in zip(inits, works):
+        w.copy_from(a)
+        w.propagate()
+    nnzp = 0
+    same = True
+    t0 = time.perf_counter()
+    for k, (a, w) in enumerate(zip(inits, works)):
+        w.copy_from(a)
+        st = w.propagate()
+        nnzp += st["nnz_swept"]
+        same &= (st["result"] == sa.SAP_CUTOFF) == (pres[k] == sa.SAP_CUTOFF)
+    elp = time.perf_counter() - t0
+    for x in inits + works + mats:
+        x.close()
+    plain = {"value": _rate(nnzp, elp), "unit": "nnz-sweeps/s", "ms_all_instances": 1000 * elp,
+             "nnz_swept": nnzp, "verdicts_equal_packed": bool(same)}
+    return {"workload": f"{len(parts)} instances, rows log-uniform [500, 500000] (seed 4), cols 1.5 x rows, "
+                        "4..64 nnz/row, mixed int/cont, instance i seed 4000+i, planted-feasible sides; "
+                        "root propagation to fixpoint",
+            "instances": len(parts), "rows": int(sum(p.nrows for p in parts)),
+            "nnz": int(sum(p.nnz for p in parts)), "packed": packed, "plain": plain,
+            "speedup_nnz_sweeps": packed["value"] / plain["value"] if plain["value"] > 0 else None}
+
+
+def run_serving(sa, args, parts):
+    """Poisson arrivals (rate, seconds from the arguments); half the requests are nodes of the
+    covering/packing 200,000 x 300,000 problem (1..50 binaries fixed), half are nodes of a
+    uniformly chosen uneven instance (1..50 integer columns fixed to 0/1).  All problems are
+    resident (warmed) before the trace; continuous batching against the plain
+    one-request-at-a-time form on the same trace."""
+    cp = sa.gen_covering_packing(200_000, 300_000, 10, seed=3)
+    srv = sa.Server(0, 4, 200 << 30, 16)
+    pids = [srv.add_problem(cp)] + [srv.add_problem(p) for p in parts]
+    t = time.perf_counter()
+    for pid in pids:
+        srv.warm(pid)
+    log(f"[bench] serving: {len(pids)} problems resident in {time.perf_counter() - t:.1f}s")
+    rng = np.random.default_rng(5)
+    n = int(args.serve_rate * args.serve_seconds)
+    arrival = np.cumsum(rng.exponential(1.0 / args.serve_rate, size=n))
+    arrival -= arrival[0]
+    kind = rng.integers(0, 2, size=n)
+    prob = np.where(kind == 0, pids[0], rng.integers(1, len(pids), size=n)).astype(np.int32)
+    probs = [cp] + list(parts)
+    intcols = [np.nonzero(p.integral)[0] for p in probs]
+    ptr = np.zeros(n + 1, dtype=np.int64)
+    cols, vals = [], []
+    for i in range(n):
+        ic = intcols[prob[i]]
+        k = int(min(len(ic), rng.integers(1, 51)))
+        c = rng.choice(ic, size=k, replace=False) if k else np.zeros(0, dtype=np.int64)
+        cols.append(c.astype(np.int32))
+        vals.append(rng.integers(0, 2, size=k).astype(np.float64))
+        ptr[i + 1] = ptr[i] + k
+    col = np.concatenate(cols)
+    val = np.concatenate(vals)
+    log(f"[bench] serving trace: {n} requests over {args.serve_seconds}s")
+    cont = srv.replay(arrival, prob, ptr, col, val, val, mode=sa.SAP_SERVE_CONTINUOUS, steprounds=sa.SAP_STEP_SHORT)
+    cst = cont["stats"]
+    # saturated capacity: the same requests all arriving at once
+    sat = srv.replay(np.zeros(n), prob, ptr, col, val, val, mode=sa.SAP_SERVE_CONTINUOUS,
+                     steprounds=sa.SAP_STEP_LONG)
+    sst = sat["stats"]
+    # plain on a prefix of the same trace (it falls behind the arrivals)
+    npl = min(n, max(200, n // 10))
+    pl = srv.replay(arrival[:npl], prob[:npl], ptr[:npl + 1], col[:ptr[npl]], val[:ptr[npl]], val[:ptr[npl]],
+                    mode=sa.SAP_SERVE_PLAIN, steprounds=sa.SAP_STEP_LONG)
+    pst = pl["stats"]
+    same = bool(np.array_equal(pl["result"], cont["result"][:npl]) and np.array_equal(pl["nchg"], cont["nchg"][:npl]))
+    srv.close()
+    return {"workload": f"Poisson {args.serve_rate:.0f} req/s for {args.serve_seconds:.0f}s; 50% nodes of the "
+                        f"200000x300000 covering/packing problem, 50% nodes of {len(parts)} uneven instances; "
+                        "1..50 fixings per request",
+            "requests": n,
+            "continuous": {"value": _rate(cst["nnz_swept"], cst["wall_s"]), "unit": "nnz-sweeps/s",
+                           "requests_per_s": n / cst["wall_s"], "p50_ms": cst["p50_ms"], "p99_ms": cst["p99_ms"],
+                           "max_ms": cst["max_ms"], "max_running": cst["max_running"], "steps": cst["steps"],
+                           "nnz_swept": cst["nnz_swept"]},
+            "saturated": {"value": _rate(sst["nnz_swept"], sst["wall_s"]), "unit": "nnz-sweeps/s",
+                          "requests_per_s": n / sst["wall_s"], "p50_ms": sst["p50_ms"], "p99_ms": sst["p99_ms"],
+                          "max_running": sst["max_running"]},
+            "plain": {"requests": npl, "value": _rate(pst["nnz_swept"], pst["wall_s"]), "unit": "nnz-sweeps/s",
+                      "requests_per_s": npl / pst["wall_s"], "p50_ms": pst["p50_ms"], "p99_ms": pst["p99_ms"],
+                      "results_equal_continuous": same},
+            "speedup_nnz_sweeps_saturated_vs_plain": (_rate(sst["nnz_swept"], sst["wall_s"])
+                                                      / max(_rate(pst["nnz_swept"], pst["wall_s"]), 1e-9))}

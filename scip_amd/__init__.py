@@ -65,6 +65,13 @@ class sap_params(C.Structure):
     ]
 
 
+class sap_tuning(C.Structure):
+    _fields_ = [("tilebins", C.c_int32), ("longrow", C.c_int32), ("capscale", C.c_int32), ("reserved", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 class sap_stats(C.Structure):
     _fields_ = [
         ("result", C.c_int32),
@@ -97,6 +104,30 @@ class sap_instance(C.Structure):
     ]
 
 
+class sap_serve_stats(C.Structure):
+    _fields_ = [
+        ("wall_s", C.c_double),
+        ("requests", C.c_int64),
+        ("nnz_swept", C.c_int64),
+        ("rows_swept", C.c_int64),
+        ("steps", C.c_int64),
+        ("admissions", C.c_int64),
+        ("uploads", C.c_int64),
+        ("evictions", C.c_int64),
+        ("max_running", C.c_int64),
+        ("peak_bytes", C.c_int64),
+        ("p50_ms", C.c_double),
+        ("p99_ms", C.c_double),
+        ("max_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SAP_SERVE_CONTINUOUS = 0
+SAP_SERVE_PLAIN = 1
+
 _P = C.c_void_p
 _I64P = C.POINTER(C.c_int64)
 _I32P = C.POINTER(C.c_int32)
@@ -113,6 +144,11 @@ EXPORTS = {
     "sap_matrix_create": (C.c_int, [_P, C.c_int64, C.c_int64, _I64P, _I32P, _DP, _DP, _DP, _U8P, _DP, _DP,
                                     C.POINTER(_P)]),
     "sap_matrix_free": (C.c_int, [_P]),
+    "sap_tuning_default": (None, [C.POINTER(sap_tuning)]),
+    "sap_context_set_tuning": (C.c_int, [_P, C.POINTER(sap_tuning)]),
+    "sap_context_get_tuning": (C.c_int, [_P, C.POINTER(sap_tuning)]),
+    "sap_autotune": (C.c_int, [_P, C.c_int64, C.c_int64, _I64P, _I32P, _DP, _DP, _DP, _U8P, _DP, _DP,
+                               C.POINTER(sap_params), C.c_int64, C.c_uint64, C.POINTER(sap_tuning), _DP, _DP]),
     "sap_matrix_info": (C.c_int, [_P, _I64P, _I64P, _I64P, _I64P]),
     "sap_propagate": (C.c_int, [_P, C.POINTER(sap_params), _DP, _DP, C.POINTER(sap_stats)]),
     "sap_bounds_create": (C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
@@ -136,6 +172,14 @@ EXPORTS = {
     "sap_slots_step": (C.c_int, [_P, C.POINTER(sap_params), C.c_int32, _I32P, _I32P, C.POINTER(sap_stats)]),
     "sap_slots_collect": (C.c_int, [_P, C.c_int32, _I32P, _I32P, _I32P, C.c_int64, _I64P, _I32P, _DP, _DP]),
     "sap_row_activities": (C.c_int, [_P, C.POINTER(sap_params), _DP, _DP, _DP, _DP, _I32P, _DP]),
+    "sap_server_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int64, C.c_int32, C.POINTER(_P)]),
+    "sap_server_free": (C.c_int, [_P]),
+    "sap_server_add_problem": (C.c_int, [_P, C.c_int64, C.c_int64, _I64P, _I32P, _DP, _DP, _DP, _U8P, _DP, _DP,
+                                         _I32P]),
+    "sap_server_warm": (C.c_int, [_P, C.c_int32]),
+    "sap_server_replay": (C.c_int, [_P, C.POINTER(sap_params), C.c_int32, C.c_int32, C.c_int32, _DP, _I32P, _I64P,
+                                    _I32P, _DP, _DP, _I32P, _I32P, _I32P, _DP, C.c_int64, _I64P, _I32P, _DP, _DP,
+                                    C.POINTER(sap_serve_stats)]),
     "sap_gen_covering_packing": (C.c_int, [C.c_int64, C.c_int64, C.c_int32, C.c_uint64, C.POINTER(sap_instance)]),
     "sap_gen_mixed": (C.c_int, [C.c_int64, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32,
                                 C.c_int32, C.c_uint64, C.POINTER(sap_instance)]),
@@ -284,6 +328,32 @@ class Context:
     def __init__(self, device: int = 0):
         self.h = _P()
         _chk(lib().sap_context_create(device, C.byref(self.h)), "sap_context_create")
+
+    def set_tuning(self, tilebins: int = 5, longrow: int = 4096, capscale: int = 100):
+        t = sap_tuning(tilebins, longrow, capscale, 0)
+        _chk(lib().sap_context_set_tuning(self.h, C.byref(t)), "sap_context_set_tuning")
+
+    def tuning(self):
+        t = sap_tuning()
+        _chk(lib().sap_context_get_tuning(self.h, C.byref(t)), "sap_context_get_tuning")
+        return t.as_dict()
+
+    def autotune(self, inst: "Instance", lb=None, ub=None, params: sap_params | None = None,
+                 samplerows: int = 200_000, seed: int = 1, apply: bool = True):
+        """Measure the candidate tunings on a row sample of inst; returns (best, best_ms, default_ms)."""
+        rp, ci, va, lh, rh, it = [np.ascontiguousarray(a) for a in (inst.rowptr, inst.colidx, inst.vals, inst.lhs,
+                                                                     inst.rhs, inst.integral)]
+        lb = np.ascontiguousarray(inst.lb if lb is None else lb, dtype=np.float64)
+        ub = np.ascontiguousarray(inst.ub if ub is None else ub, dtype=np.float64)
+        best = sap_tuning()
+        bms, dms = C.c_double(0), C.c_double(0)
+        p = params or default_params()
+        _chk(lib().sap_autotune(self.h, inst.nrows, inst.ncols, _i64p(rp), _i32p(ci), _dp(va), _dp(lh), _dp(rh),
+                                _u8p(it), _dp(lb), _dp(ub), C.byref(p), samplerows, seed, C.byref(best),
+                                C.byref(bms), C.byref(dms)), "sap_autotune")
+        if apply:
+            _chk(lib().sap_context_set_tuning(self.h, C.byref(best)), "sap_context_set_tuning")
+        return best.as_dict(), bms.value, dms.value
 
     def close(self):
         if self.h:
@@ -533,6 +603,77 @@ class Bounds:
     def close(self):
         if self.h:
             lib().sap_bounds_free(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------------------------------
+# serving
+# ------------------------------------------------------------------------------------------
+class Server:
+    """Request server (include/scip_amd_serve.h): resident problems, admission by HBM
+    footprint, continuous batching of node requests."""
+
+    def __init__(self, device: int = 0, nstreams: int = 4, hbm_budget: int = 64 << 30, slots: int = 64):
+        self.h = _P()
+        self.slots = slots
+        _chk(lib().sap_server_create(device, nstreams, hbm_budget, slots, C.byref(self.h)), "sap_server_create")
+
+    def add_problem(self, inst: Instance) -> int:
+        rp, ci, va, lh, rh, it, lb, ub = [np.ascontiguousarray(a) for a in (
+            inst.rowptr, inst.colidx, inst.vals, inst.lhs, inst.rhs, inst.integral, inst.lb, inst.ub)]
+        pid = C.c_int32(0)
+        _chk(lib().sap_server_add_problem(self.h, inst.nrows, inst.ncols, _i64p(rp), _i32p(ci), _dp(va), _dp(lh),
+                                          _dp(rh), _u8p(it), _dp(lb), _dp(ub), C.byref(pid)), "sap_server_add_problem")
+        return pid.value
+
+    def warm(self, pid: int):
+        _chk(lib().sap_server_warm(self.h, pid), "sap_server_warm")
+
+    def replay(self, arrival_s, prob, ovptr, ovcol, ovlb, ovub, mode: int = SAP_SERVE_CONTINUOUS,
+               steprounds: int = SAP_STEP_SHORT, params: sap_params | None = None, want_changes: bool = False):
+        arrival_s = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        prob = np.ascontiguousarray(prob, dtype=np.int32)
+        ovptr, ovcol, ovlb, ovub = _overlay_arrays(ovptr, ovcol, ovlb, ovub)
+        n = len(arrival_s)
+        res = np.zeros(n, dtype=np.int32)
+        rnd = np.zeros(n, dtype=np.int32)
+        nch = np.zeros(n, dtype=np.int32)
+        lat = np.zeros(n)
+        st = sap_serve_stats()
+        p = params or default_params()
+        if want_changes:
+            cap = max(1024, 256 * n)
+            while True:
+                cptr = np.zeros(n + 1, dtype=np.int64)
+                ccol = np.zeros(cap, dtype=np.int32)
+                clb = np.zeros(cap)
+                cub = np.zeros(cap)
+                rc = lib().sap_server_replay(self.h, C.byref(p), mode, steprounds, n, _dp(arrival_s), _i32p(prob),
+                                             _i64p(ovptr), _i32p(ovcol), _dp(ovlb), _dp(ovub), _i32p(res), _i32p(rnd),
+                                             _i32p(nch), _dp(lat), cap, _i64p(cptr), _i32p(ccol), _dp(clb), _dp(cub),
+                                             C.byref(st))
+                if rc == SAP_NOMEMORY and int(cptr[-1]) > cap:
+                    cap = int(cptr[-1])
+                    continue
+                _chk(rc, "sap_server_replay")
+                m = int(cptr[-1])
+                return dict(result=res, rounds=rnd, nchg=nch, latency_ms=lat, stats=st.as_dict(), chgptr=cptr,
+                            chgcol=ccol[:m], chglb=clb[:m], chgub=cub[:m])
+        _chk(lib().sap_server_replay(self.h, C.byref(p), mode, steprounds, n, _dp(arrival_s), _i32p(prob),
+                                     _i64p(ovptr), _i32p(ovcol), _dp(ovlb), _dp(ovub), _i32p(res), _i32p(rnd),
+                                     _i32p(nch), _dp(lat), 0, None, None, None, None, C.byref(st)),
+             "sap_server_replay")
+        return dict(result=res, rounds=rnd, nchg=nch, latency_ms=lat, stats=st.as_dict())
+
+    def close(self):
+        if self.h:
+            lib().sap_server_free(self.h)
             self.h = _P()
 
     def __del__(self):

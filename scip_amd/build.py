@@ -36,11 +36,13 @@ def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     eng_o = os.path.join(BUILD, "engine.o")
     gen_o = os.path.join(BUILD, "gen.o")
+    srv_o = os.path.join(BUILD, "serve.o")
     lib = os.path.join(HERE, "libscipamd.so")
     steps = [
         [HIPCC, *HIPFLAGS, "-c", os.path.join(HERE, "csrc", "engine.hip"), "-o", eng_o],
         ["gcc", *CFLAGS, "-c", os.path.join(HERE, "csrc", "gen.c"), "-o", gen_o],
-        [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", eng_o, gen_o, "-o", lib + ".tmp"],
+        ["g++", "-std=c++17", *CFLAGS, "-c", os.path.join(HERE, "csrc", "serve.cpp"), "-o", srv_o],
+        [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", eng_o, gen_o, srv_o, "-o", lib + ".tmp"],
     ]
     for cmd in steps:
         out = _run(cmd)

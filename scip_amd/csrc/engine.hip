@@ -29,6 +29,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
+#include <atomic>
+#include <thread>
 #include <vector>
 
 #include "devnum.h"
@@ -2125,6 +2128,7 @@ struct sap_context
 {
    int32_t device;
    int32_t wallclock_khz;   // rate of wall_clock64() on the device
+   sap_tuning tuning;       // applied to matrices created on this context
    hipStream_t stream;
    hipEvent_t ev[4];
    Status* hstatus;   // pinned
@@ -2152,6 +2156,7 @@ struct sap_matrix
    unsigned long long* tmask;
    int32_t* trow0;
    uint8_t* tlev;
+   sap_tuning tuning;    // row binning and launch shape it was built with
    int32_t nseg;         // packed instances (1: a single instance)
    int32_t* rowseg;      // device, by device row position (NULL when nseg == 1)
    int32_t* colseg;      // device (NULL when nseg == 1)
@@ -2324,6 +2329,33 @@ static Mat toMat(const sap_matrix* m)
    return M;
 }
 
+// host threads for the upload's preprocessing
+template <typename F>
+static void parallelFor(int64_t n, int64_t grain, F fn)
+{
+   int nt = (int)std::thread::hardware_concurrency();
+   nt = nt < 1 ? 1 : (nt > 16 ? 16 : nt);
+   if( n <= grain || nt == 1 )
+   {
+      fn(0, n, 0);
+      return;
+   }
+   const int64_t chunks = std::min<int64_t>(nt, (n + grain - 1) / grain);
+   std::vector<std::thread> th;
+   for( int64_t t = 0; t < chunks; ++t )
+      th.emplace_back([&, t]() { fn(n * t / chunks, n * (t + 1) / chunks, (int)t); });
+   for( auto& x : th )
+      x.join();
+}
+
+static inline int lenBin(int64_t L, int32_t longrow)
+{
+   for( int q = 0; q < SAP_NBINS - 2; ++q )
+      if( L <= kBinMaxLen[q] )
+         return q;
+   return L <= longrow ? SAP_NBINS - 2 : SAP_NBINS - 1;
+}
+
 extern "C" {
 
 void sap_params_default(sap_params* p)
@@ -2340,6 +2372,33 @@ void sap_params_default(sap_params* p)
    p->maxeasyactivitydelta = 1e6;
    p->maxrounds = -1;
    p->flags = 0;
+}
+
+void sap_tuning_default(sap_tuning* t)
+{
+   if( t == NULL )
+      return;
+   t->tilebins = 5;
+   t->longrow = 4096;
+   t->capscale = 100;
+   t->reserved = 0;
+}
+
+int sap_context_set_tuning(sap_context* c, const sap_tuning* t)
+{
+   if( c == NULL || t == NULL || t->tilebins < 0 || t->tilebins > 5 || t->longrow < 64 || t->longrow > (1 << 24)
+      || t->capscale < 10 || t->capscale > 1000 )
+      return SAP_INVALIDDATA;
+   c->tuning = *t;
+   return SAP_OKAY;
+}
+
+int sap_context_get_tuning(const sap_context* c, sap_tuning* t)
+{
+   if( c == NULL || t == NULL )
+      return SAP_INVALIDDATA;
+   *t = c->tuning;
+   return SAP_OKAY;
 }
 
 const char* sap_error_string(int rc)
@@ -2385,6 +2444,7 @@ int sap_context_create(int32_t device, sap_context** out)
    if( c == NULL )
       return SAP_NOMEMORY;
    c->device = device;
+   sap_tuning_default(&c->tuning);
    if( hipDeviceGetAttribute(&c->wallclock_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess
       || c->wallclock_khz <= 0 )
       c->wallclock_khz = 100000;   // 100 MHz reference clock
@@ -2442,9 +2502,13 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
       || gub == NULL || nrows < 0 || ncols < 0 || nrows > INT32_MAX || ncols > INT32_MAX || nseg < 1 )
       return SAP_INVALIDDATA;
    *out = NULL;
+   const sap_tuning tune = ctx->tuning;
    const int64_t nnz = rowptr[nrows];
    if( rowptr[0] != 0 || nnz < 0 || (nnz > 0 && (colidx == NULL || vals == NULL)) )
       return SAP_INVALIDDATA;
+   for( int64_t i = 0; i < nrows; ++i )
+      if( rowptr[i + 1] < rowptr[i] || rowptr[i + 1] > nnz )
+         return SAP_INVALIDDATA;
    // segment of every row and column (instances must not share columns)
    std::vector<int32_t> oseg, cseg;
    if( nseg > 1 )
@@ -2452,22 +2516,28 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
       if( rowoff == NULL || coloff == NULL || rowoff[0] != 0 || coloff[0] != 0 || rowoff[nseg] != nrows
          || coloff[nseg] != ncols )
          return SAP_INVALIDDATA;
-      oseg.resize((size_t)nrows);
-      cseg.resize((size_t)ncols);
       for( int32_t q = 0; q < nseg; ++q )
-      {
          if( rowoff[q + 1] < rowoff[q] || coloff[q + 1] < coloff[q] )
             return SAP_INVALIDDATA;
-         for( int64_t i = rowoff[q]; i < rowoff[q + 1]; ++i )
+      oseg.resize((size_t)nrows);
+      cseg.resize((size_t)ncols);
+      std::atomic<int> bad(0);
+      parallelFor(nseg, 1, [&](int64_t q0, int64_t q1, int) {
+         for( int64_t q = q0; q < q1; ++q )
          {
-            oseg[(size_t)i] = q;
-            for( int64_t k = rowptr[i]; k < rowptr[i + 1] && k >= 0 && k < nnz; ++k )
-               if( colidx[k] < coloff[q] || colidx[k] >= coloff[q + 1] )
-                  return SAP_INVALIDDATA;
+            for( int64_t i = rowoff[q]; i < rowoff[q + 1]; ++i )
+            {
+               oseg[(size_t)i] = (int32_t)q;
+               for( int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k )
+                  if( colidx[k] < coloff[q] || colidx[k] >= coloff[q + 1] )
+                     bad = 1;
+            }
+            for( int64_t j = coloff[q]; j < coloff[q + 1]; ++j )
+               cseg[(size_t)j] = (int32_t)q;
          }
-         for( int64_t j = coloff[q]; j < coloff[q + 1]; ++j )
-            cseg[(size_t)j] = q;
-      }
+      });
+      if( bad )
+         return SAP_INVALIDDATA;
    }
    // host-side validation and derived arrays; rows are stored grouped by length bin
    // (stable), so each bin is a contiguous range and sub-wavefront groups load contiguously
@@ -2481,90 +2551,106 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
       colflags[(size_t)j] = (uint8_t)((isint ? 1 : 0) | (isbin ? 2 : 0));
    }
    std::vector<uint8_t> obin((size_t)nrows);
-   int32_t hasforced = 0;
-   int64_t bincount[SAP_NBINS] = { 0, 0, 0, 0, 0, 0, 0, 0 };
-   {
-      std::vector<int64_t> seen((size_t)ncols, -1);
-      for( int64_t i = 0; i < nrows; ++i )
+   std::atomic<int> bad(0), forced(0);
+   std::vector<std::array<int64_t, SAP_NBINS>> tcount(16);
+   for( auto& c : tcount )
+      c.fill(0);
+   parallelFor(nrows, 1 << 16, [&](int64_t i0, int64_t i1, int t) {
+      std::vector<int32_t> tmp;
+      for( int64_t i = i0; i < i1; ++i )
       {
-         int64_t st0 = rowptr[i], e = rowptr[i + 1];
-         if( e < st0 || e > nnz )
-            return SAP_INVALIDDATA;
+         const int64_t st0 = rowptr[i], e = rowptr[i + 1];
          if( lhs[i] != lhs[i] || rhs[i] != rhs[i] )
-            return SAP_INVALIDDATA;
+            bad = 1;
+         bool ascending = true;
          for( int64_t k = st0; k < e; ++k )
          {
-            int32_t c = colidx[k];
+            const int32_t c = colidx[k];
             if( c < 0 || c >= ncols || vals[k] == 0.0 || vals[k] != vals[k] )
-               return SAP_INVALIDDATA;
-            // duplicate columns within a row are not allowed (SCIP merges them)
-            if( seen[(size_t)c] == i )
-               return SAP_INVALIDDATA;
-            seen[(size_t)c] = i;
+               bad = 1;
+            if( k > st0 && c <= colidx[k - 1] )
+               ascending = false;
          }
-         int64_t L = e - st0;
+         // duplicate columns within a row are not allowed (SCIP merges them)
+         if( !ascending )
+         {
+            tmp.assign(colidx + st0, colidx + e);
+            std::sort(tmp.begin(), tmp.end());
+            for( size_t q = 1; q < tmp.size(); ++q )
+               if( tmp[q] == tmp[q - 1] )
+                  bad = 1;
+         }
+         const int64_t L = e - st0;
          if( L == 1 )
-            hasforced = 1;
-         int b = SAP_NBINS - 1;
-         for( int q = 0; q < SAP_NBINS - 1; ++q )
-            if( L <= kBinMaxLen[q] )
-            {
-               b = q;
-               break;
-            }
+            forced = 1;
+         const int b = lenBin(L, tune.longrow);
          obin[(size_t)i] = (uint8_t)b;
-         bincount[b]++;
+         tcount[(size_t)t][(size_t)b]++;
       }
-   }
+   });
+   if( bad )
+      return SAP_INVALIDDATA;
+   const int32_t hasforced = forced.load();
+   int64_t bincount[SAP_NBINS] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+   for( auto& c : tcount )
+      for( int b = 0; b < SAP_NBINS; ++b )
+         bincount[b] += c[(size_t)b];
    int32_t binstart[SAP_NBINS + 1];
    binstart[0] = 0;
    for( int b = 0; b < SAP_NBINS; ++b )
       binstart[b + 1] = binstart[b] + (int32_t)bincount[b];
-   std::vector<int32_t> perm((size_t)nrows), iperm((size_t)nrows);
+   std::vector<int32_t> perm((size_t)nrows);
    {
       int32_t fill[SAP_NBINS];
       for( int b = 0; b < SAP_NBINS; ++b )
          fill[b] = binstart[b];
       for( int64_t i = 0; i < nrows; ++i )
-      {
-         int32_t pos = fill[obin[(size_t)i]]++;
-         perm[(size_t)pos] = (int32_t)i;
-         iperm[(size_t)i] = pos;
-      }
+         perm[(size_t)fill[obin[(size_t)i]]++] = (int32_t)i;
    }
    std::vector<int64_t> prowptr((size_t)nrows + 1);
+   prowptr[0] = 0;
+   for( int64_t r = 0; r < nrows; ++r )
+   {
+      const int64_t i = perm[(size_t)r];
+      prowptr[(size_t)r + 1] = prowptr[(size_t)r] + (rowptr[i + 1] - rowptr[i]);
+   }
    std::vector<int32_t> pcol((size_t)nnz);
    std::vector<double> pval((size_t)nnz);
    std::vector<double2> sides((size_t)nrows);
    std::vector<uint8_t> rowflags((size_t)nrows), rowbin((size_t)nrows);
    std::vector<int64_t> colptr((size_t)ncols + 1, 0);
    std::vector<int32_t> rowidx((size_t)nnz);
-   prowptr[0] = 0;
-   for( int64_t r = 0; r < nrows; ++r )
-   {
-      const int64_t i = perm[(size_t)r];
-      const int64_t st0 = rowptr[i], L = rowptr[i + 1] - st0, o = prowptr[(size_t)r];
-      bool allbin = true;
-      for( int64_t k = 0; k < L; ++k )
+   parallelFor(nrows, 1 << 16, [&](int64_t r0, int64_t r1, int) {
+      for( int64_t r = r0; r < r1; ++r )
       {
-         pcol[(size_t)(o + k)] = colidx[st0 + k];
-         pval[(size_t)(o + k)] = vals[st0 + k];
-         if( !(colflags[(size_t)colidx[st0 + k]] & 2) )
-            allbin = false;
-         colptr[(size_t)colidx[st0 + k] + 1]++;
+         const int64_t i = perm[(size_t)r];
+         const int64_t st0 = rowptr[i], L = rowptr[i + 1] - st0, o = prowptr[(size_t)r];
+         bool allbin = true;
+         for( int64_t k = 0; k < L; ++k )
+         {
+            const int32_t c = colidx[st0 + k];
+            pcol[(size_t)(o + k)] = c;
+            pval[(size_t)(o + k)] = vals[st0 + k];
+            if( !(colflags[(size_t)c] & 2) )
+               allbin = false;
+            __atomic_fetch_add(&colptr[(size_t)c + 1], (int64_t)1, __ATOMIC_RELAXED);
+         }
+         sides[(size_t)r] = make_double2(lhs[i], rhs[i]);
+         rowflags[(size_t)r] = allbin ? 1 : 0;
+         rowbin[(size_t)r] = obin[(size_t)i];
       }
-      prowptr[(size_t)r + 1] = o + L;
-      sides[(size_t)r] = make_double2(lhs[i], rhs[i]);
-      rowflags[(size_t)r] = allbin ? 1 : 0;
-      rowbin[(size_t)r] = obin[(size_t)i];
-   }
+   });
    for( int64_t j = 0; j < ncols; ++j )
       colptr[(size_t)j + 1] += colptr[(size_t)j];
    {
+      // the row order within a column is irrelevant (rows of a changed column are marked)
       std::vector<int64_t> fill(colptr.begin(), colptr.end() - 1);
-      for( int64_t r = 0; r < nrows; ++r )
-         for( int64_t k = prowptr[(size_t)r]; k < prowptr[(size_t)r + 1]; ++k )
-            rowidx[(size_t)fill[(size_t)pcol[(size_t)k]]++] = (int32_t)r;
+      parallelFor(nrows, 1 << 16, [&](int64_t r0, int64_t r1, int) {
+         for( int64_t r = r0; r < r1; ++r )
+            for( int64_t k = prowptr[(size_t)r]; k < prowptr[(size_t)r + 1]; ++k )
+               rowidx[(size_t)__atomic_fetch_add(&fill[(size_t)pcol[(size_t)k]], (int64_t)1, __ATOMIC_RELAXED)]
+                  = (int32_t)r;
+      });
    }
 
    // 64-nonzero tiles over the rows of bins 1..5 (consecutive, non-empty, <= 64 nonzeros)
@@ -2581,7 +2667,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
             ++v;
          return (uint8_t)v;
       };
-      for( int32_t r = binstart[1]; r < binstart[6]; ++r )
+      for( int32_t r = binstart[1]; r < binstart[1 + tune.tilebins]; ++r )
       {
          const int L = (int)(prowptr[(size_t)r + 1] - prowptr[(size_t)r]);
          if( cnt == 0 || cnt + L > 64 )
@@ -2606,7 +2692,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
          tmask.push_back(mask);
          tlev.push_back(levels(maxl));
       }
-      tstart.push_back(prowptr[(size_t)binstart[6]]);
+      tstart.push_back(prowptr[(size_t)binstart[1 + tune.tilebins]]);
    }
 
    HIPCHK(hipSetDevice(ctx->device));
@@ -2614,6 +2700,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
    if( m == NULL )
       return SAP_NOMEMORY;
    m->ctx = ctx;
+   m->tuning = tune;
    m->ntiles = (int32_t)tmask.size();
    m->nrows = nrows;
    m->ncols = ncols;
@@ -2702,6 +2789,109 @@ int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int6
    const double* gub, sap_matrix** out)
 {
    return matrixCreate(ctx, nrows, ncols, rowptr, colidx, vals, lhs, rhs, integral, glb, gub, 1, NULL, NULL, out);
+}
+
+static int checkParams(const sap_params* p);
+
+int sap_autotune(sap_context* ctx, int64_t nrows, int64_t ncols, const int64_t* rowptr, const int32_t* colidx,
+   const double* vals, const double* lhs, const double* rhs, const uint8_t* integral, const double* lb,
+   const double* ub, const sap_params* params, int64_t samplerows, uint64_t seed, sap_tuning* best, double* best_ms,
+   double* default_ms)
+{
+   if( ctx == NULL || rowptr == NULL || lb == NULL || ub == NULL || best == NULL || samplerows < 1 || nrows < 0 )
+      return SAP_INVALIDDATA;
+   RCCHK(checkParams(params));
+   // row sample (every row when there are few): splitmix64 scores, smallest samplerows kept
+   std::vector<int64_t> rows;
+   if( nrows <= samplerows )
+   {
+      rows.resize((size_t)nrows);
+      for( int64_t i = 0; i < nrows; ++i )
+         rows[(size_t)i] = i;
+   }
+   else
+   {
+      std::vector<std::pair<uint64_t, int64_t>> sc((size_t)nrows);
+      for( int64_t i = 0; i < nrows; ++i )
+      {
+         uint64_t z = seed + 0x9E3779B97F4A7C15ULL * (uint64_t)(i + 1);
+         z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+         z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+         sc[(size_t)i] = std::make_pair(z ^ (z >> 31), i);
+      }
+      std::nth_element(sc.begin(), sc.begin() + samplerows, sc.end());
+      rows.resize((size_t)samplerows);
+      for( int64_t q = 0; q < samplerows; ++q )
+         rows[(size_t)q] = sc[(size_t)q].second;
+      std::sort(rows.begin(), rows.end());
+   }
+   const int64_t sn = (int64_t)rows.size();
+   std::vector<int64_t> srp((size_t)sn + 1, 0);
+   std::vector<int32_t> sci;
+   std::vector<double> sva, slh((size_t)sn), srh((size_t)sn);
+   for( int64_t q = 0; q < sn; ++q )
+   {
+      const int64_t i = rows[(size_t)q];
+      for( int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k )
+      {
+         sci.push_back(colidx[k]);
+         sva.push_back(vals[k]);
+      }
+      srp[(size_t)q + 1] = (int64_t)sci.size();
+      slh[(size_t)q] = lhs[i];
+      srh[(size_t)q] = rhs[i];
+   }
+   const sap_tuning saved = ctx->tuning;
+   sap_tuning def;
+   sap_tuning_default(&def);
+   static const int32_t tbs[] = { 3, 4, 5 };
+   static const int32_t lrs[] = { 1024, 4096, 16384 };
+   static const int32_t cps[] = { 50, 100, 200 };
+   std::vector<double> wl((size_t)ncols), wu((size_t)ncols);
+   double bestt = 1e300, deft = -1.0;
+   sap_tuning bt = def;
+   int rc = SAP_OKAY;
+   for( int a = 0; a < 3 && rc == SAP_OKAY; ++a )
+      for( int l = 0; l < 3 && rc == SAP_OKAY; ++l )
+         for( int c = 0; c < 3 && rc == SAP_OKAY; ++c )
+         {
+            sap_tuning t = def;
+            t.tilebins = tbs[a];
+            t.longrow = lrs[l];
+            t.capscale = cps[c];
+            ctx->tuning = t;
+            sap_matrix* m = NULL;
+            rc = matrixCreate(ctx, sn, ncols, srp.data(), sci.data(), sva.data(), slh.data(), srh.data(), integral, lb,
+               ub, 1, NULL, NULL, &m);
+            if( rc != SAP_OKAY )
+               break;
+            double tmin = 1e300;
+            for( int rep = 0; rep < 3 && rc == SAP_OKAY; ++rep )
+            {
+               memcpy(wl.data(), lb, sizeof(double) * (size_t)ncols);
+               memcpy(wu.data(), ub, sizeof(double) * (size_t)ncols);
+               sap_stats st;
+               rc = sap_propagate(m, params, wl.data(), wu.data(), &st);
+               if( rep > 0 )   // the first call includes one-time set-up (graph capture)
+                  tmin = std::min(tmin, st.device_ms);
+            }
+            sap_matrix_free(m);
+            if( t.tilebins == def.tilebins && t.longrow == def.longrow && t.capscale == def.capscale )
+               deft = tmin;
+            if( tmin < bestt )
+            {
+               bestt = tmin;
+               bt = t;
+            }
+         }
+   ctx->tuning = saved;
+   RCCHK(rc);
+   *best = bt;
+   if( best_ms )
+      *best_ms = bestt;
+   if( default_ms )
+      *default_ms = deft;
+   return SAP_OKAY;
 }
 
 int sap_matrix_create_packed(sap_context* ctx, int32_t ninst, const int64_t* rowoff, const int64_t* coloff,
@@ -2991,10 +3181,12 @@ static Layout makeLayout(const sap_bounds* b)
    Layout L;
    const sap_matrix* m = b->mat;
    const size_t nvec = (size_t)b->V.nvec;
-   // items per workgroup and block caps per kind (kind 0 tiles, kind 1+b bin b)
+   // items per workgroup and block caps per kind (kind 0 tiles, kind 1+b bin b); the caps
+   // scale with the matrix's tuning
    static const int kPerBlock[SAP_NKINDS] = { SAP_BLOCK / SAP_WAVE, SAP_BLOCK / 4, SAP_BLOCK / 4, SAP_BLOCK / 8,
       SAP_BLOCK / 16, SAP_BLOCK / 32, SAP_BLOCK / 64, SAP_BLOCK / SAP_WAVE, 1 };
    static const int kCap[SAP_NKINDS] = { 4096, 2048, 2048, 2048, 2048, 2048, 2048, 2048, 1024 };
+   const int tb = m->tuning.tilebins;
    int acc = 0;
    for( int q = 0; q < SAP_NKINDS; ++q )
    {
@@ -3002,13 +3194,14 @@ static Layout makeLayout(const sap_bounds* b)
       size_t items;
       if( q == 0 )
          items = b->V.dense ? nvec * (size_t)(m->ntiles > 0 ? m->ntiles : (m->binstart[1] > m->binstart[0] ? 1 : 0)) : 0;
-      else if( b->V.dense && q <= 6 )
-         items = 0;   // bins 0..5 go through the tiles in dense mode
+      else if( b->V.dense && q <= 1 + tb )
+         items = 0;   // empty rows and bins 1..tilebins go through the tiles in dense mode
       else
          items = nvec * (size_t)(m->binstart[q] - m->binstart[q - 1]);
       size_t blocks = (items + kPerBlock[q] - 1) / kPerBlock[q];
-      if( blocks > (size_t)kCap[q] )
-         blocks = (size_t)kCap[q];
+      const size_t cap = std::max<size_t>(64, (size_t)kCap[q] * (size_t)m->tuning.capscale / 100);
+      if( blocks > cap )
+         blocks = cap;
       acc += (int)blocks;
    }
    L.blockstart[SAP_NKINDS] = acc;

@@ -1,0 +1,550 @@
+// serve.cpp -- request serving on top of the propagation engine: resident problems, admission
+// by HBM footprint with LRU eviction, continuous batching of node requests in per-problem slot
+// pools over several HIP streams, and a real-time replay of an arrival trace (see
+// include/scip_amd_serve.h).
+//
+// Host code only: every device operation goes through the engine's C ABI.
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <deque>
+#include <thread>
+#include <vector>
+
+#include "scip_amd_prop.h"
+#include "scip_amd_serve.h"
+
+namespace {
+
+struct Problem
+{
+   int64_t nrows = 0, ncols = 0;
+   std::vector<int64_t> rowptr;
+   std::vector<int32_t> colidx;
+   std::vector<double> vals, lhs, rhs, lb, ub;
+   std::vector<uint8_t> integral;
+   int32_t stream = 0;        // context (HIP stream) the problem lives on
+   sap_matrix* mat = NULL;    // NULL: not resident
+   sap_bounds* root = NULL;   // root fixpoint (base of every request)
+   sap_bounds* pool = NULL;   // slots
+   int32_t rootcut = 0;       // the root is infeasible: every request is a cutoff
+   int64_t bytes = 0;         // device bytes while resident
+   uint64_t lastuse = 0;
+   std::deque<int32_t> queue;         // waiting requests
+   std::vector<int32_t> slotreq;      // slot -> request (-1: free)
+   std::vector<int32_t> freeslots;
+   int32_t running = 0;
+   bool inflight = false;
+};
+
+struct Request
+{
+   double arrival = 0.0;
+   int32_t prob = 0;
+   int32_t result = 0, rounds = 0, nchg = 0;
+   double latency_ms = 0.0;
+   std::vector<int32_t> col;
+   std::vector<double> clb, cub;
+   bool done = false;
+};
+
+}   // namespace
+
+struct sap_server
+{
+   int32_t device = 0;
+   std::vector<sap_context*> ctx;
+   int64_t budget = 0;
+   int32_t slots = 0;
+   std::vector<Problem*> probs;
+   int64_t resident = 0;
+   uint64_t tick = 0;
+   // per replay
+   sap_serve_stats st;
+   int32_t ndone = 0;
+};
+
+namespace {
+
+// device bytes of a resident problem (upper bound, before it is uploaded): matrix with CSC and
+// tiles, the root vector and a slot pool of `slots` vectors with change logs
+int64_t footprint(const Problem* p, int32_t slots)
+{
+   const int64_t nnz = p->rowptr.empty() ? 0 : p->rowptr.back();
+   const int64_t mat = nnz * (8 + 4 + 4) + nnz / 4 + p->nrows * (8 + 16 + 1 + 1 + 1 + 4 + 4 + 8) + p->ncols * (1 + 8 + 4)
+      + 4096;
+   const int64_t vec1 = p->ncols * (16 + 16 + 16 + 4 + 32) + p->nrows * (4 + 1 + 16);
+   const int64_t pool = (int64_t)slots * (p->ncols * (16 + 16 + 16 + 4 + 8 + 4 + 4 + 1) + p->nrows * (4 + 1 + 16))
+      + p->ncols * 32 + 65536;
+   return mat + vec1 + pool;
+}
+
+double nowSeconds()
+{
+   using namespace std::chrono;
+   return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+void evict(sap_server* s, Problem* p)
+{
+   if( p->pool )
+      sap_bounds_free(p->pool);
+   if( p->root )
+      sap_bounds_free(p->root);
+   if( p->mat )
+      sap_matrix_free(p->mat);
+   p->pool = NULL;
+   p->root = NULL;
+   p->mat = NULL;
+   s->resident -= p->bytes;
+   p->bytes = 0;
+   s->st.evictions++;
+}
+
+// 1: resident; 0: does not fit now (wait); < 0... SAP error code
+int makeResident(sap_server* s, Problem* p, const sap_params* params)
+{
+   if( p->mat != NULL )
+      return 1;
+   const int64_t need = footprint(p, s->slots);
+   if( need > s->budget )
+      return SAP_NOMEMORY;
+   while( s->resident + need > s->budget )
+   {
+      Problem* victim = NULL;
+      for( Problem* q : s->probs )
+         if( q->mat != NULL && q->running == 0 && q->queue.empty() && !q->inflight
+            && (victim == NULL || q->lastuse < victim->lastuse) )
+            victim = q;
+      if( victim == NULL )
+         return 0;
+      evict(s, victim);
+   }
+   sap_context* c = s->ctx[(size_t)p->stream];
+   int rc = sap_matrix_create(c, p->nrows, p->ncols, p->rowptr.data(), p->colidx.data(), p->vals.data(),
+      p->lhs.data(), p->rhs.data(), p->integral.data(), p->lb.data(), p->ub.data(), &p->mat);
+   if( rc != SAP_OKAY )
+      return rc;
+   rc = sap_bounds_create_mode(p->mat, 1, SAP_MODE_DENSE, &p->root);
+   if( rc == SAP_OKAY )
+      rc = sap_bounds_upload(p->root, 0, p->lb.data(), p->ub.data());
+   sap_stats rst;
+   memset(&rst, 0, sizeof(rst));
+   if( rc == SAP_OKAY )
+   {
+      sap_params rp = *params;
+      rp.flags = 0;
+      rc = sap_propagate_resident(p->root, &rp, &rst);
+   }
+   if( rc == SAP_OKAY )
+      rc = sap_bounds_create(p->mat, s->slots, &p->pool);
+   if( rc != SAP_OKAY )
+   {
+      p->bytes = 0;
+      if( p->pool )
+         sap_bounds_free(p->pool);
+      if( p->root )
+         sap_bounds_free(p->root);
+      sap_matrix_free(p->mat);
+      p->pool = NULL;
+      p->root = NULL;
+      p->mat = NULL;
+      return rc;
+   }
+   p->rootcut = rst.result == SAP_CUTOFF ? 1 : 0;
+   p->slotreq.assign((size_t)s->slots, -1);
+   p->freeslots.clear();
+   for( int32_t k = s->slots - 1; k >= 0; --k )
+      p->freeslots.push_back(k);
+   p->running = 0;
+   p->bytes = need;
+   s->resident += need;
+   s->st.uploads++;
+   s->st.peak_bytes = std::max(s->st.peak_bytes, s->resident);
+   return 1;
+}
+
+void finish(sap_server* s, Request& r, double t0, int32_t result, int32_t rounds)
+{
+   r.result = result;
+   r.rounds = rounds;
+   r.nchg = (int32_t)r.col.size();
+   r.latency_ms = 1000.0 * (nowSeconds() - t0 - r.arrival);
+   r.done = true;
+   s->ndone++;
+}
+
+// admit up to n queued requests of p into free slots
+int admit(sap_server* s, Problem* p, std::vector<Request>& req, const int64_t* ovptr, const int32_t* ovcol,
+   const double* ovlb, const double* ovub, int32_t maxn, double t0)
+{
+   if( p->rootcut )
+   {
+      while( !p->queue.empty() )
+      {
+         finish(s, req[(size_t)p->queue.front()], t0, SAP_CUTOFF, 0);
+         p->queue.pop_front();
+      }
+      return SAP_OKAY;
+   }
+   std::vector<int32_t> slots;
+   std::vector<int64_t> ptr(1, 0);
+   std::vector<int32_t> col;
+   std::vector<double> lo, hi;
+   while( !p->queue.empty() && !p->freeslots.empty() && (int32_t)slots.size() < maxn )
+   {
+      const int32_t i = p->queue.front();
+      p->queue.pop_front();
+      const int32_t k = p->freeslots.back();
+      p->freeslots.pop_back();
+      p->slotreq[(size_t)k] = i;
+      slots.push_back(k);
+      for( int64_t q = ovptr[i]; q < ovptr[i + 1]; ++q )
+      {
+         col.push_back(ovcol[q]);
+         lo.push_back(ovlb[q]);
+         hi.push_back(ovub[q]);
+      }
+      ptr.push_back((int64_t)col.size());
+   }
+   if( slots.empty() )
+      return SAP_OKAY;
+   const int rc = sap_slots_admit(p->pool, (int32_t)slots.size(), slots.data(), p->root, 0, ptr.data(),
+      col.empty() ? NULL : col.data(), lo.empty() ? NULL : lo.data(), hi.empty() ? NULL : hi.data(),
+      SAP_FLAG_BASE_IS_FIXPOINT);
+   if( rc != SAP_OKAY )
+      return rc;
+   p->running += (int32_t)slots.size();
+   p->lastuse = ++s->tick;
+   s->st.admissions++;
+   return SAP_OKAY;
+}
+
+// wait for p's step and collect its finished slots
+int harvest(sap_server* s, Problem* p, std::vector<Request>& req, bool wantchg, double t0)
+{
+   int32_t nf = 0;
+   std::vector<int32_t> fin((size_t)s->slots);
+   sap_stats st;
+   int rc = sap_slots_wait(p->pool, &nf, fin.data(), &st);
+   p->inflight = false;
+   if( rc != SAP_OKAY )
+      return rc;
+   s->st.nnz_swept += st.nnz_swept;
+   s->st.rows_swept += st.rows_swept;
+   if( nf == 0 )
+      return SAP_OKAY;
+   std::vector<int32_t> res((size_t)nf), rnd((size_t)nf);
+   std::vector<int64_t> cptr((size_t)nf + 1);
+   int64_t cap = 64 * (int64_t)nf + 1024;
+   std::vector<int32_t> ccol;
+   std::vector<double> clb, cub;
+   for( ;; )
+   {
+      ccol.resize((size_t)cap);
+      clb.resize(wantchg ? (size_t)cap : 1);
+      cub.resize(wantchg ? (size_t)cap : 1);
+      rc = sap_slots_collect(p->pool, nf, fin.data(), res.data(), rnd.data(), cap, cptr.data(), ccol.data(),
+         wantchg ? clb.data() : NULL, wantchg ? cub.data() : NULL);
+      if( rc == SAP_NOMEMORY && cptr[(size_t)nf] > cap )
+      {
+         cap = cptr[(size_t)nf];
+         continue;
+      }
+      break;
+   }
+   if( rc != SAP_OKAY )
+      return rc;
+   for( int32_t q = 0; q < nf; ++q )
+   {
+      const int32_t k = fin[(size_t)q];
+      Request& r = req[(size_t)p->slotreq[(size_t)k]];
+      if( wantchg )
+      {
+         r.col.assign(ccol.begin() + cptr[(size_t)q], ccol.begin() + cptr[(size_t)q + 1]);
+         r.clb.assign(clb.begin() + cptr[(size_t)q], clb.begin() + cptr[(size_t)q + 1]);
+         r.cub.assign(cub.begin() + cptr[(size_t)q], cub.begin() + cptr[(size_t)q + 1]);
+      }
+      else
+         r.col.resize((size_t)(cptr[(size_t)q + 1] - cptr[(size_t)q]));
+      finish(s, r, t0, res[(size_t)q], rnd[(size_t)q]);
+      p->slotreq[(size_t)k] = -1;
+      p->freeslots.push_back(k);
+   }
+   p->running -= nf;
+   p->lastuse = ++s->tick;
+   return SAP_OKAY;
+}
+
+void waitUntil(double t)
+{
+   for( ;; )
+   {
+      const double d = t - nowSeconds();
+      if( d <= 0.0 )
+         return;
+      if( d > 3e-4 )
+         std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(1e6 * d) - 200));
+   }
+}
+
+}   // namespace
+
+extern "C" {
+
+int sap_server_create(int32_t device, int32_t nstreams, int64_t hbm_budget, int32_t slots_per_pool, sap_server** out)
+{
+   if( out == NULL || nstreams < 1 || nstreams > 16 || hbm_budget <= 0 || slots_per_pool < 1 )
+      return SAP_INVALIDDATA;
+   *out = NULL;
+   sap_server* s = new sap_server();
+   s->device = device;
+   s->budget = hbm_budget;
+   s->slots = slots_per_pool;
+   for( int32_t k = 0; k < nstreams; ++k )
+   {
+      sap_context* c = NULL;
+      const int rc = sap_context_create(device, &c);
+      if( rc != SAP_OKAY )
+      {
+         for( sap_context* d : s->ctx )
+            sap_context_free(d);
+         delete s;
+         return rc;
+      }
+      s->ctx.push_back(c);
+   }
+   *out = s;
+   return SAP_OKAY;
+}
+
+int sap_server_free(sap_server* s)
+{
+   if( s == NULL )
+      return SAP_OKAY;
+   for( Problem* p : s->probs )
+   {
+      if( p->mat )
+         evict(s, p);
+      delete p;
+   }
+   for( sap_context* c : s->ctx )
+      sap_context_free(c);
+   delete s;
+   return SAP_OKAY;
+}
+
+int sap_server_add_problem(sap_server* s, int64_t nrows, int64_t ncols, const int64_t* rowptr, const int32_t* colidx,
+   const double* vals, const double* lhs, const double* rhs, const uint8_t* integral, const double* lb,
+   const double* ub, int32_t* id)
+{
+   if( s == NULL || id == NULL || rowptr == NULL || nrows < 0 || ncols < 0 || lhs == NULL || rhs == NULL
+      || integral == NULL || lb == NULL || ub == NULL || rowptr[0] != 0 || rowptr[nrows] < 0 )
+      return SAP_INVALIDDATA;
+   const int64_t nnz = rowptr[nrows];
+   if( nnz > 0 && (colidx == NULL || vals == NULL) )
+      return SAP_INVALIDDATA;
+   Problem* p = new Problem();
+   p->nrows = nrows;
+   p->ncols = ncols;
+   p->rowptr.assign(rowptr, rowptr + nrows + 1);
+   p->colidx.assign(colidx, colidx + nnz);
+   p->vals.assign(vals, vals + nnz);
+   p->lhs.assign(lhs, lhs + nrows);
+   p->rhs.assign(rhs, rhs + nrows);
+   p->integral.assign(integral, integral + ncols);
+   p->lb.assign(lb, lb + ncols);
+   p->ub.assign(ub, ub + ncols);
+   p->stream = (int32_t)(s->probs.size() % s->ctx.size());
+   *id = (int32_t)s->probs.size();
+   s->probs.push_back(p);
+   return SAP_OKAY;
+}
+
+int sap_server_warm(sap_server* s, int32_t id)
+{
+   if( s == NULL || id < 0 || id >= (int32_t)s->probs.size() )
+      return SAP_INVALIDDATA;
+   sap_params prm;
+   sap_params_default(&prm);
+   const int r = makeResident(s, s->probs[(size_t)id], &prm);
+   return r == 1 ? SAP_OKAY : (r == 0 ? SAP_NOMEMORY : r);
+}
+
+int sap_server_replay(sap_server* s, const sap_params* params, int32_t mode, int32_t steprounds, int32_t nreq,
+   const double* arrival_s, const int32_t* prob, const int64_t* ovptr, const int32_t* ovcol, const double* ovlb,
+   const double* ovub, int32_t* result, int32_t* rounds, int32_t* nchg, double* latency_ms, int64_t chgcap,
+   int64_t* chgptr, int32_t* chgcol, double* chglb, double* chgub, sap_serve_stats* stats)
+{
+   if( s == NULL || params == NULL || nreq < 0 || (nreq > 0 && (arrival_s == NULL || prob == NULL || ovptr == NULL))
+      || (mode != SAP_SERVE_CONTINUOUS && mode != SAP_SERVE_PLAIN)
+      || (steprounds != SAP_STEP_SHORT && steprounds != SAP_STEP_LONG) || (chgptr != NULL && chgcol == NULL) )
+      return SAP_INVALIDDATA;
+   for( int32_t i = 0; i < nreq; ++i )
+      if( prob[i] < 0 || prob[i] >= (int32_t)s->probs.size() || (i > 0 && arrival_s[i] < arrival_s[i - 1])
+         || ovptr[i + 1] < ovptr[i] )
+         return SAP_INVALIDDATA;
+   memset(&s->st, 0, sizeof(s->st));
+   s->st.requests = nreq;
+   s->st.peak_bytes = s->resident;
+   s->ndone = 0;
+   std::vector<Request> req((size_t)nreq);
+   for( int32_t i = 0; i < nreq; ++i )
+   {
+      req[(size_t)i].arrival = arrival_s[i];
+      req[(size_t)i].prob = prob[i];
+   }
+   const bool wantchg = chgptr != NULL;
+   int rc = SAP_OKAY;
+   const double t0 = nowSeconds();
+   int32_t next = 0;
+   if( mode == SAP_SERVE_PLAIN )
+   {
+      // one request at a time, in arrival order
+      for( int32_t i = 0; i < nreq && rc == SAP_OKAY; ++i )
+      {
+         waitUntil(t0 + arrival_s[i]);
+         Problem* p = s->probs[(size_t)prob[i]];
+         int r = makeResident(s, p, params);
+         if( r != 1 )
+         {
+            rc = r == 0 ? SAP_NOMEMORY : r;
+            break;
+         }
+         p->queue.push_back(i);
+         rc = admit(s, p, req, ovptr, ovcol, ovlb, ovub, 1, t0);
+         s->st.max_running = std::max<int64_t>(s->st.max_running, p->running);
+         while( rc == SAP_OKAY && p->running > 0 )
+         {
+            rc = sap_slots_launch(p->pool, params, steprounds);
+            if( rc == SAP_OKAY )
+            {
+               p->inflight = true;
+               s->st.steps++;
+               rc = harvest(s, p, req, wantchg, t0);
+            }
+         }
+      }
+   }
+   else
+   {
+      std::vector<Problem*> active;
+      while( s->ndone < nreq && rc == SAP_OKAY )
+      {
+         const double now = nowSeconds() - t0;
+         while( next < nreq && arrival_s[next] <= now )
+         {
+            s->probs[(size_t)prob[next]]->queue.push_back(next);
+            ++next;
+         }
+         // admissions (problems with waiting requests, oldest request first)
+         for( Problem* p : s->probs )
+         {
+            if( p->queue.empty() )
+               continue;
+            const int r = makeResident(s, p, params);
+            if( r < 0 )
+            {
+               rc = r;
+               break;
+            }
+            if( r == 1 )
+               rc = admit(s, p, req, ovptr, ovcol, ovlb, ovub, s->slots, t0);
+            if( rc != SAP_OKAY )
+               break;
+         }
+         if( rc != SAP_OKAY )
+            break;
+         // one step for every pool with running slots: launch all, then wait in turn
+         active.clear();
+         int64_t running = 0;
+         for( Problem* p : s->probs )
+            if( p->running > 0 )
+            {
+               running += p->running;
+               rc = sap_slots_launch(p->pool, params, steprounds);
+               if( rc != SAP_OKAY )
+                  break;
+               p->inflight = true;
+               s->st.steps++;
+               active.push_back(p);
+            }
+         s->st.max_running = std::max(s->st.max_running, running);
+         for( Problem* p : active )
+         {
+            const int r = harvest(s, p, req, wantchg, t0);
+            if( rc == SAP_OKAY )
+               rc = r;
+         }
+         if( active.empty() && next < nreq )
+         {
+            bool waiting = false;
+            for( Problem* p : s->probs )
+               waiting = waiting || !p->queue.empty();
+            if( !waiting )
+               waitUntil(t0 + arrival_s[next]);
+         }
+      }
+   }
+   const double t1 = nowSeconds();
+   s->st.wall_s = t1 - t0 - (nreq > 0 ? arrival_s[0] : 0.0);
+   std::vector<double> lat;
+   int64_t tot = 0;
+   for( int32_t i = 0; i < nreq; ++i )
+   {
+      const Request& r = req[(size_t)i];
+      if( result )
+         result[i] = r.result;
+      if( rounds )
+         rounds[i] = r.rounds;
+      if( nchg )
+         nchg[i] = r.nchg;
+      if( latency_ms )
+         latency_ms[i] = r.latency_ms;
+      lat.push_back(r.latency_ms);
+      if( chgptr )
+      {
+         chgptr[i] = tot;
+         tot += (int64_t)r.col.size();
+      }
+   }
+   if( chgptr )
+   {
+      chgptr[nreq] = tot;
+      if( tot > chgcap )
+      {
+         if( rc == SAP_OKAY )
+            rc = SAP_NOMEMORY;
+      }
+      else
+         for( int32_t i = 0; i < nreq; ++i )
+         {
+            const Request& r = req[(size_t)i];
+            for( size_t q = 0; q < r.col.size(); ++q )
+            {
+               chgcol[chgptr[i] + (int64_t)q] = r.col[q];
+               if( chglb )
+                  chglb[chgptr[i] + (int64_t)q] = r.clb[q];
+               if( chgub )
+                  chgub[chgptr[i] + (int64_t)q] = r.cub[q];
+            }
+         }
+   }
+   if( !lat.empty() )
+   {
+      std::sort(lat.begin(), lat.end());
+      auto pct = [&](double f) { return lat[std::min(lat.size() - 1, (size_t)(f * (double)(lat.size() - 1) + 0.5))]; };
+      s->st.p50_ms = pct(0.50);
+      s->st.p99_ms = pct(0.99);
+      s->st.max_ms = lat.back();
+   }
+   if( stats )
+      *stats = s->st;
+   return rc;
+}
+
+}   // extern "C"
