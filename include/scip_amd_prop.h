@@ -93,19 +93,20 @@ int sap_device_count(int32_t* ndevices);
 int sap_context_create(int32_t device, sap_context** ctx);
 int sap_context_free(sap_context* ctx);
 
-/* Row binning and launch shape of the matrices a context creates.  Rows are grouped by
- * length; in the one-vector (dense) work mode the rows of bins 1..tilebins (at most 4, 8,
- * 16, 32, 64 nonzeros for tilebins 1..5) are swept in 64-nonzero tiles, shorter groups of
- * lanes take the other short bins, one wavefront takes a row of up to longrow nonzeros and a
- * whole workgroup a longer one; capscale (percent) scales the workgroup count of each kind
- * of sweep work.  sap_autotune measures candidates on a row sample of a matrix.  No
- * counterpart in SCIP (whose propagation has no launch shape). */
+/* Row binning and launch shape of the matrices a context creates.  Rows are grouped in bins
+ * by length (at most 4, 8, 16, 32, 64 nonzeros for bins 1..5, then longer).  The rows of bins
+ * 1..rowbins are swept one row per lane; in the one-vector (dense) work mode the rows of bins
+ * rowbins+1..tilebins are swept in 64-nonzero tiles; groups of lanes take the other short
+ * bins, one wavefront takes a row of up to longrow nonzeros and a whole workgroup a longer
+ * one; capscale (percent) scales the workgroup count of each kind of sweep work.
+ * sap_autotune measures candidates on a row sample of a matrix.  No counterpart in SCIP
+ * (whose propagation has no launch shape). */
 typedef struct sap_tuning
 {
    int32_t tilebins;    /* 0..5, default 5 */
    int32_t longrow;     /* 64..2^24, default 4096 */
    int32_t capscale;    /* 10..1000, default 100 */
-   int32_t reserved;
+   int32_t rowbins;     /* 0..3, default 0 */
 } sap_tuning;
 void sap_tuning_default(sap_tuning* tuning);
 int sap_context_set_tuning(sap_context* ctx, const sap_tuning* tuning);
@@ -122,8 +123,8 @@ int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int6
    const uint8_t* integral, const double* glb, const double* gub, sap_matrix** mat);
 int sap_matrix_free(sap_matrix* mat);
 /* Autotuner: builds the system restricted to a sample of samplerows rows (all rows if fewer;
- * chosen by seed) once per candidate tuning (tilebins 3..5 x longrow 1024/4096/16384 x
- * capscale 50/100/200), propagates lb/ub with params on it and returns the candidate with
+ * chosen by seed) once per candidate tuning ((rowbins, tilebins) in (0,3) (0,4) (0,5) (2,5)
+ * (3,4) (3,5) x longrow 1024/4096/16384 x capscale 50/100/200), propagates lb/ub with params on it and returns the candidate with
  * the least device time in best (times in ms: best_ms, default_ms for the default tuning).
  * Results are identical under every tuning; only the launch shape differs.  The context's
  * own tuning is left unchanged: apply the result with sap_context_set_tuning. */

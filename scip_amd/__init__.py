@@ -66,10 +66,10 @@ class sap_params(C.Structure):
 
 
 class sap_tuning(C.Structure):
-    _fields_ = [("tilebins", C.c_int32), ("longrow", C.c_int32), ("capscale", C.c_int32), ("reserved", C.c_int32)]
+    _fields_ = [("tilebins", C.c_int32), ("longrow", C.c_int32), ("capscale", C.c_int32), ("rowbins", C.c_int32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class sap_stats(C.Structure):
@@ -329,8 +329,8 @@ class Context:
         self.h = _P()
         _chk(lib().sap_context_create(device, C.byref(self.h)), "sap_context_create")
 
-    def set_tuning(self, tilebins: int = 5, longrow: int = 4096, capscale: int = 100):
-        t = sap_tuning(tilebins, longrow, capscale, 0)
+    def set_tuning(self, tilebins: int = 5, longrow: int = 4096, capscale: int = 100, rowbins: int = 0):
+        t = sap_tuning(tilebins, longrow, capscale, rowbins)
         _chk(lib().sap_context_set_tuning(self.h, C.byref(t)), "sap_context_set_tuning")
 
     def tuning(self):

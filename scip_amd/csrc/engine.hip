@@ -107,7 +107,7 @@ struct Mat
    // independent instances packed block-diagonally: rows and columns carry their instance
    // (segment); verdicts, cutoffs and round counts are kept per (vector, segment) unit
    int32_t nseg;
-   int32_t pad2;
+   int32_t rowbins;         // bins 1..rowbins are swept one row per lane
    const int32_t* rowseg;   // NULL when nseg == 1
    const int32_t* colseg;
 };
@@ -124,6 +124,7 @@ struct Vec
    long long* flb;          // forced lb candidates (single-variable rows)
    long long* fub;
    int32_t* coltag;         // last round a column got a candidate
+   int32_t* chgtag;         // last round a column's bounds changed (its rows are marked after)
    int32_t* rowmark;        // round a row is marked for
    uint8_t* redundant;      // nvec*nrows
    int32_t* vcutoff;        // per unit (vector, segment): infeasibility detected
@@ -1025,6 +1026,127 @@ __device__ __forceinline__ void sweepGroups(const Mat& M, const Vec& V, const Pa
 }
 
 // ------------------------------------------------------------------------------------------
+// sweep: one row per lane (rows of <= MAXL nonzeros).  The canonical tree of the row is formed
+// in registers by a binary counter over positions 0..MAXL-1: position k closes the subtrees
+// of the trailing one bits of k; positions past the row contribute the identity (0, 0).  The
+// row decision runs once per row; the screening pass reloads the row's nonzeros.
+// ------------------------------------------------------------------------------------------
+template <int MAXL>
+__device__ __forceinline__ void sweepRows(const Mat& M, const Vec& V, const Params& P, const Items& I, int lblock,
+   int nblocks, unsigned long long& nnzacc, unsigned long long& rowsacc, WaveQueue& Q)
+{
+   constexpr int LV = (MAXL <= 4) ? 2 : ((MAXL <= 8) ? 3 : 4);
+   const int tid = lblock * SAP_BLOCK + (int)threadIdx.x;
+   const uint32_t nthreads = (uint32_t)nblocks * SAP_BLOCK;
+   const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
+   // wavefront-uniform trip count so that queue pushes stay convergent
+   for( uint32_t it0 = (uint32_t)(tid - lane); it0 < I.n; it0 += nthreads )
+   {
+      const uint32_t it = it0 + (uint32_t)lane;
+      int32_t row = 0, v = 0;
+      const bool live = it < I.n && fetchItem(M, V, I, it, row, v);
+      const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
+      const int64_t s = live ? M.rowptr[row] : 0;
+      const int n = live ? (int)(M.rowptr[row + 1] - s) : 0;
+      const double2 sides = live ? M.sides[row] : make_double2(0.0, 0.0);
+      const bool allbin = live && (M.rowflags[row] & 1) != 0;
+      DD stmn[LV], stmx[LV];
+      DD rmn = { 0.0, 0.0 }, rmx = { 0.0, 0.0 };
+      unsigned long long cnt = 0;
+      double mad = 0.0;
+      bool madinf = false;
+#pragma unroll
+      for( int k = 0; k < MAXL; ++k )
+      {
+         double cmin = 0.0, cmax = 0.0;
+         if( k < n )
+         {
+            const Elem e = loadElem(M, bnd, s + k);
+            int kmin, kmax;
+            double m1;
+            bool mi;
+            elemContrib(P, e, allbin, cmin, cmax, kmin, kmax, m1, mi);
+            cnt += packKinds(kmin, kmax);
+            mad = dmax(mad, m1);
+            madinf = madinf || mi;
+         }
+         DD nmn = { cmin, 0.0 };
+         DD nmx = { cmax, 0.0 };
+         int lvl = 0;
+#pragma unroll
+         for( int l = 0; l < LV; ++l )
+         {
+            if( lvl == l && ((k >> l) & 1) != 0 )
+            {
+               nmn = pairAdd(stmn[l], nmn);
+               nmx = pairAdd(stmx[l], nmx);
+               lvl = l + 1;
+            }
+         }
+         if( k == MAXL - 1 )
+         {
+            rmn = nmn;
+            rmx = nmx;
+         }
+         else
+         {
+#pragma unroll
+            for( int l = 0; l < LV; ++l )
+               if( l == lvl )
+               {
+                  stmn[l] = nmn;
+                  stmx[l] = nmx;
+               }
+         }
+      }
+      RowState rs;
+      rs.minact = rmn;
+      rs.maxact = rmx;
+#pragma unroll
+      for( int q = 0; q < 8; ++q )
+         rs.c[q] = (int)((cnt >> (8 * q)) & 0xFF);
+      rs.mad = madinf ? P.infinity : mad;
+      const RowDecision d = decideRow(P, rs, sides.x, sides.y, n);
+      if( live )
+      {
+         Ctx C;
+         C.M = &M;
+         C.V = &V;
+         C.P = &P;
+         C.v = v;
+         C.round = I.round;
+         C.row = row;
+         if( d.tighten && d.easy )
+            easyRowCutoff(C, rs, sides.x, sides.y);
+         rowVerdict(C, rs, sides.x, sides.y, row);
+         nnzacc += (unsigned long long)n;
+         rowsacc += 1;
+         if( !V.single )
+            noteRound(M, V, v, row, I.round);
+      }
+      const bool tight = live && d.tighten;
+#pragma unroll 1
+      for( int k = 0; k < MAXL; ++k )
+      {
+         const bool act = tight && k < n;
+         if( __ballot(act) == 0 )
+            break;
+         Elem e;
+         e.a = 0.0;
+         e.lb = 0.0;
+         e.ub = 0.0;
+         e.j = 0;
+         e.cf = 0;
+         if( act )
+            e = loadElem(M, bnd, s + k);
+         const bool pass = act && screenElem(P, rs, d, sides.x, sides.y, e);
+         pushQueue(M, V, P, Q, pass, s + k, row, v, rs, d, I.round);
+         maybeDrain(M, V, P, Q, I.round);
+      }
+   }
+}
+
+// ------------------------------------------------------------------------------------------
 // sweep of 64-nonzero tiles (dense mode, rows of 1..64 nonzeros): a wavefront loads one tile,
 // lane i holds nonzero tstart+i at position p of its row; the canonical tree of every row is
 // formed by shifts: at width w a lane at a position aligned to 2w adds the partial held by the
@@ -1527,9 +1649,24 @@ __global__ __launch_bounds__(SAP_BLOCK, 4) void k_sweep(Mat M, Vec V, Params P, 
       }
       break;
    case 1: sweepGroups<4>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
-   case 2: sweepGroups<4>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
-   case 3: sweepGroups<8>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
-   case 4: sweepGroups<16>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
+   case 2:
+      if( M.rowbins >= 1 )
+         sweepRows<4>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+      else
+         sweepGroups<4>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+      break;
+   case 3:
+      if( M.rowbins >= 2 )
+         sweepRows<8>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+      else
+         sweepGroups<8>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+      break;
+   case 4:
+      if( M.rowbins >= 3 )
+         sweepRows<16>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+      else
+         sweepGroups<16>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+      break;
    case 5: sweepGroups<32>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
    case 6: sweepGroups<64>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
    case 7: sweepWaves(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
@@ -1550,7 +1687,7 @@ __global__ __launch_bounds__(SAP_BLOCK, 4) void k_sweep(Mat M, Vec V, Params P, 
 // the column step for one (column, vector) whose candidate slots may hold values;
 // returns the number of changed bounds
 __device__ __forceinline__ int columnStep(const Mat& M, const Vec& V, const Params& P, int32_t j, int32_t v,
-   int32_t next, int npar, bool& anymarked)
+   int32_t next)
 {
    Status* st = V.st;
    const size_t flat = (size_t)v * (size_t)M.ncols + (size_t)j;
@@ -1646,9 +1783,16 @@ __device__ __forceinline__ int columnStep(const Mat& M, const Vec& V, const Para
       }
    }
    // a vector stops after its own maxrounds rounds: rows are not marked beyond
-   if( P.maxrounds >= 0 && next - V.vstart[v] + 1 > P.maxrounds )
-      return (chl ? 1 : 0) + (chu ? 1 : 0);
-   // mark the rows of the column for the next round (eventExecLinear marks them)
+   if( !(P.maxrounds >= 0 && next - V.vstart[v] + 1 > P.maxrounds) )
+      V.chgtag[flat] = next - 1;   // k_mark marks its rows unless the unit is cut off
+   return (chl ? 1 : 0) + (chu ? 1 : 0);
+}
+
+// mark the rows of a changed column for the next round (eventExecLinear marks them)
+__device__ __forceinline__ void markColumn(const Mat& M, const Vec& V, int32_t j, int32_t v, int32_t next, int npar,
+   bool& anymarked)
+{
+   Status* st = V.st;
    for( int64_t q = M.colptr[j]; q < M.colptr[j + 1]; ++q )
    {
       const int32_t row = M.rowidx[q];
@@ -1669,7 +1813,6 @@ __device__ __forceinline__ int columnStep(const Mat& M, const Vec& V, const Para
          waveAppendRow(st, V.wl[npar], npar, first, M.rowbin[row], make_int2(row, v));
       }
    }
-   return (chl ? 1 : 0) + (chu ? 1 : 0);
 }
 
 __global__ __launch_bounds__(SAP_BLOCK) void k_columns(Mat M, Vec V, Params P)
@@ -1679,9 +1822,7 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_columns(Mat M, Vec V, Params P)
       return;
    const int32_t round = st->round;
    const int32_t next = round + 1;
-   const int npar = next & 1;
    unsigned long long nchg = 0;
-   bool anymarked = false;
    if( V.dense )
    {
       const size_t n = (size_t)V.nvec * (size_t)M.ncols;
@@ -1691,7 +1832,7 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_columns(Mat M, Vec V, Params P)
             continue;
          const int32_t v = (int32_t)(flat / (size_t)M.ncols);
          const int32_t j = (int32_t)(flat - (size_t)v * (size_t)M.ncols);
-         nchg += (unsigned long long)columnStep(M, V, P, j, v, next, npar, anymarked);
+         nchg += (unsigned long long)columnStep(M, V, P, j, v, next);
       }
    }
    else
@@ -1700,12 +1841,50 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_columns(Mat M, Vec V, Params P)
       for( uint32_t idx = blockIdx.x * SAP_BLOCK + threadIdx.x; idx < n; idx += gridDim.x * SAP_BLOCK )
       {
          const int2 t = V.touched[idx];
-         nchg += (unsigned long long)columnStep(M, V, P, t.x, t.y, next, npar, anymarked);
+         nchg += (unsigned long long)columnStep(M, V, P, t.x, t.y, next);
+      }
+   }
+   blockStats(nchg, 0, st->slot_chg, NULL);
+}
+
+// second half of the column step: rows of the columns changed in this round are marked for the
+// next one, unless their unit was cut off in the round (acceptance and marking are separate
+// launches so that a round ending in infeasibility marks nothing)
+__global__ __launch_bounds__(SAP_BLOCK) void k_mark(Mat M, Vec V)
+{
+   Status* st = V.st;
+   if( st->done || (V.single && st->cutoff) )
+      return;
+   const int32_t round = st->round;
+   const int32_t next = round + 1;
+   const int npar = next & 1;
+   bool anymarked = false;
+   if( V.dense )
+   {
+      const size_t n = (size_t)V.nvec * (size_t)M.ncols;
+      for( size_t flat = blockIdx.x * (size_t)SAP_BLOCK + threadIdx.x; flat < n; flat += (size_t)gridDim.x * SAP_BLOCK )
+      {
+         if( V.chgtag[flat] != round )
+            continue;
+         const int32_t v = (int32_t)(flat / (size_t)M.ncols);
+         const int32_t j = (int32_t)(flat - (size_t)v * (size_t)M.ncols);
+         if( V.vcutoff[unitOfCol(M, v, j)] == 0 )
+            markColumn(M, V, j, v, next, npar, anymarked);
+      }
+   }
+   else
+   {
+      const uint32_t n = st->touched;
+      for( uint32_t idx = blockIdx.x * SAP_BLOCK + threadIdx.x; idx < n; idx += gridDim.x * SAP_BLOCK )
+      {
+         const int2 t = V.touched[idx];
+         const size_t flat = (size_t)t.y * (size_t)M.ncols + (size_t)t.x;
+         if( V.chgtag[flat] == round && V.vcutoff[unitOfCol(M, t.y, t.x)] == 0 )
+            markColumn(M, V, t.x, t.y, next, npar, anymarked);
       }
    }
    if( __ballot(anymarked) != 0 && ((int)threadIdx.x & (SAP_WAVE - 1)) == 0 && st->anymark[npar] == 0 )
       st->anymark[npar] = 1;
-   blockStats(nchg, 0, st->slot_chg, NULL);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2323,7 +2502,7 @@ static Mat toMat(const sap_matrix* m)
    M.trow0 = m->trow0;
    M.tlev = m->tlev;
    M.nseg = m->nseg;
-   M.pad2 = 0;
+   M.rowbins = m->tuning.rowbins;
    M.rowseg = m->nseg > 1 ? m->rowseg : NULL;
    M.colseg = m->nseg > 1 ? m->colseg : NULL;
    return M;
@@ -2381,13 +2560,13 @@ void sap_tuning_default(sap_tuning* t)
    t->tilebins = 5;
    t->longrow = 4096;
    t->capscale = 100;
-   t->reserved = 0;
+   t->rowbins = 0;
 }
 
 int sap_context_set_tuning(sap_context* c, const sap_tuning* t)
 {
    if( c == NULL || t == NULL || t->tilebins < 0 || t->tilebins > 5 || t->longrow < 64 || t->longrow > (1 << 24)
-      || t->capscale < 10 || t->capscale > 1000 )
+      || t->capscale < 10 || t->capscale > 1000 || t->rowbins < 0 || t->rowbins > 3 )
       return SAP_INVALIDDATA;
    c->tuning = *t;
    return SAP_OKAY;
@@ -2667,7 +2846,8 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
             ++v;
          return (uint8_t)v;
       };
-      for( int32_t r = binstart[1]; r < binstart[1 + tune.tilebins]; ++r )
+      const int32_t tfirst = binstart[1 + std::min(tune.rowbins, tune.tilebins)];
+      for( int32_t r = tfirst; r < binstart[1 + tune.tilebins]; ++r )
       {
          const int L = (int)(prowptr[(size_t)r + 1] - prowptr[(size_t)r]);
          if( cnt == 0 || cnt + L > 64 )
@@ -2692,7 +2872,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
          tmask.push_back(mask);
          tlev.push_back(levels(maxl));
       }
-      tstart.push_back(prowptr[(size_t)binstart[1 + tune.tilebins]]);
+      tstart.push_back(prowptr[(size_t)std::max(tfirst, binstart[1 + tune.tilebins])]);
    }
 
    HIPCHK(hipSetDevice(ctx->device));
@@ -2844,18 +3024,20 @@ int sap_autotune(sap_context* ctx, int64_t nrows, int64_t ncols, const int64_t* 
    const sap_tuning saved = ctx->tuning;
    sap_tuning def;
    sap_tuning_default(&def);
-   static const int32_t tbs[] = { 3, 4, 5 };
+   static const int32_t rbs[] = { 0, 0, 0, 2, 3, 3 };
+   static const int32_t tbs[] = { 3, 4, 5, 5, 4, 5 };
    static const int32_t lrs[] = { 1024, 4096, 16384 };
    static const int32_t cps[] = { 50, 100, 200 };
    std::vector<double> wl((size_t)ncols), wu((size_t)ncols);
    double bestt = 1e300, deft = -1.0;
    sap_tuning bt = def;
    int rc = SAP_OKAY;
-   for( int a = 0; a < 3 && rc == SAP_OKAY; ++a )
+   for( int a = 0; a < 6 && rc == SAP_OKAY; ++a )
       for( int l = 0; l < 3 && rc == SAP_OKAY; ++l )
          for( int c = 0; c < 3 && rc == SAP_OKAY; ++c )
          {
             sap_tuning t = def;
+            t.rowbins = rbs[a];
             t.tilebins = tbs[a];
             t.longrow = lrs[l];
             t.capscale = cps[c];
@@ -2876,7 +3058,8 @@ int sap_autotune(sap_context* ctx, int64_t nrows, int64_t ncols, const int64_t* 
                   tmin = std::min(tmin, st.device_ms);
             }
             sap_matrix_free(m);
-            if( t.tilebins == def.tilebins && t.longrow == def.longrow && t.capscale == def.capscale )
+            if( t.tilebins == def.tilebins && t.longrow == def.longrow && t.capscale == def.capscale
+               && t.rowbins == def.rowbins )
                deft = tmin;
             if( tmin < bestt )
             {
@@ -2941,6 +3124,7 @@ static void boundsRelease(sap_bounds* b)
    (void)hipFree(V.flb);
    (void)hipFree(V.fub);
    (void)hipFree(V.coltag);
+   (void)hipFree(V.chgtag);
    (void)hipFree(V.rowmark);
    (void)hipFree(V.redundant);
    (void)hipFree(V.vcutoff);
@@ -3050,6 +3234,7 @@ int sap_bounds_create_mode(sap_matrix* m, int32_t nvec, int32_t mode, sap_bounds
       BALLOC(V.fub, NC);
    }
    BALLOC(V.coltag, NC);
+   BALLOC(V.chgtag, NC);
    BALLOC(V.rowmark, NR);
    BALLOC(V.redundant, NR);
    BALLOC(V.vcutoff, (size_t)nvec * (size_t)m->nseg);
@@ -3089,6 +3274,7 @@ int sap_bounds_create_mode(sap_matrix* m, int32_t nvec, int32_t mode, sap_bounds
       k_fill_i64<<<gridFor(NC), SAP_BLOCK, 0, s>>>(V.fub, KEY_POS_INF, NC);
    }
    k_fill_i32<<<gridFor(NC), SAP_BLOCK, 0, s>>>(V.coltag, 0, NC);
+   k_fill_i32<<<gridFor(NC), SAP_BLOCK, 0, s>>>(V.chgtag, 0, NC);
    k_fill_i32<<<gridFor(NR), SAP_BLOCK, 0, s>>>(V.rowmark, 0, NR);
    k_fill_i32<<<gridFor((size_t)nvec), SAP_BLOCK, 0, s>>>(V.vpend, 0, (size_t)nvec);
    k_fill_i32<<<gridFor((size_t)nvec), SAP_BLOCK, 0, s>>>(V.vstart, 0, (size_t)nvec);
@@ -3186,19 +3372,21 @@ static Layout makeLayout(const sap_bounds* b)
    static const int kPerBlock[SAP_NKINDS] = { SAP_BLOCK / SAP_WAVE, SAP_BLOCK / 4, SAP_BLOCK / 4, SAP_BLOCK / 8,
       SAP_BLOCK / 16, SAP_BLOCK / 32, SAP_BLOCK / 64, SAP_BLOCK / SAP_WAVE, 1 };
    static const int kCap[SAP_NKINDS] = { 4096, 2048, 2048, 2048, 2048, 2048, 2048, 2048, 1024 };
-   const int tb = m->tuning.tilebins;
+   const int tb = m->tuning.tilebins, rb = m->tuning.rowbins;
    int acc = 0;
    for( int q = 0; q < SAP_NKINDS; ++q )
    {
       L.blockstart[q] = acc;
+      const int bin = q - 1;
       size_t items;
       if( q == 0 )
          items = b->V.dense ? nvec * (size_t)(m->ntiles > 0 ? m->ntiles : (m->binstart[1] > m->binstart[0] ? 1 : 0)) : 0;
-      else if( b->V.dense && q <= 1 + tb )
-         items = 0;   // empty rows and bins 1..tilebins go through the tiles in dense mode
+      else if( b->V.dense && (bin == 0 || (bin > rb && bin <= tb)) )
+         items = 0;   // empty rows and the tiled bins go through the tiles in dense mode
       else
          items = nvec * (size_t)(m->binstart[q] - m->binstart[q - 1]);
-      size_t blocks = (items + kPerBlock[q] - 1) / kPerBlock[q];
+      const int per = (bin >= 1 && bin <= rb) ? SAP_BLOCK : kPerBlock[q];   // one row per lane
+      size_t blocks = (items + per - 1) / per;
       const size_t cap = std::max<size_t>(64, (size_t)kCap[q] * (size_t)m->tuning.capscale / 100);
       if( blocks > cap )
          blocks = cap;
@@ -3253,6 +3441,7 @@ static void enqueueRound(sap_bounds* b, const Mat& M, const Params& P, const Lay
    if( e1 != NULL )
       (void)hipEventRecord(e1, s);
    k_columns<<<colgrid, SAP_BLOCK, 0, s>>>(M, b->V, P);
+   k_mark<<<colgrid, SAP_BLOCK, 0, s>>>(M, b->V);
 }
 
 // host-driven loop: one synchronisation per round
