@@ -90,6 +90,7 @@ struct Mat
    const double* vals;
    const double2* sides;    // (lhs, rhs)
    const uint8_t* colflags; // bit0 integral, bit1 binary
+   const uint8_t* nzflags;  // colflags of each nonzero's column, in nonzero order (read coalesced)
    const uint8_t* rowflags; // bit0 all-binary row
    const uint8_t* rowbin;
    const int64_t* colptr;   // CSC (bin-ordered row ids) for row marking
@@ -725,7 +726,7 @@ __device__ __forceinline__ Elem loadElem(const Mat& M, const double2* bnd, int64
    double2 b = bnd[e.j];
    e.lb = b.x;
    e.ub = b.y;
-   e.cf = M.colflags[e.j];
+   e.cf = M.nzflags[k];
    return e;
 }
 
@@ -2324,6 +2325,7 @@ struct sap_matrix
    double* vals;
    double2* sides;
    uint8_t* colflags;
+   uint8_t* nzflags;
    uint8_t* rowflags;
    uint8_t* rowbin;
    int64_t* colptr;
@@ -2489,6 +2491,7 @@ static Mat toMat(const sap_matrix* m)
    M.vals = m->vals;
    M.sides = m->sides;
    M.colflags = m->colflags;
+   M.nzflags = m->nzflags;
    M.rowflags = m->rowflags;
    M.rowbin = m->rowbin;
    M.colptr = m->colptr;
@@ -2656,6 +2659,7 @@ static void matrixRelease(sap_matrix* m)
    (void)hipFree(m->vals);
    (void)hipFree(m->sides);
    (void)hipFree(m->colflags);
+   (void)hipFree(m->nzflags);
    (void)hipFree(m->rowflags);
    (void)hipFree(m->rowbin);
    (void)hipFree(m->colptr);
@@ -2795,6 +2799,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
    }
    std::vector<int32_t> pcol((size_t)nnz);
    std::vector<double> pval((size_t)nnz);
+   std::vector<uint8_t> pflag((size_t)nnz);
    std::vector<double2> sides((size_t)nrows);
    std::vector<uint8_t> rowflags((size_t)nrows), rowbin((size_t)nrows);
    std::vector<int64_t> colptr((size_t)ncols + 1, 0);
@@ -2810,6 +2815,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
             const int32_t c = colidx[st0 + k];
             pcol[(size_t)(o + k)] = c;
             pval[(size_t)(o + k)] = vals[st0 + k];
+            pflag[(size_t)(o + k)] = colflags[(size_t)c];
             if( !(colflags[(size_t)c] & 2) )
                allbin = false;
             __atomic_fetch_add(&colptr[(size_t)c + 1], (int64_t)1, __ATOMIC_RELAXED);
@@ -2905,6 +2911,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
    MALLOC_OR_FAIL(m->vals, nnz);
    MALLOC_OR_FAIL(m->sides, nrows);
    MALLOC_OR_FAIL(m->colflags, ncols);
+   MALLOC_OR_FAIL(m->nzflags, nnz);
    MALLOC_OR_FAIL(m->rowflags, nrows);
    MALLOC_OR_FAIL(m->rowbin, nrows);
    MALLOC_OR_FAIL(m->colptr, ncols + 1);
@@ -2929,6 +2936,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
    {
       HIPCHK(hipMemcpyAsync(m->colidx, pcol.data(), sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice, s));
       HIPCHK(hipMemcpyAsync(m->vals, pval.data(), sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(m->nzflags, pflag.data(), (size_t)nnz, hipMemcpyHostToDevice, s));
       HIPCHK(hipMemcpyAsync(m->rowidx, rowidx.data(), sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice, s));
    }
    if( nrows > 0 )

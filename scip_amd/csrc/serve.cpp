@@ -75,7 +75,7 @@ namespace {
 int64_t footprint(const Problem* p, int32_t slots)
 {
    const int64_t nnz = p->rowptr.empty() ? 0 : p->rowptr.back();
-   const int64_t mat = nnz * (8 + 4 + 4) + nnz / 4 + p->nrows * (8 + 16 + 1 + 1 + 1 + 4 + 4 + 8) + p->ncols * (1 + 8 + 4)
+   const int64_t mat = nnz * (8 + 4 + 4 + 1) + nnz / 4 + p->nrows * (8 + 16 + 1 + 1 + 1 + 4 + 4 + 8) + p->ncols * (1 + 8 + 4)
       + 4096;
    const int64_t vec1 = p->ncols * (16 + 16 + 16 + 4 + 4 + 32) + p->nrows * (4 + 1 + 16);
    const int64_t pool = (int64_t)slots * (p->ncols * (16 + 16 + 16 + 4 + 4 + 8 + 4 + 4 + 1) + p->nrows * (4 + 1 + 16))
