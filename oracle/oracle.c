@@ -22,6 +22,23 @@
  *   src/scip/cons_linear.c:17162-17279 eventExecLinear
  *   src/scip/solve.c:723-795           propagateDomains round loop
  *   src/scip/cons.c:7309-7366          delayed (un)marking of constraints for propagation
+ *
+ * What pins each part (tests/test_oracle_arith.py, tests/test_oracle_prop.py):
+ *   numerics, TwoSum/Sum21/TwoProd   bit-exact against SCIP's own dbldblarith.h (oracle/_ref) and
+ *                                    the known-answer vectors of tests/src/misc/dbldblarith.c
+ *   canonical tree                   exact on integers, tiling/padding invariance, fsum accuracy
+ *   single-row tightening, acceptance, infeasibility, redundancy, maxactdelta, force rule:
+ *                                    closed-form cases computed by hand
+ *   propagation fixpoints            brute-force soundness on tiny integer systems, completeness
+ *                                    on infeasible binary systems, Jacobi = SCIP order on binary
+ *                                    systems (unique fixpoint), mutual fixpoints otherwise
+ * Parity unpinned (no reference vector or closed form decides them; SCIP itself cannot be
+ * built here):
+ *   - orc_propagate_scip's exact general-integer / continuous bounds where the row order and
+ *     the staleness of SCIP's incremental updates matter (pinned only by soundness and the
+ *     mutual-fixpoint property);
+ *   - the reliable residual recompute (reliableResidual, cons_linear.c:2580): exercised, but
+ *     no closed-form case separates it from the double-double residual.
  */
 #include "oracle.h"
 
