@@ -104,8 +104,8 @@ int sap_context_free(sap_context* ctx);
 typedef struct sap_tuning
 {
    int32_t tilebins;    /* 0..5, default 5 */
-   int32_t longrow;     /* 64..2^24, default 4096 */
-   int32_t capscale;    /* 10..1000, default 100 */
+   int32_t longrow;     /* 64..2^24, default 1024 */
+   int32_t capscale;    /* 10..1000, default 50 */
    int32_t rowbins;     /* 0..3, default 0 */
 } sap_tuning;
 void sap_tuning_default(sap_tuning* tuning);

@@ -11,8 +11,10 @@ raises, and every propagation call goes through the device engine.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -190,6 +192,20 @@ EXPORTS = {
 
 _lib = None
 
+# live handles, closed children-first when the interpreter exits (a context outlives its
+# matrices, a matrix its bound sets; finalizers alone run in no particular order)
+_LIVE = {k: weakref.WeakSet() for k in ("server", "bounds", "matrix", "context")}
+
+
+@atexit.register
+def _close_live_handles():
+    for kind in ("server", "bounds", "matrix", "context"):
+        for obj in list(_LIVE[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
 
 def lib() -> C.CDLL:
     """Load libscipamd.so (raises if it has not been built)."""
@@ -328,8 +344,9 @@ class Context:
     def __init__(self, device: int = 0):
         self.h = _P()
         _chk(lib().sap_context_create(device, C.byref(self.h)), "sap_context_create")
+        _LIVE["context"].add(self)
 
-    def set_tuning(self, tilebins: int = 5, longrow: int = 4096, capscale: int = 100, rowbins: int = 0):
+    def set_tuning(self, tilebins: int = 5, longrow: int = 1024, capscale: int = 50, rowbins: int = 0):
         t = sap_tuning(tilebins, longrow, capscale, rowbins)
         _chk(lib().sap_context_set_tuning(self.h, C.byref(t)), "sap_context_set_tuning")
 
@@ -379,6 +396,7 @@ class Matrix:
         rp, ci, va, lh, rh, it, lb, ub = self._keep
         _chk(lib().sap_matrix_create(ctx.h, inst.nrows, inst.ncols, _i64p(rp), _i32p(ci), _dp(va), _dp(lh), _dp(rh),
                                      _u8p(it), _dp(lb), _dp(ub), C.byref(self.h)), "sap_matrix_create")
+        _LIVE["matrix"].add(self)
 
     def info(self):
         a, b, c, d = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
@@ -449,6 +467,7 @@ class PackedMatrix(Matrix):
         _chk(lib().sap_matrix_create_packed(ctx.h, len(self.parts), _i64p(self.rowoff), _i64p(self.coloff),
                                             inst.nrows, inst.ncols, _i64p(rp), _i32p(ci), _dp(va), _dp(lh), _dp(rh),
                                             _u8p(it), _dp(lb), _dp(ub), C.byref(self.h)), "sap_matrix_create_packed")
+        _LIVE["matrix"].add(self)
 
     def propagate_packed(self, lb=None, ub=None, params: sap_params | None = None):
         """Returns (lb, ub, result[ninst], rounds[ninst], stats)."""
@@ -478,6 +497,7 @@ class Bounds:
         self.nvec = nvec
         self.h = _P()
         _chk(lib().sap_bounds_create_mode(mat.h, nvec, mode, C.byref(self.h)), "sap_bounds_create_mode")
+        _LIVE["bounds"].add(self)
 
     def upload(self, lb, ub, v: int = -1):
         lb = np.ascontiguousarray(lb, dtype=np.float64)
@@ -623,6 +643,7 @@ class Server:
         self.h = _P()
         self.slots = slots
         _chk(lib().sap_server_create(device, nstreams, hbm_budget, slots, C.byref(self.h)), "sap_server_create")
+        _LIVE["server"].add(self)
 
     def add_problem(self, inst: Instance) -> int:
         rp, ci, va, lh, rh, it, lb, ub = [np.ascontiguousarray(a) for a in (

@@ -678,45 +678,100 @@ struct RowDecision
    bool tighten;
    bool easy;
    bool force;
+   bool cut;         // infeasible (propagateCons / tightenVarBoundsEasy tests)
+   bool redundant;   // locally redundant
 };
 
+// minimal / maximal activity with delta 0 for goodrelax FALSE and TRUE (getMinActivity /
+// getMaxActivity above); the double-double part is shared, the cases as in those functions
+__device__ __forceinline__ void minActBoth(const Params& p, DD act, int pinf, int ninf, int phuge, int nhuge,
+   double& resF, bool& infF, double& resT)
+{
+   const DD t = sum21(act, -0.0);
+   const double plain = ddVal(t);
+   const double relaxed = phuge > 0 ? ddVal(sum21(t, phuge * p.hugeval)) : plain;
+   if( ninf > 0 || (pinf == 0 && nhuge > 0) )
+   {
+      resF = resT = -p.infinity;
+      infF = true;
+   }
+   else if( pinf > 0 )
+   {
+      resF = resT = p.infinity;
+      infF = true;
+   }
+   else
+   {
+      resT = relaxed;
+      infF = phuge > 0;
+      resF = infF ? -p.infinity : plain;
+   }
+}
+
+__device__ __forceinline__ void maxActBoth(const Params& p, DD act, int pinf, int ninf, int phuge, int nhuge,
+   double& resF, bool& infF, double& resT)
+{
+   const DD t = sum21(act, -0.0);
+   const double plain = ddVal(t);
+   const double relaxed = nhuge > 0 ? ddVal(sum21(t, -nhuge * p.hugeval)) : plain;
+   if( pinf > 0 || (ninf == 0 && phuge > 0) )
+   {
+      resF = resT = p.infinity;
+      infF = true;
+   }
+   else if( ninf > 0 )
+   {
+      resF = resT = -p.infinity;
+      infF = true;
+   }
+   else
+   {
+      resT = relaxed;
+      infF = nhuge > 0;
+      resF = infF ? p.infinity : plain;
+   }
+}
+
+// row-level decisions of tightenBounds (cons_linear.c:7014-7086) together with the row's
+// verdict: the cutoff tests of tightenVarBoundsEasy (cons_linear.c:5450, :5504) and the
+// infeasibility / redundancy test at the end of propagateCons (cons_linear.c:7717-7753)
 __device__ __forceinline__ RowDecision decideRow(const Params& p, const RowState& rs, double lhs, double rhs, int64_t n)
 {
    RowDecision d;
    int infmin = rs.c[0] + rs.c[1] + rs.c[2] + rs.c[3];
    int infmax = rs.c[4] + rs.c[5] + rs.c[6] + rs.c[7];
+   double minF, maxF, minT, maxT;
+   bool f1, f2;
+   minActBoth(p, rs.minact, rs.c[0], rs.c[1], rs.c[2], rs.c[3], minF, f1, minT);
+   maxActBoth(p, rs.maxact, rs.c[4], rs.c[5], rs.c[6], rs.c[7], maxF, f2, maxT);
    d.force = (n == 1);
    d.tighten = n > 0 && !(infmin > 1 && infmax > 1);
    if( d.tighten && isFeasZero(p, rs.mad) )
       d.tighten = false;
    if( d.tighten && !isInf(p, rs.mad) )
    {
-      double minact, maxact;
-      bool t1, t2, f1, f2;
-      getMinAct(p, rs.minact, rs.c[0], rs.c[1], rs.c[2], rs.c[3], 0.0, false, minact, t1, f1);
-      getMaxAct(p, rs.maxact, rs.c[4], rs.c[5], rs.c[6], rs.c[7], 0.0, false, maxact, t2, f2);
-      double slack = (isInf(p, rhs) || f1) ? p.infinity : (rhs - minact);
-      double surplus = (isInf(p, -lhs) || f2) ? p.infinity : (maxact - lhs);
+      double slack = (isInf(p, rhs) || f1) ? p.infinity : (rhs - minF);
+      double surplus = (isInf(p, -lhs) || f2) ? p.infinity : (maxF - lhs);
       if( isLE(p, rs.mad, dmin(slack, surplus)) )
          d.tighten = false;
    }
    d.easy = isLT(p, rs.mad, p.maxeasy);
+   const bool easycut = d.tighten && d.easy
+      && ((!isInf(p, rhs) && isFeasLT(p, rhs, ddVal(rs.minact))) || (!isInf(p, -lhs) && isFeasLT(p, ddVal(rs.maxact), lhs)));
+   d.cut = easycut || isFeasGT(p, minT, rhs) || isFeasLT(p, maxT, lhs);
+   d.redundant = !d.cut && isGE(p, minT, lhs) && isLE(p, maxT, rhs);
    return d;
 }
 
-// infeasibility / redundancy test at the end of propagateCons (cons_linear.c:7717-7753)
-__device__ __forceinline__ void rowVerdict(const Ctx& C, const RowState& rs, double lhs, double rhs, int32_t row)
+// the row's verdict (computed by decideRow) applied by one lane of the row
+__device__ __forceinline__ void rowOutcome(const Ctx& C, const RowDecision& d)
 {
-   const Params& p = *C.P;
-   double minact, maxact;
-   bool t1, t2, f1, f2;
-   getMinAct(p, rs.minact, rs.c[0], rs.c[1], rs.c[2], rs.c[3], 0.0, true, minact, t1, f1);
-   getMaxAct(p, rs.maxact, rs.c[4], rs.c[5], rs.c[6], rs.c[7], 0.0, true, maxact, t2, f2);
-   if( isFeasGT(p, minact, rhs) || isFeasLT(p, maxact, lhs) )
+   if( d.cut )
       setCutoff(C);
-   else if( isGE(p, minact, lhs) && isLE(p, maxact, rhs) )
-      C.V->redundant[(size_t)C.v * (size_t)C.M->nrows + (size_t)row] = 1;
+   else if( d.redundant )
+      C.V->redundant[(size_t)C.v * (size_t)C.M->nrows + (size_t)C.row] = 1;
 }
+
 
 __device__ __forceinline__ Elem loadElem(const Mat& M, const double2* bnd, int64_t k)
 {
@@ -893,13 +948,6 @@ __device__ __forceinline__ void maybeDrain(const Mat& M, const Vec& V, const Par
       drainQueue(M, V, P, Q, SAP_WAVE, round);
 }
 
-// the row-level cutoff tests of tightenVarBoundsEasy (cons_linear.c:5450, :5504)
-__device__ __forceinline__ void easyRowCutoff(const Ctx& C, const RowState& rs, double lhs, double rhs)
-{
-   const Params& p = *C.P;
-   if( (!isInf(p, rhs) && isFeasLT(p, rhs, ddVal(rs.minact))) || (!isInf(p, -lhs) && isFeasLT(p, ddVal(rs.maxact), lhs)) )
-      setCutoff(C);
-}
 
 // ------------------------------------------------------------------------------------------
 // work items of one bin: dense mode enumerates (vector, row) over the bin's row range and
@@ -1014,9 +1062,7 @@ __device__ __forceinline__ void sweepGroups(const Mat& M, const Vec& V, const Pa
       pushQueue(M, V, P, Q, pass, s + glane, row, v, rs, d, I.round);
       if( live && glane == 0 )
       {
-         if( d.tighten && d.easy )
-            easyRowCutoff(C, rs, sides.x, sides.y);
-         rowVerdict(C, rs, sides.x, sides.y, row);
+         rowOutcome(C, d);
          nnzacc += (unsigned long long)n;
          rowsacc += 1;
          if( !V.single )
@@ -1117,9 +1163,7 @@ __device__ __forceinline__ void sweepRows(const Mat& M, const Vec& V, const Para
          C.v = v;
          C.round = I.round;
          C.row = row;
-         if( d.tighten && d.easy )
-            easyRowCutoff(C, rs, sides.x, sides.y);
-         rowVerdict(C, rs, sides.x, sides.y, row);
+         rowOutcome(C, d);
          nnzacc += (unsigned long long)n;
          rowsacc += 1;
          if( !V.single )
@@ -1259,9 +1303,7 @@ __device__ __forceinline__ void sweepTiles(const Mat& M, const Vec& V, const Par
       pushQueue(M, V, P, Q, pass, e0 + lane, row, v, rs, d, I.round);
       if( marked && p == 0 )
       {
-         if( d.tighten && d.easy )
-            easyRowCutoff(C, rs, sides.x, sides.y);
-         rowVerdict(C, rs, sides.x, sides.y, row);
+         rowOutcome(C, d);
          nnzacc += (unsigned long long)L;
          rowsacc += 1;
          if( !V.single )
@@ -1455,9 +1497,7 @@ __device__ __forceinline__ void sweepWaves(const Mat& M, const Vec& V, const Par
          candidatesRange(M, V, P, Q, rs, d, sides.x, sides.y, bnd, s, n, 0, nch, row, v, I.round);
       if( lane == 0 )
       {
-         if( d.tighten && d.easy )
-            easyRowCutoff(C, rs, sides.x, sides.y);
-         rowVerdict(C, rs, sides.x, sides.y, row);
+         rowOutcome(C, d);
          nnzacc += (unsigned long long)n;
          rowsacc += 1;
          if( !V.single )
@@ -1526,9 +1566,7 @@ __device__ __forceinline__ void sweepBlocks(const Mat& M, const Vec& V, const Pa
          candidatesRange(M, V, P, Q, rs, d, sides.x, sides.y, bnd, s, n, c0, mine, row, v, I.round);
       if( threadIdx.x == 0 )
       {
-         if( d.tighten && d.easy )
-            easyRowCutoff(C, rs, sides.x, sides.y);
-         rowVerdict(C, rs, sides.x, sides.y, row);
+         rowOutcome(C, d);
          nnzacc += (unsigned long long)n;
          rowsacc += 1;
          if( !V.single )
@@ -2309,6 +2347,7 @@ struct sap_context
    int32_t device;
    int32_t wallclock_khz;   // rate of wall_clock64() on the device
    sap_tuning tuning;       // applied to matrices created on this context
+   int64_t nmatrices;       // live matrices (a context outlives its matrices)
    hipStream_t stream;
    hipEvent_t ev[4];
    Status* hstatus;   // pinned
@@ -2344,6 +2383,7 @@ struct sap_matrix
    std::vector<int64_t>* rowoff;   // host: segment row / column ranges (caller order)
    std::vector<int64_t>* coloff;
    sap_bounds* single;   // cached one-vector state for sap_propagate
+   int64_t nbounds;      // live caller-created bound sets (a matrix outlives them)
 };
 
 // rounds captured into one hipGraph: a short graph for the first launch of a call (many
@@ -2401,6 +2441,7 @@ struct sap_bounds
    int32_t* hpend;       // pinned: vpend then vcutoff after the last step (2*nvec)
    hipEvent_t evs[2];    // around a step
    int64_t gen;          // bumped by every write to the vectors' bounds outside the change log
+   int32_t internal;     // the matrix's own one-vector state (not counted in nbounds)
    // per vector: the base vector it was last started from; while that base is unchanged, only
    // the columns of the change log and of the overlay differ from it
    std::vector<const sap_bounds*>* srcb;
@@ -2561,8 +2602,8 @@ void sap_tuning_default(sap_tuning* t)
    if( t == NULL )
       return;
    t->tilebins = 5;
-   t->longrow = 4096;
-   t->capscale = 100;
+   t->longrow = 1024;
+   t->capscale = 50;
    t->rowbins = 0;
 }
 
@@ -2642,6 +2683,11 @@ int sap_context_free(sap_context* c)
 {
    if( c == NULL )
       return SAP_OKAY;
+   if( c->nmatrices > 0 )
+   {
+      fprintf(stderr, "[scip_amd] sap_context_free: %ld matrices still live; free them first\n", (long)c->nmatrices);
+      return SAP_INVALIDCALL;
+   }
    (void)hipSetDevice(c->device);
    (void)hipStreamSynchronize(c->stream);
    for( int i = 0; i < 4; ++i )
@@ -2968,6 +3014,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
       HIPCHK(hipStreamSynchronize(s));
    }
    HIPCHK(hipStreamSynchronize(s));
+   ctx->nmatrices++;
    *out = m;
    return SAP_OKAY;
 }
@@ -3099,10 +3146,16 @@ int sap_matrix_free(sap_matrix* m)
 {
    if( m == NULL )
       return SAP_OKAY;
+   if( m->nbounds > 0 )
+   {
+      fprintf(stderr, "[scip_amd] sap_matrix_free: %ld bound sets still live; free them first\n", (long)m->nbounds);
+      return SAP_INVALIDCALL;
+   }
    (void)hipSetDevice(m->ctx->device);
    (void)hipStreamSynchronize(m->ctx->stream);
    if( m->single )
       sap_bounds_free(m->single);
+   m->ctx->nmatrices--;
    matrixRelease(m);
    free(m);
    return SAP_OKAY;
@@ -3290,6 +3343,7 @@ int sap_bounds_create_mode(sap_matrix* m, int32_t nvec, int32_t mode, sap_bounds
    HIPCHK(hipGetLastError());
    HIPCHK(hipMemsetAsync(b->dstatus, 0, sizeof(Status), s));
    HIPCHK(hipStreamSynchronize(s));
+   m->nbounds++;
    *out = b;
    return SAP_OKAY;
 }
@@ -3300,6 +3354,8 @@ int sap_bounds_free(sap_bounds* b)
       return SAP_OKAY;
    (void)hipSetDevice(b->mat->ctx->device);
    (void)hipStreamSynchronize(b->mat->ctx->stream);
+   if( !b->internal )
+      b->mat->nbounds--;
    boundsRelease(b);
    free(b);
    return SAP_OKAY;
@@ -3628,6 +3684,16 @@ static int checkParams(const sap_params* p)
    return SAP_OKAY;
 }
 
+// the matrix's own one-vector state (for sap_propagate / packed / autotune), not counted
+// among the caller's bound sets
+static int createSingle(sap_matrix* m)
+{
+   RCCHK(sap_bounds_create(m, 1, &m->single));
+   m->single->internal = 1;
+   m->nbounds--;
+   return SAP_OKAY;
+}
+
 int sap_propagate_resident(sap_bounds* b, const sap_params* params, sap_stats* stats)
 {
    if( b == NULL )
@@ -3661,7 +3727,7 @@ int sap_propagate(sap_matrix* m, const sap_params* params, double* lb, double* u
       return SAP_INVALIDDATA;
    RCCHK(checkParams(params));
    if( m->single == NULL )
-      RCCHK(sap_bounds_create(m, 1, &m->single));
+      RCCHK(createSingle(m));
    RCCHK(sap_bounds_upload(m->single, 0, lb, ub));
    sap_stats local;
    memset(&local, 0, sizeof(local));
@@ -4076,7 +4142,7 @@ int sap_propagate_packed(sap_matrix* m, const sap_params* params, double* lb, do
    RCCHK(checkParams(params));
    HIPCHK(hipSetDevice(m->ctx->device));
    if( m->single == NULL )
-      RCCHK(sap_bounds_create(m, 1, &m->single));
+      RCCHK(createSingle(m));
    sap_bounds* b = m->single;
    hipStream_t s = m->ctx->stream;
    const int32_t ns = m->nseg;
@@ -4313,7 +4379,7 @@ int sap_row_activities(sap_matrix* m, const sap_params* params, const double* lb
    HIPCHK(hipSetDevice(c->device));
    hipStream_t s = c->stream;
    if( m->single == NULL )
-      RCCHK(sap_bounds_create(m, 1, &m->single));
+      RCCHK(createSingle(m));
    RCCHK(sap_bounds_upload(m->single, 0, lb, ub));
    const int64_t nr = m->nrows;
    double *dmin = NULL, *dmax = NULL, *dmad = NULL;

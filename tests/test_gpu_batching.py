@@ -335,3 +335,20 @@ def test_slots_api_errors(gpu_ctx):
     sl.close()
     base.close()
     mat.close()
+
+
+def test_parents_outlive_children(gpu_ctx):
+    """A context with live matrices, or a matrix with live bound sets, is not freed."""
+    import ctypes as C
+    ctx = sa.Context(0)
+    inst = sa.gen_covering_packing(100, 200, 5, seed=1)
+    mat = sa.Matrix(ctx, inst)
+    b = sa.Bounds(mat, 2)
+    L = sa.lib()
+    assert L.sap_matrix_free(mat.h) == sa.SAP_INVALIDCALL
+    assert L.sap_context_free(ctx.h) == sa.SAP_INVALIDCALL
+    lb, ub, st = mat.propagate()       # still usable
+    assert st["rounds"] >= 1
+    b.close()
+    mat.close()
+    ctx.close()
