@@ -28,6 +28,8 @@ SAP_NOMEMORY = -1
 SAP_INVALIDCALL = -8
 SAP_INVALIDDATA = -9
 SAP_NOTIMPLEMENTED = -18
+SAP_READERROR = -2
+SAP_NOFILE = -4
 
 SAP_DIDNOTFIND = 3
 SAP_CUTOFF = 7
@@ -188,6 +190,7 @@ EXPORTS = {
     "sap_gen_node_fixings": (C.c_int, [C.POINTER(sap_instance), C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
                                        _I64P, _I32P, _DP]),
     "sap_instance_free": (None, [C.POINTER(sap_instance)]),
+    "sap_read_mps": (C.c_int, [C.c_char_p, C.POINTER(sap_instance), _I32P]),
 }
 
 _lib = None
@@ -321,6 +324,14 @@ def gen_mixed(nrows: int, ncols: int, seed: int = 2, lenmode: int = 0, median: f
     _chk(lib().sap_gen_mixed(nrows, ncols, lenmode, median, sigma, minlen, maxlen, 1 if planted else 0, seed,
                              C.byref(inst)), "sap_gen_mixed")
     return _from_c(inst)
+
+
+def read_mps(path: str):
+    """Read an MPS file (include/scip_amd_mps.h); returns (Instance, number of semi-continuous columns)."""
+    inst = sap_instance()
+    nsc = C.c_int32(0)
+    _chk(lib().sap_read_mps(os.fsencode(path), C.byref(inst), C.byref(nsc)), f"sap_read_mps({path})")
+    return _from_c(inst), nsc.value
 
 
 def gen_node_fixings(inst: Instance, nnodes: int, minfix: int = 1, maxfix: int = 50, seed: int = 3):

@@ -37,12 +37,14 @@ def build(verbose: bool = True) -> str:
     eng_o = os.path.join(BUILD, "engine.o")
     gen_o = os.path.join(BUILD, "gen.o")
     srv_o = os.path.join(BUILD, "serve.o")
+    mps_o = os.path.join(BUILD, "mps.o")
     lib = os.path.join(HERE, "libscipamd.so")
     steps = [
         [HIPCC, *HIPFLAGS, "-c", os.path.join(HERE, "csrc", "engine.hip"), "-o", eng_o],
         ["gcc", *CFLAGS, "-c", os.path.join(HERE, "csrc", "gen.c"), "-o", gen_o],
         ["g++", "-std=c++17", *CFLAGS, "-c", os.path.join(HERE, "csrc", "serve.cpp"), "-o", srv_o],
-        [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", eng_o, gen_o, srv_o, "-o", lib + ".tmp"],
+        ["gcc", *CFLAGS, "-c", os.path.join(HERE, "csrc", "mps.c"), "-o", mps_o],
+        [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", eng_o, gen_o, srv_o, mps_o, "-o", lib + ".tmp"],
     ]
     for cmd in steps:
         out = _run(cmd)
