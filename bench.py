@@ -101,11 +101,13 @@ def main():
     ap.add_argument("--cols", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", action="store_true", help="only run the copy kernel used to scale PMC bytes")
-    ap.add_argument("--variant", choices=("both", "literal", "planted"), default="both",
-                    help="literal: only the headline instance (for profiling); planted: only the feasible variant")
-    ap.add_argument("--legs", default="nodes,packed,serve",
-                    help="batched legs to run after the headline: nodes, packed, serve (comma list, or none)")
+    ap.add_argument("--variant", choices=("both", "literal", "planted", "none"), default="both",
+                    help="literal: only the headline instance (for profiling); planted: only the feasible variant; "
+                         "none: only the batched legs")
+    ap.add_argument("--legs", default="nodes,probe,packed,serve",
+                    help="batched legs to run after the headline: nodes, probe, packed, serve (comma list, or none)")
     ap.add_argument("--nodes", type=int, default=1024, help="node vectors of the batched-nodes leg")
+    ap.add_argument("--probe-vectors", type=int, default=2048, help="branch vectors per launch of the probing leg")
     ap.add_argument("--packed-instances", type=int, default=512, help="instances of the packed leg")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the served arrival trace")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate (requests/s)")
@@ -122,6 +124,10 @@ def main():
     import scip_amd as sa
 
     ctx = sa.Context(0)
+    if args.variant == "none":
+        print(json.dumps(run_legs(sa, ctx, args, {})), flush=True)
+        ctx.close()
+        return
     if args.variant != "both":
         out = run_workload(sa, ctx, args, planted=(args.variant == "planted"))
         print(json.dumps(out), flush=True)
@@ -142,18 +148,24 @@ def main():
         fb = cpu_baseline(budget_s=6.0, planted=True)
         main_out["config"]["feasible_variant"]["cpu_baseline_value"] = fb["value"]
         main_out["config"]["feasible_variant"]["cpu_time_to_fixpoint_ms_100k_sample"] = fb["time_to_fixpoint_ms"]
+    print(json.dumps(run_legs(sa, ctx, args, main_out)), flush=True)
+    ctx.close()
+
+
+def run_legs(sa, ctx, args, main_out):
     legs = [x for x in args.legs.split(",") if x and x != "none"]
     parts = None
     if "nodes" in legs:
         main_out["batched_nodes"] = run_batched_nodes(sa, ctx, args)
+    if "probe" in legs:
+        main_out["probing"] = run_probing(sa, ctx, args)
     if "packed" in legs or "serve" in legs:
         parts = gen_uneven_instances(sa, args.packed_instances)
     if "packed" in legs:
         main_out["packed_instances"] = run_packed(sa, ctx, args, parts)
     if "serve" in legs:
         main_out["serving"] = run_serving(sa, args, parts)
-    print(json.dumps(main_out), flush=True)
-    ctx.close()
+    return main_out
 
 
 def run_workload(sa, ctx, args, planted: bool):
@@ -305,6 +317,62 @@ def run_batched_nodes(sa, ctx, args):
                         "from the root fixpoint (resident)",
             "nodes": nn, "root_rounds": rst["rounds"], "batched": batched, "plain": plain,
             "speedup_nnz_sweeps": batched["value"] / plain["value"] if plain["value"] > 0 else None}
+
+
+def run_probing(sa, ctx, args):
+    """Probing of every unfixed binary of the covering/packing 200,000 x 300,000 instance (seed 3)
+    at its root fixpoint: both branches x = 0 / x = 1 of each, propagated from the resident
+    fixpoint.  Batched: sap_probe with --probe-vectors branch vectors per launch.  Plain: one
+    probe (two branches) per call on a sample of the same columns, as a loop of
+    SCIPapplyProbingVar would run them; the sample's verdicts and pair lists are checked equal."""
+    log("[bench] probing: 200000x300000 covering/packing, all unfixed binaries")
+    inst = sa.gen_covering_packing(200_000, 300_000, 10, seed=3)
+    mat = sa.Matrix(ctx, inst)
+    base = sa.Bounds(mat, 1)
+    base.upload(inst.lb, inst.ub)
+    base.propagate()
+    rl, ru = base.download(0)
+    cols = np.nonzero((inst.integral != 0) & (rl == 0.0) & (ru == 1.0))[0].astype(np.int32)
+    p = sa.default_params(flags=sa.SAP_FLAG_BASE_IS_FIXPOINT)
+    pv = sa.Bounds(mat, args.probe_vectors)
+    pv.probe(base, 0, cols[:4 * args.probe_vectors], p)   # warm-up (graphs, buffers)
+    t0 = time.perf_counter()
+    out = pv.probe(base, 0, cols, p)
+    el = time.perf_counter() - t0
+    st = out["stats"]
+    ver = out["verdict"]
+    batched = {"probes_per_s": len(cols) / el, "branches_per_s": 2 * len(cols) / el, "seconds": el,
+               "value": _rate(st["nnz_swept"], el), "unit": "nnz-sweeps/s", "nnz_swept": st["nnz_swept"],
+               "down_infeasible": int((ver == 1).sum()), "up_infeasible": int((ver == 2).sum()),
+               "both_infeasible": int((ver == 3).sum()), "pairs": int(out["dedptr"][-1]),
+               "max_rounds": int(out["rounds"].max()) if len(cols) else 0}
+    pv.close()
+    # plain: one probe per call, on an evenly spaced sample
+    ns = min(len(cols), 2000)
+    sample = cols[np.linspace(0, len(cols) - 1, ns).astype(np.int64)] if ns else cols
+    one = sa.Bounds(mat, 2)
+    one.probe(base, 0, sample[:16], p)
+    nnz = 0
+    same = True
+    pos = np.searchsorted(cols, sample)
+    t0 = time.perf_counter()
+    for k, x in enumerate(sample):
+        o = one.probe(base, 0, sample[k:k + 1], p)
+        nnz += o["stats"]["nnz_swept"]
+        q = int(pos[k])
+        a, e = out["dedptr"][q], out["dedptr"][q + 1]
+        same &= bool(o["verdict"][0] == ver[q] and np.array_equal(o["dedcol"], out["dedcol"][a:e]))
+    elp = time.perf_counter() - t0
+    one.close()
+    plain = {"probes_per_s": ns / elp if elp > 0 else None, "sample": int(ns), "seconds": elp,
+             "value": _rate(nnz, elp), "unit": "nnz-sweeps/s", "equal_to_batched": same}
+    base.close()
+    mat.close()
+    return {"workload": "covering/packing 200000x300000, 10 nnz/row, seed 3; probe every unfixed binary at the "
+                        "root fixpoint (resident)",
+            "probes": int(len(cols)), "batched": batched, "plain": plain,
+            "speedup_probes_per_s": (batched["probes_per_s"] / plain["probes_per_s"]
+                                     if plain["probes_per_s"] else None)}
 
 
 def gen_uneven_instances(sa, count, seed=4):

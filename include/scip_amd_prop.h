@@ -185,6 +185,32 @@ int sap_propagate_nodes(sap_bounds* bnd, const sap_params* params, int32_t nvec,
    int32_t* result, int32_t* rounds, int64_t chgcap, int64_t* chgptr, int32_t* chgcol, double* chglb,
    double* chgub, sap_stats* stats);
 
+/* Probing on binary columns: for each probe column x (integral, bounds exactly [0,1] in the
+ * base vector) both branches x = 0 and x = 1 are propagated to fixpoint from vector basevec of
+ * base, two node vectors per probe, as many probes per launch as bnd has vector pairs.  This
+ * is the work of SCIPapplyProbingVar (prop_probing.c:1203-1311) for all probe columns at once;
+ * the output is what SCIPanalyzeDeductionsProbing (prop_probing.c:1314) needs from the two
+ * branches:
+ *   verdict[p]  SAP_PROBE_DOWN_INFEASIBLE (x = 0 is cut off) | SAP_PROBE_UP_INFEASIBLE (x = 1);
+ *   rounds      (may be NULL) 2*nprobe: rounds of branch x = 0, then x = 1, per probe;
+ *   the columns other than x whose bounds changed in both branches (only these can give a
+ *   fixing, a tightening of the union bounds or an aggregation), sorted by column: entries
+ *   dedptr[p]..dedptr[p+1]-1 of dedcol with the bounds in the x = 0 branch (dlb0, dub0) and in
+ *   the x = 1 branch (dlb1, dub1); empty when a branch is cut off.
+ * Implications from columns changed in one branch only are not listed (sap_propagate_nodes
+ * with the same two overlays gives them).  verdict, rounds and dedptr (nprobe+1) are always
+ * filled; if dedptr[nprobe] > dedcap the lists are not written and SAP_NOMEMORY is returned;
+ * dedcol == NULL asks for verdicts and counts only.  A probe column that is not an unfixed
+ * binary of the base is SAP_INVALIDDATA.  stats->result is SAP_CUTOFF when both branches of
+ * some probe are cut off (the base itself is infeasible), SAP_REDUCEDDOM when some branch is cut
+ * off or some column changed in both branches, else SAP_DIDNOTFIND.
+ * SAP_FLAG_BASE_IS_FIXPOINT in params->flags as in sap_propagate_nodes. */
+#define SAP_PROBE_DOWN_INFEASIBLE 1
+#define SAP_PROBE_UP_INFEASIBLE   2
+int sap_probe(sap_bounds* bnd, const sap_params* params, int32_t nprobe, const int32_t* probecol,
+   const sap_bounds* base, int32_t basevec, int32_t* verdict, int32_t* rounds, int64_t dedcap, int64_t* dedptr,
+   int32_t* dedcol, double* dlb0, double* dub0, double* dlb1, double* dub1, sap_stats* stats);
+
 /* Independent instances packed block-diagonally into one resident matrix (one launch per
  * round for all of them; replaces one consPropLinear fixpoint per SCIP instance).  Instance
  * s owns rows [rowoff[s], rowoff[s+1]) and columns [coloff[s], coloff[s+1]) of the

@@ -172,3 +172,38 @@ def canonical_sum(x):
     hi, lo = C.c_double(), C.c_double()
     lib().orc_canonical_sum(x.ctypes.data_as(_DP), len(x), C.byref(hi), C.byref(lo))
     return hi.value, lo.value
+
+
+def probe(om: OMatrix, lb, ub, probecols, prm: orc_params | None = None):
+    """Probing on binary columns, restated from prop_probing.c: each probe column x is fixed
+    to 0 and to 1 (SCIPapplyProbingVar, prop_probing.c:1203-1311, here with linear
+    propagation as the only propagator: the Jacobi twin), and the two branches are compared
+    as SCIPanalyzeDeductionsProbing (prop_probing.c:1314-1500) compares them.
+
+    Returns per probe: verdict bits (1: x = 0 infeasible, 2: x = 1 infeasible), the rounds of
+    both branches, and -- when neither branch is infeasible -- the columns other than x whose
+    bounds differ from (lb, ub) in both branches, ascending, with their bounds in each branch.
+    """
+    lb = np.asarray(lb, dtype=np.float64)
+    ub = np.asarray(ub, dtype=np.float64)
+    out = []
+    for x in probecols:
+        branch = []
+        for val in (0.0, 1.0):
+            l0 = lb.copy()
+            u0 = ub.copy()
+            l0[x] = u0[x] = val
+            pl, pu, r = propagate_jacobi(om, l0, u0, prm)
+            branch.append((pl, pu, r))
+        verdict = (1 if branch[0][2]["cutoff"] else 0) | (2 if branch[1][2]["cutoff"] else 0)
+        rec = dict(verdict=verdict, rounds=(branch[0][2]["rounds"], branch[1][2]["rounds"]),
+                   cols=np.zeros(0, dtype=np.int32), b0=np.zeros((0, 2)), b1=np.zeros((0, 2)))
+        if verdict == 0:
+            (l0, u0, _), (l1, u1, _) = branch
+            ch0 = (l0 != lb) | (u0 != ub)
+            ch1 = (l1 != lb) | (u1 != ub)
+            ch0[x] = ch1[x] = False
+            cols = np.nonzero(ch0 & ch1)[0].astype(np.int32)
+            rec.update(cols=cols, b0=np.stack([l0[cols], u0[cols]], 1), b1=np.stack([l1[cols], u1[cols]], 1))
+        out.append(rec)
+    return out

@@ -41,6 +41,9 @@ SAP_FLAG_HOSTLOOP = 0x2
 SAP_STEP_SHORT = 2
 SAP_STEP_LONG = 8
 
+SAP_PROBE_DOWN_INFEASIBLE = 1
+SAP_PROBE_UP_INFEASIBLE = 2
+
 SAP_MODE_AUTO = 0
 SAP_MODE_DENSE = 1
 SAP_MODE_SPARSE = 2
@@ -166,6 +169,8 @@ EXPORTS = {
                                       _I32P, _I32P, _DP, _DP, C.POINTER(sap_stats)]),
     "sap_propagate_nodes": (C.c_int, [_P, C.POINTER(sap_params), C.c_int32, _P, C.c_int32, _I64P, _I32P, _DP, _DP,
                                       _I32P, _I32P, C.c_int64, _I64P, _I32P, _DP, _DP, C.POINTER(sap_stats)]),
+    "sap_probe": (C.c_int, [_P, C.POINTER(sap_params), C.c_int32, _I32P, _P, C.c_int32, _I32P, _I32P, C.c_int64,
+                            _I64P, _I32P, _DP, _DP, _DP, _DP, C.POINTER(sap_stats)]),
     "sap_matrix_create_packed": (C.c_int, [_P, C.c_int32, _I64P, _I64P, C.c_int64, C.c_int64, _I64P, _I32P, _DP,
                                            _DP, _DP, _U8P, _DP, _DP, C.POINTER(_P)]),
     "sap_propagate_packed": (C.c_int, [_P, C.POINTER(sap_params), _DP, _DP, _I32P, _I32P, C.POINTER(sap_stats)]),
@@ -581,6 +586,34 @@ class Bounds:
             out = dict(result=res, rounds=rnd, chgptr=cptr, stats=st.as_dict())
             if want_changes:
                 out.update(chgcol=ccol[:n], chglb=clb[:n], chgub=cub[:n])
+            return out
+
+    def probe(self, base: "Bounds", basevec, probecol, params: sap_params | None = None,
+              dedcap: int | None = None, want_pairs: bool = True):
+        """Both branches x = 0 / x = 1 of every probe column from a resident base vector:
+        per-probe verdict bits, per-branch rounds and the columns changed in both branches."""
+        probecol = np.ascontiguousarray(probecol, dtype=np.int32)
+        n = len(probecol)
+        ver = np.zeros(max(n, 1), dtype=np.int32)
+        rnd = np.zeros(max(2 * n, 1), dtype=np.int32)
+        cap = int(dedcap if dedcap is not None else max(1024, 16 * n))
+        while True:
+            dptr = np.zeros(n + 1, dtype=np.int64)
+            arrs = [np.zeros(max(cap, 1), dtype=np.int32)] + [np.zeros(max(cap, 1)) for _ in range(4)] \
+                if want_pairs else [None] * 5
+            st = sap_stats()
+            p = params or default_params()
+            rc = lib().sap_probe(self.h, C.byref(p), n, _i32p(probecol), base.h, basevec, _i32p(ver), _i32p(rnd),
+                                 cap, _i64p(dptr), _i32p(arrs[0]), _dp(arrs[1]), _dp(arrs[2]), _dp(arrs[3]),
+                                 _dp(arrs[4]), C.byref(st))
+            if rc == SAP_NOMEMORY and dedcap is None and int(dptr[-1]) > cap:
+                cap = int(dptr[-1])   # run again with room for every pair (same results)
+                continue
+            _chk(rc, "sap_probe")
+            k = int(dptr[-1])
+            out = dict(verdict=ver[:n], rounds=rnd[:2 * n].reshape(n, 2), dedptr=dptr, stats=st.as_dict())
+            if want_pairs:
+                out.update(dedcol=arrs[0][:k], dlb0=arrs[1][:k], dub0=arrs[2][:k], dlb1=arrs[3][:k], dub1=arrs[4][:k])
             return out
 
     # ---- continuous batching ----

@@ -2269,6 +2269,51 @@ __global__ void k_collect(Vec V, int64_t ncols, const int32_t* slots, int32_t n,
    }
 }
 
+// probing: *bad receives the smallest index i whose column pcol[i] is not an integral column
+// with base bounds exactly [0, 1]
+__global__ void k_probe_check(Mat M, const double2* base, const int32_t* pcol, int32_t n, int32_t* bad)
+{
+   for( int32_t i = blockIdx.x * (int32_t)blockDim.x + (int32_t)threadIdx.x; i < n; i += (int32_t)(gridDim.x * blockDim.x) )
+   {
+      const int32_t j = pcol[i];
+      const double2 bj = base[j];
+      if( !((M.colflags[j] & 1) && bj.x == 0.0 && bj.y == 1.0) )
+         atomicMin(bad, i);
+   }
+}
+
+// probing: probe p owns vectors 2p (x = 0) and 2p+1 (x = 1).  When neither branch is cut off,
+// the columns other than x found in both change logs are written at outoff[p] + (count[p]++)
+// with their bounds in both branches.  The shorter log is scanned and the other branch's log
+// bits are tested, so the work is the smaller change count.
+__global__ void k_probe_pairs(Vec V, int64_t ncols, const int32_t* pcol, int32_t np, const int64_t* outoff,
+   int32_t* count, int32_t* outcol, double2* out0, double2* out1)
+{
+   for( int32_t p = blockIdx.y; p < np; p += gridDim.y )
+   {
+      const int32_t v0 = 2 * p, v1 = 2 * p + 1;
+      if( V.vcutoff[v0] != 0 || V.vcutoff[v1] != 0 )
+         continue;
+      const int32_t n0 = V.vlogn[v0], n1 = V.vlogn[v1];
+      const int32_t vs = n0 <= n1 ? v0 : v1;
+      const int32_t vo = n0 <= n1 ? v1 : v0;
+      const int32_t ns = n0 <= n1 ? n0 : n1;
+      const int32_t x = pcol[p];
+      const int32_t* log = V.log + (size_t)vs * (size_t)ncols;
+      for( int32_t q = blockIdx.x * (int32_t)blockDim.x + (int32_t)threadIdx.x; q < ns; q += (int32_t)(gridDim.x * blockDim.x) )
+      {
+         const int32_t j = log[q];
+         const size_t flat = (size_t)vo * (size_t)ncols + (size_t)j;
+         if( j == x || ((V.logbits[flat >> 5] >> (flat & 31)) & 1u) == 0 )
+            continue;
+         const int64_t at = outoff[p] + atomicAdd(&count[p], 1);
+         outcol[at] = j;
+         out0[at] = V.bnd[(size_t)v0 * (size_t)ncols + (size_t)j];
+         out1[at] = V.bnd[(size_t)v1 * (size_t)ncols + (size_t)j];
+      }
+   }
+}
+
 // bounds of all vectors from a base vector plus overlays
 __global__ void k_broadcast_base(double2* bnd, const double2* base, int64_t ncols, int32_t nvec)
 {
@@ -2433,6 +2478,10 @@ struct sap_bounds
    int32_t* outcol;      // gathered changes of a collection
    double2* outbnd;
    int64_t outcap;
+   int32_t* pcol;        // nvec: probe columns of a probing chunk
+   int32_t* pcount;      // nvec: per-probe output counters
+   double2* outbnd2;     // probing: bounds in the second branch, beside outbnd (capacity outcap2)
+   int64_t outcap2;
    int32_t logready;     // change log allocated and cleared
    int32_t slotsmode;    // slots are in use (continuous batching)
    int32_t inflight;     // a step was launched and not yet waited for
@@ -3205,6 +3254,9 @@ static void boundsRelease(sap_bounds* b)
    (void)hipFree(b->outoff);
    (void)hipFree(b->outcol);
    (void)hipFree(b->outbnd);
+   (void)hipFree(b->pcol);
+   (void)hipFree(b->pcount);
+   (void)hipFree(b->outbnd2);
    delete b->active;
    b->active = NULL;
    delete b->srcb;
@@ -4130,6 +4182,206 @@ int sap_propagate_nodes(sap_bounds* b, const sap_params* params, int32_t nvec, c
       for( int32_t v = 0; v < nvec; ++v )
          anycut |= res[(size_t)v] == SAP_CUTOFF;
       stats->result = anycut ? SAP_CUTOFF : (stats->nchgbds > 0 ? SAP_REDUCEDDOM : SAP_DIDNOTFIND);
+   }
+   return SAP_OKAY;
+}
+
+// probing buffers: probe columns and counters (nvec each), pair outputs for n entries
+static int ensureProbe(sap_bounds* b, int64_t n)
+{
+   if( b->pcol == NULL )
+   {
+      RCCHK(devAlloc(&b->pcol, (size_t)b->nvec, NULL));
+      RCCHK(devAlloc(&b->pcount, (size_t)b->nvec, NULL));
+   }
+   RCCHK(ensureOut(b, n));
+   if( b->outbnd2 == NULL || b->outcap2 < b->outcap )
+   {
+      RCCHK(growDev(&b->outbnd2, 0, b->outcap));
+      b->outcap2 = b->outcap;
+   }
+   return SAP_OKAY;
+}
+
+int sap_probe(sap_bounds* b, const sap_params* params, int32_t nprobe, const int32_t* probecol,
+   const sap_bounds* base, int32_t basevec, int32_t* verdict, int32_t* rounds, int64_t dedcap, int64_t* dedptr,
+   int32_t* dedcol, double* dlb0, double* dub0, double* dlb1, double* dub1, sap_stats* stats)
+{
+   if( b == NULL || base == NULL || base->mat != b->mat || basevec < 0 || basevec >= base->nvec || b->nvec < 2
+      || nprobe < 0 || (nprobe > 0 && (probecol == NULL || verdict == NULL)) || b->mat->nseg != 1 || dedcap < 0
+      || (dedcol != NULL && (dedptr == NULL || dlb0 == NULL || dub0 == NULL || dlb1 == NULL || dub1 == NULL)) )
+      return SAP_INVALIDDATA;
+   RCCHK(checkParams(params));
+   sap_matrix* m = b->mat;
+   const int64_t nc = m->ncols;
+   for( int32_t p = 0; p < nprobe; ++p )
+      if( probecol[p] < 0 || probecol[p] >= nc )
+         return SAP_INVALIDDATA;
+   if( b->slotsmode )
+      return SAP_INVALIDCALL;
+   HIPCHK(hipSetDevice(m->ctx->device));
+   hipStream_t s = m->ctx->stream;
+   const double2* dbase = base->V.bnd + (size_t)basevec * (size_t)nc;
+   if( base == b )
+   {
+      // every vector of b is overwritten by the branches: keep a copy of the base
+      HIPCHK(hipMemcpyAsync(b->base2, dbase, sizeof(double2) * (size_t)nc, hipMemcpyDeviceToDevice, s));
+      dbase = b->base2;
+   }
+   RCCHK(ensureProbe(b, 0));
+   const int32_t per = b->nvec / 2;
+   // every probe column must be an unfixed binary of the base
+   for( int32_t p0 = 0; p0 < nprobe; p0 += b->nvec )
+   {
+      const int32_t n = std::min<int32_t>(b->nvec, nprobe - p0);
+      int32_t bad = INT32_MAX;
+      HIPCHK(hipMemcpyAsync(b->pcol, probecol + p0, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, s));
+      k_fill_i32<<<1, 64, 0, s>>>(b->pcount, INT32_MAX, 1);
+      k_probe_check<<<gridFor((size_t)n), SAP_BLOCK, 0, s>>>(toMat(m), dbase, b->pcol, n, b->pcount);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(&bad, b->pcount, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if( bad != INT32_MAX )
+         return SAP_INVALIDDATA;
+   }
+   sap_params prm = *params;
+   std::vector<int64_t> cnt((size_t)nprobe + 1, 0);
+   std::vector<int32_t> hcol;
+   std::vector<double2> hb0, hb1;
+   sap_stats acc;
+   memset(&acc, 0, sizeof(acc));
+   int32_t anyboth = 0, anyinfo = 0;
+   std::vector<int64_t> ovptr, off;
+   std::vector<int32_t> ovcol, ident, res, rnd, logn, cut, pc;
+   std::vector<double> ovlb, ovub;
+   for( int32_t p0 = 0; p0 < nprobe; p0 += per )
+   {
+      const int32_t np = std::min<int32_t>(per, nprobe - p0);
+      const int32_t nv = 2 * np;
+      ovptr.resize((size_t)nv + 1);
+      ovcol.resize((size_t)nv);
+      ovlb.resize((size_t)nv);
+      ovub.resize((size_t)nv);
+      for( int32_t v = 0; v <= nv; ++v )
+         ovptr[(size_t)v] = v;
+      for( int32_t p = 0; p < np; ++p )
+      {
+         const int32_t x = probecol[p0 + p];
+         ovcol[(size_t)(2 * p)] = x;
+         ovlb[(size_t)(2 * p)] = 0.0;
+         ovub[(size_t)(2 * p)] = 0.0;
+         ovcol[(size_t)(2 * p + 1)] = x;
+         ovlb[(size_t)(2 * p + 1)] = 1.0;
+         ovub[(size_t)(2 * p + 1)] = 1.0;
+      }
+      int64_t nov = 0;
+      RCCHK(stageOverlay(b, nv, ovptr.data(), ovcol.data(), ovlb.data(), ovub.data(), &nov));
+      sap_stats cs;
+      memset(&cs, 0, sizeof(cs));
+      RCCHK(batchCall(b, &prm, nv, base == b ? NULL : base, basevec, dbase, nov, &cs));
+      acc.rounds = std::max(acc.rounds, cs.rounds);
+      acc.nchgbds += cs.nchgbds;
+      acc.rows_swept += cs.rows_swept;
+      acc.nnz_swept += cs.nnz_swept;
+      acc.device_ms += cs.device_ms;
+      acc.sweep_ms += cs.sweep_ms;
+      acc.sweep_launches += cs.sweep_launches;
+      // room for the columns changed in both branches: at most the shorter log per probe
+      logn.resize((size_t)nv);
+      cut.resize((size_t)nv);
+      HIPCHK(hipMemcpyAsync(logn.data(), b->V.vlogn, sizeof(int32_t) * (size_t)nv, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(cut.data(), b->V.vcutoff, sizeof(int32_t) * (size_t)nv, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      off.assign((size_t)np + 1, 0);
+      int32_t maxs = 0;
+      for( int32_t p = 0; p < np; ++p )
+      {
+         const size_t v0 = (size_t)(2 * p), v1 = v0 + 1;
+         const int32_t ns = (cut[v0] || cut[v1]) ? 0 : std::min(logn[v0], logn[v1]);
+         off[(size_t)p + 1] = off[(size_t)p] + ns;
+         maxs = std::max(maxs, ns);
+      }
+      const int64_t room = off[(size_t)np];
+      pc.assign((size_t)np, 0);
+      if( room > 0 )
+      {
+         RCCHK(ensureProbe(b, room));
+         HIPCHK(hipMemcpyAsync(b->pcol, probecol + p0, sizeof(int32_t) * (size_t)np, hipMemcpyHostToDevice, s));
+         HIPCHK(hipMemcpyAsync(b->outoff, off.data(), sizeof(int64_t) * (size_t)np, hipMemcpyHostToDevice, s));
+         k_fill_i32<<<gridFor((size_t)np), SAP_BLOCK, 0, s>>>(b->pcount, 0, (size_t)np);
+         k_probe_pairs<<<slotGrid((size_t)maxs, np), SAP_BLOCK, 0, s>>>(b->V, nc, b->pcol, np, b->outoff, b->pcount,
+            b->outcol, b->outbnd, b->outbnd2);
+         HIPCHK(hipGetLastError());
+         HIPCHK(hipMemcpyAsync(pc.data(), b->pcount, sizeof(int32_t) * (size_t)np, hipMemcpyDeviceToHost, s));
+         HIPCHK(hipStreamSynchronize(s));
+      }
+      int64_t got = 0;
+      for( int32_t p = 0; p < np; ++p )
+         got += pc[(size_t)p];
+      std::vector<int32_t> tc((size_t)room);
+      std::vector<double2> t0((size_t)room), t1((size_t)room);
+      if( got > 0 )
+      {
+         HIPCHK(hipMemcpyAsync(tc.data(), b->outcol, sizeof(int32_t) * (size_t)room, hipMemcpyDeviceToHost, s));
+         HIPCHK(hipMemcpyAsync(t0.data(), b->outbnd, sizeof(double2) * (size_t)room, hipMemcpyDeviceToHost, s));
+         HIPCHK(hipMemcpyAsync(t1.data(), b->outbnd2, sizeof(double2) * (size_t)room, hipMemcpyDeviceToHost, s));
+         HIPCHK(hipStreamSynchronize(s));
+      }
+      // verdicts and rounds; the change logs are cleared for the next chunk
+      ident.resize((size_t)nv);
+      res.resize((size_t)nv);
+      rnd.resize((size_t)nv);
+      for( int32_t v = 0; v < nv; ++v )
+         ident[(size_t)v] = v;
+      RCCHK(collectSlots(b, nv, ident.data(), res.data(), rnd.data(), 0, NULL, NULL, NULL, NULL));
+      std::vector<int64_t> idx;
+      for( int32_t p = 0; p < np; ++p )
+      {
+         const int32_t q = p0 + p;
+         verdict[q] = (res[(size_t)(2 * p)] == SAP_CUTOFF ? SAP_PROBE_DOWN_INFEASIBLE : 0)
+            | (res[(size_t)(2 * p + 1)] == SAP_CUTOFF ? SAP_PROBE_UP_INFEASIBLE : 0);
+         anyboth |= verdict[q] == (SAP_PROBE_DOWN_INFEASIBLE | SAP_PROBE_UP_INFEASIBLE);
+         anyinfo |= verdict[q] != 0 || pc[(size_t)p] > 0;
+         if( rounds )
+         {
+            rounds[2 * (size_t)q] = rnd[(size_t)(2 * p)];
+            rounds[2 * (size_t)q + 1] = rnd[(size_t)(2 * p + 1)];
+         }
+         // the order within a probe depends on atomics: sort by column
+         const int64_t a = off[(size_t)p];
+         idx.resize((size_t)pc[(size_t)p]);
+         for( int32_t k = 0; k < pc[(size_t)p]; ++k )
+            idx[(size_t)k] = a + k;
+         std::sort(idx.begin(), idx.end(), [&](int64_t u, int64_t w) { return tc[(size_t)u] < tc[(size_t)w]; });
+         for( int64_t k : idx )
+         {
+            hcol.push_back(tc[(size_t)k]);
+            hb0.push_back(t0[(size_t)k]);
+            hb1.push_back(t1[(size_t)k]);
+         }
+         cnt[(size_t)q + 1] = pc[(size_t)p];
+      }
+   }
+   for( int32_t p = 0; p < nprobe; ++p )
+      cnt[(size_t)p + 1] += cnt[(size_t)p];
+   if( dedptr )
+      memcpy(dedptr, cnt.data(), sizeof(int64_t) * ((size_t)nprobe + 1));
+   if( stats )
+   {
+      *stats = acc;
+      stats->result = anyboth ? SAP_CUTOFF : (anyinfo ? SAP_REDUCEDDOM : SAP_DIDNOTFIND);
+   }
+   if( dedcol == NULL )
+      return SAP_OKAY;
+   if( cnt[(size_t)nprobe] > dedcap )
+      return SAP_NOMEMORY;
+   for( size_t k = 0; k < hcol.size(); ++k )
+   {
+      dedcol[k] = hcol[k];
+      dlb0[k] = hb0[k].x;
+      dub0[k] = hb0[k].y;
+      dlb1[k] = hb1[k].x;
+      dub1[k] = hb1[k].y;
    }
    return SAP_OKAY;
 }

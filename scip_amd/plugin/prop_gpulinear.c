@@ -8,10 +8,19 @@
  * SCIPinferVarLbCons / SCIPinferVarUbCons (scip_var.c:6965-7174), so every returned bound is
  * handed to SCIPinferVarLbProp / SCIPinferVarUbProp (scip_var.c:7589/7705) unchanged.
  *
+ * In presolving it can also probe: every unfixed binary column is fixed to 0 and to 1 and
+ * both branches are propagated on the device, two node vectors per probe, many probes per
+ * launch (the work of SCIPapplyProbingVar, prop_probing.c:1203, for all binaries at once).  An
+ * infeasible branch fixes the column to the other value (prop_probing.c:477-556); otherwise
+ * the columns changed in both branches go to SCIP's own SCIPanalyzeDeductionsProbing
+ * (prop_probing.c:1314), which derives fixings, tightenings, aggregations and implications.
+ *
  * Parameters mirror SCIP's (numerics/..., constraints/linear/maxeasyactivitydelta); the
  * propagator's own:
- *   propagating/gpulinear/device     HIP device ordinal
- *   propagating/gpulinear/maxrounds  device rounds per call (-1: to fixpoint)
+ *   propagating/gpulinear/device          HIP device ordinal
+ *   propagating/gpulinear/maxrounds       device rounds per call (-1: to fixpoint)
+ *   propagating/gpulinear/probing         probe the binaries in presolving
+ *   propagating/gpulinear/probingvectors  node vectors of one probing launch (two per probe)
  *
  * Error convention: every HIP or engine failure becomes a SCIP_RETCODE (the engine's return
  * codes have SCIP_RETCODE values); nothing aborts.
@@ -21,6 +30,7 @@
 
 #include "scip/scip.h"
 #include "scip/cons_linear.h"
+#include "scip/prop_probing.h"
 #include "scip_amd_prop.h"
 #include "prop_gpulinear.h"
 
@@ -36,6 +46,8 @@
 
 #define DEFAULT_DEVICE             0
 #define DEFAULT_MAXROUNDS         -1
+#define DEFAULT_PROBING        FALSE
+#define DEFAULT_PROBINGVECTORS  2048
 
 /** propagator data */
 struct SCIP_PropData
@@ -50,6 +62,14 @@ struct SCIP_PropData
    SCIP_Longint          ncalls;             /**< PROPEXEC calls that ran on the device */
    SCIP_Longint          ncutoffs;           /**< cutoffs found */
    SCIP_Longint          nchgbds;            /**< bound changes handed to SCIP */
+   SCIP_Bool             probing;            /**< parameter: probe the binaries in presolving */
+   int                   probingvectors;     /**< parameter: node vectors of one probing launch */
+   sap_bounds*           probebase;          /**< probing: the presolving fixpoint (one vector) */
+   sap_bounds*           probeset;           /**< probing: the branch vectors */
+   SCIP_Longint          nprobed;            /**< probes run on the device */
+   SCIP_Longint          nprobefixed;        /**< columns fixed by probing */
+   SCIP_Longint          nprobeaggr;         /**< aggregations found by probing */
+   SCIP_Longint          nprobeimpl;         /**< implications found by probing */
 };
 
 /*
@@ -104,6 +124,17 @@ SCIP_RETCODE freeMatrix(
    SCIP_PROPDATA*        propdata            /**< propagator data */
    )
 {
+   /* bound sets before their matrix */
+   if( propdata->probeset != NULL )
+   {
+      SCIP_CALL( engineCall(sap_bounds_free(propdata->probeset), "sap_bounds_free") );
+      propdata->probeset = NULL;
+   }
+   if( propdata->probebase != NULL )
+   {
+      SCIP_CALL( engineCall(sap_bounds_free(propdata->probebase), "sap_bounds_free") );
+      propdata->probebase = NULL;
+   }
    if( propdata->mat != NULL )
    {
       SCIP_CALL( engineCall(sap_matrix_free(propdata->mat), "sap_matrix_free") );
@@ -283,6 +314,139 @@ SCIP_RETCODE propagateOnDevice(
    return SCIP_OKAY;
 }
 
+/** probing on the device from the presolving fixpoint lb/ub (the bounds just handed to SCIP):
+ *  both branches of every unfixed binary, then the deductions of prop_probing.c's applyProbing
+ *  (:477-556 for an infeasible branch, SCIPanalyzeDeductionsProbing for the others) */
+static
+SCIP_RETCODE probeOnDevice(
+   SCIP*                 scip,               /**< SCIP data structure */
+   SCIP_PROPDATA*        propdata,           /**< propagator data */
+   const double*         lb,                 /**< array of nvars: presolving fixpoint, lower bounds */
+   const double*         ub,                 /**< array of nvars: presolving fixpoint, upper bounds */
+   int*                  nfixedvars,         /**< pointer to count fixings */
+   int*                  naggrvars,          /**< pointer to count aggregations */
+   int*                  nchgbds,            /**< pointer to count tightenings */
+   SCIP_Bool*            cutoff              /**< pointer to store whether infeasibility was detected */
+   )
+{
+   sap_params params;
+   sap_stats stats;
+   int32_t* probecol;
+   int32_t* verdict;
+   int64_t* dedptr;
+   int32_t* dedcol = NULL;
+   double* ded[4] = { NULL, NULL, NULL, NULL };
+   SCIP_VAR** pairvars = NULL;
+   int64_t cap;
+   int nprobe = 0;
+   int rc;
+   int p;
+   int j;
+   int k;
+
+   *cutoff = FALSE;
+   SCIP_CALL( SCIPallocBufferArray(scip, &probecol, propdata->nvars) );
+   for( j = 0; j < propdata->nvars; ++j )
+      if( SCIPvarIsBinary(propdata->vars[j]) && lb[j] == 0.0 && ub[j] == 1.0 )
+         probecol[nprobe++] = (int32_t)j;
+   if( nprobe == 0 )
+   {
+      SCIPfreeBufferArray(scip, &probecol);
+      return SCIP_OKAY;
+   }
+   if( propdata->probebase == NULL )
+   {
+      SCIP_CALL( engineCall(sap_bounds_create(propdata->mat, 1, &propdata->probebase), "sap_bounds_create") );
+   }
+   if( propdata->probeset == NULL )
+   {
+      SCIP_CALL( engineCall(sap_bounds_create(propdata->mat, propdata->probingvectors, &propdata->probeset),
+            "sap_bounds_create") );
+   }
+   SCIP_CALL( engineCall(sap_bounds_upload(propdata->probebase, 0, lb, ub), "sap_bounds_upload") );
+   SCIP_CALL( getEngineParams(scip, propdata, &params) );
+   params.flags |= SAP_FLAG_BASE_IS_FIXPOINT;
+
+   SCIP_CALL( SCIPallocBufferArray(scip, &verdict, nprobe) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &dedptr, nprobe + 1) );
+   cap = 16 * (int64_t)nprobe;
+   for( ;; )
+   {
+      SCIP_CALL( SCIPallocBufferArray(scip, &dedcol, cap) );
+      for( k = 0; k < 4; ++k )
+      {
+         SCIP_CALL( SCIPallocBufferArray(scip, &ded[k], cap) );
+      }
+      memset(&stats, 0, sizeof(stats));
+      rc = sap_probe(propdata->probeset, &params, nprobe, probecol, propdata->probebase, 0, verdict, NULL, cap, dedptr,
+         dedcol, ded[0], ded[1], ded[2], ded[3], &stats);
+      if( rc != SAP_NOMEMORY || dedptr[nprobe] <= cap )
+         break;
+      /* too little room for the pairs: again with enough (same results) */
+      cap = dedptr[nprobe];
+      for( k = 3; k >= 0; --k )
+         SCIPfreeBufferArray(scip, &ded[k]);
+      SCIPfreeBufferArray(scip, &dedcol);
+   }
+   SCIP_CALL( engineCall(rc, "sap_probe") );
+   propdata->nprobed += nprobe;
+
+   SCIP_CALL( SCIPallocBufferArray(scip, &pairvars, propdata->nvars) );
+   for( p = 0; p < nprobe && !*cutoff; ++p )
+   {
+      SCIP_VAR* var = propdata->vars[probecol[p]];
+      SCIP_Bool infeasible = FALSE;
+      SCIP_Bool fixed = FALSE;
+      int n = (int)(dedptr[p + 1] - dedptr[p]);
+      int nfix = 0;
+      int naggr = 0;
+      int nimpl = 0;
+      int nchg = 0;
+
+      /* fixed by the deductions of an earlier probe */
+      if( SCIPvarGetLbGlobal(var) > 0.5 || SCIPvarGetUbGlobal(var) < 0.5 )
+         continue;
+      if( verdict[p] == (SAP_PROBE_DOWN_INFEASIBLE | SAP_PROBE_UP_INFEASIBLE) )
+      {
+         *cutoff = TRUE;
+         break;
+      }
+      if( verdict[p] != 0 )
+      {
+         SCIP_CALL( SCIPfixVar(scip, var, (verdict[p] & SAP_PROBE_UP_INFEASIBLE) ? 0.0 : 1.0, &infeasible, &fixed) );
+         *cutoff = infeasible;
+         if( fixed )
+         {
+            ++(*nfixedvars);
+            ++propdata->nprobefixed;
+         }
+         continue;
+      }
+      if( n == 0 )
+         continue;
+      for( k = 0; k < n; ++k )
+         pairvars[k] = propdata->vars[dedcol[dedptr[p] + k]];
+      SCIP_CALL( SCIPanalyzeDeductionsProbing(scip, var, 0.0, 1.0, n, pairvars, NULL, NULL, ded[0] + dedptr[p],
+            ded[1] + dedptr[p], NULL, NULL, ded[2] + dedptr[p], ded[3] + dedptr[p], &nfix, &naggr, &nimpl, &nchg,
+            cutoff) );
+      *nfixedvars += nfix;
+      *naggrvars += naggr;
+      *nchgbds += nchg;
+      propdata->nprobefixed += nfix;
+      propdata->nprobeaggr += naggr;
+      propdata->nprobeimpl += nimpl;
+   }
+
+   SCIPfreeBufferArray(scip, &pairvars);
+   for( k = 3; k >= 0; --k )
+      SCIPfreeBufferArray(scip, &ded[k]);
+   SCIPfreeBufferArray(scip, &dedcol);
+   SCIPfreeBufferArray(scip, &dedptr);
+   SCIPfreeBufferArray(scip, &verdict);
+   SCIPfreeBufferArray(scip, &probecol);
+   return SCIP_OKAY;
+}
+
 /*
  * callback methods of propagator
  */
@@ -404,6 +568,15 @@ SCIP_DECL_PROPPRESOL(propPresolGpulinear)
       }
       *nchgbds += nchg;
       propdata->nchgbds += nchg;
+      if( !cutoff && propdata->probing )
+      {
+         int oldfixed = *nfixedvars;
+         int oldaggr = *naggrvars;
+         int oldchg = *nchgbds;
+
+         SCIP_CALL( probeOnDevice(scip, propdata, lb, ub, nfixedvars, naggrvars, nchgbds, &cutoff) );
+         nchg += (*nfixedvars - oldfixed) + (*naggrvars - oldaggr) + (*nchgbds - oldchg);
+      }
       *result = cutoff ? SCIP_CUTOFF : (nchg > 0 ? SCIP_SUCCESS : SCIP_DIDNOTFIND);
    }
    SCIPfreeBufferArray(scip, &ub);
@@ -504,6 +677,12 @@ SCIP_RETCODE SCIPincludePropGpulinear(
    SCIP_CALL( SCIPaddIntParam(scip, "propagating/" PROP_NAME "/maxrounds",
          "maximal number of device propagation rounds per call (-1: until fixpoint)",
          &propdata->maxrounds, FALSE, DEFAULT_MAXROUNDS, -1, INT_MAX, NULL, NULL) );
+   SCIP_CALL( SCIPaddBoolParam(scip, "propagating/" PROP_NAME "/probing",
+         "probe the unfixed binaries on the device in presolving", &propdata->probing, FALSE, DEFAULT_PROBING,
+         NULL, NULL) );
+   SCIP_CALL( SCIPaddIntParam(scip, "propagating/" PROP_NAME "/probingvectors",
+         "node vectors of one probing launch (two per probed binary)", &propdata->probingvectors, TRUE,
+         DEFAULT_PROBINGVECTORS, 2, 1 << 20, NULL, NULL) );
 
    return SCIP_OKAY;
 }

@@ -2,6 +2,7 @@
  *
  * It declares only the subset of SCIP's API that the gpulinear propagator calls, with the
  * signatures of SCIP's own declarations (src/scip/scip_prop.h, scip_var.h, scip_param.h,
+ * prop_probing.h (in scip/prop_probing.h here),
  * scip_numerics.h, scip_mem.h, pub_cons.h, pub_prop.h, pub_var.h), so that the plugin
  * compiles and runs under tests/ without a SCIP library.  Built against an installed SCIP,
  * the plugin uses SCIP's own headers instead and this directory is not on the include path.
@@ -105,6 +106,8 @@ void SCIPpropSetData(SCIP_PROP* prop, SCIP_PROPDATA* propdata);
 /* scip_param.h */
 SCIP_RETCODE SCIPaddIntParam(SCIP* scip, const char* name, const char* desc, int* valueptr, SCIP_Bool isadvanced,
    int defaultvalue, int minvalue, int maxvalue, SCIP_DECL_PARAMCHGD((*paramchgd)), SCIP_PARAMDATA* paramdata);
+SCIP_RETCODE SCIPaddBoolParam(SCIP* scip, const char* name, const char* desc, SCIP_Bool* valueptr,
+   SCIP_Bool isadvanced, SCIP_Bool defaultvalue, SCIP_DECL_PARAMCHGD((*paramchgd)), SCIP_PARAMDATA* paramdata);
 SCIP_RETCODE SCIPgetRealParam(SCIP* scip, const char* name, SCIP_Real* value);
 /* scip_numerics.h */
 SCIP_Real SCIPinfinity(SCIP* scip);
@@ -124,6 +127,7 @@ SCIP_VAR** SCIPgetVars(SCIP* scip);
 /* pub_var.h / scip_var.h */
 int SCIPvarGetProbindex(SCIP_VAR* var);
 SCIP_Bool SCIPvarIsIntegral(SCIP_VAR* var);
+SCIP_Bool SCIPvarIsBinary(SCIP_VAR* var);
 SCIP_Real SCIPvarGetLbGlobal(SCIP_VAR* var);
 SCIP_Real SCIPvarGetUbGlobal(SCIP_VAR* var);
 SCIP_Real SCIPvarGetLbLocal(SCIP_VAR* var);
@@ -138,6 +142,7 @@ SCIP_RETCODE SCIPtightenVarLb(SCIP* scip, SCIP_VAR* var, SCIP_Real newbound, SCI
    SCIP_Bool* tightened);
 SCIP_RETCODE SCIPtightenVarUb(SCIP* scip, SCIP_VAR* var, SCIP_Real newbound, SCIP_Bool force, SCIP_Bool* infeasible,
    SCIP_Bool* tightened);
+SCIP_RETCODE SCIPfixVar(SCIP* scip, SCIP_VAR* var, SCIP_Real fixedval, SCIP_Bool* infeasible, SCIP_Bool* fixed);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,9 @@
  * SCIPinferVarUbCons (src/scip/scip_var.c:6965-7174, var.c:1957-2021, set.c:7711-7753), so a
  * change the plugin hands over is accepted or rejected as SCIP would.
  *
+ * Probing deductions handed to SCIPanalyzeDeductionsProbing and fixings made with SCIPfixVar
+ * are recorded for the tests (th_get_probe_records).
+ *
  * Test infrastructure: built into scip_amd/plugin/libgpulinear_testhost.so for tests/ only.
  */
 #include <math.h>
@@ -14,6 +17,7 @@
 
 #include "scip/scip.h"
 #include "scip/cons_linear.h"
+#include "scip/prop_probing.h"
 #include "prop_gpulinear.h"
 
 struct SCIP_Var
@@ -65,6 +69,13 @@ struct Scip
    int nintparams;
    char intname[TH_MAXPARAMS][96];
    int* intptr[TH_MAXPARAMS];
+   /* records: SCIPfixVar calls (var, value) and SCIPanalyzeDeductionsProbing entries (probing
+    * var, var, bounds in the left / right branch) */
+   int nfix, fixcap;
+   struct FixRec { int var; double val; }* fix;
+   int nded, dedcap;
+   struct DedRec { int probe, var; double bnd[4]; }* ded;
+   int naggr, nimpl;
 };
 
 /* ---- numerics (def.h, set.c) ---- */
@@ -92,6 +103,21 @@ SCIP_RETCODE SCIPgetRealParam(SCIP* scip, const char* name, SCIP_Real* value)
       *value = scip->maxeasy;
    else
       return SCIP_PARAMETERUNKNOWN;
+   return SCIP_OKAY;
+}
+
+SCIP_RETCODE SCIPaddBoolParam(SCIP* scip, const char* name, const char* desc, SCIP_Bool* valueptr,
+   SCIP_Bool isadvanced, SCIP_Bool defaultvalue, SCIP_DECL_PARAMCHGD((*paramchgd)), SCIP_PARAMDATA* paramdata)
+{
+   (void)desc; (void)isadvanced; (void)paramchgd; (void)paramdata;
+   if( scip->nintparams >= TH_MAXPARAMS )
+      return SCIP_NOMEMORY;
+   /* kept with the int parameters (SCIP_Bool is an unsigned int) */
+   strncpy(scip->intname[scip->nintparams], name, 95);
+   scip->intptr[scip->nintparams] = (int*)valueptr;
+   scip->nintparams++;
+   if( valueptr != NULL )
+      *valueptr = defaultvalue;
    return SCIP_OKAY;
 }
 
@@ -151,6 +177,8 @@ int SCIPgetNVars(SCIP* scip) { return scip->nvars; }
 SCIP_VAR** SCIPgetVars(SCIP* scip) { return scip->vars; }
 int SCIPvarGetProbindex(SCIP_VAR* var) { return var->probindex; }
 SCIP_Bool SCIPvarIsIntegral(SCIP_VAR* var) { return var->integral != 0; }
+/* pub_var.h: an integral variable with global bounds within [0,1] */
+SCIP_Bool SCIPvarIsBinary(SCIP_VAR* var) { return var->integral != 0 && var->glb >= 0.0 && var->gub <= 1.0; }
 SCIP_Real SCIPvarGetLbGlobal(SCIP_VAR* var) { return var->glb; }
 SCIP_Real SCIPvarGetUbGlobal(SCIP_VAR* var) { return var->gub; }
 SCIP_Real SCIPvarGetLbLocal(SCIP_VAR* var) { return var->llb; }
@@ -294,6 +322,112 @@ SCIP_RETCODE SCIPtightenVarUb(SCIP* scip, SCIP_VAR* var, SCIP_Real newbound, SCI
    return chgUb(scip, var, newbound, force, TRUE, infeasible, tightened);
 }
 
+/* ---- presolving fixings and probing deductions ---- */
+static int grow(void** p, int* cap, int need, size_t elem)
+{
+   void* q;
+   int c;
+   if( need <= *cap )
+      return 1;
+   c = *cap > 0 ? 2 * *cap : 256;
+   while( c < need )
+      c *= 2;
+   q = realloc(*p, elem * (size_t)c);
+   if( q == NULL )
+      return 0;
+   *p = q;
+   *cap = c;
+   return 1;
+}
+
+/* SCIPfixVar in presolving (scip_var.c): infeasible if the value lies outside the global
+ * bounds (feasibility tolerance) or is fractional for an integral variable; else both bounds
+ * are set to the value */
+SCIP_RETCODE SCIPfixVar(SCIP* scip, SCIP_VAR* var, SCIP_Real fixedval, SCIP_Bool* infeasible, SCIP_Bool* fixed)
+{
+   *fixed = FALSE;
+   *infeasible = (relDiff(fixedval, var->glb) < -scip->feastol || relDiff(fixedval, var->gub) > scip->feastol
+      || (var->integral && fabs(fixedval - floor(fixedval + 0.5)) > scip->feastol));
+   if( *infeasible )
+      return SCIP_OKAY;
+   if( !grow((void**)&scip->fix, &scip->fixcap, scip->nfix + 1, sizeof(*scip->fix)) )
+      return SCIP_NOMEMORY;
+   scip->fix[scip->nfix].var = var->probindex;
+   scip->fix[scip->nfix].val = fixedval;
+   scip->nfix++;
+   *fixed = !(var->glb == fixedval && var->gub == fixedval);
+   var->glb = var->gub = var->llb = var->lub = fixedval;
+   return SCIP_OKAY;
+}
+
+/* test double of SCIPanalyzeDeductionsProbing (prop_probing.c:1314-1640), for a binary probing
+ * variable with left branch x <= 0 and right branch x >= 1: records every entry, and applies
+ * what SCIP derives from the union of the two branches -- a fixing when the union is one value,
+ * else tightenings (integral: by more than 0.5; continuous: SCIPisLb/UbBetter) -- and counts
+ * the aggregations (fixed in both branches) and the implications (fixed at a union end in one
+ * branch) that SCIP would add */
+SCIP_RETCODE SCIPanalyzeDeductionsProbing(SCIP* scip, SCIP_VAR* probingvar, SCIP_Real leftub, SCIP_Real rightlb,
+   int nvars, SCIP_VAR** vars, SCIP_Real* leftimpllbs, SCIP_Real* leftimplubs, SCIP_Real* leftproplbs,
+   SCIP_Real* leftpropubs, SCIP_Real* rightimpllbs, SCIP_Real* rightimplubs, SCIP_Real* rightproplbs,
+   SCIP_Real* rightpropubs, int* nfixedvars, int* naggrvars, int* nimplications, int* nchgbds, SCIP_Bool* cutoff)
+{
+   int j;
+   (void)leftub; (void)rightlb; (void)leftimpllbs; (void)leftimplubs; (void)rightimpllbs; (void)rightimplubs;
+   *cutoff = FALSE;
+   for( j = 0; j < nvars && !*cutoff; ++j )
+   {
+      SCIP_VAR* var = vars[j];
+      double newlb = leftproplbs[j] < rightproplbs[j] ? leftproplbs[j] : rightproplbs[j];
+      double newub = leftpropubs[j] > rightpropubs[j] ? leftpropubs[j] : rightpropubs[j];
+      SCIP_Bool tightened;
+      struct DedRec* r;
+
+      if( var == probingvar )
+         continue;
+      if( !grow((void**)&scip->ded, &scip->dedcap, scip->nded + 1, sizeof(*scip->ded)) )
+         return SCIP_NOMEMORY;
+      r = &scip->ded[scip->nded++];
+      r->probe = probingvar->probindex;
+      r->var = var->probindex;
+      r->bnd[0] = leftproplbs[j];
+      r->bnd[1] = leftpropubs[j];
+      r->bnd[2] = rightproplbs[j];
+      r->bnd[3] = rightpropubs[j];
+
+      if( fabs(newlb - newub) <= scip->epsilon )
+      {
+         SCIP_Bool fixed;
+         SCIP_CALL( SCIPfixVar(scip, var, newlb, cutoff, &fixed) );
+         if( fixed )
+            (*nfixedvars)++;
+         continue;
+      }
+      if( var->integral ? newlb > var->glb + 0.5 : lbBetter(scip, newlb, var->glb, var->gub) )
+      {
+         SCIP_CALL( SCIPtightenVarLb(scip, var, newlb, TRUE, cutoff, &tightened) );
+         *nchgbds += tightened ? 1 : 0;
+      }
+      if( !*cutoff && (var->integral ? newub < var->gub - 0.5 : ubBetter(scip, newub, var->glb, var->gub)) )
+      {
+         SCIP_CALL( SCIPtightenVarUb(scip, var, newub, TRUE, cutoff, &tightened) );
+         *nchgbds += tightened ? 1 : 0;
+      }
+      if( fabs(leftproplbs[j] - leftpropubs[j]) <= scip->epsilon
+         && fabs(rightproplbs[j] - rightpropubs[j]) <= scip->epsilon )
+      {
+         (*naggrvars)++;
+         scip->naggr++;
+      }
+      else if( fabs(newlb - leftpropubs[j]) <= scip->epsilon || fabs(newub - leftproplbs[j]) <= scip->epsilon
+         || fabs(newlb - rightpropubs[j]) <= scip->epsilon || fabs(newub - rightproplbs[j]) <= scip->epsilon )
+      {
+         (*nimplications)++;
+         scip->nimpl++;
+      }
+   }
+   return SCIP_OKAY;
+}
+
 /* ---- driver (ctypes entry points for tests) ---- */
 SCIP_EXPORT SCIP* th_create(void)
 {
@@ -432,6 +566,29 @@ SCIP_EXPORT void th_get_bounds(SCIP* s, int global, double* lb, double* ub)
    }
 }
 
+/* counts of the records; with non-NULL arrays they are copied out as well */
+SCIP_EXPORT int th_get_probe_records(SCIP* s, int* nfix, int* fixvar, double* fixval, int* nded, int* dedprobe,
+   int* dedvar, double* dedbnd, int* naggr, int* nimpl)
+{
+   int i;
+   *nfix = s->nfix;
+   *nded = s->nded;
+   *naggr = s->naggr;
+   *nimpl = s->nimpl;
+   for( i = 0; fixvar != NULL && i < s->nfix; ++i )
+   {
+      fixvar[i] = s->fix[i].var;
+      fixval[i] = s->fix[i].val;
+   }
+   for( i = 0; dedprobe != NULL && i < s->nded; ++i )
+   {
+      dedprobe[i] = s->ded[i].probe;
+      dedvar[i] = s->ded[i].var;
+      memcpy(dedbnd + 4 * (size_t)i, s->ded[i].bnd, 4 * sizeof(double));
+   }
+   return SCIP_OKAY;
+}
+
 SCIP_EXPORT int th_free(SCIP* s)
 {
    int i, rc = SCIP_OKAY;
@@ -448,6 +605,8 @@ SCIP_EXPORT int th_free(SCIP* s)
    for( i = 0; i < s->nvars; ++i )
       free(s->vars[i]);
    free(s->vars);
+   free(s->fix);
+   free(s->ded);
    free(s);
    return rc;
 }

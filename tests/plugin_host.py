@@ -42,6 +42,9 @@ def lib():
         L.th_set_local_bounds.restype = None
         L.th_get_bounds.argtypes = [C.c_void_p, C.c_int, _P(np.float64), _P(np.float64)]
         L.th_get_bounds.restype = None
+        IP = C.POINTER(C.c_int)
+        L.th_get_probe_records.argtypes = [C.c_void_p, IP, C.c_void_p, C.c_void_p, IP, C.c_void_p, C.c_void_p,
+                                           C.c_void_p, IP, IP]
         _lib = L
     return _lib
 
@@ -91,6 +94,23 @@ class Host:
         ub = np.empty(self.n)
         lib().th_get_bounds(self.h, 1 if glob else 0, lb, ub)
         return lb, ub
+
+    def probe_records(self):
+        """(fixings [(var, value)], deductions [(probing var, var, lb0, ub0, lb1, ub1)], naggr, nimpl)"""
+        L = lib()
+        nf, nd, na, ni = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+        L.th_get_probe_records(self.h, C.byref(nf), None, None, C.byref(nd), None, None, None, C.byref(na),
+                               C.byref(ni))
+        fv = np.zeros(max(nf.value, 1), dtype=np.int32)
+        fx = np.zeros(max(nf.value, 1))
+        dp = np.zeros(max(nd.value, 1), dtype=np.int32)
+        dv = np.zeros(max(nd.value, 1), dtype=np.int32)
+        db = np.zeros((max(nd.value, 1), 4))
+        L.th_get_probe_records(self.h, C.byref(nf), fv.ctypes.data, fx.ctypes.data, C.byref(nd), dp.ctypes.data,
+                               dv.ctypes.data, db.ctypes.data, C.byref(na), C.byref(ni))
+        fix = [(int(fv[i]), float(fx[i])) for i in range(nf.value)]
+        ded = [(int(dp[i]), int(dv[i])) + tuple(float(x) for x in db[i]) for i in range(nd.value)]
+        return fix, ded, na.value, ni.value
 
     def close(self):
         if self.h:

@@ -25,6 +25,8 @@ def test_plugin_registers_parameters():
     h = ph.Host(inst)
     assert h.call("th_set_int_param", b"propagating/gpulinear/maxrounds", 5) == ph.SCIP_OKAY
     assert h.call("th_set_int_param", b"propagating/gpulinear/device", 0) == ph.SCIP_OKAY
+    assert h.call("th_set_int_param", b"propagating/gpulinear/probing", 1) == ph.SCIP_OKAY
+    assert h.call("th_set_int_param", b"propagating/gpulinear/probingvectors", 64) == ph.SCIP_OKAY
     assert h.call("th_set_int_param", b"propagating/gpulinear/nosuch", 0) != ph.SCIP_OKAY
     assert h.close() == ph.SCIP_OKAY
 
@@ -130,4 +132,66 @@ def test_cutoff_reported(gpu_ctx):
     rc, res = h.propexec()
     assert rc == ph.SCIP_OKAY
     assert orr["cutoff"] and res == ph.SCIP_CUTOFF
+    assert h.close() == ph.SCIP_OKAY
+
+
+def expected_probing(om, rl, ru, integral):
+    """What the plugin's presolve probing hands to SCIP, in order: probes in column order over
+    the unfixed binaries of the fixpoint, skipping a column fixed by an earlier probe; a cut-off
+    branch fixes the column (prop_probing.c:477-556); otherwise the columns changed in both
+    branches go to SCIPanalyzeDeductionsProbing, whose union fixings and tightenings are
+    applied as the test host applies them (binary columns here: a tightening is a fixing)."""
+    probecols = np.nonzero((integral != 0) & (rl == 0.0) & (ru == 1.0))[0]
+    ref = dict(zip(probecols.tolist(), orc.probe(om, rl, ru, probecols)))
+    glb, gub = rl.copy(), ru.copy()
+    fix, ded = [], []
+    for x in probecols.tolist():
+        if glb[x] > 0.5 or gub[x] < 0.5:
+            continue
+        r = ref[x]
+        assert r["verdict"] != 3
+        if r["verdict"]:
+            val = 0.0 if r["verdict"] & 2 else 1.0
+            fix.append((x, val))
+            glb[x] = gub[x] = val
+            continue
+        for k, j in enumerate(r["cols"].tolist()):
+            (l0, u0), (l1, u1) = r["b0"][k], r["b1"][k]
+            ded.append((x, j, l0, u0, l1, u1))
+            lo, hi = min(l0, l1), max(u0, u1)
+            if lo == hi:
+                fix.append((j, lo))
+                glb[j] = gub[j] = lo
+            else:
+                if lo > glb[j] + 0.5:
+                    glb[j] = lo
+                if hi < gub[j] - 0.5:
+                    gub[j] = hi
+    return fix, ded, glb, gub
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [21, 25, 27])
+def test_presolve_probing_hands_the_device_branches_to_scip(gpu_ctx, seed):
+    rng = np.random.default_rng(seed)
+    inst = random_binary_instance(rng, nrows=900, ncols=600, maxlen=10)
+    om = orc.OMatrix.from_instance(inst)
+    rl, ru, rr = orc.propagate_jacobi(om, inst.lb, inst.ub)
+    assert not rr["cutoff"]
+    print(f"[case] plugin probing seed={seed}", flush=True)
+    h = ph.Host(inst)
+    assert h.call("th_set_int_param", b"propagating/gpulinear/probing", 1) == ph.SCIP_OKAY
+    assert h.call("th_set_int_param", b"propagating/gpulinear/probingvectors", 128) == ph.SCIP_OKAY
+    assert h.call("th_initpre") == ph.SCIP_OKAY
+    rc, res, nchg = h.presol()
+    assert rc == ph.SCIP_OKAY and res == ph.SCIP_SUCCESS
+    fix, ded, naggr, nimpl = h.probe_records()
+    efix, eded, glb, gub = expected_probing(om, rl, ru, inst.integral)
+    assert fix == efix
+    assert len(ded) == len(eded) and all(np.array_equal(np.array(a), np.array(b)) for a, b in zip(ded, eded))
+    gl, gu = h.bounds(glob=True)
+    assert np.array_equal(gl, glb) and np.array_equal(gu, gub)
+    # the case finds fixings from cut-off branches and pairs for SCIP's analysis
+    assert len(efix) > 0 and len(eded) > 0
+    assert h.call("th_exitpre") == ph.SCIP_OKAY
     assert h.close() == ph.SCIP_OKAY
