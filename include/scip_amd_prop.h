@@ -140,6 +140,49 @@ int sap_matrix_info(const sap_matrix* mat, int64_t* nrows, int64_t* ncols, int64
  * detected (bounds are then unspecified), else SAP_REDUCEDDOM or SAP_DIDNOTFIND. */
 int sap_propagate(sap_matrix* mat, const sap_params* params, double* lb, double* ub, sap_stats* stats);
 
+/* Propagation with reasons, for conflict analysis (PROPRESPROP, prop.c:739-788; the role of
+ * cons_linear.c's inferinfo, :344-369, resolvePropagation :5097-5189 and analyzeConflict).
+ * Same propagation and results as sap_propagate; in addition every accepted bound change is
+ * returned as a history entry, sorted by round, column and bound (lower first):
+ *   hround   the round (1, 2, ...) whose column step accepted it; the row sweep of that round
+ *            read the bounds after all changes of the earlier rounds;
+ *   hcol, hbound (0 lower, 1 upper), hval  the column and its new bound;
+ *   hrow     the reason: the smallest row whose residual activity (the other columns' bounds
+ *            at the start of that round) gave the winning candidate.  For a lower bound of a
+ *            column with coefficient a > 0 it is the row's lhs against its maximal residual
+ *            activity, with a < 0 the rhs against the minimal one; symmetric for upper bounds.
+ *            Applied in history order, each change is implied by its row and the bounds
+ *            already applied, so the row's bounds of the other columns at the time of the
+ *            change explain it (cons_linear.c addConflictBounds).
+ * *nhist is always set; if it exceeds histcap the history is not written and SAP_NOMEMORY is
+ * returned (bounds and conflict are).  On a cutoff, conflict describes the infeasibility found
+ * in the last round (the changes of the earlier rounds are in the history):
+ *   SAP_CONFLICT_ROW     the smallest row whose activity bounds miss a side: side 0, the
+ *                        minimal activity (lower bounds where a > 0, upper where a < 0)
+ *                        exceeds rhs; side 1, the maximal activity is below lhs;
+ *   SAP_CONFLICT_COLUMN  the smallest column whose candidate bounds crossed: rowlb implied its
+ *                        lower bound (-1: the column's own lower bound), rowub its upper one
+ *                        (-1: its own upper bound); the rows' residual activities are taken as
+ *                        for history entries.
+ * One instance (not a packed matrix); rows < 2^29, columns < 2^30. */
+#define SAP_CONFLICT_NONE   0
+#define SAP_CONFLICT_ROW    1
+#define SAP_CONFLICT_COLUMN 2
+typedef struct sap_conflict
+{
+   int32_t kind;
+   int32_t round;
+   int32_t row;
+   int32_t side;
+   int32_t col;
+   int32_t rowlb;
+   int32_t rowub;
+   int32_t pad;
+} sap_conflict;
+int sap_propagate_reasons(sap_matrix* mat, const sap_params* params, double* lb, double* ub, int64_t histcap,
+   int64_t* nhist, int32_t* hround, int32_t* hcol, int32_t* hbound, double* hval, int32_t* hrow,
+   sap_conflict* conflict, sap_stats* stats);
+
 /* Device-resident bound vector(s), for callers that keep bounds in HBM across calls. */
 int sap_bounds_create(sap_matrix* mat, int32_t nvec, sap_bounds** bnd);
 /* as sap_bounds_create with an explicit work-tracking mode: SAP_MODE_DENSE scans round-tagged

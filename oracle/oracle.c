@@ -600,21 +600,44 @@ typedef struct
    double* fub;   /* forced ub candidates (min) */
    uint8_t* touched;
    int cutoff;
+   /* reasons: the smallest row that submitted each slot's best candidate (NULL: not kept),
+    * the row being swept, and the side of a row's infeasibility (0: minimal activity
+    * against rhs, 1: maximal activity against lhs) */
+   int32_t* clbrow;
+   int32_t* cubrow;
+   int32_t* flbrow;
+   int32_t* fubrow;
+   int32_t currow;
+   int rowside;
 } jstate;
 
 static inline void submitLb(jstate* js, int32_t j, double v, int force)
 {
    double* t = force ? &js->flb[j] : &js->clb[j];
+   int32_t* r = js->clbrow == NULL ? NULL : (force ? &js->flbrow[j] : &js->clbrow[j]);
    if( dkey(v) > dkey(*t) )
+   {
       *t = v;
+      if( r )
+         *r = js->currow;
+   }
+   else if( r && dkey(v) == dkey(*t) && js->currow < *r )
+      *r = js->currow;
    js->touched[j] = 1;
 }
 
 static inline void submitUb(jstate* js, int32_t j, double v, int force)
 {
    double* t = force ? &js->fub[j] : &js->cub[j];
+   int32_t* r = js->cubrow == NULL ? NULL : (force ? &js->fubrow[j] : &js->cubrow[j]);
    if( dkey(v) < dkey(*t) )
+   {
       *t = v;
+      if( r )
+         *r = js->currow;
+   }
+   else if( r && dkey(v) == dkey(*t) && js->currow < *r )
+      *r = js->currow;
    js->touched[j] = 1;
 }
 
@@ -661,6 +684,7 @@ static void jEasy(jstate* js, const rowstate* rs, int64_t i, int64_t k, int forc
          if( isFeasLT(p, rhs, minact) )
          {
             js->cutoff = 1;
+            js->rowside = 0;
             return;
          }
          slack = rhs - minact;
@@ -680,6 +704,7 @@ static void jEasy(jstate* js, const rowstate* rs, int64_t i, int64_t k, int forc
          if( isFeasLT(p, maxact, lhs) )
          {
             js->cutoff = 1;
+            js->rowside = 1;
             return;
          }
          slack = maxact - lhs;
@@ -702,6 +727,7 @@ static void jEasy(jstate* js, const rowstate* rs, int64_t i, int64_t k, int forc
          if( isFeasLT(p, rhs, minact) )
          {
             js->cutoff = 1;
+            js->rowside = 0;
             return;
          }
          slack = rhs - minact;
@@ -721,6 +747,7 @@ static void jEasy(jstate* js, const rowstate* rs, int64_t i, int64_t k, int forc
          if( isFeasLT(p, maxact, lhs) )
          {
             js->cutoff = 1;
+            js->rowside = 1;
             return;
          }
          slack = maxact - lhs;
@@ -884,6 +911,7 @@ static int jRow(jstate* js, int64_t i, double* xmin, double* xmax, dd* scratch)
    if( isFeasGT(p, minact, m->rhs[i]) || isFeasLT(p, maxact, m->lhs[i]) )
    {
       js->cutoff = 1;
+      js->rowside = isFeasGT(p, minact, m->rhs[i]) ? 0 : 1;
       return 0;
    }
    if( isGE(p, minact, m->lhs[i]) && isLE(p, maxact, m->rhs[i]) )
@@ -894,11 +922,17 @@ static int jRow(jstate* js, int64_t i, double* xmin, double* xmax, dd* scratch)
 /* column step: SCIPinferVarLbCons then SCIPinferVarUbCons (scip_var.c:6965-7174) on the
  * tightest forced and regular candidates; returns 1 if a bound changed, sets *cutoff */
 static int jColumn(const orc_params* p, int integral, double* plb, double* pub, double clb, double flb,
-   double cub, double fub, int* cutoff, int64_t* nchg)
+   double cub, double fub, int* cutoff, int64_t* nchg, int* src)
 {
    double lb0 = *plb, ub0 = *pub;
    double nlb = lb0, nub = ub0;
    int changed = 0;
+   /* src[0] / src[1]: which candidate set the new lower / upper bound, or crossed the other
+    * bound on a cutoff (0: none, 1: forced, 2: regular) */
+   int dummy[2];
+   if( src == NULL )
+      src = dummy;
+   src[0] = src[1] = 0;
 
    if( flb != -HUGE_VAL )
    {
@@ -906,11 +940,15 @@ static int jColumn(const orc_params* p, int integral, double* plb, double* pub, 
       if( isInf(p, t) || isFeasGT(p, t, ub0) )
       {
          *cutoff = 1;
+         src[0] = 1;
          return 0;
       }
       t = MIND(t, ub0);
       if( !isLE(p, t, lb0) && dkey(t) > dkey(nlb) )
+      {
          nlb = t;
+         src[0] = 1;
+      }
    }
    if( clb != -HUGE_VAL )
    {
@@ -918,11 +956,15 @@ static int jColumn(const orc_params* p, int integral, double* plb, double* pub, 
       if( isInf(p, t) || isFeasGT(p, t, ub0) )
       {
          *cutoff = 1;
+         src[0] = 2;
          return 0;
       }
       t = MIND(t, ub0);
       if( isLbBetter(p, t, lb0, ub0) && dkey(t) > dkey(nlb) )
+      {
          nlb = t;
+         src[0] = 2;
+      }
    }
    if( fub != HUGE_VAL )
    {
@@ -930,11 +972,15 @@ static int jColumn(const orc_params* p, int integral, double* plb, double* pub, 
       if( isInf(p, -t) || isFeasLT(p, t, nlb) )
       {
          *cutoff = 1;
+         src[1] = 1;
          return 0;
       }
       t = MAXD(t, nlb);
       if( !isGE(p, t, ub0) && dkey(t) < dkey(nub) )
+      {
          nub = t;
+         src[1] = 1;
+      }
    }
    if( cub != HUGE_VAL )
    {
@@ -942,11 +988,15 @@ static int jColumn(const orc_params* p, int integral, double* plb, double* pub, 
       if( isInf(p, -t) || isFeasLT(p, t, nlb) )
       {
          *cutoff = 1;
+         src[1] = 2;
          return 0;
       }
       t = MAXD(t, nlb);
       if( isUbBetter(p, t, nlb, ub0) && dkey(t) < dkey(nub) )
+      {
          nub = t;
+         src[1] = 2;
+      }
    }
    if( nlb != lb0 || dkey(nlb) != dkey(lb0) )
    {
@@ -954,17 +1004,48 @@ static int jColumn(const orc_params* p, int integral, double* plb, double* pub, 
       ++(*nchg);
       changed = 1;
    }
+   else
+      src[0] = 0;
    if( nub != ub0 || dkey(nub) != dkey(ub0) )
    {
       *pub = nub;
       ++(*nchg);
       changed = 1;
    }
+   else
+      src[1] = 0;
    return changed;
 }
 
-int orc_propagate_jacobi(const orc_matrix* m, const orc_params* p, double* lb, double* ub, int full_sweep,
-   orc_result* res)
+/* reason recording of the Jacobi twin: the history of accepted changes with the smallest row
+ * that submitted each winning candidate, and the infeasibility certificate of a cutoff */
+typedef struct
+{
+   int64_t cap;
+   int64_t n;
+   int32_t* round;
+   int32_t* col;
+   int32_t* bound;
+   double* val;
+   int32_t* row;
+   orc_conflict* conflict;
+} jreasons;
+
+static void histAdd(jreasons* R, int32_t round, int32_t j, int32_t bound, double val, int32_t row)
+{
+   if( R->n < R->cap )
+   {
+      R->round[R->n] = round;
+      R->col[R->n] = j;
+      R->bound[R->n] = bound;
+      R->val[R->n] = val;
+      R->row[R->n] = row;
+   }
+   R->n++;
+}
+
+static int jacobi(const orc_matrix* m, const orc_params* p, double* lb, double* ub, int full_sweep,
+   orc_result* res, jreasons* R)
 {
    int64_t nr = m->nrows, nc = m->ncols, i, j, k;
    int64_t L = maxRowLen(m), P = 1;
@@ -976,6 +1057,7 @@ int orc_propagate_jacobi(const orc_matrix* m, const orc_params* p, double* lb, d
    dd* scratch;
    jstate js;
    int round = 0;
+   int64_t nsave;
 
    while( P < L )
       P <<= 1;
@@ -991,6 +1073,16 @@ int orc_propagate_jacobi(const orc_matrix* m, const orc_params* p, double* lb, d
    js.flb = (double*)malloc(sizeof(double) * (size_t)(nc > 0 ? nc : 1));
    js.fub = (double*)malloc(sizeof(double) * (size_t)(nc > 0 ? nc : 1));
    js.touched = (uint8_t*)calloc((size_t)(nc > 0 ? nc : 1), 1);
+   if( R != NULL )
+   {
+      js.clbrow = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nc > 0 ? nc : 1));
+      js.cubrow = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nc > 0 ? nc : 1));
+      js.flbrow = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nc > 0 ? nc : 1));
+      js.fubrow = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nc > 0 ? nc : 1));
+      memset(R->conflict, 0, sizeof(*R->conflict));
+      R->conflict->row = R->conflict->col = R->conflict->rowlb = R->conflict->rowub = -1;
+      R->n = 0;
+   }
    for( j = 0; j < nc; ++j )
    {
       js.clb[j] = -HUGE_VAL;
@@ -1013,32 +1105,71 @@ int orc_propagate_jacobi(const orc_matrix* m, const orc_params* p, double* lb, d
          break;
       ++round;
 
-      for( i = 0; i < nr && !js.cutoff; ++i )
+      /* with reasons every marked row of the round is swept, and the smallest infeasible row
+       * is the certificate */
+      for( i = 0; i < nr && (R != NULL || !js.cutoff); ++i )
       {
+         int before = js.cutoff;
          if( redundant[i] || !(marked[i] || full_sweep) )
             continue;
          res->rows_swept++;
          res->nnz_swept += m->rowptr[i + 1] - m->rowptr[i];
+         js.cutoff = 0;
+         js.currow = (int32_t)i;
          if( jRow(&js, i, xmin, xmax, scratch) )
             redundant[i] = 1;
+         if( js.cutoff && R != NULL && R->conflict->kind == 0 )
+         {
+            R->conflict->kind = ORC_CONFLICT_ROW;
+            R->conflict->row = (int32_t)i;
+            R->conflict->side = js.rowside;
+            R->conflict->round = round;
+         }
+         js.cutoff |= before;
       }
       if( js.cutoff )
          break;
 
+      /* a round ending infeasible applies nothing: its entries are dropped below */
+      nsave = R != NULL ? R->n : 0;
       for( j = 0; j < nc; ++j )
       {
+         int src[2], wascut = js.cutoff;
          colchg[j] = 0;
          if( !js.touched[j] )
             continue;
          js.touched[j] = 0;
          colchg[j] = (uint8_t)jColumn(p, m->integral[j], &lb[j], &ub[j], js.clb[j], js.flb[j], js.cub[j], js.fub[j],
-            &js.cutoff, &res->nchgbds);
+            &js.cutoff, &res->nchgbds, src);
+         if( R != NULL )
+         {
+            int32_t rl = src[0] == 1 ? js.flbrow[j] : src[0] == 2 ? js.clbrow[j] : -1;
+            int32_t ru = src[1] == 1 ? js.fubrow[j] : src[1] == 2 ? js.cubrow[j] : -1;
+            if( js.cutoff && !wascut && R->conflict->kind == 0 )
+            {
+               /* the first (smallest) column whose bounds crossed */
+               R->conflict->kind = ORC_CONFLICT_COLUMN;
+               R->conflict->col = (int32_t)j;
+               R->conflict->rowlb = rl;
+               R->conflict->rowub = ru;
+               R->conflict->round = round;
+            }
+            else if( colchg[j] )
+            {
+               if( src[0] )
+                  histAdd(R, round, (int32_t)j, 0, lb[j], rl);
+               if( src[1] )
+                  histAdd(R, round, (int32_t)j, 1, ub[j], ru);
+            }
+         }
          js.clb[j] = -HUGE_VAL;
          js.flb[j] = -HUGE_VAL;
          js.cub[j] = HUGE_VAL;
          js.fub[j] = HUGE_VAL;
          anychg |= colchg[j];
       }
+      if( js.cutoff && R != NULL )
+         R->n = nsave;
       if( js.cutoff )
          break;
       for( i = 0; i < nr; ++i )
@@ -1068,7 +1199,38 @@ int orc_propagate_jacobi(const orc_matrix* m, const orc_params* p, double* lb, d
    free(js.flb);
    free(js.fub);
    free(js.touched);
+   free(js.clbrow);
+   free(js.cubrow);
+   free(js.flbrow);
+   free(js.fubrow);
    return 0;
+}
+
+int orc_propagate_jacobi(const orc_matrix* m, const orc_params* p, double* lb, double* ub, int full_sweep,
+   orc_result* res)
+{
+   return jacobi(m, p, lb, ub, full_sweep, res, NULL);
+}
+
+int orc_propagate_jacobi_reasons(const orc_matrix* m, const orc_params* p, double* lb, double* ub, int64_t histcap,
+   int64_t* nhist, int32_t* hround, int32_t* hcol, int32_t* hbound, double* hval, int32_t* hrow,
+   orc_conflict* conflict, orc_result* res)
+{
+   jreasons R;
+   int rc;
+   R.cap = histcap;
+   R.n = 0;
+   R.round = hround;
+   R.col = hcol;
+   R.bound = hbound;
+   R.val = hval;
+   R.row = hrow;
+   R.conflict = conflict;
+   rc = jacobi(m, p, lb, ub, 0, res, &R);
+   *nhist = R.n;
+   if( !res->cutoff )
+      conflict->kind = ORC_CONFLICT_NONE;
+   return rc;
 }
 
 /* ==========================================================================================

@@ -91,6 +91,30 @@ int orc_propagate_jacobi(const orc_matrix* m, const orc_params* p, double* lb, d
 int orc_propagate_scip(const orc_matrix* m, const orc_params* p, double* lb, double* ub,
    orc_result* res);
 
+/* the Jacobi twin with reasons (restates the reason recording of sap_propagate_reasons):
+ * history entries (round, column, bound 0 lower / 1 upper, value, the smallest row that
+ * submitted the winning candidate) in round, column, bound order, and the certificate of a
+ * cutoff: the smallest infeasible row with its side, or the smallest column whose bounds
+ * crossed with the rows of its two sides (-1: the column's own bound).  *nhist counts all
+ * entries; at most histcap are written. */
+#define ORC_CONFLICT_NONE   0
+#define ORC_CONFLICT_ROW    1
+#define ORC_CONFLICT_COLUMN 2
+typedef struct orc_conflict
+{
+   int32_t kind;
+   int32_t round;
+   int32_t row;
+   int32_t side;
+   int32_t col;
+   int32_t rowlb;
+   int32_t rowub;
+   int32_t pad;
+} orc_conflict;
+int orc_propagate_jacobi_reasons(const orc_matrix* m, const orc_params* p, double* lb, double* ub, int64_t histcap,
+   int64_t* nhist, int32_t* hround, int32_t* hcol, int32_t* hbound, double* hval, int32_t* hrow,
+   orc_conflict* conflict, orc_result* res);
+
 /* double-double primitives restated from src/scip/dbldblarith.h, exported for the
  * known-answer tests of tests/src/misc/dbldblarith.c */
 void orc_dd_sum(double a, double b, double* rhi, double* rlo);

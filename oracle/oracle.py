@@ -31,6 +31,13 @@ class orc_matrix(C.Structure):
     ]
 
 
+class orc_conflict(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("kind", "round", "row", "side", "col", "rowlb", "rowub", "pad")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
 class orc_result(C.Structure):
     _fields_ = [("cutoff", C.c_int32), ("rounds", C.c_int32), ("nchgbds", C.c_int64), ("rows_swept", C.c_int64),
                 ("nnz_swept", C.c_int64)]
@@ -68,6 +75,10 @@ def lib():
             getattr(L, n).argtypes = [C.c_double, C.c_double, C.c_double, _DP, _DP]
         L.orc_dd_pairadd.argtypes = [C.c_double] * 4 + [_DP, _DP]
         L.orc_canonical_sum.argtypes = [_DP, C.c_int64, _DP, _DP]
+        _I32P = C.POINTER(C.c_int32)
+        L.orc_propagate_jacobi_reasons.argtypes = [C.POINTER(orc_matrix), C.POINTER(orc_params), _DP, _DP, C.c_int64,
+                                                   C.POINTER(C.c_int64), _I32P, _I32P, _I32P, _DP, _I32P,
+                                                   C.POINTER(orc_conflict), C.POINTER(orc_result)]
         _lib = L
     return _lib
 
@@ -135,6 +146,32 @@ def propagate_jacobi(om: OMatrix, lb, ub, prm: orc_params | None = None, full_sw
     lib().orc_propagate_jacobi(C.byref(om.m), C.byref(prm or params()), lb.ctypes.data_as(_DP),
                                ub.ctypes.data_as(_DP), 1 if full_sweep else 0, C.byref(r))
     return lb, ub, r.as_dict()
+
+
+def propagate_reasons(om: OMatrix, lb, ub, prm: orc_params | None = None):
+    """The Jacobi twin with reasons: bounds, result, history (round, col, bound, val, row) and
+    the conflict of a cutoff -- the same record the engine's sap_propagate_reasons returns."""
+    lb = np.array(lb, dtype=np.float64, copy=True)
+    ub = np.array(ub, dtype=np.float64, copy=True)
+    I32 = C.POINTER(C.c_int32)
+    cap = max(1024, 2 * int(om.m.ncols))
+    while True:
+        l, u = lb.copy(), ub.copy()
+        hr, hc, hb, hw = (np.zeros(cap, dtype=np.int32) for _ in range(4))
+        hv = np.zeros(cap)
+        n = C.c_int64(0)
+        cf = orc_conflict()
+        r = orc_result()
+        lib().orc_propagate_jacobi_reasons(C.byref(om.m), C.byref(prm or params()), l.ctypes.data_as(_DP),
+                                           u.ctypes.data_as(_DP), cap, C.byref(n), hr.ctypes.data_as(I32),
+                                           hc.ctypes.data_as(I32), hb.ctypes.data_as(I32), hv.ctypes.data_as(_DP),
+                                           hw.ctypes.data_as(I32), C.byref(cf), C.byref(r))
+        if n.value > cap:
+            cap = n.value
+            continue
+        k = n.value
+        return dict(lb=l, ub=u, result=r.as_dict(), conflict=cf.as_dict(), nhist=k,
+                    round=hr[:k], col=hc[:k], bound=hb[:k], val=hv[:k], row=hw[:k])
 
 
 def propagate_scip(om: OMatrix, lb, ub, prm: orc_params | None = None):

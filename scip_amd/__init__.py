@@ -41,6 +41,10 @@ SAP_FLAG_HOSTLOOP = 0x2
 SAP_STEP_SHORT = 2
 SAP_STEP_LONG = 8
 
+SAP_CONFLICT_NONE = 0
+SAP_CONFLICT_ROW = 1
+SAP_CONFLICT_COLUMN = 2
+
 SAP_PROBE_DOWN_INFEASIBLE = 1
 SAP_PROBE_UP_INFEASIBLE = 2
 
@@ -77,6 +81,13 @@ class sap_tuning(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class sap_conflict(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("kind", "round", "row", "side", "col", "rowlb", "rowub", "pad")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
 class sap_stats(C.Structure):
@@ -169,6 +180,8 @@ EXPORTS = {
                                       _I32P, _I32P, _DP, _DP, C.POINTER(sap_stats)]),
     "sap_propagate_nodes": (C.c_int, [_P, C.POINTER(sap_params), C.c_int32, _P, C.c_int32, _I64P, _I32P, _DP, _DP,
                                       _I32P, _I32P, C.c_int64, _I64P, _I32P, _DP, _DP, C.POINTER(sap_stats)]),
+    "sap_propagate_reasons": (C.c_int, [_P, C.POINTER(sap_params), _DP, _DP, C.c_int64, _I64P, _I32P, _I32P, _I32P,
+                                        _DP, _I32P, C.POINTER(sap_conflict), C.POINTER(sap_stats)]),
     "sap_probe": (C.c_int, [_P, C.POINTER(sap_params), C.c_int32, _I32P, _P, C.c_int32, _I32P, _I32P, C.c_int64,
                             _I64P, _I32P, _DP, _DP, _DP, _DP, C.POINTER(sap_stats)]),
     "sap_matrix_create_packed": (C.c_int, [_P, C.c_int32, _I64P, _I64P, C.c_int64, C.c_int64, _I64P, _I32P, _DP,
@@ -427,6 +440,30 @@ class Matrix:
         p = params or default_params()
         _chk(lib().sap_propagate(self.h, C.byref(p), _dp(lb), _dp(ub), C.byref(st)), "sap_propagate")
         return lb, ub, st.as_dict()
+
+    def propagate_reasons(self, lb=None, ub=None, params: sap_params | None = None, histcap: int | None = None):
+        """Propagate one bound vector with reasons: bounds, stats, the history of accepted
+        changes (round, col, bound, val, row arrays) and the conflict of a cutoff."""
+        lb0 = np.array(self.inst.lb if lb is None else lb, dtype=np.float64, copy=True)
+        ub0 = np.array(self.inst.ub if ub is None else ub, dtype=np.float64, copy=True)
+        cap = int(histcap if histcap is not None else max(1024, 2 * self.inst.ncols))
+        p = params or default_params()
+        while True:
+            lb, ub = lb0.copy(), ub0.copy()
+            n = C.c_int64(0)
+            hr, hc, hb, hw = (np.zeros(max(cap, 1), dtype=np.int32) for _ in range(4))
+            hv = np.zeros(max(cap, 1))
+            cf = sap_conflict()
+            st = sap_stats()
+            rc = lib().sap_propagate_reasons(self.h, C.byref(p), _dp(lb), _dp(ub), cap, C.byref(n), _i32p(hr),
+                                             _i32p(hc), _i32p(hb), _dp(hv), _i32p(hw), C.byref(cf), C.byref(st))
+            if rc == SAP_NOMEMORY and histcap is None and n.value > cap:
+                cap = n.value   # again with room for the whole history (same results)
+                continue
+            _chk(rc, "sap_propagate_reasons")
+            k = n.value
+            return dict(lb=lb, ub=ub, stats=st.as_dict(), conflict=cf.as_dict(), nhist=k,
+                        round=hr[:k], col=hc[:k], bound=hb[:k], val=hv[:k], row=hw[:k])
 
     def row_activities(self, lb=None, ub=None, params: sap_params | None = None):
         nr = self.inst.nrows

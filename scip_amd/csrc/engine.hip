@@ -79,6 +79,14 @@ struct Status
    unsigned long long sweep_count;
    int32_t cont;        // continuous batching: no global round cap (per-vector caps apply)
    int32_t pad3;
+   // reason recording (one vector): infeasibility certificates (smallest caller row * 2 + side,
+   // smallest column), candidate-log and history lengths
+   int32_t cutrow;
+   int32_t cutcol;
+   uint32_t clogn;
+   uint32_t histn;
+   int32_t clogovf;
+   int32_t pad4;
 };
 
 struct Mat
@@ -111,6 +119,7 @@ struct Mat
    int32_t rowbins;         // bins 1..rowbins are swept one row per lane
    const int32_t* rowseg;   // NULL when nseg == 1
    const int32_t* colseg;
+   const int32_t* perm;     // device row position -> caller's row index
 };
 
 struct Vec
@@ -142,7 +151,42 @@ struct Vec
    int2* wl[2][SAP_NBINS];  // sparse mode: (row, vector) worklists, double-buffered by round parity
    int2* touched;           // sparse mode: (column, vector)
    Status* st;
+   // reason recording (one vector, one instance): every candidate that reaches or ties the
+   // running best of its column goes to the round's candidate log; the column step notes
+   // which key won each accepted change (rflag/rkey), k_reason finds the smallest row that
+   // submitted it (rrow) and k_history appends the change with its row to the history
+   int32_t reasons;
+   uint32_t clogcap;
+   struct CandRec* clog;
+   uint8_t* rflag;          // per column: RF_* bits of this round's outcome
+   long long* rkey;         // 2 per column: winning key of the lb / ub change
+   int32_t* rrow;           // 2 per column: smallest caller row that submitted it
+   struct HistRec* hist;
+   uint32_t histcap;
+   uint32_t pad5;
 };
+
+struct CandRec
+{
+   long long key;
+   int32_t col;
+   int32_t rowside;         // caller row * 4 + side (0 lb, 1 ub) * 2 + forced
+};
+
+struct HistRec
+{
+   double val;
+   int32_t round;
+   int32_t colside;         // column * 2 + side (0 lb, 1 ub)
+   int32_t row;
+   int32_t pad;
+};
+
+#define RF_LB      1   // the lower bound changed (or, with RF_CUT, the lower side is a row's)
+#define RF_LBFORCE 2   // ... by a forced candidate (single-variable row)
+#define RF_UB      4
+#define RF_UBFORCE 8
+#define RF_CUT     16  // the column's bounds crossed
 
 // block ranges of one sweep launch: kind 0 = tiles (dense mode), kind 1+b = bin b
 struct Layout
@@ -409,7 +453,10 @@ __device__ __forceinline__ void waveAppendRow(Status* st, int2* const* wl, int n
 // ------------------------------------------------------------------------------------------
 // candidates
 // ------------------------------------------------------------------------------------------
-struct Ctx
+// R: reason recording compiled in (a separate instantiation of the sweep, so the plain one
+// carries none of its code)
+template <bool R>
+struct CtxT
 {
    const Mat* M;
    const Vec* V;
@@ -429,7 +476,8 @@ __device__ __forceinline__ int32_t unitOfCol(const Mat& M, int32_t v, int32_t j)
    return M.colseg == NULL ? v : v * M.nseg + M.colseg[j];
 }
 
-__device__ __forceinline__ void touch(const Ctx& C, size_t flat, int32_t j)
+template <bool R>
+__device__ __forceinline__ void touch(const CtxT<R>& C, size_t flat, int32_t j)
 {
    if( C.V->dense )
    {
@@ -444,27 +492,59 @@ __device__ __forceinline__ void touch(const Ctx& C, size_t flat, int32_t j)
    }
 }
 
-__device__ __forceinline__ void submitUb(const Ctx& C, int32_t j, double val, bool force)
+// reason recording: a candidate that reaches or ties the running best of its slot goes to the
+// round's log.  The best only improves, so every candidate equal to the final best is logged.
+template <bool R>
+__device__ __forceinline__ void logCand(const CtxT<R>& C, int32_t j, long long k, int side, bool force)
+{
+   Status* st = C.V->st;
+   const uint32_t at = atomicAdd(&st->clogn, 1u);
+   if( at < C.V->clogcap )
+   {
+      CandRec r;
+      r.key = k;
+      r.col = j;
+      r.rowside = C.M->perm[C.row] * 4 + side * 2 + (force ? 1 : 0);
+      C.V->clog[at] = r;
+   }
+   else if( st->clogovf == 0 )
+      st->clogovf = 1;
+}
+
+template <bool R>
+__device__ __forceinline__ void submitUb(const CtxT<R>& C, int32_t j, double val, bool force)
 {
    size_t flat = (size_t)C.v * (size_t)C.M->ncols + (size_t)j;
    long long k = dkey(val);
    long long* slot = force ? &C.V->fub[flat] : &C.V->cub[flat];
-   if( k < *slot )
+   if constexpr( R )
+   {
+      if( k <= *slot && k <= atomicMin(slot, k) )
+         logCand(C, j, k, 1, force);
+   }
+   else if( k < *slot )
       atomicMin(slot, k);
    touch(C, flat, j);
 }
 
-__device__ __forceinline__ void submitLb(const Ctx& C, int32_t j, double val, bool force)
+template <bool R>
+__device__ __forceinline__ void submitLb(const CtxT<R>& C, int32_t j, double val, bool force)
 {
    size_t flat = (size_t)C.v * (size_t)C.M->ncols + (size_t)j;
    long long k = dkey(val);
    long long* slot = force ? &C.V->flb[flat] : &C.V->clb[flat];
-   if( k > *slot )
+   if constexpr( R )
+   {
+      if( k >= *slot && k >= atomicMax(slot, k) )
+         logCand(C, j, k, 0, force);
+   }
+   else if( k > *slot )
       atomicMax(slot, k);
    touch(C, flat, j);
 }
 
-__device__ __forceinline__ void setCutoff(const Ctx& C)
+template <bool R>
+__device__ __forceinline__ void setCutoff(const CtxT<R>& C)
 {
    C.V->vcutoff[unitOfRow(*C.M, C.v, C.row)] = 1;
    if( C.V->single )
@@ -472,7 +552,8 @@ __device__ __forceinline__ void setCutoff(const Ctx& C)
 }
 
 // tightenVarBoundsEasy (cons_linear.c:5380-5653) on start-of-round values
-__device__ __forceinline__ void candEasy(const Ctx& C, const RowState& rs, double lhs, double rhs, const Elem& e, bool force)
+template <bool R>
+__device__ __forceinline__ void candEasy(const CtxT<R>& C, const RowState& rs, double lhs, double rhs, const Elem& e, bool force)
 {
    const Params& p = *C.P;
    const bool integral = (e.cf & 1) != 0;
@@ -561,7 +642,8 @@ __device__ __forceinline__ void candEasy(const Ctx& C, const RowState& rs, doubl
 
 // consdataGetReliableResidualActivity (cons_linear.c:2580-2657): plain double sum in
 // position order over the row, skipping position skip
-__device__ __forceinline__ double reliableResidual(const Ctx& C, int64_t s, int64_t n, int64_t skip, bool isminres)
+template <bool R>
+__device__ __forceinline__ double reliableResidual(const CtxT<R>& C, int64_t s, int64_t n, int64_t skip, bool isminres)
 {
    const double2* bnd = C.V->bnd + (size_t)C.v * (size_t)C.M->ncols;
    double r = 0.0;
@@ -580,7 +662,8 @@ __device__ __forceinline__ double reliableResidual(const Ctx& C, int64_t s, int6
 }
 
 // tightenVarBounds (cons_linear.c:6700-6974) on start-of-round values
-__device__ __forceinline__ void candHard(const Ctx& C, const RowState& rs, double lhs, double rhs, const Elem& e, bool force,
+template <bool R>
+__device__ __forceinline__ void candHard(const CtxT<R>& C, const RowState& rs, double lhs, double rhs, const Elem& e, bool force,
    int64_t s, int64_t n, int64_t pos)
 {
    const Params& p = *C.P;
@@ -679,6 +762,7 @@ struct RowDecision
    bool easy;
    bool force;
    bool cut;         // infeasible (propagateCons / tightenVarBoundsEasy tests)
+   bool cutmin;      // ... by the minimal activity against rhs (else the maximal against lhs)
    bool redundant;   // locally redundant
 };
 
@@ -756,16 +840,24 @@ __device__ __forceinline__ RowDecision decideRow(const Params& p, const RowState
          d.tighten = false;
    }
    d.easy = isLT(p, rs.mad, p.maxeasy);
-   const bool easycut = d.tighten && d.easy
-      && ((!isInf(p, rhs) && isFeasLT(p, rhs, ddVal(rs.minact))) || (!isInf(p, -lhs) && isFeasLT(p, ddVal(rs.maxact), lhs)));
-   d.cut = easycut || isFeasGT(p, minT, rhs) || isFeasLT(p, maxT, lhs);
+   const bool emin = d.tighten && d.easy && !isInf(p, rhs) && isFeasLT(p, rhs, ddVal(rs.minact));
+   const bool emax = d.tighten && d.easy && !isInf(p, -lhs) && isFeasLT(p, ddVal(rs.maxact), lhs);
+   const bool fmin = isFeasGT(p, minT, rhs);
+   d.cut = emin || emax || fmin || isFeasLT(p, maxT, lhs);
+   // the side as the row's own order of tests meets it: the tightening loop tests rhs before
+   // lhs at every nonzero, then the final test checks the minimal activity first
+   d.cutmin = (emin || emax) ? emin : fmin;
    d.redundant = !d.cut && isGE(p, minT, lhs) && isLE(p, maxT, rhs);
    return d;
 }
 
 // the row's verdict (computed by decideRow) applied by one lane of the row
-__device__ __forceinline__ void rowOutcome(const Ctx& C, const RowDecision& d)
+template <bool R>
+__device__ __forceinline__ void rowOutcome(const CtxT<R>& C, const RowDecision& d)
 {
+   if constexpr( R )
+      if( d.cut )
+         atomicMin(&C.V->st->cutrow, C.M->perm[C.row] * 2 + (d.cutmin ? 0 : 1));
    if( d.cut )
       setCutoff(C);
    else if( d.redundant )
@@ -873,6 +965,7 @@ __device__ __forceinline__ bool screenElem(const Params& p, const RowState& rs, 
 }
 
 // exact candidates of queue entries [head, head+n) for lanes < n
+template <bool R>
 __device__ __forceinline__ void drainQueue(const Mat& M, const Vec& V, const Params& P, WaveQueue& Q, int n, int32_t round)
 {
    const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
@@ -894,7 +987,7 @@ __device__ __forceinline__ void drainQueue(const Mat& M, const Vec& V, const Par
          rs.c[i] = (int)((c >> (8 * i)) & 0xFF);
       rs.mad = 0.0;
       const uint32_t fl = Q.flags[q];
-      Ctx C;
+      CtxT<R> C;
       C.M = &M;
       C.V = &V;
       C.P = &P;
@@ -942,10 +1035,11 @@ __device__ __forceinline__ void pushQueue(const Mat& M, const Vec& V, const Para
 
 // drain one full wavefront of queued work if available (end of a loop iteration, where few
 // values are live; the queue then holds at most 63 entries before the next push)
+template <bool R>
 __device__ __forceinline__ void maybeDrain(const Mat& M, const Vec& V, const Params& P, WaveQueue& Q, int32_t round)
 {
    if( Q.count >= SAP_WAVE )
-      drainQueue(M, V, P, Q, SAP_WAVE, round);
+      drainQueue<R>(M, V, P, Q, SAP_WAVE, round);
 }
 
 
@@ -972,6 +1066,7 @@ __device__ __forceinline__ void noteRound(const Mat& M, const Vec& V, int32_t v,
       atomicMax(&V.vrounds[u], round);
 }
 
+template <bool R>
 __device__ __forceinline__ bool fetchItem(const Mat& M, const Vec& V, const Items& I, uint32_t it, int32_t& row,
    int32_t& v)
 {
@@ -988,13 +1083,15 @@ __device__ __forceinline__ bool fetchItem(const Mat& M, const Vec& V, const Item
       row = item.x;
       v = item.y;
    }
-   return V.vcutoff[unitOfRow(M, v, row)] == 0;
+   // with reason recording every marked row of the round is swept, so that the smallest
+   // infeasible row does not depend on timing
+   return R || V.vcutoff[unitOfRow(M, v, row)] == 0;
 }
 
 // ------------------------------------------------------------------------------------------
 // sweep: groups of G lanes, one row per group, one nonzero per lane (rows of <= G nonzeros)
 // ------------------------------------------------------------------------------------------
-template <int G>
+template <int G, bool R>
 __device__ __forceinline__ void sweepGroups(const Mat& M, const Vec& V, const Params& P, const Items& I, int lblock,
    int nblocks, unsigned long long& nnzacc, unsigned long long& rowsacc, WaveQueue& Q)
 {
@@ -1008,8 +1105,8 @@ __device__ __forceinline__ void sweepGroups(const Mat& M, const Vec& V, const Pa
    {
       const uint32_t it = it0 + (uint32_t)((tid & (SAP_WAVE - 1)) / G);
       int32_t row = 0, v = 0;
-      const bool live = it < I.n && fetchItem(M, V, I, it, row, v);
-      Ctx C;
+      const bool live = it < I.n && fetchItem<R>(M, V, I, it, row, v);
+      CtxT<R> C;
       C.M = &M;
       C.V = &V;
       C.P = &P;
@@ -1068,7 +1165,7 @@ __device__ __forceinline__ void sweepGroups(const Mat& M, const Vec& V, const Pa
          if( !V.single )
             noteRound(M, V, v, row, I.round);
       }
-      maybeDrain(M, V, P, Q, I.round);
+      maybeDrain<R>(M, V, P, Q, I.round);
    }
 }
 
@@ -1078,7 +1175,7 @@ __device__ __forceinline__ void sweepGroups(const Mat& M, const Vec& V, const Pa
 // of the trailing one bits of k; positions past the row contribute the identity (0, 0).  The
 // row decision runs once per row; the screening pass reloads the row's nonzeros.
 // ------------------------------------------------------------------------------------------
-template <int MAXL>
+template <int MAXL, bool R>
 __device__ __forceinline__ void sweepRows(const Mat& M, const Vec& V, const Params& P, const Items& I, int lblock,
    int nblocks, unsigned long long& nnzacc, unsigned long long& rowsacc, WaveQueue& Q)
 {
@@ -1091,7 +1188,7 @@ __device__ __forceinline__ void sweepRows(const Mat& M, const Vec& V, const Para
    {
       const uint32_t it = it0 + (uint32_t)lane;
       int32_t row = 0, v = 0;
-      const bool live = it < I.n && fetchItem(M, V, I, it, row, v);
+      const bool live = it < I.n && fetchItem<R>(M, V, I, it, row, v);
       const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
       const int64_t s = live ? M.rowptr[row] : 0;
       const int n = live ? (int)(M.rowptr[row + 1] - s) : 0;
@@ -1156,7 +1253,7 @@ __device__ __forceinline__ void sweepRows(const Mat& M, const Vec& V, const Para
       const RowDecision d = decideRow(P, rs, sides.x, sides.y, n);
       if( live )
       {
-         Ctx C;
+         CtxT<R> C;
          C.M = &M;
          C.V = &V;
          C.P = &P;
@@ -1186,7 +1283,7 @@ __device__ __forceinline__ void sweepRows(const Mat& M, const Vec& V, const Para
             e = loadElem(M, bnd, s + k);
          const bool pass = act && screenElem(P, rs, d, sides.x, sides.y, e);
          pushQueue(M, V, P, Q, pass, s + k, row, v, rs, d, I.round);
-         maybeDrain(M, V, P, Q, I.round);
+         maybeDrain<R>(M, V, P, Q, I.round);
       }
    }
 }
@@ -1205,6 +1302,7 @@ __device__ __forceinline__ DD shflDownDD(DD x, int w)
    return r;
 }
 
+template <bool R>
 __device__ __forceinline__ void sweepTiles(const Mat& M, const Vec& V, const Params& P, const Items& I, int lblock,
    int nblocks, unsigned long long& nnzacc, unsigned long long& rowsacc, WaveQueue& Q)
 {
@@ -1219,7 +1317,7 @@ __device__ __forceinline__ void sweepTiles(const Mat& M, const Vec& V, const Par
       const uint32_t tu = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
       const int32_t v = (int32_t)(tu / ntiles);
       const int32_t tile = (int32_t)(tu - (uint32_t)v * ntiles);
-      if( M.rowseg == NULL && V.vcutoff[v] )
+      if( M.rowseg == NULL && V.vcutoff[v] && !R )
          continue;
       const int64_t e0 = M.tstart[tile];
       const int cnt = (int)(M.tstart[tile + 1] - e0);
@@ -1291,7 +1389,7 @@ __device__ __forceinline__ void sweepTiles(const Mat& M, const Vec& V, const Par
       rs.c[7] = __popcll(__ballot(kmax == 4) & rowbits);
       if( (__ballot(madinf) & rowbits) != 0 )
          rs.mad = P.infinity;
-      Ctx C;
+      CtxT<R> C;
       C.M = &M;
       C.V = &V;
       C.P = &P;
@@ -1309,7 +1407,7 @@ __device__ __forceinline__ void sweepTiles(const Mat& M, const Vec& V, const Par
          if( !V.single )
             noteRound(M, V, v, row, I.round);
       }
-      maybeDrain(M, V, P, Q, I.round);
+      maybeDrain<R>(M, V, P, Q, I.round);
    }
 }
 
@@ -1436,6 +1534,7 @@ __device__ __forceinline__ WavePart waveSubtree(const Mat& M, const Params& P, c
    return w;
 }
 
+template <bool R>
 __device__ __forceinline__ void candidatesRange(const Mat& M, const Vec& V, const Params& P, WaveQueue& Q,
    const RowState& rs, const RowDecision& d, double lhs, double rhs, const double2* bnd, int64_t s, int64_t n,
    int64_t c0, int64_t nch, int32_t row, int32_t v, int32_t round)
@@ -1455,10 +1554,11 @@ __device__ __forceinline__ void candidatesRange(const Mat& M, const Vec& V, cons
          e = loadElem(M, bnd, s + pos);
       const bool pass = has && screenElem(P, rs, d, lhs, rhs, e);
       pushQueue(M, V, P, Q, pass, s + pos, row, v, rs, d, round);
-      maybeDrain(M, V, P, Q, round);
+      maybeDrain<R>(M, V, P, Q, round);
    }
 }
 
+template <bool R>
 __device__ __forceinline__ void sweepWaves(const Mat& M, const Vec& V, const Params& P, const Items& I, int lblock,
    int nblocks, unsigned long long& nnzacc, unsigned long long& rowsacc, WaveQueue& Q)
 {
@@ -1469,9 +1569,9 @@ __device__ __forceinline__ void sweepWaves(const Mat& M, const Vec& V, const Par
    for( uint32_t it = (uint32_t)wid; it < I.n; it += nw )
    {
       int32_t row, v;
-      if( !fetchItem(M, V, I, it, row, v) )
+      if( !fetchItem<R>(M, V, I, it, row, v) )
          continue;
-      Ctx C;
+      CtxT<R> C;
       C.M = &M;
       C.V = &V;
       C.P = &P;
@@ -1494,7 +1594,7 @@ __device__ __forceinline__ void sweepWaves(const Mat& M, const Vec& V, const Par
       rs.mad = w.inf ? P.infinity : w.mad;
       RowDecision d = decideRow(P, rs, sides.x, sides.y, n);
       if( d.tighten )
-         candidatesRange(M, V, P, Q, rs, d, sides.x, sides.y, bnd, s, n, 0, nch, row, v, I.round);
+         candidatesRange<R>(M, V, P, Q, rs, d, sides.x, sides.y, bnd, s, n, 0, nch, row, v, I.round);
       if( lane == 0 )
       {
          rowOutcome(C, d);
@@ -1507,6 +1607,7 @@ __device__ __forceinline__ void sweepWaves(const Mat& M, const Vec& V, const Par
 }
 
 // workgroup per row: wave w reduces the aligned chunk subtree [w*span, (w+1)*span)
+template <bool R>
 __device__ __forceinline__ void sweepBlocks(const Mat& M, const Vec& V, const Params& P, const Items& I, int lblock,
    int nblocks, unsigned long long& nnzacc, unsigned long long& rowsacc, WaveQueue& Q)
 {
@@ -1519,9 +1620,9 @@ __device__ __forceinline__ void sweepBlocks(const Mat& M, const Vec& V, const Pa
    for( uint32_t it = (uint32_t)lblock; it < I.n; it += (uint32_t)nblocks )
    {
       int32_t row, v;
-      if( !fetchItem(M, V, I, it, row, v) )
+      if( !fetchItem<R>(M, V, I, it, row, v) )
          continue;   // block-uniform
-      Ctx C;
+      CtxT<R> C;
       C.M = &M;
       C.V = &V;
       C.P = &P;
@@ -1563,7 +1664,7 @@ __device__ __forceinline__ void sweepBlocks(const Mat& M, const Vec& V, const Pa
       RowState rs = srs;
       RowDecision d = decideRow(P, rs, sides.x, sides.y, n);
       if( d.tighten )
-         candidatesRange(M, V, P, Q, rs, d, sides.x, sides.y, bnd, s, n, c0, mine, row, v, I.round);
+         candidatesRange<R>(M, V, P, Q, rs, d, sides.x, sides.y, bnd, s, n, c0, mine, row, v, I.round);
       if( threadIdx.x == 0 )
       {
          rowOutcome(C, d);
@@ -1610,6 +1711,7 @@ __device__ __forceinline__ void blockStats(unsigned long long a, unsigned long l
    }
 }
 
+template <bool R>
 __global__ __launch_bounds__(SAP_BLOCK, 4) void k_sweep(Mat M, Vec V, Params P, Layout L)
 {
    Status* st = V.st;
@@ -1666,7 +1768,7 @@ __global__ __launch_bounds__(SAP_BLOCK, 4) void k_sweep(Mat M, Vec V, Params P, 
    switch( kind )
    {
    case 0:
-      sweepTiles(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+      sweepTiles<R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
       // empty rows (first round only; they are never marked again): activity 0 against the
       // sides, propagateCons's infeasibility test (cons_linear.c:7721-7742)
       if( I.firstall && lblock == 0 )
@@ -1675,6 +1777,8 @@ __global__ __launch_bounds__(SAP_BLOCK, 4) void k_sweep(Mat M, Vec V, Params P, 
          for( int32_t r = M.binstart[0] + (int32_t)threadIdx.x; r < M.binstart[1]; r += SAP_BLOCK )
          {
             const double2 sd = M.sides[r];
+            if( (isFeasGT(P, 0.0, sd.y) || isFeasLT(P, 0.0, sd.x)) && R )
+               atomicMin(&st->cutrow, M.perm[r] * 2 + (isFeasGT(P, 0.0, sd.y) ? 0 : 1));
             if( isFeasGT(P, 0.0, sd.y) || isFeasLT(P, 0.0, sd.x) )
                for( int32_t v = 0; v < V.nvec; ++v )
                {
@@ -1687,33 +1791,33 @@ __global__ __launch_bounds__(SAP_BLOCK, 4) void k_sweep(Mat M, Vec V, Params P, 
             rowsacc += (unsigned long long)V.nvec * (unsigned long long)nempty;
       }
       break;
-   case 1: sweepGroups<4>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
+   case 1: sweepGroups<4, R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
    case 2:
       if( M.rowbins >= 1 )
-         sweepRows<4>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+         sweepRows<4, R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
       else
-         sweepGroups<4>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+         sweepGroups<4, R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
       break;
    case 3:
       if( M.rowbins >= 2 )
-         sweepRows<8>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+         sweepRows<8, R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
       else
-         sweepGroups<8>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+         sweepGroups<8, R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
       break;
    case 4:
       if( M.rowbins >= 3 )
-         sweepRows<16>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+         sweepRows<16, R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
       else
-         sweepGroups<16>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+         sweepGroups<16, R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
       break;
-   case 5: sweepGroups<32>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
-   case 6: sweepGroups<64>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
-   case 7: sweepWaves(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
-   default: sweepBlocks(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
+   case 5: sweepGroups<32, R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
+   case 6: sweepGroups<64, R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
+   case 7: sweepWaves<R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
+   default: sweepBlocks<R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
    }
    // remaining queued candidates of this wavefront
    if( Q.count > 0 )
-      drainQueue(M, V, P, Q, Q.count, I.round);
+      drainQueue<R>(M, V, P, Q, Q.count, I.round);
    blockStats(nnzacc, rowsacc, st->slot_nnz, st->slot_rows);
    if( threadIdx.x == 0 )
    {
@@ -1742,36 +1846,57 @@ __device__ __forceinline__ int columnStep(const Mat& M, const Vec& V, const Para
       V.fub[flat] = KEY_POS_INF;
    }
    const int32_t unit = unitOfCol(M, v, j);
-   if( V.vcutoff[unit] )
+   // with reason recording every column of a round without a row cutoff is stepped, so that
+   // the smallest crossing column does not depend on timing
+   if( V.reasons ? st->cutrow != INT32_MAX : V.vcutoff[unit] != 0 )
       return 0;
    const bool integral = (M.colflags[j] & 1) != 0;
    const double2 b0 = V.bnd[flat];
    const double lb0 = b0.x, ub0 = b0.y;
    double nlb = lb0, nub = ub0;
    bool cut = false;
+   // which candidate set each new bound (RF_* bits) and its key; on a cut, the crossing one
+   int lsrc = 0, usrc = 0;
+   long long lkey = 0, ukey = 0;
    // SCIPinferVarLbCons (scip_var.c:6965-7070): forced, then regular candidate
    if( kfl != KEY_NEG_INF )
    {
       double tt = adjustedLb(P, integral, keyToDouble(kfl));
       if( isInf(P, tt) || isFeasGT(P, tt, ub0) )
+      {
          cut = true;
+         lsrc = RF_LB | RF_LBFORCE;
+         lkey = kfl;
+      }
       else
       {
          tt = dmin(tt, ub0);
          if( !isLE(P, tt, lb0) && dkey(tt) > dkey(nlb) )
+         {
             nlb = tt;
+            lsrc = RF_LB | RF_LBFORCE;
+            lkey = kfl;
+         }
       }
    }
    if( !cut && kcl != KEY_NEG_INF )
    {
       double tt = adjustedLb(P, integral, keyToDouble(kcl));
       if( isInf(P, tt) || isFeasGT(P, tt, ub0) )
+      {
          cut = true;
+         lsrc = RF_LB;
+         lkey = kcl;
+      }
       else
       {
          tt = dmin(tt, ub0);
          if( isLbBetter(P, tt, lb0, ub0) && dkey(tt) > dkey(nlb) )
+         {
             nlb = tt;
+            lsrc = RF_LB;
+            lkey = kcl;
+         }
       }
    }
    // SCIPinferVarUbCons (scip_var.c:7071-7174) against the new lower bound
@@ -1779,28 +1904,56 @@ __device__ __forceinline__ int columnStep(const Mat& M, const Vec& V, const Para
    {
       double tt = adjustedUb(P, integral, keyToDouble(kfu));
       if( isInf(P, -tt) || isFeasLT(P, tt, nlb) )
+      {
          cut = true;
+         usrc = RF_UB | RF_UBFORCE;
+         ukey = kfu;
+      }
       else
       {
          tt = dmax(tt, nlb);
          if( !isGE(P, tt, ub0) && dkey(tt) < dkey(nub) )
+         {
             nub = tt;
+            usrc = RF_UB | RF_UBFORCE;
+            ukey = kfu;
+         }
       }
    }
    if( !cut && kcu != KEY_POS_INF )
    {
       double tt = adjustedUb(P, integral, keyToDouble(kcu));
       if( isInf(P, -tt) || isFeasLT(P, tt, nlb) )
+      {
          cut = true;
+         usrc = RF_UB;
+         ukey = kcu;
+      }
       else
       {
          tt = dmax(tt, nlb);
          if( isUbBetter(P, tt, nlb, ub0) && dkey(tt) < dkey(nub) )
+         {
             nub = tt;
+            usrc = RF_UB;
+            ukey = kcu;
+         }
       }
    }
    if( cut )
    {
+      if( V.reasons )
+      {
+         // a lower-bound candidate crossed the upper bound of the round start, or an
+         // upper-bound candidate crossed the (possibly new) lower bound: the sides without
+         // a candidate are the column's own bounds
+         V.rflag[j] = (uint8_t)(RF_CUT | lsrc | usrc);
+         V.rkey[2 * (size_t)j] = lkey;
+         V.rkey[2 * (size_t)j + 1] = ukey;
+         V.rrow[2 * (size_t)j] = INT32_MAX;
+         V.rrow[2 * (size_t)j + 1] = INT32_MAX;
+         atomicMin(&st->cutcol, j);
+      }
       V.vcutoff[unit] = 1;
       if( V.single )
          st->cutoff = 1;
@@ -1810,6 +1963,14 @@ __device__ __forceinline__ int columnStep(const Mat& M, const Vec& V, const Para
    const bool chu = dkey(nub) != dkey(ub0);
    if( !chl && !chu )
       return 0;
+   if( V.reasons )
+   {
+      V.rflag[j] = (uint8_t)((chl ? lsrc : 0) | (chu ? usrc : 0));
+      V.rkey[2 * (size_t)j] = lkey;
+      V.rkey[2 * (size_t)j + 1] = ukey;
+      V.rrow[2 * (size_t)j] = INT32_MAX;
+      V.rrow[2 * (size_t)j + 1] = INT32_MAX;
+   }
    V.bnd[flat] = make_double2(nlb, nub);
    if( V.logchg )
    {
@@ -1926,6 +2087,67 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_mark(Mat M, Vec V)
       st->anymark[npar] = 1;
 }
 
+// reason recording, after the column step: every logged candidate whose key won its column's
+// change (or crossing) offers its row; the smallest caller row is kept
+__global__ __launch_bounds__(SAP_BLOCK) void k_reason(Vec V)
+{
+   Status* st = V.st;
+   if( st->done )
+      return;
+   const uint32_t n = st->clogn < V.clogcap ? st->clogn : V.clogcap;
+   for( uint32_t i = blockIdx.x * SAP_BLOCK + threadIdx.x; i < n; i += gridDim.x * SAP_BLOCK )
+   {
+      const CandRec r = V.clog[i];
+      const int side = (r.rowside >> 1) & 1;
+      const int forced = r.rowside & 1;
+      const int f = V.rflag[r.col];
+      const int have = side ? RF_UB : RF_LB;
+      const int force = side ? RF_UBFORCE : RF_LBFORCE;
+      if( (f & have) && ((f & force) != 0) == (forced != 0) && V.rkey[2 * (size_t)r.col + side] == r.key )
+         atomicMin(&V.rrow[2 * (size_t)r.col + side], r.rowside >> 2);
+   }
+}
+
+// reason recording: the changes accepted in this round join the history with their rows
+// (not in a round that ended infeasible: those changes are not applied)
+__global__ __launch_bounds__(SAP_BLOCK) void k_history(Mat M, Vec V)
+{
+   Status* st = V.st;
+   if( st->done || st->cutoff )
+      return;
+   const int32_t round = st->rlocal;
+   for( int64_t j = blockIdx.x * (int64_t)SAP_BLOCK + threadIdx.x; j < M.ncols; j += (int64_t)gridDim.x * SAP_BLOCK )
+   {
+      const int f = V.rflag[j];
+      if( f == 0 )
+         continue;
+      V.rflag[j] = 0;
+      const double2 b = V.bnd[j];
+      for( int side = 0; side < 2; ++side )
+      {
+         if( !(f & (side ? RF_UB : RF_LB)) )
+            continue;
+         const uint32_t at = atomicAdd(&st->histn, 1u);
+         if( at < V.histcap )
+         {
+            HistRec h;
+            h.val = side ? b.y : b.x;
+            h.round = round;
+            h.colside = (int32_t)j * 2 + side;
+            h.row = V.rrow[2 * (size_t)j + side];
+            h.pad = 0;
+            V.hist[at] = h;
+         }
+      }
+   }
+}
+
+__global__ void k_reason_reset(Vec V, int64_t ncols)
+{
+   for( int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < ncols; j += (int64_t)gridDim.x * blockDim.x )
+      V.rflag[j] = 0;
+}
+
 // ------------------------------------------------------------------------------------------
 // round control and set-up kernels (kernels, not memsets, so a round can be graph-captured)
 // ------------------------------------------------------------------------------------------
@@ -1977,6 +2199,7 @@ __global__ __launch_bounds__(SAP_STAT_SLOTS) void k_round_begin(Vec V, int32_t m
    }
    st->round = r;
    st->rlocal += 1;
+   st->clogn = 0;
    st->anymark[(r + 1) & 1] = 0;
    for( int b = 0; b < SAP_NBINS; ++b )
       st->wlcount[(r + 1) & 1][b] = 0;
@@ -2018,6 +2241,11 @@ __global__ void k_call_reset(Vec V, int64_t nrows, int32_t nvec, int32_t firstal
       st->rows_swept = 0;
       st->nnz_swept = 0;
       st->nchg = 0;
+      st->cutrow = INT32_MAX;
+      st->cutcol = INT32_MAX;
+      st->clogn = 0;
+      st->histn = 0;
+      st->clogovf = 0;
       for( int b = 0; b < SAP_NBINS; ++b )
       {
          st->wlcount[0][b] = 0;
@@ -2428,6 +2656,7 @@ struct sap_matrix
    std::vector<int64_t>* rowoff;   // host: segment row / column ranges (caller order)
    std::vector<int64_t>* coloff;
    sap_bounds* single;   // cached one-vector state for sap_propagate
+   sap_bounds* rsingle;  // cached one-vector state with reason recording (sap_propagate_reasons)
    int64_t nbounds;      // live caller-created bound sets (a matrix outlives them)
 };
 
@@ -2451,6 +2680,7 @@ struct GraphKey
    int32_t single;
    int32_t dense;
    int32_t logchg;
+   int32_t reasons;
 };
 
 struct sap_bounds
@@ -2598,6 +2828,7 @@ static Mat toMat(const sap_matrix* m)
    M.rowbins = m->tuning.rowbins;
    M.rowseg = m->nseg > 1 ? m->rowseg : NULL;
    M.colseg = m->nseg > 1 ? m->colseg : NULL;
+   M.perm = m->perm;
    return M;
 }
 
@@ -3204,6 +3435,8 @@ int sap_matrix_free(sap_matrix* m)
    (void)hipStreamSynchronize(m->ctx->stream);
    if( m->single )
       sap_bounds_free(m->single);
+   if( m->rsingle )
+      sap_bounds_free(m->rsingle);
    m->ctx->nmatrices--;
    matrixRelease(m);
    free(m);
@@ -3267,6 +3500,11 @@ static void boundsRelease(sap_bounds* b)
    b->srcgen = NULL;
    (void)hipFree(V.ovl);
    (void)hipFree(V.ovln);
+   (void)hipFree(V.clog);
+   (void)hipFree(V.rflag);
+   (void)hipFree(V.rkey);
+   (void)hipFree(V.rrow);
+   (void)hipFree(V.hist);
    if( b->hstat )
       (void)hipHostFree(b->hstat);
    if( b->hpend )
@@ -3552,11 +3790,18 @@ static void enqueueRound(sap_bounds* b, const Mat& M, const Params& P, const Lay
    k_round_begin<<<1, SAP_STAT_SLOTS, 0, s>>>(b->V, P.maxrounds);
    if( e0 != NULL )
       (void)hipEventRecord(e0, s);
-   if( nsweep > 0 )
-      k_sweep<<<nsweep, SAP_BLOCK, 0, s>>>(M, b->V, P, L);
+   if( nsweep > 0 && b->V.reasons )
+      k_sweep<true><<<nsweep, SAP_BLOCK, 0, s>>>(M, b->V, P, L);
+   else if( nsweep > 0 )
+      k_sweep<false><<<nsweep, SAP_BLOCK, 0, s>>>(M, b->V, P, L);
    if( e1 != NULL )
       (void)hipEventRecord(e1, s);
    k_columns<<<colgrid, SAP_BLOCK, 0, s>>>(M, b->V, P);
+   if( b->V.reasons )
+   {
+      k_reason<<<gridFor(2 * (size_t)m->nnz, 4096), SAP_BLOCK, 0, s>>>(b->V);
+      k_history<<<gridFor((size_t)m->ncols, 4096), SAP_BLOCK, 0, s>>>(M, b->V);
+   }
    k_mark<<<colgrid, SAP_BLOCK, 0, s>>>(M, b->V);
 }
 
@@ -3638,6 +3883,7 @@ static bool ensureGraphs(sap_bounds* b, const Params& P)
    k.single = b->V.single;
    k.dense = b->V.dense;
    k.logchg = b->V.logchg;
+   k.reasons = b->V.reasons;
    if( b->graph[0].valid && b->graph[1].valid && memcmp(&k, &b->key, sizeof(k)) == 0 )
       return true;
    for( int g = 0; g < 2; ++g )
@@ -3770,6 +4016,142 @@ int sap_propagate_resident(sap_bounds* b, const sap_params* params, sap_stats* s
    RCCHK(rc);
    if( stats )
       stats->result = c->hstatus->cutoff ? SAP_CUTOFF : (c->hstatus->nchg > 0 ? SAP_REDUCEDDOM : SAP_DIDNOTFIND);
+   return SAP_OKAY;
+}
+
+static void dropGraphs(sap_bounds* b)
+{
+   for( int g = 0; g < 2; ++g )
+      if( b->graph[g].valid )
+      {
+         (void)hipGraphExecDestroy(b->graph[g].exec);
+         b->graph[g].valid = 0;
+      }
+}
+
+// the matrix's one-vector state with reason recording: candidate log of 2 entries per
+// nonzero (a row is swept once per round and submits at most a lower and an upper bound
+// candidate per nonzero), per-column outcome, winning keys and rows, and the history
+static int createReasons(sap_matrix* m)
+{
+   if( m->nrows >= ((int64_t)1 << 29) || m->ncols >= ((int64_t)1 << 30) || 2 * m->nnz + 64 > (int64_t)UINT32_MAX )
+      return SAP_NOTIMPLEMENTED;   // row * 4 and column * 2 must fit the record fields
+   sap_bounds* b = NULL;
+   RCCHK(sap_bounds_create_mode(m, 1, SAP_MODE_DENSE, &b));
+   b->internal = 1;
+   m->nbounds--;
+   m->rsingle = b;
+   Vec& V = b->V;
+   V.clogcap = (uint32_t)(2 * m->nnz + 64);
+   V.histcap = (uint32_t)std::max<int64_t>(4096, std::min<int64_t>(2 * m->ncols, (int64_t)UINT32_MAX));
+   RCCHK(devAlloc(&V.clog, (size_t)V.clogcap, &b->device_bytes));
+   RCCHK(devAlloc(&V.rflag, (size_t)m->ncols + 1, &b->device_bytes));
+   RCCHK(devAlloc(&V.rkey, 2 * (size_t)m->ncols + 2, &b->device_bytes));
+   RCCHK(devAlloc(&V.rrow, 2 * (size_t)m->ncols + 2, &b->device_bytes));
+   RCCHK(devAlloc(&V.hist, (size_t)V.histcap, &b->device_bytes));
+   V.reasons = 1;
+   return SAP_OKAY;
+}
+
+// history entry order: round, column, lower before upper
+static bool histLess(const HistRec& a, const HistRec& b)
+{
+   return a.round != b.round ? a.round < b.round : a.colside < b.colside;
+}
+
+int sap_propagate_reasons(sap_matrix* m, const sap_params* params, double* lb, double* ub, int64_t histcap,
+   int64_t* nhist, int32_t* hround, int32_t* hcol, int32_t* hbound, double* hval, int32_t* hrow,
+   sap_conflict* conflict, sap_stats* stats)
+{
+   if( m == NULL || lb == NULL || ub == NULL || nhist == NULL || histcap < 0 || m->nseg != 1
+      || (histcap > 0 && (hround == NULL || hcol == NULL || hbound == NULL || hval == NULL || hrow == NULL)) )
+      return SAP_INVALIDDATA;
+   RCCHK(checkParams(params));
+   sap_context* c = m->ctx;
+   HIPCHK(hipSetDevice(c->device));
+   hipStream_t s = c->stream;
+   if( m->rsingle == NULL )
+      RCCHK(createReasons(m));
+   sap_bounds* b = m->rsingle;
+   const Params P = toParams(params);
+   const int64_t nc = m->ncols;
+   std::vector<double> lb0(lb, lb + nc), ub0(ub, ub + nc);
+   sap_stats local;
+   for( int attempt = 0;; ++attempt )
+   {
+      RCCHK(sap_bounds_upload(b, 0, lb0.data(), ub0.data()));
+      k_reason_reset<<<gridFor((size_t)nc), SAP_BLOCK, 0, s>>>(b->V, nc);
+      HIPCHK(hipGetLastError());
+      memset(&local, 0, sizeof(local));
+      b->V.nvec = 1;
+      b->V.single = 1;
+      RCCHK(beginCall(b, 1, 1));
+      RCCHK(runRounds(b, P, &local));
+      const Status* h = c->hstatus;
+      if( h->clogovf )
+         return SAP_ERROR;   // cannot happen: the log holds two candidates per nonzero
+      if( h->histn <= b->V.histcap || attempt > 0 )
+         break;
+      // more accepted changes than the history holds: again with room for all of them
+      const uint32_t need = h->histn;
+      dropGraphs(b);
+      (void)hipFree(b->V.hist);
+      b->V.hist = NULL;
+      RCCHK(devAlloc(&b->V.hist, (size_t)need, &b->device_bytes));
+      b->V.histcap = need;
+   }
+   const Status h = *c->hstatus;
+   const bool cut = h.cutoff != 0;
+   local.result = cut ? SAP_CUTOFF : (h.nchg > 0 ? SAP_REDUCEDDOM : SAP_DIDNOTFIND);
+   if( stats )
+      *stats = local;
+   // the history of the rounds before the last (a round ending infeasible adds nothing)
+   const int64_t nh = (int64_t)std::min<uint32_t>(h.histn, b->V.histcap);
+   *nhist = nh;
+   if( conflict )
+   {
+      memset(conflict, 0, sizeof(*conflict));
+      conflict->round = cut ? h.rlocal : 0;
+      conflict->row = conflict->col = conflict->rowlb = conflict->rowub = -1;
+      if( cut && h.cutrow != INT32_MAX )
+      {
+         conflict->kind = SAP_CONFLICT_ROW;
+         conflict->row = h.cutrow >> 1;
+         conflict->side = h.cutrow & 1;
+      }
+      else if( cut && h.cutcol != INT32_MAX )
+      {
+         uint8_t f = 0;
+         int32_t rr[2];
+         HIPCHK(hipMemcpyAsync(&f, b->V.rflag + h.cutcol, 1, hipMemcpyDeviceToHost, s));
+         HIPCHK(hipMemcpyAsync(rr, b->V.rrow + 2 * (size_t)h.cutcol, sizeof(rr), hipMemcpyDeviceToHost, s));
+         HIPCHK(hipStreamSynchronize(s));
+         conflict->kind = SAP_CONFLICT_COLUMN;
+         conflict->col = h.cutcol;
+         conflict->rowlb = (f & RF_LB) ? rr[0] : -1;
+         conflict->rowub = (f & RF_UB) ? rr[1] : -1;
+      }
+   }
+   if( !cut )
+      RCCHK(sap_bounds_download(b, 0, lb, ub));
+   if( nh > histcap )
+      return SAP_NOMEMORY;
+   if( nh > 0 )
+   {
+      std::vector<HistRec> hh((size_t)nh);
+      HIPCHK(hipMemcpyAsync(hh.data(), b->V.hist, sizeof(HistRec) * (size_t)nh, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      std::sort(hh.begin(), hh.end(), histLess);
+      for( int64_t q = 0; q < nh; ++q )
+      {
+         const HistRec& r = hh[(size_t)q];
+         hround[q] = r.round;
+         hcol[q] = r.colside >> 1;
+         hbound[q] = r.colside & 1;
+         hval[q] = r.val;
+         hrow[q] = r.row;
+      }
+   }
    return SAP_OKAY;
 }
 
