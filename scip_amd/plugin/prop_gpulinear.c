@@ -15,12 +15,22 @@
  * the columns changed in both branches go to SCIP's own SCIPanalyzeDeductionsProbing
  * (prop_probing.c:1314), which derives fixings, tightenings, aggregations and implications.
  *
+ * With reasons (the default), PROPEXEC propagates with sap_propagate_reasons: the device
+ * returns every accepted change with its round and reason row, and the changes are handed to
+ * SCIP in round order with the row as inferinfo.  Each change is then implied by its row and
+ * the bounds already in force, so PROPRESPROP explains it with the row's other bounds at the
+ * change's time, as cons_linear's resolvePropagation does with its own inferinfo
+ * (cons_linear.c:344-369, :5097-5189).  A device cutoff comes with a certificate (an
+ * infeasible row, or a column whose bounds crossed with the rows of both sides), from which
+ * the propagator starts conflict analysis (cons_linear.c analyzeConflict).
+ *
  * Parameters mirror SCIP's (numerics/..., constraints/linear/maxeasyactivitydelta); the
  * propagator's own:
  *   propagating/gpulinear/device          HIP device ordinal
  *   propagating/gpulinear/maxrounds       device rounds per call (-1: to fixpoint)
  *   propagating/gpulinear/probing         probe the binaries in presolving
  *   propagating/gpulinear/probingvectors  node vectors of one probing launch (two per probe)
+ *   propagating/gpulinear/reasons         record reasons for conflict analysis
  *
  * Error convention: every HIP or engine failure becomes a SCIP_RETCODE (the engine's return
  * codes have SCIP_RETCODE values); nothing aborts.
@@ -48,6 +58,7 @@
 #define DEFAULT_MAXROUNDS         -1
 #define DEFAULT_PROBING        FALSE
 #define DEFAULT_PROBINGVECTORS  2048
+#define DEFAULT_REASONS         TRUE
 
 /** propagator data */
 struct SCIP_PropData
@@ -70,6 +81,12 @@ struct SCIP_PropData
    SCIP_Longint          nprobefixed;        /**< columns fixed by probing */
    SCIP_Longint          nprobeaggr;         /**< aggregations found by probing */
    SCIP_Longint          nprobeimpl;         /**< implications found by probing */
+   SCIP_Bool             reasons;            /**< parameter: record reasons for conflict analysis */
+   int64_t*              rowptr;             /**< host copy of the device rows (for PROPRESPROP) */
+   int32_t*              colidx;
+   double*               vals;
+   int                   nnz;
+   SCIP_Longint          nconflicts;         /**< conflict analyses started from device cutoffs */
 };
 
 /*
@@ -145,6 +162,14 @@ SCIP_RETCODE freeMatrix(
       SCIPfreeBlockMemoryArray(scip, &propdata->vars, propdata->nvars);
       propdata->vars = NULL;
    }
+   if( propdata->rowptr != NULL )
+   {
+      SCIPfreeBlockMemoryArray(scip, &propdata->vals, propdata->nnz);
+      SCIPfreeBlockMemoryArray(scip, &propdata->colidx, propdata->nnz);
+      SCIPfreeBlockMemoryArray(scip, &propdata->rowptr, propdata->nrows + 1);
+      propdata->rowptr = NULL;
+   }
+   propdata->nnz = 0;
    propdata->nvars = 0;
    propdata->nrows = 0;
    return SCIP_OKAY;
@@ -268,6 +293,14 @@ SCIP_RETCODE buildMatrix(
    {
       SCIP_CALL( engineCall(sap_matrix_create(propdata->ctx, nrows, nvars, rowptr, colidx, vals, lhs, rhs, integral,
             glb, gub, &propdata->mat), "sap_matrix_create") );
+      /* the rows stay on the host too: PROPRESPROP and conflict analysis read them */
+      SCIP_CALL( SCIPallocBlockMemoryArray(scip, &propdata->rowptr, nrows + 1) );
+      SCIP_CALL( SCIPallocBlockMemoryArray(scip, &propdata->colidx, nnz) );
+      SCIP_CALL( SCIPallocBlockMemoryArray(scip, &propdata->vals, nnz) );
+      memcpy(propdata->rowptr, rowptr, sizeof(int64_t) * (size_t)(nrows + 1));
+      memcpy(propdata->colidx, colidx, sizeof(int32_t) * (size_t)nnz);
+      memcpy(propdata->vals, vals, sizeof(double) * (size_t)nnz);
+      propdata->nnz = nnz;
    }
    propdata->nrows = nrows;
 
@@ -447,6 +480,192 @@ SCIP_RETCODE probeOnDevice(
    return SCIP_OKAY;
 }
 
+/** adds to the conflict the bounds of row's columns other than skip that make up its minimal
+ *  (maximal) activity: lower bounds where the coefficient is positive and upper bounds where it
+ *  is negative (the reverse for the maximal activity), at the time of bdchgidx (NULL: now);
+ *  cons_linear.c addConflictBounds (:5097) */
+static
+SCIP_RETCODE addRowBounds(
+   SCIP*                 scip,               /**< SCIP data structure */
+   SCIP_PROPDATA*        propdata,           /**< propagator data */
+   int                   row,                /**< row of the device matrix */
+   int                   skip,               /**< column left out, or -1 */
+   SCIP_Bool             maximal,            /**< the maximal activity's bounds, else the minimal's */
+   SCIP_BDCHGIDX*        bdchgidx            /**< time of the bounds, or NULL for the current ones */
+   )
+{
+   int64_t k;
+
+   for( k = propdata->rowptr[row]; k < propdata->rowptr[row + 1]; ++k )
+   {
+      SCIP_VAR* var = propdata->vars[propdata->colidx[k]];
+
+      if( propdata->colidx[k] == skip )
+         continue;
+      if( (propdata->vals[k] > 0.0) != maximal )
+      {
+         SCIP_CALL( SCIPaddConflictLb(scip, var, bdchgidx) );
+      }
+      else
+      {
+         SCIP_CALL( SCIPaddConflictUb(scip, var, bdchgidx) );
+      }
+   }
+   return SCIP_OKAY;
+}
+
+/** coefficient of column j in row (0.0 if absent) */
+static
+double rowCoef(
+   SCIP_PROPDATA*        propdata,           /**< propagator data */
+   int                   row,                /**< row of the device matrix */
+   int                   j                   /**< column */
+   )
+{
+   int64_t k;
+
+   for( k = propdata->rowptr[row]; k < propdata->rowptr[row + 1]; ++k )
+      if( propdata->colidx[k] == j )
+         return propdata->vals[k];
+   return 0.0;
+}
+
+/** PROPEXEC with reasons: the device's changes in round order, each with its reason row as
+ *  inferinfo; on a cutoff, the changes of the earlier rounds, then conflict analysis on the
+ *  certificate */
+static
+SCIP_RETCODE execWithReasons(
+   SCIP*                 scip,               /**< SCIP data structure */
+   SCIP_PROP*            prop,               /**< the propagator */
+   SCIP_PROPDATA*        propdata,           /**< propagator data */
+   SCIP_RESULT*          result              /**< pointer to store the result */
+   )
+{
+   sap_params params;
+   sap_stats stats;
+   sap_conflict conflict;
+   double* lb;
+   double* ub;
+   double* hval = NULL;
+   int32_t* hround = NULL;
+   int32_t* hcol = NULL;
+   int32_t* hbound = NULL;
+   int32_t* hrow = NULL;
+   int64_t nhist = 0;
+   int64_t cap = 4 * (int64_t)propdata->nvars + 64;
+   SCIP_Bool cutoff;
+   int nchg = 0;
+   int64_t q;
+   int rc;
+   int j;
+
+   SCIP_CALL( SCIPallocBufferArray(scip, &lb, propdata->nvars) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &ub, propdata->nvars) );
+   SCIP_CALL( getEngineParams(scip, propdata, &params) );
+   for( ;; )
+   {
+      for( j = 0; j < propdata->nvars; ++j )
+      {
+         lb[j] = SCIPvarGetLbLocal(propdata->vars[j]);
+         ub[j] = SCIPvarGetUbLocal(propdata->vars[j]);
+      }
+      SCIP_CALL( SCIPallocBufferArray(scip, &hround, cap) );
+      SCIP_CALL( SCIPallocBufferArray(scip, &hcol, cap) );
+      SCIP_CALL( SCIPallocBufferArray(scip, &hbound, cap) );
+      SCIP_CALL( SCIPallocBufferArray(scip, &hval, cap) );
+      SCIP_CALL( SCIPallocBufferArray(scip, &hrow, cap) );
+      memset(&stats, 0, sizeof(stats));
+      rc = sap_propagate_reasons(propdata->mat, &params, lb, ub, cap, &nhist, hround, hcol, hbound, hval, hrow,
+         &conflict, &stats);
+      if( rc != SAP_NOMEMORY || nhist <= cap )
+         break;
+      /* a longer history than the room given: again with room for all of it (same results) */
+      cap = nhist;
+      SCIPfreeBufferArray(scip, &hrow);
+      SCIPfreeBufferArray(scip, &hval);
+      SCIPfreeBufferArray(scip, &hbound);
+      SCIPfreeBufferArray(scip, &hcol);
+      SCIPfreeBufferArray(scip, &hround);
+   }
+   SCIP_CALL( engineCall(rc, "sap_propagate_reasons") );
+   ++propdata->ncalls;
+   cutoff = FALSE;
+
+   /* the changes in round order: each is implied by its row and the bounds applied before it */
+   for( q = 0; q < nhist && !cutoff; ++q )
+   {
+      SCIP_VAR* var = propdata->vars[hcol[q]];
+      SCIP_Bool infeasible = FALSE;
+      SCIP_Bool tightened = FALSE;
+
+      if( hbound[q] == 0 )
+      {
+         SCIP_CALL( SCIPinferVarLbProp(scip, var, hval[q], prop, hrow[q], TRUE, &infeasible, &tightened) );
+      }
+      else
+      {
+         SCIP_CALL( SCIPinferVarUbProp(scip, var, hval[q], prop, hrow[q], TRUE, &infeasible, &tightened) );
+      }
+      nchg += tightened ? 1 : 0;
+      cutoff = infeasible;
+   }
+   propdata->nchgbds += nchg;
+
+   if( !cutoff && stats.result == SAP_CUTOFF )
+   {
+      cutoff = TRUE;
+      ++propdata->ncutoffs;
+      /* conflict analysis on the certificate (cons_linear.c analyzeConflict) */
+      if( (SCIPgetStage(scip) == SCIP_STAGE_SOLVING || SCIPinProbing(scip)) && SCIPisConflictAnalysisApplicable(scip)
+         && conflict.kind != SAP_CONFLICT_NONE )
+      {
+         SCIP_CALL( SCIPinitConflictAnalysis(scip, SCIP_CONFTYPE_PROPAGATION, FALSE) );
+         if( conflict.kind == SAP_CONFLICT_ROW )
+         {
+            /* side 0: the minimal activity exceeds rhs; side 1: the maximal one misses lhs */
+            SCIP_CALL( addRowBounds(scip, propdata, conflict.row, -1, conflict.side == 1, NULL) );
+         }
+         else
+         {
+            SCIP_VAR* var = propdata->vars[conflict.col];
+            double a;
+
+            /* the lower side: rowlb's residual activity implied it, else the column's own bound */
+            if( conflict.rowlb >= 0 )
+            {
+               a = rowCoef(propdata, conflict.rowlb, conflict.col);
+               SCIP_CALL( addRowBounds(scip, propdata, conflict.rowlb, conflict.col, a > 0.0, NULL) );
+            }
+            else
+            {
+               SCIP_CALL( SCIPaddConflictLb(scip, var, NULL) );
+            }
+            if( conflict.rowub >= 0 )
+            {
+               a = rowCoef(propdata, conflict.rowub, conflict.col);
+               SCIP_CALL( addRowBounds(scip, propdata, conflict.rowub, conflict.col, a < 0.0, NULL) );
+            }
+            else
+            {
+               SCIP_CALL( SCIPaddConflictUb(scip, var, NULL) );
+            }
+         }
+         SCIP_CALL( SCIPanalyzeConflict(scip, 0, NULL) );
+         ++propdata->nconflicts;
+      }
+   }
+   *result = cutoff ? SCIP_CUTOFF : (nchg > 0 ? SCIP_REDUCEDDOM : SCIP_DIDNOTFIND);
+
+   SCIPfreeBufferArray(scip, &hrow);
+   SCIPfreeBufferArray(scip, &hval);
+   SCIPfreeBufferArray(scip, &hbound);
+   SCIPfreeBufferArray(scip, &hcol);
+   SCIPfreeBufferArray(scip, &hround);
+   SCIPfreeBufferArray(scip, &ub);
+   SCIPfreeBufferArray(scip, &lb);
+   return SCIP_OKAY;
+}
+
 /*
  * callback methods of propagator
  */
@@ -598,6 +817,8 @@ SCIP_DECL_PROPEXEC(propExecGpulinear)
    *result = SCIP_DIDNOTRUN;
    if( propdata->mat == NULL )
       return SCIP_OKAY;
+   if( propdata->reasons )
+      return execWithReasons(scip, prop, propdata, result);
 
    SCIP_CALL( SCIPallocBufferArray(scip, &lb, propdata->nvars) );
    SCIP_CALL( SCIPallocBufferArray(scip, &ub, propdata->nvars) );
@@ -635,14 +856,30 @@ SCIP_DECL_PROPEXEC(propExecGpulinear)
    return SCIP_OKAY;
 }
 
-/** conflict resolution: the device keeps no per-change reason, so a bound change of this
- *  propagator is reported as unexplained; conflict analysis then keeps the change itself in
- *  the conflict (valid, possibly weaker conflicts) instead of failing with
- *  SCIP_PLUGINNOTFOUND for a propagator without this callback (prop.c:780-783) */
+/** conflict resolution (prop.c:739-788): inferinfo is the reason row of the change.  A lower
+ *  bound of a column with a positive coefficient came from the row's lhs and its maximal
+ *  residual activity, with a negative one from the rhs and the minimal residual; symmetric
+ *  for upper bounds; the bounds making up that residual at the change's time explain it
+ *  (cons_linear.c resolvePropagation / addConflictBounds).  Without a recorded row (reasons
+ *  off, or a change from before the current matrix) the change is reported as unexplained,
+ *  which conflict analysis accepts (the change itself stays in the conflict). */
 static
 SCIP_DECL_PROPRESPROP(propRespropGpulinear)
 {  /*lint --e{715}*/
+   SCIP_PROPDATA* propdata = SCIPpropGetData(prop);
+   int j = SCIPvarGetProbindex(infervar);
+   double a;
+
    *result = SCIP_DIDNOTFIND;
+   if( !propdata->reasons || propdata->rowptr == NULL || inferinfo < 0 || inferinfo >= propdata->nrows || j < 0
+      || j >= propdata->nvars || propdata->vars[j] != infervar )
+      return SCIP_OKAY;
+   a = rowCoef(propdata, inferinfo, j);
+   if( a == 0.0 )
+      return SCIP_OKAY;
+   SCIP_CALL( addRowBounds(scip, propdata, inferinfo, j, (boundtype == SCIP_BOUNDTYPE_LOWER) == (a > 0.0),
+         bdchgidx) );
+   *result = SCIP_SUCCESS;
    return SCIP_OKAY;
 }
 
@@ -680,6 +917,9 @@ SCIP_RETCODE SCIPincludePropGpulinear(
    SCIP_CALL( SCIPaddBoolParam(scip, "propagating/" PROP_NAME "/probing",
          "probe the unfixed binaries on the device in presolving", &propdata->probing, FALSE, DEFAULT_PROBING,
          NULL, NULL) );
+   SCIP_CALL( SCIPaddBoolParam(scip, "propagating/" PROP_NAME "/reasons",
+         "record a reason row for every device bound change (conflict analysis)", &propdata->reasons, FALSE,
+         DEFAULT_REASONS, NULL, NULL) );
    SCIP_CALL( SCIPaddIntParam(scip, "propagating/" PROP_NAME "/probingvectors",
          "node vectors of one probing launch (two per probed binary)", &propdata->probingvectors, TRUE,
          DEFAULT_PROBINGVECTORS, 2, 1 << 20, NULL, NULL) );

@@ -2,7 +2,7 @@
  *
  * It declares only the subset of SCIP's API that the gpulinear propagator calls, with the
  * signatures of SCIP's own declarations (src/scip/scip_prop.h, scip_var.h, scip_param.h,
- * prop_probing.h (in scip/prop_probing.h here),
+ * prop_probing.h (in scip/prop_probing.h here), scip_conflict.h, scip_general.h,
  * scip_numerics.h, scip_mem.h, pub_cons.h, pub_prop.h, pub_var.h), so that the plugin
  * compiles and runs under tests/ without a SCIP library.  Built against an installed SCIP,
  * the plugin uses SCIP's own headers instead and this directory is not on the include path.
@@ -61,6 +61,8 @@ typedef struct SCIP_ParamData SCIP_PARAMDATA;
 typedef struct SCIP_Param SCIP_PARAM;
 typedef struct SCIP_BdChgIdx SCIP_BDCHGIDX;
 typedef enum SCIP_BoundType { SCIP_BOUNDTYPE_LOWER = 0, SCIP_BOUNDTYPE_UPPER = 1 } SCIP_BOUNDTYPE;
+typedef enum SCIP_Stage { SCIP_STAGE_PRESOLVING = 6, SCIP_STAGE_SOLVING = 9 } SCIP_STAGE;   /* type_set.h values */
+typedef enum SCIP_ConflictType { SCIP_CONFTYPE_UNKNOWN = 0, SCIP_CONFTYPE_PROPAGATION = 1 } SCIP_CONFTYPE;
 
 #define SCIP_DECL_PARAMCHGD(x) SCIP_RETCODE x (SCIP* scip, SCIP_PARAM* param)
 #define SCIP_DECL_PROPFREE(x) SCIP_RETCODE x (SCIP* scip, SCIP_PROP* prop)
@@ -109,6 +111,15 @@ SCIP_RETCODE SCIPaddIntParam(SCIP* scip, const char* name, const char* desc, int
 SCIP_RETCODE SCIPaddBoolParam(SCIP* scip, const char* name, const char* desc, SCIP_Bool* valueptr,
    SCIP_Bool isadvanced, SCIP_Bool defaultvalue, SCIP_DECL_PARAMCHGD((*paramchgd)), SCIP_PARAMDATA* paramdata);
 SCIP_RETCODE SCIPgetRealParam(SCIP* scip, const char* name, SCIP_Real* value);
+/* scip_general.h / scip_probing.h */
+SCIP_STAGE SCIPgetStage(SCIP* scip);
+SCIP_Bool SCIPinProbing(SCIP* scip);
+/* scip_conflict.h */
+SCIP_Bool SCIPisConflictAnalysisApplicable(SCIP* scip);
+SCIP_RETCODE SCIPinitConflictAnalysis(SCIP* scip, SCIP_CONFTYPE conftype, SCIP_Bool iscutoffinvolved);
+SCIP_RETCODE SCIPaddConflictLb(SCIP* scip, SCIP_VAR* var, SCIP_BDCHGIDX* bdchgidx);
+SCIP_RETCODE SCIPaddConflictUb(SCIP* scip, SCIP_VAR* var, SCIP_BDCHGIDX* bdchgidx);
+SCIP_RETCODE SCIPanalyzeConflict(SCIP* scip, int validdepth, SCIP_Bool* success);
 /* scip_numerics.h */
 SCIP_Real SCIPinfinity(SCIP* scip);
 SCIP_Real SCIPepsilon(SCIP* scip);

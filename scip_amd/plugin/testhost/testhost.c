@@ -8,7 +8,11 @@
  * change the plugin hands over is accepted or rejected as SCIP would.
  *
  * Probing deductions handed to SCIPanalyzeDeductionsProbing and fixings made with SCIPfixVar
- * are recorded for the tests (th_get_probe_records).
+ * are recorded for the tests (th_get_probe_records).  Local bound changes made through
+ * SCIPinferVarLbProp / SCIPinferVarUbProp are kept in order with their inferinfo; a bound
+ * change index is a position in that list, and SCIPaddConflictLb / SCIPaddConflictUb record
+ * the bound in force before that position (the bound "at the time" of SCIP's bdchgidx), so a
+ * test can check that a conflict set explains a change.
  *
  * Test infrastructure: built into scip_amd/plugin/libgpulinear_testhost.so for tests/ only.
  */
@@ -25,6 +29,12 @@ struct SCIP_Var
    int probindex;
    int integral;
    double glb, gub, llb, lub;
+   double llb0, lub0;   /* local bounds at the start of the node (before the recorded changes) */
+};
+
+struct SCIP_BdChgIdx
+{
+   int pos;             /* position in the change list */
 };
 
 struct SCIP_Cons
@@ -76,6 +86,13 @@ struct Scip
    int nded, dedcap;
    struct DedRec { int probe, var; double bnd[4]; }* ded;
    int naggr, nimpl;
+   /* local bound changes of the node, in order, and the conflict candidate set */
+   int nchg, chgcap;
+   struct ChgRec { int var, bound, inferinfo; double oldval, newval; }* chg;
+   int nconf, confcap;
+   struct ConfRec { int var, bound; double val; }* conf;
+   int nanalyzed;
+   int stage;
 };
 
 /* ---- numerics (def.h, set.c) ---- */
@@ -299,17 +316,88 @@ static SCIP_RETCODE chgUb(SCIP* scip, SCIP_VAR* var, double nb, SCIP_Bool force,
    return SCIP_OKAY;
 }
 
+static int grow(void** p, int* cap, int need, size_t elem);
+
+static SCIP_RETCODE noteChange(SCIP* scip, SCIP_VAR* var, int bound, int inferinfo, double oldval, double newval)
+{
+   struct ChgRec* r;
+   if( !grow((void**)&scip->chg, &scip->chgcap, scip->nchg + 1, sizeof(*scip->chg)) )
+      return SCIP_NOMEMORY;
+   r = &scip->chg[scip->nchg++];
+   r->var = var->probindex;
+   r->bound = bound;
+   r->inferinfo = inferinfo;
+   r->oldval = oldval;
+   r->newval = newval;
+   return SCIP_OKAY;
+}
+
 SCIP_RETCODE SCIPinferVarLbProp(SCIP* scip, SCIP_VAR* var, SCIP_Real newbound, SCIP_PROP* inferprop, int inferinfo,
    SCIP_Bool force, SCIP_Bool* infeasible, SCIP_Bool* tightened)
 {
-   (void)inferprop; (void)inferinfo;
-   return chgLb(scip, var, newbound, force, FALSE, infeasible, tightened);
+   double old = var->llb;
+   SCIP_Bool t = FALSE;
+   (void)inferprop;
+   SCIP_CALL( chgLb(scip, var, newbound, force, FALSE, infeasible, &t) );
+   if( tightened )
+      *tightened = t;
+   return t ? noteChange(scip, var, 0, inferinfo, old, var->llb) : SCIP_OKAY;
 }
 SCIP_RETCODE SCIPinferVarUbProp(SCIP* scip, SCIP_VAR* var, SCIP_Real newbound, SCIP_PROP* inferprop, int inferinfo,
    SCIP_Bool force, SCIP_Bool* infeasible, SCIP_Bool* tightened)
 {
-   (void)inferprop; (void)inferinfo;
-   return chgUb(scip, var, newbound, force, FALSE, infeasible, tightened);
+   double old = var->lub;
+   SCIP_Bool t = FALSE;
+   (void)inferprop;
+   SCIP_CALL( chgUb(scip, var, newbound, force, FALSE, infeasible, &t) );
+   if( tightened )
+      *tightened = t;
+   return t ? noteChange(scip, var, 1, inferinfo, old, var->lub) : SCIP_OKAY;
+}
+
+/* ---- conflict analysis ---- */
+SCIP_STAGE SCIPgetStage(SCIP* scip) { return (SCIP_STAGE)scip->stage; }
+SCIP_Bool SCIPinProbing(SCIP* scip) { (void)scip; return FALSE; }
+SCIP_Bool SCIPisConflictAnalysisApplicable(SCIP* scip) { (void)scip; return TRUE; }
+SCIP_RETCODE SCIPinitConflictAnalysis(SCIP* scip, SCIP_CONFTYPE conftype, SCIP_Bool iscutoffinvolved)
+{
+   (void)conftype; (void)iscutoffinvolved;
+   scip->nconf = 0;
+   return SCIP_OKAY;
+}
+
+/* the local bound of var in force just before change position pos (all recorded changes:
+ * pos = nchg) */
+static double boundAt(SCIP* scip, SCIP_VAR* var, int bound, int pos)
+{
+   double v = bound == 0 ? var->llb0 : var->lub0;
+   int i;
+   for( i = 0; i < pos && i < scip->nchg; ++i )
+      if( scip->chg[i].var == var->probindex && scip->chg[i].bound == bound )
+         v = scip->chg[i].newval;
+   return v;
+}
+
+static SCIP_RETCODE addConflict(SCIP* scip, SCIP_VAR* var, int bound, SCIP_BDCHGIDX* bdchgidx)
+{
+   struct ConfRec* r;
+   if( !grow((void**)&scip->conf, &scip->confcap, scip->nconf + 1, sizeof(*scip->conf)) )
+      return SCIP_NOMEMORY;
+   r = &scip->conf[scip->nconf++];
+   r->var = var->probindex;
+   r->bound = bound;
+   r->val = boundAt(scip, var, bound, bdchgidx != NULL ? bdchgidx->pos : scip->nchg);
+   return SCIP_OKAY;
+}
+SCIP_RETCODE SCIPaddConflictLb(SCIP* scip, SCIP_VAR* var, SCIP_BDCHGIDX* bdchgidx) { return addConflict(scip, var, 0, bdchgidx); }
+SCIP_RETCODE SCIPaddConflictUb(SCIP* scip, SCIP_VAR* var, SCIP_BDCHGIDX* bdchgidx) { return addConflict(scip, var, 1, bdchgidx); }
+SCIP_RETCODE SCIPanalyzeConflict(SCIP* scip, int validdepth, SCIP_Bool* success)
+{
+   (void)validdepth;
+   scip->nanalyzed++;
+   if( success != NULL )
+      *success = TRUE;
+   return SCIP_OKAY;
 }
 SCIP_RETCODE SCIPtightenVarLb(SCIP* scip, SCIP_VAR* var, SCIP_Real newbound, SCIP_Bool force, SCIP_Bool* infeasible,
    SCIP_Bool* tightened)
@@ -442,6 +530,7 @@ SCIP_EXPORT SCIP* th_create(void)
    s->boundstreps = 0.05;
    s->recompfac = 1e6;
    s->maxeasy = 1e6;
+   s->stage = SCIP_STAGE_SOLVING;
    return s;
 }
 
@@ -458,8 +547,8 @@ SCIP_EXPORT int th_add_vars(SCIP* s, int n, const double* lb, const double* ub, 
          return SCIP_NOMEMORY;
       v->probindex = i;
       v->integral = integral[i];
-      v->glb = v->llb = lb[i];
-      v->gub = v->lub = ub[i];
+      v->glb = v->llb = v->llb0 = lb[i];
+      v->gub = v->lub = v->lub0 = ub[i];
       s->vars[i] = v;
    }
    s->nvars = n;
@@ -551,9 +640,61 @@ SCIP_EXPORT void th_set_local_bounds(SCIP* s, const double* lb, const double* ub
    int i;
    for( i = 0; i < s->nvars; ++i )
    {
-      s->vars[i]->llb = lb[i];
-      s->vars[i]->lub = ub[i];
+      s->vars[i]->llb = s->vars[i]->llb0 = lb[i];
+      s->vars[i]->lub = s->vars[i]->lub0 = ub[i];
    }
+   s->nchg = 0;   /* a new node: no local changes yet */
+}
+
+/* conflict analysis resolving recorded change pos (prop.c:739-788): the propagator's
+ * PROPRESPROP gets the changed variable, its inferinfo, the bound type and the change's index;
+ * the conflict set it adds is then read with th_get_conflict */
+SCIP_EXPORT int th_resprop_change(SCIP* s, int pos, int* result)
+{
+   SCIP_RESULT r = SCIP_DIDNOTRUN;
+   SCIP_BDCHGIDX idx;
+   SCIP_RETCODE rc;
+   const struct ChgRec* c;
+   if( pos < 0 || pos >= s->nchg )
+      return SCIP_INVALIDDATA;
+   if( s->prop->resprop == NULL )
+      return SCIP_PLUGINNOTFOUND;
+   c = &s->chg[pos];
+   idx.pos = pos;
+   s->nconf = 0;
+   rc = s->prop->resprop(s, s->prop, s->vars[c->var], c->inferinfo,
+      c->bound ? SCIP_BOUNDTYPE_UPPER : SCIP_BOUNDTYPE_LOWER, &idx, c->newval, &r);
+   *result = (int)r;
+   return rc;
+}
+
+/* the recorded local changes; with non-NULL arrays they are copied out */
+SCIP_EXPORT int th_get_changes(SCIP* s, int* var, int* bound, int* inferinfo, double* oldval, double* newval)
+{
+   int i;
+   for( i = 0; var != NULL && i < s->nchg; ++i )
+   {
+      var[i] = s->chg[i].var;
+      bound[i] = s->chg[i].bound;
+      inferinfo[i] = s->chg[i].inferinfo;
+      oldval[i] = s->chg[i].oldval;
+      newval[i] = s->chg[i].newval;
+   }
+   return s->nchg;
+}
+
+/* the conflict candidate set and the number of SCIPanalyzeConflict calls */
+SCIP_EXPORT int th_get_conflict(SCIP* s, int* var, int* bound, double* val, int* nanalyzed)
+{
+   int i;
+   for( i = 0; var != NULL && i < s->nconf; ++i )
+   {
+      var[i] = s->conf[i].var;
+      bound[i] = s->conf[i].bound;
+      val[i] = s->conf[i].val;
+   }
+   *nanalyzed = s->nanalyzed;
+   return s->nconf;
 }
 
 SCIP_EXPORT void th_get_bounds(SCIP* s, int global, double* lb, double* ub)
@@ -607,6 +748,8 @@ SCIP_EXPORT int th_free(SCIP* s)
    free(s->vars);
    free(s->fix);
    free(s->ded);
+   free(s->chg);
+   free(s->conf);
    free(s);
    return rc;
 }

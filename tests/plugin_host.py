@@ -45,6 +45,9 @@ def lib():
         IP = C.POINTER(C.c_int)
         L.th_get_probe_records.argtypes = [C.c_void_p, IP, C.c_void_p, C.c_void_p, IP, C.c_void_p, C.c_void_p,
                                            C.c_void_p, IP, IP]
+        L.th_resprop_change.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        L.th_get_changes.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+        L.th_get_conflict.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -111,6 +114,30 @@ class Host:
         fix = [(int(fv[i]), float(fx[i])) for i in range(nf.value)]
         ded = [(int(dp[i]), int(dv[i])) + tuple(float(x) for x in db[i]) for i in range(nd.value)]
         return fix, ded, na.value, ni.value
+
+    def changes(self):
+        """Local bound changes handed to SCIP: (var, bound 0/1, inferinfo, old, new) arrays."""
+        L = lib()
+        n = L.th_get_changes(self.h, None, None, None, None, None)
+        v, b, i = (np.zeros(max(n, 1), dtype=np.int32) for _ in range(3))
+        o, w = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        L.th_get_changes(self.h, v.ctypes.data, b.ctypes.data, i.ctypes.data, o.ctypes.data, w.ctypes.data)
+        return v[:n], b[:n], i[:n], o[:n], w[:n]
+
+    def resprop_change(self, pos):
+        r = C.c_int(0)
+        rc = lib().th_resprop_change(self.h, pos, C.byref(r))
+        return rc, r.value
+
+    def conflict(self):
+        """Conflict candidate set [(var, bound 0/1, value)] and the number of analyses."""
+        L = lib()
+        na = C.c_int(0)
+        n = L.th_get_conflict(self.h, None, None, None, C.byref(na))
+        v, b = np.zeros(max(n, 1), dtype=np.int32), np.zeros(max(n, 1), dtype=np.int32)
+        x = np.zeros(max(n, 1))
+        L.th_get_conflict(self.h, v.ctypes.data, b.ctypes.data, x.ctypes.data, C.byref(na))
+        return [(int(v[k]), int(b[k]), float(x[k])) for k in range(n)], na.value
 
     def close(self):
         if self.h:

@@ -10,7 +10,8 @@ import pytest
 import scip_amd as sa
 from oracle import oracle as orc
 from tests.reasons_check import check_conflict, replay_history
-from tests.util import INF, make_instance, random_binary_instance, random_integer_instance
+from tests.util import (INF, late_cut_instance, make_instance, planted_integer_instance, random_binary_instance,
+                        random_integer_instance)
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -71,6 +72,19 @@ def test_random_binary(gpu_ctx, seed):
 def test_random_integer(gpu_ctx, seed):
     inst = random_integer_instance(np.random.default_rng(950 + seed), 400, 300, 8, sidejitter=0.3)
     compare(gpu_ctx, inst, f"integer{seed}")
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_planted_integer(gpu_ctx, seed):
+    compare(gpu_ctx, planted_integer_instance(np.random.default_rng(950 + seed), 400, 300, 6), f"pint{seed}")
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_row_certificate_after_history(gpu_ctx, seed):
+    inst = random_binary_instance(np.random.default_rng(900 + seed), 1500, 1000, 10, hidden=True)
+    o = orc.propagate_reasons(orc.OMatrix.from_instance(inst), inst.lb, inst.ub)
+    g = compare(gpu_ctx, late_cut_instance(inst, o["round"], o["col"], o["bound"], o["val"]), f"late{seed}")
+    assert g["conflict"]["kind"] == sa.SAP_CONFLICT_ROW and g["nhist"] > 0
 
 
 @pytest.mark.parametrize("seed", range(4))

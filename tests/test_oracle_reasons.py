@@ -8,7 +8,8 @@ import pytest
 import scip_amd as sa
 from oracle import oracle as orc
 from tests.reasons_check import check_conflict, replay_history
-from tests.util import INF, make_instance, random_binary_instance, random_integer_instance
+from tests.util import (INF, late_cut_instance, make_instance, planted_integer_instance, random_binary_instance,
+                        random_integer_instance)
 
 
 def run_and_check(inst):
@@ -38,6 +39,21 @@ def test_binary_reasons(seed):
 def test_integer_reasons(seed):
     inst = random_integer_instance(np.random.default_rng(950 + seed), 60, 40, 6, sidejitter=0.3)
     run_and_check(inst)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_planted_integer_reasons(seed):
+    r = run_and_check(planted_integer_instance(np.random.default_rng(950 + seed), 400, 300, 6))
+    assert r["nhist"] > 100 and r["result"]["rounds"] > 5
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_row_certificate_after_rounds_of_history(seed):
+    inst = random_binary_instance(np.random.default_rng(900 + seed), 1500, 1000, 10, hidden=True)
+    o = orc.propagate_reasons(orc.OMatrix.from_instance(inst), inst.lb, inst.ub)
+    t = late_cut_instance(inst, o["round"], o["col"], o["bound"], o["val"])
+    r = run_and_check(t)
+    assert r["conflict"]["kind"] == 1 and r["conflict"]["round"] >= 3 and r["nhist"] > 0
 
 
 @pytest.mark.parametrize("seed", range(4))

@@ -31,7 +31,7 @@ def test_plugin_registers_parameters():
     assert h.close() == ph.SCIP_OKAY
 
 
-def test_conflict_resolution_reports_unexplained_change():
+def test_conflict_resolution_without_a_matrix_reports_unexplained_change():
     inst = sa.gen_covering_packing(50, 40, 4, seed=3)
     h = ph.Host(inst)
     for upper in (False, True):
@@ -194,4 +194,87 @@ def test_presolve_probing_hands_the_device_branches_to_scip(gpu_ctx, seed):
     # the case finds fixings from cut-off branches and pairs for SCIP's analysis
     assert len(efix) > 0 and len(eded) > 0
     assert h.call("th_exitpre") == ph.SCIP_OKAY
+    assert h.close() == ph.SCIP_OKAY
+
+
+def conflict_arrays(n, cset):
+    """Bounds of a conflict set as arrays; columns not in the set get infinite bounds, so a
+    check that needs them fails."""
+    lb = np.full(n, -1e20)
+    ub = np.full(n, 1e20)
+    for j, b, v in cset:
+        if b == 0:
+            lb[j] = v
+        else:
+            ub[j] = v
+    return lb, ub
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["binary", "mixed", "integer"])
+def test_propexec_changes_carry_reasons_that_explain_them(gpu_ctx, case):
+    from tests.reasons_check import check_implied
+    from tests.util import planted_integer_instance
+    if case == "binary":
+        inst = random_binary_instance(np.random.default_rng(904), 1500, 1000, 10, hidden=True)
+    elif case == "mixed":
+        inst = sa.gen_mixed(3000, 3000, seed=61, planted=True)
+    else:
+        inst = planted_integer_instance(np.random.default_rng(951), 400, 300, 6)
+    mat = sa.Matrix(gpu_ctx, inst)
+    ref = mat.propagate_reasons()
+    mat.close()
+    assert ref["stats"]["result"] != sa.SAP_CUTOFF and ref["nhist"] > 0
+    print(f"[case] plugin reasons {case} nhist={ref['nhist']}", flush=True)
+    h = ph.Host(inst)
+    assert h.call("th_initsol") == ph.SCIP_OKAY
+    rc, res = h.propexec()
+    assert rc == ph.SCIP_OKAY and res == ph.SCIP_REDUCEDDOM
+    var, bound, info, old, new = h.changes()
+    # the device history, in order, with the reason rows as inferinfo
+    assert np.array_equal(var, ref["col"]) and np.array_equal(bound, ref["bound"])
+    assert np.array_equal(info, ref["row"]) and np.array_equal(new, ref["val"])
+    # every change is explained by its row and the bounds in force at its time
+    for pos in range(len(var)):
+        rc, r = h.resprop_change(pos)
+        assert rc == ph.SCIP_OKAY and r == ph.SCIP_SUCCESS
+        cset, _ = h.conflict()
+        lb, ub = conflict_arrays(inst.ncols, cset)
+        check_implied(inst, int(info[pos]), int(var[pos]), int(bound[pos]), float(new[pos]), lb, ub)
+    assert h.call("th_exitsol") == ph.SCIP_OKAY
+    assert h.close() == ph.SCIP_OKAY
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(4))
+def test_propexec_cutoff_starts_conflict_analysis(gpu_ctx, seed):
+    from tests.reasons_check import check_conflict
+    from tests.util import late_cut_instance, random_integer_instance
+    if seed < 2:
+        # a row certificate after rounds of history
+        base = random_binary_instance(np.random.default_rng(900 + seed), 1500, 1000, 10, hidden=True)
+        o = orc.propagate_reasons(orc.OMatrix.from_instance(base), base.lb, base.ub)
+        inst = late_cut_instance(base, o["round"], o["col"], o["bound"], o["val"])
+    elif seed == 2:
+        inst = random_binary_instance(np.random.default_rng(901), 1500, 1000, 10, hidden=False)
+    else:
+        inst = random_integer_instance(np.random.default_rng(953), 400, 300, 8, sidejitter=0.3)
+    mat = sa.Matrix(gpu_ctx, inst)
+    ref = mat.propagate_reasons()
+    mat.close()
+    if ref["stats"]["result"] != sa.SAP_CUTOFF:
+        pytest.skip("instance not infeasible")
+    print(f"[case] plugin conflict seed={seed} kind={ref['conflict']['kind']}", flush=True)
+    h = ph.Host(inst)
+    assert h.call("th_initsol") == ph.SCIP_OKAY
+    rc, res = h.propexec()
+    assert rc == ph.SCIP_OKAY and res == ph.SCIP_CUTOFF
+    var, bound, info, old, new = h.changes()
+    assert np.array_equal(var, ref["col"]) and np.array_equal(info, ref["row"])
+    cset, nanalyzed = h.conflict()
+    assert nanalyzed == 1 and len(cset) > 0
+    # the conflict set alone proves the node infeasible
+    lb, ub = conflict_arrays(inst.ncols, cset)
+    check_conflict(inst, ref["conflict"], lb, ub)
+    assert h.call("th_exitsol") == ph.SCIP_OKAY
     assert h.close() == ph.SCIP_OKAY

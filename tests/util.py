@@ -93,3 +93,43 @@ def apply_fixings(lb, ub, ptr, cols, vals, v):
 def same_bounds(a, b):
     """Numeric equality of bound arrays (-0.0 == +0.0)."""
     return np.array_equal(np.asarray(a), np.asarray(b))
+
+
+def planted_integer_instance(rng, nrows, ncols, maxlen, coefrange=6, boundrange=6):
+    """Small general-integer instance with sides tight around a planted integer point (feasible,
+    with several propagation rounds)."""
+    lb = -rng.integers(0, boundrange, size=ncols).astype(float)
+    ub = lb + rng.integers(1, 2 * boundrange, size=ncols)
+    xs = np.array([rng.integers(int(l), int(u) + 1) for l, u in zip(lb, ub)], dtype=float)
+    rows = []
+    for _ in range(nrows):
+        L = int(rng.integers(2, maxlen + 1))
+        cols = rng.choice(ncols, size=min(L, ncols), replace=False)
+        coefs = rng.integers(1, coefrange + 1, size=len(cols)) * rng.choice([-1, 1], size=len(cols))
+        act = float(coefs @ xs[cols])
+        kind = rng.integers(0, 3)
+        lo = act - float(rng.integers(0, 3)) if kind != 1 else -INF
+        hi = act + float(rng.integers(0, 3)) if kind != 0 else INF
+        rows.append(([(int(c), float(a)) for c, a in zip(cols, coefs)], lo, hi))
+    return make_instance(rows, lb, ub, [1] * ncols)
+
+
+def late_cut_instance(inst, rounds, cols, bounds, vals):
+    """inst plus one row x_k + x_m <= 1 over two binary columns that its propagation lifts to 1
+    in the same round r >= 2 (from a reason history: rounds, cols, bounds, vals), so the system
+    is found infeasible by that row in round r + 1, after r rounds of history; None if no such
+    pair exists."""
+    byround = {}
+    for r, j, b, v in zip(rounds, cols, bounds, vals):
+        if r >= 2 and b == 0 and v == 1.0 and inst.lb[j] == 0.0 and inst.ub[j] == 1.0:
+            byround.setdefault(int(r), []).append(int(j))
+    for r in sorted(byround):
+        js = byround[r]
+        if len(js) >= 2:
+            rows = []
+            for i in range(inst.nrows):
+                a, e = inst.rowptr[i], inst.rowptr[i + 1]
+                rows.append((list(zip(inst.colidx[a:e].tolist(), inst.vals[a:e].tolist())), inst.lhs[i], inst.rhs[i]))
+            rows.append(([(js[0], 1.0), (js[1], 1.0)], -INF, 1.0))
+            return make_instance(rows, inst.lb, inst.ub, inst.integral)
+    return None
