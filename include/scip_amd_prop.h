@@ -65,6 +65,12 @@ typedef struct sap_params
 /* run the round loop from the host with one synchronisation per round instead of launching
  * hipGraph-captured groups of rounds (same results; for debugging and comparison) */
 #define SAP_FLAG_HOSTLOOP 0x2
+/* also run ranged-row propagation in every round (rangedRowPropagation, cons_linear.c:5715-6697;
+ * SCIP's constraints/linear/rangedrowpropagation, on by default outside presolving): ranged
+ * rows and equations of at least 3 nonzeros with both sides below 1e20 in magnitude get the
+ * gcd-based fixings, bounds and infeasibility test, as propagation applies it after the root
+ * node (no artificial constraints).  Not available together with reasons. */
+#define SAP_FLAG_RANGEDROWS 0x4
 
 /* per-call statistics */
 typedef struct sap_stats

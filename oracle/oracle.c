@@ -264,7 +264,7 @@ void orc_params_default(orc_params* p)
    p->recompfac = 1e6;
    p->maxeasyactivitydelta = 1e6;
    p->maxrounds = -1;
-   p->reserved = 0;
+   p->rangedrows = 0;
 }
 
 void orc_compute_binary(int64_t ncols, const uint8_t* integral, const double* glb, const double* gub,
@@ -861,6 +861,258 @@ static void jHard(jstate* js, const rowstate* rs, int64_t i, int64_t k, int forc
    }
 }
 
+/* ranged-row propagation of one row (rangedRowPropagation, cons_linear.c:5715-6697) on the
+ * start-of-round bounds, without artificial constraints (cons_linear.c:5784, propagation after
+ * the root): deductions go in as forced candidates (SCIPinferVarFixCons / SCIPinferVarLbCons
+ * and UbCons with force, scip_var.c:6896-6945); the enumeration of check-group values gives up
+ * after ORC_RANGED_STEPS steps (the engine's limit; SCIP has none) */
+#define ORC_RANGED_STEPS (1 << 20)
+
+static inline double epsFloor(const orc_params* p, double x) { return floor(x + p->epsilon); }  /* EPSFLOOR, def.h:200 */
+static inline double epsCeil(const orc_params* p, double x) { return ceil(x - p->epsilon); }    /* EPSCEIL, def.h:201 */
+static inline int epsIntegral(const orc_params* p, double x) { return x - floor(x + p->epsilon) <= p->epsilon; }
+
+static long long gcdLL(long long a, long long b)
+{
+   while( b != 0 )
+   {
+      long long t = a % b;
+      a = b;
+      b = t;
+   }
+   return a;
+}
+
+static void jRanged(jstate* js, int64_t i)
+{
+   const orc_params* p = js->p;
+   const orc_matrix* m = js->m;
+   int64_t s = m->rowptr[i], e = m->rowptr[i + 1], n = e - s, k, nfixed = 0, nunfixed, ninf = 0, ncont = 0;
+   int64_t first = -1, last = -1, w = -1, steps = 0;
+   double fixedact = 0.0, lhs, rhs, minact = 0.0, maxact = 0.0, g, minvalue = 0.0, maxvalue = 0.0, value;
+   int possiblegcd = 1, gcdisone = 1, nsols = 0;
+   long long gcd, gcdinf = -1;
+   uint8_t* inchk;
+
+   if( n < 3 || isInf(p, -m->lhs[i]) || isInf(p, m->rhs[i]) )
+      return;
+   /* fixed activity, last nonzero first (cons_linear.c:5809-5818) */
+   for( k = e - 1; k >= s; --k )
+   {
+      int32_t j = m->colidx[k];
+      if( isEQ(p, js->lb[j], js->ub[j]) )
+      {
+         fixedact += js->lb[j] * m->vals[k];
+         ++nfixed;
+      }
+   }
+   if( isHuge(p, fabs(fixedact)) )
+      return;
+   lhs = m->lhs[i] - fixedact;
+   rhs = m->rhs[i] - fixedact;
+   nunfixed = n - nfixed;
+   inchk = (uint8_t*)calloc((size_t)n, 1);
+
+   /* partition (cons_linear.c:5835-5960) */
+   for( k = s; k < e; ++k )
+   {
+      int32_t j = m->colidx[k];
+      double a = m->vals[k];
+      int integ = m->integral[j] != 0;
+      int group = integ && epsIntegral(p, a) && !isEQ(p, fabs(a), 1.0);
+      if( isEQ(p, js->lb[j], js->ub[j]) )
+         continue;
+      if( group )
+         break;
+      ncont += integ ? 0 : 1;
+      gcdisone = gcdisone && isEQ(p, fabs(a), 1.0);
+      possiblegcd = 0;
+      inchk[k - s] = 1;
+      ++ninf;
+   }
+   if( k == e || ncont + 2 > nunfixed )
+      goto TERMINATE;
+   gcd = (long long)(fabs(m->vals[k]) + p->feastol);
+   for( ; k < e; ++k )
+   {
+      int32_t j = m->colidx[k];
+      double a = m->vals[k];
+      int integ = m->integral[j] != 0;
+      if( isEQ(p, js->lb[j], js->ub[j]) )
+         continue;
+      if( !integ || !epsIntegral(p, a) || isEQ(p, fabs(a), 1.0) )
+      {
+         ncont += integ ? 0 : 1;
+         gcdisone = gcdisone && isEQ(p, fabs(a), 1.0);
+         possiblegcd = 0;
+         inchk[k - s] = 1;
+         ++ninf;
+      }
+      else
+      {
+         long long t = gcdLL(gcd, (long long)(fabs(a) + p->feastol));
+         if( t == 1 )
+         {
+            inchk[k - s] = 1;
+            ++ninf;
+         }
+         else
+            gcd = t;
+      }
+   }
+   if( ninf == 0 )
+      goto TERMINATE;
+
+   /* activity range of the check group, last first (cons_linear.c:5968-6012) */
+   for( k = e - 1; k >= s; --k )
+   {
+      int32_t j;
+      double a;
+      int mininv = 0, maxinv = 0;
+      if( !inchk[k - s] )
+         continue;
+      j = m->colidx[k];
+      a = m->vals[k];
+      if( last < 0 )
+         last = k;
+      first = k;
+      if( isInf(p, -js->lb[j]) )
+      {
+         if( a < 0.0 )
+            maxinv = 1;
+         else
+            mininv = 1;
+      }
+      else if( a < 0.0 )
+         maxact += a * js->lb[j];
+      else
+         minact += a * js->lb[j];
+      if( isInf(p, js->ub[j]) )
+      {
+         if( a > 0.0 )
+            maxinv = 1;
+         else
+            mininv = 1;
+      }
+      else if( a > 0.0 )
+         maxact += a * js->ub[j];
+      else
+         minact += a * js->ub[j];
+      if( mininv || maxinv || isHuge(p, -minact) || isHuge(p, maxact) )
+         goto TERMINATE;
+   }
+   g = (double)gcd;
+   /* no value of the gcd part fits between the sides (cons_linear.c:6029-6040) */
+   if( !epsIntegral(p, (lhs - maxact) / g) && isGT(p, epsCeil(p, (lhs - maxact) / g) * g, rhs - minact) )
+   {
+      js->cutoff = 1;
+      js->rowside = 0;
+      goto TERMINATE;
+   }
+   if( ncont != 0 )
+      goto TERMINATE;
+   if( possiblegcd )
+   {
+      gcdinf = (long long)(fabs(m->vals[last]) + p->feastol);
+      for( k = last; k >= s && gcdinf >= 2; --k )
+         if( inchk[k - s] )
+            gcdinf = gcdLL(gcdinf, (long long)(fabs(m->vals[k]) + p->feastol));
+   }
+   else if( gcdisone )
+      gcdinf = 1;
+   if( gcdinf < 1 )
+      goto TERMINATE;
+
+   /* solutions of the ranged row (cons_linear.c:6058-6135) */
+   for( value = epsCeil(p, minact - p->feastol); isLE(p, value, maxact); value += (double)gcdinf )
+   {
+      double value2;
+      if( ++steps > ORC_RANGED_STEPS )
+         goto TERMINATE;
+      value2 = value + g * epsCeil(p, (lhs - value) / g);
+      if( !isGE(p, value2, lhs) )
+         value2 += g;
+      if( isLE(p, value2, rhs) )
+      {
+         if( ++nsols == 3 )
+            break;
+         if( nsols == 1 )
+            minvalue = value;
+         maxvalue = value;
+      }
+   }
+   if( nsols == 3 )
+   {
+      for( value = epsFloor(p, maxact + p->feastol); isGE(p, value, minact); value -= (double)gcdinf )
+      {
+         double value2;
+         if( ++steps > 2 * (int64_t)ORC_RANGED_STEPS )
+            goto TERMINATE;
+         value2 = value + g * epsFloor(p, (rhs - value) / g);
+         if( !isLE(p, value2, rhs) )
+            value2 -= g;
+         if( isGE(p, value2, lhs) )
+         {
+            maxvalue = value;
+            break;
+         }
+      }
+   }
+   if( nsols == 0 )
+   {
+      js->cutoff = 1;
+      js->rowside = 0;
+      goto TERMINATE;
+   }
+   if( ninf == nunfixed - 1 )
+      for( k = s; k < e && w < 0; ++k )
+         if( !inchk[k - s] && !isEQ(p, js->lb[m->colidx[k]], js->ub[m->colidx[k]]) )
+            w = k;
+   if( ninf == 1 )
+   {
+      /* cons_linear.c:6154-6180, 6340-6388 */
+      double a = m->vals[first];
+      int32_t j = m->colidx[first];
+      if( nsols == 1 )
+      {
+         submitLb(js, j, maxvalue / a, 1);
+         submitUb(js, j, maxvalue / a, 1);
+      }
+      else
+      {
+         double nlb = a < 0.0 ? maxvalue / a : minvalue / a;
+         double nub = a < 0.0 ? minvalue / a : maxvalue / a;
+         if( nlb > js->lb[j] )
+            submitLb(js, j, nlb, 1);
+         if( nub < js->ub[j] )
+            submitUb(js, j, nub, 1);
+      }
+   }
+   else if( w >= 0 )
+   {
+      /* cons_linear.c:6184-6270, 6390-6505 */
+      double a = m->vals[w];
+      int32_t j = m->colidx[w];
+      if( nsols == 1 )
+      {
+         double fix = a < 0.0 ? epsFloor(p, (lhs - maxvalue) / a) : epsCeil(p, (lhs - maxvalue) / a);
+         submitLb(js, j, fix, 1);
+         submitUb(js, j, fix, 1);
+      }
+      else
+      {
+         double nlb = a < 0.0 ? epsFloor(p, (rhs - minvalue) / a) : epsCeil(p, (lhs - maxvalue) / a);
+         double nub = a < 0.0 ? epsFloor(p, (lhs - maxvalue) / a) : epsCeil(p, (rhs - minvalue) / a);
+         if( nlb > js->lb[j] )
+            submitLb(js, j, nlb, 1);
+         if( nub < js->ub[j] )
+            submitUb(js, j, nub, 1);
+      }
+   }
+TERMINATE:
+   free(inchk);
+}
+
 /* one row of one Jacobi round: tightenBounds + the infeasibility/redundancy test of
  * propagateCons; returns 1 if the row became redundant */
 static int jRow(jstate* js, int64_t i, double* xmin, double* xmax, dd* scratch)
@@ -903,6 +1155,9 @@ static int jRow(jstate* js, int64_t i, double* xmin, double* xmax, dd* scratch)
             jHard(js, &rs, i, k, force);
       }
    }
+   /* propagateCons: ranged rows after the tightening (cons_linear.c:7694-7700) */
+   if( !js->cutoff && p->rangedrows )
+      jRanged(js, i);
    if( js->cutoff )
       return 0;
 

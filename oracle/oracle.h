@@ -35,7 +35,7 @@ typedef struct orc_params
    double  recompfac;             /* numerics/recomputefac (1e6) */
    double  maxeasyactivitydelta;  /* constraints/linear/maxeasyactivitydelta (1e6) */
    int32_t maxrounds;             /* propagating/maxrounds (-1: unlimited) */
-   int32_t reserved;
+   int32_t rangedrows;            /* constraints/linear/rangedrowpropagation (Jacobi twin only) */
 } orc_params;
 
 typedef struct orc_matrix

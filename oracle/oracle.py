@@ -19,7 +19,7 @@ class orc_params(C.Structure):
     _fields_ = [
         ("infinity", C.c_double), ("epsilon", C.c_double), ("sumepsilon", C.c_double), ("feastol", C.c_double),
         ("boundstreps", C.c_double), ("hugeval", C.c_double), ("recompfac", C.c_double),
-        ("maxeasyactivitydelta", C.c_double), ("maxrounds", C.c_int32), ("reserved", C.c_int32),
+        ("maxeasyactivitydelta", C.c_double), ("maxrounds", C.c_int32), ("rangedrows", C.c_int32),
     ]
 
 

@@ -37,6 +37,7 @@ SAP_REDUCEDDOM = 10
 
 SAP_FLAG_BASE_IS_FIXPOINT = 0x1
 SAP_FLAG_HOSTLOOP = 0x2
+SAP_FLAG_RANGEDROWS = 0x4
 
 SAP_STEP_SHORT = 2
 SAP_STEP_LONG = 8

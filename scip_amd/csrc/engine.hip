@@ -120,6 +120,9 @@ struct Mat
    const int32_t* rowseg;   // NULL when nseg == 1
    const int32_t* colseg;
    const int32_t* perm;     // device row position -> caller's row index
+   const int32_t* rlist;    // rows with two finite sides and at least 3 nonzeros (rowflags bit 1)
+   int32_t nranged;
+   int32_t pad2;
 };
 
 struct Vec
@@ -164,6 +167,7 @@ struct Vec
    struct HistRec* hist;
    uint32_t histcap;
    uint32_t pad5;
+   uint8_t* rgflag;         // ranged-row propagation: per (vector, nonzero) check-group flags
 };
 
 struct CandRec
@@ -2142,6 +2146,303 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_history(Mat M, Vec V)
    }
 }
 
+// ------------------------------------------------------------------------------------------
+// ranged-row propagation (rangedRowPropagation, cons_linear.c:5715-6697), as propagation runs
+// it after the node's root (no artificial constraints: cons_linear.c:5784).  A ranged row or
+// equation with at least 3 nonzeros is split into the unfixed integral columns whose integral
+// coefficients share a gcd >= 2 and the rest (the check group); the check group's activity
+// range must meet a value that the gcd part can complement into [lhs, rhs].  No such value:
+// infeasible; one or two check-group values or a single column outside the check group give
+// fixings and bounds, submitted as forced candidates (SCIPinferVarFixCons / SCIPinferVarLbCons
+// with force).  One thread per (vector, row), on the bounds at the start of the round.  The
+// enumeration of check-group values stops after SAP_RANGED_STEPS steps, and the row then
+// yields nothing.
+// ------------------------------------------------------------------------------------------
+#define SAP_RANGED_STEPS (1 << 20)
+
+__device__ __forceinline__ double epsFloor(const Params& p, double x) { return floor(x + p.epsilon); }
+__device__ __forceinline__ double epsCeil(const Params& p, double x) { return ceil(x - p.epsilon); }
+__device__ __forceinline__ bool epsIntegral(const Params& p, double x) { return x - floor(x + p.epsilon) <= p.epsilon; }
+
+__device__ __forceinline__ long long gcdLL(long long a, long long b)
+{
+   while( b != 0 )
+   {
+      const long long t = a % b;
+      a = b;
+      b = t;
+   }
+   return a;
+}
+
+__device__ void rangedRow(const Mat& M, const Vec& V, const Params& P, int32_t v, int32_t row)
+{
+   const double2 sd = M.sides[row];
+   const int64_t s = M.rowptr[row], e = M.rowptr[row + 1];
+   if( e - s < 3 || isInf(P, -sd.x) || isInf(P, sd.y) )
+      return;
+   const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
+   uint8_t* inchk = V.rgflag + (size_t)v * (size_t)M.rowptr[M.nrows];
+   CtxT<false> C;
+   C.M = &M;
+   C.V = &V;
+   C.P = &P;
+   C.v = v;
+   C.round = V.st->round;
+   C.row = row;
+
+   // activity of the fixed columns, summed from the last nonzero to the first
+   double fixedact = 0.0;
+   int64_t nfixed = 0;
+   for( int64_t k = e - 1; k >= s; --k )
+   {
+      const double2 b = bnd[M.colidx[k]];
+      if( isEQ(P, b.x, b.y) )
+      {
+         fixedact += b.x * M.vals[k];
+         ++nfixed;
+      }
+   }
+   if( isHuge(P, fabs(fixedact)) )
+      return;
+   const double lhs = sd.x - fixedact, rhs = sd.y - fixedact;
+   const int64_t nunfixed = (e - s) - nfixed;
+
+   // partition: up to the first unfixed gcd-group column, every other unfixed column joins the
+   // check group; from there, a gcd-group column joins it when its gcd with the running gcd is 1
+   bool possiblegcd = true, gcdisone = true;
+   int64_t ninf = 0, ncont = 0, k = s;
+   for( ; k < e; ++k )
+   {
+      const double a = M.vals[k];
+      const double2 b = bnd[M.colidx[k]];
+      const bool fixed = isEQ(P, b.x, b.y);
+      const bool integ = (M.nzflags[k] & 1) != 0;
+      const bool group = integ && epsIntegral(P, a) && !isEQ(P, fabs(a), 1.0);
+      inchk[k] = 0;
+      if( fixed )
+         continue;
+      if( group )
+         break;
+      ncont += integ ? 0 : 1;
+      gcdisone = gcdisone && isEQ(P, fabs(a), 1.0);
+      possiblegcd = false;
+      inchk[k] = 1;
+      ++ninf;
+   }
+   if( k == e || ncont + 2 > nunfixed )
+      return;
+   long long gcd = (long long)(fabs(M.vals[k]) + P.feastol);
+   for( ; k < e; ++k )
+   {
+      const double a = M.vals[k];
+      const double2 b = bnd[M.colidx[k]];
+      const bool integ = (M.nzflags[k] & 1) != 0;
+      inchk[k] = 0;
+      if( isEQ(P, b.x, b.y) )
+         continue;
+      if( !integ || !epsIntegral(P, a) || isEQ(P, fabs(a), 1.0) )
+      {
+         ncont += integ ? 0 : 1;
+         gcdisone = gcdisone && isEQ(P, fabs(a), 1.0);
+         possiblegcd = false;
+         inchk[k] = 1;
+         ++ninf;
+      }
+      else
+      {
+         const long long g = gcdLL(gcd, (long long)(fabs(a) + P.feastol));
+         if( g == 1 )
+         {
+            inchk[k] = 1;
+            ++ninf;
+         }
+         else
+            gcd = g;
+      }
+   }
+   if( ninf == 0 )
+      return;
+
+   // activity range of the check group, from its last column to its first
+   double minact = 0.0, maxact = 0.0;
+   int64_t last = -1, first = -1;
+   for( k = e - 1; k >= s; --k )
+   {
+      if( !inchk[k] )
+         continue;
+      last = last < 0 ? k : last;
+      first = k;
+      const double a = M.vals[k];
+      const double2 b = bnd[M.colidx[k]];
+      bool mininv = false, maxinv = false;
+      if( isInf(P, -b.x) )
+         (a < 0.0 ? maxinv : mininv) = true;
+      else if( a < 0.0 )
+         maxact += a * b.x;
+      else
+         minact += a * b.x;
+      if( isInf(P, b.y) )
+         (a > 0.0 ? maxinv : mininv) = true;
+      else if( a > 0.0 )
+         maxact += a * b.y;
+      else
+         minact += a * b.y;
+      if( mininv || maxinv || isHuge(P, -minact) || isHuge(P, maxact) )
+         return;
+   }
+   const double g = (double)gcd;
+   if( !epsIntegral(P, (lhs - maxact) / g) && isGT(P, epsCeil(P, (lhs - maxact) / g) * g, rhs - minact) )
+   {
+      setCutoff(C);
+      return;
+   }
+   if( ncont != 0 )
+      return;
+   long long gcdinf = -1;
+   if( possiblegcd )
+   {
+      gcdinf = (long long)(fabs(M.vals[last]) + P.feastol);
+      for( k = last; k >= s && gcdinf >= 2; --k )
+         if( inchk[k] )
+            gcdinf = gcdLL(gcdinf, (long long)(fabs(M.vals[k]) + P.feastol));
+   }
+   else if( gcdisone )
+      gcdinf = 1;
+   if( gcdinf < 1 )
+      return;
+
+   // check-group values that the gcd part can complete into [lhs, rhs]: the first, second and
+   // last of them
+   const double step = (double)gcdinf;
+   int nsols = 0;
+   double minvalue = 0.0, maxvalue = 0.0;
+   int64_t steps = 0;
+   for( double value = epsCeil(P, minact - P.feastol); isLE(P, value, maxact); value += step )
+   {
+      if( ++steps > SAP_RANGED_STEPS )
+         return;
+      double value2 = value + g * epsCeil(P, (lhs - value) / g);
+      if( !isGE(P, value2, lhs) )
+         value2 += g;
+      if( isLE(P, value2, rhs) )
+      {
+         if( ++nsols == 3 )
+            break;
+         if( nsols == 1 )
+            minvalue = value;
+         maxvalue = value;
+      }
+   }
+   if( nsols == 3 )
+   {
+      for( double value = epsFloor(P, maxact + P.feastol); isGE(P, value, minact); value -= step )
+      {
+         if( ++steps > 2 * (int64_t)SAP_RANGED_STEPS )
+            return;
+         double value2 = value + g * epsFloor(P, (rhs - value) / g);
+         if( !isLE(P, value2, rhs) )
+            value2 -= g;
+         if( isGE(P, value2, lhs) )
+         {
+            maxvalue = value;
+            break;
+         }
+      }
+   }
+   if( nsols == 0 )
+   {
+      setCutoff(C);
+      return;
+   }
+   // the single unfixed column outside the check group, if there is exactly one
+   int64_t w = -1;
+   if( ninf == nunfixed - 1 )
+      for( k = s; k < e; ++k )
+         if( !inchk[k] && !isEQ(P, bnd[M.colidx[k]].x, bnd[M.colidx[k]].y) )
+         {
+            w = k;
+            break;
+         }
+   if( ninf == 1 )
+   {
+      const double a = M.vals[first];
+      const int32_t j = M.colidx[first];
+      if( nsols == 1 )
+      {
+         submitLb(C, j, maxvalue / a, true);
+         submitUb(C, j, maxvalue / a, true);
+      }
+      else
+      {
+         const double nlb = a < 0.0 ? maxvalue / a : minvalue / a;
+         const double nub = a < 0.0 ? minvalue / a : maxvalue / a;
+         if( nlb > bnd[j].x )
+            submitLb(C, j, nlb, true);
+         if( nub < bnd[j].y )
+            submitUb(C, j, nub, true);
+      }
+   }
+   else if( w >= 0 )
+   {
+      const double a = M.vals[w];
+      const int32_t j = M.colidx[w];
+      if( nsols == 1 )
+      {
+         const double fix = a < 0.0 ? epsFloor(P, (lhs - maxvalue) / a) : epsCeil(P, (lhs - maxvalue) / a);
+         submitLb(C, j, fix, true);
+         submitUb(C, j, fix, true);
+      }
+      else
+      {
+         const double nlb = a < 0.0 ? epsFloor(P, (rhs - minvalue) / a) : epsCeil(P, (lhs - maxvalue) / a);
+         const double nub = a < 0.0 ? epsFloor(P, (lhs - maxvalue) / a) : epsCeil(P, (rhs - minvalue) / a);
+         if( nlb > bnd[j].x )
+            submitLb(C, j, nlb, true);
+         if( nub < bnd[j].y )
+            submitUb(C, j, nub, true);
+      }
+   }
+}
+
+// ranged-row step of a round (before the row sweep, on the same start-of-round bounds): the
+// eligible rows marked for this round and not redundant
+__global__ __launch_bounds__(SAP_BLOCK) void k_ranged(Mat M, Vec V, Params P)
+{
+   Status* st = V.st;
+   if( st->done )
+      return;
+   const int32_t round = st->round;
+   if( V.dense )
+   {
+      const bool all = st->firstall && st->rlocal == 1;
+      const size_t n = (size_t)V.nvec * (size_t)M.nranged;
+      for( size_t i = blockIdx.x * (size_t)SAP_BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * SAP_BLOCK )
+      {
+         const int32_t v = (int32_t)(i / (size_t)M.nranged);
+         const int32_t row = M.rlist[i - (size_t)v * (size_t)M.nranged];
+         const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)row;
+         if( (all || V.rowmark[rflat] == round) && !V.redundant[rflat] && !V.vcutoff[unitOfRow(M, v, row)] )
+            rangedRow(M, V, P, v, row);
+      }
+   }
+   else
+   {
+      for( int b = 0; b < SAP_NBINS; ++b )
+      {
+         const uint32_t n = st->wlcount[round & 1][b];
+         const int2* wl = V.wl[round & 1][b];
+         for( uint32_t i = blockIdx.x * SAP_BLOCK + threadIdx.x; i < n; i += gridDim.x * SAP_BLOCK )
+         {
+            const int2 it = wl[i];
+            const size_t rflat = (size_t)it.y * (size_t)M.nrows + (size_t)it.x;
+            if( (M.rowflags[it.x] & 2) && !V.redundant[rflat] && !V.vcutoff[unitOfRow(M, it.y, it.x)] )
+               rangedRow(M, V, P, it.y, it.x);
+         }
+      }
+   }
+}
+
 __global__ void k_reason_reset(Vec V, int64_t ncols)
 {
    for( int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < ncols; j += (int64_t)gridDim.x * blockDim.x )
@@ -2643,6 +2944,8 @@ struct sap_matrix
    int64_t* colptr;
    int32_t* rowidx;
    int32_t* perm;        // device row position -> caller's row index
+   int32_t* rlist;       // device rows eligible for ranged-row propagation
+   int32_t nranged;
    int32_t binstart[SAP_NBINS + 1];
    int32_t ntiles;
    int64_t* tstart;
@@ -2829,6 +3132,9 @@ static Mat toMat(const sap_matrix* m)
    M.rowseg = m->nseg > 1 ? m->rowseg : NULL;
    M.colseg = m->nseg > 1 ? m->colseg : NULL;
    M.perm = m->perm;
+   M.rlist = m->rlist;
+   M.nranged = m->nranged;
+   M.pad2 = 0;
    return M;
 }
 
@@ -2991,6 +3297,7 @@ static void matrixRelease(sap_matrix* m)
    (void)hipFree(m->colptr);
    (void)hipFree(m->rowidx);
    (void)hipFree(m->perm);
+   (void)hipFree(m->rlist);
    (void)hipFree(m->tstart);
    (void)hipFree(m->tmask);
    (void)hipFree(m->trow0);
@@ -3147,7 +3454,10 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
             __atomic_fetch_add(&colptr[(size_t)c + 1], (int64_t)1, __ATOMIC_RELAXED);
          }
          sides[(size_t)r] = make_double2(lhs[i], rhs[i]);
-         rowflags[(size_t)r] = allbin ? 1 : 0;
+         // bit 1: eligible for ranged-row propagation (two sides of magnitude below SCIP's
+         // default infinity, at least 3 nonzeros)
+         const bool ranged = L >= 3 && fabs(lhs[i]) < 1e20 && fabs(rhs[i]) < 1e20;
+         rowflags[(size_t)r] = (uint8_t)((allbin ? 1 : 0) | (ranged ? 2 : 0));
          rowbin[(size_t)r] = obin[(size_t)i];
       }
    });
@@ -3217,7 +3527,13 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
    m->nrows = nrows;
    m->ncols = ncols;
    m->nnz = nnz;
-   m->hasforced = hasforced;
+   std::vector<int32_t> rlist;
+   for( int64_t r = 0; r < nrows; ++r )
+      if( rowflags[(size_t)r] & 2 )
+         rlist.push_back((int32_t)r);
+   m->nranged = (int32_t)rlist.size();
+   // ranged-row propagation submits forced candidates, so matrices with eligible rows keep them
+   m->hasforced = hasforced || m->nranged > 0;
    for( int b = 0; b <= SAP_NBINS; ++b )
       m->binstart[b] = binstart[b];
    int rc = SAP_OKAY;
@@ -3243,6 +3559,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
    MALLOC_OR_FAIL(m->colptr, ncols + 1);
    MALLOC_OR_FAIL(m->rowidx, nnz);
    MALLOC_OR_FAIL(m->perm, nrows);
+   MALLOC_OR_FAIL(m->rlist, rlist.size() + 1);
    MALLOC_OR_FAIL(m->tstart, tstart.size());
    MALLOC_OR_FAIL(m->tmask, tmask.size());
    MALLOC_OR_FAIL(m->trow0, trow0.size());
@@ -3271,6 +3588,8 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
       HIPCHK(hipMemcpyAsync(m->rowflags, rowflags.data(), (size_t)nrows, hipMemcpyHostToDevice, s));
       HIPCHK(hipMemcpyAsync(m->rowbin, rowbin.data(), (size_t)nrows, hipMemcpyHostToDevice, s));
       HIPCHK(hipMemcpyAsync(m->perm, perm.data(), sizeof(int32_t) * (size_t)nrows, hipMemcpyHostToDevice, s));
+      if( !rlist.empty() )
+         HIPCHK(hipMemcpyAsync(m->rlist, rlist.data(), sizeof(int32_t) * rlist.size(), hipMemcpyHostToDevice, s));
    }
    if( ncols > 0 )
       HIPCHK(hipMemcpyAsync(m->colflags, colflags.data(), (size_t)ncols, hipMemcpyHostToDevice, s));
@@ -3505,6 +3824,7 @@ static void boundsRelease(sap_bounds* b)
    (void)hipFree(V.rkey);
    (void)hipFree(V.rrow);
    (void)hipFree(V.hist);
+   (void)hipFree(V.rgflag);
    if( b->hstat )
       (void)hipHostFree(b->hstat);
    if( b->hpend )
@@ -3788,6 +4108,8 @@ static void enqueueRound(sap_bounds* b, const Mat& M, const Params& P, const Lay
    const int nsweep = L.blockstart[SAP_NKINDS];
    const unsigned colgrid = gridFor((size_t)b->V.nvec * (size_t)m->ncols, 4096);
    k_round_begin<<<1, SAP_STAT_SLOTS, 0, s>>>(b->V, P.maxrounds);
+   if( (P.flags & SAP_FLAG_RANGEDROWS) && m->nranged > 0 )
+      k_ranged<<<gridFor((size_t)b->V.nvec * (size_t)m->nranged, 4096), SAP_BLOCK, 0, s>>>(M, b->V, P);
    if( e0 != NULL )
       (void)hipEventRecord(e0, s);
    if( nsweep > 0 && b->V.reasons )
@@ -3805,9 +4127,19 @@ static void enqueueRound(sap_bounds* b, const Mat& M, const Params& P, const Lay
    k_mark<<<colgrid, SAP_BLOCK, 0, s>>>(M, b->V);
 }
 
+// ranged-row propagation keeps one check-group flag per (vector, nonzero); allocated on the
+// first call that asks for it, before any round graph that uses it is captured
+static int ensureRanged(sap_bounds* b, const Params& P)
+{
+   if( !(P.flags & SAP_FLAG_RANGEDROWS) || b->mat->nranged == 0 || b->V.rgflag != NULL )
+      return SAP_OKAY;
+   return devAlloc(&b->V.rgflag, (size_t)b->nvec * (size_t)b->mat->nnz + 1, &b->device_bytes);
+}
+
 // host-driven loop: one synchronisation per round
 static int runRoundsHost(sap_bounds* b, const Params& P, sap_stats* stats)
 {
+   RCCHK(ensureRanged(b, P));
    sap_matrix* m = b->mat;
    sap_context* c = m->ctx;
    hipStream_t s = c->stream;
@@ -3878,7 +4210,7 @@ static bool ensureGraphs(sap_bounds* b, const Params& P)
    GraphKey k;
    memset(&k, 0, sizeof(k));
    k.P = P;
-   k.P.flags = 0;   // flags steer the host side only
+   k.P.flags = P.flags & SAP_FLAG_RANGEDROWS;   // the other flags steer the host side only
    k.nvec = b->V.nvec;
    k.single = b->V.single;
    k.dense = b->V.dense;
@@ -3907,6 +4239,7 @@ static bool ensureGraphs(sap_bounds* b, const Params& P)
 // synchronisation per launch (per round with SAP_FLAG_HOSTLOOP)
 static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats)
 {
+   RCCHK(ensureRanged(b, P));
    if( (P.flags & SAP_FLAG_HOSTLOOP) || b->graphbroken || !ensureGraphs(b, P) )
       return runRoundsHost(b, P, stats);
    sap_context* c = b->mat->ctx;
@@ -4067,6 +4400,8 @@ int sap_propagate_reasons(sap_matrix* m, const sap_params* params, double* lb, d
       || (histcap > 0 && (hround == NULL || hcol == NULL || hbound == NULL || hval == NULL || hrow == NULL)) )
       return SAP_INVALIDDATA;
    RCCHK(checkParams(params));
+   if( params->flags & SAP_FLAG_RANGEDROWS )
+      return SAP_NOTIMPLEMENTED;   // ranged-row deductions have no residual-activity reason
    sap_context* c = m->ctx;
    HIPCHK(hipSetDevice(c->device));
    hipStream_t s = c->stream;
@@ -4883,6 +5218,7 @@ int sap_slots_launch(sap_bounds* b, const sap_params* params, int32_t rounds)
    HIPCHK(hipSetDevice(c->device));
    hipStream_t s = c->stream;
    const Params P = toParams(params);
+   RCCHK(ensureRanged(b, P));
    HIPCHK(hipEventRecord(b->evs[0], s));
    k_step_begin<<<1, SAP_STAT_SLOTS, 0, s>>>(b->V);
    HIPCHK(hipGetLastError());
