@@ -69,7 +69,7 @@ typedef struct sap_params
  * SCIP's constraints/linear/rangedrowpropagation, on by default outside presolving): ranged
  * rows and equations of at least 3 nonzeros with both sides below 1e20 in magnitude get the
  * gcd-based fixings, bounds and infeasibility test, as propagation applies it after the root
- * node (no artificial constraints).  Not available together with reasons. */
+ * node (no artificial constraints). */
 #define SAP_FLAG_RANGEDROWS 0x4
 
 /* per-call statistics */
@@ -153,24 +153,29 @@ int sap_propagate(sap_matrix* mat, const sap_params* params, double* lb, double*
  *   hround   the round (1, 2, ...) whose column step accepted it; the row sweep of that round
  *            read the bounds after all changes of the earlier rounds;
  *   hcol, hbound (0 lower, 1 upper), hval  the column and its new bound;
- *   hrow     the reason: the smallest row whose residual activity (the other columns' bounds
- *            at the start of that round) gave the winning candidate.  For a lower bound of a
- *            column with coefficient a > 0 it is the row's lhs against its maximal residual
- *            activity, with a < 0 the rhs against the minimal one; symmetric for upper bounds.
+ *   hrow     the reason: the smallest row that gave the winning candidate, from the bounds
+ *            at the start of that round;
+ *   hrule    (may be NULL) how: 0, the row's residual activity -- for a lower bound of a
+ *            column with coefficient a > 0 the row's lhs against its maximal residual
+ *            activity, with a < 0 the rhs against the minimal one, symmetric for upper bounds,
+ *            so the row's bounds of the other columns explain it (cons_linear.c
+ *            addConflictBounds); 1, the row's ranged-row gcd test (SAP_FLAG_RANGEDROWS), which
+ *            reads both bounds of all the row's columns, the changed one included (its bounds
+ *            before the change; cons_linear.c addConflictFixedVars).
  *            Applied in history order, each change is implied by its row and the bounds
- *            already applied, so the row's bounds of the other columns at the time of the
- *            change explain it (cons_linear.c addConflictBounds).
+ *            already applied.
  * *nhist is always set; if it exceeds histcap the history is not written and SAP_NOMEMORY is
  * returned (bounds and conflict are).  On a cutoff, conflict describes the infeasibility found
  * in the last round (the changes of the earlier rounds are in the history):
- *   SAP_CONFLICT_ROW     the smallest row whose activity bounds miss a side: side 0, the
- *                        minimal activity (lower bounds where a > 0, upper where a < 0)
- *                        exceeds rhs; side 1, the maximal activity is below lhs;
+ *   SAP_CONFLICT_ROW     the smallest row that proves infeasibility: side 0, its minimal
+ *                        activity (lower bounds where a > 0, upper where a < 0) exceeds rhs;
+ *                        side 1, its maximal activity is below lhs; side 2, its ranged-row gcd
+ *                        test fails (all bounds of its columns);
  *   SAP_CONFLICT_COLUMN  the smallest column whose candidate bounds crossed: rowlb implied its
  *                        lower bound (-1: the column's own lower bound), rowub its upper one
- *                        (-1: its own upper bound); the rows' residual activities are taken as
- *                        for history entries.
- * One instance (not a packed matrix); rows < 2^29, columns < 2^30. */
+ *                        (-1: its own upper bound), by the rules in bits 0 (rowlb) and 1 (rowub)
+ *                        of rules, as for history entries.
+ * One instance (not a packed matrix); rows < 2^28, columns < 2^30. */
 #define SAP_CONFLICT_NONE   0
 #define SAP_CONFLICT_ROW    1
 #define SAP_CONFLICT_COLUMN 2
@@ -183,10 +188,10 @@ typedef struct sap_conflict
    int32_t col;
    int32_t rowlb;
    int32_t rowub;
-   int32_t pad;
+   int32_t rules;
 } sap_conflict;
 int sap_propagate_reasons(sap_matrix* mat, const sap_params* params, double* lb, double* ub, int64_t histcap,
-   int64_t* nhist, int32_t* hround, int32_t* hcol, int32_t* hbound, double* hval, int32_t* hrow,
+   int64_t* nhist, int32_t* hround, int32_t* hcol, int32_t* hbound, double* hval, int32_t* hrow, int32_t* hrule,
    sap_conflict* conflict, sap_stats* stats);
 
 /* Device-resident bound vector(s), for callers that keep bounds in HBM across calls. */

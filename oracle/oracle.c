@@ -608,6 +608,7 @@ typedef struct
    int32_t* flbrow;
    int32_t* fubrow;
    int32_t currow;
+   int32_t currule;   /* 0: residual activity, 1: ranged-row test; rows are kept as row * 2 + rule */
    int rowside;
 } jstate;
 
@@ -615,14 +616,15 @@ static inline void submitLb(jstate* js, int32_t j, double v, int force)
 {
    double* t = force ? &js->flb[j] : &js->clb[j];
    int32_t* r = js->clbrow == NULL ? NULL : (force ? &js->flbrow[j] : &js->clbrow[j]);
+   int32_t key = js->currow * 2 + js->currule;
    if( dkey(v) > dkey(*t) )
    {
       *t = v;
       if( r )
-         *r = js->currow;
+         *r = key;
    }
-   else if( r && dkey(v) == dkey(*t) && js->currow < *r )
-      *r = js->currow;
+   else if( r && dkey(v) == dkey(*t) && key < *r )
+      *r = key;
    js->touched[j] = 1;
 }
 
@@ -630,14 +632,15 @@ static inline void submitUb(jstate* js, int32_t j, double v, int force)
 {
    double* t = force ? &js->fub[j] : &js->cub[j];
    int32_t* r = js->cubrow == NULL ? NULL : (force ? &js->fubrow[j] : &js->cubrow[j]);
+   int32_t key = js->currow * 2 + js->currule;
    if( dkey(v) < dkey(*t) )
    {
       *t = v;
       if( r )
-         *r = js->currow;
+         *r = key;
    }
-   else if( r && dkey(v) == dkey(*t) && js->currow < *r )
-      *r = js->currow;
+   else if( r && dkey(v) == dkey(*t) && key < *r )
+      *r = key;
    js->touched[j] = 1;
 }
 
@@ -1006,7 +1009,7 @@ static void jRanged(jstate* js, int64_t i)
    if( !epsIntegral(p, (lhs - maxact) / g) && isGT(p, epsCeil(p, (lhs - maxact) / g) * g, rhs - minact) )
    {
       js->cutoff = 1;
-      js->rowside = 0;
+      js->rowside = 2;
       goto TERMINATE;
    }
    if( ncont != 0 )
@@ -1061,7 +1064,7 @@ static void jRanged(jstate* js, int64_t i)
    if( nsols == 0 )
    {
       js->cutoff = 1;
-      js->rowside = 0;
+      js->rowside = 2;
       goto TERMINATE;
    }
    if( ninf == nunfixed - 1 )
@@ -1157,7 +1160,11 @@ static int jRow(jstate* js, int64_t i, double* xmin, double* xmax, dd* scratch)
    }
    /* propagateCons: ranged rows after the tightening (cons_linear.c:7694-7700) */
    if( !js->cutoff && p->rangedrows )
+   {
+      js->currule = 1;
       jRanged(js, i);
+      js->currule = 0;
+   }
    if( js->cutoff )
       return 0;
 
@@ -1283,10 +1290,12 @@ typedef struct
    int32_t* bound;
    double* val;
    int32_t* row;
+   int32_t* rule;
    orc_conflict* conflict;
 } jreasons;
 
-static void histAdd(jreasons* R, int32_t round, int32_t j, int32_t bound, double val, int32_t row)
+/* rowkey: row * 2 + rule */
+static void histAdd(jreasons* R, int32_t round, int32_t j, int32_t bound, double val, int32_t rowkey)
 {
    if( R->n < R->cap )
    {
@@ -1294,7 +1303,9 @@ static void histAdd(jreasons* R, int32_t round, int32_t j, int32_t bound, double
       R->col[R->n] = j;
       R->bound[R->n] = bound;
       R->val[R->n] = val;
-      R->row[R->n] = row;
+      R->row[R->n] = rowkey >> 1;
+      if( R->rule )
+         R->rule[R->n] = rowkey & 1;
    }
    R->n++;
 }
@@ -1405,8 +1416,9 @@ static int jacobi(const orc_matrix* m, const orc_params* p, double* lb, double* 
                /* the first (smallest) column whose bounds crossed */
                R->conflict->kind = ORC_CONFLICT_COLUMN;
                R->conflict->col = (int32_t)j;
-               R->conflict->rowlb = rl;
-               R->conflict->rowub = ru;
+               R->conflict->rowlb = rl < 0 ? -1 : rl >> 1;
+               R->conflict->rowub = ru < 0 ? -1 : ru >> 1;
+               R->conflict->rules = (rl < 0 ? 0 : (rl & 1)) | (ru < 0 ? 0 : (ru & 1) << 1);
                R->conflict->round = round;
             }
             else if( colchg[j] )
@@ -1468,7 +1480,7 @@ int orc_propagate_jacobi(const orc_matrix* m, const orc_params* p, double* lb, d
 }
 
 int orc_propagate_jacobi_reasons(const orc_matrix* m, const orc_params* p, double* lb, double* ub, int64_t histcap,
-   int64_t* nhist, int32_t* hround, int32_t* hcol, int32_t* hbound, double* hval, int32_t* hrow,
+   int64_t* nhist, int32_t* hround, int32_t* hcol, int32_t* hbound, double* hval, int32_t* hrow, int32_t* hrule,
    orc_conflict* conflict, orc_result* res)
 {
    jreasons R;
@@ -1480,6 +1492,7 @@ int orc_propagate_jacobi_reasons(const orc_matrix* m, const orc_params* p, doubl
    R.bound = hbound;
    R.val = hval;
    R.row = hrow;
+   R.rule = hrule;
    R.conflict = conflict;
    rc = jacobi(m, p, lb, ub, 0, res, &R);
    *nhist = R.n;

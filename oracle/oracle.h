@@ -93,10 +93,11 @@ int orc_propagate_scip(const orc_matrix* m, const orc_params* p, double* lb, dou
 
 /* the Jacobi twin with reasons (restates the reason recording of sap_propagate_reasons):
  * history entries (round, column, bound 0 lower / 1 upper, value, the smallest row that
- * submitted the winning candidate) in round, column, bound order, and the certificate of a
- * cutoff: the smallest infeasible row with its side, or the smallest column whose bounds
- * crossed with the rows of its two sides (-1: the column's own bound).  *nhist counts all
- * entries; at most histcap are written. */
+ * submitted the winning candidate, and its rule: 0 residual activity, 1 ranged-row test) in
+ * round, column, bound order, and the certificate of a cutoff: the smallest infeasible row
+ * with its side (0 minimal activity, 1 maximal activity, 2 ranged-row test), or the smallest
+ * column whose bounds crossed with the rows of its two sides (-1: the column's own bound).
+ * *nhist counts all entries; at most histcap are written. */
 #define ORC_CONFLICT_NONE   0
 #define ORC_CONFLICT_ROW    1
 #define ORC_CONFLICT_COLUMN 2
@@ -109,10 +110,10 @@ typedef struct orc_conflict
    int32_t col;
    int32_t rowlb;
    int32_t rowub;
-   int32_t pad;
+   int32_t rules;   /* bit 0: rowlb's rule, bit 1: rowub's (1: ranged-row test) */
 } orc_conflict;
 int orc_propagate_jacobi_reasons(const orc_matrix* m, const orc_params* p, double* lb, double* ub, int64_t histcap,
-   int64_t* nhist, int32_t* hround, int32_t* hcol, int32_t* hbound, double* hval, int32_t* hrow,
+   int64_t* nhist, int32_t* hround, int32_t* hcol, int32_t* hbound, double* hval, int32_t* hrow, int32_t* hrule,
    orc_conflict* conflict, orc_result* res);
 
 /* double-double primitives restated from src/scip/dbldblarith.h, exported for the

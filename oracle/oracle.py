@@ -32,10 +32,10 @@ class orc_matrix(C.Structure):
 
 
 class orc_conflict(C.Structure):
-    _fields_ = [(n, C.c_int32) for n in ("kind", "round", "row", "side", "col", "rowlb", "rowub", "pad")]
+    _fields_ = [(n, C.c_int32) for n in ("kind", "round", "row", "side", "col", "rowlb", "rowub", "rules")]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class orc_result(C.Structure):
@@ -77,7 +77,7 @@ def lib():
         L.orc_canonical_sum.argtypes = [_DP, C.c_int64, _DP, _DP]
         _I32P = C.POINTER(C.c_int32)
         L.orc_propagate_jacobi_reasons.argtypes = [C.POINTER(orc_matrix), C.POINTER(orc_params), _DP, _DP, C.c_int64,
-                                                   C.POINTER(C.c_int64), _I32P, _I32P, _I32P, _DP, _I32P,
+                                                   C.POINTER(C.c_int64), _I32P, _I32P, _I32P, _DP, _I32P, _I32P,
                                                    C.POINTER(orc_conflict), C.POINTER(orc_result)]
         _lib = L
     return _lib
@@ -157,7 +157,7 @@ def propagate_reasons(om: OMatrix, lb, ub, prm: orc_params | None = None):
     cap = max(1024, 2 * int(om.m.ncols))
     while True:
         l, u = lb.copy(), ub.copy()
-        hr, hc, hb, hw = (np.zeros(cap, dtype=np.int32) for _ in range(4))
+        hr, hc, hb, hw, hu = (np.zeros(cap, dtype=np.int32) for _ in range(5))
         hv = np.zeros(cap)
         n = C.c_int64(0)
         cf = orc_conflict()
@@ -165,13 +165,13 @@ def propagate_reasons(om: OMatrix, lb, ub, prm: orc_params | None = None):
         lib().orc_propagate_jacobi_reasons(C.byref(om.m), C.byref(prm or params()), l.ctypes.data_as(_DP),
                                            u.ctypes.data_as(_DP), cap, C.byref(n), hr.ctypes.data_as(I32),
                                            hc.ctypes.data_as(I32), hb.ctypes.data_as(I32), hv.ctypes.data_as(_DP),
-                                           hw.ctypes.data_as(I32), C.byref(cf), C.byref(r))
+                                           hw.ctypes.data_as(I32), hu.ctypes.data_as(I32), C.byref(cf), C.byref(r))
         if n.value > cap:
             cap = n.value
             continue
         k = n.value
         return dict(lb=l, ub=u, result=r.as_dict(), conflict=cf.as_dict(), nhist=k,
-                    round=hr[:k], col=hc[:k], bound=hb[:k], val=hv[:k], row=hw[:k])
+                    round=hr[:k], col=hc[:k], bound=hb[:k], val=hv[:k], row=hw[:k], rule=hu[:k])
 
 
 def propagate_scip(om: OMatrix, lb, ub, prm: orc_params | None = None):

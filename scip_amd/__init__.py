@@ -85,10 +85,10 @@ class sap_tuning(C.Structure):
 
 
 class sap_conflict(C.Structure):
-    _fields_ = [(n, C.c_int32) for n in ("kind", "round", "row", "side", "col", "rowlb", "rowub", "pad")]
+    _fields_ = [(n, C.c_int32) for n in ("kind", "round", "row", "side", "col", "rowlb", "rowub", "rules")]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class sap_stats(C.Structure):
@@ -182,7 +182,7 @@ EXPORTS = {
     "sap_propagate_nodes": (C.c_int, [_P, C.POINTER(sap_params), C.c_int32, _P, C.c_int32, _I64P, _I32P, _DP, _DP,
                                       _I32P, _I32P, C.c_int64, _I64P, _I32P, _DP, _DP, C.POINTER(sap_stats)]),
     "sap_propagate_reasons": (C.c_int, [_P, C.POINTER(sap_params), _DP, _DP, C.c_int64, _I64P, _I32P, _I32P, _I32P,
-                                        _DP, _I32P, C.POINTER(sap_conflict), C.POINTER(sap_stats)]),
+                                        _DP, _I32P, _I32P, C.POINTER(sap_conflict), C.POINTER(sap_stats)]),
     "sap_probe": (C.c_int, [_P, C.POINTER(sap_params), C.c_int32, _I32P, _P, C.c_int32, _I32P, _I32P, C.c_int64,
                             _I64P, _I32P, _DP, _DP, _DP, _DP, C.POINTER(sap_stats)]),
     "sap_matrix_create_packed": (C.c_int, [_P, C.c_int32, _I64P, _I64P, C.c_int64, C.c_int64, _I64P, _I32P, _DP,
@@ -452,19 +452,20 @@ class Matrix:
         while True:
             lb, ub = lb0.copy(), ub0.copy()
             n = C.c_int64(0)
-            hr, hc, hb, hw = (np.zeros(max(cap, 1), dtype=np.int32) for _ in range(4))
+            hr, hc, hb, hw, hu = (np.zeros(max(cap, 1), dtype=np.int32) for _ in range(5))
             hv = np.zeros(max(cap, 1))
             cf = sap_conflict()
             st = sap_stats()
             rc = lib().sap_propagate_reasons(self.h, C.byref(p), _dp(lb), _dp(ub), cap, C.byref(n), _i32p(hr),
-                                             _i32p(hc), _i32p(hb), _dp(hv), _i32p(hw), C.byref(cf), C.byref(st))
+                                             _i32p(hc), _i32p(hb), _dp(hv), _i32p(hw), _i32p(hu), C.byref(cf),
+                                             C.byref(st))
             if rc == SAP_NOMEMORY and histcap is None and n.value > cap:
                 cap = n.value   # again with room for the whole history (same results)
                 continue
             _chk(rc, "sap_propagate_reasons")
             k = n.value
             return dict(lb=lb, ub=ub, stats=st.as_dict(), conflict=cf.as_dict(), nhist=k,
-                        round=hr[:k], col=hc[:k], bound=hb[:k], val=hv[:k], row=hw[:k])
+                        round=hr[:k], col=hc[:k], bound=hb[:k], val=hv[:k], row=hw[:k], rule=hu[:k])
 
     def row_activities(self, lb=None, ub=None, params: sap_params | None = None):
         nr = self.inst.nrows

@@ -79,8 +79,8 @@ struct Status
    unsigned long long sweep_count;
    int32_t cont;        // continuous batching: no global round cap (per-vector caps apply)
    int32_t pad3;
-   // reason recording (one vector): infeasibility certificates (smallest caller row * 2 + side,
-   // smallest column), candidate-log and history lengths
+   // reason recording (one vector): infeasibility certificates (smallest caller row * 8 + the
+   // row's test code, see RowDecision; smallest column), candidate-log and history lengths
    int32_t cutrow;
    int32_t cutcol;
    uint32_t clogn;
@@ -163,7 +163,7 @@ struct Vec
    struct CandRec* clog;
    uint8_t* rflag;          // per column: RF_* bits of this round's outcome
    long long* rkey;         // 2 per column: winning key of the lb / ub change
-   int32_t* rrow;           // 2 per column: smallest caller row that submitted it
+   int32_t* rrow;           // 2 per column: smallest caller row * 2 + rule that submitted it
    struct HistRec* hist;
    uint32_t histcap;
    uint32_t pad5;
@@ -174,7 +174,7 @@ struct CandRec
 {
    long long key;
    int32_t col;
-   int32_t rowside;         // caller row * 4 + side (0 lb, 1 ub) * 2 + forced
+   int32_t rowside;         // caller row * 8 + rule * 4 + side (0 lb, 1 ub) * 2 + forced
 };
 
 struct HistRec
@@ -462,6 +462,7 @@ __device__ __forceinline__ void waveAppendRow(Status* st, int2* const* wl, int n
 template <bool R>
 struct CtxT
 {
+   int32_t rule = 0;   // reason rule of its candidates: 0 residual activity, 1 ranged row
    const Mat* M;
    const Vec* V;
    const Params* P;
@@ -508,7 +509,7 @@ __device__ __forceinline__ void logCand(const CtxT<R>& C, int32_t j, long long k
       CandRec r;
       r.key = k;
       r.col = j;
-      r.rowside = C.M->perm[C.row] * 4 + side * 2 + (force ? 1 : 0);
+      r.rowside = C.M->perm[C.row] * 8 + C.rule * 4 + side * 2 + (force ? 1 : 0);
       C.V->clog[at] = r;
    }
    else if( st->clogovf == 0 )
@@ -766,7 +767,9 @@ struct RowDecision
    bool easy;
    bool force;
    bool cut;         // infeasible (propagateCons / tightenVarBoundsEasy tests)
-   bool cutmin;      // ... by the minimal activity against rhs (else the maximal against lhs)
+   int cutcode;      // ... found by, in the row's own order of tests: 0 / 1 the tightening loop's
+                     // minimal-activity / maximal-activity test, 3 / 4 the final test's (2 is
+                     // the ranged-row test, which runs between them)
    bool redundant;   // locally redundant
 };
 
@@ -848,9 +851,9 @@ __device__ __forceinline__ RowDecision decideRow(const Params& p, const RowState
    const bool emax = d.tighten && d.easy && !isInf(p, -lhs) && isFeasLT(p, ddVal(rs.maxact), lhs);
    const bool fmin = isFeasGT(p, minT, rhs);
    d.cut = emin || emax || fmin || isFeasLT(p, maxT, lhs);
-   // the side as the row's own order of tests meets it: the tightening loop tests rhs before
-   // lhs at every nonzero, then the final test checks the minimal activity first
-   d.cutmin = (emin || emax) ? emin : fmin;
+   // the tightening loop tests rhs before lhs at every nonzero; the final test checks the
+   // minimal activity first
+   d.cutcode = (emin || emax) ? (emin ? 0 : 1) : (fmin ? 3 : 4);
    d.redundant = !d.cut && isGE(p, minT, lhs) && isLE(p, maxT, rhs);
    return d;
 }
@@ -861,7 +864,7 @@ __device__ __forceinline__ void rowOutcome(const CtxT<R>& C, const RowDecision& 
 {
    if constexpr( R )
       if( d.cut )
-         atomicMin(&C.V->st->cutrow, C.M->perm[C.row] * 2 + (d.cutmin ? 0 : 1));
+         atomicMin(&C.V->st->cutrow, C.M->perm[C.row] * 8 + d.cutcode);
    if( d.cut )
       setCutoff(C);
    else if( d.redundant )
@@ -1782,7 +1785,7 @@ __global__ __launch_bounds__(SAP_BLOCK, 4) void k_sweep(Mat M, Vec V, Params P, 
          {
             const double2 sd = M.sides[r];
             if( (isFeasGT(P, 0.0, sd.y) || isFeasLT(P, 0.0, sd.x)) && R )
-               atomicMin(&st->cutrow, M.perm[r] * 2 + (isFeasGT(P, 0.0, sd.y) ? 0 : 1));
+               atomicMin(&st->cutrow, M.perm[r] * 8 + (isFeasGT(P, 0.0, sd.y) ? 3 : 4));
             if( isFeasGT(P, 0.0, sd.y) || isFeasLT(P, 0.0, sd.x) )
                for( int32_t v = 0; v < V.nvec; ++v )
                {
@@ -2108,7 +2111,7 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_reason(Vec V)
       const int have = side ? RF_UB : RF_LB;
       const int force = side ? RF_UBFORCE : RF_LBFORCE;
       if( (f & have) && ((f & force) != 0) == (forced != 0) && V.rkey[2 * (size_t)r.col + side] == r.key )
-         atomicMin(&V.rrow[2 * (size_t)r.col + side], r.rowside >> 2);
+         atomicMin(&V.rrow[2 * (size_t)r.col + side], (r.rowside >> 3) * 2 + ((r.rowside >> 2) & 1));
    }
 }
 
@@ -2160,6 +2163,14 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_history(Mat M, Vec V)
 // ------------------------------------------------------------------------------------------
 #define SAP_RANGED_STEPS (1 << 20)
 
+template <bool R>
+__device__ __forceinline__ void rangedCut(const CtxT<R>& C)
+{
+   if constexpr( R )
+      atomicMin(&C.V->st->cutrow, C.M->perm[C.row] * 8 + 2);
+   setCutoff(C);
+}
+
 __device__ __forceinline__ double epsFloor(const Params& p, double x) { return floor(x + p.epsilon); }
 __device__ __forceinline__ double epsCeil(const Params& p, double x) { return ceil(x - p.epsilon); }
 __device__ __forceinline__ bool epsIntegral(const Params& p, double x) { return x - floor(x + p.epsilon) <= p.epsilon; }
@@ -2175,6 +2186,7 @@ __device__ __forceinline__ long long gcdLL(long long a, long long b)
    return a;
 }
 
+template <bool R>
 __device__ void rangedRow(const Mat& M, const Vec& V, const Params& P, int32_t v, int32_t row)
 {
    const double2 sd = M.sides[row];
@@ -2183,7 +2195,8 @@ __device__ void rangedRow(const Mat& M, const Vec& V, const Params& P, int32_t v
       return;
    const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
    uint8_t* inchk = V.rgflag + (size_t)v * (size_t)M.rowptr[M.nrows];
-   CtxT<false> C;
+   CtxT<R> C;
+   C.rule = 1;
    C.M = &M;
    C.V = &V;
    C.P = &P;
@@ -2294,7 +2307,7 @@ __device__ void rangedRow(const Mat& M, const Vec& V, const Params& P, int32_t v
    const double g = (double)gcd;
    if( !epsIntegral(P, (lhs - maxact) / g) && isGT(P, epsCeil(P, (lhs - maxact) / g) * g, rhs - minact) )
    {
-      setCutoff(C);
+      rangedCut(C);
       return;
    }
    if( ncont != 0 )
@@ -2352,7 +2365,7 @@ __device__ void rangedRow(const Mat& M, const Vec& V, const Params& P, int32_t v
    }
    if( nsols == 0 )
    {
-      setCutoff(C);
+      rangedCut(C);
       return;
    }
    // the single unfixed column outside the check group, if there is exactly one
@@ -2407,6 +2420,7 @@ __device__ void rangedRow(const Mat& M, const Vec& V, const Params& P, int32_t v
 
 // ranged-row step of a round (before the row sweep, on the same start-of-round bounds): the
 // eligible rows marked for this round and not redundant
+template <bool R>
 __global__ __launch_bounds__(SAP_BLOCK) void k_ranged(Mat M, Vec V, Params P)
 {
    Status* st = V.st;
@@ -2422,8 +2436,8 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_ranged(Mat M, Vec V, Params P)
          const int32_t v = (int32_t)(i / (size_t)M.nranged);
          const int32_t row = M.rlist[i - (size_t)v * (size_t)M.nranged];
          const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)row;
-         if( (all || V.rowmark[rflat] == round) && !V.redundant[rflat] && !V.vcutoff[unitOfRow(M, v, row)] )
-            rangedRow(M, V, P, v, row);
+         if( (all || V.rowmark[rflat] == round) && !V.redundant[rflat] && (R || !V.vcutoff[unitOfRow(M, v, row)]) )
+            rangedRow<R>(M, V, P, v, row);
       }
    }
    else
@@ -2436,8 +2450,8 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_ranged(Mat M, Vec V, Params P)
          {
             const int2 it = wl[i];
             const size_t rflat = (size_t)it.y * (size_t)M.nrows + (size_t)it.x;
-            if( (M.rowflags[it.x] & 2) && !V.redundant[rflat] && !V.vcutoff[unitOfRow(M, it.y, it.x)] )
-               rangedRow(M, V, P, it.y, it.x);
+            if( (M.rowflags[it.x] & 2) && !V.redundant[rflat] && (R || !V.vcutoff[unitOfRow(M, it.y, it.x)]) )
+               rangedRow<R>(M, V, P, it.y, it.x);
          }
       }
    }
@@ -4108,8 +4122,10 @@ static void enqueueRound(sap_bounds* b, const Mat& M, const Params& P, const Lay
    const int nsweep = L.blockstart[SAP_NKINDS];
    const unsigned colgrid = gridFor((size_t)b->V.nvec * (size_t)m->ncols, 4096);
    k_round_begin<<<1, SAP_STAT_SLOTS, 0, s>>>(b->V, P.maxrounds);
-   if( (P.flags & SAP_FLAG_RANGEDROWS) && m->nranged > 0 )
-      k_ranged<<<gridFor((size_t)b->V.nvec * (size_t)m->nranged, 4096), SAP_BLOCK, 0, s>>>(M, b->V, P);
+   if( (P.flags & SAP_FLAG_RANGEDROWS) && m->nranged > 0 && b->V.reasons )
+      k_ranged<true><<<gridFor((size_t)b->V.nvec * (size_t)m->nranged, 4096), SAP_BLOCK, 0, s>>>(M, b->V, P);
+   else if( (P.flags & SAP_FLAG_RANGEDROWS) && m->nranged > 0 )
+      k_ranged<false><<<gridFor((size_t)b->V.nvec * (size_t)m->nranged, 4096), SAP_BLOCK, 0, s>>>(M, b->V, P);
    if( e0 != NULL )
       (void)hipEventRecord(e0, s);
    if( nsweep > 0 && b->V.reasons )
@@ -4367,8 +4383,8 @@ static void dropGraphs(sap_bounds* b)
 // candidate per nonzero), per-column outcome, winning keys and rows, and the history
 static int createReasons(sap_matrix* m)
 {
-   if( m->nrows >= ((int64_t)1 << 29) || m->ncols >= ((int64_t)1 << 30) || 2 * m->nnz + 64 > (int64_t)UINT32_MAX )
-      return SAP_NOTIMPLEMENTED;   // row * 4 and column * 2 must fit the record fields
+   if( m->nrows >= ((int64_t)1 << 28) || m->ncols >= ((int64_t)1 << 30) || 2 * m->nnz + 64 > (int64_t)UINT32_MAX )
+      return SAP_NOTIMPLEMENTED;   // row * 8 and column * 2 must fit the record fields
    sap_bounds* b = NULL;
    RCCHK(sap_bounds_create_mode(m, 1, SAP_MODE_DENSE, &b));
    b->internal = 1;
@@ -4393,15 +4409,13 @@ static bool histLess(const HistRec& a, const HistRec& b)
 }
 
 int sap_propagate_reasons(sap_matrix* m, const sap_params* params, double* lb, double* ub, int64_t histcap,
-   int64_t* nhist, int32_t* hround, int32_t* hcol, int32_t* hbound, double* hval, int32_t* hrow,
+   int64_t* nhist, int32_t* hround, int32_t* hcol, int32_t* hbound, double* hval, int32_t* hrow, int32_t* hrule,
    sap_conflict* conflict, sap_stats* stats)
 {
    if( m == NULL || lb == NULL || ub == NULL || nhist == NULL || histcap < 0 || m->nseg != 1
       || (histcap > 0 && (hround == NULL || hcol == NULL || hbound == NULL || hval == NULL || hrow == NULL)) )
       return SAP_INVALIDDATA;
    RCCHK(checkParams(params));
-   if( params->flags & SAP_FLAG_RANGEDROWS )
-      return SAP_NOTIMPLEMENTED;   // ranged-row deductions have no residual-activity reason
    sap_context* c = m->ctx;
    HIPCHK(hipSetDevice(c->device));
    hipStream_t s = c->stream;
@@ -4451,8 +4465,9 @@ int sap_propagate_reasons(sap_matrix* m, const sap_params* params, double* lb, d
       if( cut && h.cutrow != INT32_MAX )
       {
          conflict->kind = SAP_CONFLICT_ROW;
-         conflict->row = h.cutrow >> 1;
-         conflict->side = h.cutrow & 1;
+         const int code = h.cutrow & 7;
+         conflict->row = h.cutrow >> 3;
+         conflict->side = code == 2 ? 2 : (code == 0 || code == 3) ? 0 : 1;
       }
       else if( cut && h.cutcol != INT32_MAX )
       {
@@ -4463,8 +4478,9 @@ int sap_propagate_reasons(sap_matrix* m, const sap_params* params, double* lb, d
          HIPCHK(hipStreamSynchronize(s));
          conflict->kind = SAP_CONFLICT_COLUMN;
          conflict->col = h.cutcol;
-         conflict->rowlb = (f & RF_LB) ? rr[0] : -1;
-         conflict->rowub = (f & RF_UB) ? rr[1] : -1;
+         conflict->rowlb = (f & RF_LB) ? rr[0] >> 1 : -1;
+         conflict->rowub = (f & RF_UB) ? rr[1] >> 1 : -1;
+         conflict->rules = ((f & RF_LB) ? (rr[0] & 1) : 0) | ((f & RF_UB) ? (rr[1] & 1) << 1 : 0);
       }
    }
    if( !cut )
@@ -4484,7 +4500,9 @@ int sap_propagate_reasons(sap_matrix* m, const sap_params* params, double* lb, d
          hcol[q] = r.colside >> 1;
          hbound[q] = r.colside & 1;
          hval[q] = r.val;
-         hrow[q] = r.row;
+         hrow[q] = r.row >> 1;
+         if( hrule )
+            hrule[q] = r.row & 1;
       }
    }
    return SAP_OKAY;

@@ -134,6 +134,23 @@ SCIP_RETCODE getEngineParams(
    return SCIP_OKAY;
 }
 
+/** SCIP's constraints/linear/rangedrowpropagation (default TRUE) switches the device's
+ *  ranged-row propagation on, as it does cons_linear's outside presolving (cons_linear.c:16164) */
+static
+SCIP_RETCODE rangedRowFlag(
+   SCIP*                 scip,               /**< SCIP data structure */
+   sap_params*           params              /**< parameters to complete */
+   )
+{
+   SCIP_Bool ranged = TRUE;
+
+   if( SCIPgetBoolParam(scip, "constraints/linear/rangedrowpropagation", &ranged) != SCIP_OKAY )
+      ranged = TRUE;
+   if( ranged )
+      params->flags |= SAP_FLAG_RANGEDROWS;
+   return SCIP_OKAY;
+}
+
 /** releases the device matrix and the column map */
 static
 SCIP_RETCODE freeMatrix(
@@ -338,6 +355,10 @@ SCIP_RETCODE propagateOnDevice(
       ub[j] = local ? SCIPvarGetUbLocal(var) : SCIPvarGetUbGlobal(var);
    }
    SCIP_CALL( getEngineParams(scip, propdata, &params) );
+   if( local )
+   {
+      SCIP_CALL( rangedRowFlag(scip, &params) );
+   }
    memset(&stats, 0, sizeof(stats));
    SCIP_CALL( engineCall(sap_propagate(propdata->mat, &params, lb, ub, &stats), "sap_propagate") );
    *cutoff = (stats.result == SAP_CUTOFF);
@@ -514,6 +535,43 @@ SCIP_RETCODE addRowBounds(
    return SCIP_OKAY;
 }
 
+/** adds both bounds of every column of row at the time of bdchgidx (NULL: now): the reason of a
+ *  ranged-row deduction, which reads the fixed columns' values and the other columns' ranges
+ *  (cons_linear.c addConflictFixedVars :4981 and analyzeConflictRangedRow :5657 add the fixed
+ *  columns and the columns whose bounds are not global; all bounds are a superset) */
+static
+SCIP_RETCODE addRowAllBounds(
+   SCIP*                 scip,               /**< SCIP data structure */
+   SCIP_PROPDATA*        propdata,           /**< propagator data */
+   int                   row,                /**< row of the device matrix */
+   SCIP_BDCHGIDX*        bdchgidx            /**< time of the bounds, or NULL for the current ones */
+   )
+{
+   int64_t k;
+
+   for( k = propdata->rowptr[row]; k < propdata->rowptr[row + 1]; ++k )
+   {
+      SCIP_VAR* var = propdata->vars[propdata->colidx[k]];
+
+      SCIP_CALL( SCIPaddConflictLb(scip, var, bdchgidx) );
+      SCIP_CALL( SCIPaddConflictUb(scip, var, bdchgidx) );
+   }
+   return SCIP_OKAY;
+}
+
+/** the reason of bound boundtype of column j derived from row by rule (0: residual activity,
+ *  1: ranged-row test), at the time of bdchgidx */
+static
+SCIP_RETCODE addReason(
+   SCIP*                 scip,               /**< SCIP data structure */
+   SCIP_PROPDATA*        propdata,           /**< propagator data */
+   int                   row,                /**< row of the device matrix */
+   int                   rule,               /**< reason rule */
+   int                   j,                  /**< the column whose bound was derived */
+   SCIP_BOUNDTYPE        boundtype,          /**< the bound derived */
+   SCIP_BDCHGIDX*        bdchgidx            /**< time of the bounds, or NULL for the current ones */
+   );
+
 /** coefficient of column j in row (0.0 if absent) */
 static
 double rowCoef(
@@ -530,9 +588,32 @@ double rowCoef(
    return 0.0;
 }
 
-/** PROPEXEC with reasons: the device's changes in round order, each with its reason row as
- *  inferinfo; on a cutoff, the changes of the earlier rounds, then conflict analysis on the
- *  certificate */
+static
+SCIP_RETCODE addReason(
+   SCIP*                 scip,               /**< SCIP data structure */
+   SCIP_PROPDATA*        propdata,           /**< propagator data */
+   int                   row,                /**< row of the device matrix */
+   int                   rule,               /**< reason rule */
+   int                   j,                  /**< the column whose bound was derived */
+   SCIP_BOUNDTYPE        boundtype,          /**< the bound derived */
+   SCIP_BDCHGIDX*        bdchgidx            /**< time of the bounds, or NULL for the current ones */
+   )
+{
+   double a;
+
+   if( rule == 1 )
+      return addRowAllBounds(scip, propdata, row, bdchgidx);
+   /* a lower bound of a column with a positive coefficient came from lhs and the maximal
+    * residual activity, with a negative one from rhs and the minimal residual; symmetric for
+    * upper bounds */
+   a = rowCoef(propdata, row, j);
+   SCIP_CALL( addRowBounds(scip, propdata, row, j, (boundtype == SCIP_BOUNDTYPE_LOWER) == (a > 0.0), bdchgidx) );
+   return SCIP_OKAY;
+}
+
+/** PROPEXEC with reasons: the device's changes in round order, each with its reason row and
+ *  rule as inferinfo (row * 2 + rule); on a cutoff, the changes of the earlier rounds, then
+ *  conflict analysis on the certificate */
 static
 SCIP_RETCODE execWithReasons(
    SCIP*                 scip,               /**< SCIP data structure */
@@ -551,6 +632,7 @@ SCIP_RETCODE execWithReasons(
    int32_t* hcol = NULL;
    int32_t* hbound = NULL;
    int32_t* hrow = NULL;
+   int32_t* hrule = NULL;
    int64_t nhist = 0;
    int64_t cap = 4 * (int64_t)propdata->nvars + 64;
    SCIP_Bool cutoff;
@@ -562,6 +644,7 @@ SCIP_RETCODE execWithReasons(
    SCIP_CALL( SCIPallocBufferArray(scip, &lb, propdata->nvars) );
    SCIP_CALL( SCIPallocBufferArray(scip, &ub, propdata->nvars) );
    SCIP_CALL( getEngineParams(scip, propdata, &params) );
+   SCIP_CALL( rangedRowFlag(scip, &params) );
    for( ;; )
    {
       for( j = 0; j < propdata->nvars; ++j )
@@ -574,13 +657,15 @@ SCIP_RETCODE execWithReasons(
       SCIP_CALL( SCIPallocBufferArray(scip, &hbound, cap) );
       SCIP_CALL( SCIPallocBufferArray(scip, &hval, cap) );
       SCIP_CALL( SCIPallocBufferArray(scip, &hrow, cap) );
+      SCIP_CALL( SCIPallocBufferArray(scip, &hrule, cap) );
       memset(&stats, 0, sizeof(stats));
       rc = sap_propagate_reasons(propdata->mat, &params, lb, ub, cap, &nhist, hround, hcol, hbound, hval, hrow,
-         &conflict, &stats);
+         hrule, &conflict, &stats);
       if( rc != SAP_NOMEMORY || nhist <= cap )
          break;
       /* a longer history than the room given: again with room for all of it (same results) */
       cap = nhist;
+      SCIPfreeBufferArray(scip, &hrule);
       SCIPfreeBufferArray(scip, &hrow);
       SCIPfreeBufferArray(scip, &hval);
       SCIPfreeBufferArray(scip, &hbound);
@@ -600,11 +685,13 @@ SCIP_RETCODE execWithReasons(
 
       if( hbound[q] == 0 )
       {
-         SCIP_CALL( SCIPinferVarLbProp(scip, var, hval[q], prop, hrow[q], TRUE, &infeasible, &tightened) );
+         SCIP_CALL( SCIPinferVarLbProp(scip, var, hval[q], prop, 2 * hrow[q] + hrule[q], TRUE, &infeasible,
+               &tightened) );
       }
       else
       {
-         SCIP_CALL( SCIPinferVarUbProp(scip, var, hval[q], prop, hrow[q], TRUE, &infeasible, &tightened) );
+         SCIP_CALL( SCIPinferVarUbProp(scip, var, hval[q], prop, 2 * hrow[q] + hrule[q], TRUE, &infeasible,
+               &tightened) );
       }
       nchg += tightened ? 1 : 0;
       cutoff = infeasible;
@@ -620,7 +707,12 @@ SCIP_RETCODE execWithReasons(
          && conflict.kind != SAP_CONFLICT_NONE )
       {
          SCIP_CALL( SCIPinitConflictAnalysis(scip, SCIP_CONFTYPE_PROPAGATION, FALSE) );
-         if( conflict.kind == SAP_CONFLICT_ROW )
+         if( conflict.kind == SAP_CONFLICT_ROW && conflict.side == 2 )
+         {
+            /* the ranged-row test failed */
+            SCIP_CALL( addRowAllBounds(scip, propdata, conflict.row, NULL) );
+         }
+         else if( conflict.kind == SAP_CONFLICT_ROW )
          {
             /* side 0: the minimal activity exceeds rhs; side 1: the maximal one misses lhs */
             SCIP_CALL( addRowBounds(scip, propdata, conflict.row, -1, conflict.side == 1, NULL) );
@@ -628,13 +720,12 @@ SCIP_RETCODE execWithReasons(
          else
          {
             SCIP_VAR* var = propdata->vars[conflict.col];
-            double a;
 
-            /* the lower side: rowlb's residual activity implied it, else the column's own bound */
+            /* each side: the row that implied it, else the column's own bound */
             if( conflict.rowlb >= 0 )
             {
-               a = rowCoef(propdata, conflict.rowlb, conflict.col);
-               SCIP_CALL( addRowBounds(scip, propdata, conflict.rowlb, conflict.col, a > 0.0, NULL) );
+               SCIP_CALL( addReason(scip, propdata, conflict.rowlb, conflict.rules & 1, conflict.col,
+                     SCIP_BOUNDTYPE_LOWER, NULL) );
             }
             else
             {
@@ -642,8 +733,8 @@ SCIP_RETCODE execWithReasons(
             }
             if( conflict.rowub >= 0 )
             {
-               a = rowCoef(propdata, conflict.rowub, conflict.col);
-               SCIP_CALL( addRowBounds(scip, propdata, conflict.rowub, conflict.col, a < 0.0, NULL) );
+               SCIP_CALL( addReason(scip, propdata, conflict.rowub, (conflict.rules >> 1) & 1, conflict.col,
+                     SCIP_BOUNDTYPE_UPPER, NULL) );
             }
             else
             {
@@ -656,6 +747,7 @@ SCIP_RETCODE execWithReasons(
    }
    *result = cutoff ? SCIP_CUTOFF : (nchg > 0 ? SCIP_REDUCEDDOM : SCIP_DIDNOTFIND);
 
+   SCIPfreeBufferArray(scip, &hrule);
    SCIPfreeBufferArray(scip, &hrow);
    SCIPfreeBufferArray(scip, &hval);
    SCIPfreeBufferArray(scip, &hbound);
@@ -856,29 +948,25 @@ SCIP_DECL_PROPEXEC(propExecGpulinear)
    return SCIP_OKAY;
 }
 
-/** conflict resolution (prop.c:739-788): inferinfo is the reason row of the change.  A lower
- *  bound of a column with a positive coefficient came from the row's lhs and its maximal
- *  residual activity, with a negative one from the rhs and the minimal residual; symmetric
- *  for upper bounds; the bounds making up that residual at the change's time explain it
- *  (cons_linear.c resolvePropagation / addConflictBounds).  Without a recorded row (reasons
- *  off, or a change from before the current matrix) the change is reported as unexplained,
- *  which conflict analysis accepts (the change itself stays in the conflict). */
+/** conflict resolution (prop.c:739-788): inferinfo is the reason row * 2 + rule of the change
+ *  (cons_linear.c keeps the position and PROPRULE there, :344-369).  Rule 0: the bounds making
+ *  up the row's residual activity at the change's time explain it (resolvePropagation /
+ *  addConflictBounds); rule 1 (ranged-row test): the bounds of all the row's columns.  Without
+ *  a recorded row (reasons off, or a change from before the current matrix) the change is
+ *  reported as unexplained, which conflict analysis accepts (the change itself stays in the
+ *  conflict). */
 static
 SCIP_DECL_PROPRESPROP(propRespropGpulinear)
 {  /*lint --e{715}*/
    SCIP_PROPDATA* propdata = SCIPpropGetData(prop);
    int j = SCIPvarGetProbindex(infervar);
-   double a;
+   int row = inferinfo >> 1;
 
    *result = SCIP_DIDNOTFIND;
-   if( !propdata->reasons || propdata->rowptr == NULL || inferinfo < 0 || inferinfo >= propdata->nrows || j < 0
-      || j >= propdata->nvars || propdata->vars[j] != infervar )
+   if( !propdata->reasons || propdata->rowptr == NULL || inferinfo < 0 || row >= propdata->nrows || j < 0
+      || j >= propdata->nvars || propdata->vars[j] != infervar || rowCoef(propdata, row, j) == 0.0 )
       return SCIP_OKAY;
-   a = rowCoef(propdata, inferinfo, j);
-   if( a == 0.0 )
-      return SCIP_OKAY;
-   SCIP_CALL( addRowBounds(scip, propdata, inferinfo, j, (boundtype == SCIP_BOUNDTYPE_LOWER) == (a > 0.0),
-         bdchgidx) );
+   SCIP_CALL( addReason(scip, propdata, row, inferinfo & 1, j, boundtype, bdchgidx) );
    *result = SCIP_SUCCESS;
    return SCIP_OKAY;
 }

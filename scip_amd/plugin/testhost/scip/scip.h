@@ -111,6 +111,7 @@ SCIP_RETCODE SCIPaddIntParam(SCIP* scip, const char* name, const char* desc, int
 SCIP_RETCODE SCIPaddBoolParam(SCIP* scip, const char* name, const char* desc, SCIP_Bool* valueptr,
    SCIP_Bool isadvanced, SCIP_Bool defaultvalue, SCIP_DECL_PARAMCHGD((*paramchgd)), SCIP_PARAMDATA* paramdata);
 SCIP_RETCODE SCIPgetRealParam(SCIP* scip, const char* name, SCIP_Real* value);
+SCIP_RETCODE SCIPgetBoolParam(SCIP* scip, const char* name, SCIP_Bool* value);
 /* scip_general.h / scip_probing.h */
 SCIP_STAGE SCIPgetStage(SCIP* scip);
 SCIP_Bool SCIPinProbing(SCIP* scip);

@@ -93,6 +93,7 @@ struct Scip
    struct ConfRec { int var, bound; double val; }* conf;
    int nanalyzed;
    int stage;
+   int rangedrowprop;   /* constraints/linear/rangedrowpropagation (SCIP's default TRUE) */
 };
 
 /* ---- numerics (def.h, set.c) ---- */
@@ -120,6 +121,14 @@ SCIP_RETCODE SCIPgetRealParam(SCIP* scip, const char* name, SCIP_Real* value)
       *value = scip->maxeasy;
    else
       return SCIP_PARAMETERUNKNOWN;
+   return SCIP_OKAY;
+}
+
+SCIP_RETCODE SCIPgetBoolParam(SCIP* scip, const char* name, SCIP_Bool* value)
+{
+   if( strcmp(name, "constraints/linear/rangedrowpropagation") != 0 )
+      return SCIP_PARAMETERUNKNOWN;
+   *value = scip->rangedrowprop ? TRUE : FALSE;
    return SCIP_OKAY;
 }
 
@@ -531,6 +540,7 @@ SCIP_EXPORT SCIP* th_create(void)
    s->recompfac = 1e6;
    s->maxeasy = 1e6;
    s->stage = SCIP_STAGE_SOLVING;
+   s->rangedrowprop = 1;
    return s;
 }
 
@@ -586,6 +596,9 @@ SCIP_EXPORT int th_add_rows(SCIP* s, int nrows, const long long* rowptr, const i
 }
 
 SCIP_EXPORT int th_include_gpulinear(SCIP* s) { return SCIPincludePropGpulinear(s); }
+
+/* SCIP's own parameter constraints/linear/rangedrowpropagation */
+SCIP_EXPORT void th_set_rangedrowprop(SCIP* s, int value) { s->rangedrowprop = value; }
 
 SCIP_EXPORT int th_set_int_param(SCIP* s, const char* name, int value)
 {

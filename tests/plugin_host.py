@@ -45,6 +45,8 @@ def lib():
         IP = C.POINTER(C.c_int)
         L.th_get_probe_records.argtypes = [C.c_void_p, IP, C.c_void_p, C.c_void_p, IP, C.c_void_p, C.c_void_p,
                                            C.c_void_p, IP, IP]
+        L.th_set_rangedrowprop.argtypes = [C.c_void_p, C.c_int]
+        L.th_set_rangedrowprop.restype = None
         L.th_resprop_change.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
         L.th_get_changes.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         L.th_get_conflict.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]

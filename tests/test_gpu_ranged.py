@@ -102,12 +102,3 @@ def test_nodes(gpu_ctx):
         nodes.close()
     base.close()
     mat.close()
-
-
-def test_reasons_reject_ranged_rows(gpu_ctx):
-    inst = ranged_system(np.random.default_rng(1201), 4, 7)
-    mat = sa.Matrix(gpu_ctx, inst)
-    with pytest.raises(sa.SapError) as e:
-        mat.propagate_reasons(params=sa.default_params(flags=RANGED))
-    assert e.value.rc == sa.SAP_NOTIMPLEMENTED
-    mat.close()

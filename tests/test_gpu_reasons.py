@@ -21,23 +21,24 @@ def say(*a):
     print(*a, flush=True)
 
 
-def compare(gpu_ctx, inst, name, exact=True):
-    say(f"[case] reasons {name} rows={inst.nrows} cols={inst.ncols}")
+def compare(gpu_ctx, inst, name, exact=True, ranged=False):
+    say(f"[case] reasons {name} rows={inst.nrows} cols={inst.ncols} ranged={ranged}")
     mat = sa.Matrix(gpu_ctx, inst)
-    g = mat.propagate_reasons()
-    o = orc.propagate_reasons(orc.OMatrix.from_instance(inst), inst.lb, inst.ub)
+    prm = sa.default_params(flags=sa.SAP_FLAG_RANGEDROWS if ranged else 0)
+    g = mat.propagate_reasons(params=prm)
+    o = orc.propagate_reasons(orc.OMatrix.from_instance(inst), inst.lb, inst.ub, orc.params(rangedrows=int(ranged)))
     cut = g["stats"]["result"] == sa.SAP_CUTOFF
     assert cut == bool(o["result"]["cutoff"]), name
     assert g["stats"]["rounds"] == o["result"]["rounds"], name
     assert g["conflict"] == o["conflict"], (name, g["conflict"], o["conflict"])
     assert g["nhist"] == o["nhist"], name
-    for k in ("round", "col", "bound", "row"):
+    for k in ("round", "col", "bound", "row", "rule"):
         assert np.array_equal(g[k], o[k]), (name, k)
     assert np.array_equal(g["val"].view(np.int64), o["val"].view(np.int64)), name
     if not cut:
         assert np.array_equal(g["lb"], o["lb"]) and np.array_equal(g["ub"], o["ub"]), name
         # the same propagation as without reasons
-        pl, pu, ps = mat.propagate()
+        pl, pu, ps = mat.propagate(params=prm)
         assert np.array_equal(pl, g["lb"]) and np.array_equal(pu, g["ub"]) and ps["rounds"] == g["stats"]["rounds"]
     if exact:
         _, lb, ub = replay_history(inst, inst.lb, inst.ub, g)
@@ -113,3 +114,30 @@ def test_history_capacity(gpu_ctx):
     for k in ("round", "col", "bound", "val", "row"):
         assert np.array_equal(full[k], again[k]), k
     mat.close()
+
+
+def test_ranged_small_systems(gpu_ctx):
+    from tests.test_oracle_ranged import ranged_system
+    rules = 0
+    for seed in range(40):
+        inst = ranged_system(np.random.default_rng(1200 + seed), 4, 7, shift=(seed % 4 == 3) * float(seed % 3 + 1))
+        rules += int(np.sum(compare(gpu_ctx, inst, f"rsmall{seed}", ranged=True)["rule"]))
+    assert rules > 5
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_ranged_large(gpu_ctx, seed):
+    from tests.test_oracle_ranged import ranged_system
+    g = compare(gpu_ctx, ranged_system(np.random.default_rng(1300 + seed), 1500, 3000, slack=False), f"rlarge{seed}",
+                ranged=True)
+    assert int(np.sum(g["rule"])) > 0
+
+
+@pytest.mark.parametrize("name", ["bell5", "p0548", "misc03", "rgn", "gt2"])
+def test_ranged_real_instances(gpu_ctx, name):
+    inst, _ = sa.read_mps(os.path.join(HERE, "golden", "mps", name + ".mps"))
+    compare(gpu_ctx, inst, name, ranged=True)
+
+
+def test_ranged_mixed(gpu_ctx):
+    compare(gpu_ctx, sa.gen_mixed(20000, 20000, seed=71, planted=True), "rmixed", exact=False, ranged=True)

@@ -12,15 +12,15 @@ from tests.util import (INF, late_cut_instance, make_instance, planted_integer_i
                         random_integer_instance)
 
 
-def run_and_check(inst):
+def run_and_check(inst, prm=None):
     om = orc.OMatrix.from_instance(inst)
-    r = orc.propagate_reasons(om, inst.lb, inst.ub)
-    jl, ju, jr = orc.propagate_jacobi(om, inst.lb, inst.ub)
+    r = orc.propagate_reasons(om, inst.lb, inst.ub, prm)
+    jl, ju, jr = orc.propagate_jacobi(om, inst.lb, inst.ub, prm)
     assert r["result"]["cutoff"] == jr["cutoff"] and r["result"]["rounds"] == jr["rounds"]
     starts, lb, ub = replay_history(inst, inst.lb, inst.ub, r)
     if jr["cutoff"]:
         # the last round's start: every change of the earlier rounds applied
-        check_conflict(inst, r["conflict"], lb, ub)
+        r["certified"] = check_conflict(inst, r["conflict"], lb, ub)
         assert r["conflict"]["round"] == jr["rounds"]
     else:
         assert r["conflict"]["kind"] == 0
@@ -84,12 +84,12 @@ def test_known_certificates():
     inst = make_instance([([(0, 1.0), (1, 1.0)], 3.0, INF), ([(1, 1.0), (2, 1.0)], -INF, 1.0)],
                          [0.0] * 3, [1.0] * 3, [1] * 3)
     r = run_and_check(inst)
-    assert r["conflict"] == dict(kind=1, round=1, row=0, side=1, col=-1, rowlb=-1, rowub=-1)
+    assert r["conflict"] == dict(kind=1, round=1, row=0, side=1, col=-1, rowlb=-1, rowub=-1, rules=0)
     # x + y >= 2 and x + z <= 0.5: row 0 lifts x to 1 while row 1 pushes it to 0 in the same round
     inst = make_instance([([(0, 1.0), (1, 1.0)], 2.0, INF), ([(0, 1.0), (2, 1.0)], -INF, 0.5)],
                          [0.0] * 3, [1.0] * 3, [1] * 3)
     r = run_and_check(inst)
-    assert r["conflict"] == dict(kind=2, round=1, row=-1, side=0, col=0, rowlb=0, rowub=1)
+    assert r["conflict"] == dict(kind=2, round=1, row=-1, side=0, col=0, rowlb=0, rowub=1, rules=0)
     assert r["nhist"] == 0
 
 
@@ -127,3 +127,37 @@ def test_checks_reject_wrong_reasons():
     bad["round"][4] = 1         # x3 >= 5 read in round 1, before x1 = 1
     with pytest.raises(AssertionError):
         replay_history(inst, inst.lb, inst.ub, bad)
+
+
+RANGED = dict(rangedrows=1)
+
+
+def test_ranged_reference_examples_with_reasons():
+    inst = make_instance([([(0, 12.0), (1, 9.0), (2, -1.0)], 0.0, 0.0)], [-INF, -INF, 1.0], [INF, INF, 2.0], [1, 1, 1])
+    r = run_and_check(inst, orc.params(**RANGED))
+    assert r["conflict"] == dict(kind=1, round=1, row=0, side=2, col=-1, rowlb=-1, rowub=-1, rules=0)
+    inst = make_instance([([(0, 12.0), (1, 9.0), (2, -1.0)], 1.0, 1.0)], [-INF, -INF, 1.0], [INF, INF, 2.0], [1, 1, 1])
+    r = run_and_check(inst, orc.params(**RANGED))
+    got = list(zip(r["round"].tolist(), r["col"].tolist(), r["bound"].tolist(), r["val"].tolist(), r["row"].tolist(),
+                   r["rule"].tolist()))
+    assert got == [(1, 2, 0, 2.0, 0, 1)]
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_ranged_reasons_on_small_systems(seed):
+    from tests.test_oracle_ranged import ranged_system
+    inst = ranged_system(np.random.default_rng(1200 + seed), 4, 7, shift=(seed % 4 == 3) * float(seed % 3 + 1))
+    r = run_and_check(inst, orc.params(**RANGED))
+    if r["result"]["cutoff"]:
+        assert r["certified"]
+
+
+def test_ranged_reasons_are_exercised():
+    from tests.test_oracle_ranged import ranged_system
+    rules = sides = 0
+    for seed in range(40):
+        inst = ranged_system(np.random.default_rng(1200 + seed), 4, 7, shift=(seed % 4 == 3) * float(seed % 3 + 1))
+        r = orc.propagate_reasons(orc.OMatrix.from_instance(inst), inst.lb, inst.ub, orc.params(**RANGED))
+        rules += int(np.sum(r["rule"]))
+        sides += int(r["conflict"]["kind"] == 1 and r["conflict"]["side"] == 2)
+    assert rules > 5 and sides > 0, (rules, sides)

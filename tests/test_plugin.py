@@ -50,8 +50,10 @@ def test_plugin_without_device_returns_a_scip_error():
     assert h.close() == ph.SCIP_OKAY
 
 
-def _oracle(inst, lb, ub):
-    return orc.propagate_jacobi(orc.OMatrix.from_instance(inst), lb, ub)
+def _oracle(inst, lb, ub, ranged=True):
+    """The oracle's fixpoint; PROPEXEC propagates ranged rows (SCIP's default
+    constraints/linear/rangedrowpropagation), presolving does not."""
+    return orc.propagate_jacobi(orc.OMatrix.from_instance(inst), lb, ub, orc.params(rangedrows=int(ranged)))
 
 
 @pytest.mark.gpu
@@ -109,7 +111,7 @@ def test_propexec_nodes_match_oracle(gpu_ctx):
 @pytest.mark.gpu
 def test_presol_tightens_global_bounds(gpu_ctx):
     inst = sa.gen_mixed(2000, 2000, seed=4, planted=True)
-    ol, ou, _ = _oracle(inst, inst.lb, inst.ub)
+    ol, ou, _ = _oracle(inst, inst.lb, inst.ub, ranged=False)
     h = ph.Host(inst)
     assert h.call("th_initpre") == ph.SCIP_OKAY
     rc, res, nchg = h.presol()
@@ -213,7 +215,7 @@ def conflict_arrays(n, cset):
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", ["binary", "mixed", "integer"])
 def test_propexec_changes_carry_reasons_that_explain_them(gpu_ctx, case):
-    from tests.reasons_check import check_implied
+    from tests.reasons_check import check_implied, check_ranged_implied
     from tests.util import planted_integer_instance
     if case == "binary":
         inst = random_binary_instance(np.random.default_rng(904), 1500, 1000, 10, hidden=True)
@@ -222,7 +224,7 @@ def test_propexec_changes_carry_reasons_that_explain_them(gpu_ctx, case):
     else:
         inst = planted_integer_instance(np.random.default_rng(951), 400, 300, 6)
     mat = sa.Matrix(gpu_ctx, inst)
-    ref = mat.propagate_reasons()
+    ref = mat.propagate_reasons(params=sa.default_params(flags=sa.SAP_FLAG_RANGEDROWS))
     mat.close()
     assert ref["stats"]["result"] != sa.SAP_CUTOFF and ref["nhist"] > 0
     print(f"[case] plugin reasons {case} nhist={ref['nhist']}", flush=True)
@@ -231,16 +233,19 @@ def test_propexec_changes_carry_reasons_that_explain_them(gpu_ctx, case):
     rc, res = h.propexec()
     assert rc == ph.SCIP_OKAY and res == ph.SCIP_REDUCEDDOM
     var, bound, info, old, new = h.changes()
-    # the device history, in order, with the reason rows as inferinfo
+    # the device history, in order, with reason row * 2 + rule as inferinfo
     assert np.array_equal(var, ref["col"]) and np.array_equal(bound, ref["bound"])
-    assert np.array_equal(info, ref["row"]) and np.array_equal(new, ref["val"])
+    assert np.array_equal(info, 2 * ref["row"] + ref["rule"]) and np.array_equal(new, ref["val"])
     # every change is explained by its row and the bounds in force at its time
     for pos in range(len(var)):
         rc, r = h.resprop_change(pos)
         assert rc == ph.SCIP_OKAY and r == ph.SCIP_SUCCESS
         cset, _ = h.conflict()
         lb, ub = conflict_arrays(inst.ncols, cset)
-        check_implied(inst, int(info[pos]), int(var[pos]), int(bound[pos]), float(new[pos]), lb, ub)
+        if info[pos] & 1:
+            check_ranged_implied(inst, int(info[pos]) >> 1, int(var[pos]), int(bound[pos]), float(new[pos]), lb, ub)
+        else:
+            check_implied(inst, int(info[pos]) >> 1, int(var[pos]), int(bound[pos]), float(new[pos]), lb, ub)
     assert h.call("th_exitsol") == ph.SCIP_OKAY
     assert h.close() == ph.SCIP_OKAY
 
@@ -260,7 +265,7 @@ def test_propexec_cutoff_starts_conflict_analysis(gpu_ctx, seed):
     else:
         inst = random_integer_instance(np.random.default_rng(953), 400, 300, 8, sidejitter=0.3)
     mat = sa.Matrix(gpu_ctx, inst)
-    ref = mat.propagate_reasons()
+    ref = mat.propagate_reasons(params=sa.default_params(flags=sa.SAP_FLAG_RANGEDROWS))
     mat.close()
     if ref["stats"]["result"] != sa.SAP_CUTOFF:
         pytest.skip("instance not infeasible")
@@ -270,7 +275,7 @@ def test_propexec_cutoff_starts_conflict_analysis(gpu_ctx, seed):
     rc, res = h.propexec()
     assert rc == ph.SCIP_OKAY and res == ph.SCIP_CUTOFF
     var, bound, info, old, new = h.changes()
-    assert np.array_equal(var, ref["col"]) and np.array_equal(info, ref["row"])
+    assert np.array_equal(var, ref["col"]) and np.array_equal(info, 2 * ref["row"] + ref["rule"])
     cset, nanalyzed = h.conflict()
     assert nanalyzed == 1 and len(cset) > 0
     # the conflict set alone proves the node infeasible
@@ -278,3 +283,24 @@ def test_propexec_cutoff_starts_conflict_analysis(gpu_ctx, seed):
     check_conflict(inst, ref["conflict"], lb, ub)
     assert h.call("th_exitsol") == ph.SCIP_OKAY
     assert h.close() == ph.SCIP_OKAY
+
+
+@pytest.mark.gpu
+def test_propexec_without_ranged_rows_follows_scip_parameter(gpu_ctx):
+    # constraints/linear/rangedrowpropagation = FALSE: the device skips the ranged-row test
+    from tests.test_oracle_ranged import ranged_system
+    inst = ranged_system(np.random.default_rng(1300), 1500, 3000, slack=False)
+    with_r = _oracle(inst, inst.lb, inst.ub, ranged=True)
+    without = _oracle(inst, inst.lb, inst.ub, ranged=False)
+    assert not np.array_equal(with_r[0], without[0]) or not np.array_equal(with_r[1], without[1])
+    for ranged, (ol, ou, _) in ((True, with_r), (False, without)):
+        for reasons in (0, 1):
+            h = ph.Host(inst)
+            ph.lib().th_set_rangedrowprop(h.h, int(ranged))
+            assert h.call("th_set_int_param", b"propagating/gpulinear/reasons", reasons) == ph.SCIP_OKAY
+            assert h.call("th_initsol") == ph.SCIP_OKAY
+            rc, res = h.propexec()
+            assert rc == ph.SCIP_OKAY and res == ph.SCIP_REDUCEDDOM
+            lb, ub = h.bounds()
+            assert np.array_equal(lb, ol) and np.array_equal(ub, ou), (ranged, reasons)
+            assert h.close() == ph.SCIP_OKAY
