@@ -22,6 +22,10 @@
  *   src/scip/cons_linear.c:17162-17279 eventExecLinear
  *   src/scip/solve.c:723-795           propagateDomains round loop
  *   src/scip/cons.c:7309-7366          delayed (un)marking of constraints for propagation
+ *   src/scip/cons_linear.c:5715-6697  rangedRowPropagation (jRanged, params.rangedrows)
+ *   src/scip/cons_linear.c:344-369, 4981-5189, 5657  inferinfo, conflict bounds (the reason
+ *                                    recording of orc_propagate_jacobi_reasons)
+ *   src/scip/prop_probing.c:1203-1640 probing branches and deductions (oracle.py probe())
  *
  * What pins each part (tests/test_oracle_arith.py, tests/test_oracle_prop.py):
  *   numerics, TwoSum/Sum21/TwoProd   bit-exact against SCIP's own dbldblarith.h (oracle/_ref) and
@@ -32,6 +36,17 @@
  *   propagation fixpoints            brute-force soundness on tiny integer systems, completeness
  *                                    on infeasible binary systems, Jacobi = SCIP order on binary
  *                                    systems (unique fixpoint), mutual fixpoints otherwise
+ *   ranged rows                      the two examples of the reference's own documentation of
+ *                                    rangedRowPropagation (cons_linear.c:5702-5711), a hand-
+ *                                    computed bound, soundness against enumeration of every
+ *                                    integer point (tests/test_oracle_ranged.py)
+ *   reasons                          every history entry implied by its row and the bounds of the
+ *                                    earlier rounds (exact rational arithmetic; enumeration for
+ *                                    ranged-rule entries), every certificate proving
+ *                                    infeasibility, hand-computed histories and certificates
+ *                                    (tests/test_oracle_reasons.py)
+ *   probing                          hand-computed pair lists and soundness against enumeration
+ *                                    of every 0/1 point (tests/test_oracle_probe.py)
  * Parity unpinned (no reference vector or closed form decides them; SCIP itself cannot be
  * built here):
  *   - orc_propagate_scip's exact general-integer / continuous bounds where the row order and
