@@ -264,6 +264,18 @@ int sap_propagate_nodes(sap_bounds* bnd, const sap_params* params, int32_t nvec,
 int sap_probe(sap_bounds* bnd, const sap_params* params, int32_t nprobe, const int32_t* probecol,
    const sap_bounds* base, int32_t basevec, int32_t* verdict, int32_t* rounds, int64_t dedcap, int64_t* dedptr,
    int32_t* dedcol, double* dlb0, double* dub0, double* dlb1, double* dub1, sap_stats* stats);
+/* Split probing, the propagation part of strong branching (SCIPgetVarStrongbranchWithPropagation,
+ * scip_var.c:4139-4440, for many candidates at once): probe p branches column probecol[p] into
+ * x <= downub[p] (SAP_PROBE_DOWN_INFEASIBLE) and x >= uplb[p] (SAP_PROBE_UP_INFEASIBLE), with
+ * base lb <= downub[p] < uplb[p] <= base ub (else SAP_INVALIDDATA); SCIP's strong branching
+ * uses downub = feasCeil(solval - 1), uplb = feasFloor(solval + 1).  Outputs as sap_probe; the
+ * pair lists give the valid bounds of both children (SCIP's newlbs/newubs: the union of the
+ * children's bounds differs from the base only on columns changed in both), and when neither
+ * branch is infeasible they include the probe column itself with its bounds in both branches. */
+int sap_probe_split(sap_bounds* bnd, const sap_params* params, int32_t nprobe, const int32_t* probecol,
+   const double* downub, const double* uplb, const sap_bounds* base, int32_t basevec, int32_t* verdict,
+   int32_t* rounds, int64_t dedcap, int64_t* dedptr, int32_t* dedcol, double* dlb0, double* dub0, double* dlb1,
+   double* dub1, sap_stats* stats);
 
 /* Independent instances packed block-diagonally into one resident matrix (one launch per
  * round for all of them; replaces one consPropLinear fixpoint per SCIP instance).  Instance

@@ -211,11 +211,13 @@ def canonical_sum(x):
     return hi.value, lo.value
 
 
-def probe(om: OMatrix, lb, ub, probecols, prm: orc_params | None = None):
+def probe(om: OMatrix, lb, ub, probecols, prm: orc_params | None = None, downub=None, uplb=None):
     """Probing on binary columns, restated from prop_probing.c: each probe column x is fixed
     to 0 and to 1 (SCIPapplyProbingVar, prop_probing.c:1203-1311, here with linear
     propagation as the only propagator: the Jacobi twin), and the two branches are compared
-    as SCIPanalyzeDeductionsProbing (prop_probing.c:1314-1500) compares them.
+    as SCIPanalyzeDeductionsProbing (prop_probing.c:1314-1500) compares them.  With
+    downub/uplb the branches are x <= downub[q] and x >= uplb[q] (the children of strong
+    branching with propagation, scip_var.c:4139-4440).
 
     Returns per probe: verdict bits (1: x = 0 infeasible, 2: x = 1 infeasible), the rounds of
     both branches, and -- when neither branch is infeasible -- the columns other than x whose
@@ -223,13 +225,19 @@ def probe(om: OMatrix, lb, ub, probecols, prm: orc_params | None = None):
     """
     lb = np.asarray(lb, dtype=np.float64)
     ub = np.asarray(ub, dtype=np.float64)
+    split = downub is not None
     out = []
-    for x in probecols:
+    for q, x in enumerate(probecols):
         branch = []
-        for val in (0.0, 1.0):
+        for side in (0, 1):
             l0 = lb.copy()
             u0 = ub.copy()
-            l0[x] = u0[x] = val
+            if split and side == 0:
+                u0[x] = downub[q]
+            elif split:
+                l0[x] = uplb[q]
+            else:
+                l0[x] = u0[x] = float(side)
             pl, pu, r = propagate_jacobi(om, l0, u0, prm)
             branch.append((pl, pu, r))
         verdict = (1 if branch[0][2]["cutoff"] else 0) | (2 if branch[1][2]["cutoff"] else 0)
@@ -239,7 +247,10 @@ def probe(om: OMatrix, lb, ub, probecols, prm: orc_params | None = None):
             (l0, u0, _), (l1, u1, _) = branch
             ch0 = (l0 != lb) | (u0 != ub)
             ch1 = (l1 != lb) | (u1 != ub)
-            ch0[x] = ch1[x] = False
+            # binary probing: the probe column is fixed in both branches and not listed; split
+            # probing (strong branching): it is always listed, as SCIP's analysis of a
+            # non-binary probing variable includes it
+            ch0[x] = ch1[x] = split
             cols = np.nonzero(ch0 & ch1)[0].astype(np.int32)
             rec.update(cols=cols, b0=np.stack([l0[cols], u0[cols]], 1), b1=np.stack([l1[cols], u1[cols]], 1))
         out.append(rec)

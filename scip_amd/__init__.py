@@ -185,6 +185,8 @@ EXPORTS = {
                                         _DP, _I32P, _I32P, C.POINTER(sap_conflict), C.POINTER(sap_stats)]),
     "sap_probe": (C.c_int, [_P, C.POINTER(sap_params), C.c_int32, _I32P, _P, C.c_int32, _I32P, _I32P, C.c_int64,
                             _I64P, _I32P, _DP, _DP, _DP, _DP, C.POINTER(sap_stats)]),
+    "sap_probe_split": (C.c_int, [_P, C.POINTER(sap_params), C.c_int32, _I32P, _DP, _DP, _P, C.c_int32, _I32P, _I32P,
+                                  C.c_int64, _I64P, _I32P, _DP, _DP, _DP, _DP, C.POINTER(sap_stats)]),
     "sap_matrix_create_packed": (C.c_int, [_P, C.c_int32, _I64P, _I64P, C.c_int64, C.c_int64, _I64P, _I32P, _DP,
                                            _DP, _DP, _U8P, _DP, _DP, C.POINTER(_P)]),
     "sap_propagate_packed": (C.c_int, [_P, C.POINTER(sap_params), _DP, _DP, _I32P, _I32P, C.POINTER(sap_stats)]),
@@ -628,11 +630,16 @@ class Bounds:
             return out
 
     def probe(self, base: "Bounds", basevec, probecol, params: sap_params | None = None,
-              dedcap: int | None = None, want_pairs: bool = True):
-        """Both branches x = 0 / x = 1 of every probe column from a resident base vector:
-        per-probe verdict bits, per-branch rounds and the columns changed in both branches."""
+              dedcap: int | None = None, want_pairs: bool = True, downub=None, uplb=None):
+        """Both branches x = 0 / x = 1 of every probe column (with downub/uplb: x <= downub,
+        x >= uplb) from a resident base vector: per-probe verdict bits, per-branch rounds and
+        the columns changed in both branches."""
         probecol = np.ascontiguousarray(probecol, dtype=np.int32)
         n = len(probecol)
+        split = downub is not None
+        if split:
+            downub = np.ascontiguousarray(downub, dtype=np.float64)
+            uplb = np.ascontiguousarray(uplb, dtype=np.float64)
         ver = np.zeros(max(n, 1), dtype=np.int32)
         rnd = np.zeros(max(2 * n, 1), dtype=np.int32)
         cap = int(dedcap if dedcap is not None else max(1024, 16 * n))
@@ -642,13 +649,18 @@ class Bounds:
                 if want_pairs else [None] * 5
             st = sap_stats()
             p = params or default_params()
-            rc = lib().sap_probe(self.h, C.byref(p), n, _i32p(probecol), base.h, basevec, _i32p(ver), _i32p(rnd),
-                                 cap, _i64p(dptr), _i32p(arrs[0]), _dp(arrs[1]), _dp(arrs[2]), _dp(arrs[3]),
-                                 _dp(arrs[4]), C.byref(st))
+            if split:
+                rc = lib().sap_probe_split(self.h, C.byref(p), n, _i32p(probecol), _dp(downub), _dp(uplb), base.h,
+                                           basevec, _i32p(ver), _i32p(rnd), cap, _i64p(dptr), _i32p(arrs[0]),
+                                           _dp(arrs[1]), _dp(arrs[2]), _dp(arrs[3]), _dp(arrs[4]), C.byref(st))
+            else:
+                rc = lib().sap_probe(self.h, C.byref(p), n, _i32p(probecol), base.h, basevec, _i32p(ver),
+                                     _i32p(rnd), cap, _i64p(dptr), _i32p(arrs[0]), _dp(arrs[1]), _dp(arrs[2]),
+                                     _dp(arrs[3]), _dp(arrs[4]), C.byref(st))
             if rc == SAP_NOMEMORY and dedcap is None and int(dptr[-1]) > cap:
                 cap = int(dptr[-1])   # run again with room for every pair (same results)
                 continue
-            _chk(rc, "sap_probe")
+            _chk(rc, "sap_probe_split" if split else "sap_probe")
             k = int(dptr[-1])
             out = dict(verdict=ver[:n], rounds=rnd[:2 * n].reshape(n, 2), dedptr=dptr, stats=st.as_dict())
             if want_pairs:

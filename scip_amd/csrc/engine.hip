@@ -2825,6 +2825,14 @@ __global__ void k_probe_check(Mat M, const double2* base, const int32_t* pcol, i
    }
 }
 
+// probing: bounds of column cols[i] in vector vec[i] (vec == NULL: in src itself)
+__global__ void k_gather_bounds(const double2* src, int64_t ncols, const int32_t* cols, const int32_t* vec, int32_t n,
+   double2* out)
+{
+   for( int32_t i = blockIdx.x * (int32_t)blockDim.x + (int32_t)threadIdx.x; i < n; i += (int32_t)(gridDim.x * blockDim.x) )
+      out[i] = src[(vec != NULL ? (size_t)vec[i] * (size_t)ncols : 0) + (size_t)cols[i]];
+}
+
 // probing: probe p owns vectors 2p (x = 0) and 2p+1 (x = 1).  When neither branch is cut off,
 // the columns other than x found in both change logs are written at outoff[p] + (count[p]++)
 // with their bounds in both branches.  The shorter log is scanned and the other branch's log
@@ -4938,14 +4946,20 @@ static int ensureProbe(sap_bounds* b, int64_t n)
    return SAP_OKAY;
 }
 
-int sap_probe(sap_bounds* b, const sap_params* params, int32_t nprobe, const int32_t* probecol,
-   const sap_bounds* base, int32_t basevec, int32_t* verdict, int32_t* rounds, int64_t dedcap, int64_t* dedptr,
-   int32_t* dedcol, double* dlb0, double* dub0, double* dlb1, double* dub1, sap_stats* stats)
+// probing of binaries (downub == NULL: branches x = 0 / x = 1) or of split branches
+// x <= downub[p] / x >= uplb[p]; in the split form the probe column itself is listed with its
+// bounds in both branches
+static int probeCore(sap_bounds* b, const sap_params* params, int32_t nprobe, const int32_t* probecol,
+   const double* downub, const double* uplb, const sap_bounds* base, int32_t basevec, int32_t* verdict,
+   int32_t* rounds, int64_t dedcap, int64_t* dedptr, int32_t* dedcol, double* dlb0, double* dub0, double* dlb1,
+   double* dub1, sap_stats* stats)
 {
    if( b == NULL || base == NULL || base->mat != b->mat || basevec < 0 || basevec >= base->nvec || b->nvec < 2
       || nprobe < 0 || (nprobe > 0 && (probecol == NULL || verdict == NULL)) || b->mat->nseg != 1 || dedcap < 0
-      || (dedcol != NULL && (dedptr == NULL || dlb0 == NULL || dub0 == NULL || dlb1 == NULL || dub1 == NULL)) )
+      || (dedcol != NULL && (dedptr == NULL || dlb0 == NULL || dub0 == NULL || dlb1 == NULL || dub1 == NULL))
+      || (downub != NULL && nprobe > 0 && uplb == NULL) )
       return SAP_INVALIDDATA;
+   const bool split = downub != NULL;
    RCCHK(checkParams(params));
    sap_matrix* m = b->mat;
    const int64_t nc = m->ncols;
@@ -4963,14 +4977,27 @@ int sap_probe(sap_bounds* b, const sap_params* params, int32_t nprobe, const int
       HIPCHK(hipMemcpyAsync(b->base2, dbase, sizeof(double2) * (size_t)nc, hipMemcpyDeviceToDevice, s));
       dbase = b->base2;
    }
-   RCCHK(ensureProbe(b, 0));
+   RCCHK(ensureProbe(b, b->nvec));
    const int32_t per = b->nvec / 2;
-   // every probe column must be an unfixed binary of the base
+   // binary probing: every probe column must be an unfixed binary of the base; split probing:
+   // base lb <= downub < uplb <= base ub (the base bounds of the probe columns are kept)
+   std::vector<double2> pbase(split ? (size_t)nprobe : 0);
    for( int32_t p0 = 0; p0 < nprobe; p0 += b->nvec )
    {
       const int32_t n = std::min<int32_t>(b->nvec, nprobe - p0);
-      int32_t bad = INT32_MAX;
       HIPCHK(hipMemcpyAsync(b->pcol, probecol + p0, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, s));
+      if( split )
+      {
+         k_gather_bounds<<<gridFor((size_t)n), SAP_BLOCK, 0, s>>>(dbase, nc, b->pcol, NULL, n, b->outbnd);
+         HIPCHK(hipGetLastError());
+         HIPCHK(hipMemcpyAsync(pbase.data() + p0, b->outbnd, sizeof(double2) * (size_t)n, hipMemcpyDeviceToHost, s));
+         HIPCHK(hipStreamSynchronize(s));
+         for( int32_t q = p0; q < p0 + n; ++q )
+            if( !(downub[q] >= pbase[(size_t)q].x && downub[q] < uplb[q] && uplb[q] <= pbase[(size_t)q].y) )
+               return SAP_INVALIDDATA;
+         continue;
+      }
+      int32_t bad = INT32_MAX;
       k_fill_i32<<<1, 64, 0, s>>>(b->pcount, INT32_MAX, 1);
       k_probe_check<<<gridFor((size_t)n), SAP_BLOCK, 0, s>>>(toMat(m), dbase, b->pcol, n, b->pcount);
       HIPCHK(hipGetLastError());
@@ -5001,13 +5028,14 @@ int sap_probe(sap_bounds* b, const sap_params* params, int32_t nprobe, const int
          ovptr[(size_t)v] = v;
       for( int32_t p = 0; p < np; ++p )
       {
-         const int32_t x = probecol[p0 + p];
+         const int32_t q = p0 + p;
+         const int32_t x = probecol[q];
          ovcol[(size_t)(2 * p)] = x;
-         ovlb[(size_t)(2 * p)] = 0.0;
-         ovub[(size_t)(2 * p)] = 0.0;
+         ovlb[(size_t)(2 * p)] = split ? pbase[(size_t)q].x : 0.0;
+         ovub[(size_t)(2 * p)] = split ? downub[q] : 0.0;
          ovcol[(size_t)(2 * p + 1)] = x;
-         ovlb[(size_t)(2 * p + 1)] = 1.0;
-         ovub[(size_t)(2 * p + 1)] = 1.0;
+         ovlb[(size_t)(2 * p + 1)] = split ? uplb[q] : 1.0;
+         ovub[(size_t)(2 * p + 1)] = split ? pbase[(size_t)q].y : 1.0;
       }
       int64_t nov = 0;
       RCCHK(stageOverlay(b, nv, ovptr.data(), ovcol.data(), ovlb.data(), ovub.data(), &nov));
@@ -5062,6 +5090,24 @@ int sap_probe(sap_bounds* b, const sap_params* params, int32_t nprobe, const int
          HIPCHK(hipMemcpyAsync(t1.data(), b->outbnd2, sizeof(double2) * (size_t)room, hipMemcpyDeviceToHost, s));
          HIPCHK(hipStreamSynchronize(s));
       }
+      // split probing: the probe columns' own bounds in both branches
+      std::vector<double2> x0, x1;
+      if( split )
+      {
+         std::vector<int32_t> xv((size_t)nv), xc((size_t)nv);
+         for( int32_t v = 0; v < nv; ++v )
+         {
+            xv[(size_t)v] = v;
+            xc[(size_t)v] = probecol[p0 + v / 2];
+         }
+         x0.resize((size_t)nv);
+         HIPCHK(hipMemcpyAsync(b->pcol, xc.data(), sizeof(int32_t) * (size_t)nv, hipMemcpyHostToDevice, s));
+         HIPCHK(hipMemcpyAsync(b->pcount, xv.data(), sizeof(int32_t) * (size_t)nv, hipMemcpyHostToDevice, s));
+         k_gather_bounds<<<gridFor((size_t)nv), SAP_BLOCK, 0, s>>>(b->V.bnd, nc, b->pcol, b->pcount, nv, b->outbnd);
+         HIPCHK(hipGetLastError());
+         HIPCHK(hipMemcpyAsync(x0.data(), b->outbnd, sizeof(double2) * (size_t)nv, hipMemcpyDeviceToHost, s));
+         HIPCHK(hipStreamSynchronize(s));
+      }
       // verdicts and rounds; the change logs are cleared for the next chunk
       ident.resize((size_t)nv);
       res.resize((size_t)nv);
@@ -5088,13 +5134,29 @@ int sap_probe(sap_bounds* b, const sap_params* params, int32_t nprobe, const int
          for( int32_t k = 0; k < pc[(size_t)p]; ++k )
             idx[(size_t)k] = a + k;
          std::sort(idx.begin(), idx.end(), [&](int64_t u, int64_t w) { return tc[(size_t)u] < tc[(size_t)w]; });
+         const int32_t x = probecol[q];
+         const bool addx = split && verdict[q] == 0;
+         bool xdone = !addx;
          for( int64_t k : idx )
          {
+            if( !xdone && x < tc[(size_t)k] )
+            {
+               hcol.push_back(x);
+               hb0.push_back(x0[(size_t)(2 * p)]);
+               hb1.push_back(x0[(size_t)(2 * p + 1)]);
+               xdone = true;
+            }
             hcol.push_back(tc[(size_t)k]);
             hb0.push_back(t0[(size_t)k]);
             hb1.push_back(t1[(size_t)k]);
          }
-         cnt[(size_t)q + 1] = pc[(size_t)p];
+         if( !xdone )
+         {
+            hcol.push_back(x);
+            hb0.push_back(x0[(size_t)(2 * p)]);
+            hb1.push_back(x0[(size_t)(2 * p + 1)]);
+         }
+         cnt[(size_t)q + 1] = pc[(size_t)p] + (addx ? 1 : 0);
       }
    }
    for( int32_t p = 0; p < nprobe; ++p )
@@ -5119,6 +5181,26 @@ int sap_probe(sap_bounds* b, const sap_params* params, int32_t nprobe, const int
       dub1[k] = hb1[k].y;
    }
    return SAP_OKAY;
+}
+
+int sap_probe(sap_bounds* b, const sap_params* params, int32_t nprobe, const int32_t* probecol,
+   const sap_bounds* base, int32_t basevec, int32_t* verdict, int32_t* rounds, int64_t dedcap, int64_t* dedptr,
+   int32_t* dedcol, double* dlb0, double* dub0, double* dlb1, double* dub1, sap_stats* stats)
+{
+   return probeCore(b, params, nprobe, probecol, NULL, NULL, base, basevec, verdict, rounds, dedcap, dedptr, dedcol,
+      dlb0, dub0, dlb1, dub1, stats);
+}
+
+int sap_probe_split(sap_bounds* b, const sap_params* params, int32_t nprobe, const int32_t* probecol,
+   const double* downub, const double* uplb, const sap_bounds* base, int32_t basevec, int32_t* verdict,
+   int32_t* rounds, int64_t dedcap, int64_t* dedptr, int32_t* dedcol, double* dlb0, double* dub0, double* dlb1,
+   double* dub1, sap_stats* stats)
+{
+   if( nprobe > 0 && (downub == NULL || uplb == NULL) )
+      return SAP_INVALIDDATA;
+   static const double none = 0.0;
+   return probeCore(b, params, nprobe, probecol, downub != NULL ? downub : &none, uplb != NULL ? uplb : &none, base,
+      basevec, verdict, rounds, dedcap, dedptr, dedcol, dlb0, dub0, dlb1, dub1, stats);
 }
 
 int sap_propagate_packed(sap_matrix* m, const sap_params* params, double* lb, double* ub, int32_t* result,

@@ -139,3 +139,41 @@ def test_probe_real_instances(gpu_ctx, name):
     b.close()
     base.close()
     mat.close()
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_split_probe_matches_oracle(gpu_ctx, seed):
+    # strong-branching children x <= floor(mid) / x >= floor(mid) + 1 of general-integer columns
+    from tests.util import planted_integer_instance
+    inst = planted_integer_instance(np.random.default_rng(960 + seed), 400, 300, 6)
+    om = orc.OMatrix.from_instance(inst)
+    # from the unpropagated bounds (every row swept first): both infeasible children and
+    # long pair lists occur
+    rl, ru = inst.lb, inst.ub
+    cols = np.nonzero(ru - rl >= 1)[0].astype(np.int32)
+    down = np.floor((rl[cols] + ru[cols]) / 2)
+    up = down + 1
+    say(f"[case] split probe seed={seed} nprobe={len(cols)}")
+    mat = sa.Matrix(gpu_ctx, inst)
+    base = sa.Bounds(mat, 1)
+    base.upload(rl, ru)
+    b = sa.Bounds(mat, 32)
+    out = b.probe(base, 0, cols, sa.default_params(flags=0), downub=down, uplb=up)
+    ref = orc.probe(om, rl, ru, cols, downub=down, uplb=up)
+    for p, r in enumerate(ref):
+        assert out["verdict"][p] == r["verdict"], p
+        assert tuple(out["rounds"][p]) == r["rounds"], p
+        a, e = out["dedptr"][p], out["dedptr"][p + 1]
+        assert np.array_equal(out["dedcol"][a:e], r["cols"]), p
+        for k, (lo, hi) in (("0", ("dlb0", "dub0")), ("1", ("dlb1", "dub1"))):
+            got = np.stack([out[lo][a:e], out[hi][a:e]], 1)
+            assert np.array_equal(got.view(np.int64), r["b" + k].view(np.int64)), (p, k)
+    assert any(r["verdict"] for r in ref) and sum(len(r["cols"]) for r in ref) > len(ref)
+    # a split outside the base bounds, or down >= up, is rejected
+    for dn, u in ((down[0] + 1, up[0]), (rl[cols[0]] - 1, up[0])):
+        with pytest.raises(sa.SapError) as e:
+            b.probe(base, 0, cols[:1], downub=np.array([dn]), uplb=np.array([u]))
+        assert e.value.rc == sa.SAP_INVALIDDATA
+    b.close()
+    base.close()
+    mat.close()

@@ -84,3 +84,49 @@ def test_enumeration_cases_are_not_vacuous():
         pairs += sum(len(r["cols"]) for r in res)
         verdicts += sum(r["verdict"] != 0 for r in res)
     assert pairs > 20 and verdicts > 5, (pairs, verdicts)
+
+
+def test_split_known_answer():
+    # x + y = 4 with x, y integers in [0, 4]; branches x <= 1 and x >= 2
+    inst = make_instance([([(0, 1.0), (1, 1.0)], 4.0, 4.0)], [0.0, 0.0], [4.0, 4.0], [1, 1])
+    om = orc.OMatrix.from_instance(inst)
+    (r,) = orc.probe(om, inst.lb, inst.ub, [0], downub=[1.0], uplb=[2.0])
+    assert r["verdict"] == 0
+    # the probe column itself and y, changed in both branches
+    assert list(r["cols"]) == [0, 1]
+    assert r["b0"].tolist() == [[0.0, 1.0], [3.0, 4.0]]
+    assert r["b1"].tolist() == [[2.0, 4.0], [0.0, 2.0]]
+
+
+def integer_points(inst):
+    doms = [range(int(l), int(u) + 1) for l, u in zip(inst.lb, inst.ub)]
+    pts = np.array(list(itertools.product(*doms)), dtype=float)
+    ok = np.ones(len(pts), dtype=bool)
+    for i in range(len(inst.lhs)):
+        a, e = inst.rowptr[i], inst.rowptr[i + 1]
+        act = pts[:, inst.colidx[a:e]] @ inst.vals[a:e]
+        ok &= (act >= inst.lhs[i] - 1e-9) & (act <= inst.rhs[i] + 1e-9)
+    return pts[ok]
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_split_probing_sound_against_enumeration(seed):
+    from tests.util import planted_integer_instance
+    inst = planted_integer_instance(np.random.default_rng(1500 + seed), 6, 6, 4, coefrange=4, boundrange=3)
+    om = orc.OMatrix.from_instance(inst)
+    F = integer_points(inst)
+    cols = [j for j in range(6) if inst.ub[j] > inst.lb[j]]
+    d = [float(np.floor((inst.lb[j] + inst.ub[j]) / 2)) for j in cols]
+    res = orc.probe(om, inst.lb, inst.ub, cols, downub=d, uplb=[v + 1.0 for v in d])
+    for q, (x, r) in enumerate(zip(cols, res)):
+        down = F[F[:, x] <= d[q]]
+        up = F[F[:, x] >= d[q] + 1]
+        if r["verdict"] & 1:
+            assert len(down) == 0
+        if r["verdict"] & 2:
+            assert len(up) == 0
+        if r["verdict"] == 0:
+            assert x in r["cols"]
+        for k, j in enumerate(r["cols"]):
+            for pts, b in ((down, r["b0"][k]), (up, r["b1"][k])):
+                assert np.all((pts[:, j] >= b[0] - 1e-9) & (pts[:, j] <= b[1] + 1e-9)), (seed, x, j)
