@@ -102,3 +102,61 @@ def test_nodes(gpu_ctx):
         nodes.close()
     base.close()
     mat.close()
+
+
+@pytest.mark.parametrize("mode", [sa.SAP_MODE_DENSE, sa.SAP_MODE_SPARSE])
+def test_slots(gpu_ctx, mode):
+    # continuous batching with ranged rows: nodes admitted over several steps, each equal to the
+    # oracle's result for that node alone
+    inst = ranged_system(np.random.default_rng(1311), 1500, 3000, slack=False)
+    om = orc.OMatrix.from_instance(inst)
+    prm = orc.params(rangedrows=1)
+    rl, ru, rr = orc.propagate_jacobi(om, inst.lb, inst.ub, prm)
+    free = np.nonzero(rl < ru)[0]
+    rng = np.random.default_rng(9)
+    nnode = 30
+    ptr, cols, vals = random_fixings(rng, len(free), nnode, 4)
+    cols = free[cols].astype(np.int32)
+    vals = np.array([rl[c] if v == 0 else ru[c] for c, v in zip(cols, vals)])
+    say(f"[case] ranged slots mode={mode}")
+    mat = sa.Matrix(gpu_ctx, inst)
+    base = sa.Bounds(mat, 1)
+    base.upload(rl, ru)
+    sl = sa.Bounds(mat, 8, mode)
+    p = sa.default_params(flags=RANGED)
+    pending = list(range(nnode))
+    free_slots = list(range(8))
+    owner = {}
+    results = {}
+    for _ in range(2000):
+        while pending and free_slots:
+            v = pending.pop(0)
+            s = free_slots.pop(0)
+            owner[s] = v
+            sl.admit([s], base, 0, [0, ptr[v + 1] - ptr[v]], cols[ptr[v]:ptr[v + 1]], vals[ptr[v]:ptr[v + 1]],
+                     vals[ptr[v]:ptr[v + 1]])
+        fin, _ = sl.step(p, rounds=sa.SAP_STEP_SHORT)
+        if len(fin):
+            c = sl.collect(fin)
+            for i, s in enumerate(fin.tolist()):
+                v = owner.pop(s)
+                a, e = c["chgptr"][i], c["chgptr"][i + 1]
+                results[v] = (int(c["result"][i]), c["chgcol"][a:e].copy(), c["chglb"][a:e].copy(),
+                              c["chgub"][a:e].copy())
+                free_slots.append(s)
+        if len(results) == nnode:
+            break
+    assert len(results) == nnode
+    for v in range(nnode):
+        lb0, ub0 = apply_fixings(rl, ru, ptr, cols, vals, v)
+        ol, ou, orr = orc.propagate_jacobi(om, lb0, ub0, prm)
+        res, cc, cl, cu = results[v]
+        assert (res == sa.SAP_CUTOFF) == bool(orr["cutoff"]), v
+        if not orr["cutoff"]:
+            gl, gu = lb0.copy(), ub0.copy()
+            gl[cc] = cl
+            gu[cc] = cu
+            assert np.array_equal(gl, ol) and np.array_equal(gu, ou), v
+    sl.close()
+    base.close()
+    mat.close()
