@@ -113,6 +113,7 @@ typedef struct sap_tuning
    int32_t longrow;     /* 64..2^24, default 1024 */
    int32_t capscale;    /* 10..1000, default 50 */
    int32_t rowbins;     /* 0..3, default 0 */
+   int32_t tilewidth;   /* nonzeros per lane in a tile: 1 or 2, default 2 */
 } sap_tuning;
 void sap_tuning_default(sap_tuning* tuning);
 int sap_context_set_tuning(sap_context* ctx, const sap_tuning* tuning);

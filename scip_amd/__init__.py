@@ -78,7 +78,8 @@ class sap_params(C.Structure):
 
 
 class sap_tuning(C.Structure):
-    _fields_ = [("tilebins", C.c_int32), ("longrow", C.c_int32), ("capscale", C.c_int32), ("rowbins", C.c_int32)]
+    _fields_ = [("tilebins", C.c_int32), ("longrow", C.c_int32), ("capscale", C.c_int32), ("rowbins", C.c_int32),
+                ("tilewidth", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -378,8 +379,9 @@ class Context:
         _chk(lib().sap_context_create(device, C.byref(self.h)), "sap_context_create")
         _LIVE["context"].add(self)
 
-    def set_tuning(self, tilebins: int = 5, longrow: int = 1024, capscale: int = 50, rowbins: int = 0):
-        t = sap_tuning(tilebins, longrow, capscale, rowbins)
+    def set_tuning(self, tilebins: int = 5, longrow: int = 1024, capscale: int = 50, rowbins: int = 0,
+                   tilewidth: int = 2):
+        t = sap_tuning(tilebins, longrow, capscale, rowbins, tilewidth)
         _chk(lib().sap_context_set_tuning(self.h, C.byref(t)), "sap_context_set_tuning")
 
     def tuning(self):

@@ -116,6 +116,9 @@ struct Mat
    const unsigned long long* tmask;
    const int32_t* trow0;
    const uint8_t* tlev;     // levels of the shift tree: ceil(log2(longest row of the tile))
+   const unsigned long long* todd;   // tile width 2: bit i set where lane i starts a row of odd length
+   int32_t twidth;          // nonzeros per lane in a tile (1 or 2)
+   int32_t pad3;
    // independent instances packed block-diagonally: rows and columns carry their instance
    // (segment); verdicts, cutoffs and round counts are kept per (vector, segment) unit
    int32_t nseg;
@@ -1422,6 +1425,151 @@ __device__ __forceinline__ void sweepTiles(const Mat& M, const Vec& V, const Par
 }
 
 // ------------------------------------------------------------------------------------------
+// sweep of 128-position tiles (dense mode, tile width 2): lane i holds positions 2i and 2i+1
+// of its tile, every row starts at an even position (a row of odd length leaves its last
+// position empty), so both positions of a lane belong to one row.  The leaf pair of the
+// canonical tree (positions 2q, 2q+1 of a row) is added inside the lane; the levels above it
+// are the shift tree of the width-1 tiles over lanes: lane q of a row adds lane q+w when q is
+// aligned to 2w and q+w is still inside the row.  Per 64 lanes the row decision, the tree
+// levels and the tile bookkeeping thus serve up to 128 nonzeros.
+// ------------------------------------------------------------------------------------------
+template <bool R>
+__device__ __forceinline__ void sweepTiles2(const Mat& M, const Vec& V, const Params& P, const Items& I, int lblock,
+   int nblocks, unsigned long long& nnzacc, unsigned long long& rowsacc, WaveQueue& Q)
+{
+   const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
+   const int wid = lblock * (SAP_BLOCK / SAP_WAVE) + (int)threadIdx.x / SAP_WAVE;
+   const uint32_t nw = (uint32_t)nblocks * (SAP_BLOCK / SAP_WAVE);
+   const uint32_t ntiles = (uint32_t)M.ntiles;
+   const uint32_t ntot = (uint32_t)V.nvec * ntiles;
+   const uint64_t lemask = (lane == 63) ? ~0ULL : ((2ULL << lane) - 1);
+   for( uint32_t t = (uint32_t)wid; t < ntot; t += nw )
+   {
+      const uint32_t tu = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+      const int32_t v = (int32_t)(tu / ntiles);
+      const int32_t tile = (int32_t)(tu - (uint32_t)v * ntiles);
+      if( M.rowseg == NULL && V.vcutoff[v] && !R )
+         continue;
+      const int64_t e0 = M.tstart[tile];
+      const uint64_t mask = M.tmask[tile];
+      const uint64_t odd = M.todd[tile];
+      const int lanes = ((int)(M.tstart[tile + 1] - e0) + __popcll(odd)) >> 1;
+      const int32_t row0 = M.trow0[tile];
+      const int lev = M.tlev[tile];
+      const bool has = lane < lanes;
+      // my row: start lane sl, end lane ns (exclusive), lane position q, length L
+      const uint64_t le = mask & lemask;
+      const int k = __popcll(le) - 1;
+      const int sl = 63 - __clzll((long long)le);
+      const uint64_t above = (sl >= 63) ? 0ULL : (mask >> (sl + 1));
+      const int ns = above ? (sl + 1 + __ffsll((unsigned long long)above) - 1) : lanes;
+      const int Ll = ns - sl;
+      const int L = 2 * Ll - (int)((odd >> sl) & 1);
+      const int q = lane - sl;
+      const int64_t k0 = e0 + 2 * lane - __popcll(odd & ((1ULL << sl) - 1));
+      const bool has1 = 2 * q + 1 < L;
+      const int32_t row = row0 + k;
+      const bool marked = has
+         && (I.firstall || V.rowmark[(size_t)v * (size_t)M.nrows + (size_t)row] == I.round)
+         && (M.rowseg == NULL || V.vcutoff[unitOfRow(M, v, row)] == 0);
+      if( __ballot(marked) == 0 )
+         continue;
+
+      const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
+      Elem e, f;
+      e.a = 0.0;
+      e.lb = 0.0;
+      e.ub = 0.0;
+      e.j = 0;
+      e.cf = 0;
+      f = e;
+      bool allbin = false;
+      double2 sides = make_double2(0.0, 0.0);
+      if( marked )
+      {
+         e = loadElem(M, bnd, k0);
+         if( has1 )
+            f = loadElem(M, bnd, k0 + 1);
+         allbin = (M.rowflags[row] & 1) != 0;
+         sides = M.sides[row];
+      }
+      double cmin = 0.0, cmax = 0.0, mad = 0.0;
+      int kmin = 0, kmax = 0, kmin1 = 0, kmax1 = 0;
+      bool madinf = false;
+      DD dmn, dmx;
+      if( marked )
+      {
+         elemContrib(P, e, allbin, cmin, cmax, kmin, kmax, mad, madinf);
+         dmn = { cmin, 0.0 };
+         dmx = { cmax, 0.0 };
+         if( has1 )
+         {
+            double fmin, fmax, fmad;
+            bool finf;
+            elemContrib(P, f, allbin, fmin, fmax, kmin1, kmax1, fmad, finf);
+            dmn = pairAdd(dmn, DD{ fmin, 0.0 });
+            dmx = pairAdd(dmx, DD{ fmax, 0.0 });
+            mad = dmax(mad, fmad);
+            madinf = madinf || finf;
+         }
+      }
+      else
+      {
+         dmn = { 0.0, 0.0 };
+         dmx = { 0.0, 0.0 };
+      }
+      for( int w = 1; w < (1 << lev) / 2; w <<= 1 )
+      {
+         const DD omn = shflDownDD(dmn, w);
+         const DD omx = shflDownDD(dmx, w);
+         const double omad = __shfl_down(mad, w, SAP_WAVE);
+         if( (q & (2 * w - 1)) == 0 && q + w < Ll )
+         {
+            dmn = pairAdd(dmn, omn);
+            dmx = pairAdd(dmx, omx);
+            mad = dmax(mad, omad);
+         }
+      }
+      RowState rs;
+      rs.minact = shflDD(dmn, sl);
+      rs.maxact = shflDD(dmx, sl);
+      rs.mad = __shfl(mad, sl, SAP_WAVE);
+      const uint64_t rowbits = ((Ll >= 64) ? ~0ULL : ((1ULL << Ll) - 1)) << sl;
+#pragma unroll
+      for( int c = 0; c < 4; ++c )
+      {
+         rs.c[c] = __popcll(__ballot(kmin == c + 1) & rowbits) + __popcll(__ballot(kmin1 == c + 1) & rowbits);
+         rs.c[4 + c] = __popcll(__ballot(kmax == c + 1) & rowbits) + __popcll(__ballot(kmax1 == c + 1) & rowbits);
+      }
+      if( (__ballot(madinf) & rowbits) != 0 )
+         rs.mad = P.infinity;
+      const RowDecision d = decideRow(P, rs, sides.x, sides.y, L);
+      const bool scr = marked && d.tighten;
+      const bool pass = scr && screenElem(P, rs, d, sides.x, sides.y, e);
+      const bool pass1 = scr && has1 && screenElem(P, rs, d, sides.x, sides.y, f);
+      if( marked && q == 0 )
+      {
+         CtxT<R> C;
+         C.M = &M;
+         C.V = &V;
+         C.P = &P;
+         C.v = v;
+         C.round = I.round;
+         C.row = row;
+         rowOutcome(C, d);
+         nnzacc += (unsigned long long)L;
+         rowsacc += 1;
+         if( !V.single )
+            noteRound(M, V, v, row, I.round);
+      }
+      pushQueue(M, V, P, Q, pass, k0, row, v, rs, d, I.round);
+      maybeDrain<R>(M, V, P, Q, I.round);
+      pushQueue(M, V, P, Q, pass1, k0 + 1, row, v, rs, d, I.round);
+      maybeDrain<R>(M, V, P, Q, I.round);
+   }
+}
+
+// ------------------------------------------------------------------------------------------
 // sweep of one long row by one wavefront: the chunks [c0, c0+nch) of 64 positions form an
 // aligned subtree of the canonical tree; the wave returns its root (all lanes) and the
 // per-lane counters / maxactdelta contributions.
@@ -1778,7 +1926,10 @@ __global__ __launch_bounds__(SAP_BLOCK, 4) void k_sweep(Mat M, Vec V, Params P, 
    switch( kind )
    {
    case 0:
-      sweepTiles<R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+      if( M.twidth == 2 )
+         sweepTiles2<R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+      else
+         sweepTiles<R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
       // empty rows (first round only; they are never marked again): activity 0 against the
       // sides, propagateCons's infeasibility test (cons_linear.c:7721-7742)
       if( I.firstall && lblock == 0 )
@@ -2978,6 +3129,7 @@ struct sap_matrix
    unsigned long long* tmask;
    int32_t* trow0;
    uint8_t* tlev;
+   unsigned long long* todd;
    sap_tuning tuning;    // row binning and launch shape it was built with
    int32_t nseg;         // packed instances (1: a single instance)
    int32_t* rowseg;      // device, by device row position (NULL when nseg == 1)
@@ -3154,6 +3306,9 @@ static Mat toMat(const sap_matrix* m)
    M.tmask = m->tmask;
    M.trow0 = m->trow0;
    M.tlev = m->tlev;
+   M.todd = m->todd;
+   M.twidth = m->tuning.tilewidth;
+   M.pad3 = 0;
    M.nseg = m->nseg;
    M.rowbins = m->tuning.rowbins;
    M.rowseg = m->nseg > 1 ? m->rowseg : NULL;
@@ -3218,12 +3373,14 @@ void sap_tuning_default(sap_tuning* t)
    t->longrow = 1024;
    t->capscale = 50;
    t->rowbins = 0;
+   t->tilewidth = 2;
 }
 
 int sap_context_set_tuning(sap_context* c, const sap_tuning* t)
 {
    if( c == NULL || t == NULL || t->tilebins < 0 || t->tilebins > 5 || t->longrow < 64 || t->longrow > (1 << 24)
-      || t->capscale < 10 || t->capscale > 1000 || t->rowbins < 0 || t->rowbins > 3 )
+      || t->capscale < 10 || t->capscale > 1000 || t->rowbins < 0 || t->rowbins > 3
+      || t->tilewidth < 1 || t->tilewidth > 2 )
       return SAP_INVALIDDATA;
    c->tuning = *t;
    return SAP_OKAY;
@@ -3330,6 +3487,7 @@ static void matrixRelease(sap_matrix* m)
    (void)hipFree(m->tmask);
    (void)hipFree(m->trow0);
    (void)hipFree(m->tlev);
+   (void)hipFree(m->todd);
    (void)hipFree(m->rowseg);
    (void)hipFree(m->colseg);
    delete m->rowoff;
@@ -3459,26 +3617,31 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
    // length keep their bin order, so the layout is deterministic.
    const int32_t tfirst = binstart[1 + std::min(tune.rowbins, tune.tilebins)];
    const int32_t tend = std::max(tfirst, binstart[1 + tune.tilebins]);
+   // With tile width 2 a row takes its length rounded up to even positions of a 128-position
+   // tile.
+   const int tw = tune.tilewidth;
    std::vector<int32_t> tilerows;   // rows per tile, in tile order
    if( tend > tfirst )
    {
-      std::vector<std::vector<int32_t>> bylen(SAP_WAVE + 1);
+      const int cap = SAP_WAVE * tw;
+      std::vector<std::vector<int32_t>> bysize((size_t)SAP_WAVE + 2);
       for( int32_t r = tfirst; r < tend; ++r )
       {
          const int32_t i = perm[(size_t)r];
-         bylen[(size_t)(rowptr[i + 1] - rowptr[i])].push_back(i);
+         const int64_t L = rowptr[i + 1] - rowptr[i];
+         bysize[(size_t)((L + tw - 1) / tw * tw)].push_back(i);
       }
-      std::vector<size_t> head(SAP_WAVE + 1, 0);
+      std::vector<size_t> head((size_t)SAP_WAVE + 2, 0);
       int32_t w = tfirst;
       for( int big = SAP_WAVE; big >= 1; --big )
-         while( head[(size_t)big] < bylen[(size_t)big].size() )
+         while( head[(size_t)big] < bysize[(size_t)big].size() )
          {
-            perm[(size_t)w++] = bylen[(size_t)big][head[(size_t)big]++];
-            int space = SAP_WAVE - big, nr = 1;
+            perm[(size_t)w++] = bysize[(size_t)big][head[(size_t)big]++];
+            int space = cap - big, nr = 1;
             for( int l = std::min(space, big); l >= 1 && space > 0; --l )
-               while( space >= l && head[(size_t)l] < bylen[(size_t)l].size() )
+               while( space >= l && head[(size_t)l] < bysize[(size_t)l].size() )
                {
-                  perm[(size_t)w++] = bylen[(size_t)l][head[(size_t)l]++];
+                  perm[(size_t)w++] = bysize[(size_t)l][head[(size_t)l]++];
                   space -= l;
                   ++nr;
                }
@@ -3541,6 +3704,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
    std::vector<unsigned long long> tmask;
    std::vector<int32_t> trow0;
    std::vector<uint8_t> tlev;
+   std::vector<unsigned long long> todd;
    {
       auto levels = [](int l) {
          int v = 0;
@@ -3553,16 +3717,19 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
       {
          tstart.push_back(prowptr[(size_t)r]);
          trow0.push_back(r);
-         unsigned long long mask = 0;
-         int cnt = 0, maxl = 0;
+         unsigned long long mask = 0, odd = 0;
+         int lanes = 0, maxl = 0;
          for( int32_t q = 0; q < nr; ++q, ++r )
          {
             const int L = (int)(prowptr[(size_t)r + 1] - prowptr[(size_t)r]);
-            mask |= 1ULL << cnt;
-            cnt += L;
+            mask |= 1ULL << lanes;
+            if( tw == 2 && (L & 1) )
+               odd |= 1ULL << lanes;
+            lanes += (L + tw - 1) / tw;
             maxl = L > maxl ? L : maxl;
          }
          tmask.push_back(mask);
+         todd.push_back(odd);
          tlev.push_back(levels(maxl));
       }
       tstart.push_back(prowptr[(size_t)tend]);
@@ -3626,6 +3793,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
    MALLOC_OR_FAIL(m->tmask, tmask.size());
    MALLOC_OR_FAIL(m->trow0, trow0.size());
    MALLOC_OR_FAIL(m->tlev, tlev.size());
+   MALLOC_OR_FAIL(m->todd, todd.size());
    m->nseg = nseg;
    if( nseg > 1 )
    {
@@ -3663,6 +3831,8 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
       HIPCHK(hipMemcpyAsync(m->tmask, tmask.data(), sizeof(unsigned long long) * tmask.size(), hipMemcpyHostToDevice, s));
       HIPCHK(hipMemcpyAsync(m->trow0, trow0.data(), sizeof(int32_t) * trow0.size(), hipMemcpyHostToDevice, s));
       HIPCHK(hipMemcpyAsync(m->tlev, tlev.data(), tlev.size(), hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(m->todd, todd.data(), sizeof(unsigned long long) * todd.size(), hipMemcpyHostToDevice,
+         s));
    }
    if( nseg > 1 )
    {

@@ -14,15 +14,15 @@ def say(*a):
     print(*a, flush=True)
 
 
-@pytest.mark.parametrize("tilebins,longrow,capscale,rowbins", [(5, 4096, 100, 0), (0, 4096, 100, 0), (3, 64, 10, 0),
-                                                               (4, 1024, 200, 0), (1, 128, 1000, 0), (5, 65536, 50, 0),
-                                                               (5, 4096, 100, 1), (5, 4096, 100, 2), (5, 4096, 100, 3),
-                                                               (2, 4096, 100, 3), (0, 256, 100, 3)])
-def test_tunings_give_identical_results(gpu_ctx, tilebins, longrow, capscale, rowbins):
+@pytest.mark.parametrize("tilebins,longrow,capscale,rowbins,tilewidth", [
+    (5, 4096, 100, 0, 2), (5, 4096, 100, 0, 1), (0, 4096, 100, 0, 2), (3, 64, 10, 0, 2), (3, 64, 10, 0, 1),
+    (4, 1024, 200, 0, 2), (1, 128, 1000, 0, 2), (1, 128, 1000, 0, 1), (5, 65536, 50, 0, 2), (5, 4096, 100, 1, 2),
+    (5, 4096, 100, 2, 2), (5, 4096, 100, 3, 2), (5, 4096, 100, 3, 1), (2, 4096, 100, 3, 2), (0, 256, 100, 3, 2)])
+def test_tunings_give_identical_results(gpu_ctx, tilebins, longrow, capscale, rowbins, tilewidth):
     inst = sa.gen_mixed(20000, 20000, seed=31, planted=True, maxlen=3000)
     ol, ou, orr = orc.propagate_jacobi(orc.OMatrix.from_instance(inst), inst.lb, inst.ub)
-    say(f"[case] tuning {tilebins} {longrow} {capscale} {rowbins}")
-    gpu_ctx.set_tuning(tilebins, longrow, capscale, rowbins)
+    say(f"[case] tuning {tilebins} {longrow} {capscale} {rowbins} {tilewidth}")
+    gpu_ctx.set_tuning(tilebins, longrow, capscale, rowbins, tilewidth)
     try:
         mat = sa.Matrix(gpu_ctx, inst)
         for mode in (sa.SAP_MODE_DENSE, sa.SAP_MODE_SPARSE):
@@ -47,4 +47,4 @@ def test_autotune_returns_a_valid_tuning(gpu_ctx):
     assert best["tilebins"] in (3, 4, 5) and best["longrow"] in (1024, 4096, 16384) and best["rowbins"] in (0, 2, 3)
     assert best["capscale"] in (50, 100, 200)
     assert 0 < bms
-    assert gpu_ctx.tuning() == {"tilebins": 5, "longrow": 1024, "capscale": 50, "rowbins": 0}
+    assert gpu_ctx.tuning() == {"tilebins": 5, "longrow": 1024, "capscale": 50, "rowbins": 0, "tilewidth": 2}
