@@ -100,16 +100,17 @@ int sap_context_create(int32_t device, sap_context** ctx);
 int sap_context_free(sap_context* ctx);
 
 /* Row binning and launch shape of the matrices a context creates.  Rows are grouped in bins
- * by length (at most 4, 8, 16, 32, 64 nonzeros for bins 1..5, then longer).  The rows of bins
- * 1..rowbins are swept one row per lane; in the one-vector (dense) work mode the rows of bins
- * rowbins+1..tilebins are swept in 64-nonzero tiles; groups of lanes take the other short
- * bins, one wavefront takes a row of up to longrow nonzeros and a whole workgroup a longer
- * one; capscale (percent) scales the workgroup count of each kind of sweep work.
+ * by length (at most 4, 8, 16, 32, 64, 128 nonzeros for bins 1..6, then longer).  The rows of
+ * bins 1..rowbins are swept one row per lane; in the work modes that sweep every row (dense)
+ * the rows of bins rowbins+1..tilebins are packed into tiles of 64 lanes holding tilewidth
+ * nonzeros each (bin 6 only with tilewidth 2); groups of lanes take the other short bins, one
+ * wavefront takes a row of up to longrow nonzeros and a whole workgroup a longer one;
+ * capscale (percent) scales the workgroup count of each kind of sweep work.
  * sap_autotune measures candidates on a row sample of a matrix.  No counterpart in SCIP
  * (whose propagation has no launch shape). */
 typedef struct sap_tuning
 {
-   int32_t tilebins;    /* 0..5, default 5 */
+   int32_t tilebins;    /* 0..6, default 6 (6 requires tilewidth 2) */
    int32_t longrow;     /* 64..2^24, default 1024 */
    int32_t capscale;    /* 10..1000, default 50 */
    int32_t rowbins;     /* 0..3, default 0 */
@@ -130,8 +131,9 @@ int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int6
    const uint8_t* integral, const double* glb, const double* gub, sap_matrix** mat);
 int sap_matrix_free(sap_matrix* mat);
 /* Autotuner: builds the system restricted to a sample of samplerows rows (all rows if fewer;
- * chosen by seed) once per candidate tuning ((rowbins, tilebins) in (0,3) (0,4) (0,5) (2,5)
- * (3,4) (3,5) x longrow 1024/4096/16384 x capscale 50/100/200), propagates lb/ub with params on it and returns the candidate with
+ * chosen by seed) once per candidate tuning ((rowbins, tilebins) in (0,3) (0,4) (0,5) (0,6)
+ * (2,6) (3,6) x longrow 1024/4096/16384 x capscale 50/100/200, tilewidth 2), propagates lb/ub
+ * with params on it and returns the candidate with
  * the least device time in best (times in ms: best_ms, default_ms for the default tuning).
  * Results are identical under every tuning; only the launch shape differs.  The context's
  * own tuning is left unchanged: apply the result with sap_context_set_tuning. */

@@ -39,7 +39,7 @@
 
 using namespace sap;
 
-#define SAP_NBINS 8
+#define SAP_NBINS 9
 #define SAP_NKINDS (SAP_NBINS + 1)
 #define SAP_BLOCK 256
 #define SAP_WAVE 64
@@ -47,9 +47,11 @@ using namespace sap;
 #define SAP_ROUND_CAP 10000000
 #define SAP_STAT_SLOTS 256
 
-// row-length bins: 0 empty rows; 1..5 at most 4,8,16,32,64 nonzeros (64-nonzero tiles in dense
-// mode, groups of 4,8,16,32,64 lanes in sparse mode); 6 wavefront per row; 7 workgroup per row
-static const int32_t kBinMaxLen[SAP_NBINS - 1] = { 0, 4, 8, 16, 32, 64, 4096 };
+// row-length bins: 0 empty rows; 1..5 at most 4,8,16,32,64 nonzeros (tiles in dense mode,
+// groups of 4,8,16,32,64 lanes in sparse mode); 6 at most 128 nonzeros (tiles of width 2 in
+// dense mode, else a wavefront per row); 7 wavefront per row; 8 workgroup per row (rows longer
+// than the tuning's longrow, from any of bins 6..8)
+static const int32_t kBinMaxLen[SAP_NBINS - 2] = { 0, 4, 8, 16, 32, 64, 128 };
 
 struct Status
 {
@@ -107,9 +109,10 @@ struct Mat
    const int64_t* colptr;   // CSC (device row ids) for row marking
    const int32_t* rowidx;
    int32_t binstart[SAP_NBINS + 1];
-   // tiles: the rows of the tiled bins packed first-fit decreasing into runs of at most 64
-   // consecutive nonzeros; tile t holds nonzeros [tstart[t], tstart[t+1]) and rows trow0[t]..,
-   // with bit i of tmask[t] set where nonzero tstart[t]+i starts a row
+   // tiles: the rows of the tiled bins packed first-fit decreasing into runs of consecutive
+   // rows of at most 64 lanes (tile width 1: a nonzero per lane; width 2: two, rows starting at
+   // even positions); tile t holds nonzeros [tstart[t], tstart[t+1]) and rows trow0[t]..,
+   // with bit i of tmask[t] set where lane i starts a row
    int32_t ntiles;
    int32_t pad;
    const int64_t* tstart;
@@ -1973,7 +1976,8 @@ __global__ __launch_bounds__(SAP_BLOCK, 4) void k_sweep(Mat M, Vec V, Params P, 
       break;
    case 5: sweepGroups<32, R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
    case 6: sweepGroups<64, R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
-   case 7: sweepWaves<R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
+   case 7:
+   case 8: sweepWaves<R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
    default: sweepBlocks<R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q); break;
    }
    // remaining queued candidates of this wavefront
@@ -3341,10 +3345,12 @@ static void parallelFor(int64_t n, int64_t grain, F fn)
 
 static inline int lenBin(int64_t L, int32_t longrow)
 {
-   for( int q = 0; q < SAP_NBINS - 2; ++q )
+   for( int q = 0; q < 6; ++q )
       if( L <= kBinMaxLen[q] )
          return q;
-   return L <= longrow ? SAP_NBINS - 2 : SAP_NBINS - 1;
+   if( L > longrow )
+      return SAP_NBINS - 1;
+   return L <= kBinMaxLen[6] ? 6 : 7;
 }
 
 extern "C" {
@@ -3369,7 +3375,7 @@ void sap_tuning_default(sap_tuning* t)
 {
    if( t == NULL )
       return;
-   t->tilebins = 5;
+   t->tilebins = 6;
    t->longrow = 1024;
    t->capscale = 50;
    t->rowbins = 0;
@@ -3378,9 +3384,9 @@ void sap_tuning_default(sap_tuning* t)
 
 int sap_context_set_tuning(sap_context* c, const sap_tuning* t)
 {
-   if( c == NULL || t == NULL || t->tilebins < 0 || t->tilebins > 5 || t->longrow < 64 || t->longrow > (1 << 24)
+   if( c == NULL || t == NULL || t->tilebins < 0 || t->tilebins > 6 || t->longrow < 64 || t->longrow > (1 << 24)
       || t->capscale < 10 || t->capscale > 1000 || t->rowbins < 0 || t->rowbins > 3
-      || t->tilewidth < 1 || t->tilewidth > 2 )
+      || t->tilewidth < 1 || t->tilewidth > 2 || (t->tilebins == 6 && t->tilewidth != 2) )
       return SAP_INVALIDDATA;
    c->tuning = *t;
    return SAP_OKAY;
@@ -3593,7 +3599,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
    if( bad )
       return SAP_INVALIDDATA;
    const int32_t hasforced = forced.load();
-   int64_t bincount[SAP_NBINS] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+   int64_t bincount[SAP_NBINS] = {};
    for( auto& c : tcount )
       for( int b = 0; b < SAP_NBINS; ++b )
          bincount[b] += c[(size_t)b];
@@ -3609,12 +3615,13 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
       for( int64_t i = 0; i < nrows; ++i )
          perm[(size_t)fill[obin[(size_t)i]]++] = (int32_t)i;
    }
-   // the tiled bins (1..64 nonzeros per row) are re-ordered tile by tile: first-fit decreasing
-   // on the row lengths, i.e. each tile is opened by the longest row left and filled with the
-   // longest rows that still fit.  A tile's cost is nearly independent of how many of its 64
-   // lanes are used, so fewer, fuller tiles mean less work; on log-normal row lengths this
-   // packs ~100 % of the lanes against ~82 % for consecutive rows in bin order.  Rows of one
-   // length keep their bin order, so the layout is deterministic.
+   // the tiled bins (1..64 nonzeros per row, 1..128 with tile width 2) are re-ordered tile by
+   // tile: first-fit decreasing on the row lengths, i.e. each tile is opened by the longest row
+   // left and filled with the longest rows that still fit.  A tile's cost is nearly
+   // independent of how many of its 64 lanes are used, so fewer, fuller tiles mean less work;
+   // on log-normal row lengths this packs ~100 % of the lanes against ~82 % for consecutive
+   // rows in bin order.  Rows of one length keep their bin order, so the layout is
+   // deterministic.
    const int32_t tfirst = binstart[1 + std::min(tune.rowbins, tune.tilebins)];
    const int32_t tend = std::max(tfirst, binstart[1 + tune.tilebins]);
    // With tile width 2 a row takes its length rounded up to even positions of a 128-position
@@ -3624,16 +3631,16 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
    if( tend > tfirst )
    {
       const int cap = SAP_WAVE * tw;
-      std::vector<std::vector<int32_t>> bysize((size_t)SAP_WAVE + 2);
+      std::vector<std::vector<int32_t>> bysize((size_t)cap + 2);
       for( int32_t r = tfirst; r < tend; ++r )
       {
          const int32_t i = perm[(size_t)r];
          const int64_t L = rowptr[i + 1] - rowptr[i];
          bysize[(size_t)((L + tw - 1) / tw * tw)].push_back(i);
       }
-      std::vector<size_t> head((size_t)SAP_WAVE + 2, 0);
+      std::vector<size_t> head((size_t)cap + 2, 0);
       int32_t w = tfirst;
-      for( int big = SAP_WAVE; big >= 1; --big )
+      for( int big = cap; big >= 1; --big )
          while( head[(size_t)big] < bysize[(size_t)big].size() )
          {
             perm[(size_t)w++] = bysize[(size_t)big][head[(size_t)big]++];
@@ -3699,7 +3706,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
       });
    }
 
-   // 64-nonzero tiles over the packed rows of the tiled bins (consecutive rows, <= 64 nonzeros)
+   // tiles over the packed rows of the tiled bins (consecutive rows, at most 64 lanes)
    std::vector<int64_t> tstart;
    std::vector<unsigned long long> tmask;
    std::vector<int32_t> trow0;
@@ -3911,8 +3918,8 @@ int sap_autotune(sap_context* ctx, int64_t nrows, int64_t ncols, const int64_t* 
    const sap_tuning saved = ctx->tuning;
    sap_tuning def;
    sap_tuning_default(&def);
-   static const int32_t rbs[] = { 0, 0, 0, 2, 3, 3 };
-   static const int32_t tbs[] = { 3, 4, 5, 5, 4, 5 };
+   static const int32_t rbs[] = { 0, 0, 0, 0, 2, 3 };
+   static const int32_t tbs[] = { 3, 4, 5, 6, 6, 6 };
    static const int32_t lrs[] = { 1024, 4096, 16384 };
    static const int32_t cps[] = { 50, 100, 200 };
    std::vector<double> wl((size_t)ncols), wu((size_t)ncols);
@@ -4277,8 +4284,8 @@ static Layout makeLayout(const sap_bounds* b)
    // items per workgroup and block caps per kind (kind 0 tiles, kind 1+b bin b); the caps
    // scale with the matrix's tuning
    static const int kPerBlock[SAP_NKINDS] = { SAP_BLOCK / SAP_WAVE, SAP_BLOCK / 4, SAP_BLOCK / 4, SAP_BLOCK / 8,
-      SAP_BLOCK / 16, SAP_BLOCK / 32, SAP_BLOCK / 64, SAP_BLOCK / SAP_WAVE, 1 };
-   static const int kCap[SAP_NKINDS] = { 4096, 2048, 2048, 2048, 2048, 2048, 2048, 2048, 1024 };
+      SAP_BLOCK / 16, SAP_BLOCK / 32, SAP_BLOCK / 64, SAP_BLOCK / SAP_WAVE, SAP_BLOCK / SAP_WAVE, 1 };
+   static const int kCap[SAP_NKINDS] = { 4096, 2048, 2048, 2048, 2048, 2048, 2048, 2048, 2048, 1024 };
    const int tb = m->tuning.tilebins, rb = m->tuning.rowbins;
    int acc = 0;
    for( int q = 0; q < SAP_NKINDS; ++q )
