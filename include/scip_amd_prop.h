@@ -103,18 +103,18 @@ int sap_context_free(sap_context* ctx);
  * by length (at most 4, 8, 16, 32, 64, 128 nonzeros for bins 1..6, then longer).  The rows of
  * bins 1..rowbins are swept one row per lane; in the work modes that sweep every row (dense)
  * the rows of bins rowbins+1..tilebins are packed into tiles of 64 lanes holding tilewidth
- * nonzeros each (bin 6 only with tilewidth 2); groups of lanes take the other short bins, one
- * wavefront takes a row of up to longrow nonzeros and a whole workgroup a longer one;
- * capscale (percent) scales the workgroup count of each kind of sweep work.
+ * nonzeros each (bin 6 only with tilewidth 2 or 4); groups of lanes take the other short
+ * bins, one wavefront takes a row of up to longrow nonzeros and a whole workgroup a longer
+ * one; capscale (percent) scales the workgroup count of each kind of sweep work.
  * sap_autotune measures candidates on a row sample of a matrix.  No counterpart in SCIP
  * (whose propagation has no launch shape). */
 typedef struct sap_tuning
 {
-   int32_t tilebins;    /* 0..6, default 6 (6 requires tilewidth 2) */
+   int32_t tilebins;    /* 0..6, default 6 (6 requires tilewidth 2 or 4) */
    int32_t longrow;     /* 64..2^24, default 1024 */
    int32_t capscale;    /* 10..1000, default 50 */
    int32_t rowbins;     /* 0..3, default 0 */
-   int32_t tilewidth;   /* nonzeros per lane in a tile: 1 or 2, default 2 */
+   int32_t tilewidth;   /* nonzeros per lane in a tile: 1, 2 or 4, default 4 */
 } sap_tuning;
 void sap_tuning_default(sap_tuning* tuning);
 int sap_context_set_tuning(sap_context* ctx, const sap_tuning* tuning);
@@ -132,7 +132,7 @@ int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int6
 int sap_matrix_free(sap_matrix* mat);
 /* Autotuner: builds the system restricted to a sample of samplerows rows (all rows if fewer;
  * chosen by seed) once per candidate tuning ((rowbins, tilebins) in (0,3) (0,4) (0,5) (0,6)
- * (2,6) (3,6) x longrow 1024/4096/16384 x capscale 50/100/200, tilewidth 2), propagates lb/ub
+ * (2,6) (3,6) x longrow 1024/4096/16384 x capscale 50/100/200, tilewidth 4), propagates lb/ub
  * with params on it and returns the candidate with
  * the least device time in best (times in ms: best_ms, default_ms for the default tuning).
  * Results are identical under every tuning; only the launch shape differs.  The context's
@@ -197,7 +197,8 @@ int sap_propagate_reasons(sap_matrix* mat, const sap_params* params, double* lb,
    int64_t* nhist, int32_t* hround, int32_t* hcol, int32_t* hbound, double* hval, int32_t* hrow, int32_t* hrule,
    sap_conflict* conflict, sap_stats* stats);
 
-/* Device-resident bound vector(s), for callers that keep bounds in HBM across calls. */
+/* Device-resident bound vector(s), for callers that keep bounds in HBM across calls
+ * (1 <= nvec < 2^30). */
 int sap_bounds_create(sap_matrix* mat, int32_t nvec, sap_bounds** bnd);
 /* as sap_bounds_create with an explicit work-tracking mode: SAP_MODE_DENSE scans round-tagged
  * row/column marks (no list atomics; best for one or few vectors), SAP_MODE_SPARSE keeps

@@ -380,7 +380,7 @@ class Context:
         _LIVE["context"].add(self)
 
     def set_tuning(self, tilebins: int = 6, longrow: int = 1024, capscale: int = 50, rowbins: int = 0,
-                   tilewidth: int = 2):
+                   tilewidth: int = 4):
         t = sap_tuning(tilebins, longrow, capscale, rowbins, tilewidth)
         _chk(lib().sap_context_set_tuning(self.h, C.byref(t)), "sap_context_set_tuning")
 

@@ -119,8 +119,9 @@ struct Mat
    const unsigned long long* tmask;
    const int32_t* trow0;
    const uint8_t* tlev;     // levels of the shift tree: ceil(log2(longest row of the tile))
-   const unsigned long long* todd;   // tile width 2: bit i set where lane i starts a row of odd length
-   int32_t twidth;          // nonzeros per lane in a tile (1 or 2)
+   const unsigned long long* tpad;   // wide tiles: two masks per tile; bit i of tpad[2t] + 2 * bit i
+                                     // of tpad[2t+1] = empty positions of the row starting at lane i
+   int32_t twidth;          // nonzeros per lane in a tile (1, 2 or 4)
    int32_t pad3;
    // independent instances packed block-diagonally: rows and columns carry their instance
    // (segment); verdicts, cutoffs and round counts are kept per (vector, segment) unit
@@ -495,9 +496,9 @@ __device__ __forceinline__ void touch(const CtxT<R>& C, size_t flat, int32_t j)
 {
    if( C.V->dense )
    {
-      // round tag scanned by k_columns; plain store, every writer stores the same value
-      if( C.V->coltag[flat] != C.round )
-         C.V->coltag[flat] = C.round;
+      // round tag scanned by k_columns; plain store (not read first), every writer stores the
+      // same value
+      C.V->coltag[flat] = C.round;
    }
    else
    {
@@ -536,8 +537,8 @@ __device__ __forceinline__ void submitUb(const CtxT<R>& C, int32_t j, double val
       if( k <= *slot && k <= atomicMin(slot, k) )
          logCand(C, j, k, 1, force);
    }
-   else if( k < *slot )
-      atomicMin(slot, k);
+   else
+      atomicMin(slot, k);   // result unused: the wavefront does not wait for it
    touch(C, flat, j);
 }
 
@@ -552,7 +553,7 @@ __device__ __forceinline__ void submitLb(const CtxT<R>& C, int32_t j, double val
       if( k >= *slot && k >= atomicMax(slot, k) )
          logCand(C, j, k, 0, force);
    }
-   else if( k > *slot )
+   else
       atomicMax(slot, k);
    touch(C, flat, j);
 }
@@ -912,13 +913,15 @@ struct WaveQueue
 {
    int64_t* k;              // nonzero index
    int32_t* row;
-   int32_t* v;
+   int32_t* v;              // vector index (< 2^30); bit 30 easy, bit 31 force
    double* mnhi;
    double* mnlo;
    double* mxhi;
    double* mxlo;
    unsigned long long* cnt; // eight counters, 8 bits each, saturated at 255
-   uint32_t* flags;         // bit0 easy, bit1 force
+   double* lb;              // the nonzero's column bounds (start of round) and column
+   double* ub;
+   int32_t* j;
    int head;
    int count;
 };
@@ -991,7 +994,9 @@ __device__ __forceinline__ void drainQueue(const Mat& M, const Vec& V, const Par
       const int q = (Q.head + lane) & (SAP_QCAP - 1);
       const int64_t k = Q.k[q];
       const int32_t row = Q.row[q];
-      const int32_t v = Q.v[q];
+      const uint32_t vf = (uint32_t)Q.v[q];
+      const int32_t v = (int32_t)(vf & 0x3FFFFFFFu);
+      const uint32_t fl = vf >> 30;
       RowState rs;
       rs.minact.hi = Q.mnhi[q];
       rs.minact.lo = Q.mnlo[q];
@@ -1002,7 +1007,6 @@ __device__ __forceinline__ void drainQueue(const Mat& M, const Vec& V, const Par
       for( int i = 0; i < 8; ++i )
          rs.c[i] = (int)((c >> (8 * i)) & 0xFF);
       rs.mad = 0.0;
-      const uint32_t fl = Q.flags[q];
       CtxT<R> C;
       C.M = &M;
       C.V = &V;
@@ -1010,8 +1014,12 @@ __device__ __forceinline__ void drainQueue(const Mat& M, const Vec& V, const Par
       C.v = v;
       C.round = round;
       C.row = row;
-      const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
-      const Elem e = loadElem(M, bnd, k);
+      Elem e;
+      e.a = M.vals[k];
+      e.cf = M.nzflags[k];
+      e.j = Q.j[q];
+      e.lb = Q.lb[q];
+      e.ub = Q.ub[q];
       const double2 sides = M.sides[row];
       if( fl & 1 )
          candEasy(C, rs, sides.x, sides.y, e, (fl & 2) != 0);
@@ -1028,7 +1036,7 @@ __device__ __forceinline__ void drainQueue(const Mat& M, const Vec& V, const Par
 
 // append the lanes with pred; drain a full wavefront of work when one is available
 __device__ __forceinline__ void pushQueue(const Mat& M, const Vec& V, const Params& P, WaveQueue& Q, bool pred,
-   int64_t k, int32_t row, int32_t v, const RowState& rs, const RowDecision& d, int32_t round)
+   int64_t k, const Elem& e, int32_t row, int32_t v, const RowState& rs, const RowDecision& d, int32_t round)
 {
    const uint64_t m = __ballot(pred);
    if( m == 0 )
@@ -1038,13 +1046,15 @@ __device__ __forceinline__ void pushQueue(const Mat& M, const Vec& V, const Para
       const int q = (Q.head + Q.count + __popcll(m & laneMaskLt())) & (SAP_QCAP - 1);
       Q.k[q] = k;
       Q.row[q] = row;
-      Q.v[q] = v;
+      Q.v[q] = (int32_t)((uint32_t)v | (d.easy ? 1u << 30 : 0u) | (d.force ? 1u << 31 : 0u));
+      Q.lb[q] = e.lb;
+      Q.ub[q] = e.ub;
+      Q.j[q] = e.j;
       Q.mnhi[q] = rs.minact.hi;
       Q.mnlo[q] = rs.minact.lo;
       Q.mxhi[q] = rs.maxact.hi;
       Q.mxlo[q] = rs.maxact.lo;
       Q.cnt[q] = packCounts(rs);
-      Q.flags[q] = (d.easy ? 1u : 0u) | (d.force ? 2u : 0u);
    }
    Q.count += __popcll(m);
 }
@@ -1172,7 +1182,7 @@ __device__ __forceinline__ void sweepGroups(const Mat& M, const Vec& V, const Pa
 
       RowDecision d = decideRow(P, rs, sides.x, sides.y, n);
       const bool pass = has && d.tighten && screenElem(P, rs, d, sides.x, sides.y, e);
-      pushQueue(M, V, P, Q, pass, s + glane, row, v, rs, d, I.round);
+      pushQueue(M, V, P, Q, pass, s + glane, e, row, v, rs, d, I.round);
       if( live && glane == 0 )
       {
          rowOutcome(C, d);
@@ -1298,7 +1308,7 @@ __device__ __forceinline__ void sweepRows(const Mat& M, const Vec& V, const Para
          if( act )
             e = loadElem(M, bnd, s + k);
          const bool pass = act && screenElem(P, rs, d, sides.x, sides.y, e);
-         pushQueue(M, V, P, Q, pass, s + k, row, v, rs, d, I.round);
+         pushQueue(M, V, P, Q, pass, s + k, e, row, v, rs, d, I.round);
          maybeDrain<R>(M, V, P, Q, I.round);
       }
    }
@@ -1414,7 +1424,7 @@ __device__ __forceinline__ void sweepTiles(const Mat& M, const Vec& V, const Par
       C.row = row;
       const RowDecision d = decideRow(P, rs, sides.x, sides.y, L);
       const bool pass = marked && d.tighten && screenElem(P, rs, d, sides.x, sides.y, e);
-      pushQueue(M, V, P, Q, pass, e0 + lane, row, v, rs, d, I.round);
+      pushQueue(M, V, P, Q, pass, e0 + lane, e, row, v, rs, d, I.round);
       if( marked && p == 0 )
       {
          rowOutcome(C, d);
@@ -1428,18 +1438,21 @@ __device__ __forceinline__ void sweepTiles(const Mat& M, const Vec& V, const Par
 }
 
 // ------------------------------------------------------------------------------------------
-// sweep of 128-position tiles (dense mode, tile width 2): lane i holds positions 2i and 2i+1
-// of its tile, every row starts at an even position (a row of odd length leaves its last
-// position empty), so both positions of a lane belong to one row.  The leaf pair of the
-// canonical tree (positions 2q, 2q+1 of a row) is added inside the lane; the levels above it
+// sweep of wide tiles (dense mode, tile width W = 2 or 4): lane i holds positions W*i ..
+// W*i+W-1 of its tile, every row starts at a position divisible by W (a row whose length is
+// not leaves the rest of its last lane empty), so all positions of a lane belong to one row.
+// The lowest log2(W) levels of the canonical tree are added inside the lane (positions 2p and
+// 2p+1, then 4p and 4p+2, each when the right one is still inside the row); the levels above
 // are the shift tree of the width-1 tiles over lanes: lane q of a row adds lane q+w when q is
 // aligned to 2w and q+w is still inside the row.  Per 64 lanes the row decision, the tree
-// levels and the tile bookkeeping thus serve up to 128 nonzeros.
+// levels and the tile bookkeeping thus serve up to 64*W nonzeros.
 // ------------------------------------------------------------------------------------------
-template <bool R>
-__device__ __forceinline__ void sweepTiles2(const Mat& M, const Vec& V, const Params& P, const Items& I, int lblock,
+template <bool R, int W>
+__device__ __forceinline__ void sweepTilesW(const Mat& M, const Vec& V, const Params& P, const Items& I, int lblock,
    int nblocks, unsigned long long& nnzacc, unsigned long long& rowsacc, WaveQueue& Q)
 {
+   static_assert(W == 2 || W == 4, "tile width");
+   constexpr int LW = W == 2 ? 1 : 2;
    const int lane = (int)threadIdx.x & (SAP_WAVE - 1);
    const int wid = lblock * (SAP_BLOCK / SAP_WAVE) + (int)threadIdx.x / SAP_WAVE;
    const uint32_t nw = (uint32_t)nblocks * (SAP_BLOCK / SAP_WAVE);
@@ -1455,8 +1468,10 @@ __device__ __forceinline__ void sweepTiles2(const Mat& M, const Vec& V, const Pa
          continue;
       const int64_t e0 = M.tstart[tile];
       const uint64_t mask = M.tmask[tile];
-      const uint64_t odd = M.todd[tile];
-      const int lanes = ((int)(M.tstart[tile + 1] - e0) + __popcll(odd)) >> 1;
+      // empty positions at the end of each row, by the row's first lane: pad0 + 2 * pad1
+      const uint64_t pad0 = M.tpad[2 * (size_t)tile];
+      const uint64_t pad1 = M.tpad[2 * (size_t)tile + 1];
+      const int lanes = ((int)(M.tstart[tile + 1] - e0) + __popcll(pad0) + 2 * __popcll(pad1)) >> LW;
       const int32_t row0 = M.trow0[tile];
       const int lev = M.tlev[tile];
       const bool has = lane < lanes;
@@ -1467,10 +1482,11 @@ __device__ __forceinline__ void sweepTiles2(const Mat& M, const Vec& V, const Pa
       const uint64_t above = (sl >= 63) ? 0ULL : (mask >> (sl + 1));
       const int ns = above ? (sl + 1 + __ffsll((unsigned long long)above) - 1) : lanes;
       const int Ll = ns - sl;
-      const int L = 2 * Ll - (int)((odd >> sl) & 1);
+      const int L = W * Ll - (int)((pad0 >> sl) & 1) - 2 * (int)((pad1 >> sl) & 1);
       const int q = lane - sl;
-      const int64_t k0 = e0 + 2 * lane - __popcll(odd & ((1ULL << sl) - 1));
-      const bool has1 = 2 * q + 1 < L;
+      const uint64_t below = (1ULL << sl) - 1;
+      const int64_t k0 = e0 + W * lane - __popcll(pad0 & below) - 2 * __popcll(pad1 & below);
+      const int nin = L - W * q;   // positions of this lane inside the row (>= W: all)
       const int32_t row = row0 + k;
       const bool marked = has
          && (I.firstall || V.rowmark[(size_t)v * (size_t)M.nrows + (size_t)row] == I.round)
@@ -1479,77 +1495,97 @@ __device__ __forceinline__ void sweepTiles2(const Mat& M, const Vec& V, const Pa
          continue;
 
       const double2* bnd = V.bnd + (size_t)v * (size_t)M.ncols;
-      Elem e, f;
-      e.a = 0.0;
-      e.lb = 0.0;
-      e.ub = 0.0;
-      e.j = 0;
-      e.cf = 0;
-      f = e;
+      Elem e[W];
+#pragma unroll
+      for( int i = 0; i < W; ++i )
+      {
+         e[i].a = 0.0;
+         e[i].lb = 0.0;
+         e[i].ub = 0.0;
+         e[i].j = 0;
+         e[i].cf = 0;
+      }
       bool allbin = false;
       double2 sides = make_double2(0.0, 0.0);
       if( marked )
       {
-         e = loadElem(M, bnd, k0);
-         if( has1 )
-            f = loadElem(M, bnd, k0 + 1);
+#pragma unroll
+         for( int i = 0; i < W; ++i )
+            if( i < nin )
+               e[i] = loadElem(M, bnd, k0 + i);
          allbin = (M.rowflags[row] & 1) != 0;
          sides = M.sides[row];
       }
-      double cmin = 0.0, cmax = 0.0, mad = 0.0;
-      int kmin = 0, kmax = 0, kmin1 = 0, kmax1 = 0;
+      DD dmn[W], dmx[W];
+      int kmin[W], kmax[W];
+      double mad = 0.0;
       bool madinf = false;
-      DD dmn, dmx;
-      if( marked )
+#pragma unroll
+      for( int i = 0; i < W; ++i )
       {
-         elemContrib(P, e, allbin, cmin, cmax, kmin, kmax, mad, madinf);
-         dmn = { cmin, 0.0 };
-         dmx = { cmax, 0.0 };
-         if( has1 )
+         double cmin = 0.0, cmax = 0.0, md = 0.0;
+         bool mi = false;
+         kmin[i] = 0;
+         kmax[i] = 0;
+         if( marked && i < nin )
          {
-            double fmin, fmax, fmad;
-            bool finf;
-            elemContrib(P, f, allbin, fmin, fmax, kmin1, kmax1, fmad, finf);
-            dmn = pairAdd(dmn, DD{ fmin, 0.0 });
-            dmx = pairAdd(dmx, DD{ fmax, 0.0 });
-            mad = dmax(mad, fmad);
-            madinf = madinf || finf;
+            elemContrib(P, e[i], allbin, cmin, cmax, kmin[i], kmax[i], md, mi);
+            mad = i == 0 ? md : dmax(mad, md);
+            madinf = madinf || mi;
          }
+         dmn[i] = { cmin, 0.0 };
+         dmx[i] = { cmax, 0.0 };
       }
-      else
+      // levels inside the lane
+#pragma unroll
+      for( int h = 1; h < W; h <<= 1 )
+#pragma unroll
+         for( int i = 0; i < W; i += 2 * h )
+            if( i + h < nin )
+            {
+               dmn[i] = pairAdd(dmn[i], dmn[i + h]);
+               dmx[i] = pairAdd(dmx[i], dmx[i + h]);
+            }
+      DD smn = dmn[0], smx = dmx[0];
+      for( int w = 1; w < ((1 << lev) >> LW); w <<= 1 )
       {
-         dmn = { 0.0, 0.0 };
-         dmx = { 0.0, 0.0 };
-      }
-      for( int w = 1; w < (1 << lev) / 2; w <<= 1 )
-      {
-         const DD omn = shflDownDD(dmn, w);
-         const DD omx = shflDownDD(dmx, w);
+         const DD omn = shflDownDD(smn, w);
+         const DD omx = shflDownDD(smx, w);
          const double omad = __shfl_down(mad, w, SAP_WAVE);
          if( (q & (2 * w - 1)) == 0 && q + w < Ll )
          {
-            dmn = pairAdd(dmn, omn);
-            dmx = pairAdd(dmx, omx);
+            smn = pairAdd(smn, omn);
+            smx = pairAdd(smx, omx);
             mad = dmax(mad, omad);
          }
       }
       RowState rs;
-      rs.minact = shflDD(dmn, sl);
-      rs.maxact = shflDD(dmx, sl);
+      rs.minact = shflDD(smn, sl);
+      rs.maxact = shflDD(smx, sl);
       rs.mad = __shfl(mad, sl, SAP_WAVE);
       const uint64_t rowbits = ((Ll >= 64) ? ~0ULL : ((1ULL << Ll) - 1)) << sl;
 #pragma unroll
       for( int c = 0; c < 4; ++c )
       {
-         rs.c[c] = __popcll(__ballot(kmin == c + 1) & rowbits) + __popcll(__ballot(kmin1 == c + 1) & rowbits);
-         rs.c[4 + c] = __popcll(__ballot(kmax == c + 1) & rowbits) + __popcll(__ballot(kmax1 == c + 1) & rowbits);
+         int a = 0, b = 0;
+#pragma unroll
+         for( int i = 0; i < W; ++i )
+         {
+            a += __popcll(__ballot(kmin[i] == c + 1) & rowbits);
+            b += __popcll(__ballot(kmax[i] == c + 1) & rowbits);
+         }
+         rs.c[c] = a;
+         rs.c[4 + c] = b;
       }
       if( (__ballot(madinf) & rowbits) != 0 )
          rs.mad = P.infinity;
       const RowDecision d = decideRow(P, rs, sides.x, sides.y, L);
       const bool scr = marked && d.tighten;
-      const bool pass = scr && screenElem(P, rs, d, sides.x, sides.y, e);
-      const bool pass1 = scr && has1 && screenElem(P, rs, d, sides.x, sides.y, f);
+      uint32_t pass = 0;
+#pragma unroll
+      for( int i = 0; i < W; ++i )
+         if( scr && i < nin && screenElem(P, rs, d, sides.x, sides.y, e[i]) )
+            pass |= 1u << i;
       if( marked && q == 0 )
       {
          CtxT<R> C;
@@ -1565,10 +1601,18 @@ __device__ __forceinline__ void sweepTiles2(const Mat& M, const Vec& V, const Pa
          if( !V.single )
             noteRound(M, V, v, row, I.round);
       }
-      pushQueue(M, V, P, Q, pass, k0, row, v, rs, d, I.round);
-      maybeDrain<R>(M, V, P, Q, I.round);
-      pushQueue(M, V, P, Q, pass1, k0 + 1, row, v, rs, d, I.round);
-      maybeDrain<R>(M, V, P, Q, I.round);
+      // one push (and one drain site) per position of the lane
+#pragma unroll 1
+      for( int i = 0; i < W; ++i )
+      {
+         Elem x = e[0];
+#pragma unroll
+         for( int u = 1; u < W; ++u )
+            if( i == u )
+               x = e[u];
+         pushQueue(M, V, P, Q, ((pass >> i) & 1) != 0, k0 + i, x, row, v, rs, d, I.round);
+         maybeDrain<R>(M, V, P, Q, I.round);
+      }
    }
 }
 
@@ -1714,7 +1758,7 @@ __device__ __forceinline__ void candidatesRange(const Mat& M, const Vec& V, cons
       if( has )
          e = loadElem(M, bnd, s + pos);
       const bool pass = has && screenElem(P, rs, d, lhs, rhs, e);
-      pushQueue(M, V, P, Q, pass, s + pos, row, v, rs, d, round);
+      pushQueue(M, V, P, Q, pass, s + pos, e, row, v, rs, d, round);
       maybeDrain<R>(M, V, P, Q, round);
    }
 }
@@ -1912,7 +1956,9 @@ __global__ __launch_bounds__(SAP_BLOCK, 4) void k_sweep(Mat M, Vec V, Params P, 
    __shared__ double qmxhi[NW][SAP_QCAP];
    __shared__ double qmxlo[NW][SAP_QCAP];
    __shared__ unsigned long long qcnt[NW][SAP_QCAP];
-   __shared__ uint32_t qflags[NW][SAP_QCAP];
+   __shared__ double qlb[NW][SAP_QCAP];
+   __shared__ double qub[NW][SAP_QCAP];
+   __shared__ int32_t qj[NW][SAP_QCAP];
    const int wave = (int)threadIdx.x / SAP_WAVE;
    WaveQueue Q;
    Q.k = qk[wave];
@@ -1923,14 +1969,18 @@ __global__ __launch_bounds__(SAP_BLOCK, 4) void k_sweep(Mat M, Vec V, Params P, 
    Q.mxhi = qmxhi[wave];
    Q.mxlo = qmxlo[wave];
    Q.cnt = qcnt[wave];
-   Q.flags = qflags[wave];
+   Q.lb = qlb[wave];
+   Q.ub = qub[wave];
+   Q.j = qj[wave];
    Q.head = 0;
    Q.count = 0;
    switch( kind )
    {
    case 0:
       if( M.twidth == 2 )
-         sweepTiles2<R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+         sweepTilesW<R, 2>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
+      else if( M.twidth == 4 )
+         sweepTilesW<R, 4>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
       else
          sweepTiles<R>(M, V, P, I, lblock, nblocks, nnzacc, rowsacc, Q);
       // empty rows (first round only; they are never marked again): activity 0 against the
@@ -3133,7 +3183,7 @@ struct sap_matrix
    unsigned long long* tmask;
    int32_t* trow0;
    uint8_t* tlev;
-   unsigned long long* todd;
+   unsigned long long* tpad;
    sap_tuning tuning;    // row binning and launch shape it was built with
    int32_t nseg;         // packed instances (1: a single instance)
    int32_t* rowseg;      // device, by device row position (NULL when nseg == 1)
@@ -3310,7 +3360,7 @@ static Mat toMat(const sap_matrix* m)
    M.tmask = m->tmask;
    M.trow0 = m->trow0;
    M.tlev = m->tlev;
-   M.todd = m->todd;
+   M.tpad = m->tpad;
    M.twidth = m->tuning.tilewidth;
    M.pad3 = 0;
    M.nseg = m->nseg;
@@ -3379,14 +3429,14 @@ void sap_tuning_default(sap_tuning* t)
    t->longrow = 1024;
    t->capscale = 50;
    t->rowbins = 0;
-   t->tilewidth = 2;
+   t->tilewidth = 4;
 }
 
 int sap_context_set_tuning(sap_context* c, const sap_tuning* t)
 {
    if( c == NULL || t == NULL || t->tilebins < 0 || t->tilebins > 6 || t->longrow < 64 || t->longrow > (1 << 24)
       || t->capscale < 10 || t->capscale > 1000 || t->rowbins < 0 || t->rowbins > 3
-      || t->tilewidth < 1 || t->tilewidth > 2 || (t->tilebins == 6 && t->tilewidth != 2) )
+      || (t->tilewidth != 1 && t->tilewidth != 2 && t->tilewidth != 4) || (t->tilebins == 6 && t->tilewidth == 1) )
       return SAP_INVALIDDATA;
    c->tuning = *t;
    return SAP_OKAY;
@@ -3493,7 +3543,7 @@ static void matrixRelease(sap_matrix* m)
    (void)hipFree(m->tmask);
    (void)hipFree(m->trow0);
    (void)hipFree(m->tlev);
-   (void)hipFree(m->todd);
+   (void)hipFree(m->tpad);
    (void)hipFree(m->rowseg);
    (void)hipFree(m->colseg);
    delete m->rowoff;
@@ -3624,8 +3674,8 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
    // deterministic.
    const int32_t tfirst = binstart[1 + std::min(tune.rowbins, tune.tilebins)];
    const int32_t tend = std::max(tfirst, binstart[1 + tune.tilebins]);
-   // With tile width 2 a row takes its length rounded up to even positions of a 128-position
-   // tile.
+   // With tile width W > 1 a row takes its length rounded up to a multiple of W positions of
+   // a tile of 64*W positions.
    const int tw = tune.tilewidth;
    std::vector<int32_t> tilerows;   // rows per tile, in tile order
    if( tend > tfirst )
@@ -3711,7 +3761,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
    std::vector<unsigned long long> tmask;
    std::vector<int32_t> trow0;
    std::vector<uint8_t> tlev;
-   std::vector<unsigned long long> todd;
+   std::vector<unsigned long long> tpad;   // two masks per tile (wide tiles)
    {
       auto levels = [](int l) {
          int v = 0;
@@ -3724,19 +3774,23 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
       {
          tstart.push_back(prowptr[(size_t)r]);
          trow0.push_back(r);
-         unsigned long long mask = 0, odd = 0;
+         unsigned long long mask = 0, pad0 = 0, pad1 = 0;
          int lanes = 0, maxl = 0;
          for( int32_t q = 0; q < nr; ++q, ++r )
          {
             const int L = (int)(prowptr[(size_t)r + 1] - prowptr[(size_t)r]);
+            const int pad = (tw - L % tw) % tw;
             mask |= 1ULL << lanes;
-            if( tw == 2 && (L & 1) )
-               odd |= 1ULL << lanes;
+            if( pad & 1 )
+               pad0 |= 1ULL << lanes;
+            if( pad & 2 )
+               pad1 |= 1ULL << lanes;
             lanes += (L + tw - 1) / tw;
             maxl = L > maxl ? L : maxl;
          }
          tmask.push_back(mask);
-         todd.push_back(odd);
+         tpad.push_back(pad0);
+         tpad.push_back(pad1);
          tlev.push_back(levels(maxl));
       }
       tstart.push_back(prowptr[(size_t)tend]);
@@ -3800,7 +3854,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
    MALLOC_OR_FAIL(m->tmask, tmask.size());
    MALLOC_OR_FAIL(m->trow0, trow0.size());
    MALLOC_OR_FAIL(m->tlev, tlev.size());
-   MALLOC_OR_FAIL(m->todd, todd.size());
+   MALLOC_OR_FAIL(m->tpad, tpad.size());
    m->nseg = nseg;
    if( nseg > 1 )
    {
@@ -3838,7 +3892,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
       HIPCHK(hipMemcpyAsync(m->tmask, tmask.data(), sizeof(unsigned long long) * tmask.size(), hipMemcpyHostToDevice, s));
       HIPCHK(hipMemcpyAsync(m->trow0, trow0.data(), sizeof(int32_t) * trow0.size(), hipMemcpyHostToDevice, s));
       HIPCHK(hipMemcpyAsync(m->tlev, tlev.data(), tlev.size(), hipMemcpyHostToDevice, s));
-      HIPCHK(hipMemcpyAsync(m->todd, todd.data(), sizeof(unsigned long long) * todd.size(), hipMemcpyHostToDevice,
+      HIPCHK(hipMemcpyAsync(m->tpad, tpad.data(), sizeof(unsigned long long) * tpad.size(), hipMemcpyHostToDevice,
          s));
    }
    if( nseg > 1 )
@@ -4102,7 +4156,7 @@ int sap_bounds_create(sap_matrix* m, int32_t nvec, sap_bounds** out)
 
 int sap_bounds_create_mode(sap_matrix* m, int32_t nvec, int32_t mode, sap_bounds** out)
 {
-   if( m == NULL || out == NULL || nvec < 1 || mode < SAP_MODE_AUTO || mode > SAP_MODE_SPARSE )
+   if( m == NULL || out == NULL || nvec < 1 || nvec >= (1 << 30) || mode < SAP_MODE_AUTO || mode > SAP_MODE_SPARSE )
       return SAP_INVALIDDATA;
    *out = NULL;
    if( (int64_t)nvec * m->nrows > (int64_t)UINT32_MAX || (int64_t)nvec * m->ncols > (int64_t)UINT32_MAX )

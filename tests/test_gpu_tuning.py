@@ -16,6 +16,7 @@ def say(*a):
 
 @pytest.mark.parametrize("tilebins,longrow,capscale,rowbins,tilewidth", [
     (6, 4096, 100, 0, 2), (6, 64, 100, 0, 2), (6, 100, 100, 0, 2), (6, 4096, 100, 3, 2), (5, 4096, 100, 0, 2),
+    (6, 4096, 100, 0, 4), (6, 100, 50, 0, 4), (3, 4096, 100, 0, 4), (6, 4096, 100, 2, 4),
     (5, 4096, 100, 0, 1), (0, 4096, 100, 0, 2), (3, 64, 10, 0, 2), (3, 64, 10, 0, 1),
     (4, 1024, 200, 0, 2), (1, 128, 1000, 0, 2), (1, 128, 1000, 0, 1), (5, 65536, 50, 0, 2), (5, 4096, 100, 1, 2),
     (5, 4096, 100, 2, 2), (5, 4096, 100, 3, 2), (5, 4096, 100, 3, 1), (2, 4096, 100, 3, 2), (0, 256, 100, 3, 2)])
@@ -48,10 +49,10 @@ def test_autotune_returns_a_valid_tuning(gpu_ctx):
     assert best["tilebins"] in (3, 4, 5, 6) and best["longrow"] in (1024, 4096, 16384) and best["rowbins"] in (0, 2, 3)
     assert best["capscale"] in (50, 100, 200)
     assert 0 < bms
-    assert gpu_ctx.tuning() == {"tilebins": 6, "longrow": 1024, "capscale": 50, "rowbins": 0, "tilewidth": 2}
+    assert gpu_ctx.tuning() == {"tilebins": 6, "longrow": 1024, "capscale": 50, "rowbins": 0, "tilewidth": 4}
 
 
 def test_width_one_tiles_hold_rows_of_at_most_64(gpu_ctx):
     with pytest.raises(sa.SapError):
         gpu_ctx.set_tuning(6, 1024, 50, 0, 1)
-    assert gpu_ctx.tuning()["tilebins"] == 6 and gpu_ctx.tuning()["tilewidth"] == 2
+    assert gpu_ctx.tuning()["tilebins"] == 6 and gpu_ctx.tuning()["tilewidth"] == 4
