@@ -74,7 +74,7 @@ struct Status
    unsigned long long slot_rows[SAP_STAT_SLOTS];
    unsigned long long slot_chg[SAP_STAT_SLOTS];
    // sweep launch duration from the constant-rate wall clock: earliest workgroup start and
-   // latest workgroup end per slot, folded by the next round's control kernel
+   // latest workgroup end per slot, folded at the end of the round
    unsigned long long tslot0[SAP_STAT_SLOTS];
    unsigned long long tslot1[SAP_STAT_SLOTS];
    unsigned long long sweep_ticks;
@@ -2203,6 +2203,53 @@ __device__ __forceinline__ int columnStep(const Mat& M, const Vec& V, const Para
    return (chl ? 1 : 0) + (chu ? 1 : 0);
 }
 
+// fold the sweep's per-slot wall-clock stamps into the launch time (all SAP_STAT_SLOTS threads
+// of one workgroup)
+__device__ __forceinline__ void foldStamps(Status* st)
+{
+   __shared__ unsigned long long tmin[SAP_STAT_SLOTS], tmax[SAP_STAT_SLOTS];
+   const int t = (int)threadIdx.x;
+   tmin[t] = st->tslot0[t];
+   tmax[t] = st->tslot1[t];
+   st->tslot0[t] = ~0ULL;
+   st->tslot1[t] = 0;
+   __syncthreads();
+   for( int w = SAP_STAT_SLOTS / 2; w > 0; w >>= 1 )
+   {
+      if( t < w )
+      {
+         tmin[t] = tmin[t] < tmin[t + w] ? tmin[t] : tmin[t + w];
+         tmax[t] = tmax[t] > tmax[t + w] ? tmax[t] : tmax[t + w];
+      }
+      __syncthreads();
+   }
+   if( t == 0 && tmin[0] != ~0ULL && tmax[0] >= tmin[0] )
+   {
+      st->sweep_ticks += tmax[0] - tmin[0];
+      st->sweep_count += 1;
+   }
+}
+
+// the stop test of the round after the current one: nothing marked (queued), cutoff of the
+// single vector, or the call's round cap.  k_round_begin applies it at the start of a round;
+// the host applies it to the status it reads back after a graph group, so that a call whose
+// last round ended the group launches no further group.
+__host__ __device__ __forceinline__ bool stopsNext(const Status& st, bool dense, int32_t maxrounds)
+{
+   const int32_t r = st.round + 1;
+   bool more;
+   if( dense )
+      more = (st.rlocal == 0 && st.firstall) || st.anymark[r & 1];
+   else
+   {
+      uint32_t in = 0;
+      for( int b = 0; b < SAP_NBINS; ++b )
+         in += st.wlcount[r & 1][b];
+      more = in != 0;
+   }
+   return !more || st.cutoff || (!st.cont && maxrounds >= 0 && st.rlocal >= maxrounds);
+}
+
 // mark the rows of a changed column for the next round (eventExecLinear marks them)
 __device__ __forceinline__ void markColumn(const Mat& M, const Vec& V, int32_t j, int32_t v, int32_t next, int npar,
    bool& anymarked)
@@ -2265,11 +2312,9 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_columns(Mat M, Vec V, Params P)
 // second half of the column step: rows of the columns changed in this round are marked for the
 // next one, unless their unit was cut off in the round (acceptance and marking are separate
 // launches so that a round ending in infeasibility marks nothing)
-__global__ __launch_bounds__(SAP_BLOCK) void k_mark(Mat M, Vec V)
+__device__ __forceinline__ void markRound(const Mat& M, const Vec& V)
 {
    Status* st = V.st;
-   if( st->done || (V.single && st->cutoff) )
-      return;
    const int32_t round = st->round;
    const int32_t next = round + 1;
    const int npar = next & 1;
@@ -2300,6 +2345,13 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_mark(Mat M, Vec V)
    }
    if( __ballot(anymarked) != 0 && ((int)threadIdx.x & (SAP_WAVE - 1)) == 0 && st->anymark[npar] == 0 )
       st->anymark[npar] = 1;
+}
+
+__global__ __launch_bounds__(SAP_BLOCK) void k_mark(Mat M, Vec V)
+{
+   Status* st = V.st;
+   if( !(st->done || (V.single && st->cutoff)) )
+      markRound(M, V);
 }
 
 // reason recording, after the column step: every logged candidate whose key won its column's
@@ -2679,47 +2731,16 @@ __global__ void k_reason_reset(Vec V, int64_t ncols)
 __global__ __launch_bounds__(SAP_STAT_SLOTS) void k_round_begin(Vec V, int32_t maxrounds)
 {
    Status* st = V.st;
-   __shared__ unsigned long long tmin[SAP_STAT_SLOTS], tmax[SAP_STAT_SLOTS];
    const int t = (int)threadIdx.x;
-   tmin[t] = st->tslot0[t];
-   tmax[t] = st->tslot1[t];
-   st->tslot0[t] = ~0ULL;
-   st->tslot1[t] = 0;
-   __syncthreads();
-   for( int w = SAP_STAT_SLOTS / 2; w > 0; w >>= 1 )
-   {
-      if( t < w )
-      {
-         tmin[t] = tmin[t] < tmin[t + w] ? tmin[t] : tmin[t + w];
-         tmax[t] = tmax[t] > tmax[t + w] ? tmax[t] : tmax[t + w];
-      }
-      __syncthreads();
-   }
-   if( t != 0 )
+   foldStamps(st);
+   if( t != 0 || st->done )
       return;
-   if( tmin[0] != ~0ULL && tmax[0] >= tmin[0] )
-   {
-      st->sweep_ticks += tmax[0] - tmin[0];
-      st->sweep_count += 1;
-   }
-   if( st->done )
-      return;
-   const int32_t r = st->round + 1;
-   bool more;
-   if( V.dense )
-      more = (st->rlocal == 0 && st->firstall) || st->anymark[r & 1];
-   else
-   {
-      uint32_t in = 0;
-      for( int b = 0; b < SAP_NBINS; ++b )
-         in += st->wlcount[r & 1][b];
-      more = in != 0;
-   }
-   if( !more || st->cutoff || (!st->cont && maxrounds >= 0 && st->rlocal >= maxrounds) )
+   if( stopsNext(*st, V.dense != 0, maxrounds) )
    {
       st->done = 1;
       return;
    }
+   const int32_t r = st->round + 1;
    st->round = r;
    st->rlocal += 1;
    st->clogn = 0;
@@ -3199,6 +3220,7 @@ struct sap_matrix
 // propagations end after one or two rounds) and a long one for the rest
 #define SAP_GRAPH_SHORT 2
 #define SAP_GRAPH_LONG 8
+#define SAP_NGRAPHS 4
 
 struct RoundGraph
 {
@@ -3228,7 +3250,7 @@ struct sap_bounds
    double* stage;   // device staging for host lb/ub (2*ncols)
    int32_t roundbase;
    int32_t graphbroken;   // capture failed once: use the host-driven loop
-   RoundGraph graph[2];
+   RoundGraph graph[SAP_NGRAPHS];   // SAP_GRAPH_SHORT, SAP_GRAPH_LONG, 1 and 4 rounds
    GraphKey key;
    // persistent staging for batches and slots (grown on demand, never per call)
    int64_t* ovptr;       // nvec+1
@@ -4134,7 +4156,7 @@ static void boundsRelease(sap_bounds* b)
    (void)hipFree(V.touched);
    (void)hipFree(b->dstatus);
    (void)hipFree(b->stage);
-   for( int g = 0; g < 2; ++g )
+   for( int g = 0; g < SAP_NGRAPHS; ++g )
    {
       if( b->graph[g].valid )
          (void)hipGraphExecDestroy(b->graph[g].exec);
@@ -4364,6 +4386,22 @@ static Layout makeLayout(const sap_bounds* b)
    return L;
 }
 
+// host twin of foldStamps, on a status read back from the device
+static void hostFoldStamps(Status* h)
+{
+   unsigned long long tmin = ~0ULL, tmax = 0;
+   for( int q = 0; q < SAP_STAT_SLOTS; ++q )
+   {
+      tmin = h->tslot0[q] < tmin ? h->tslot0[q] : tmin;
+      tmax = h->tslot1[q] > tmax ? h->tslot1[q] : tmax;
+   }
+   if( tmin != ~0ULL && tmax >= tmin )
+   {
+      h->sweep_ticks += tmax - tmin;
+      h->sweep_count += 1;
+   }
+}
+
 static void foldStats(sap_context* c)
 {
    // fold the per-workgroup statistic slots into the host copy of the status
@@ -4443,6 +4481,7 @@ static int runRoundsHost(sap_bounds* b, const Params& P, sap_stats* stats)
    const Layout L = makeLayout(b);
    double sweepms = 0.0;
    int64_t launches = 0;
+   int32_t started = 0;
    HIPCHK(hipEventRecord(c->ev[0], s));
    for( int it = 0;; ++it )
    {
@@ -4457,12 +4496,18 @@ static int runRoundsHost(sap_bounds* b, const Params& P, sap_stats* stats)
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(c->hstatus, b->dstatus, sizeof(Status), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
+      // the round ran (its sweep was timed) when its control kernel advanced the round count;
+      // the last one already reports done at its end
+      if( c->hstatus->rlocal > started )
+      {
+         started = c->hstatus->rlocal;
+         float ms = 0.0f;
+         HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+         sweepms += ms;
+         ++launches;
+      }
       if( c->hstatus->done )
          break;
-      float ms = 0.0f;
-      HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-      sweepms += ms;
-      ++launches;
    }
    HIPCHK(hipEventRecord(c->ev[1], s));
    HIPCHK(hipEventSynchronize(c->ev[1]));
@@ -4512,15 +4557,19 @@ static bool ensureGraphs(sap_bounds* b, const Params& P)
    k.dense = b->V.dense;
    k.logchg = b->V.logchg;
    k.reasons = b->V.reasons;
-   if( b->graph[0].valid && b->graph[1].valid && memcmp(&k, &b->key, sizeof(k)) == 0 )
+   bool valid = memcmp(&k, &b->key, sizeof(k)) == 0;
+   for( int g = 0; g < SAP_NGRAPHS; ++g )
+      valid = valid && b->graph[g].valid;
+   if( valid )
       return true;
-   for( int g = 0; g < 2; ++g )
+   for( int g = 0; g < SAP_NGRAPHS; ++g )
       if( b->graph[g].valid )
       {
          (void)hipGraphExecDestroy(b->graph[g].exec);
          b->graph[g].valid = 0;
       }
-   if( !captureGraph(b, P, SAP_GRAPH_SHORT, &b->graph[0]) || !captureGraph(b, P, SAP_GRAPH_LONG, &b->graph[1]) )
+   if( !captureGraph(b, P, SAP_GRAPH_SHORT, &b->graph[0]) || !captureGraph(b, P, SAP_GRAPH_LONG, &b->graph[1])
+      || !captureGraph(b, P, 1, &b->graph[2]) || !captureGraph(b, P, 4, &b->graph[3]) )
    {
       (void)hipGetLastError();
       b->graphbroken = 1;
@@ -4531,8 +4580,11 @@ static bool ensureGraphs(sap_bounds* b, const Params& P)
    return true;
 }
 
-// rounds until the device reports done: graph launches of several rounds each, one
-// synchronisation per launch (per round with SAP_FLAG_HOSTLOOP)
+// rounds until the device reports done: graph launches of 1, 2, 4, then 8 rounds each, one
+// synchronisation per launch (per round with SAP_FLAG_HOSTLOOP).  The last kernel of every
+// round already decides whether another round would run, so a call ends right after the
+// group of its last round; growing groups bound both the synchronisations (log2 of the rounds)
+// and the rounds launched after the end (fewer than the rounds run).
 static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats)
 {
    RCCHK(ensureRanged(b, P));
@@ -4551,21 +4603,31 @@ static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats)
          fprintf(stderr, "[scip_amd] round cap %d exceeded\n", SAP_ROUND_CAP);
          return SAP_ERROR;
       }
-      RoundGraph* g = &b->graph[it == 0 ? 0 : 1];
+      static const int kSchedule[4] = { 2, 0, 3, 1 };   // 1, 2, 4, 8 rounds
+      RoundGraph* g = &b->graph[kSchedule[it < 3 ? it : 3]];
       HIPCHK(hipGraphLaunch(g->exec, s));
       HIPCHK(hipMemcpyAsync(c->hstatus, b->dstatus, sizeof(Status), hipMemcpyDeviceToHost, s));
+      // the end event of the call rides on every group's synchronisation, so that the last
+      // one needs no further round trip
+      HIPCHK(hipEventRecord(c->ev[1], s));
       HIPCHK(hipStreamSynchronize(s));
       prev = c->hstatus->rlocal;
       if( c->hstatus->done )
          break;
+      if( stopsNext(*c->hstatus, b->V.dense != 0, P.maxrounds) )
+      {
+         // the group's last round was the call's last: no further group; its sweep stamps,
+         // which the next round's control kernel would fold, are folded here (the next call
+         // resets them on the device)
+         hostFoldStamps(c->hstatus);
+         break;
+      }
    }
    // sweep launch durations from the in-kernel wall clock (events recorded inside a graph do
    // not time on this runtime)
    sweepms = (double)c->hstatus->sweep_ticks / (double)c->wallclock_khz;
    launches = (int64_t)c->hstatus->sweep_count;
    (void)prev;
-   HIPCHK(hipEventRecord(c->ev[1], s));
-   HIPCHK(hipEventSynchronize(c->ev[1]));
    float tot = 0.0f;
    HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[1]));
    foldStats(c);
@@ -4650,7 +4712,7 @@ int sap_propagate_resident(sap_bounds* b, const sap_params* params, sap_stats* s
 
 static void dropGraphs(sap_bounds* b)
 {
-   for( int g = 0; g < 2; ++g )
+   for( int g = 0; g < SAP_NGRAPHS; ++g )
       if( b->graph[g].valid )
       {
          (void)hipGraphExecDestroy(b->graph[g].exec);
