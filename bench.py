@@ -461,6 +461,9 @@ def run_serving(sa, args, parts):
     cp = sa.gen_covering_packing(200_000, 300_000, 10, seed=3)
     srv = sa.Server(0, 4, 200 << 30, 16)
     pids = [srv.add_problem(cp)] + [srv.add_problem(p) for p in parts]
+    # half of all requests are nodes of the covering/packing problem: its pool gets more slots
+    # (256 x ~26 MB of per-slot state); the 512 instances keep 16 each
+    srv.set_slots(pids[0], 256)
     t = time.perf_counter()
     for pid in pids:
         srv.warm(pid)

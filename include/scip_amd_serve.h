@@ -51,7 +51,8 @@ typedef struct sap_serve_stats
    double  max_ms;
 } sap_serve_stats;
 
-/* nstreams HIP streams on device; hbm_budget bytes for resident problems; slots per pool */
+/* nstreams HIP streams on device; hbm_budget bytes for resident problems; slots per pool
+ * (the default of every problem, see sap_server_set_slots) */
 int sap_server_create(int32_t device, int32_t nstreams, int64_t hbm_budget, int32_t slots_per_pool,
    sap_server** srv);
 int sap_server_free(sap_server* srv);
@@ -60,6 +61,12 @@ int sap_server_free(sap_server* srv);
 int sap_server_add_problem(sap_server* srv, int64_t nrows, int64_t ncols, const int64_t* rowptr,
    const int32_t* colidx, const double* vals, const double* lhs, const double* rhs, const uint8_t* integral,
    const double* lb, const double* ub, int32_t* id);
+
+/* slots of the problem's pool (default: the server's slots_per_pool), e.g. many for a problem
+ * that receives most requests; the pool is sized when the problem is uploaded, so a resident
+ * problem is released and uploaded again at its next use.  SAP_INVALIDCALL while the problem
+ * has requests running or queued. */
+int sap_server_set_slots(sap_server* srv, int32_t id, int32_t slots);
 
 /* upload a problem now and compute its root fixpoint (otherwise done at its first request) */
 int sap_server_warm(sap_server* srv, int32_t id);

@@ -203,6 +203,7 @@ EXPORTS = {
     "sap_server_add_problem": (C.c_int, [_P, C.c_int64, C.c_int64, _I64P, _I32P, _DP, _DP, _DP, _U8P, _DP, _DP,
                                          _I32P]),
     "sap_server_warm": (C.c_int, [_P, C.c_int32]),
+    "sap_server_set_slots": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "sap_server_replay": (C.c_int, [_P, C.POINTER(sap_params), C.c_int32, C.c_int32, C.c_int32, _DP, _I32P, _I64P,
                                     _I32P, _DP, _DP, _I32P, _I32P, _I32P, _DP, C.c_int64, _I64P, _I32P, _DP, _DP,
                                     C.POINTER(sap_serve_stats)]),
@@ -752,6 +753,9 @@ class Server:
 
     def warm(self, pid: int):
         _chk(lib().sap_server_warm(self.h, pid), "sap_server_warm")
+
+    def set_slots(self, pid: int, slots: int):
+        _chk(lib().sap_server_set_slots(self.h, pid, slots), "sap_server_set_slots")
 
     def replay(self, arrival_s, prob, ovptr, ovcol, ovlb, ovub, mode: int = SAP_SERVE_CONTINUOUS,
                steprounds: int = SAP_STEP_SHORT, params: sap_params | None = None, want_changes: bool = False):

@@ -31,6 +31,7 @@ struct Problem
    sap_matrix* mat = NULL;    // NULL: not resident
    sap_bounds* root = NULL;   // root fixpoint (base of every request)
    sap_bounds* pool = NULL;   // slots
+   int32_t slots = 0;         // slots of the pool
    int32_t rootcut = 0;       // the root is infeasible: every request is a cutoff
    int64_t bytes = 0;         // device bytes while resident
    uint64_t lastuse = 0;
@@ -110,7 +111,7 @@ int makeResident(sap_server* s, Problem* p, const sap_params* params)
 {
    if( p->mat != NULL )
       return 1;
-   const int64_t need = footprint(p, s->slots);
+   const int64_t need = footprint(p, p->slots);
    if( need > s->budget )
       return SAP_NOMEMORY;
    while( s->resident + need > s->budget )
@@ -141,7 +142,7 @@ int makeResident(sap_server* s, Problem* p, const sap_params* params)
       rc = sap_propagate_resident(p->root, &rp, &rst);
    }
    if( rc == SAP_OKAY )
-      rc = sap_bounds_create(p->mat, s->slots, &p->pool);
+      rc = sap_bounds_create(p->mat, p->slots, &p->pool);
    if( rc != SAP_OKAY )
    {
       p->bytes = 0;
@@ -156,9 +157,9 @@ int makeResident(sap_server* s, Problem* p, const sap_params* params)
       return rc;
    }
    p->rootcut = rst.result == SAP_CUTOFF ? 1 : 0;
-   p->slotreq.assign((size_t)s->slots, -1);
+   p->slotreq.assign((size_t)p->slots, -1);
    p->freeslots.clear();
-   for( int32_t k = s->slots - 1; k >= 0; --k )
+   for( int32_t k = p->slots - 1; k >= 0; --k )
       p->freeslots.push_back(k);
    p->running = 0;
    p->bytes = need;
@@ -228,7 +229,7 @@ int admit(sap_server* s, Problem* p, std::vector<Request>& req, const int64_t* o
 int harvest(sap_server* s, Problem* p, std::vector<Request>& req, bool wantchg, double t0)
 {
    int32_t nf = 0;
-   std::vector<int32_t> fin((size_t)s->slots);
+   std::vector<int32_t> fin((size_t)p->slots);
    sap_stats st;
    int rc = sap_slots_wait(p->pool, &nf, fin.data(), &st);
    p->inflight = false;
@@ -360,8 +361,26 @@ int sap_server_add_problem(sap_server* s, int64_t nrows, int64_t ncols, const in
    p->lb.assign(lb, lb + ncols);
    p->ub.assign(ub, ub + ncols);
    p->stream = (int32_t)(s->probs.size() % s->ctx.size());
+   p->slots = s->slots;
    *id = (int32_t)s->probs.size();
    s->probs.push_back(p);
+   return SAP_OKAY;
+}
+
+int sap_server_set_slots(sap_server* s, int32_t id, int32_t slots)
+{
+   if( s == NULL || id < 0 || id >= (int32_t)s->probs.size() || slots < 1 )
+      return SAP_INVALIDDATA;
+   Problem* p = s->probs[(size_t)id];
+   if( p->running > 0 || !p->queue.empty() || p->inflight )
+      return SAP_INVALIDCALL;
+   if( p->mat != NULL && p->slots != slots )
+   {
+      // the pool is sized at upload: release the problem, the next use uploads it again
+      evict(s, p);
+      s->st.evictions--;
+   }
+   p->slots = slots;
    return SAP_OKAY;
 }
 
@@ -453,7 +472,7 @@ int sap_server_replay(sap_server* s, const sap_params* params, int32_t mode, int
                break;
             }
             if( r == 1 )
-               rc = admit(s, p, req, ovptr, ovcol, ovlb, ovub, s->slots, t0);
+               rc = admit(s, p, req, ovptr, ovcol, ovlb, ovub, p->slots, t0);
             if( rc != SAP_OKAY )
                break;
          }

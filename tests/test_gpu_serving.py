@@ -58,10 +58,12 @@ def trace(ps, rts, n, rate, seed):
             np.array(val, dtype=np.float64))
 
 
-def run(ps, tr, mode, budget=8 << 30, slots=16, nstreams=3):
+def run(ps, tr, mode, budget=8 << 30, slots=16, nstreams=3, pool=None):
     srv = sa.Server(0, nstreams, budget, slots)
     for p in ps:
         srv.add_problem(p)
+    for pid, n in (pool or {}).items():
+        srv.set_slots(pid, n)
     arrival, prob, ptr, col, val = tr
     out = srv.replay(arrival, prob, ptr, col, val, val, mode=mode, want_changes=True)
     srv.close()
@@ -141,3 +143,31 @@ def test_problem_larger_than_budget_is_an_error():
         srv.replay(*tr[:4], tr[4], tr[4])
     assert e.value.rc == sa.SAP_NOMEMORY
     srv.close()
+
+
+def test_pool_sizes_do_not_change_results():
+    # per-problem pools: a large one for problem 0, a single slot for problem 2, and a resize
+    # of a resident problem between two replays
+    ps = problems()
+    rts = roots(ps)
+    tr = trace(ps, rts, n=160, rate=20000.0, seed=6)
+    say(f"[case] serving pool sizes {len(tr[0])} requests")
+    base = run(ps, tr, sa.SAP_SERVE_CONTINUOUS)
+    sized = run(ps, tr, sa.SAP_SERVE_CONTINUOUS, pool={0: 64, 2: 1})
+    for i in range(len(tr[0])):
+        same_request(base, sized, i)
+    srv = sa.Server(0, 2, 8 << 30, 8)
+    for p in ps:
+        srv.add_problem(p)
+    srv.warm(0)
+    arrival, prob, ptr, col, val = tr
+    a = srv.replay(arrival, prob, ptr, col, val, val, mode=sa.SAP_SERVE_CONTINUOUS, want_changes=True)
+    srv.set_slots(0, 40)
+    b = srv.replay(arrival, prob, ptr, col, val, val, mode=sa.SAP_SERVE_CONTINUOUS, want_changes=True)
+    assert b["stats"]["uploads"] >= 1
+    with pytest.raises(sa.SapError):
+        srv.set_slots(0, 0)
+    srv.close()
+    for i in range(len(tr[0])):
+        same_request(a, b, i)
+        same_request(a, base, i)
