@@ -132,7 +132,7 @@ int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int6
 int sap_matrix_free(sap_matrix* mat);
 /* Autotuner: builds the system restricted to a sample of samplerows rows (all rows if fewer;
  * chosen by seed) once per candidate tuning ((rowbins, tilebins) in (0,3) (0,4) (0,5) (0,6)
- * (2,6) (3,6) x longrow 1024/4096/16384 x capscale 50/100/200, tilewidth 4), propagates lb/ub
+ * (2,6) (3,6) x tilewidth 2/4 x longrow 1024/4096/16384 x capscale 25/50/100), propagates lb/ub
  * with params on it and returns the candidate with
  * the least device time in best (times in ms: best_ms, default_ms for the default tuning).
  * Results are identical under every tuning; only the launch shape differs.  The context's

@@ -3996,47 +3996,50 @@ int sap_autotune(sap_context* ctx, int64_t nrows, int64_t ncols, const int64_t* 
    sap_tuning_default(&def);
    static const int32_t rbs[] = { 0, 0, 0, 0, 2, 3 };
    static const int32_t tbs[] = { 3, 4, 5, 6, 6, 6 };
+   static const int32_t tws[] = { 2, 4 };
    static const int32_t lrs[] = { 1024, 4096, 16384 };
-   static const int32_t cps[] = { 50, 100, 200 };
+   static const int32_t cps[] = { 25, 50, 100 };
    std::vector<double> wl((size_t)ncols), wu((size_t)ncols);
    double bestt = 1e300, deft = -1.0;
    sap_tuning bt = def;
    int rc = SAP_OKAY;
    for( int a = 0; a < 6 && rc == SAP_OKAY; ++a )
-      for( int l = 0; l < 3 && rc == SAP_OKAY; ++l )
-         for( int c = 0; c < 3 && rc == SAP_OKAY; ++c )
-         {
-            sap_tuning t = def;
-            t.rowbins = rbs[a];
-            t.tilebins = tbs[a];
-            t.longrow = lrs[l];
-            t.capscale = cps[c];
-            ctx->tuning = t;
-            sap_matrix* m = NULL;
-            rc = matrixCreate(ctx, sn, ncols, srp.data(), sci.data(), sva.data(), slh.data(), srh.data(), integral, lb,
-               ub, 1, NULL, NULL, &m);
-            if( rc != SAP_OKAY )
-               break;
-            double tmin = 1e300;
-            for( int rep = 0; rep < 3 && rc == SAP_OKAY; ++rep )
+      for( int w = 0; w < 2 && rc == SAP_OKAY; ++w )
+         for( int l = 0; l < 3 && rc == SAP_OKAY; ++l )
+            for( int c = 0; c < 3 && rc == SAP_OKAY; ++c )
             {
-               memcpy(wl.data(), lb, sizeof(double) * (size_t)ncols);
-               memcpy(wu.data(), ub, sizeof(double) * (size_t)ncols);
-               sap_stats st;
-               rc = sap_propagate(m, params, wl.data(), wu.data(), &st);
-               if( rep > 0 )   // the first call includes one-time set-up (graph capture)
-                  tmin = std::min(tmin, st.device_ms);
+               sap_tuning t = def;
+               t.rowbins = rbs[a];
+               t.tilebins = tbs[a];
+               t.tilewidth = tws[w];
+               t.longrow = lrs[l];
+               t.capscale = cps[c];
+               ctx->tuning = t;
+               sap_matrix* m = NULL;
+               rc = matrixCreate(ctx, sn, ncols, srp.data(), sci.data(), sva.data(), slh.data(), srh.data(),
+                  integral, lb, ub, 1, NULL, NULL, &m);
+               if( rc != SAP_OKAY )
+                  break;
+               double tmin = 1e300;
+               for( int rep = 0; rep < 3 && rc == SAP_OKAY; ++rep )
+               {
+                  memcpy(wl.data(), lb, sizeof(double) * (size_t)ncols);
+                  memcpy(wu.data(), ub, sizeof(double) * (size_t)ncols);
+                  sap_stats st;
+                  rc = sap_propagate(m, params, wl.data(), wu.data(), &st);
+                  if( rep > 0 )   // the first call includes one-time set-up (graph capture)
+                     tmin = std::min(tmin, st.device_ms);
+               }
+               sap_matrix_free(m);
+               if( t.tilebins == def.tilebins && t.longrow == def.longrow && t.capscale == def.capscale
+                  && t.rowbins == def.rowbins && t.tilewidth == def.tilewidth )
+                  deft = tmin;
+               if( tmin < bestt )
+               {
+                  bestt = tmin;
+                  bt = t;
+               }
             }
-            sap_matrix_free(m);
-            if( t.tilebins == def.tilebins && t.longrow == def.longrow && t.capscale == def.capscale
-               && t.rowbins == def.rowbins )
-               deft = tmin;
-            if( tmin < bestt )
-            {
-               bestt = tmin;
-               bt = t;
-            }
-         }
    ctx->tuning = saved;
    RCCHK(rc);
    *best = bt;

@@ -47,7 +47,7 @@ def test_autotune_returns_a_valid_tuning(gpu_ctx):
     best, bms, dms = gpu_ctx.autotune(inst, samplerows=20000, apply=False)
     say(f"[info] autotune best={best} best_ms={bms:.3f} default_ms={dms:.3f}")
     assert best["tilebins"] in (3, 4, 5, 6) and best["longrow"] in (1024, 4096, 16384) and best["rowbins"] in (0, 2, 3)
-    assert best["capscale"] in (50, 100, 200)
+    assert best["capscale"] in (25, 50, 100) and best["tilewidth"] in (2, 4)
     assert 0 < bms
     assert gpu_ctx.tuning() == {"tilebins": 6, "longrow": 1024, "capscale": 50, "rowbins": 0, "tilewidth": 4}
 
