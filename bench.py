@@ -394,6 +394,13 @@ def run_packed(sa, ctx, args, parts):
     """All instances packed block-diagonally into one resident matrix, propagated to their
     fixpoints from the root bounds in one launch per round (per-instance verdicts); against one
     resident matrix per instance propagated one after another."""
+    # row binning and tile shape from the autotuner, on a row sample of the packed system; the
+    # plain form below runs under the same tuning
+    t = time.perf_counter()
+    packed_inst, _, _ = sa.pack_instances(parts)
+    tuning, best_ms, default_ms = ctx.autotune(packed_inst, samplerows=100_000, apply=True)
+    del packed_inst
+    log(f"[bench] packed autotune {time.perf_counter() - t:.1f}s: {tuning} ({best_ms:.3f} vs default {default_ms:.3f} ms)")
     t = time.perf_counter()
     pm = sa.PackedMatrix(ctx, parts)
     log(f"[bench] packed upload {time.perf_counter() - t:.1f}s, device bytes {pm.info()['device_bytes']}")
@@ -444,11 +451,13 @@ def run_packed(sa, ctx, args, parts):
         x.close()
     plain = {"value": _rate(nnzp, elp), "unit": "nnz-sweeps/s", "ms_all_instances": 1000 * elp,
              "nnz_swept": nnzp, "verdicts_equal_packed": bool(same)}
+    ctx.set_tuning()
     return {"workload": f"{len(parts)} instances, rows log-uniform [500, 500000] (seed 4), cols 1.5 x rows, "
                         "4..64 nnz/row, mixed int/cont, instance i seed 4000+i, planted-feasible sides; "
                         "root propagation to fixpoint",
             "instances": len(parts), "rows": int(sum(p.nrows for p in parts)),
             "nnz": int(sum(p.nnz for p in parts)), "packed": packed, "plain": plain,
+            "tuning": {"autotuned": tuning, "sample_rows": 100_000, "best_ms": best_ms, "default_ms": default_ms},
             "speedup_nnz_sweeps": packed["value"] / plain["value"] if plain["value"] > 0 else None}
 
 

@@ -3948,7 +3948,9 @@ int sap_autotune(sap_context* ctx, int64_t nrows, int64_t ncols, const int64_t* 
    const double* ub, const sap_params* params, int64_t samplerows, uint64_t seed, sap_tuning* best, double* best_ms,
    double* default_ms)
 {
-   if( ctx == NULL || rowptr == NULL || lb == NULL || ub == NULL || best == NULL || samplerows < 1 || nrows < 0 )
+   if( ctx == NULL || rowptr == NULL || lb == NULL || ub == NULL || best == NULL || samplerows < 1 || nrows < 0
+      || lhs == NULL || rhs == NULL || integral == NULL || ncols < 0
+      || (rowptr[nrows] > 0 && (colidx == NULL || vals == NULL)) )
       return SAP_INVALIDDATA;
    RCCHK(checkParams(params));
    // row sample (every row when there are few): splitmix64 scores, smallest samplerows kept
@@ -3991,6 +3993,26 @@ int sap_autotune(sap_context* ctx, int64_t nrows, int64_t ncols, const int64_t* 
       slh[(size_t)q] = lhs[i];
       srh[(size_t)q] = rhs[i];
    }
+   // only the columns the sample uses, renumbered in their order (rows stay ascending): the
+   // sample propagates the same on them alone, and a system with many more columns than the
+   // sample touches (packed instances) does not copy whole bound vectors per candidate
+   std::vector<int32_t> used(sci);
+   std::sort(used.begin(), used.end());
+   used.erase(std::unique(used.begin(), used.end()), used.end());
+   if( !used.empty() && (used.front() < 0 || (int64_t)used.back() >= ncols) )
+      return SAP_INVALIDDATA;
+   for( int32_t& c : sci )
+      c = (int32_t)(std::lower_bound(used.begin(), used.end(), c) - used.begin());
+   const int64_t snc = (int64_t)used.size();
+   std::vector<uint8_t> sint((size_t)snc);
+   std::vector<double> slb((size_t)snc), sub((size_t)snc);
+   for( int64_t u = 0; u < snc; ++u )
+   {
+      const int32_t j = used[(size_t)u];
+      sint[(size_t)u] = integral[j];
+      slb[(size_t)u] = lb[j];
+      sub[(size_t)u] = ub[j];
+   }
    const sap_tuning saved = ctx->tuning;
    sap_tuning def;
    sap_tuning_default(&def);
@@ -3999,7 +4021,7 @@ int sap_autotune(sap_context* ctx, int64_t nrows, int64_t ncols, const int64_t* 
    static const int32_t tws[] = { 2, 4 };
    static const int32_t lrs[] = { 1024, 4096, 16384 };
    static const int32_t cps[] = { 25, 50, 100 };
-   std::vector<double> wl((size_t)ncols), wu((size_t)ncols);
+   std::vector<double> wl((size_t)snc), wu((size_t)snc);
    double bestt = 1e300, deft = -1.0;
    sap_tuning bt = def;
    int rc = SAP_OKAY;
@@ -4016,15 +4038,15 @@ int sap_autotune(sap_context* ctx, int64_t nrows, int64_t ncols, const int64_t* 
                t.capscale = cps[c];
                ctx->tuning = t;
                sap_matrix* m = NULL;
-               rc = matrixCreate(ctx, sn, ncols, srp.data(), sci.data(), sva.data(), slh.data(), srh.data(),
-                  integral, lb, ub, 1, NULL, NULL, &m);
+               rc = matrixCreate(ctx, sn, snc, srp.data(), sci.data(), sva.data(), slh.data(), srh.data(),
+                  sint.data(), slb.data(), sub.data(), 1, NULL, NULL, &m);
                if( rc != SAP_OKAY )
                   break;
                double tmin = 1e300;
                for( int rep = 0; rep < 3 && rc == SAP_OKAY; ++rep )
                {
-                  memcpy(wl.data(), lb, sizeof(double) * (size_t)ncols);
-                  memcpy(wu.data(), ub, sizeof(double) * (size_t)ncols);
+                  wl = slb;
+                  wu = sub;
                   sap_stats st;
                   rc = sap_propagate(m, params, wl.data(), wu.data(), &st);
                   if( rep > 0 )   // the first call includes one-time set-up (graph capture)
