@@ -13,14 +13,18 @@
 //                   row's infeasibility / redundancy test (propagateCons, :7721-7753) runs
 //                   on the same state.
 //   k_columns     : every touched column applies SCIPinferVarLbCons / SCIPinferVarUbCons
-//                   (scip_var.c:6965-7174) to its tightest regular and forced candidates,
-//                   writes the new bounds and appends the rows of changed columns (CSC) to
-//                   the next round's per-bin worklists.
+//                   (scip_var.c:6965-7174) to its tightest regular and forced candidates and
+//                   writes the new bounds.
+//   k_mark        : the rows of changed columns (CSC) are marked for the next round (dense
+//                   mode) or appended to its per-bin worklists (sparse mode).
+// Rounds run in captured graph groups of 1, 2, 4 and 8; the host applies the same stop test
+// as k_round_begin to the status it reads back after each group.
 //
-// Bins: rows of length <= 4/8/16/32/64 are swept by groups of that many lanes (one nonzero
-// per lane, several rows per wavefront); longer rows by one wavefront (64-element chunks);
-// the longest by a whole workgroup.  All tilings compute the same canonical tree, so the
-// result does not depend on the binning.
+// Bins: in the dense mode rows of up to 128 nonzeros are swept in tiles of 64 lanes holding
+// 1, 2 or 4 nonzeros each (rows packed first-fit decreasing); in the sparse mode rows of
+// length <= 4/8/16/32/64 are swept by groups of that many lanes; longer rows by one wavefront
+// (64-element chunks), the longest by a whole workgroup.  All tilings compute the same
+// canonical tree, so the result does not depend on the binning.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
