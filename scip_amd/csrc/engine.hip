@@ -1104,7 +1104,8 @@ __device__ __forceinline__ bool fetchItem(const Mat& M, const Vec& V, const Item
    {
       v = (int32_t)(it / (uint32_t)I.binlen);
       row = I.binbase + (int32_t)(it - (uint32_t)v * (uint32_t)I.binlen);
-      if( !I.firstall && V.rowmark[(size_t)v * (size_t)M.nrows + (size_t)row] != I.round )
+      const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)row;
+      if( !I.firstall && (V.rowmark[rflat] != I.round || V.redundant[rflat]) )
          return false;
    }
    else
@@ -1365,7 +1366,9 @@ __device__ __forceinline__ void sweepTiles(const Mat& M, const Vec& V, const Par
       const int p = lane - sl;
       const int32_t row = row0 + k;
       const bool marked = has
-         && (I.firstall || V.rowmark[(size_t)v * (size_t)M.nrows + (size_t)row] == I.round)
+         && (I.firstall
+            || (V.rowmark[(size_t)v * (size_t)M.nrows + (size_t)row] == I.round
+               && !V.redundant[(size_t)v * (size_t)M.nrows + (size_t)row]))
          && (M.rowseg == NULL || V.vcutoff[unitOfRow(M, v, row)] == 0);
       if( __ballot(marked) == 0 )
          continue;
@@ -1493,7 +1496,9 @@ __device__ __forceinline__ void sweepTilesW(const Mat& M, const Vec& V, const Pa
       const int nin = L - W * q;   // positions of this lane inside the row (>= W: all)
       const int32_t row = row0 + k;
       const bool marked = has
-         && (I.firstall || V.rowmark[(size_t)v * (size_t)M.nrows + (size_t)row] == I.round)
+         && (I.firstall
+            || (V.rowmark[(size_t)v * (size_t)M.nrows + (size_t)row] == I.round
+               && !V.redundant[(size_t)v * (size_t)M.nrows + (size_t)row]))
          && (M.rowseg == NULL || V.vcutoff[unitOfRow(M, v, row)] == 0);
       if( __ballot(marked) == 0 )
          continue;
@@ -2259,6 +2264,28 @@ __device__ __forceinline__ void markColumn(const Mat& M, const Vec& V, int32_t j
    bool& anymarked)
 {
    Status* st = V.st;
+   if( V.dense )
+   {
+      // Round tags are stored without being read first, also on locally redundant rows, which
+      // the dense sweep skips.  Whether the vector has a non-redundant row marked for the
+      // round (its pending round, the round's "any row marked") reads the redundant flags
+      // only until such a row is known.
+      bool known = V.vpend[v] == next;
+      for( int64_t q = M.colptr[j]; q < M.colptr[j + 1]; ++q )
+      {
+         const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)M.rowidx[q];
+         V.rowmark[rflat] = next;
+         if( !known && !V.redundant[rflat] )
+            known = true;
+      }
+      if( known )
+      {
+         if( V.vpend[v] != next )
+            V.vpend[v] = next;   // every writer of this round stores the same value
+         anymarked = true;
+      }
+      return;
+   }
    for( int64_t q = M.colptr[j]; q < M.colptr[j + 1]; ++q )
    {
       const int32_t row = M.rowidx[q];
@@ -2267,17 +2294,8 @@ __device__ __forceinline__ void markColumn(const Mat& M, const Vec& V, int32_t j
          continue;
       if( V.vpend[v] != next )
          V.vpend[v] = next;   // every writer of this round stores the same value
-      if( V.dense )
-      {
-         if( V.rowmark[rflat] != next )
-            V.rowmark[rflat] = next;
-         anymarked = true;
-      }
-      else
-      {
-         const bool first = atomicMax(&V.rowmark[rflat], next) < next;
-         waveAppendRow(st, V.wl[npar], npar, first, M.rowbin[row], make_int2(row, v));
-      }
+      const bool first = atomicMax(&V.rowmark[rflat], next) < next;
+      waveAppendRow(st, V.wl[npar], npar, first, M.rowbin[row], make_int2(row, v));
    }
 }
 
