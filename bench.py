@@ -114,8 +114,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is designed but not built "
-                                   "(see DESIGN.md); run with --gpus 1"}), flush=True)
+        print(json.dumps({"error": "this engine runs on one GPU (one process per GPU with its own share of "
+                                   "the rows or of the node vectors is the sharding it would use, not built); "
+                                   "run with --gpus 1"}), flush=True)
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
