@@ -8,9 +8,10 @@
  * instead of one sap_propagate per node.
  *
  * Continuous batching: requests are admitted into the slots of their problem's slot pool
- * (sap_slots_*) as they arrive and as slots free up; every scheduling iteration launches one
- * step (a few rounds) for every pool with running slots, on several HIP streams so that pools
- * of different problems overlap on the device; finished slots are collected at once.
+ * (sap_slots_*) as they arrive and as slots free up; every pool with running slots is stepped
+ * (a few rounds per step) on one of several HIP streams, so that pools of different problems
+ * overlap on the device, and is collected and stepped again as soon as its own step has ended
+ * (sap_slots_query), without waiting for the other pools.
  * Admission by HBM footprint: a problem occupies device memory for its matrix, its root
  * vector and its slot pool; when a request needs a problem that is not resident, least
  * recently used problems without running or queued requests are evicted until it fits the

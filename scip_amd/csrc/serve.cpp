@@ -451,7 +451,9 @@ int sap_server_replay(sap_server* s, const sap_params* params, int32_t mode, int
    }
    else
    {
-      std::vector<Problem*> active;
+      // event-driven: a pool is collected as soon as its step has ended and, with requests
+      // still running or waiting, stepped again at once; arrivals and admissions are handled
+      // between these checks, so no pool waits for the steps of the others
       while( s->ndone < nreq && rc == SAP_OKAY )
       {
          const double now = nowSeconds() - t0;
@@ -460,10 +462,10 @@ int sap_server_replay(sap_server* s, const sap_params* params, int32_t mode, int
             s->probs[(size_t)prob[next]]->queue.push_back(next);
             ++next;
          }
-         // admissions (problems with waiting requests, oldest request first)
+         // admissions into pools between steps (problems with waiting requests)
          for( Problem* p : s->probs )
          {
-            if( p->queue.empty() )
+            if( p->queue.empty() || p->inflight )
                continue;
             const int r = makeResident(s, p, params);
             if( r < 0 )
@@ -478,32 +480,47 @@ int sap_server_replay(sap_server* s, const sap_params* params, int32_t mode, int
          }
          if( rc != SAP_OKAY )
             break;
-         // one step for every pool with running slots: launch all, then wait in turn
-         active.clear();
+         // a step for every pool with running slots and none in flight
          int64_t running = 0;
          for( Problem* p : s->probs )
-            if( p->running > 0 )
+         {
+            running += p->running;
+            if( p->running > 0 && !p->inflight )
             {
-               running += p->running;
                rc = sap_slots_launch(p->pool, params, steprounds);
                if( rc != SAP_OKAY )
                   break;
                p->inflight = true;
                s->st.steps++;
-               active.push_back(p);
             }
-         s->st.max_running = std::max(s->st.max_running, running);
-         for( Problem* p : active )
-         {
-            const int r = harvest(s, p, req, wantchg, t0);
-            if( rc == SAP_OKAY )
-               rc = r;
          }
-         if( active.empty() && next < nreq )
+         if( rc != SAP_OKAY )
+            break;
+         s->st.max_running = std::max(s->st.max_running, running);
+         // collect the pools whose step has ended
+         bool busy = false;
+         for( Problem* p : s->probs )
+         {
+            if( !p->inflight )
+               continue;
+            int32_t ready = 0;
+            rc = sap_slots_query(p->pool, &ready);
+            if( rc != SAP_OKAY )
+               break;
+            if( ready )
+               rc = harvest(s, p, req, wantchg, t0);
+            else
+               busy = true;
+            if( rc != SAP_OKAY )
+               break;
+         }
+         if( rc != SAP_OKAY )
+            break;
+         if( !busy && next < nreq )
          {
             bool waiting = false;
             for( Problem* p : s->probs )
-               waiting = waiting || !p->queue.empty();
+               waiting = waiting || !p->queue.empty() || p->running > 0;
             if( !waiting )
                waitUntil(t0 + arrival_s[next]);
          }
