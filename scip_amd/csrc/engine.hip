@@ -1096,7 +1096,7 @@ __device__ __forceinline__ void noteRound(const Mat& M, const Vec& V, int32_t v,
       atomicMax(&V.vrounds[u], round);
 }
 
-template <bool R>
+template <bool R, bool CUTCHECK = true>
 __device__ __forceinline__ bool fetchItem(const Mat& M, const Vec& V, const Items& I, uint32_t it, int32_t& row,
    int32_t& v)
 {
@@ -1116,7 +1116,7 @@ __device__ __forceinline__ bool fetchItem(const Mat& M, const Vec& V, const Item
    }
    // with reason recording every marked row of the round is swept, so that the smallest
    // infeasible row does not depend on timing
-   return R || V.vcutoff[unitOfRow(M, v, row)] == 0;
+   return R || !CUTCHECK || V.vcutoff[unitOfRow(M, v, row)] == 0;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1833,9 +1833,13 @@ __device__ __forceinline__ void sweepBlocks(const Mat& M, const Vec& V, const Pa
    const int32_t rlocal = I.rlocal;
    for( uint32_t it = (uint32_t)lblock; it < I.n; it += (uint32_t)nblocks )
    {
+      // every wavefront must take the same branch around the barriers below: the item is
+      // fetched without the test of its unit's cutoff flag, which another workgroup may set
+      // while the wavefronts read it (a row of a unit cut off in this round is then swept
+      // anyway; its unit's results are void)
       int32_t row, v;
-      if( !fetchItem<R>(M, V, I, it, row, v) )
-         continue;   // block-uniform
+      if( !fetchItem<R, false>(M, V, I, it, row, v) )
+         continue;   // workgroup-uniform
       CtxT<R> C;
       C.M = &M;
       C.V = &V;
