@@ -9,7 +9,7 @@ import pytest
 
 import scip_amd as sa
 from oracle import oracle as orc
-from tests.util import apply_fixings, random_binary_instance, random_fixings, same_bounds
+from tests.util import apply_fixings, make_instance, random_binary_instance, random_fixings, same_bounds
 
 pytestmark = pytest.mark.gpu
 
@@ -95,6 +95,55 @@ def test_nodes_from_resident_base(gpu_ctx, mode, flags):
     nodes.close()
     base.close()
     mat.close()
+
+
+def tight_long_rows(rng, nrows, ncols):
+    """Binary rows, three in four of 2..399 nonzeros, with sides at most 3 from the activity
+    of a hidden 0/1 point: heavy fixings cut many nodes off, in different rounds."""
+    xs = rng.integers(0, 2, size=ncols).astype(float)
+    rows = []
+    for i in range(nrows):
+        n = int(rng.integers(2, 400)) if i % 4 else int(rng.integers(2, 40))
+        cs = rng.choice(ncols, size=n, replace=False)
+        a = rng.integers(1, 6, size=n).astype(float) * rng.choice([-1.0, 1.0], size=n)
+        act = float(a @ xs[cs])
+        slack = float(rng.integers(0, 4))
+        entries = [(int(c), float(x)) for c, x in zip(cs, a)]
+        rows.append((entries, -1e20, act + slack) if i % 2 else (entries, act - slack, 1e20))
+    return make_instance(rows, [0.0] * ncols, [1.0] * ncols, [1] * ncols)
+
+
+@pytest.mark.parametrize("mode", [sa.SAP_MODE_DENSE, sa.SAP_MODE_SPARSE])
+def test_nodes_cut_off_while_long_rows_are_swept(gpu_ctx, mode):
+    # rows longer than longrow = 64 are swept by a whole workgroup; vectors of the same batch
+    # are cut off in rounds 1..4 while these sweeps run, and the other vectors' results must
+    # not depend on it
+    rng = np.random.default_rng(41)
+    inst = tight_long_rows(rng, 800, 600)
+    om = orc.OMatrix.from_instance(inst)
+    rl, ru, rr = orc.propagate_jacobi(om, inst.lb, inst.ub)
+    assert not rr["cutoff"]
+    free = np.nonzero(rl < ru)[0]
+    ptr, cols, vals = random_fixings(rng, len(free), 64, 150)
+    cols = free[cols].astype(np.int32)
+    say(f"[case] long-row nodes mode={mode}")
+    gpu_ctx.set_tuning(3, 64, 50, 0, 4)
+    try:
+        mat = sa.Matrix(gpu_ctx, inst)
+        base = sa.Bounds(mat, 1)
+        base.upload(rl, ru)
+        nodes = sa.Bounds(mat, 64, mode)
+        for _ in range(2):
+            out = nodes.propagate_nodes(base, 0, ptr, cols, vals, vals, sa.default_params())
+            ncut = int((out["result"] == sa.SAP_CUTOFF).sum())
+            assert 10 <= ncut <= 54, ncut
+            for v in range(64):
+                check_node_out(om, rl, ru, ptr, cols, vals, out, v, check_rounds=True)
+        nodes.close()
+        base.close()
+        mat.close()
+    finally:
+        gpu_ctx.set_tuning()
 
 
 def test_nodes_change_capacity(gpu_ctx):
