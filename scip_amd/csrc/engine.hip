@@ -2968,50 +2968,91 @@ __global__ void k_slot_restore(Vec V, const double2* base, int64_t ncols, const 
    }
 }
 
-// marks for the first round of the admitted slots: the rows of the overlay columns (base is a
-// fixpoint), or every row
-__global__ void k_slot_mark(Mat M, Vec V, const int32_t* slots, int32_t n, const int64_t* ovptr, const int32_t* ovcol,
-   int32_t allrows)
+// one row of vector v marked for the round `first` (dense: round tag; sparse: worklist entry)
+__device__ __forceinline__ void slotMarkRow(const Mat& M, const Vec& V, Status* st, int32_t v, int32_t row,
+   int32_t first, int par, bool& any)
 {
+   const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)row;
+   if( V.vpend[v] != first )
+      V.vpend[v] = first;
+   if( V.dense )
+   {
+      V.rowmark[rflat] = first;
+      any = true;
+   }
+   else
+   {
+      const bool fresh = atomicMax(&V.rowmark[rflat], first) < first;
+      waveAppendRow(st, V.wl[par], par, fresh, M.rowbin[row], make_int2(row, v));
+   }
+}
+
+// marks for the first round of the admitted slots: the rows of the overlay columns (base is a
+// fixpoint), or every row.  Overlay columns are taken SAP_BLOCK at a time and their (column,
+// row) pairs are numbered across the workgroup (a scan of the column lengths in LDS), so a
+// wavefront appends up to 64 rows to the worklists at once: one thread per column would leave
+// most lanes idle and make one append per row of the longest column, and the appends of all
+// slots contend on the few worklist counters.  Launched with SAP_BLOCK threads.
+__global__ void __launch_bounds__(SAP_BLOCK) k_slot_mark(Mat M, Vec V, const int32_t* slots, int32_t n,
+   const int64_t* ovptr, const int32_t* ovcol, int32_t allrows)
+{
+   __shared__ int64_t soff[SAP_BLOCK + 1];   // soff[k]: pairs of the chunk's columns before column k
+   __shared__ int64_t sbeg[SAP_BLOCK];       // colptr of column k
    Status* st = V.st;
    const int32_t first = st->round + 1;
    const int par = first & 1;
+   const int t = (int)threadIdx.x;
    bool any = false;
    for( int32_t i = blockIdx.y; i < n; i += gridDim.y )
    {
       const int32_t v = slots[i];
-      const int64_t cnt = allrows ? M.nrows : ovptr[i + 1] - ovptr[i];
-      for( int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < cnt; q += (int64_t)gridDim.x * blockDim.x )
+      if( allrows )
       {
-         int64_t r0, r1;
-         if( allrows )
+         for( int64_t q = blockIdx.x * (int64_t)blockDim.x + t; q < M.nrows; q += (int64_t)gridDim.x * blockDim.x )
+            slotMarkRow(M, V, st, v, (int32_t)q, first, par, any);
+         continue;
+      }
+      const int64_t a = ovptr[i];
+      const int64_t cnt = ovptr[i + 1] - a;
+      // workgroup-uniform loop: c0 and cnt depend on the block indexes only
+      for( int64_t c0 = (int64_t)blockIdx.x * SAP_BLOCK; c0 < cnt; c0 += (int64_t)gridDim.x * SAP_BLOCK )
+      {
+         int64_t len = 0, beg = 0;
+         if( c0 + t < cnt )
          {
-            r0 = q;
-            r1 = q + 1;
+            const int32_t j = ovcol[a + c0 + t];
+            beg = M.colptr[j];
+            len = M.colptr[j + 1] - beg;
          }
-         else
+         if( t == 0 )
+            soff[0] = 0;
+         soff[t + 1] = len;
+         sbeg[t] = beg;
+         __syncthreads();
+         for( int d = 1; d < SAP_BLOCK; d <<= 1 )
          {
-            const int32_t j = ovcol[ovptr[i] + q];
-            r0 = M.colptr[j];
-            r1 = M.colptr[j + 1];
+            const int64_t add = t >= d ? soff[t + 1 - d] : 0;
+            __syncthreads();
+            soff[t + 1] += add;
+            __syncthreads();
          }
-         for( int64_t r = r0; r < r1; ++r )
+         const int64_t total = soff[SAP_BLOCK];
+         for( int64_t f = t; f < total; f += SAP_BLOCK )
          {
-            const int32_t row = allrows ? (int32_t)r : M.rowidx[r];
-            const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)row;
-            if( V.vpend[v] != first )
-               V.vpend[v] = first;
-            if( V.dense )
+            // the column of pair f: the largest k with soff[k] <= f (soff[0] = 0 <= f; such a k
+            // has soff[k + 1] > f, so a column with rows)
+            int lo = 0, hi = SAP_BLOCK - 1;
+            while( lo < hi )
             {
-               V.rowmark[rflat] = first;
-               any = true;
+               const int mid = (lo + hi + 1) >> 1;
+               if( soff[mid] <= f )
+                  lo = mid;
+               else
+                  hi = mid - 1;
             }
-            else
-            {
-               const bool fresh = atomicMax(&V.rowmark[rflat], first) < first;
-               waveAppendRow(st, V.wl[par], par, fresh, M.rowbin[row], make_int2(row, v));
-            }
+            slotMarkRow(M, V, st, v, M.rowidx[sbeg[lo] + (f - soff[lo])], first, par, any);
          }
+         __syncthreads();   // soff and sbeg are rewritten by the next chunk
       }
    }
    if( __ballot(any) != 0 && ((int)threadIdx.x & (SAP_WAVE - 1)) == 0 )
