@@ -2360,13 +2360,68 @@ __device__ __forceinline__ void markRound(const Mat& M, const Vec& V)
    }
    else
    {
+      // The touched entries are taken SAP_BLOCK at a time.  The rows of the changed columns
+      // among them are numbered across the workgroup (a scan of the column lengths in LDS),
+      // so a wavefront appends up to 64 rows to the worklists at once instead of one append
+      // per row of its longest column.  st->touched is not written in this launch, so the
+      // loop around the barriers is workgroup-uniform.
+      __shared__ int64_t soff[SAP_BLOCK + 1];   // soff[k]: rows of the chunk's columns before entry k
+      __shared__ int64_t sbeg[SAP_BLOCK];       // colptr of entry k's column
+      __shared__ int32_t svec[SAP_BLOCK];       // entry k's vector
+      const int t = (int)threadIdx.x;
       const uint32_t n = st->touched;
-      for( uint32_t idx = blockIdx.x * SAP_BLOCK + threadIdx.x; idx < n; idx += gridDim.x * SAP_BLOCK )
+      for( uint32_t c0 = blockIdx.x * SAP_BLOCK; c0 < n; c0 += gridDim.x * SAP_BLOCK )
       {
-         const int2 t = V.touched[idx];
-         const size_t flat = (size_t)t.y * (size_t)M.ncols + (size_t)t.x;
-         if( V.chgtag[flat] == round && V.vcutoff[unitOfCol(M, t.y, t.x)] == 0 )
-            markColumn(M, V, t.x, t.y, next, npar, anymarked);
+         int64_t len = 0, beg = 0;
+         int32_t v = 0;
+         if( c0 + (uint32_t)t < n )
+         {
+            const int2 e = V.touched[c0 + (uint32_t)t];
+            const size_t flat = (size_t)e.y * (size_t)M.ncols + (size_t)e.x;
+            if( V.chgtag[flat] == round && V.vcutoff[unitOfCol(M, e.y, e.x)] == 0 )
+            {
+               beg = M.colptr[e.x];
+               len = M.colptr[e.x + 1] - beg;
+               v = e.y;
+            }
+         }
+         if( t == 0 )
+            soff[0] = 0;
+         soff[t + 1] = len;
+         sbeg[t] = beg;
+         svec[t] = v;
+         __syncthreads();
+         for( int d = 1; d < SAP_BLOCK; d <<= 1 )
+         {
+            const int64_t add = t >= d ? soff[t + 1 - d] : 0;
+            __syncthreads();
+            soff[t + 1] += add;
+            __syncthreads();
+         }
+         const int64_t total = soff[SAP_BLOCK];
+         for( int64_t f = t; f < total; f += SAP_BLOCK )
+         {
+            // the entry of row f: the largest k with soff[k] <= f, which has soff[k + 1] > f
+            int lo = 0, hi = SAP_BLOCK - 1;
+            while( lo < hi )
+            {
+               const int mid = (lo + hi + 1) >> 1;
+               if( soff[mid] <= f )
+                  lo = mid;
+               else
+                  hi = mid - 1;
+            }
+            const int32_t rv = svec[lo];
+            const int32_t row = M.rowidx[sbeg[lo] + (f - soff[lo])];
+            const size_t rflat = (size_t)rv * (size_t)M.nrows + (size_t)row;
+            if( V.redundant[rflat] )
+               continue;
+            if( V.vpend[rv] != next )
+               V.vpend[rv] = next;   // every writer of this round stores the same value
+            const bool fresh = atomicMax(&V.rowmark[rflat], next) < next;
+            waveAppendRow(st, V.wl[npar], npar, fresh, M.rowbin[row], make_int2(row, rv));
+         }
+         __syncthreads();   // soff, sbeg and svec are rewritten by the next chunk
       }
    }
    if( __ballot(anymarked) != 0 && ((int)threadIdx.x & (SAP_WAVE - 1)) == 0 && st->anymark[npar] == 0 )
