@@ -65,7 +65,7 @@ struct Status
    int32_t cutoff;      // single-vector mode: infeasibility detected
    int32_t firstall;    // dense mode: the first round of the call sweeps every row
    int32_t anymark[2];  // dense mode: some row is marked for the round of this parity
-   int32_t pad;
+   int32_t rgenm1;      // generation of the locally-redundant row flags, minus one (redGen)
    uint32_t wlcount[2][SAP_NBINS];   // sparse mode worklist lengths
    uint32_t touched;                 // sparse mode touched-column count
    uint32_t pad2;
@@ -485,6 +485,20 @@ struct CtxT
    int32_t row;
 };
 
+// A row is locally redundant in the current call when its flag holds the call's generation.
+// Generations run 1..255. A call that starts generation 1 first clears the flags of every
+// vector to 0, and a slot admission clears its slot's flags, so no flag read in a call holds
+// the generation of an earlier one. A zero-filled status block reads as generation 1.
+__device__ __forceinline__ uint8_t redGen(const Vec& V)
+{
+   return (uint8_t)(V.st->rgenm1 + 1);
+}
+
+__device__ __forceinline__ bool rowRedundant(const Vec& V, size_t rflat)
+{
+   return V.redundant[rflat] == redGen(V);
+}
+
 __device__ __forceinline__ int32_t unitOfRow(const Mat& M, int32_t v, int32_t row)
 {
    return M.rowseg == NULL ? v : v * M.nseg + M.rowseg[row];
@@ -882,7 +896,7 @@ __device__ __forceinline__ void rowOutcome(const CtxT<R>& C, const RowDecision& 
    if( d.cut )
       setCutoff(C);
    else if( d.redundant )
-      C.V->redundant[(size_t)C.v * (size_t)C.M->nrows + (size_t)C.row] = 1;
+      C.V->redundant[(size_t)C.v * (size_t)C.M->nrows + (size_t)C.row] = redGen(*C.V);
 }
 
 
@@ -1105,7 +1119,7 @@ __device__ __forceinline__ bool fetchItem(const Mat& M, const Vec& V, const Item
       v = (int32_t)(it / (uint32_t)I.binlen);
       row = I.binbase + (int32_t)(it - (uint32_t)v * (uint32_t)I.binlen);
       const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)row;
-      if( !I.firstall && (V.rowmark[rflat] != I.round || V.redundant[rflat]) )
+      if( !I.firstall && (V.rowmark[rflat] != I.round || rowRedundant(V, rflat)) )
          return false;
    }
    else
@@ -1368,7 +1382,7 @@ __device__ __forceinline__ void sweepTiles(const Mat& M, const Vec& V, const Par
       const bool marked = has
          && (I.firstall
             || (V.rowmark[(size_t)v * (size_t)M.nrows + (size_t)row] == I.round
-               && !V.redundant[(size_t)v * (size_t)M.nrows + (size_t)row]))
+               && !rowRedundant(V, (size_t)v * (size_t)M.nrows + (size_t)row)))
          && (M.rowseg == NULL || V.vcutoff[unitOfRow(M, v, row)] == 0);
       if( __ballot(marked) == 0 )
          continue;
@@ -1498,7 +1512,7 @@ __device__ __forceinline__ void sweepTilesW(const Mat& M, const Vec& V, const Pa
       const bool marked = has
          && (I.firstall
             || (V.rowmark[(size_t)v * (size_t)M.nrows + (size_t)row] == I.round
-               && !V.redundant[(size_t)v * (size_t)M.nrows + (size_t)row]))
+               && !rowRedundant(V, (size_t)v * (size_t)M.nrows + (size_t)row)))
          && (M.rowseg == NULL || V.vcutoff[unitOfRow(M, v, row)] == 0);
       if( __ballot(marked) == 0 )
          continue;
@@ -2279,7 +2293,7 @@ __device__ __forceinline__ void markColumn(const Mat& M, const Vec& V, int32_t j
       {
          const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)M.rowidx[q];
          V.rowmark[rflat] = next;
-         if( !known && !V.redundant[rflat] )
+         if( !known && !rowRedundant(V, rflat) )
             known = true;
       }
       if( known )
@@ -2294,7 +2308,7 @@ __device__ __forceinline__ void markColumn(const Mat& M, const Vec& V, int32_t j
    {
       const int32_t row = M.rowidx[q];
       const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)row;
-      if( V.redundant[rflat] )
+      if( rowRedundant(V, rflat) )
          continue;
       if( V.vpend[v] != next )
          V.vpend[v] = next;   // every writer of this round stores the same value
@@ -2414,7 +2428,7 @@ __device__ __forceinline__ void markRound(const Mat& M, const Vec& V)
             const int32_t rv = svec[lo];
             const int32_t row = M.rowidx[sbeg[lo] + (f - soff[lo])];
             const size_t rflat = (size_t)rv * (size_t)M.nrows + (size_t)row;
-            if( V.redundant[rflat] )
+            if( rowRedundant(V, rflat) )
                continue;
             if( V.vpend[rv] != next )
                V.vpend[rv] = next;   // every writer of this round stores the same value
@@ -2777,7 +2791,7 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_ranged(Mat M, Vec V, Params P)
          const int32_t v = (int32_t)(i / (size_t)M.nranged);
          const int32_t row = M.rlist[i - (size_t)v * (size_t)M.nranged];
          const size_t rflat = (size_t)v * (size_t)M.nrows + (size_t)row;
-         if( (all || V.rowmark[rflat] == round) && !V.redundant[rflat] && (R || !V.vcutoff[unitOfRow(M, v, row)]) )
+         if( (all || V.rowmark[rflat] == round) && !rowRedundant(V, rflat) && (R || !V.vcutoff[unitOfRow(M, v, row)]) )
             rangedRow<R>(M, V, P, v, row);
       }
    }
@@ -2791,7 +2805,7 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_ranged(Mat M, Vec V, Params P)
          {
             const int2 it = wl[i];
             const size_t rflat = (size_t)it.y * (size_t)M.nrows + (size_t)it.x;
-            if( (M.rowflags[it.x] & 2) && !V.redundant[rflat] && (R || !V.vcutoff[unitOfRow(M, it.y, it.x)]) )
+            if( (M.rowflags[it.x] & 2) && !rowRedundant(V, rflat) && (R || !V.vcutoff[unitOfRow(M, it.y, it.x)]) )
                rangedRow<R>(M, V, P, it.y, it.x);
          }
       }
@@ -2831,11 +2845,12 @@ __global__ __launch_bounds__(SAP_STAT_SLOTS) void k_round_begin(Vec V, int32_t m
    st->touched = 0;
 }
 
-// per-call reset of the vector state; status counters for the call
-__global__ void k_call_reset(Vec V, int64_t nrows, int32_t nvec, int32_t firstall)
+// per-call reset of the vector state; status counters for the call.  gen (1..255) is the
+// call's generation of the locally-redundant row flags (redGen); the call that starts
+// generation 1 clears the first nclear flags (those of every vector) first
+__global__ void k_call_reset(Vec V, size_t nclear, int32_t nvec, int32_t firstall, int32_t gen)
 {
-   const size_t tot = (size_t)nvec * (size_t)nrows;
-   for( size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x )
+   for( size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nclear; i += (size_t)gridDim.x * blockDim.x )
       V.redundant[i] = 0;
    const size_t gi = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
    const size_t nunits = (size_t)nvec * (size_t)V.nseg;
@@ -2860,6 +2875,7 @@ __global__ void k_call_reset(Vec V, int64_t nrows, int32_t nvec, int32_t firstal
       st->done = 0;
       st->cutoff = 0;
       st->firstall = firstall;
+      st->rgenm1 = gen - 1;
       st->anymark[0] = 0;
       st->anymark[1] = 0;
       st->touched = 0;
@@ -3372,6 +3388,7 @@ struct sap_bounds
    double* stage;   // device staging for host lb/ub (2*ncols)
    int32_t roundbase;
    int32_t graphbroken;   // capture failed once: use the host-driven loop
+   int32_t redgen;        // generation of the redundant-row flags of the last call (0: none yet)
    RoundGraph graph[SAP_NGRAPHS];   // SAP_GRAPH_SHORT, SAP_GRAPH_LONG, 1 and 4 rounds
    GraphKey key;
    // persistent staging for batches and slots (grown on demand, never per call)
@@ -4788,9 +4805,20 @@ static int beginCall(sap_bounds* b, int32_t nvec, int32_t firstall)
 {
    sap_matrix* m = b->mat;
    hipStream_t s = m->ctx->stream;
-   const size_t NR = (size_t)nvec * (size_t)m->nrows;
-   k_call_reset<<<gridFor(NR > (size_t)nvec ? NR : (size_t)nvec), SAP_BLOCK, 0, s>>>(b->V, m->nrows, nvec, firstall);
+   // the redundant-row flags are tagged with a generation per call instead of being cleared
+   // per call (nvec * nrows bytes); they are cleared only when the generation starts again
+   // at 1 (the first call on b, and every 255th call after it)
+   const int32_t gen = (b->redgen >= 1 && b->redgen < 255) ? b->redgen + 1 : 1;
+   const size_t nclear = gen == 1 ? (size_t)b->nvec * (size_t)m->nrows : 0;
+   const size_t nunits = (size_t)nvec * (size_t)m->nseg;
+   size_t work = nclear > nunits ? nclear : nunits;
+   if( work < (size_t)nvec )
+      work = (size_t)nvec;
+   if( work < SAP_STAT_SLOTS )
+      work = SAP_STAT_SLOTS;
+   k_call_reset<<<gridFor(work), SAP_BLOCK, 0, s>>>(b->V, nclear, nvec, firstall, gen);
    HIPCHK(hipGetLastError());
+   b->redgen = gen;
    return SAP_OKAY;
 }
 

@@ -146,6 +146,37 @@ def test_nodes_cut_off_while_long_rows_are_swept(gpu_ctx, mode):
         gpu_ctx.set_tuning()
 
 
+def test_redundant_row_flags_over_many_calls(gpu_ctx):
+    # the locally-redundant row flags carry a generation per call and are cleared only every
+    # 255 calls: 300 calls on one bounds object, alternating two node sets and two batch sizes
+    # (so vectors sit out calls), must each give the answer of the first call with those inputs
+    inst, om, rl, ru, ptr, cols, vals = node_problem(7, nvec=40, maxfix=12)
+    vals2 = 1.0 - vals
+    mat = sa.Matrix(gpu_ctx, inst)
+    base = sa.Bounds(mat, 1)
+    base.upload(rl, ru)
+    nodes = sa.Bounds(mat, 40)
+    p = sa.default_params(flags=sa.SAP_FLAG_BASE_IS_FIXPOINT)
+    cases = [(ptr, vals), (ptr, vals2), (ptr[:21], vals), (ptr[:21], vals2)]
+    first = [None] * len(cases)
+    say("[case] 300 node calls on one bounds object")
+    for call in range(300):
+        k = call % len(cases)
+        pp, vv = cases[k]
+        nv = len(pp) - 1
+        out = nodes.propagate_nodes(base, 0, pp, cols[:pp[-1]], vv[:pp[-1]], vv[:pp[-1]], p)
+        if first[k] is None:
+            first[k] = out
+            for v in range(nv):
+                check_node_out(om, rl, ru, pp, cols, vv, out, v, check_rounds=False)
+            continue
+        for key in ("result", "rounds", "chgptr", "chgcol", "chglb", "chgub"):
+            assert np.array_equal(out[key], first[k][key]), (call, key)
+    nodes.close()
+    base.close()
+    mat.close()
+
+
 def test_nodes_change_capacity(gpu_ctx):
     inst, om, rl, ru, ptr, cols, vals = node_problem(6, nvec=16)
     mat = sa.Matrix(gpu_ctx, inst)
