@@ -3059,39 +3059,58 @@ __device__ __forceinline__ void slotMarkRow(const Mat& M, const Vec& V, Status* 
 }
 
 // marks for the first round of the admitted slots: the rows of the overlay columns (base is a
-// fixpoint), or every row.  Overlay columns are taken SAP_BLOCK at a time and their (column,
-// row) pairs are numbered across the workgroup (a scan of the column lengths in LDS), so a
-// wavefront appends up to 64 rows to the worklists at once: one thread per column would leave
-// most lanes idle and make one append per row of the longest column, and the appends of all
-// slots contend on the few worklist counters.  Launched with SAP_BLOCK threads.
+// fixpoint), or every row (grid: rows per slot in x, slots in y).  The overlay entries of all
+// slots (ovptr[0] = 0 .. ovptr[n]) are taken SAP_BLOCK at a time over a one-dimensional grid,
+// each thread finding its slot by binary search in ovptr, and the (column, row) pairs of a
+// chunk are numbered across the workgroup (a scan of the column lengths in LDS).  So a
+// wavefront appends up to 64 rows to the worklists at once, also when every slot has a single
+// overlay column (probing): a workgroup or thread per slot or column would leave most lanes
+// idle and make one append per row, and the appends of all slots contend on the few worklist
+// counters.  Launched with SAP_BLOCK threads.
 __global__ void __launch_bounds__(SAP_BLOCK) k_slot_mark(Mat M, Vec V, const int32_t* slots, int32_t n,
    const int64_t* ovptr, const int32_t* ovcol, int32_t allrows)
 {
    __shared__ int64_t soff[SAP_BLOCK + 1];   // soff[k]: pairs of the chunk's columns before column k
    __shared__ int64_t sbeg[SAP_BLOCK];       // colptr of column k
+   __shared__ int32_t svec[SAP_BLOCK];       // vector of column k
    Status* st = V.st;
    const int32_t first = st->round + 1;
    const int par = first & 1;
    const int t = (int)threadIdx.x;
    bool any = false;
-   for( int32_t i = blockIdx.y; i < n; i += gridDim.y )
+   if( allrows )
    {
-      const int32_t v = slots[i];
-      if( allrows )
+      for( int32_t i = blockIdx.y; i < n; i += gridDim.y )
       {
+         const int32_t v = slots[i];
          for( int64_t q = blockIdx.x * (int64_t)blockDim.x + t; q < M.nrows; q += (int64_t)gridDim.x * blockDim.x )
             slotMarkRow(M, V, st, v, (int32_t)q, first, par, any);
-         continue;
       }
-      const int64_t a = ovptr[i];
-      const int64_t cnt = ovptr[i + 1] - a;
-      // workgroup-uniform loop: c0 and cnt depend on the block indexes only
-      for( int64_t c0 = (int64_t)blockIdx.x * SAP_BLOCK; c0 < cnt; c0 += (int64_t)gridDim.x * SAP_BLOCK )
+   }
+   else if( blockIdx.y == 0 && n > 0 )   // one-dimensional grid over the overlay entries
+   {
+      const int64_t nov = ovptr[n];
+      // workgroup-uniform loop: c0 and nov depend on the block index and the arguments only
+      for( int64_t c0 = (int64_t)blockIdx.x * SAP_BLOCK; c0 < nov; c0 += (int64_t)gridDim.x * SAP_BLOCK )
       {
          int64_t len = 0, beg = 0;
-         if( c0 + t < cnt )
+         int32_t v = 0;
+         const int64_t e = c0 + t;
+         if( e < nov )
          {
-            const int32_t j = ovcol[a + c0 + t];
+            // the slot of entry e: the largest i < n with ovptr[i] <= e (ovptr[0] = 0 <= e; such
+            // an i has ovptr[i + 1] > e, as ovptr[n] = nov > e)
+            int32_t lo = 0, hi = n - 1;
+            while( lo < hi )
+            {
+               const int32_t mid = lo + (hi - lo + 1) / 2;
+               if( ovptr[mid] <= e )
+                  lo = mid;
+               else
+                  hi = mid - 1;
+            }
+            v = slots[lo];
+            const int32_t j = ovcol[e];
             beg = M.colptr[j];
             len = M.colptr[j + 1] - beg;
          }
@@ -3099,6 +3118,7 @@ __global__ void __launch_bounds__(SAP_BLOCK) k_slot_mark(Mat M, Vec V, const int
             soff[0] = 0;
          soff[t + 1] = len;
          sbeg[t] = beg;
+         svec[t] = v;
          __syncthreads();
          for( int d = 1; d < SAP_BLOCK; d <<= 1 )
          {
@@ -3121,9 +3141,9 @@ __global__ void __launch_bounds__(SAP_BLOCK) k_slot_mark(Mat M, Vec V, const int
                else
                   hi = mid - 1;
             }
-            slotMarkRow(M, V, st, v, M.rowidx[sbeg[lo] + (f - soff[lo])], first, par, any);
+            slotMarkRow(M, V, st, svec[lo], M.rowidx[sbeg[lo] + (f - soff[lo])], first, par, any);
          }
-         __syncthreads();   // soff and sbeg are rewritten by the next chunk
+         __syncthreads();   // soff, sbeg and svec are rewritten by the next chunk
       }
    }
    if( __ballot(any) != 0 && ((int)threadIdx.x & (SAP_WAVE - 1)) == 0 )
@@ -5193,7 +5213,7 @@ static int admitSlots(sap_bounds* b, int32_t n, const int32_t* hslots, const int
       k_slot_overlay<<<slotGrid((size_t)(nov / n + 1), n), SAP_BLOCK, 0, s>>>(b->V, m->ncols, dslots, n, b->ovptr,
          b->ovcol, b->ovlb, b->ovub);
    if( allrows || nov > 0 )
-      k_slot_mark<<<slotGrid(allrows ? (size_t)m->nrows : (size_t)(nov / n + 1), n), SAP_BLOCK, 0, s>>>(M, b->V,
+      k_slot_mark<<<allrows ? slotGrid((size_t)m->nrows, n) : dim3(gridFor((size_t)nov)), SAP_BLOCK, 0, s>>>(M, b->V,
          dslots, n, b->ovptr, b->ovcol, allrows);
    HIPCHK(hipGetLastError());
    return SAP_OKAY;
@@ -5348,7 +5368,7 @@ static int batchCall(sap_bounds* b, const sap_params* params, int32_t nvec, cons
          k_slot_overlay<<<slotGrid((size_t)(nov / nvec + 1), nvec), SAP_BLOCK, 0, s>>>(b->V, m->ncols, b->slotidx,
             nvec, b->ovptr, b->ovcol, b->ovlb, b->ovub);
       if( fromfix && nov > 0 )
-         k_slot_mark<<<slotGrid((size_t)(nov / nvec + 1), nvec), SAP_BLOCK, 0, s>>>(toMat(m), b->V, b->slotidx, nvec,
+         k_slot_mark<<<dim3(gridFor((size_t)nov)), SAP_BLOCK, 0, s>>>(toMat(m), b->V, b->slotidx, nvec,
             b->ovptr, b->ovcol, 0);
       rc = hipToRc(hipGetLastError());
    }
