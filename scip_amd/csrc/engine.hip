@@ -3068,11 +3068,13 @@ __device__ __forceinline__ void slotMarkRow(const Mat& M, const Vec& V, Status* 
 // idle and make one append per row, and the appends of all slots contend on the few worklist
 // counters.  Launched with SAP_BLOCK threads.
 __global__ void __launch_bounds__(SAP_BLOCK) k_slot_mark(Mat M, Vec V, const int32_t* slots, int32_t n,
-   const int64_t* ovptr, const int32_t* ovcol, int32_t allrows)
+   const int64_t* ovptr, const int32_t* ovcol, int32_t allrows, int32_t chunk)
 {
    __shared__ int64_t soff[SAP_BLOCK + 1];   // soff[k]: pairs of the chunk's columns before column k
    __shared__ int64_t sbeg[SAP_BLOCK];       // colptr of column k
    __shared__ int32_t svec[SAP_BLOCK];       // vector of column k
+   __shared__ uint32_t scnt[SAP_NBINS];      // fresh rows of one pass per length bin
+   __shared__ uint32_t sbase[SAP_NBINS];     // their first worklist position
    Status* st = V.st;
    const int32_t first = st->round + 1;
    const int par = first & 1;
@@ -3091,12 +3093,12 @@ __global__ void __launch_bounds__(SAP_BLOCK) k_slot_mark(Mat M, Vec V, const int
    {
       const int64_t nov = ovptr[n];
       // workgroup-uniform loop: c0 and nov depend on the block index and the arguments only
-      for( int64_t c0 = (int64_t)blockIdx.x * SAP_BLOCK; c0 < nov; c0 += (int64_t)gridDim.x * SAP_BLOCK )
+      for( int64_t c0 = (int64_t)blockIdx.x * chunk; c0 < nov; c0 += (int64_t)gridDim.x * chunk )
       {
          int64_t len = 0, beg = 0;
          int32_t v = 0;
          const int64_t e = c0 + t;
-         if( e < nov )
+         if( t < chunk && e < nov )
          {
             // the slot of entry e: the largest i < n with ovptr[i] <= e (ovptr[0] = 0 <= e; such
             // an i has ovptr[i + 1] > e, as ovptr[n] = nov > e)
@@ -3128,20 +3130,57 @@ __global__ void __launch_bounds__(SAP_BLOCK) k_slot_mark(Mat M, Vec V, const int
             __syncthreads();
          }
          const int64_t total = soff[SAP_BLOCK];
-         for( int64_t f = t; f < total; f += SAP_BLOCK )
+         // passes of SAP_BLOCK pairs (workgroup-uniform: total is read from LDS); the fresh
+         // rows of a pass are counted per length bin in LDS and appended with one global
+         // atomic per bin and workgroup
+         for( int64_t f0 = 0; f0 < total; f0 += SAP_BLOCK )
          {
-            // the column of pair f: the largest k with soff[k] <= f (soff[0] = 0 <= f; such a k
-            // has soff[k + 1] > f, so a column with rows)
-            int lo = 0, hi = SAP_BLOCK - 1;
-            while( lo < hi )
+            const int64_t f = f0 + t;
+            bool fresh = false;
+            int bin = 0;
+            int2 item = make_int2(0, 0);
+            if( t < SAP_NBINS )
+               scnt[t] = 0;
+            if( f < total )
             {
-               const int mid = (lo + hi + 1) >> 1;
-               if( soff[mid] <= f )
-                  lo = mid;
+               // the column of pair f: the largest k with soff[k] <= f (soff[0] = 0 <= f; such
+               // a k has soff[k + 1] > f, so a column with rows)
+               int lo = 0, hi = SAP_BLOCK - 1;
+               while( lo < hi )
+               {
+                  const int mid = (lo + hi + 1) >> 1;
+                  if( soff[mid] <= f )
+                     lo = mid;
+                  else
+                     hi = mid - 1;
+               }
+               const int32_t rv = svec[lo];
+               const int32_t row = M.rowidx[sbeg[lo] + (f - soff[lo])];
+               const size_t rflat = (size_t)rv * (size_t)M.nrows + (size_t)row;
+               if( V.vpend[rv] != first )
+                  V.vpend[rv] = first;
+               if( V.dense )
+               {
+                  V.rowmark[rflat] = first;
+                  any = true;
+               }
                else
-                  hi = mid - 1;
+               {
+                  fresh = atomicMax(&V.rowmark[rflat], first) < first;
+                  bin = M.rowbin[row];
+                  item = make_int2(row, rv);
+               }
             }
-            slotMarkRow(M, V, st, svec[lo], M.rowidx[sbeg[lo] + (f - soff[lo])], first, par, any);
+            __syncthreads();   // scnt cleared
+            uint32_t pos = 0;
+            if( fresh )
+               pos = atomicAdd(&scnt[bin], 1u);
+            __syncthreads();   // scnt complete
+            if( t < SAP_NBINS && scnt[t] > 0 )
+               sbase[t] = atomicAdd(&st->wlcount[par][t], scnt[t]);
+            __syncthreads();   // sbase set (read below, rewritten only after the next pass's second barrier)
+            if( fresh )
+               V.wl[par][bin][sbase[bin] + pos] = item;
          }
          __syncthreads();   // soff, sbeg and svec are rewritten by the next chunk
       }
@@ -5158,6 +5197,22 @@ static int stageOverlay(sap_bounds* b, int32_t n, const int64_t* ovptr, const in
    return SAP_OKAY;
 }
 
+// overlay entries per workgroup chunk of k_slot_mark (1..SAP_BLOCK) and its grid: about 256
+// workgroups whatever the number of entries, so few entries (one per slot when probing) still
+// spread over the chip and many entries make few worklist appends
+static int32_t slotMarkChunk(int64_t nov)
+{
+   const int64_t c = (nov + 255) / 256;
+   return (int32_t)(c < 1 ? 1 : (c > SAP_BLOCK ? SAP_BLOCK : c));
+}
+
+static unsigned slotMarkBlocks(int64_t nov)
+{
+   const int64_t c = slotMarkChunk(nov);
+   const int64_t g = (nov + c - 1) / c;
+   return (unsigned)(g < 1 ? 1 : (g > 8192 ? 8192 : g));
+}
+
 static dim3 slotGrid(size_t perslot, int32_t n)
 {
    return dim3(gridFor(perslot, 64), (unsigned)std::min<int32_t>(n > 0 ? n : 1, 65535));
@@ -5213,8 +5268,8 @@ static int admitSlots(sap_bounds* b, int32_t n, const int32_t* hslots, const int
       k_slot_overlay<<<slotGrid((size_t)(nov / n + 1), n), SAP_BLOCK, 0, s>>>(b->V, m->ncols, dslots, n, b->ovptr,
          b->ovcol, b->ovlb, b->ovub);
    if( allrows || nov > 0 )
-      k_slot_mark<<<allrows ? slotGrid((size_t)m->nrows, n) : dim3(gridFor((size_t)nov)), SAP_BLOCK, 0, s>>>(M, b->V,
-         dslots, n, b->ovptr, b->ovcol, allrows);
+      k_slot_mark<<<allrows ? slotGrid((size_t)m->nrows, n) : dim3(slotMarkBlocks(nov)), SAP_BLOCK, 0, s>>>(M, b->V,
+         dslots, n, b->ovptr, b->ovcol, allrows, slotMarkChunk(nov));
    HIPCHK(hipGetLastError());
    return SAP_OKAY;
 }
@@ -5368,8 +5423,8 @@ static int batchCall(sap_bounds* b, const sap_params* params, int32_t nvec, cons
          k_slot_overlay<<<slotGrid((size_t)(nov / nvec + 1), nvec), SAP_BLOCK, 0, s>>>(b->V, m->ncols, b->slotidx,
             nvec, b->ovptr, b->ovcol, b->ovlb, b->ovub);
       if( fromfix && nov > 0 )
-         k_slot_mark<<<dim3(gridFor((size_t)nov)), SAP_BLOCK, 0, s>>>(toMat(m), b->V, b->slotidx, nvec,
-            b->ovptr, b->ovcol, 0);
+         k_slot_mark<<<dim3(slotMarkBlocks(nov)), SAP_BLOCK, 0, s>>>(toMat(m), b->V, b->slotidx, nvec,
+            b->ovptr, b->ovcol, 0, slotMarkChunk(nov));
       rc = hipToRc(hipGetLastError());
    }
    if( rc == SAP_OKAY && !fromfix && !b->V.dense )
