@@ -468,6 +468,30 @@ __device__ __forceinline__ void waveAppendRow(Status* st, int2* const* wl, int n
    }
 }
 
+// one pass of (row, vector) worklist appends by a whole workgroup of SAP_BLOCK threads: the
+// fresh rows are counted per length bin in LDS (scnt, sbase: SAP_NBINS entries each) and
+// appended with one global atomic per bin.  Every thread of the workgroup calls it (barriers
+// inside).  scnt[t] is cleared by thread t, which also was its only reader in the previous
+// pass; sbase is rewritten only after the second barrier, which every reader of the previous
+// pass has passed.
+__device__ __forceinline__ void blockAppendRows(Status* st, int2* const* wl, int npar, bool fresh, int bin,
+   int2 item, uint32_t* scnt, uint32_t* sbase)
+{
+   const int t = (int)threadIdx.x;
+   if( t < SAP_NBINS )
+      scnt[t] = 0;
+   __syncthreads();
+   uint32_t pos = 0;
+   if( fresh )
+      pos = atomicAdd(&scnt[bin], 1u);
+   __syncthreads();
+   if( t < SAP_NBINS && scnt[t] > 0 )
+      sbase[t] = atomicAdd(&st->wlcount[npar][t], scnt[t]);
+   __syncthreads();
+   if( fresh )
+      wl[bin][sbase[bin] + pos] = item;
+}
+
 // ------------------------------------------------------------------------------------------
 // candidates
 // ------------------------------------------------------------------------------------------
@@ -2382,6 +2406,8 @@ __device__ __forceinline__ void markRound(const Mat& M, const Vec& V)
       __shared__ int64_t soff[SAP_BLOCK + 1];   // soff[k]: rows of the chunk's columns before entry k
       __shared__ int64_t sbeg[SAP_BLOCK];       // colptr of entry k's column
       __shared__ int32_t svec[SAP_BLOCK];       // entry k's vector
+      __shared__ uint32_t scnt[SAP_NBINS];
+      __shared__ uint32_t sbase[SAP_NBINS];
       const int t = (int)threadIdx.x;
       const uint32_t n = st->touched;
       for( uint32_t c0 = blockIdx.x * SAP_BLOCK; c0 < n; c0 += gridDim.x * SAP_BLOCK )
@@ -2413,27 +2439,39 @@ __device__ __forceinline__ void markRound(const Mat& M, const Vec& V)
             __syncthreads();
          }
          const int64_t total = soff[SAP_BLOCK];
-         for( int64_t f = t; f < total; f += SAP_BLOCK )
+         // passes of SAP_BLOCK rows (workgroup-uniform: total is read from LDS), appended per
+         // workgroup
+         for( int64_t f0 = 0; f0 < total; f0 += SAP_BLOCK )
          {
-            // the entry of row f: the largest k with soff[k] <= f, which has soff[k + 1] > f
-            int lo = 0, hi = SAP_BLOCK - 1;
-            while( lo < hi )
+            const int64_t f = f0 + t;
+            bool fresh = false;
+            int bin = 0;
+            int2 item = make_int2(0, 0);
+            if( f < total )
             {
-               const int mid = (lo + hi + 1) >> 1;
-               if( soff[mid] <= f )
-                  lo = mid;
-               else
-                  hi = mid - 1;
+               // the entry of row f: the largest k with soff[k] <= f, which has soff[k + 1] > f
+               int lo = 0, hi = SAP_BLOCK - 1;
+               while( lo < hi )
+               {
+                  const int mid = (lo + hi + 1) >> 1;
+                  if( soff[mid] <= f )
+                     lo = mid;
+                  else
+                     hi = mid - 1;
+               }
+               const int32_t rv = svec[lo];
+               const int32_t row = M.rowidx[sbeg[lo] + (f - soff[lo])];
+               const size_t rflat = (size_t)rv * (size_t)M.nrows + (size_t)row;
+               if( !rowRedundant(V, rflat) )
+               {
+                  if( V.vpend[rv] != next )
+                     V.vpend[rv] = next;   // every writer of this round stores the same value
+                  fresh = atomicMax(&V.rowmark[rflat], next) < next;
+                  bin = M.rowbin[row];
+                  item = make_int2(row, rv);
+               }
             }
-            const int32_t rv = svec[lo];
-            const int32_t row = M.rowidx[sbeg[lo] + (f - soff[lo])];
-            const size_t rflat = (size_t)rv * (size_t)M.nrows + (size_t)row;
-            if( rowRedundant(V, rflat) )
-               continue;
-            if( V.vpend[rv] != next )
-               V.vpend[rv] = next;   // every writer of this round stores the same value
-            const bool fresh = atomicMax(&V.rowmark[rflat], next) < next;
-            waveAppendRow(st, V.wl[npar], npar, fresh, M.rowbin[row], make_int2(row, rv));
+            blockAppendRows(st, V.wl[npar], npar, fresh, bin, item, scnt, sbase);
          }
          __syncthreads();   // soff, sbeg and svec are rewritten by the next chunk
       }
@@ -3139,8 +3177,6 @@ __global__ void __launch_bounds__(SAP_BLOCK) k_slot_mark(Mat M, Vec V, const int
             bool fresh = false;
             int bin = 0;
             int2 item = make_int2(0, 0);
-            if( t < SAP_NBINS )
-               scnt[t] = 0;
             if( f < total )
             {
                // the column of pair f: the largest k with soff[k] <= f (soff[0] = 0 <= f; such
@@ -3171,16 +3207,7 @@ __global__ void __launch_bounds__(SAP_BLOCK) k_slot_mark(Mat M, Vec V, const int
                   item = make_int2(row, rv);
                }
             }
-            __syncthreads();   // scnt cleared
-            uint32_t pos = 0;
-            if( fresh )
-               pos = atomicAdd(&scnt[bin], 1u);
-            __syncthreads();   // scnt complete
-            if( t < SAP_NBINS && scnt[t] > 0 )
-               sbase[t] = atomicAdd(&st->wlcount[par][t], scnt[t]);
-            __syncthreads();   // sbase set (read below, rewritten only after the next pass's second barrier)
-            if( fresh )
-               V.wl[par][bin][sbase[bin] + pos] = item;
+            blockAppendRows(st, V.wl[par], par, fresh, bin, item, scnt, sbase);
          }
          __syncthreads();   // soff, sbeg and svec are rewritten by the next chunk
       }
