@@ -133,8 +133,8 @@ int sap_matrix_free(sap_matrix* mat);
 /* Autotuner: builds the system restricted to a sample of samplerows rows (all rows if fewer;
  * chosen by seed) once per candidate tuning ((rowbins, tilebins) in (0,3) (0,4) (0,5) (0,6)
  * (2,6) (3,6) x tilewidth 2/4 x longrow 1024/4096/16384 x capscale 25/50/100), propagates lb/ub
- * with params on it and returns the candidate with
- * the least device time in best (times in ms: best_ms, default_ms for the default tuning).
+ * with params on it (rounds driven from the host) and returns the candidate with the least
+ * row-sweep time in best (times in ms: best_ms, default_ms for the default tuning).
  * Results are identical under every tuning; only the launch shape differs.  The context's
  * own tuning is left unchanged: apply the result with sap_context_set_tuning. */
 int sap_autotune(sap_context* ctx, int64_t nrows, int64_t ncols, const int64_t* rowptr, const int32_t* colidx,

@@ -3398,6 +3398,8 @@ struct sap_context
    sap_tuning tuning;       // applied to matrices created on this context
    int64_t nmatrices;       // live matrices (a context outlives its matrices)
    hipStream_t stream;
+   hipStream_t cstream;     // round graphs are captured here: never launched on, so a capture
+                            // never begins on a stream with work of other bound sets in flight
    hipEvent_t ev[4];
    Status* hstatus;   // pinned
 };
@@ -3454,14 +3456,16 @@ struct RoundGraph
    hipEvent_t ev[2 * SAP_GRAPH_LONG];   // around each captured sweep launch
 };
 
+// everything a captured round bakes into its kernel arguments: the parameters the kernels
+// read, the vector state (device pointers included: a buffer that is reallocated after the
+// capture must not be reached through the old pointer), the matrix and the sweep layout.  The
+// raw bytes are compared, so padding only ever causes a spurious recapture.
 struct GraphKey
 {
    Params P;
-   int32_t nvec;
-   int32_t single;
-   int32_t dense;
-   int32_t logchg;
-   int32_t reasons;
+   Vec V;
+   Mat M;
+   Layout L;
 };
 
 struct sap_bounds
@@ -3508,6 +3512,21 @@ struct sap_bounds
    std::vector<const sap_bounds*>* srcb;
    std::vector<int32_t>* srcv;
    std::vector<int64_t>* srcgen;
+};
+
+// the fields of the vector state that one call narrows (the vectors it works on, the
+// single-vector shortcut, change logging); only these are put back after the call, so that a
+// buffer allocated during the call (ranged-row flags) stays attached to the bound set
+struct CallScope
+{
+   int32_t nvec, single, logchg;
+   explicit CallScope(const Vec& V) : nvec(V.nvec), single(V.single), logchg(V.logchg) {}
+   void restore(Vec& V) const
+   {
+      V.nvec = nvec;
+      V.single = single;
+      V.logchg = logchg;
+   }
 };
 
 static int hipToRc(hipError_t e)
@@ -3745,6 +3764,7 @@ int sap_context_create(int32_t device, sap_context** out)
       || c->wallclock_khz <= 0 )
       c->wallclock_khz = 100000;   // 100 MHz reference clock
    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+   HIPCHK(hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));
    for( int i = 0; i < 4; ++i )
       HIPCHK(hipEventCreate(&c->ev[i]));
    HIPCHK(hipHostMalloc((void**)&c->hstatus, sizeof(Status), hipHostMallocDefault));
@@ -3766,6 +3786,7 @@ int sap_context_free(sap_context* c)
    for( int i = 0; i < 4; ++i )
       (void)hipEventDestroy(c->ev[i]);
    (void)hipHostFree(c->hstatus);
+   (void)hipStreamDestroy(c->cstream);
    (void)hipStreamDestroy(c->stream);
    free(c);
    return SAP_OKAY;
@@ -4267,15 +4288,20 @@ int sap_autotune(sap_context* ctx, int64_t nrows, int64_t ncols, const int64_t* 
                   sint.data(), slb.data(), sub.data(), 1, NULL, NULL, &m);
                if( rc != SAP_OKAY )
                   break;
+               // candidates differ only in the sweep's layout: they are ranked by the sweep time,
+               // measured with HIP events around every sweep launch of the host-driven round loop
+               // (no round graphs are captured for a matrix that lives for three calls)
+               sap_params hp = *params;
+               hp.flags |= SAP_FLAG_HOSTLOOP;
                double tmin = 1e300;
                for( int rep = 0; rep < 3 && rc == SAP_OKAY; ++rep )
                {
                   wl = slb;
                   wu = sub;
                   sap_stats st;
-                  rc = sap_propagate(m, params, wl.data(), wu.data(), &st);
-                  if( rep > 0 )   // the first call includes one-time set-up (graph capture)
-                     tmin = std::min(tmin, st.device_ms);
+                  rc = sap_propagate(m, &hp, wl.data(), wu.data(), &st);
+                  if( rep > 0 )   // the first call includes one-time set-up
+                     tmin = std::min(tmin, st.sweep_ms);
                }
                sap_matrix_free(m);
                if( t.tilebins == def.tilebins && t.longrow == def.longrow && t.capscale == def.capscale
@@ -4682,11 +4708,11 @@ static void fillStats(sap_context* c, double tot, double sweepms, int64_t launch
    stats->sweep_launches = launches;
 }
 
-// enqueue one round: control, sweep (between two events when given), column step
-static void enqueueRound(sap_bounds* b, const Mat& M, const Params& P, const Layout& L, hipEvent_t e0, hipEvent_t e1)
+// enqueue one round on stream s: control, sweep (between two events when given), column step
+static void enqueueRound(sap_bounds* b, const Mat& M, const Params& P, const Layout& L, hipStream_t s, hipEvent_t e0,
+   hipEvent_t e1)
 {
    sap_matrix* m = b->mat;
-   hipStream_t s = m->ctx->stream;
    const int nsweep = L.blockstart[SAP_NKINDS];
    const unsigned colgrid = gridFor((size_t)b->V.nvec * (size_t)m->ncols, 4096);
    k_round_begin<<<1, SAP_STAT_SLOTS, 0, s>>>(b->V, P.maxrounds);
@@ -4742,7 +4768,7 @@ static int runRoundsHost(sap_bounds* b, const Params& P, sap_stats* stats)
          fprintf(stderr, "[scip_amd] round cap %d exceeded\n", SAP_ROUND_CAP);
          return SAP_ERROR;
       }
-      enqueueRound(b, M, P, L, c->ev[2], c->ev[3]);
+      enqueueRound(b, M, P, L, s, c->ev[2], c->ev[3]);
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(c->hstatus, b->dstatus, sizeof(Status), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
@@ -4768,17 +4794,18 @@ static int runRoundsHost(sap_bounds* b, const Params& P, sap_stats* stats)
    return SAP_OKAY;
 }
 
-// capture `rounds` rounds into g; on failure the stream is left out of capture mode
+// capture `rounds` rounds into g on the context's capture stream, which nothing is ever
+// launched on; on failure the stream is left out of capture mode
 static bool captureGraph(sap_bounds* b, const Params& P, int rounds, RoundGraph* g)
 {
    sap_matrix* m = b->mat;
-   hipStream_t s = m->ctx->stream;
+   hipStream_t s = m->ctx->cstream;
    const Mat M = toMat(m);
    const Layout L = makeLayout(b);
    if( hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess )
       return false;
    for( int r = 0; r < rounds; ++r )
-      enqueueRound(b, M, P, L, NULL, NULL);
+      enqueueRound(b, M, P, L, s, NULL, NULL);
    hipGraph_t graph = NULL;
    const hipError_t e1 = hipStreamEndCapture(s, &graph);
    if( e1 != hipSuccess || graph == NULL || hipGetLastError() != hipSuccess )
@@ -4796,28 +4823,26 @@ static bool captureGraph(sap_bounds* b, const Params& P, int rounds, RoundGraph*
    return true;
 }
 
+static void dropGraphs(sap_bounds* b);
+
 static bool ensureGraphs(sap_bounds* b, const Params& P)
 {
    GraphKey k;
    memset(&k, 0, sizeof(k));
-   k.P = P;
-   k.P.flags = P.flags & SAP_FLAG_RANGEDROWS;   // the other flags steer the host side only
-   k.nvec = b->V.nvec;
-   k.single = b->V.single;
-   k.dense = b->V.dense;
-   k.logchg = b->V.logchg;
-   k.reasons = b->V.reasons;
+   Params kp = P;
+   kp.flags = P.flags & SAP_FLAG_RANGEDROWS;   // the other flags steer the host side only
+   const Mat M = toMat(b->mat);
+   const Layout L = makeLayout(b);
+   memcpy(&k.P, &kp, sizeof(Params));
+   memcpy(&k.V, &b->V, sizeof(Vec));
+   memcpy(&k.M, &M, sizeof(Mat));
+   memcpy(&k.L, &L, sizeof(Layout));
    bool valid = memcmp(&k, &b->key, sizeof(k)) == 0;
    for( int g = 0; g < SAP_NGRAPHS; ++g )
       valid = valid && b->graph[g].valid;
    if( valid )
       return true;
-   for( int g = 0; g < SAP_NGRAPHS; ++g )
-      if( b->graph[g].valid )
-      {
-         (void)hipGraphExecDestroy(b->graph[g].exec);
-         b->graph[g].valid = 0;
-      }
+   dropGraphs(b);
    if( !captureGraph(b, P, SAP_GRAPH_SHORT, &b->graph[0]) || !captureGraph(b, P, SAP_GRAPH_LONG, &b->graph[1])
       || !captureGraph(b, P, 1, &b->graph[2]) || !captureGraph(b, P, 4, &b->graph[3]) )
    {
@@ -4955,7 +4980,7 @@ int sap_propagate_resident(sap_bounds* b, const sap_params* params, sap_stats* s
    const Params P = toParams(params);
    if( b->slotsmode )
       return SAP_INVALIDCALL;   // the vectors serve as slots of continuous batching
-   Vec saved = b->V;
+   const CallScope saved(b->V);
    b->V.nvec = 1;
    b->V.single = (m->nseg == 1) ? 1 : 0;   // packed instances: a cutoff stops only its instance
    int rc = beginCall(b, 1, b->V.dense);
@@ -4963,7 +4988,7 @@ int sap_propagate_resident(sap_bounds* b, const sap_params* params, sap_stats* s
       rc = queueAllRows(b, 1);
    if( rc == SAP_OKAY )
       rc = runRounds(b, P, stats);
-   b->V = saved;
+   saved.restore(b->V);
    touchedVectors(b, 0);
    RCCHK(rc);
    if( stats )
@@ -4971,8 +4996,16 @@ int sap_propagate_resident(sap_bounds* b, const sap_params* params, sap_stats* s
    return SAP_OKAY;
 }
 
+// destroy the round graphs of b; the stream they are launched on is drained first, so no
+// launch of them can still be in flight
 static void dropGraphs(sap_bounds* b)
 {
+   bool any = false;
+   for( int g = 0; g < SAP_NGRAPHS; ++g )
+      any = any || b->graph[g].valid;
+   if( !any )
+      return;
+   (void)hipStreamSynchronize(b->mat->ctx->stream);
    for( int g = 0; g < SAP_NGRAPHS; ++g )
       if( b->graph[g].valid )
       {
@@ -5128,6 +5161,7 @@ int sap_propagate(sap_matrix* m, const sap_params* params, double* lb, double* u
       *stats = local;
    return SAP_OKAY;
 }
+
 
 }   // extern "C"
 
@@ -5439,7 +5473,7 @@ static int batchCall(sap_bounds* b, const sap_params* params, int32_t nvec, cons
    HIPCHK(hipMemcpyAsync(b->slotidx, ident.data(), sizeof(int32_t) * (size_t)nvec, hipMemcpyHostToDevice, s));
    // the call works on the first nvec vectors; the single-vector cutoff shortcut only
    // applies when the batch is one vector
-   Vec saved = b->V;
+   const CallScope saved(b->V);
    b->V.nvec = nvec;
    b->V.single = (nvec == 1 && m->nseg == 1) ? 1 : 0;
    b->V.logchg = 1;
@@ -5458,7 +5492,7 @@ static int batchCall(sap_bounds* b, const sap_params* params, int32_t nvec, cons
       rc = queueAllRows(b, nvec);
    if( rc == SAP_OKAY )
       rc = runRounds(b, P, stats);
-   b->V = saved;
+   saved.restore(b->V);
    return rc;
 }
 
@@ -5924,6 +5958,8 @@ int sap_slots_launch(sap_bounds* b, const sap_params* params, int32_t rounds)
    if( b == NULL || !b->slotsmode || (rounds != SAP_STEP_SHORT && rounds != SAP_STEP_LONG) )
       return b != NULL && !b->slotsmode ? SAP_INVALIDCALL : SAP_INVALIDDATA;
    RCCHK(checkParams(params));
+   if( b->inflight )
+      return SAP_INVALIDCALL;   // the previous step has not been waited for
    sap_matrix* m = b->mat;
    sap_context* c = m->ctx;
    HIPCHK(hipSetDevice(c->device));
@@ -5940,7 +5976,7 @@ int sap_slots_launch(sap_bounds* b, const sap_params* params, int32_t rounds)
       const Mat M = toMat(m);
       const Layout L = makeLayout(b);
       for( int r = 0; r < rounds; ++r )
-         enqueueRound(b, M, P, L, NULL, NULL);
+         enqueueRound(b, M, P, L, s, NULL, NULL);
       HIPCHK(hipGetLastError());
    }
    HIPCHK(hipMemcpyAsync(b->hstat, b->dstatus, sizeof(Status), hipMemcpyDeviceToHost, s));
