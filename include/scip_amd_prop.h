@@ -71,6 +71,11 @@ typedef struct sap_params
  * gcd-based fixings, bounds and infeasibility test, as propagation applies it after the root
  * node (no artificial constraints). */
 #define SAP_FLAG_RANGEDROWS 0x4
+/* consPropLinear with tightenbounds FALSE (cons_linear.c:16147-16161, the
+ * constraints/linear/tightenboundsfreq gating): every row is only tested for infeasibility and
+ * local redundancy (propagateCons, :7717-7753); no bound is tightened, and ranged-row
+ * propagation, which runs only together with the tightening (:7694), is off */
+#define SAP_FLAG_NOTIGHTEN 0x8
 
 /* per-call statistics */
 typedef struct sap_stats
@@ -83,6 +88,9 @@ typedef struct sap_stats
    double  device_ms;    /* device time of the propagation kernels (HIP events) */
    double  sweep_ms;     /* device time of the row-sweep kernels alone */
    int64_t sweep_launches;
+   int64_t nrecompute;   /* residual activities recomputed in plain double because the
+                            incremental value was unreliable (the path of
+                            consdataGetReliableResidualActivity, cons_linear.c:2580-2657) */
 } sap_stats;
 
 typedef struct sap_context sap_context;

@@ -280,6 +280,8 @@ void orc_params_default(orc_params* p)
    p->maxeasyactivitydelta = 1e6;
    p->maxrounds = -1;
    p->rangedrows = 0;
+   p->notighten = 0;
+   p->pad = 0;
 }
 
 void orc_compute_binary(int64_t ncols, const uint8_t* integral, const double* glb, const double* gub,
@@ -625,6 +627,7 @@ typedef struct
    int32_t currow;
    int32_t currule;   /* 0: residual activity, 1: ranged-row test; rows are kept as row * 2 + rule */
    int rowside;
+   int64_t nrecomp;   /* reliable residual recomputes */
 } jstate;
 
 static inline void submitLb(jstate* js, int32_t j, double v, int force)
@@ -809,7 +812,7 @@ static void jHard(jstate* js, const rowstate* rs, int64_t i, int64_t k, int forc
             int unrel = isUpdateUnreliable(p, minres, lastmin);
             if( unrel )
             {
-               minres = reliableResidual(m, i, k, js->lb, js->ub, 1);
+               minres = (js->nrecomp++, reliableResidual(m, i, k, js->lb, js->ub, 1));
                newub = (rhs - minres) / a;
                unrel = isInf(p, -minres)
                   || (!isUbBetter(p, newub, lb, ub) && (!isFeasLT(p, newub, ub) || !integral)
@@ -827,7 +830,7 @@ static void jHard(jstate* js, const rowstate* rs, int64_t i, int64_t k, int forc
          {
             if( isUpdateUnreliable(p, maxres, lastmax) )
             {
-               maxres = reliableResidual(m, i, k, js->lb, js->ub, 0);
+               maxres = (js->nrecomp++, reliableResidual(m, i, k, js->lb, js->ub, 0));
                newlb = (lhs - maxres) / a;
                if( isInf(p, maxres) || (!isLbBetter(p, newlb, lb, ub) && (!isFeasGT(p, newlb, lb) || !integral)
                      && (!force || !isGT(p, newlb, lb))) )
@@ -848,7 +851,7 @@ static void jHard(jstate* js, const rowstate* rs, int64_t i, int64_t k, int forc
             int unrel = isUpdateUnreliable(p, minres, lastmin);
             if( unrel )
             {
-               minres = reliableResidual(m, i, k, js->lb, js->ub, 1);
+               minres = (js->nrecomp++, reliableResidual(m, i, k, js->lb, js->ub, 1));
                newlb = (rhs - minres) / a;
                unrel = isInf(p, -minres)
                   || (!isLbBetter(p, newlb, lb, ub) && (!isFeasGT(p, newlb, lb) || !integral)
@@ -866,7 +869,7 @@ static void jHard(jstate* js, const rowstate* rs, int64_t i, int64_t k, int forc
          {
             if( isUpdateUnreliable(p, maxres, lastmax) )
             {
-               maxres = reliableResidual(m, i, k, js->lb, js->ub, 0);
+               maxres = (js->nrecomp++, reliableResidual(m, i, k, js->lb, js->ub, 0));
                newub = (lhs - maxres) / a;
                /* the reference combines the last two tests with && here (cons_linear.c:6942-6944) */
                if( isInf(p, maxres) || (!isUbBetter(p, newub, lb, ub) && (!isFeasLT(p, newub, ub) && !integral)
@@ -1148,7 +1151,7 @@ static int jRow(jstate* js, int64_t i, double* xmin, double* xmax, dd* scratch)
 
    infmin = rs.minneginf + rs.minposinf + rs.minneghuge + rs.minposhuge;
    infmax = rs.maxneginf + rs.maxposinf + rs.maxneghuge + rs.maxposhuge;
-   tighten = nv > 0 && !(infmin > 1 && infmax > 1);
+   tighten = !p->notighten && nv > 0 && !(infmin > 1 && infmax > 1);
    force = (nv == 1);
    if( tighten && isFeasZero(p, rs.maxactdelta) )
       tighten = 0;
@@ -1173,8 +1176,8 @@ static int jRow(jstate* js, int64_t i, double* xmin, double* xmax, dd* scratch)
             jHard(js, &rs, i, k, force);
       }
    }
-   /* propagateCons: ranged rows after the tightening (cons_linear.c:7694-7700) */
-   if( !js->cutoff && p->rangedrows )
+   /* propagateCons: ranged rows after the tightening, and only with it (cons_linear.c:7694-7700) */
+   if( !js->cutoff && p->rangedrows && !p->notighten )
    {
       js->currule = 1;
       jRanged(js, i);
@@ -1469,6 +1472,7 @@ static int jacobi(const orc_matrix* m, const orc_params* p, double* lb, double* 
    }
    res->cutoff = js.cutoff;
    res->rounds = round;
+   res->nrecompute = js.nrecomp;
 
    free(marked);
    free(redundant);
@@ -1566,6 +1570,7 @@ typedef struct
    uint8_t* nextmark;
    int64_t  nchg;
    int      cutoff;
+   int64_t  nrecomp;     /* reliable residual recomputes */
 } gsstate;
 
 static const orc_params* g_sortp;
@@ -2022,6 +2027,7 @@ static double gsReliableResidual(gsstate* st, gsrow* r, int skip, int isminres)
 {
    int v;
    double res = 0.0;
+   st->nrecomp++;
    for( v = 0; v < r->nvars; ++v )
    {
       int32_t j;
@@ -2382,7 +2388,9 @@ static void gsPropagateCons(gsstate* st, int64_t ri)
    double minact, maxact;
    int t1, t2, f1, f2;
 
-   gsTightenBounds(st, ri);
+   /* tightenbounds FALSE (consPropLinear, cons_linear.c:16147-16161): only the final test */
+   if( !p->notighten )
+      gsTightenBounds(st, ri);
    if( st->cutoff )
       return;
    gsEnsureValid(st, r, 1);
@@ -2500,6 +2508,7 @@ int orc_propagate_scip(const orc_matrix* m, const orc_params* p, double* lb, dou
    res->cutoff = st.cutoff;
    res->rounds = round;
    res->nchgbds = st.nchg;
+   res->nrecompute = st.nrecomp;
 
    free(st.rows);
    free(st.svar);

@@ -36,6 +36,9 @@ typedef struct orc_params
    double  maxeasyactivitydelta;  /* constraints/linear/maxeasyactivitydelta (1e6) */
    int32_t maxrounds;             /* propagating/maxrounds (-1: unlimited) */
    int32_t rangedrows;            /* constraints/linear/rangedrowpropagation (Jacobi twin only) */
+   int32_t notighten;             /* consPropLinear's tightenbounds FALSE (cons_linear.c:16147-16161):
+                                     rows are only tested for infeasibility and redundancy */
+   int32_t pad;
 } orc_params;
 
 typedef struct orc_matrix
@@ -58,6 +61,7 @@ typedef struct orc_result
    int64_t nchgbds;               /* accepted bound changes */
    int64_t rows_swept;            /* rows swept over all rounds */
    int64_t nnz_swept;             /* nonzeros swept over all rounds */
+   int64_t nrecompute;            /* reliable residual recomputes (cons_linear.c:2580-2657) */
 } orc_result;
 
 typedef struct orc_rowact

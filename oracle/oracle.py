@@ -20,6 +20,7 @@ class orc_params(C.Structure):
         ("infinity", C.c_double), ("epsilon", C.c_double), ("sumepsilon", C.c_double), ("feastol", C.c_double),
         ("boundstreps", C.c_double), ("hugeval", C.c_double), ("recompfac", C.c_double),
         ("maxeasyactivitydelta", C.c_double), ("maxrounds", C.c_int32), ("rangedrows", C.c_int32),
+        ("notighten", C.c_int32), ("pad", C.c_int32),
     ]
 
 
@@ -40,7 +41,7 @@ class orc_conflict(C.Structure):
 
 class orc_result(C.Structure):
     _fields_ = [("cutoff", C.c_int32), ("rounds", C.c_int32), ("nchgbds", C.c_int64), ("rows_swept", C.c_int64),
-                ("nnz_swept", C.c_int64)]
+                ("nnz_swept", C.c_int64), ("nrecompute", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -38,6 +38,7 @@ SAP_REDUCEDDOM = 10
 SAP_FLAG_BASE_IS_FIXPOINT = 0x1
 SAP_FLAG_HOSTLOOP = 0x2
 SAP_FLAG_RANGEDROWS = 0x4
+SAP_FLAG_NOTIGHTEN = 0x8
 
 SAP_STEP_SHORT = 2
 SAP_STEP_LONG = 8
@@ -102,6 +103,7 @@ class sap_stats(C.Structure):
         ("device_ms", C.c_double),
         ("sweep_ms", C.c_double),
         ("sweep_launches", C.c_int64),
+        ("nrecompute", C.c_int64),
     ]
 
     def as_dict(self):

@@ -93,6 +93,9 @@ struct Status
    uint32_t histn;
    int32_t clogovf;
    int32_t pad4;
+   // residual activities recomputed in plain double because the incremental one was
+   // unreliable (consdataGetReliableResidualActivity's path, counted per call)
+   unsigned long long nrecomp;
 };
 
 struct Mat
@@ -703,6 +706,7 @@ template <bool R>
 __device__ __forceinline__ double reliableResidual(const CtxT<R>& C, int64_t s, int64_t n, int64_t skip, bool isminres)
 {
    const double2* bnd = C.V->bnd + (size_t)C.v * (size_t)C.M->ncols;
+   atomicAdd(&C.V->st->nrecomp, 1ULL);   // rare path: a counter on one address costs nothing
    double r = 0.0;
    for( int64_t q = 0; q < n; ++q )
    {
@@ -888,7 +892,7 @@ __device__ __forceinline__ RowDecision decideRow(const Params& p, const RowState
    minActBoth(p, rs.minact, rs.c[0], rs.c[1], rs.c[2], rs.c[3], minF, f1, minT);
    maxActBoth(p, rs.maxact, rs.c[4], rs.c[5], rs.c[6], rs.c[7], maxF, f2, maxT);
    d.force = (n == 1);
-   d.tighten = n > 0 && !(infmin > 1 && infmax > 1);
+   d.tighten = n > 0 && !(p.flags & SAP_FLAG_NOTIGHTEN) && !(infmin > 1 && infmax > 1);
    if( d.tighten && isFeasZero(p, rs.mad) )
       d.tighten = false;
    if( d.tighten && !isInf(p, rs.mad) )
@@ -2920,6 +2924,7 @@ __global__ void k_call_reset(Vec V, size_t nclear, int32_t nvec, int32_t firstal
       st->rows_swept = 0;
       st->nnz_swept = 0;
       st->nchg = 0;
+      st->nrecomp = 0;
       st->cutrow = INT32_MAX;
       st->cutcol = INT32_MAX;
       st->clogn = 0;
@@ -3599,6 +3604,8 @@ static Params toParams(const sap_params* p)
    q.maxeasy = p->maxeasyactivitydelta;
    q.maxrounds = p->maxrounds;
    q.flags = p->flags;
+   if( q.flags & SAP_FLAG_NOTIGHTEN )
+      q.flags &= ~SAP_FLAG_RANGEDROWS;   // ranged rows run only together with the tightening
    return q;
 }
 
@@ -4706,6 +4713,7 @@ static void fillStats(sap_context* c, double tot, double sweepms, int64_t launch
    stats->device_ms = tot;
    stats->sweep_ms = sweepms;
    stats->sweep_launches = launches;
+   stats->nrecompute = (int64_t)h->nrecomp;
 }
 
 // enqueue one round on stream s: control, sweep (between two events when given), column step
@@ -4830,7 +4838,7 @@ static bool ensureGraphs(sap_bounds* b, const Params& P)
    GraphKey k;
    memset(&k, 0, sizeof(k));
    Params kp = P;
-   kp.flags = P.flags & SAP_FLAG_RANGEDROWS;   // the other flags steer the host side only
+   kp.flags = P.flags & (SAP_FLAG_RANGEDROWS | SAP_FLAG_NOTIGHTEN);   // the other flags steer the host side only
    const Mat M = toMat(b->mat);
    const Layout L = makeLayout(b);
    memcpy(&k.P, &kp, sizeof(Params));

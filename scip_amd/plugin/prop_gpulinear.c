@@ -134,19 +134,68 @@ SCIP_RETCODE getEngineParams(
    return SCIP_OKAY;
 }
 
-/** SCIP's constraints/linear/rangedrowpropagation (default TRUE) switches the device's
- *  ranged-row propagation on, as it does cons_linear's outside presolving (cons_linear.c:16164) */
+/** an int parameter of the linear constraint handler, or its default when it is not there */
 static
-SCIP_RETCODE rangedRowFlag(
+SCIP_RETCODE linearIntParam(
    SCIP*                 scip,               /**< SCIP data structure */
+   const char*           name,               /**< parameter name */
+   int                   defaultvalue,       /**< cons_linear.c's default */
+   int*                  value               /**< pointer to store the value */
+   )
+{
+   if( SCIPgetIntParam(scip, name, value) != SCIP_OKAY )
+      *value = defaultvalue;
+   return SCIP_OKAY;
+}
+
+/** consPropLinear's gating of a propagation call (cons_linear.c:16147-16171), applied to the
+ *  node propagation of PROPEXEC.  This propagator's own frequency takes the place of the linear
+ *  handler's propagation frequency, which is -1 when the handler's propagation is switched off
+ *  in favour of this one.
+ *  - In probing the bounds are always tightened and ranged rows are never propagated.
+ *  - Otherwise the bounds are tightened at the depths that constraints/linear/tightenboundsfreq
+ *    selects (SAP_FLAG_NOTIGHTEN elsewhere: rows are only tested for infeasibility and
+ *    redundancy), and ranged-row propagation needs constraints/linear/rangedrowpropagation,
+ *    no repropagation, a depth of at most rangedrowmaxdepth, a depth that rangedrowfreq
+ *    selects and a stage other than PRESOLVING. */
+static
+SCIP_RETCODE linearGating(
+   SCIP*                 scip,               /**< SCIP data structure */
+   SCIP_PROP*            prop,               /**< the propagator */
    sap_params*           params              /**< parameters to complete */
    )
 {
-   SCIP_Bool ranged = TRUE;
+   SCIP_Bool tightenbounds;
+   SCIP_Bool ranged = FALSE;
 
-   if( SCIPgetBoolParam(scip, "constraints/linear/rangedrowpropagation", &ranged) != SCIP_OKAY )
-      ranged = TRUE;
-   if( ranged )
+   if( SCIPinProbing(scip) )
+      tightenbounds = TRUE;
+   else
+   {
+      int depth = SCIPgetDepth(scip);
+      int propfreq = SCIPpropGetFreq(prop);
+      int tbfreq;
+      int rrfreq;
+      int rrmaxdepth;
+      int f;
+
+      SCIP_CALL( linearIntParam(scip, "constraints/linear/tightenboundsfreq", 1, &tbfreq) );
+      SCIP_CALL( linearIntParam(scip, "constraints/linear/rangedrowfreq", 1, &rrfreq) );
+      SCIP_CALL( linearIntParam(scip, "constraints/linear/rangedrowmaxdepth", INT_MAX, &rrmaxdepth) );
+      f = propfreq * tbfreq;
+      tightenbounds = (tbfreq >= 0) && ((f == 0 && depth == 0) || (f >= 1 && (depth % f == 0)));
+
+      if( SCIPgetBoolParam(scip, "constraints/linear/rangedrowpropagation", &ranged) != SCIP_OKAY )
+         ranged = TRUE;
+      ranged = ranged && !SCIPinRepropagation(scip);
+      ranged = ranged && (depth <= rrmaxdepth);
+      f = propfreq * rrfreq;
+      ranged = ranged && (rrfreq >= 0) && ((f == 0 && depth == 0) || (f >= 1 && (depth % f == 0)));
+      ranged = ranged && (SCIPgetStage(scip) != SCIP_STAGE_PRESOLVING);
+   }
+   if( !tightenbounds )
+      params->flags |= SAP_FLAG_NOTIGHTEN;
+   else if( ranged )
       params->flags |= SAP_FLAG_RANGEDROWS;
    return SCIP_OKAY;
 }
@@ -337,6 +386,7 @@ SCIP_RETCODE buildMatrix(
 static
 SCIP_RETCODE propagateOnDevice(
    SCIP*                 scip,               /**< SCIP data structure */
+   SCIP_PROP*            prop,               /**< the propagator */
    SCIP_PROPDATA*        propdata,           /**< propagator data */
    SCIP_Bool             local,              /**< local (node) bounds, else global bounds */
    double*               lb,                 /**< array of nvars: new lower bounds */
@@ -357,7 +407,7 @@ SCIP_RETCODE propagateOnDevice(
    SCIP_CALL( getEngineParams(scip, propdata, &params) );
    if( local )
    {
-      SCIP_CALL( rangedRowFlag(scip, &params) );
+      SCIP_CALL( linearGating(scip, prop, &params) );
    }
    memset(&stats, 0, sizeof(stats));
    SCIP_CALL( engineCall(sap_propagate(propdata->mat, &params, lb, ub, &stats), "sap_propagate") );
@@ -644,7 +694,7 @@ SCIP_RETCODE execWithReasons(
    SCIP_CALL( SCIPallocBufferArray(scip, &lb, propdata->nvars) );
    SCIP_CALL( SCIPallocBufferArray(scip, &ub, propdata->nvars) );
    SCIP_CALL( getEngineParams(scip, propdata, &params) );
-   SCIP_CALL( rangedRowFlag(scip, &params) );
+   SCIP_CALL( linearGating(scip, prop, &params) );
    for( ;; )
    {
       for( j = 0; j < propdata->nvars; ++j )
@@ -853,7 +903,7 @@ SCIP_DECL_PROPPRESOL(propPresolGpulinear)
 
    SCIP_CALL( SCIPallocBufferArray(scip, &lb, propdata->nvars) );
    SCIP_CALL( SCIPallocBufferArray(scip, &ub, propdata->nvars) );
-   SCIP_CALL( propagateOnDevice(scip, propdata, FALSE, lb, ub, &cutoff) );
+   SCIP_CALL( propagateOnDevice(scip, prop, propdata, FALSE, lb, ub, &cutoff) );
    if( cutoff )
       *result = SCIP_CUTOFF;
    else
@@ -914,7 +964,7 @@ SCIP_DECL_PROPEXEC(propExecGpulinear)
 
    SCIP_CALL( SCIPallocBufferArray(scip, &lb, propdata->nvars) );
    SCIP_CALL( SCIPallocBufferArray(scip, &ub, propdata->nvars) );
-   SCIP_CALL( propagateOnDevice(scip, propdata, TRUE, lb, ub, &cutoff) );
+   SCIP_CALL( propagateOnDevice(scip, prop, propdata, TRUE, lb, ub, &cutoff) );
    if( cutoff )
       *result = SCIP_CUTOFF;
    else

@@ -61,7 +61,13 @@ typedef struct SCIP_ParamData SCIP_PARAMDATA;
 typedef struct SCIP_Param SCIP_PARAM;
 typedef struct SCIP_BdChgIdx SCIP_BDCHGIDX;
 typedef enum SCIP_BoundType { SCIP_BOUNDTYPE_LOWER = 0, SCIP_BOUNDTYPE_UPPER = 1 } SCIP_BOUNDTYPE;
-typedef enum SCIP_Stage { SCIP_STAGE_PRESOLVING = 6, SCIP_STAGE_SOLVING = 9 } SCIP_STAGE;   /* type_set.h values */
+typedef enum SCIP_Stage
+{
+   SCIP_STAGE_INIT = 0, SCIP_STAGE_PROBLEM = 1, SCIP_STAGE_TRANSFORMING = 2, SCIP_STAGE_TRANSFORMED = 3,
+   SCIP_STAGE_INITPRESOLVE = 4, SCIP_STAGE_PRESOLVING = 5, SCIP_STAGE_EXITPRESOLVE = 6, SCIP_STAGE_PRESOLVED = 7,
+   SCIP_STAGE_INITSOLVE = 8, SCIP_STAGE_SOLVING = 9, SCIP_STAGE_SOLVED = 10, SCIP_STAGE_EXITSOLVE = 11,
+   SCIP_STAGE_FREETRANS = 12, SCIP_STAGE_FREE = 13
+} SCIP_STAGE;   /* type_set.h:41-56 values */
 typedef enum SCIP_ConflictType { SCIP_CONFTYPE_UNKNOWN = 0, SCIP_CONFTYPE_PROPAGATION = 1 } SCIP_CONFTYPE;
 
 #define SCIP_DECL_PARAMCHGD(x) SCIP_RETCODE x (SCIP* scip, SCIP_PARAM* param)
@@ -104,6 +110,7 @@ SCIP_RETCODE SCIPsetPropPresol(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPPRESOL
 SCIP_RETCODE SCIPsetPropResprop(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPRESPROP((*propresprop)));
 /* pub_prop.h */
 SCIP_PROPDATA* SCIPpropGetData(SCIP_PROP* prop);
+int SCIPpropGetFreq(SCIP_PROP* prop);
 void SCIPpropSetData(SCIP_PROP* prop, SCIP_PROPDATA* propdata);
 /* scip_param.h */
 SCIP_RETCODE SCIPaddIntParam(SCIP* scip, const char* name, const char* desc, int* valueptr, SCIP_Bool isadvanced,
@@ -112,9 +119,12 @@ SCIP_RETCODE SCIPaddBoolParam(SCIP* scip, const char* name, const char* desc, SC
    SCIP_Bool isadvanced, SCIP_Bool defaultvalue, SCIP_DECL_PARAMCHGD((*paramchgd)), SCIP_PARAMDATA* paramdata);
 SCIP_RETCODE SCIPgetRealParam(SCIP* scip, const char* name, SCIP_Real* value);
 SCIP_RETCODE SCIPgetBoolParam(SCIP* scip, const char* name, SCIP_Bool* value);
-/* scip_general.h / scip_probing.h */
+SCIP_RETCODE SCIPgetIntParam(SCIP* scip, const char* name, int* value);
+/* scip_general.h / scip_probing.h / scip_tree.h */
 SCIP_STAGE SCIPgetStage(SCIP* scip);
 SCIP_Bool SCIPinProbing(SCIP* scip);
+int SCIPgetDepth(SCIP* scip);
+SCIP_Bool SCIPinRepropagation(SCIP* scip);
 /* scip_conflict.h */
 SCIP_Bool SCIPisConflictAnalysisApplicable(SCIP* scip);
 SCIP_RETCODE SCIPinitConflictAnalysis(SCIP* scip, SCIP_CONFTYPE conftype, SCIP_Bool iscutoffinvolved);

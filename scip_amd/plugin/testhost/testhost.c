@@ -63,6 +63,7 @@ struct SCIP_Prop
    SCIP_DECL_PROPEXITSOL((*exitsol));
    SCIP_DECL_PROPPRESOL((*presol));
    SCIP_DECL_PROPRESPROP((*resprop));
+   int freq;
    char name[64];
 };
 
@@ -94,6 +95,11 @@ struct Scip
    int nanalyzed;
    int stage;
    int rangedrowprop;   /* constraints/linear/rangedrowpropagation (SCIP's default TRUE) */
+   /* constraints/linear/{tightenboundsfreq,rangedrowfreq,rangedrowmaxdepth} (cons_linear.c's
+    * defaults 1, 1, INT_MAX) */
+   int tightenboundsfreq, rangedrowfreq, rangedrowmaxdepth;
+   /* the node: in probing, depth, in repropagation */
+   int inprobing, depth, inreprop;
 };
 
 /* ---- numerics (def.h, set.c) ---- */
@@ -119,6 +125,19 @@ SCIP_RETCODE SCIPgetRealParam(SCIP* scip, const char* name, SCIP_Real* value)
       *value = scip->recompfac;
    else if( strcmp(name, "constraints/linear/maxeasyactivitydelta") == 0 )
       *value = scip->maxeasy;
+   else
+      return SCIP_PARAMETERUNKNOWN;
+   return SCIP_OKAY;
+}
+
+SCIP_RETCODE SCIPgetIntParam(SCIP* scip, const char* name, int* value)
+{
+   if( strcmp(name, "constraints/linear/tightenboundsfreq") == 0 )
+      *value = scip->tightenboundsfreq;
+   else if( strcmp(name, "constraints/linear/rangedrowfreq") == 0 )
+      *value = scip->rangedrowfreq;
+   else if( strcmp(name, "constraints/linear/rangedrowmaxdepth") == 0 )
+      *value = scip->rangedrowmaxdepth;
    else
       return SCIP_PARAMETERUNKNOWN;
    return SCIP_OKAY;
@@ -165,7 +184,7 @@ SCIP_RETCODE SCIPaddIntParam(SCIP* scip, const char* name, const char* desc, int
 SCIP_RETCODE SCIPincludePropBasic(SCIP* scip, SCIP_PROP** propptr, const char* name, const char* desc, int priority,
    int freq, SCIP_Bool delay, SCIP_PROPTIMING timingmask, SCIP_DECL_PROPEXEC((*propexec)), SCIP_PROPDATA* propdata)
 {
-   (void)desc; (void)priority; (void)freq; (void)delay; (void)timingmask;
+   (void)desc; (void)priority; (void)delay; (void)timingmask;
    if( scip->prop != NULL )
       return SCIP_INVALIDDATA;   /* one propagator per test host */
    scip->prop = (SCIP_PROP*)calloc(1, sizeof(SCIP_PROP));
@@ -174,6 +193,7 @@ SCIP_RETCODE SCIPincludePropBasic(SCIP* scip, SCIP_PROP** propptr, const char* n
    strncpy(scip->prop->name, name, 63);
    scip->prop->exec = propexec;
    scip->prop->data = propdata;
+   scip->prop->freq = freq;
    if( propptr != NULL )
       *propptr = scip->prop;
    return SCIP_OKAY;
@@ -192,6 +212,7 @@ SCIP_RETCODE SCIPsetPropPresol(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPPRESOL
 }
 SCIP_RETCODE SCIPsetPropResprop(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPRESPROP((*f))) { (void)scip; prop->resprop = f; return SCIP_OKAY; }
 SCIP_PROPDATA* SCIPpropGetData(SCIP_PROP* prop) { return prop->data; }
+int SCIPpropGetFreq(SCIP_PROP* prop) { return prop->freq; }
 void SCIPpropSetData(SCIP_PROP* prop, SCIP_PROPDATA* propdata) { prop->data = propdata; }
 
 /* ---- problem access ---- */
@@ -366,7 +387,9 @@ SCIP_RETCODE SCIPinferVarUbProp(SCIP* scip, SCIP_VAR* var, SCIP_Real newbound, S
 
 /* ---- conflict analysis ---- */
 SCIP_STAGE SCIPgetStage(SCIP* scip) { return (SCIP_STAGE)scip->stage; }
-SCIP_Bool SCIPinProbing(SCIP* scip) { (void)scip; return FALSE; }
+SCIP_Bool SCIPinProbing(SCIP* scip) { return scip->inprobing ? TRUE : FALSE; }
+int SCIPgetDepth(SCIP* scip) { return scip->depth; }
+SCIP_Bool SCIPinRepropagation(SCIP* scip) { return scip->inreprop ? TRUE : FALSE; }
 SCIP_Bool SCIPisConflictAnalysisApplicable(SCIP* scip) { (void)scip; return TRUE; }
 SCIP_RETCODE SCIPinitConflictAnalysis(SCIP* scip, SCIP_CONFTYPE conftype, SCIP_Bool iscutoffinvolved)
 {
@@ -541,6 +564,9 @@ SCIP_EXPORT SCIP* th_create(void)
    s->maxeasy = 1e6;
    s->stage = SCIP_STAGE_SOLVING;
    s->rangedrowprop = 1;
+   s->tightenboundsfreq = 1;
+   s->rangedrowfreq = 1;
+   s->rangedrowmaxdepth = INT_MAX;
    return s;
 }
 
@@ -599,6 +625,30 @@ SCIP_EXPORT int th_include_gpulinear(SCIP* s) { return SCIPincludePropGpulinear(
 
 /* SCIP's own parameter constraints/linear/rangedrowpropagation */
 SCIP_EXPORT void th_set_rangedrowprop(SCIP* s, int value) { s->rangedrowprop = value; }
+
+/* SCIP's int parameters constraints/linear/{tightenboundsfreq,rangedrowfreq,rangedrowmaxdepth} */
+SCIP_EXPORT int th_set_linear_int(SCIP* s, const char* name, int value)
+{
+   if( strcmp(name, "tightenboundsfreq") == 0 )
+      s->tightenboundsfreq = value;
+   else if( strcmp(name, "rangedrowfreq") == 0 )
+      s->rangedrowfreq = value;
+   else if( strcmp(name, "rangedrowmaxdepth") == 0 )
+      s->rangedrowmaxdepth = value;
+   else
+      return SCIP_PARAMETERUNKNOWN;
+   return SCIP_OKAY;
+}
+
+/* the node the next PROPEXEC runs at: SCIP stage (type_set.h values), probing, depth,
+ * repropagation */
+SCIP_EXPORT void th_set_node(SCIP* s, int stage, int inprobing, int depth, int inreprop)
+{
+   s->stage = stage;
+   s->inprobing = inprobing;
+   s->depth = depth;
+   s->inreprop = inreprop;
+}
 
 SCIP_EXPORT int th_set_int_param(SCIP* s, const char* name, int value)
 {

@@ -18,6 +18,10 @@ SCIP_CUTOFF = 7
 SCIP_REDUCEDDOM = 10
 SCIP_SUCCESS = 17
 
+# SCIP_STAGE values (src/scip/type_set.h:41-56)
+SCIP_STAGE_PRESOLVING = 5
+SCIP_STAGE_SOLVING = 9
+
 _lib = None
 _P = np.ctypeslib.ndpointer
 
@@ -47,6 +51,9 @@ def lib():
                                            C.c_void_p, IP, IP]
         L.th_set_rangedrowprop.argtypes = [C.c_void_p, C.c_int]
         L.th_set_rangedrowprop.restype = None
+        L.th_set_linear_int.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        L.th_set_node.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.th_set_node.restype = None
         L.th_resprop_change.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
         L.th_get_changes.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         L.th_get_conflict.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
@@ -74,6 +81,12 @@ class Host:
 
     def call(self, name, *args):
         return getattr(lib(), name)(self.h, *args)
+
+    def set_node(self, stage=SCIP_STAGE_SOLVING, inprobing=False, depth=0, inreprop=False):
+        lib().th_set_node(self.h, stage, 1 if inprobing else 0, depth, 1 if inreprop else 0)
+
+    def set_linear_int(self, name, value):
+        assert lib().th_set_linear_int(self.h, name.encode(), value) == SCIP_OKAY
 
     def propexec(self):
         r = C.c_int(0)

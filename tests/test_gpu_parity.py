@@ -271,3 +271,61 @@ def test_graph_rounds_equal_host_loop(gpu_ctx, planted):
     d = mat.propagate(params=sa.default_params(maxrounds=2, flags=sa.SAP_FLAG_HOSTLOOP))
     assert c[0].tobytes() == d[0].tobytes() and c[2]["rounds"] == d[2]["rounds"] <= 2
     mat.close()
+
+
+@pytest.mark.parametrize("badrow", [False, True])
+def test_notighten_flag(gpu_ctx, badrow):
+    # SAP_FLAG_NOTIGHTEN (consPropLinear's tightenbounds FALSE): rows are only tested; with
+    # badrow one row's rhs lies below its finite minimal activity, so the final test cuts off
+    inst = sa.gen_mixed(20000, 20000, seed=6, planted=True)
+    if badrow:
+        mn, _, _, _ = orc.row_activities(orc.OMatrix.from_instance(inst), inst.lb, inst.ub)
+        i = int(np.nonzero(np.abs(mn) < 1e15)[0][3])
+        inst.rhs[i] = mn[i] - 1.0
+        inst.lhs[i] = -INF
+    say(f"[case] notighten badrow={badrow}")
+    ol, ou, orr = orc.propagate_jacobi(orc.OMatrix.from_instance(inst), inst.lb, inst.ub,
+                                       orc.params(notighten=1, rangedrows=1))
+    assert bool(orr["cutoff"]) == badrow and orr["rounds"] == 1 and orr["nchgbds"] == 0
+    mat = sa.Matrix(gpu_ctx, inst)
+    for flags in (sa.SAP_FLAG_NOTIGHTEN, sa.SAP_FLAG_NOTIGHTEN | sa.SAP_FLAG_RANGEDROWS):
+        gl, gu, gs = mat.propagate(inst.lb, inst.ub, sa.default_params(flags=flags))
+        assert (gs["result"] == sa.SAP_CUTOFF) == badrow
+        assert gs["rounds"] == 1 and gs["nchgbds"] == 0
+        assert np.array_equal(gl, inst.lb) and np.array_equal(gu, inst.ub)
+    mat.close()
+
+
+@pytest.mark.parametrize("name", ["rhs_ub", "lhs_lb", "lhs_ub_neg", "rhs_lb_neg"])
+def test_reliable_residual_recompute(gpu_ctx, name):
+    # the cancellation cases whose bound only the plain-double recompute of
+    # consdataGetReliableResidualActivity gives; the device counts the recomputes it made
+    from tests.test_oracle_prop import RELIABLE, reliable_instance
+    _, _, _, side, expect = RELIABLE[name]
+    inst = reliable_instance(name)
+    gl, gu, gs = check_single(gpu_ctx, inst, name=f"reliable_{name}")
+    assert (gl[3] if side == "lb" else gu[3]) == expect
+    assert gs["nrecompute"] == 1 and gs["nchgbds"] == 1
+
+
+def test_recompute_count_matches_twin(gpu_ctx):
+    # on the mixed generator the reliability recompute is rare; the device counts the same
+    # recomputes as the twin, round by round and row by row
+    inst = sa.gen_mixed(60000, 60000, seed=1, planted=True)
+    ol, ou, orr = orc.propagate_jacobi(orc.OMatrix.from_instance(inst), inst.lb, inst.ub)
+    gl, gu, gs = check_single(gpu_ctx, inst, name="recompute_count")
+    assert gs["nrecompute"] == orr["nrecompute"]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_unique_integer_fixpoint_equals_scip_order(gpu_ctx, seed):
+    # integer data with boundstreps 0: the fixpoint is unique, so the device must reach SCIP's
+    # own (Gauss-Seidel) fixpoint exactly, not only the Jacobi twin's
+    from tests.test_oracle_fixpoint import integer_instance
+    rng = np.random.default_rng(6000 + seed)
+    inst, _ = integer_instance(rng, nrows=300, ncols=200, maxlen=8, coefrange=20, boundrange=0, ubmax=1000,
+                               planted=True)
+    gl, gu, gs = check_single(gpu_ctx, inst, name=f"unique_int{seed}", boundstreps=0.0)
+    sl, su, sr = orc.propagate_scip(orc.OMatrix.from_instance(inst), inst.lb, inst.ub, orc.params(boundstreps=0.0))
+    assert not sr["cutoff"] and gs["result"] == sa.SAP_REDUCEDDOM
+    assert np.array_equal(gl, sl) and np.array_equal(gu, su)

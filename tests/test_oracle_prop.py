@@ -236,3 +236,80 @@ def test_integer_bounds_mostly_agree_across_schedules():
     it = inst.integral.astype(bool)
     agree = np.mean((jl[it] == gl[it]) & (ju[it] == gu[it]))
     assert agree > 0.9
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_notighten_only_tests_rows(seed):
+    # consPropLinear with tightenbounds FALSE (cons_linear.c:16147-16161): propagateCons only
+    # runs its final test (:7717-7742), so no bound moves and the verdict is "some row's
+    # activity range misses its sides", with exact integer activities on this instance
+    rng = np.random.default_rng(900 + seed)
+    inst = random_integer_instance(rng, nrows=60, ncols=30, maxlen=6)
+    if seed % 2:
+        # a row whose rhs lies below its minimal activity
+        s, e = inst.rowptr[7], inst.rowptr[8]
+        a = inst.vals[s:e]
+        inst.rhs[7] = float(a @ np.where(a > 0, inst.lb[inst.colidx[s:e]], inst.ub[inst.colidx[s:e]])) - 1.0
+        inst.lhs[7] = -INF
+    om = orc.OMatrix.from_instance(inst)
+    expect_cut = False
+    for i in range(inst.nrows):
+        s, e = inst.rowptr[i], inst.rowptr[i + 1]
+        a = inst.vals[s:e]
+        lo = np.where(a > 0, inst.lb[inst.colidx[s:e]], inst.ub[inst.colidx[s:e]])
+        hi = np.where(a > 0, inst.ub[inst.colidx[s:e]], inst.lb[inst.colidx[s:e]])
+        mn, mx = float(a @ lo), float(a @ hi)
+        expect_cut |= mn > inst.rhs[i] + 1e-6 * max(1.0, abs(mn), abs(inst.rhs[i])) or \
+            mx < inst.lhs[i] - 1e-6 * max(1.0, abs(mx), abs(inst.lhs[i]))
+    for fn in (orc.propagate_jacobi, orc.propagate_scip):
+        for ranged in (0, 1):
+            lb, ub, r = fn(om, inst.lb, inst.ub, orc.params(notighten=1, rangedrows=ranged))
+            assert np.array_equal(lb, inst.lb) and np.array_equal(ub, inst.ub)
+            assert r["nchgbds"] == 0 and r["rounds"] <= 1
+            assert bool(r["cutoff"]) == expect_cut == bool(seed % 2)
+    # with tightening the same instances do move bounds or cut off
+    lb, ub, r = orc.propagate_jacobi(om, inst.lb, inst.ub)
+    assert r["cutoff"] or r["nchgbds"] > 0 or not expect_cut
+
+
+# consdataGetReliableResidualActivity (cons_linear.c:2580-2657) forced by cancellation: the
+# row x0 + x1 - x2 + a*y with x0 = x2 = 1e14 and x1 = 0.3 fixed, y in [1, 2] continuous, |a| =
+# 1e7 (maxactdelta 1e7: the hard path).  y's residual activity is 0.3 in double-double, a
+# 1/3.3e7 fraction of the activity, so SCIPisUpdateUnreliable (set.c:8032, recomputefac 1e6)
+# holds and the residual is recomputed as a plain double sum in row order:
+# fl(fl(1e14 + 0.3) - 1e14) = 0.296875 (the spacing of doubles near 1e14 is 1/64).  The bound
+# from the recomputed residual differs from the double-double one ((1.5e7 - 0.3) / 1e7 =
+# 1.49999997), so the expected values below pin that the recompute ran and what it summed.
+RELIABLE = {
+    # name: (coefficient of y, lhs, rhs, bound of y, expected value)
+    "rhs_ub": (1e7, -INF, 1.5e7, "ub", (1.5e7 - 0.296875) / 1e7),      # = 1.4999999703125
+    "lhs_lb": (1e7, 1.5e7, INF, "lb", (1.5e7 - 0.296875) / 1e7),
+    "lhs_ub_neg": (-1e7, -1.5e7, INF, "ub", (-1.5e7 - 0.296875) / -1e7),  # = 1.5000000296875
+    "rhs_lb_neg": (-1e7, -INF, -1.5e7, "lb", (-1.5e7 - 0.296875) / -1e7),
+}
+
+
+def reliable_instance(name):
+    a, lhs, rhs, _, _ = RELIABLE[name]
+    return make_instance([([(0, 1.0), (1, 1.0), (2, -1.0), (3, a)], lhs, rhs)],
+                         [1e14, 0.3, 1e14, 1.0], [1e14, 0.3, 1e14, 2.0], [0, 0, 0, 0])
+
+
+def test_reliable_residual_constants():
+    assert np.float64(1e14) + np.float64(0.3) - np.float64(1e14) == 0.296875
+    assert (1.5e7 - 0.296875) / 1e7 == 1.4999999703125 != (1.5e7 - 0.3) / 1e7
+    assert (-1.5e7 - 0.296875) / -1e7 == 1.5000000296875 != (-1.5e7 - 0.3) / -1e7
+
+
+@pytest.mark.parametrize("name", sorted(RELIABLE))
+def test_reliable_residual_recompute(name):
+    _, _, _, side, expect = RELIABLE[name]
+    inst = reliable_instance(name)
+    om = orc.OMatrix.from_instance(inst)
+    for fn in (orc.propagate_jacobi, orc.propagate_scip):
+        lb, ub, r = fn(om, inst.lb, inst.ub)
+        assert not r["cutoff"] and r["nchgbds"] == 1 and r["nrecompute"] == 1, (fn.__name__, r)
+        got = lb[3] if side == "lb" else ub[3]
+        assert got == expect, (fn.__name__, got, expect)
+        other = ub[3] if side == "lb" else lb[3]
+        assert other == (2.0 if side == "lb" else 1.0)

@@ -10,7 +10,7 @@ import pytest
 import scip_amd as sa
 from oracle import oracle as orc
 from tests import plugin_host as ph
-from tests.util import random_binary_instance, random_fixings, apply_fixings
+from tests.util import INF, apply_fixings, make_instance, random_binary_instance, random_fixings
 
 
 def test_testhost_builds_and_links_the_plugin():
@@ -304,3 +304,86 @@ def test_propexec_without_ranged_rows_follows_scip_parameter(gpu_ctx):
             lb, ub = h.bounds()
             assert np.array_equal(lb, ol) and np.array_equal(ub, ou), (ranged, reasons)
             assert h.close() == ph.SCIP_OKAY
+
+
+def _ranged_instance():
+    from tests.test_oracle_ranged import ranged_system
+    inst = ranged_system(np.random.default_rng(1300), 1500, 3000, slack=False)
+    with_r = _oracle(inst, inst.lb, inst.ub, ranged=True)
+    without = _oracle(inst, inst.lb, inst.ub, ranged=False)
+    assert not np.array_equal(with_r[0], without[0]) or not np.array_equal(with_r[1], without[1])
+    return inst, with_r, without
+
+
+# consPropLinear's gating (cons_linear.c:16147-16171) at the node PROPEXEC runs at:
+# (stage, in probing, depth, in repropagation, linear int parameters) -> ranged rows expected
+GATING = {
+    "root": ((ph.SCIP_STAGE_SOLVING, False, 0, False, {}), True),
+    "probing": ((ph.SCIP_STAGE_SOLVING, True, 0, False, {}), False),
+    "probing_presolving": ((ph.SCIP_STAGE_PRESOLVING, True, 0, False, {}), False),
+    "presolving_stage": ((ph.SCIP_STAGE_PRESOLVING, False, 0, False, {}), False),
+    "repropagation": ((ph.SCIP_STAGE_SOLVING, False, 0, True, {}), False),
+    "beyond_maxdepth": ((ph.SCIP_STAGE_SOLVING, False, 3, False, {"rangedrowmaxdepth": 2}), False),
+    "within_maxdepth": ((ph.SCIP_STAGE_SOLVING, False, 2, False, {"rangedrowmaxdepth": 2}), True),
+    "freq_skips_depth": ((ph.SCIP_STAGE_SOLVING, False, 3, False, {"rangedrowfreq": 2}), False),
+    "freq_hits_depth": ((ph.SCIP_STAGE_SOLVING, False, 4, False, {"rangedrowfreq": 2}), True),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(GATING))
+def test_propexec_ranged_rows_follow_conspropLinear_gating(gpu_ctx, case):
+    (stage, probing, depth, reprop, ints), ranged = GATING[case]
+    inst, with_r, without = _ranged_instance()
+    ol, ou, _ = with_r if ranged else without
+    for reasons in (0, 1):
+        h = ph.Host(inst)
+        assert h.call("th_set_int_param", b"propagating/gpulinear/reasons", reasons) == ph.SCIP_OKAY
+        for k, v in ints.items():
+            h.set_linear_int(k, v)
+        assert h.call("th_initsol") == ph.SCIP_OKAY
+        h.set_node(stage, probing, depth, reprop)
+        rc, res = h.propexec()
+        assert rc == ph.SCIP_OKAY and res == ph.SCIP_REDUCEDDOM
+        lb, ub = h.bounds()
+        assert np.array_equal(lb, ol) and np.array_equal(ub, ou), (case, reasons)
+        assert h.close() == ph.SCIP_OKAY
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("depth,tightens", [(1, False), (2, True), (0, True)])
+def test_propexec_tightenboundsfreq(gpu_ctx, depth, tightens):
+    # tightenboundsfreq 2 with the propagator's frequency 1: bounds are tightened at depths 0, 2,
+    # 4, ...; elsewhere rows are only tested (cons_linear.c:16155-16157)
+    inst = sa.gen_mixed(3000, 3000, seed=0, planted=True)
+    ol, ou, orr = orc.propagate_jacobi(orc.OMatrix.from_instance(inst), inst.lb, inst.ub,
+                                       orc.params(rangedrows=1, notighten=int(not tightens)))
+    assert not orr["cutoff"]
+    for reasons in (0, 1):
+        h = ph.Host(inst)
+        assert h.call("th_set_int_param", b"propagating/gpulinear/reasons", reasons) == ph.SCIP_OKAY
+        h.set_linear_int("tightenboundsfreq", 2)
+        assert h.call("th_initsol") == ph.SCIP_OKAY
+        h.set_node(ph.SCIP_STAGE_SOLVING, False, depth, False)
+        rc, res = h.propexec()
+        assert rc == ph.SCIP_OKAY
+        assert res == (ph.SCIP_REDUCEDDOM if tightens else ph.SCIP_DIDNOTFIND)
+        lb, ub = h.bounds()
+        assert np.array_equal(lb, ol) and np.array_equal(ub, ou)
+        if not tightens:
+            assert np.array_equal(lb, inst.lb) and np.array_equal(ub, inst.ub)
+        assert h.close() == ph.SCIP_OKAY
+
+
+@pytest.mark.gpu
+def test_propexec_without_tightening_still_finds_infeasible_rows(gpu_ctx):
+    # tightenboundsfreq -1: no tightening at any depth, but the final test of propagateCons
+    # (cons_linear.c:7717-7742) still cuts off a row whose minimal activity exceeds its rhs
+    inst = make_instance([([(0, 1.0), (1, 1.0)], 3.0, INF), ([(1, 1.0), (2, 1.0)], -INF, 1.0)],
+                         [0.0, 0.0, 0.0], [1.0, 1.0, 1.0], [1, 1, 1])
+    h = ph.Host(inst)
+    h.set_linear_int("tightenboundsfreq", -1)
+    assert h.call("th_initsol") == ph.SCIP_OKAY
+    rc, res = h.propexec()
+    assert rc == ph.SCIP_OKAY and res == ph.SCIP_CUTOFF
+    assert h.close() == ph.SCIP_OKAY
