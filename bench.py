@@ -36,35 +36,62 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(budget_s: float = 12.0, planted: bool = False):
-    """SCIP-order (Gauss-Seidel) CPU restatement, single thread, on a bounded sample of the
-    same workload: the same generator at 100,000 x 100,000, repeated until the budget."""
+def host_cpu():
+    """The host the CPU legs ran on: CPU model, logical CPUs, CPUs this process may use."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"model": model, "nproc": os.cpu_count(), "usable": usable}
+
+
+def cpu_baseline(rows: int, cols: int, budget_s: float = 20.0):
+    """SCIP-order (Gauss-Seidel) CPU restatement (oracle/oracle.c orc_propagate_scip), single
+    thread, on the timed workload itself: the large instance with literal sides and with
+    planted-feasible sides, each propagated to its fixpoint from the root bounds, repeated
+    until half the budget is spent (at least once)."""
     import scip_amd as sa
     from oracle import oracle as orc
-    inst = sa.gen_mixed(100_000, 100_000, seed=2, planted=planted)
-    om = orc.OMatrix.from_instance(inst)
-    nnz = 0
-    runs = 0
-    t0 = time.perf_counter()
-    ttf = []
-    while True:
-        t = time.perf_counter()
-        _, _, r = orc.propagate_scip(om, inst.lb, inst.ub)
-        ttf.append(time.perf_counter() - t)
-        nnz += r["nnz_swept"]
-        runs += 1
-        if time.perf_counter() - t0 >= budget_s:
-            break
-    el = time.perf_counter() - t0
+    legs = {}
+    for name, planted in (("literal", False), ("planted", True)):
+        inst = sa.gen_mixed(rows, cols, seed=2, planted=planted)
+        om = orc.OMatrix.from_instance(inst)
+        ttf = []
+        t0 = time.perf_counter()
+        while True:
+            t = time.perf_counter()
+            _, _, r = orc.propagate_scip(om, inst.lb, inst.ub)
+            ttf.append(time.perf_counter() - t)
+            if time.perf_counter() - t0 >= budget_s / 2:
+                break
+        legs[name] = {"time_to_fixpoint_ms": 1000 * min(ttf), "runs": len(ttf), "rounds": r["rounds"],
+                      "nnz_swept": r["nnz_swept"], "result": "cutoff" if r["cutoff"] else "reduceddom",
+                      "nnz_sweeps_per_s": r["nnz_swept"] / min(ttf)}
+        del om, inst
+    pl = legs["planted"]
     return {
-        "value": nnz / el,
+        "value": pl["nnz_sweeps_per_s"],
         "unit": "nnz-sweeps/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"SCIP-order restatement (oracle/oracle.c orc_propagate_scip), 1 thread, same generator at "
-                   f"100000x100000 (nnz {inst.nnz}), {runs} runs to fixpoint in {el:.1f}s; "
-                   f"time-to-fixpoint {1000 * min(ttf):.1f} ms"),
-        "time_to_fixpoint_ms": 1000 * min(ttf),
+        "sample": (f"SCIP-order restatement (oracle/oracle.c orc_propagate_scip), 1 thread, on the timed "
+                   f"{rows}x{cols} instance itself: literal sides (cut off after "
+                   f"{legs['literal']['nnz_swept']} swept nonzeros) and planted-feasible sides "
+                   f"({pl['rounds']} rounds, {pl['nnz_swept']} swept nonzeros); value is the planted "
+                   "variant's nnz-sweeps/s, the one that runs the fixpoint loop; time to fixpoint of both "
+                   "beside it"),
+        "time_to_fixpoint_ms": {k: v["time_to_fixpoint_ms"] for k, v in legs.items()},
+        "legs": legs,
+        "host": host_cpu(),
     }
 
 
@@ -101,15 +128,17 @@ def main():
     ap.add_argument("--cols", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", action="store_true", help="only run the copy kernel used to scale PMC bytes")
+    ap.add_argument("--gather-probe", action="store_true",
+                    help="only run the memory-floor kernel of the row sweep on the large instance (for profiling)")
     ap.add_argument("--variant", choices=("both", "literal", "planted", "none"), default="both",
                     help="literal: only the headline instance (for profiling); planted: only the feasible variant; "
                          "none: only the batched legs")
-    ap.add_argument("--legs", default="nodes,probe,packed,serve",
+    ap.add_argument("--legs", default="cfg1,nodes,probe,packed,serve",
                     help="batched legs to run after the headline: nodes, probe, packed, serve (comma list, or none)")
     ap.add_argument("--nodes", type=int, default=1024, help="node vectors of the batched-nodes leg")
     ap.add_argument("--probe-vectors", type=int, default=2048, help="branch vectors per launch of the probing leg")
     ap.add_argument("--packed-instances", type=int, default=512, help="instances of the packed leg")
-    ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the served arrival trace")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="length of the served arrival trace")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate (requests/s)")
     args = ap.parse_args()
 
@@ -123,6 +152,21 @@ def main():
         return
 
     import scip_amd as sa
+
+    if args.gather_probe:
+        ctx = sa.Context(0)
+        inst = sa.gen_mixed(args.rows, args.cols, seed=2)
+        mat = sa.Matrix(ctx, inst)
+        b = sa.Bounds(mat, 1)
+        b.upload(inst.lb, inst.ub)
+        ms = b.bench_gather(max(args.steps, 1))
+        by = SWEEP_BYTES_PER_NNZ * inst.nnz + SWEEP_BYTES_PER_ROW * inst.nrows
+        print(json.dumps({"gather_floor_ms": ms, "algorithmic_bytes": by, "achieved_GBs": by / (ms * 1e-3) / 1e9}),
+              flush=True)
+        b.close()
+        mat.close()
+        ctx.close()
+        return
 
     ctx = sa.Context(0)
     if args.variant == "none":
@@ -145,17 +189,64 @@ def main():
     }
     if not args.no_cpu:
         log("[bench] CPU baseline leg")
-        main_out["cpu_baseline"] = cpu_baseline()
-        fb = cpu_baseline(budget_s=6.0, planted=True)
-        main_out["config"]["feasible_variant"]["cpu_baseline_value"] = fb["value"]
-        main_out["config"]["feasible_variant"]["cpu_time_to_fixpoint_ms_100k_sample"] = fb["time_to_fixpoint_ms"]
+        cb = cpu_baseline(args.rows, args.cols)
+        main_out["cpu_baseline"] = cb
+        main_out["config"]["gpu_vs_cpu_time_to_fixpoint"] = {
+            "literal": cb["time_to_fixpoint_ms"]["literal"] / main_out["ms_per_step"],
+            "planted": cb["time_to_fixpoint_ms"]["planted"] / feas["ms_per_step"]}
     print(json.dumps(run_legs(sa, ctx, args, main_out)), flush=True)
     ctx.close()
+
+
+def run_config1(sa, ctx, args):
+    """The reference's CPU-runnable case: covering/packing 2,000 x 5,000, 10 nonzeros per row,
+    seed 1, root bounds to fixpoint.  Time to fixpoint of the plain GPU path (bounds resident,
+    one copy of the root bounds and one propagation per step) against the SCIP-order CPU
+    restatement, single thread.  Pure binary: the fixpoint is unique, so both must agree."""
+    from oracle import oracle as orc
+    inst = sa.gen_covering_packing(2000, 5000, 10, seed=1)
+    mat = sa.Matrix(ctx, inst)
+    init = sa.Bounds(mat, 1)
+    init.upload(inst.lb, inst.ub)
+    work = sa.Bounds(mat, 1)
+    for _ in range(20):
+        work.copy_from(init)
+        st = work.propagate()
+    reps = 500
+    t0 = time.perf_counter()
+    dev = 0.0
+    for _ in range(reps):
+        work.copy_from(init)
+        st = work.propagate()
+        dev += st["device_ms"]
+    gpu_ms = 1000 * (time.perf_counter() - t0) / reps
+    gl, gu = work.download(0)
+    work.close()
+    init.close()
+    mat.close()
+    om = orc.OMatrix.from_instance(inst)
+    ttf = []
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 2.0:
+        t = time.perf_counter()
+        sl, su, sr = orc.propagate_scip(om, inst.lb, inst.ub)
+        ttf.append(time.perf_counter() - t)
+    same = bool((st["result"] == sa.SAP_CUTOFF) == bool(sr["cutoff"])
+                and (sr["cutoff"] or (np.array_equal(gl, sl) and np.array_equal(gu, su))))
+    return {"workload": "covering/packing 2000x5000, 10 nnz/row, coefficients U{1..100}, sides 0.4 x row sum, "
+                        "seed 1; root bounds to fixpoint",
+            "gpu_ms_to_fixpoint": gpu_ms, "gpu_device_ms": dev / reps, "gpu_rounds": st["rounds"],
+            "gpu_nchgbds": st["nchgbds"], "gpu_nnz_swept": st["nnz_swept"],
+            "cpu_ms_to_fixpoint": 1000 * min(ttf), "cpu_runs": len(ttf), "cpu_rounds": sr["rounds"], "cpu_cores": 1,
+            "cpu_kind": "SCIP-order restatement (oracle/oracle.c orc_propagate_scip)",
+            "speedup_time_to_fixpoint": (1000 * min(ttf)) / gpu_ms, "same_fixpoint": same}
 
 
 def run_legs(sa, ctx, args, main_out):
     legs = [x for x in args.legs.split(",") if x and x != "none"]
     parts = None
+    if "cfg1" in legs:
+        main_out["config1_cpu_vs_gpu"] = run_config1(sa, ctx, args)
     if "nodes" in legs:
         main_out["batched_nodes"] = run_batched_nodes(sa, ctx, args)
     if "probe" in legs:
@@ -204,6 +295,9 @@ def run_workload(sa, ctx, args, planted: bool):
     log(f"[bench] {args.steps} steps in {el:.3f}s; rounds={rounds} result={st['result']} nchg={st['nchgbds']}")
 
     value = tot_nnz / el
+    # the memory floor: the sweep's streams and bound gathers over every nonzero, no arithmetic
+    floor_ms = init.bench_gather(20)
+    floor_bytes = SWEEP_BYTES_PER_NNZ * inst.nnz + SWEEP_BYTES_PER_ROW * inst.nrows
     sweep_bytes = SWEEP_BYTES_PER_NNZ * tot_nnz + SWEEP_BYTES_PER_ROW * tot_rows
     achieved = sweep_bytes / (tot_sweep_ms * 1e-3) / 1e9 if tot_sweep_ms > 0 else 0.0
     roofline = {
@@ -217,6 +311,13 @@ def run_workload(sa, ctx, args, planted: bool):
         "launches": tot_launch,
         "avg_launch_us": round(1000 * tot_sweep_ms / max(tot_launch, 1), 2),
         "algorithmic_bytes_per_launch": int(sweep_bytes / max(tot_launch, 1)),
+        "gather_floor": {
+            "what": "k_gather_floor: the sweep's value/column streams and (lb, ub) gather of every nonzero, "
+                    "no arithmetic (sap_bench_gather, HIP events, 20 launches)",
+            "ms": round(floor_ms, 4),
+            "achieved": round(floor_bytes / (floor_ms * 1e-3) / 1e9, 1) if floor_ms > 0 else None,
+            "frac": round(floor_bytes / (floor_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if floor_ms > 0 else None,
+        },
     }
     out = {
         "metric": METRIC,
@@ -243,6 +344,10 @@ def run_workload(sa, ctx, args, planted: bool):
             "nominal_nnz_sweeps_per_step": rounds * inst.nnz,
             "device_ms_per_step": tot_dev_ms / args.steps,
             "result": {sa.SAP_CUTOFF: "cutoff", sa.SAP_REDUCEDDOM: "reduceddom"}.get(st["result"], "didnotfind"),
+            "note": ("one-round cutoff: the literal sides are jointly infeasible at this size (the oracle agrees), "
+                     "so a step is one sweep plus the column step that finds it; the planted variant "
+                     "(feasible_variant) runs the fixpoint loop" if st["result"] == sa.SAP_CUTOFF and rounds == 1
+                     else "propagated to its fixpoint"),
             "parallelism": "single GPU",
         },
         "roofline": roofline,
@@ -281,19 +386,40 @@ def run_batched_nodes(sa, ctx, args):
         out = nodes.propagate_nodes(base, 0, ptr, col, val, val, p)
     steps = max(args.steps, 1)
     nnz = rows = 0
+    devms = 0.0
     t0 = time.perf_counter()
     for _ in range(steps):
         out = nodes.propagate_nodes(base, 0, ptr, col, val, val, p)
         nnz += out["stats"]["nnz_swept"]
         rows += out["stats"]["rows_swept"]
+        devms += out["stats"]["device_ms"]
     el = time.perf_counter() - t0
     res = out["result"]
+    # batched algorithmic bytes per call: the matrix once for the union of rows any vector
+    # swept (12 B per nonzero: value and column index), per vector and swept nonzero the bound
+    # gather (16 B), per swept row its pointer and sides (20 B), per changed column its bounds
+    # read and written and its candidate keys (32 B).  A row is swept only when it holds an
+    # overlay or a changed column of its vector, so the rows holding any of those bound the union.
+    hit = np.zeros(inst.ncols, dtype=bool)
+    hit[col] = True
+    hit[out["chgcol"]] = True
+    nzrow = np.repeat(np.arange(inst.nrows), np.diff(inst.rowptr))
+    rowhit = np.zeros(inst.nrows, dtype=bool)
+    rowhit[nzrow[hit[inst.colidx]]] = True
+    nnz_union = int(np.diff(inst.rowptr)[rowhit].sum())
+    bbytes = 12 * nnz_union + 16 * (nnz // steps) + 20 * (rows // steps) + 32 * int(out["chgptr"][-1])
+    bach = bbytes / (devms / steps * 1e-3) / 1e9 if devms > 0 else 0.0
     batched = {"value": _rate(nnz, el), "unit": "nnz-sweeps/s", "ms_per_batch": 1000 * el / steps,
                "nodes_per_s": nn * steps / el, "nnz_swept_per_batch": nnz // steps,
                "rows_swept_per_batch": rows // steps,
                "cutoff_nodes": int((res == sa.SAP_CUTOFF).sum()),
                "reduceddom_nodes": int((res == sa.SAP_REDUCEDDOM).sum()),
-               "max_rounds": int(out["rounds"].max()), "changed_bounds": int(out["chgptr"][-1])}
+               "max_rounds": int(out["rounds"].max()), "changed_bounds": int(out["chgptr"][-1]),
+               "roofline": {"bound": "hbm", "algorithmic_bytes_per_call": bbytes, "nnz_union": nnz_union,
+                            "device_ms_per_call": devms / steps, "achieved": round(bach, 1), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(bach / HBM_PEAK_GBS, 4),
+                            "formula": "12*nnz_union + 16*sum nnz_swept + 20*sum rows_swept + 32*changed columns, "
+                                       "over the device time of the whole call"}}
     nodes.close()
     # plain: the same nodes one at a time through the server's C loop (PLAIN mode, all
     # arriving at once) -- admission, launch sequence and collection per node
@@ -380,13 +506,9 @@ def gen_uneven_instances(sa, count, seed=4):
     """Instances with rows log-uniform in [500, 500,000], columns 1.5 x rows, 4..64 nonzeros per
     row and the mixed integer/continuous distributions of the large instance; instance i uses
     seed 4000+i.  Sides are centred on a planted feasible point (sidemode 1)."""
-    rng = np.random.default_rng(seed)
-    m = np.rint(np.exp(rng.uniform(np.log(500), np.log(500_000), size=count))).astype(np.int64)
-    log(f"[bench] generating {count} uneven instances (rows {m.sum()})")
+    log(f"[bench] generating {count} uneven instances (rows {int(sa.uneven_sizes(count, seed).sum())})")
     t = time.perf_counter()
-    with ThreadPoolExecutor(16) as ex:
-        parts = list(ex.map(lambda i: sa.gen_mixed(int(m[i]), int(round(1.5 * m[i])), seed=4000 + i, lenmode=1,
-                                                   minlen=4, maxlen=64, planted=True), range(count)))
+    parts = sa.gen_uneven_instances(count, seed, threads=16)
     log(f"[bench] generated nnz={sum(p.nnz for p in parts)} in {time.perf_counter() - t:.1f}s")
     return parts
 
@@ -462,71 +584,114 @@ def run_packed(sa, ctx, args, parts):
             "speedup_nnz_sweeps": packed["value"] / plain["value"] if plain["value"] > 0 else None}
 
 
+def _pct(x, q):
+    return float(np.percentile(x, q)) if len(x) else None
+
+
+def _serve_summary(r, n, sel=None):
+    """Throughput and latency of one replay (over the requests sel, default all)."""
+    st = r["stats"]
+    lat = r["latency_ms"] if sel is None else r["latency_ms"][sel]
+    return {"value": _rate(st["nnz_swept"], st["wall_s"]), "unit": "nnz-sweeps/s", "requests": n,
+            "requests_per_s": n / st["wall_s"] if st["wall_s"] > 0 else 0.0, "wall_s": st["wall_s"],
+            "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99), "max_ms": float(lat.max()) if len(lat) else None,
+            "steps": st["steps"], "max_running": st["max_running"], "uploads": st["uploads"],
+            "evictions": st["evictions"], "peak_bytes": st["peak_bytes"], "nnz_swept": st["nnz_swept"]}
+
+
 def run_serving(sa, args, parts):
-    """Poisson arrivals (rate, seconds from the arguments); half the requests are nodes of the
-    covering/packing 200,000 x 300,000 problem (1..50 binaries fixed), half are nodes of a
-    uniformly chosen uneven instance (1..50 integer columns fixed to 0/1).  All problems are
-    resident (warmed) before the trace; continuous batching against the plain
-    one-request-at-a-time form on the same trace."""
+    """Poisson arrivals (rate, seconds from the arguments).  Half the requests are nodes of the
+    covering/packing 200,000 x 300,000 problem (1..50 binaries fixed); half are nodes (1..50
+    integer columns fixed to 0/1) of one of the 512 uneven instances, chosen with Zipf(1.2)
+    popularity over a random order, so request sizes follow the instances' size law.  The HBM
+    budget holds the covering/packing problem and 85 % of the instances' footprint: instances
+    are uploaded on demand and evicted least recently used during the trace.  Every replay
+    starts from the same resident set (the most popular instances that fit).  Continuous
+    batching against the plain one-request-at-a-time form: each at the offered rate (latency)
+    and saturated, all requests at t = 0 (capacity)."""
     cp = sa.gen_covering_packing(200_000, 300_000, 10, seed=3)
-    srv = sa.Server(0, 4, 200 << 30, 16)
-    pids = [srv.add_problem(cp)] + [srv.add_problem(p) for p in parts]
-    # half of all requests are nodes of the covering/packing problem: its pool gets more slots
-    # (256 x ~26 MB of per-slot state); the 512 instances keep 16 each
-    srv.set_slots(pids[0], 256)
-    t = time.perf_counter()
-    for pid in pids:
-        srv.warm(pid)
-    log(f"[bench] serving: {len(pids)} problems resident in {time.perf_counter() - t:.1f}s")
-    rng = np.random.default_rng(5)
-    n = int(args.serve_rate * args.serve_seconds)
-    arrival = np.cumsum(rng.exponential(1.0 / args.serve_rate, size=n))
-    arrival -= arrival[0]
-    kind = rng.integers(0, 2, size=n)
-    prob = np.where(kind == 0, pids[0], rng.integers(1, len(pids), size=n)).astype(np.int32)
     probs = [cp] + list(parts)
-    intcols = [np.nonzero(p.integral)[0] for p in probs]
-    ptr = np.zeros(n + 1, dtype=np.int64)
-    cols, vals = [], []
-    for i in range(n):
-        ic = intcols[prob[i]]
-        k = int(min(len(ic), rng.integers(1, 51)))
-        c = ic[np.unique(rng.integers(0, len(ic), size=k))] if k else np.zeros(0, dtype=np.int64)
-        cols.append(c.astype(np.int32))
-        vals.append(rng.integers(0, 2, size=len(c)).astype(np.float64))
-        ptr[i + 1] = ptr[i] + len(c)
-    col = np.concatenate(cols)
-    val = np.concatenate(vals)
+    arrival, prob, ptr, col, val, order = sa.gen_serving_trace(probs, args.serve_rate, args.serve_seconds, seed=5)
+    n = len(arrival)
+    srv = sa.Server(0, 4, 1 << 50, 8)
+    pids = [srv.add_problem(p) for p in probs]
+    assert pids == list(range(len(probs)))
+    srv.set_slots(0, 256)
+    foot = np.array([srv.footprint(pid) for pid in pids], dtype=np.int64)
+    budget = int(foot[0] + 0.85 * foot[1:].sum())
+    srv.set_budget(budget)
+    # the steady resident set: the covering/packing problem and the most popular instances
+    # that fit the budget, warmed least popular first so LRU order follows popularity
+    hot, used = [], foot[0]
+    for pid in order:
+        if used + foot[pid] > budget:
+            break
+        hot.append(int(pid))
+        used += foot[pid]
+
+    def warm_state():
+        srv.warm(0)
+        for pid in reversed(hot):
+            srv.warm(pid)
+
+    t = time.perf_counter()
+    warm_state()
+    log(f"[bench] serving: budget {budget / 2**30:.1f} GiB of {foot.sum() / 2**30:.1f} GiB; "
+        f"{len(hot)} instances warm in {time.perf_counter() - t:.1f}s")
+    is_cp = prob == 0
     log(f"[bench] serving trace: {n} requests over {args.serve_seconds}s")
+
+    def prefix(m, at_zero=False):
+        return (np.zeros(m) if at_zero else arrival[:m], prob[:m], ptr[:m + 1], col[:ptr[m]], val[:ptr[m]],
+                val[:ptr[m]])
+
+    # continuous batching at the offered rate
+    warm_state()
     cont = srv.replay(arrival, prob, ptr, col, val, val, mode=sa.SAP_SERVE_CONTINUOUS, steprounds=sa.SAP_STEP_SHORT)
-    cst = cont["stats"]
-    # saturated capacity: the same requests all arriving at once
+    log(f"[bench] serving continuous: {cont['stats']}")
+    # saturated capacity: the same requests all at t = 0
+    warm_state()
     sat = srv.replay(np.zeros(n), prob, ptr, col, val, val, mode=sa.SAP_SERVE_CONTINUOUS,
                      steprounds=sa.SAP_STEP_LONG)
-    sst = sat["stats"]
-    # plain on a prefix of the same trace (it falls behind the arrivals)
-    npl = min(n, max(200, n // 10))
-    pl = srv.replay(arrival[:npl], prob[:npl], ptr[:npl + 1], col[:ptr[npl]], val[:ptr[npl]], val[:ptr[npl]],
-                    mode=sa.SAP_SERVE_PLAIN, steprounds=sa.SAP_STEP_LONG)
-    pst = pl["stats"]
-    same = bool(np.array_equal(pl["result"], cont["result"][:npl]) and np.array_equal(pl["nchg"], cont["nchg"][:npl]))
+    # plain: saturated on a prefix (capacity), and at the offered rate on a 10 s prefix (latency)
+    npl = min(n, max(2000, n // 12))
+    warm_state()
+    plsat = srv.replay(*prefix(npl, at_zero=True), mode=sa.SAP_SERVE_PLAIN, steprounds=sa.SAP_STEP_LONG)
+    nof = min(n, int(10 * args.serve_rate))
+    warm_state()
+    plof = srv.replay(*prefix(nof), mode=sa.SAP_SERVE_PLAIN, steprounds=sa.SAP_STEP_LONG)
+    same = bool(np.array_equal(plof["result"], cont["result"][:nof]) and np.array_equal(plof["nchg"], cont["nchg"][:nof])
+                and np.array_equal(sat["result"], cont["result"]) and np.array_equal(sat["nchg"], cont["nchg"]))
     srv.close()
+    c_all = _serve_summary(cont, n)
+    c_all.update({
+        "covering_packing_nodes": {"requests": int(is_cp.sum()), "p50_ms": _pct(cont["latency_ms"][is_cp], 50),
+                                   "p99_ms": _pct(cont["latency_ms"][is_cp], 99)},
+        "instance_nodes_resident": {"requests": int((~is_cp & ~cont["cold"]).sum()),
+                                    "p50_ms": _pct(cont["latency_ms"][~is_cp & ~cont["cold"]], 50),
+                                    "p99_ms": _pct(cont["latency_ms"][~is_cp & ~cont["cold"]], 99)},
+        "instance_nodes_cold": {"requests": int(cont["cold"].sum()),
+                                "p50_ms": _pct(cont["latency_ms"][cont["cold"]], 50),
+                                "p99_ms": _pct(cont["latency_ms"][cont["cold"]], 99)},
+    })
+    p_of = _serve_summary(plof, nof)
+    p_of["p50_ms_resident"] = _pct(plof["latency_ms"][~plof["cold"]], 50)
+    p_of["p99_ms_resident"] = _pct(plof["latency_ms"][~plof["cold"]], 99)
+    c_sat = _serve_summary(sat, n)
+    p_sat = _serve_summary(plsat, npl)
     return {"workload": f"Poisson {args.serve_rate:.0f} req/s for {args.serve_seconds:.0f}s; 50% nodes of the "
-                        f"200000x300000 covering/packing problem, 50% nodes of {len(parts)} uneven instances; "
-                        "1..50 fixings per request",
+                        f"200000x300000 covering/packing problem (1..50 binaries fixed), 50% nodes of the "
+                        f"{len(parts)} uneven instances (Zipf 1.2 popularity, 1..50 integer columns fixed); "
+                        f"HBM budget {budget / 2**30:.1f} GiB of {foot.sum() / 2**30:.1f} GiB total footprint "
+                        "(LRU eviction and on-demand upload); same warm set before each replay",
             "requests": n,
-            "continuous": {"value": _rate(cst["nnz_swept"], cst["wall_s"]), "unit": "nnz-sweeps/s",
-                           "requests_per_s": n / cst["wall_s"], "p50_ms": cst["p50_ms"], "p99_ms": cst["p99_ms"],
-                           "max_ms": cst["max_ms"], "max_running": cst["max_running"], "steps": cst["steps"],
-                           "nnz_swept": cst["nnz_swept"]},
-            "saturated": {"value": _rate(sst["nnz_swept"], sst["wall_s"]), "unit": "nnz-sweeps/s",
-                          "requests_per_s": n / sst["wall_s"], "p50_ms": sst["p50_ms"], "p99_ms": sst["p99_ms"],
-                          "max_running": sst["max_running"]},
-            "plain": {"requests": npl, "value": _rate(pst["nnz_swept"], pst["wall_s"]), "unit": "nnz-sweeps/s",
-                      "requests_per_s": npl / pst["wall_s"], "p50_ms": pst["p50_ms"], "p99_ms": pst["p99_ms"],
-                      "results_equal_continuous": same},
-            "speedup_nnz_sweeps_saturated_vs_plain": (_rate(sst["nnz_swept"], sst["wall_s"])
-                                                      / max(_rate(pst["nnz_swept"], pst["wall_s"]), 1e-9))}
+            "continuous": c_all,
+            "continuous_saturated": c_sat,
+            "plain_offered_rate": p_of,
+            "plain_saturated": p_sat,
+            "results_equal_across_forms": same,
+            "speedup_nnz_sweeps_saturated": c_sat["value"] / max(p_sat["value"], 1e-9),
+            "speedup_requests_saturated": c_sat["requests_per_s"] / max(p_sat["requests_per_s"], 1e-9)}
 
 
 if __name__ == "__main__":

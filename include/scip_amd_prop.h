@@ -337,6 +337,12 @@ int sap_slots_collect(sap_bounds* bnd, int32_t n, const int32_t* slots, int32_t*
  * {minposinf, minneginf, minposhuge, minneghuge, maxposinf, maxneginf, maxposhuge,
  * maxneghuge}, maxactdelta as in consdataRecomputeMaxActivityDelta (cons_linear.c:1542).
  * Any output pointer may be NULL. */
+/* Measurement: the memory floor of the row sweep on this matrix -- its value and column-index
+ * streams and the (lb, ub) gather of every nonzero from vector 0 of b, without the sweep's
+ * arithmetic -- timed with HIP events over reps launches; *ms_per_pass is the mean launch time.
+ * No counterpart in SCIP. */
+int sap_bench_gather(sap_bounds* b, int32_t reps, double* ms_per_pass);
+
 int sap_row_activities(sap_matrix* mat, const sap_params* params, const double* lb, const double* ub,
    double* minact, double* maxact, int32_t* counts, double* maxactdelta);
 

@@ -69,7 +69,16 @@ int sap_server_add_problem(sap_server* srv, int64_t nrows, int64_t ncols, const 
  * has requests running or queued. */
 int sap_server_set_slots(sap_server* srv, int32_t id, int32_t slots);
 
-/* upload a problem now and compute its root fixpoint (otherwise done at its first request) */
+/* device bytes the problem occupies while resident (matrix, root vector, slot pool): what
+ * admission counts against the budget */
+int sap_server_footprint(sap_server* srv, int32_t id, int64_t* bytes);
+
+/* change the HBM budget; resident problems without running or queued requests are evicted,
+ * least recently used first, until the resident ones fit (SAP_NOMEMORY if they cannot) */
+int sap_server_set_budget(sap_server* srv, int64_t hbm_budget);
+
+/* upload a problem now and compute its root fixpoint (otherwise done at its first request);
+ * counts as a use of the problem for the least-recently-used order of eviction */
 int sap_server_warm(sap_server* srv, int32_t id);
 
 /* Replay a request trace in real time.  Request i arrives arrival_s[i] seconds after the start
@@ -85,6 +94,10 @@ int sap_server_replay(sap_server* srv, const sap_params* params, int32_t mode, i
    const double* arrival_s, const int32_t* prob, const int64_t* ovptr, const int32_t* ovcol, const double* ovlb,
    const double* ovub, int32_t* result, int32_t* rounds, int32_t* nchg, double* latency_ms, int64_t chgcap,
    int64_t* chgptr, int32_t* chgcol, double* chglb, double* chgub, sap_serve_stats* stats);
+
+/* per request of the last replay (nreq entries): 1 if its problem was uploaded while it
+ * waited (a cold request), else 0 */
+int sap_server_cold(sap_server* srv, int32_t nreq, uint8_t* cold);
 
 #ifdef __cplusplus
 }

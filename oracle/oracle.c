@@ -1329,7 +1329,7 @@ static void histAdd(jreasons* R, int32_t round, int32_t j, int32_t bound, double
 }
 
 static int jacobi(const orc_matrix* m, const orc_params* p, double* lb, double* ub, int full_sweep,
-   orc_result* res, jreasons* R)
+   const uint8_t* mark0, orc_result* res, jreasons* R)
 {
    int64_t nr = m->nrows, nc = m->ncols, i, j, k;
    int64_t L = maxRowLen(m), P = 1;
@@ -1374,7 +1374,10 @@ static int jacobi(const orc_matrix* m, const orc_params* p, double* lb, double* 
       js.cub[j] = HUGE_VAL;
       js.fub[j] = HUGE_VAL;
    }
-   memset(marked, 1, (size_t)(nr > 0 ? nr : 1));
+   if( mark0 != NULL )
+      memcpy(marked, mark0, (size_t)nr);
+   else
+      memset(marked, 1, (size_t)(nr > 0 ? nr : 1));
 
    for( ;; )
    {
@@ -1495,7 +1498,13 @@ static int jacobi(const orc_matrix* m, const orc_params* p, double* lb, double* 
 int orc_propagate_jacobi(const orc_matrix* m, const orc_params* p, double* lb, double* ub, int full_sweep,
    orc_result* res)
 {
-   return jacobi(m, p, lb, ub, full_sweep, res, NULL);
+   return jacobi(m, p, lb, ub, full_sweep, NULL, res, NULL);
+}
+
+int orc_propagate_jacobi_marked(const orc_matrix* m, const orc_params* p, double* lb, double* ub,
+   const uint8_t* mark0, orc_result* res)
+{
+   return jacobi(m, p, lb, ub, 0, mark0, res, NULL);
 }
 
 int orc_propagate_jacobi_reasons(const orc_matrix* m, const orc_params* p, double* lb, double* ub, int64_t histcap,
@@ -1513,7 +1522,7 @@ int orc_propagate_jacobi_reasons(const orc_matrix* m, const orc_params* p, doubl
    R.row = hrow;
    R.rule = hrule;
    R.conflict = conflict;
-   rc = jacobi(m, p, lb, ub, 0, res, &R);
+   rc = jacobi(m, p, lb, ub, 0, NULL, res, &R);
    *nhist = R.n;
    if( !res->cutoff )
       conflict->kind = ORC_CONFLICT_NONE;
@@ -1573,9 +1582,11 @@ typedef struct
    int64_t  nrecomp;     /* reliable residual recomputes */
 } gsstate;
 
-static const orc_params* g_sortp;
-static const gsstate* g_sortst;
-static int64_t g_sortrow;
+/* the qsort comparator's context; thread-local, so that tests may run the oracle on
+ * several instances at once from threads */
+static __thread const orc_params* g_sortp;
+static __thread const gsstate* g_sortst;
+static __thread int64_t g_sortrow;
 
 /* consdataCompVarProp (cons_linear.c:3191-3257) on the row's original positions */
 static int cmpVarProp(const void* pa, const void* pb)

@@ -91,6 +91,12 @@ void orc_row_activities(const orc_matrix* m, const orc_params* p, const double* 
 int orc_propagate_jacobi(const orc_matrix* m, const orc_params* p, double* lb, double* ub,
    int full_sweep, orc_result* res);
 
+/* the Jacobi twin started with only the rows flagged in mark0 (nrows bytes) marked: the
+ * device's SAP_FLAG_BASE_IS_FIXPOINT start, where only rows of changed columns are swept in
+ * the first round (same fixpoint when the rest is already one) */
+int orc_propagate_jacobi_marked(const orc_matrix* m, const orc_params* p, double* lb, double* ub,
+   const uint8_t* mark0, orc_result* res);
+
 /* SCIP-order Gauss-Seidel restatement; lb/ub are updated in place */
 int orc_propagate_scip(const orc_matrix* m, const orc_params* p, double* lb, double* ub,
    orc_result* res);

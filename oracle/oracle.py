@@ -67,6 +67,8 @@ def lib():
         L.orc_propagate_jacobi.argtypes = [C.POINTER(orc_matrix), C.POINTER(orc_params), _DP, _DP, C.c_int,
                                            C.POINTER(orc_result)]
         L.orc_propagate_scip.argtypes = [C.POINTER(orc_matrix), C.POINTER(orc_params), _DP, _DP, C.POINTER(orc_result)]
+        L.orc_propagate_jacobi_marked.argtypes = [C.POINTER(orc_matrix), C.POINTER(orc_params), _DP, _DP,
+                                                  C.POINTER(C.c_uint8), C.POINTER(orc_result)]
         L.orc_row_activities.argtypes = [C.POINTER(orc_matrix), C.POINTER(orc_params), _DP, _DP, C.c_int,
                                          C.POINTER(orc_rowact)]
         L.orc_compute_binary.argtypes = [C.c_int64, C.POINTER(C.c_uint8), _DP, _DP, C.POINTER(C.c_uint8)]
@@ -147,6 +149,32 @@ def propagate_jacobi(om: OMatrix, lb, ub, prm: orc_params | None = None, full_sw
     lib().orc_propagate_jacobi(C.byref(om.m), C.byref(prm or params()), lb.ctypes.data_as(_DP),
                                ub.ctypes.data_as(_DP), 1 if full_sweep else 0, C.byref(r))
     return lb, ub, r.as_dict()
+
+
+def propagate_jacobi_marked(om: OMatrix, lb, ub, mark0, prm: orc_params | None = None):
+    """The Jacobi twin started with only the rows flagged in mark0 marked (the device's
+    SAP_FLAG_BASE_IS_FIXPOINT start)."""
+    lb = np.array(lb, dtype=np.float64, copy=True)
+    ub = np.array(ub, dtype=np.float64, copy=True)
+    mk = np.ascontiguousarray(mark0, dtype=np.uint8)
+    assert len(mk) == om.m.nrows
+    r = orc_result()
+    lib().orc_propagate_jacobi_marked(C.byref(om.m), C.byref(prm or params()), lb.ctypes.data_as(_DP),
+                                      ub.ctypes.data_as(_DP), mk.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(r))
+    return lb, ub, r.as_dict()
+
+
+def rows_of_columns(om: OMatrix, cols):
+    """nrows flags: the rows that contain one of the given columns."""
+    hit = np.zeros(int(om.m.ncols), dtype=bool)
+    hit[np.asarray(cols, dtype=np.int64)] = True
+    nz = hit[om.colidx]
+    rows = getattr(om, "_nzrow", None)
+    if rows is None:
+        rows = om._nzrow = np.repeat(np.arange(int(om.m.nrows), dtype=np.int32), np.diff(om.rowptr))
+    out = np.zeros(int(om.m.nrows), dtype=np.uint8)
+    out[rows[nz]] = 1
+    return out
 
 
 def propagate_reasons(om: OMatrix, lb, ub, prm: orc_params | None = None):

@@ -205,6 +205,10 @@ EXPORTS = {
     "sap_server_add_problem": (C.c_int, [_P, C.c_int64, C.c_int64, _I64P, _I32P, _DP, _DP, _DP, _U8P, _DP, _DP,
                                          _I32P]),
     "sap_server_warm": (C.c_int, [_P, C.c_int32]),
+    "sap_server_footprint": (C.c_int, [_P, C.c_int32, _I64P]),
+    "sap_bench_gather": (C.c_int, [_P, C.c_int32, _DP]),
+    "sap_server_set_budget": (C.c_int, [_P, C.c_int64]),
+    "sap_server_cold": (C.c_int, [_P, C.c_int32, _U8P]),
     "sap_server_set_slots": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "sap_server_replay": (C.c_int, [_P, C.POINTER(sap_params), C.c_int32, C.c_int32, C.c_int32, _DP, _I32P, _I64P,
                                     _I32P, _DP, _DP, _I32P, _I32P, _I32P, _DP, C.c_int64, _I64P, _I32P, _DP, _DP,
@@ -497,6 +501,57 @@ class Matrix:
             pass
 
 
+def uneven_sizes(count: int, seed: int = 4):
+    """Row counts of the uneven-instances workload: log-uniform in [500, 500,000]."""
+    rng = np.random.default_rng(seed)
+    return np.rint(np.exp(rng.uniform(np.log(500), np.log(500_000), size=count))).astype(np.int64)
+
+
+def gen_uneven_instances(count: int = 512, seed: int = 4, threads: int = 16, which=None):
+    """The uneven-instances workload: instance i has rows from uneven_sizes, 1.5 x as many
+    columns, 4..64 nonzeros per row, the integer/continuous value, bound and coefficient
+    distributions of gen_mixed, sides centred on a planted feasible point, seed 4000 + i
+    (seed 4 draws the sizes).  which: the instance indices to build (default all)."""
+    from concurrent.futures import ThreadPoolExecutor
+    m = uneven_sizes(count, seed)
+    idx = range(count) if which is None else list(which)
+    one = lambda i: gen_mixed(int(m[i]), int(round(1.5 * m[i])), seed=1000 * seed + i, lenmode=1, minlen=4,
+                              maxlen=64, planted=True)
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        return list(ex.map(one, idx))
+
+
+def gen_serving_trace(probs, rate: float, seconds: float, seed: int = 5, zipf: float = 1.2):
+    """The serving workload's request trace over probs = [covering/packing problem, instance
+    1, ..., instance k]: Poisson arrivals at rate for seconds; half the requests are nodes of
+    probs[0], half nodes of an instance chosen with Zipf(zipf) popularity over a random order of
+    the instances; a node fixes 1..50 distinct integer columns to 0 or 1.  Returns (arrival_s,
+    prob, ovptr, ovcol, ovval, popularity order of the instances (indices into probs))."""
+    rng = np.random.default_rng(seed)
+    k = len(probs) - 1
+    order = 1 + rng.permutation(k)
+    w = 1.0 / np.arange(1, k + 1) ** zipf
+    w /= w.sum()
+    n = int(rate * seconds)
+    arrival = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    arrival -= arrival[0]
+    kind = rng.integers(0, 2, size=n)
+    prob = np.where(kind == 0, 0, order[rng.choice(k, size=n, p=w)]).astype(np.int32)
+    intcols = [np.nonzero(p.integral)[0] for p in probs]
+    ptr = np.zeros(n + 1, dtype=np.int64)
+    cols, vals = [], []
+    for i in range(n):
+        ic = intcols[prob[i]]
+        m = int(min(len(ic), rng.integers(1, 51)))
+        c = ic[np.unique(rng.integers(0, len(ic), size=m))] if m else np.zeros(0, dtype=np.int64)
+        cols.append(c.astype(np.int32))
+        vals.append(rng.integers(0, 2, size=len(c)).astype(np.float64))
+        ptr[i + 1] = ptr[i] + len(c)
+    col = np.concatenate(cols) if cols else np.zeros(0, dtype=np.int32)
+    val = np.concatenate(vals) if vals else np.zeros(0)
+    return arrival, prob, ptr, col, val, order
+
+
 def pack_instances(insts):
     """Concatenate instances block-diagonally: returns (Instance, rowoff, coloff)."""
     rowoff = np.zeros(len(insts) + 1, dtype=np.int64)
@@ -571,6 +626,12 @@ class Bounds:
         lb, ub = np.zeros(n), np.zeros(n)
         _chk(lib().sap_bounds_download(self.h, v, _dp(lb), _dp(ub)), "sap_bounds_download")
         return lb, ub
+
+    def bench_gather(self, reps: int = 20) -> float:
+        """Mean time (ms) of the row sweep's bare memory traffic over this matrix (sap_bench_gather)."""
+        ms = C.c_double(0.0)
+        _chk(lib().sap_bench_gather(self.h, reps, C.byref(ms)), "sap_bench_gather")
+        return ms.value
 
     def copy_from(self, src: "Bounds", dv: int = 0, sv: int = 0):
         _chk(lib().sap_bounds_copy(self.h, dv, src.h, sv), "sap_bounds_copy")
@@ -753,6 +814,15 @@ class Server:
                                           _dp(rh), _u8p(it), _dp(lb), _dp(ub), C.byref(pid)), "sap_server_add_problem")
         return pid.value
 
+    def set_budget(self, hbm_budget: int):
+        _chk(lib().sap_server_set_budget(self.h, hbm_budget), "sap_server_set_budget")
+
+    def footprint(self, pid: int) -> int:
+        """Device bytes the problem occupies while resident (what admission counts)."""
+        b = C.c_int64(0)
+        _chk(lib().sap_server_footprint(self.h, pid, C.byref(b)), "sap_server_footprint")
+        return b.value
+
     def warm(self, pid: int):
         _chk(lib().sap_server_warm(self.h, pid), "sap_server_warm")
 
@@ -793,7 +863,9 @@ class Server:
                                      _i64p(ovptr), _i32p(ovcol), _dp(ovlb), _dp(ovub), _i32p(res), _i32p(rnd),
                                      _i32p(nch), _dp(lat), 0, None, None, None, None, C.byref(st)),
              "sap_server_replay")
-        return dict(result=res, rounds=rnd, nchg=nch, latency_ms=lat, stats=st.as_dict())
+        cold = np.zeros(n, dtype=np.uint8)
+        _chk(lib().sap_server_cold(self.h, n, _u8p(cold)), "sap_server_cold")
+        return dict(result=res, rounds=rnd, nchg=nch, latency_ms=lat, stats=st.as_dict(), cold=cold.astype(bool))
 
     def close(self):
         if self.h:

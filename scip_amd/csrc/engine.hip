@@ -3369,6 +3369,32 @@ __global__ void k_deinterleave(const double2* src, double* lb, double* ub, int64
 }
 
 // row activities for diagnostics: one wavefront per row (chunked canonical tree)
+// the memory floor of the row sweep: the sweep's own streams and gathers without its
+// arithmetic.  Lane i of the grid reads nonzeros 4i..4i+3 (values and column indices,
+// coalesced, as a width-4 tile does) and gathers the four columns' (lb, ub); the sum is kept
+// live by a store that only a sum of exactly -1 would make (never, for the bench's data).
+__global__ __launch_bounds__(SAP_BLOCK) void k_gather_floor(Mat M, const double2* bnd, int64_t nnz, double* sink)
+{
+   const int64_t nq = (nnz + 3) / 4;
+   double acc = 0.0;
+   for( int64_t q = (int64_t)blockIdx.x * SAP_BLOCK + threadIdx.x; q < nq; q += (int64_t)gridDim.x * SAP_BLOCK )
+   {
+      const int64_t k0 = 4 * q;
+#pragma unroll
+      for( int t = 0; t < 4; ++t )
+      {
+         const int64_t k = k0 + t;
+         if( k < nnz )
+         {
+            const double2 b = bnd[M.colidx[k]];
+            acc += M.vals[k] * (b.x + b.y);
+         }
+      }
+   }
+   if( acc == -1.0 )
+      sink[0] = acc;
+}
+
 __global__ __launch_bounds__(SAP_BLOCK) void k_row_activities(Mat M, Params P, const double2* bnd, const int32_t* perm,
    double* minact, double* maxact, int32_t* counts, double* mad)
 {
@@ -6091,6 +6117,33 @@ int sap_slots_collect(sap_bounds* b, int32_t n, const int32_t* slots, int32_t* r
    RCCHK(collectSlots(b, n, slots, result, rounds, chgcap, chgptr, chgcol, chglb, chgub));
    for( int32_t i = 0; i < n; ++i )
       (*b->active)[(size_t)slots[i]] = 0;
+   return SAP_OKAY;
+}
+
+int sap_bench_gather(sap_bounds* b, int32_t reps, double* ms_per_pass)
+{
+   if( b == NULL || reps < 1 || ms_per_pass == NULL )
+      return SAP_INVALIDDATA;
+   sap_matrix* m = b->mat;
+   sap_context* c = m->ctx;
+   HIPCHK(hipSetDevice(c->device));
+   hipStream_t s = c->stream;
+   double* sink = NULL;
+   RCCHK(devAlloc(&sink, 1, NULL));
+   const Mat M = toMat(m);
+   // as many workgroups as the sweep keeps resident (4 waves per SIMD)
+   const unsigned grid = gridFor((size_t)((m->nnz + 3) / 4), 256 * 4 * 4 / (SAP_BLOCK / SAP_WAVE));
+   k_gather_floor<<<grid, SAP_BLOCK, 0, s>>>(M, b->V.bnd, m->nnz, sink);   // warm-up
+   HIPCHK(hipEventRecord(c->ev[0], s));
+   for( int32_t r = 0; r < reps; ++r )
+      k_gather_floor<<<grid, SAP_BLOCK, 0, s>>>(M, b->V.bnd, m->nnz, sink);
+   HIPCHK(hipEventRecord(c->ev[1], s));
+   HIPCHK(hipGetLastError());
+   HIPCHK(hipEventSynchronize(c->ev[1]));
+   float ms = 0.0f;
+   HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+   (void)hipFree(sink);
+   *ms_per_pass = (double)ms / reps;
    return SAP_OKAY;
 }
 

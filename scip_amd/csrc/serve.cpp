@@ -51,6 +51,7 @@ struct Request
    std::vector<int32_t> col;
    std::vector<double> clb, cub;
    bool done = false;
+   bool cold = false;   // its problem was uploaded while it waited
 };
 
 }   // namespace
@@ -67,6 +68,7 @@ struct sap_server
    // per replay
    sap_serve_stats st;
    int32_t ndone = 0;
+   std::vector<uint8_t> cold;   // per request of the last replay: waited for an upload
 };
 
 namespace {
@@ -106,6 +108,17 @@ void evict(sap_server* s, Problem* p)
    s->st.evictions++;
 }
 
+// the least recently used resident problem without running, queued or in-flight work
+Problem* lruVictim(sap_server* s)
+{
+   Problem* victim = NULL;
+   for( Problem* q : s->probs )
+      if( q->mat != NULL && q->running == 0 && q->queue.empty() && !q->inflight
+         && (victim == NULL || q->lastuse < victim->lastuse) )
+         victim = q;
+   return victim;
+}
+
 // 1: resident; 0: does not fit now (wait); < 0... SAP error code
 int makeResident(sap_server* s, Problem* p, const sap_params* params)
 {
@@ -116,11 +129,7 @@ int makeResident(sap_server* s, Problem* p, const sap_params* params)
       return SAP_NOMEMORY;
    while( s->resident + need > s->budget )
    {
-      Problem* victim = NULL;
-      for( Problem* q : s->probs )
-         if( q->mat != NULL && q->running == 0 && q->queue.empty() && !q->inflight
-            && (victim == NULL || q->lastuse < victim->lastuse) )
-            victim = q;
+      Problem* victim = lruVictim(s);
       if( victim == NULL )
          return 0;
       evict(s, victim);
@@ -384,13 +393,48 @@ int sap_server_set_slots(sap_server* s, int32_t id, int32_t slots)
    return SAP_OKAY;
 }
 
+int sap_server_cold(sap_server* s, int32_t nreq, uint8_t* cold)
+{
+   if( s == NULL || cold == NULL || nreq != (int32_t)s->cold.size() )
+      return SAP_INVALIDDATA;
+   memcpy(cold, s->cold.data(), (size_t)nreq);
+   return SAP_OKAY;
+}
+
+int sap_server_set_budget(sap_server* s, int64_t hbm_budget)
+{
+   if( s == NULL || hbm_budget <= 0 )
+      return SAP_INVALIDDATA;
+   s->budget = hbm_budget;
+   while( s->resident > s->budget )
+   {
+      Problem* victim = lruVictim(s);
+      if( victim == NULL )
+         return SAP_NOMEMORY;
+      evict(s, victim);
+   }
+   return SAP_OKAY;
+}
+
+int sap_server_footprint(sap_server* s, int32_t id, int64_t* bytes)
+{
+   if( s == NULL || bytes == NULL || id < 0 || id >= (int32_t)s->probs.size() )
+      return SAP_INVALIDDATA;
+   const Problem* p = s->probs[(size_t)id];
+   *bytes = footprint(p, p->slots);
+   return SAP_OKAY;
+}
+
 int sap_server_warm(sap_server* s, int32_t id)
 {
    if( s == NULL || id < 0 || id >= (int32_t)s->probs.size() )
       return SAP_INVALIDDATA;
    sap_params prm;
    sap_params_default(&prm);
-   const int r = makeResident(s, s->probs[(size_t)id], &prm);
+   Problem* p = s->probs[(size_t)id];
+   const int r = makeResident(s, p, &prm);
+   if( r == 1 )
+      p->lastuse = ++s->tick;   // warming counts as a use for the LRU order
    return r == 1 ? SAP_OKAY : (r == 0 ? SAP_NOMEMORY : r);
 }
 
@@ -428,6 +472,7 @@ int sap_server_replay(sap_server* s, const sap_params* params, int32_t mode, int
       {
          waitUntil(t0 + arrival_s[i]);
          Problem* p = s->probs[(size_t)prob[i]];
+         req[(size_t)i].cold = p->mat == NULL;
          int r = makeResident(s, p, params);
          if( r != 1 )
          {
@@ -467,12 +512,16 @@ int sap_server_replay(sap_server* s, const sap_params* params, int32_t mode, int
          {
             if( p->queue.empty() || p->inflight )
                continue;
+            const bool wascold = p->mat == NULL;
             const int r = makeResident(s, p, params);
             if( r < 0 )
             {
                rc = r;
                break;
             }
+            if( r == 1 && wascold )
+               for( const int32_t i : p->queue )
+                  req[(size_t)i].cold = true;
             if( r == 1 )
                rc = admit(s, p, req, ovptr, ovcol, ovlb, ovub, p->slots, t0);
             if( rc != SAP_OKAY )
@@ -528,6 +577,9 @@ int sap_server_replay(sap_server* s, const sap_params* params, int32_t mode, int
    }
    const double t1 = nowSeconds();
    s->st.wall_s = t1 - t0 - (nreq > 0 ? arrival_s[0] : 0.0);
+   s->cold.assign((size_t)nreq, 0);
+   for( int32_t i = 0; i < nreq; ++i )
+      s->cold[(size_t)i] = req[(size_t)i].cold ? 1 : 0;
    std::vector<double> lat;
    int64_t tot = 0;
    for( int32_t i = 0; i < nreq; ++i )
