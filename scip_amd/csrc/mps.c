@@ -485,7 +485,8 @@ int sap_read_mps(const char* path, sap_instance* out, int32_t* nsemicont)
       if( semic[j] )
          lb[j] = 0.0;
    /* CSR with ascending columns; repeated entries of a row are summed, zero sums dropped */
-   qsort(trip, (size_t)ntrip, sizeof(triplet), cmpTriplet);
+   if( ntrip > 1 )   /* a file without entries has no triplet array at all */
+      qsort(trip, (size_t)ntrip, sizeof(triplet), cmpTriplet);
    out->nrows = nrows;
    out->ncols = ncols;
    out->rowptr = (int64_t*)calloc((size_t)nrows + 1, sizeof(int64_t));
