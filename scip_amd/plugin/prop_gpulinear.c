@@ -1025,6 +1025,206 @@ SCIP_DECL_PROPRESPROP(propRespropGpulinear)
  * propagator specific interface methods
  */
 
+/** the propagation part of strong branching for many candidates at once on the device
+ *  (SCIPgetVarStrongbranchWithPropagation, scip_var.c:4139-4440, without the LP solve): the
+ *  children x <= feasCeil(solval - 1) and x >= feasFloor(solval + 1) of every candidate are
+ *  propagated from the node's local bounds in one batched device call (sap_probe_split), and
+ *  the deductions a strong-branching rule draws from them (branch_relpscost.c:1992-2070) are
+ *  collected against the node:
+ *  - a child whose branching bound leaves the local domain, or whose propagation ends in a
+ *    cutoff, is infeasible (downinf / upinf);
+ *  - both children infeasible: the node is infeasible (*cutoff);
+ *  - one child infeasible: the candidate's bound of the other child holds at the node;
+ *  - both feasible: the union of the children's bounds holds at the node (SCIP's newlbs /
+ *    newubs), which differs from the node's bounds only on columns changed in both children.
+ *  newlbs / newubs (indexed by SCIPvarGetProbindex, or NULL) receive the node's local bounds
+ *  tightened by all of these; the caller applies them as a branching rule does. */
+SCIP_RETCODE SCIPgpulinearStrongbranch(
+   SCIP*                 scip,               /**< SCIP data structure */
+   int                   ncands,             /**< number of candidates */
+   SCIP_VAR**            cands,              /**< integral candidate variables (columns of the matrix) */
+   const SCIP_Real*      solvals,            /**< LP values of the candidates */
+   SCIP_Bool*            downinf,            /**< array of ncands: the down child is infeasible */
+   SCIP_Bool*            upinf,              /**< array of ncands: the up child is infeasible */
+   SCIP_Real*            newlbs,             /**< array of the matrix's columns: valid lower bounds, or NULL */
+   SCIP_Real*            newubs,             /**< array of the matrix's columns: valid upper bounds, or NULL */
+   SCIP_Bool*            cutoff              /**< pointer to store whether the node is infeasible */
+   )
+{
+   SCIP_PROP* prop = SCIPfindProp(scip, PROP_NAME);
+   SCIP_PROPDATA* propdata;
+   sap_params params;
+   sap_stats stats;
+   double* lb;
+   double* ub;
+   int32_t* pcol;
+   int32_t* pidx;
+   double* downub;
+   double* uplb;
+   int32_t* verdict;
+   int64_t* dedptr;
+   int32_t* dedcol = NULL;
+   double* ded[4] = { NULL, NULL, NULL, NULL };
+   int64_t cap;
+   int np = 0;
+   int rc;
+   int c;
+   int j;
+   int k;
+
+   if( prop == NULL || ncands < 0 || (ncands > 0 && (cands == NULL || solvals == NULL || downinf == NULL
+            || upinf == NULL)) || cutoff == NULL || ((newlbs == NULL) != (newubs == NULL)) )
+      return SCIP_INVALIDDATA;
+   propdata = SCIPpropGetData(prop);
+   if( propdata->mat == NULL )
+      return SCIP_INVALIDCALL;   /* no device matrix (before INITSOL/presolving) */
+   *cutoff = FALSE;
+   SCIP_CALL( SCIPallocBufferArray(scip, &lb, propdata->nvars) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &ub, propdata->nvars) );
+   for( j = 0; j < propdata->nvars; ++j )
+   {
+      lb[j] = SCIPvarGetLbLocal(propdata->vars[j]);
+      ub[j] = SCIPvarGetUbLocal(propdata->vars[j]);
+   }
+   if( newlbs != NULL )
+   {
+      memcpy(newlbs, lb, sizeof(SCIP_Real) * (size_t)propdata->nvars);
+      memcpy(newubs, ub, sizeof(SCIP_Real) * (size_t)propdata->nvars);
+   }
+   SCIP_CALL( SCIPallocBufferArray(scip, &pcol, ncands + 1) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &pidx, ncands + 1) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &downub, ncands + 1) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &uplb, ncands + 1) );
+   for( c = 0; c < ncands; ++c )
+   {
+      int col = SCIPvarGetProbindex(cands[c]);
+      double dub = SCIPfeasCeil(scip, solvals[c] - 1.0);
+      double ulb = SCIPfeasFloor(scip, solvals[c] + 1.0);
+
+      if( col < 0 || col >= propdata->nvars || propdata->vars[col] != cands[c] || !SCIPvarIsIntegral(cands[c]) )
+      {
+         SCIPfreeBufferArray(scip, &uplb);
+         SCIPfreeBufferArray(scip, &downub);
+         SCIPfreeBufferArray(scip, &pidx);
+         SCIPfreeBufferArray(scip, &pcol);
+         SCIPfreeBufferArray(scip, &ub);
+         SCIPfreeBufferArray(scip, &lb);
+         return SCIP_INVALIDDATA;
+      }
+      /* a branching bound outside the local domain makes that child infeasible without any
+       * propagation (scip_var.c:4266-4313) */
+      upinf[c] = ulb > ub[col] + 0.5;
+      downinf[c] = dub < lb[col] - 0.5;
+      if( !upinf[c] && !downinf[c] )
+      {
+         pcol[np] = col;
+         pidx[np] = c;
+         downub[np] = dub;
+         uplb[np] = ulb;
+         ++np;
+      }
+   }
+   if( np > 0 )
+   {
+      if( propdata->probebase == NULL )
+      {
+         SCIP_CALL( engineCall(sap_bounds_create(propdata->mat, 1, &propdata->probebase), "sap_bounds_create") );
+      }
+      if( propdata->probeset == NULL )
+      {
+         SCIP_CALL( engineCall(sap_bounds_create(propdata->mat, propdata->probingvectors, &propdata->probeset),
+               "sap_bounds_create") );
+      }
+      SCIP_CALL( engineCall(sap_bounds_upload(propdata->probebase, 0, lb, ub), "sap_bounds_upload") );
+      SCIP_CALL( getEngineParams(scip, propdata, &params) );   /* node propagation: every row in round 1 */
+      SCIP_CALL( SCIPallocBufferArray(scip, &verdict, np) );
+      SCIP_CALL( SCIPallocBufferArray(scip, &dedptr, np + 1) );
+      cap = 16 * (int64_t)np + 64;
+      for( ;; )
+      {
+         SCIP_CALL( SCIPallocBufferArray(scip, &dedcol, cap) );
+         for( k = 0; k < 4; ++k )
+         {
+            SCIP_CALL( SCIPallocBufferArray(scip, &ded[k], cap) );
+         }
+         memset(&stats, 0, sizeof(stats));
+         rc = sap_probe_split(propdata->probeset, &params, np, pcol, downub, uplb, propdata->probebase, 0, verdict,
+            NULL, cap, dedptr, dedcol, ded[0], ded[1], ded[2], ded[3], &stats);
+         if( rc != SAP_NOMEMORY || dedptr[np] <= cap )
+            break;
+         cap = dedptr[np];   /* too little room for the pair lists: again with enough (same results) */
+         for( k = 3; k >= 0; --k )
+            SCIPfreeBufferArray(scip, &ded[k]);
+         SCIPfreeBufferArray(scip, &dedcol);
+      }
+      if( rc == SAP_OKAY )
+      {
+         for( k = 0; k < np; ++k )
+         {
+            c = pidx[k];
+            downinf[c] = (verdict[k] & SAP_PROBE_DOWN_INFEASIBLE) != 0;
+            upinf[c] = (verdict[k] & SAP_PROBE_UP_INFEASIBLE) != 0;
+            if( newlbs != NULL && !downinf[c] && !upinf[c] )
+            {
+               int64_t q;
+               /* columns changed in both children: the union of their bounds */
+               for( q = dedptr[k]; q < dedptr[k + 1]; ++q )
+               {
+                  int v = dedcol[q];
+                  double ulbv = ded[0][q] < ded[2][q] ? ded[0][q] : ded[2][q];
+                  double uubv = ded[1][q] > ded[3][q] ? ded[1][q] : ded[3][q];
+                  newlbs[v] = ulbv > newlbs[v] ? ulbv : newlbs[v];
+                  newubs[v] = uubv < newubs[v] ? uubv : newubs[v];
+               }
+            }
+         }
+      }
+      for( k = 3; k >= 0; --k )
+         SCIPfreeBufferArray(scip, &ded[k]);
+      SCIPfreeBufferArray(scip, &dedcol);
+      SCIPfreeBufferArray(scip, &dedptr);
+      SCIPfreeBufferArray(scip, &verdict);
+      if( rc != SAP_OKAY )
+      {
+         SCIPfreeBufferArray(scip, &uplb);
+         SCIPfreeBufferArray(scip, &downub);
+         SCIPfreeBufferArray(scip, &pidx);
+         SCIPfreeBufferArray(scip, &pcol);
+         SCIPfreeBufferArray(scip, &ub);
+         SCIPfreeBufferArray(scip, &lb);
+         return engineCall(rc, "sap_probe_split");
+      }
+   }
+   /* the branching bound of the feasible child of a candidate with one infeasible child */
+   for( c = 0; c < ncands && !*cutoff; ++c )
+   {
+      int col = SCIPvarGetProbindex(cands[c]);
+      if( downinf[c] && upinf[c] )
+         *cutoff = TRUE;
+      else if( newlbs != NULL && downinf[c] )
+      {
+         double ulb = SCIPfeasFloor(scip, solvals[c] + 1.0);
+         newlbs[col] = ulb > newlbs[col] ? ulb : newlbs[col];
+      }
+      else if( newlbs != NULL && upinf[c] )
+      {
+         double dub = SCIPfeasCeil(scip, solvals[c] - 1.0);
+         newubs[col] = dub < newubs[col] ? dub : newubs[col];
+      }
+   }
+   /* deductions of different candidates may contradict each other only if the node is infeasible */
+   for( j = 0; newlbs != NULL && j < propdata->nvars && !*cutoff; ++j )
+      if( newlbs[j] > newubs[j] + SCIPfeastol(scip) )
+         *cutoff = TRUE;
+   SCIPfreeBufferArray(scip, &uplb);
+   SCIPfreeBufferArray(scip, &downub);
+   SCIPfreeBufferArray(scip, &pidx);
+   SCIPfreeBufferArray(scip, &pcol);
+   SCIPfreeBufferArray(scip, &ub);
+   SCIPfreeBufferArray(scip, &lb);
+   return SCIP_OKAY;
+}
+
 /** creates the gpulinear propagator and includes it in SCIP */
 SCIP_RETCODE SCIPincludePropGpulinear(
    SCIP*                 scip                /**< SCIP data structure */

@@ -22,6 +22,26 @@ SCIP_RETCODE SCIPincludePropGpulinear(
    SCIP*                 scip                /**< SCIP data structure */
    );
 
+/** the propagation part of strong branching (SCIPgetVarStrongbranchWithPropagation,
+ *  src/scip/scip_var.c:4139-4440, without the LP) for many candidates at once on the device:
+ *  per candidate whether the down child x <= feasCeil(solval - 1) and the up child
+ *  x >= feasFloor(solval + 1) are infeasible, and the bounds valid at the node that follow
+ *  (the union of both children's bounds, the feasible child's branching bound when the other
+ *  child is infeasible, branch_relpscost.c:1992-2070), indexed by SCIPvarGetProbindex;
+ *  *cutoff when some candidate has two infeasible children */
+SCIP_EXPORT
+SCIP_RETCODE SCIPgpulinearStrongbranch(
+   SCIP*                 scip,               /**< SCIP data structure */
+   int                   ncands,             /**< number of candidates */
+   SCIP_VAR**            cands,              /**< integral candidate variables */
+   const SCIP_Real*      solvals,            /**< LP values of the candidates */
+   SCIP_Bool*            downinf,            /**< array of ncands: the down child is infeasible */
+   SCIP_Bool*            upinf,              /**< array of ncands: the up child is infeasible */
+   SCIP_Real*            newlbs,             /**< array of the active variables: valid lower bounds, or NULL */
+   SCIP_Real*            newubs,             /**< array of the active variables: valid upper bounds, or NULL */
+   SCIP_Bool*            cutoff              /**< pointer to store whether the node is infeasible */
+   );
+
 #ifdef __cplusplus
 }
 #endif

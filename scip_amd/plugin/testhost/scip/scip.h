@@ -110,6 +110,7 @@ SCIP_RETCODE SCIPsetPropPresol(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPPRESOL
 SCIP_RETCODE SCIPsetPropResprop(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPRESPROP((*propresprop)));
 /* pub_prop.h */
 SCIP_PROPDATA* SCIPpropGetData(SCIP_PROP* prop);
+SCIP_PROP* SCIPfindProp(SCIP* scip, const char* name);
 int SCIPpropGetFreq(SCIP_PROP* prop);
 void SCIPpropSetData(SCIP_PROP* prop, SCIP_PROPDATA* propdata);
 /* scip_param.h */
@@ -138,6 +139,8 @@ SCIP_Real SCIPsumepsilon(SCIP* scip);
 SCIP_Real SCIPfeastol(SCIP* scip);
 SCIP_Real SCIPgetHugeValue(SCIP* scip);
 SCIP_Bool SCIPisInfinity(SCIP* scip, SCIP_Real val);
+SCIP_Real SCIPfeasFloor(SCIP* scip, SCIP_Real val);
+SCIP_Real SCIPfeasCeil(SCIP* scip, SCIP_Real val);
 /* scip_cons.h / pub_cons.h */
 SCIP_CONSHDLR* SCIPfindConshdlr(SCIP* scip, const char* name);
 int SCIPconshdlrGetNConss(SCIP_CONSHDLR* conshdlr);

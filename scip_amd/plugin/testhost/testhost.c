@@ -109,6 +109,9 @@ SCIP_Real SCIPsumepsilon(SCIP* scip) { return scip->sumepsilon; }
 SCIP_Real SCIPfeastol(SCIP* scip) { return scip->feastol; }
 SCIP_Real SCIPgetHugeValue(SCIP* scip) { return scip->hugeval; }
 SCIP_Bool SCIPisInfinity(SCIP* scip, SCIP_Real val) { return val >= scip->infinity; }
+/* set.c SCIPsetFeasFloor / SCIPsetFeasCeil: floor(x + feastol), ceil(x - feastol) */
+SCIP_Real SCIPfeasFloor(SCIP* scip, SCIP_Real val) { return floor(val + scip->feastol); }
+SCIP_Real SCIPfeasCeil(SCIP* scip, SCIP_Real val) { return ceil(val - scip->feastol); }
 
 static double relDiff(double a, double b)
 {
@@ -213,6 +216,10 @@ SCIP_RETCODE SCIPsetPropPresol(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPPRESOL
 SCIP_RETCODE SCIPsetPropResprop(SCIP* scip, SCIP_PROP* prop, SCIP_DECL_PROPRESPROP((*f))) { (void)scip; prop->resprop = f; return SCIP_OKAY; }
 SCIP_PROPDATA* SCIPpropGetData(SCIP_PROP* prop) { return prop->data; }
 int SCIPpropGetFreq(SCIP_PROP* prop) { return prop->freq; }
+SCIP_PROP* SCIPfindProp(SCIP* scip, const char* name)
+{
+   return (scip->prop != NULL && strcmp(scip->prop->name, name) == 0) ? scip->prop : NULL;
+}
 void SCIPpropSetData(SCIP_PROP* prop, SCIP_PROPDATA* propdata) { prop->data = propdata; }
 
 /* ---- problem access ---- */
@@ -728,6 +735,34 @@ SCIP_EXPORT int th_resprop_change(SCIP* s, int pos, int* result)
    rc = s->prop->resprop(s, s->prop, s->vars[c->var], c->inferinfo,
       c->bound ? SCIP_BOUNDTYPE_UPPER : SCIP_BOUNDTYPE_LOWER, &idx, c->newval, &r);
    *result = (int)r;
+   return rc;
+}
+
+/* strong branching on the device (SCIPgpulinearStrongbranch) for candidate columns cand with
+ * LP values solval, at the current local bounds */
+SCIP_EXPORT int th_strongbranch(SCIP* s, int ncands, const int* cand, const double* solval, int* downinf, int* upinf,
+   double* newlbs, double* newubs, int* cutoff)
+{
+   SCIP_VAR** vars = (SCIP_VAR**)malloc(sizeof(SCIP_VAR*) * (size_t)(ncands > 0 ? ncands : 1));
+   SCIP_Bool* di = (SCIP_Bool*)malloc(sizeof(SCIP_Bool) * (size_t)(ncands > 0 ? ncands : 1));
+   SCIP_Bool* ui = (SCIP_Bool*)malloc(sizeof(SCIP_Bool) * (size_t)(ncands > 0 ? ncands : 1));
+   SCIP_Bool cut = FALSE;
+   SCIP_RETCODE rc;
+   int c;
+   if( vars == NULL || di == NULL || ui == NULL )
+      return SCIP_NOMEMORY;
+   for( c = 0; c < ncands; ++c )
+      vars[c] = s->vars[cand[c]];
+   rc = SCIPgpulinearStrongbranch(s, ncands, vars, solval, di, ui, newlbs, newubs, &cut);
+   for( c = 0; c < ncands; ++c )
+   {
+      downinf[c] = (int)di[c];
+      upinf[c] = (int)ui[c];
+   }
+   *cutoff = (int)cut;
+   free(vars);
+   free(di);
+   free(ui);
    return rc;
 }
 

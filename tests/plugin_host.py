@@ -54,6 +54,9 @@ def lib():
         L.th_set_linear_int.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
         L.th_set_node.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.th_set_node.restype = None
+        IPn = _P(np.int32)
+        L.th_strongbranch.argtypes = [C.c_void_p, C.c_int, IPn, _P(np.float64), IPn, IPn, _P(np.float64),
+                                      _P(np.float64), C.POINTER(C.c_int)]
         L.th_resprop_change.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
         L.th_get_changes.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         L.th_get_conflict.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
@@ -87,6 +90,19 @@ class Host:
 
     def set_linear_int(self, name, value):
         assert lib().th_set_linear_int(self.h, name.encode(), value) == SCIP_OKAY
+
+    def strongbranch(self, cands, solvals):
+        """SCIPgpulinearStrongbranch at the current local bounds: (downinf, upinf, newlbs, newubs,
+        cutoff, rc)."""
+        n = len(cands)
+        di = np.zeros(max(n, 1), dtype=np.int32)
+        ui = np.zeros(max(n, 1), dtype=np.int32)
+        nl = np.zeros(self.n)
+        nu = np.zeros(self.n)
+        cut = C.c_int(0)
+        rc = lib().th_strongbranch(self.h, n, np.ascontiguousarray(cands, dtype=np.int32),
+                                   np.ascontiguousarray(solvals, dtype=np.float64), di, ui, nl, nu, C.byref(cut))
+        return di[:n].astype(bool), ui[:n].astype(bool), nl, nu, bool(cut.value), rc
 
     def propexec(self):
         r = C.c_int(0)

@@ -387,3 +387,70 @@ def test_propexec_without_tightening_still_finds_infeasible_rows(gpu_ctx):
     rc, res = h.propexec()
     assert rc == ph.SCIP_OKAY and res == ph.SCIP_CUTOFF
     assert h.close() == ph.SCIP_OKAY
+
+
+def _strongbranch_expectation(inst, lb, ub, cands, solvals):
+    """Per candidate, both children propagated alone by the oracle's Jacobi twin; then the
+    deductions of a strong-branching rule (branch_relpscost.c:1992-2070) against the node."""
+    import math
+    om = orc.OMatrix.from_instance(inst)
+    feastol = 1e-6
+    nl, nu = lb.copy(), ub.copy()
+    di, ui = [], []
+    cut = False
+    for c, x in zip(cands, solvals):
+        dub = math.ceil(x - 1.0 - feastol)
+        ulb = math.floor(x + 1.0 + feastol)
+        kids = []
+        for side, b in (("down", dub), ("up", ulb)):
+            l, u = lb.copy(), ub.copy()
+            if side == "down":
+                u[c] = b
+            else:
+                l[c] = b
+            ol, ou, r = orc.propagate_jacobi(om, l, u)
+            kids.append((bool(r["cutoff"]), ol, ou))
+        (dcut, dl, du), (ucut, ul, uu) = kids
+        di.append(dcut)
+        ui.append(ucut)
+        if dcut and ucut:
+            cut = True
+        elif dcut:
+            nl[c] = max(nl[c], ulb)
+        elif ucut:
+            nu[c] = min(nu[c], dub)
+        else:
+            nl = np.maximum(nl, np.minimum(dl, ul))
+            nu = np.minimum(nu, np.maximum(du, uu))
+    cut = cut or bool(np.any(nl > nu + feastol))
+    return np.array(di), np.array(ui), nl, nu, cut
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_strongbranch_children_match_the_oracle(gpu_ctx, seed):
+    # the strong-branching consumer of the batched split probing: every candidate's two
+    # children against the oracle's propagation of that child alone
+    from tests.test_oracle_fixpoint import integer_instance
+    rng = np.random.default_rng(7000 + seed)
+    inst, _ = integer_instance(rng, nrows=300, ncols=200, maxlen=8, coefrange=20, boundrange=0, ubmax=10,
+                               planted=True)
+    rl, ru, rr = _oracle(inst, inst.lb, inst.ub, ranged=False)
+    assert not rr["cutoff"]
+    free = np.nonzero(rl < ru)[0]
+    cands = rng.choice(free, size=min(24, len(free)), replace=False).astype(np.int32)
+    solvals = rl[cands] + (ru[cands] - rl[cands]) * rng.uniform(0.05, 0.95, size=len(cands))
+    solvals = np.where(np.abs(solvals - np.round(solvals)) < 0.01, solvals + 0.3, solvals)
+    edi, eui, enl, enu, ecut = _strongbranch_expectation(inst, rl, ru, cands, solvals)
+    h = ph.Host(inst)
+    assert h.call("th_initsol") == ph.SCIP_OKAY
+    h.set_local(rl, ru)
+    di, ui, nl, nu, cut, rc = h.strongbranch(cands, solvals)
+    assert rc == ph.SCIP_OKAY
+    assert np.array_equal(di, edi) and np.array_equal(ui, eui)
+    assert cut == ecut
+    if not ecut:
+        assert np.array_equal(nl, enl) and np.array_equal(nu, enu)
+        # the deductions are real: some candidate has an infeasible child or a tighter union
+        assert edi.any() or eui.any() or (enl > rl).any() or (enu < ru).any()
+    assert h.close() == ph.SCIP_OKAY
