@@ -35,7 +35,15 @@
  *                                    closed-form cases computed by hand
  *   propagation fixpoints            brute-force soundness on tiny integer systems, completeness
  *                                    on infeasible binary systems, Jacobi = SCIP order on binary
- *                                    systems (unique fixpoint), mutual fixpoints otherwise
+ *                                    systems (unique fixpoint), mutual fixpoints otherwise;
+ *                                    both schedules = the greatest supported box, found by
+ *                                    enumeration, on integer systems with boundstreps 0
+ *                                    (tests/test_oracle_fixpoint.py)
+ *   reliable residual recompute      hand-computed cancellation cases whose bound only the
+ *                                    plain-double recompute gives (RELIABLE in
+ *                                    tests/test_oracle_prop.py)
+ *   tightenbounds FALSE (notighten)  closed form: only rows whose activity range misses a side
+ *                                    cut off, nothing moves
  *   ranged rows                      the two examples of the reference's own documentation of
  *                                    rangedRowPropagation (cons_linear.c:5702-5711), a hand-
  *                                    computed bound, soundness against enumeration of every
@@ -47,13 +55,12 @@
  *                                    (tests/test_oracle_reasons.py)
  *   probing                          hand-computed pair lists and soundness against enumeration
  *                                    of every 0/1 point (tests/test_oracle_probe.py)
- * Parity unpinned (no reference vector or closed form decides them; SCIP itself cannot be
+ * Parity unpinned (no reference vector or closed form decides it; SCIP itself cannot be
  * built here):
- *   - orc_propagate_scip's exact general-integer / continuous bounds where the row order and
- *     the staleness of SCIP's incremental updates matter (pinned only by soundness and the
- *     mutual-fixpoint property);
- *   - the reliable residual recompute (reliableResidual, cons_linear.c:2580): exercised, but
- *     no closed-form case separates it from the double-double residual.
+ *   - orc_propagate_scip's exact general-integer / continuous bounds where the fixpoint is
+ *     order dependent (boundstreps > 0) and the staleness of SCIP's incremental updates
+ *     matters; pinned only by soundness, the mutual-fixpoint property and exact agreement
+ *     wherever the fixpoint is unique.
  */
 #include "oracle.h"
 
