@@ -138,6 +138,12 @@ int sap_matrix_create(sap_context* ctx, int64_t nrows, int64_t ncols, const int6
    const int32_t* colidx, const double* vals, const double* lhs, const double* rhs,
    const uint8_t* integral, const double* glb, const double* gub, sap_matrix** mat);
 int sap_matrix_free(sap_matrix* mat);
+/* Hand a matrix, with its bound sets, over to another context of the same device: its later
+ * work runs on that context's stream.  For a loader thread that builds matrices on a context
+ * of its own while another thread propagates on others (the request server).  No work of the
+ * matrix may be in flight (a launched slots step must have been waited for).  No counterpart
+ * in SCIP. */
+int sap_matrix_set_context(sap_matrix* mat, sap_context* ctx);
 /* Autotuner: builds the system restricted to a sample of samplerows rows (all rows if fewer;
  * chosen by seed) once per candidate tuning ((rowbins, tilebins) in (0,3) (0,4) (0,5) (0,6)
  * (2,6) (3,6) x tilewidth 2/4 x longrow 1024/4096/16384 x capscale 25/50/100), propagates lb/ub
@@ -339,9 +345,10 @@ int sap_slots_collect(sap_bounds* bnd, int32_t n, const int32_t* slots, int32_t*
  * Any output pointer may be NULL. */
 /* Measurement: the memory floor of the row sweep on this matrix -- its value and column-index
  * streams and the (lb, ub) gather of every nonzero from vector 0 of b, without the sweep's
- * arithmetic -- timed with HIP events over reps launches; *ms_per_pass is the mean launch time.
- * No counterpart in SCIP. */
-int sap_bench_gather(sap_bounds* b, int32_t reps, double* ms_per_pass);
+ * arithmetic -- timed with HIP events over reps launches of blocks workgroups (0: one lane
+ * per four nonzeros, no grid-stride loop); *ms_per_pass is the mean launch time.  No
+ * counterpart in SCIP. */
+int sap_bench_gather(sap_bounds* b, int32_t reps, int32_t blocks, double* ms_per_pass);
 
 int sap_row_activities(sap_matrix* mat, const sap_params* params, const double* lb, const double* ub,
    double* minact, double* maxact, int32_t* counts, double* maxactdelta);

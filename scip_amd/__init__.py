@@ -206,7 +206,8 @@ EXPORTS = {
                                          _I32P]),
     "sap_server_warm": (C.c_int, [_P, C.c_int32]),
     "sap_server_footprint": (C.c_int, [_P, C.c_int32, _I64P]),
-    "sap_bench_gather": (C.c_int, [_P, C.c_int32, _DP]),
+    "sap_bench_gather": (C.c_int, [_P, C.c_int32, C.c_int32, _DP]),
+    "sap_matrix_set_context": (C.c_int, [_P, _P]),
     "sap_server_set_budget": (C.c_int, [_P, C.c_int64]),
     "sap_server_cold": (C.c_int, [_P, C.c_int32, _U8P]),
     "sap_server_set_slots": (C.c_int, [_P, C.c_int32, C.c_int32]),
@@ -627,10 +628,10 @@ class Bounds:
         _chk(lib().sap_bounds_download(self.h, v, _dp(lb), _dp(ub)), "sap_bounds_download")
         return lb, ub
 
-    def bench_gather(self, reps: int = 20) -> float:
+    def bench_gather(self, reps: int = 20, blocks: int = 0) -> float:
         """Mean time (ms) of the row sweep's bare memory traffic over this matrix (sap_bench_gather)."""
         ms = C.c_double(0.0)
-        _chk(lib().sap_bench_gather(self.h, reps, C.byref(ms)), "sap_bench_gather")
+        _chk(lib().sap_bench_gather(self.h, reps, blocks, C.byref(ms)), "sap_bench_gather")
         return ms.value
 
     def copy_from(self, src: "Bounds", dv: int = 0, sv: int = 0):

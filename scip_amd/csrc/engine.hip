@@ -700,13 +700,17 @@ __device__ __forceinline__ void candEasy(const CtxT<R>& C, const RowState& rs, d
    }
 }
 
+// recomputes of this workgroup (consdataGetReliableResidualActivity's path), added to the
+// status once at the end of the sweep: a counter in LDS keeps global atomics out of the sweep
+__shared__ unsigned int s_nrecomp;
+
 // consdataGetReliableResidualActivity (cons_linear.c:2580-2657): plain double sum in
 // position order over the row, skipping position skip
 template <bool R>
 __device__ __forceinline__ double reliableResidual(const CtxT<R>& C, int64_t s, int64_t n, int64_t skip, bool isminres)
 {
    const double2* bnd = C.V->bnd + (size_t)C.v * (size_t)C.M->ncols;
-   atomicAdd(&C.V->st->nrecomp, 1ULL);   // rare path: a counter on one address costs nothing
+   atomicAdd(&s_nrecomp, 1u);
    double r = 0.0;
    for( int64_t q = 0; q < n; ++q )
    {
@@ -1978,6 +1982,9 @@ __global__ __launch_bounds__(SAP_BLOCK, 4) void k_sweep(Mat M, Vec V, Params P, 
    if( st->done )
       return;
    const unsigned long long tstart = (unsigned long long)wall_clock64();
+   if( threadIdx.x == 0 )
+      s_nrecomp = 0;
+   __syncthreads();
    const int blk = (int)blockIdx.x;
    int kind = 0;
    while( kind < SAP_NKINDS - 1 && blk >= L.blockstart[kind + 1] )
@@ -2088,9 +2095,11 @@ __global__ __launch_bounds__(SAP_BLOCK, 4) void k_sweep(Mat M, Vec V, Params P, 
    // remaining queued candidates of this wavefront
    if( Q.count > 0 )
       drainQueue<R>(M, V, P, Q, Q.count, I.round);
-   blockStats(nnzacc, rowsacc, st->slot_nnz, st->slot_rows);
+   blockStats(nnzacc, rowsacc, st->slot_nnz, st->slot_rows);   // ends with a barrier
    if( threadIdx.x == 0 )
    {
+      if( s_nrecomp != 0 )
+         atomicAdd(&st->nrecomp, (unsigned long long)s_nrecomp);
       const int slot = (int)(blockIdx.x & (SAP_STAT_SLOTS - 1));
       atomicMin(&st->tslot0[slot], tstart);
       atomicMax(&st->tslot1[slot], (unsigned long long)wall_clock64());
@@ -3809,9 +3818,10 @@ int sap_context_free(sap_context* c)
 {
    if( c == NULL )
       return SAP_OKAY;
-   if( c->nmatrices > 0 )
+   const int64_t live = __atomic_load_n(&c->nmatrices, __ATOMIC_RELAXED);
+   if( live > 0 )
    {
-      fprintf(stderr, "[scip_amd] sap_context_free: %ld matrices still live; free them first\n", (long)c->nmatrices);
+      fprintf(stderr, "[scip_amd] sap_context_free: %ld matrices still live; free them first\n", (long)live);
       return SAP_INVALIDCALL;
    }
    (void)hipSetDevice(c->device);
@@ -4208,7 +4218,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
       HIPCHK(hipStreamSynchronize(s));
    }
    HIPCHK(hipStreamSynchronize(s));
-   ctx->nmatrices++;
+   __atomic_fetch_add(&ctx->nmatrices, 1, __ATOMIC_RELAXED);
    *out = m;
    return SAP_OKAY;
 }
@@ -4381,9 +4391,25 @@ int sap_matrix_free(sap_matrix* m)
       sap_bounds_free(m->single);
    if( m->rsingle )
       sap_bounds_free(m->rsingle);
-   m->ctx->nmatrices--;
+   __atomic_fetch_sub(&m->ctx->nmatrices, 1, __ATOMIC_RELAXED);
    matrixRelease(m);
    free(m);
+   return SAP_OKAY;
+}
+
+int sap_matrix_set_context(sap_matrix* m, sap_context* ctx)
+{
+   if( m == NULL || ctx == NULL || ctx->device != m->ctx->device )
+      return SAP_INVALIDDATA;
+   if( ctx == m->ctx )
+      return SAP_OKAY;
+   HIPCHK(hipSetDevice(ctx->device));
+   // the matrix's work on its old stream has ended (engine calls return with their work
+   // done; a slots step must have been waited for)
+   HIPCHK(hipStreamSynchronize(m->ctx->stream));
+   __atomic_fetch_sub(&m->ctx->nmatrices, 1, __ATOMIC_RELAXED);
+   __atomic_fetch_add(&ctx->nmatrices, 1, __ATOMIC_RELAXED);
+   m->ctx = ctx;
    return SAP_OKAY;
 }
 
@@ -6003,16 +6029,14 @@ int sap_slots_launch(sap_bounds* b, const sap_params* params, int32_t rounds)
    HIPCHK(hipEventRecord(b->evs[0], s));
    k_step_begin<<<1, SAP_STAT_SLOTS, 0, s>>>(b->V);
    HIPCHK(hipGetLastError());
-   if( !(P.flags & SAP_FLAG_HOSTLOOP) && !b->graphbroken && ensureGraphs(b, P) )
-      HIPCHK(hipGraphLaunch(b->graph[rounds == SAP_STEP_SHORT ? 0 : 1].exec, s));
-   else
-   {
-      const Mat M = toMat(m);
-      const Layout L = makeLayout(b);
-      for( int r = 0; r < rounds; ++r )
-         enqueueRound(b, M, P, L, s, NULL, NULL);
-      HIPCHK(hipGetLastError());
-   }
+   // the step's rounds are enqueued directly, not launched as a captured graph: the server
+   // steps many pools on several streams, and graphs saved only ~2 % of its saturated
+   // capacity (37.0-37.2k against 37.8-38.3k requests/s on the serving workload)
+   const Mat M = toMat(m);
+   const Layout L = makeLayout(b);
+   for( int r = 0; r < rounds; ++r )
+      enqueueRound(b, M, P, L, s, NULL, NULL);
+   HIPCHK(hipGetLastError());
    HIPCHK(hipMemcpyAsync(b->hstat, b->dstatus, sizeof(Status), hipMemcpyDeviceToHost, s));
    HIPCHK(hipMemcpyAsync(b->hpend, b->V.vpend, sizeof(int32_t) * (size_t)b->nvec, hipMemcpyDeviceToHost, s));
    HIPCHK(hipMemcpyAsync(b->hpend + b->nvec, b->V.vcutoff, sizeof(int32_t) * (size_t)b->nvec, hipMemcpyDeviceToHost,
@@ -6120,9 +6144,9 @@ int sap_slots_collect(sap_bounds* b, int32_t n, const int32_t* slots, int32_t* r
    return SAP_OKAY;
 }
 
-int sap_bench_gather(sap_bounds* b, int32_t reps, double* ms_per_pass)
+int sap_bench_gather(sap_bounds* b, int32_t reps, int32_t blocks, double* ms_per_pass)
 {
-   if( b == NULL || reps < 1 || ms_per_pass == NULL )
+   if( b == NULL || reps < 1 || blocks < 0 || ms_per_pass == NULL )
       return SAP_INVALIDDATA;
    sap_matrix* m = b->mat;
    sap_context* c = m->ctx;
@@ -6131,8 +6155,9 @@ int sap_bench_gather(sap_bounds* b, int32_t reps, double* ms_per_pass)
    double* sink = NULL;
    RCCHK(devAlloc(&sink, 1, NULL));
    const Mat M = toMat(m);
-   // as many workgroups as the sweep keeps resident (4 waves per SIMD)
-   const unsigned grid = gridFor((size_t)((m->nnz + 3) / 4), 256 * 4 * 4 / (SAP_BLOCK / SAP_WAVE));
+   // blocks 0: one lane per four nonzeros, no grid-stride loop (the most loads in flight)
+   const size_t lanes = (size_t)((m->nnz + 3) / 4);
+   const unsigned grid = blocks > 0 ? (unsigned)blocks : (unsigned)std::max<size_t>(1, (lanes + SAP_BLOCK - 1) / SAP_BLOCK);
    k_gather_floor<<<grid, SAP_BLOCK, 0, s>>>(M, b->V.bnd, m->nnz, sink);   // warm-up
    HIPCHK(hipEventRecord(c->ev[0], s));
    for( int32_t r = 0; r < reps; ++r )

@@ -3,6 +3,12 @@
 // pools over several HIP streams, and a real-time replay of an arrival trace (see
 // include/scip_amd_serve.h).
 //
+// Uploads run on a loader thread with a context (HIP stream) of its own, so that building a
+// problem's matrix, root fixpoint and slot pool never stalls the scheduling loop: the loop
+// reserves the problem's footprint (evicting idle problems), hands the problem to the loader,
+// keeps stepping the resident pools, and moves the finished matrix to its serving context
+// (sap_matrix_set_context) when it collects it.
+//
 // Host code only: every device operation goes through the engine's C ABI.
 #include <stdint.h>
 #include <stdio.h>
@@ -11,7 +17,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <deque>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -40,6 +48,14 @@ struct Problem
    std::vector<int32_t> freeslots;
    int32_t running = 0;
    bool inflight = false;
+   // upload by the loader thread: reserved and handed over, not yet collected
+   bool loading = false;
+   int loadrc = SAP_OKAY;
+   sap_params lparams;            // parameters of the root propagation
+   sap_matrix* lmat = NULL;       // what the loader built (on its own context)
+   sap_bounds* lroot = NULL;
+   sap_bounds* lpool = NULL;
+   int32_t lrootcut = 0;
 };
 
 struct Request
@@ -69,6 +85,15 @@ struct sap_server
    sap_serve_stats st;
    int32_t ndone = 0;
    std::vector<uint8_t> cold;   // per request of the last replay: waited for an upload
+   // the loader thread and its context; jobs and finished uploads are guarded by mu
+   sap_context* lctx = NULL;
+   std::thread loader;
+   std::mutex mu;
+   std::condition_variable cv;
+   std::deque<Problem*> jobs;
+   std::deque<Problem*> loaded;
+   bool stop = false;
+   int32_t nloading = 0;   // submitted and not yet collected (scheduling thread only)
 };
 
 namespace {
@@ -119,14 +144,75 @@ Problem* lruVictim(sap_server* s)
    return victim;
 }
 
-// 1: resident; 0: does not fit now (wait); < 0... SAP error code
-int makeResident(sap_server* s, Problem* p, const sap_params* params)
+// builds p's matrix, root fixpoint and slot pool on context c (the loader's); on failure
+// nothing is left allocated
+int buildProblem(Problem* p, sap_context* c)
 {
-   if( p->mat != NULL )
-      return 1;
-   const int64_t need = footprint(p, p->slots);
-   if( need > s->budget )
-      return SAP_NOMEMORY;
+   p->lmat = NULL;
+   p->lroot = NULL;
+   p->lpool = NULL;
+   int rc = sap_matrix_create(c, p->nrows, p->ncols, p->rowptr.data(), p->colidx.data(), p->vals.data(),
+      p->lhs.data(), p->rhs.data(), p->integral.data(), p->lb.data(), p->ub.data(), &p->lmat);
+   if( rc != SAP_OKAY )
+   {
+      p->lmat = NULL;
+      return rc;
+   }
+   rc = sap_bounds_create_mode(p->lmat, 1, SAP_MODE_DENSE, &p->lroot);
+   if( rc == SAP_OKAY )
+      rc = sap_bounds_upload(p->lroot, 0, p->lb.data(), p->ub.data());
+   sap_stats rst;
+   memset(&rst, 0, sizeof(rst));
+   if( rc == SAP_OKAY )
+   {
+      sap_params rp = p->lparams;
+      rp.flags = 0;
+      rc = sap_propagate_resident(p->lroot, &rp, &rst);
+   }
+   if( rc == SAP_OKAY )
+      rc = sap_bounds_create(p->lmat, p->slots, &p->lpool);
+   if( rc != SAP_OKAY )
+   {
+      if( p->lpool )
+         sap_bounds_free(p->lpool);
+      if( p->lroot )
+         sap_bounds_free(p->lroot);
+      sap_matrix_free(p->lmat);
+      p->lpool = NULL;
+      p->lroot = NULL;
+      p->lmat = NULL;
+      return rc;
+   }
+   p->lrootcut = rst.result == SAP_CUTOFF ? 1 : 0;
+   return SAP_OKAY;
+}
+
+// the loader thread: builds the problems handed to it, one after another
+void loaderMain(sap_server* s)
+{
+   for( ;; )
+   {
+      Problem* p = NULL;
+      {
+         std::unique_lock<std::mutex> lock(s->mu);
+         s->cv.wait(lock, [s] { return s->stop || !s->jobs.empty(); });
+         if( s->jobs.empty() )
+            return;   // stop requested and nothing left
+         p = s->jobs.front();
+         s->jobs.pop_front();
+      }
+      const int rc = buildProblem(p, s->lctx);
+      {
+         std::lock_guard<std::mutex> lock(s->mu);
+         p->loadrc = rc;
+         s->loaded.push_back(p);
+      }
+   }
+}
+
+// makes room for need bytes: 1 fits (reserved), 0 not now (the resident problems are busy)
+int reserve(sap_server* s, int64_t need)
+{
    while( s->resident + need > s->budget )
    {
       Problem* victim = lruVictim(s);
@@ -134,48 +220,120 @@ int makeResident(sap_server* s, Problem* p, const sap_params* params)
          return 0;
       evict(s, victim);
    }
-   sap_context* c = s->ctx[(size_t)p->stream];
-   int rc = sap_matrix_create(c, p->nrows, p->ncols, p->rowptr.data(), p->colidx.data(), p->vals.data(),
-      p->lhs.data(), p->rhs.data(), p->integral.data(), p->lb.data(), p->ub.data(), &p->mat);
-   if( rc != SAP_OKAY )
-      return rc;
-   rc = sap_bounds_create_mode(p->mat, 1, SAP_MODE_DENSE, &p->root);
+   s->resident += need;
+   s->st.peak_bytes = std::max(s->st.peak_bytes, s->resident);
+   return 1;
+}
+
+// the built problem takes its place: its matrix moves to the problem's serving context
+int handOver(sap_server* s, Problem* p)
+{
+   p->loading = false;
+   int rc = p->loadrc;
    if( rc == SAP_OKAY )
-      rc = sap_bounds_upload(p->root, 0, p->lb.data(), p->ub.data());
-   sap_stats rst;
-   memset(&rst, 0, sizeof(rst));
-   if( rc == SAP_OKAY )
-   {
-      sap_params rp = *params;
-      rp.flags = 0;
-      rc = sap_propagate_resident(p->root, &rp, &rst);
-   }
-   if( rc == SAP_OKAY )
-      rc = sap_bounds_create(p->mat, p->slots, &p->pool);
+      rc = sap_matrix_set_context(p->lmat, s->ctx[(size_t)p->stream]);
    if( rc != SAP_OKAY )
    {
+      if( p->lpool )
+         sap_bounds_free(p->lpool);
+      if( p->lroot )
+         sap_bounds_free(p->lroot);
+      if( p->lmat )
+         sap_matrix_free(p->lmat);
+      p->lpool = NULL;
+      p->lroot = NULL;
+      p->lmat = NULL;
+      s->resident -= p->bytes;
       p->bytes = 0;
-      if( p->pool )
-         sap_bounds_free(p->pool);
-      if( p->root )
-         sap_bounds_free(p->root);
-      sap_matrix_free(p->mat);
-      p->pool = NULL;
-      p->root = NULL;
-      p->mat = NULL;
       return rc;
    }
-   p->rootcut = rst.result == SAP_CUTOFF ? 1 : 0;
+   p->mat = p->lmat;
+   p->root = p->lroot;
+   p->pool = p->lpool;
+   p->rootcut = p->lrootcut;
+   p->lmat = NULL;
+   p->lroot = NULL;
+   p->lpool = NULL;
    p->slotreq.assign((size_t)p->slots, -1);
    p->freeslots.clear();
    for( int32_t k = p->slots - 1; k >= 0; --k )
       p->freeslots.push_back(k);
    p->running = 0;
-   p->bytes = need;
-   s->resident += need;
    s->st.uploads++;
-   s->st.peak_bytes = std::max(s->st.peak_bytes, s->resident);
-   return 1;
+   return SAP_OKAY;
+}
+
+// starts an upload of p on the loader thread: 0 submitted or waiting for room, < 0 error
+int submitLoad(sap_server* s, Problem* p, const sap_params* params)
+{
+   const int64_t need = footprint(p, p->slots);
+   if( need > s->budget )
+      return SAP_NOMEMORY;
+   if( !reserve(s, need) )
+      return 0;
+   p->bytes = need;
+   p->lparams = *params;
+   p->loading = true;
+   p->loadrc = SAP_OKAY;
+   s->nloading++;
+   {
+      std::lock_guard<std::mutex> lock(s->mu);
+      s->jobs.push_back(p);
+   }
+   s->cv.notify_one();
+   return 0;
+}
+
+// hands over every upload the loader has finished; the first error is returned
+int collectLoads(sap_server* s)
+{
+   std::deque<Problem*> fin;
+   {
+      std::lock_guard<std::mutex> lock(s->mu);
+      fin.swap(s->loaded);
+   }
+   int rc = SAP_OKAY;
+   for( Problem* p : fin )
+   {
+      s->nloading--;
+      const int r = handOver(s, p);
+      if( r != SAP_OKAY && rc == SAP_OKAY )
+         rc = r;
+   }
+   return rc;
+}
+
+// waits until the loader has finished every submitted upload and hands them over
+int drainLoads(sap_server* s)
+{
+   int rc = SAP_OKAY;
+   while( s->nloading > 0 )
+   {
+      const int r = collectLoads(s);
+      if( r != SAP_OKAY && rc == SAP_OKAY )
+         rc = r;
+      if( s->nloading > 0 )
+         std::this_thread::sleep_for(std::chrono::microseconds(50));
+   }
+   return rc;
+}
+
+// 1: resident (uploaded now if it was not); 0: does not fit now; < 0 SAP error code.  The
+// upload runs on this thread (the loader has no work at the times this is called).
+int makeResident(sap_server* s, Problem* p, const sap_params* params)
+{
+   if( p->mat != NULL )
+      return 1;
+   const int64_t need = footprint(p, p->slots);
+   if( need > s->budget )
+      return SAP_NOMEMORY;
+   if( !reserve(s, need) )
+      return 0;
+   p->bytes = need;
+   p->lparams = *params;
+   p->loadrc = buildProblem(p, s->lctx);
+   const int rc = handOver(s, p);
+   return rc == SAP_OKAY ? 1 : rc;
 }
 
 void finish(sap_server* s, Request& r, double t0, int32_t result, int32_t rounds)
@@ -315,7 +473,7 @@ int sap_server_create(int32_t device, int32_t nstreams, int64_t hbm_budget, int3
    s->device = device;
    s->budget = hbm_budget;
    s->slots = slots_per_pool;
-   for( int32_t k = 0; k < nstreams; ++k )
+   for( int32_t k = 0; k <= nstreams; ++k )
    {
       sap_context* c = NULL;
       const int rc = sap_context_create(device, &c);
@@ -323,11 +481,17 @@ int sap_server_create(int32_t device, int32_t nstreams, int64_t hbm_budget, int3
       {
          for( sap_context* d : s->ctx )
             sap_context_free(d);
+         if( s->lctx )
+            sap_context_free(s->lctx);
          delete s;
          return rc;
       }
-      s->ctx.push_back(c);
+      if( k < nstreams )
+         s->ctx.push_back(c);
+      else
+         s->lctx = c;   // the loader's own context
    }
+   s->loader = std::thread(loaderMain, s);
    *out = s;
    return SAP_OKAY;
 }
@@ -336,6 +500,14 @@ int sap_server_free(sap_server* s)
 {
    if( s == NULL )
       return SAP_OKAY;
+   (void)drainLoads(s);
+   {
+      std::lock_guard<std::mutex> lock(s->mu);
+      s->stop = true;
+   }
+   s->cv.notify_one();
+   if( s->loader.joinable() )
+      s->loader.join();
    for( Problem* p : s->probs )
    {
       if( p->mat )
@@ -344,6 +516,7 @@ int sap_server_free(sap_server* s)
    }
    for( sap_context* c : s->ctx )
       sap_context_free(c);
+   sap_context_free(s->lctx);
    delete s;
    return SAP_OKAY;
 }
@@ -381,7 +554,7 @@ int sap_server_set_slots(sap_server* s, int32_t id, int32_t slots)
    if( s == NULL || id < 0 || id >= (int32_t)s->probs.size() || slots < 1 )
       return SAP_INVALIDDATA;
    Problem* p = s->probs[(size_t)id];
-   if( p->running > 0 || !p->queue.empty() || p->inflight )
+   if( p->running > 0 || !p->queue.empty() || p->inflight || p->loading )
       return SAP_INVALIDCALL;
    if( p->mat != NULL && p->slots != slots )
    {
@@ -501,29 +674,33 @@ int sap_server_replay(sap_server* s, const sap_params* params, int32_t mode, int
       // between these checks, so no pool waits for the steps of the others
       while( s->ndone < nreq && rc == SAP_OKAY )
       {
+         // problems the loader has finished join their serving contexts
+         rc = collectLoads(s);
+         if( rc != SAP_OKAY )
+            break;
          const double now = nowSeconds() - t0;
          while( next < nreq && arrival_s[next] <= now )
          {
-            s->probs[(size_t)prob[next]]->queue.push_back(next);
+            Problem* p = s->probs[(size_t)prob[next]];
+            req[(size_t)next].cold = p->mat == NULL;   // it waits for an upload
+            p->queue.push_back(next);
             ++next;
          }
-         // admissions into pools between steps (problems with waiting requests)
+         // admissions into pools between steps (problems with waiting requests); a problem
+         // that is not resident is handed to the loader, and its requests wait for it
          for( Problem* p : s->probs )
          {
-            if( p->queue.empty() || p->inflight )
+            if( p->queue.empty() || p->inflight || p->loading )
                continue;
-            const bool wascold = p->mat == NULL;
-            const int r = makeResident(s, p, params);
-            if( r < 0 )
+            if( p->mat == NULL )
             {
-               rc = r;
-               break;
+               rc = submitLoad(s, p, params);
+               if( rc != SAP_OKAY && rc != 0 )
+                  break;
+               rc = SAP_OKAY;
+               continue;
             }
-            if( r == 1 && wascold )
-               for( const int32_t i : p->queue )
-                  req[(size_t)i].cold = true;
-            if( r == 1 )
-               rc = admit(s, p, req, ovptr, ovcol, ovlb, ovub, p->slots, t0);
+            rc = admit(s, p, req, ovptr, ovcol, ovlb, ovub, p->slots, t0);
             if( rc != SAP_OKAY )
                break;
          }
@@ -575,6 +752,10 @@ int sap_server_replay(sap_server* s, const sap_params* params, int32_t mode, int
          }
       }
    }
+   // uploads still running (an error ended the loop early) are waited for and kept
+   const int drc = drainLoads(s);
+   if( rc == SAP_OKAY )
+      rc = drc;
    const double t1 = nowSeconds();
    s->st.wall_s = t1 - t0 - (nreq > 0 ? arrival_s[0] : 0.0);
    s->cold.assign((size_t)nreq, 0);
