@@ -343,6 +343,18 @@ int sap_slots_collect(sap_bounds* bnd, int32_t n, const int32_t* slots, int32_t*
  * {minposinf, minneginf, minposhuge, minneghuge, maxposinf, maxneginf, maxposhuge,
  * maxneghuge}, maxactdelta as in consdataRecomputeMaxActivityDelta (cons_linear.c:1542).
  * Any output pointer may be NULL. */
+/* sap_propagate on bounds already in device memory, allocating nothing.  No counterpart in
+ * SCIP: it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  dlb and dub are device pointers to
+ * ncols doubles, the node's bounds, overwritten with the fixpoint unless the result is a
+ * cutoff.  The work arrays live in one static device array per GPU (about 61 MiB, held until
+ * the process ends), so the matrix may have at most 2^20 columns and 2^20 rows and one
+ * instance, and ranged-row propagation is refused (SAP_INVALIDDATA); rounds run from a host
+ * loop.  Calls on one GPU share that array: calls that overlap on one GPU (several threads or
+ * contexts) race.  The results equal sap_propagate's. */
+int sap_propagate_device_noalloc(sap_matrix* mat, const sap_params* params, double* dlb, double* dub,
+   sap_stats* stats);
+
 /* Measurement: the memory floor of the row sweep on this matrix -- its value and column-index
  * streams and the (lb, ub) gather of every nonzero from vector 0 of b, without the sweep's
  * arithmetic -- timed with HIP events over reps launches of blocks workgroups (0: one lane

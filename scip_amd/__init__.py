@@ -208,6 +208,8 @@ EXPORTS = {
     "sap_server_footprint": (C.c_int, [_P, C.c_int32, _I64P]),
     "sap_bench_gather": (C.c_int, [_P, C.c_int32, C.c_int32, _DP]),
     "sap_matrix_set_context": (C.c_int, [_P, _P]),
+    "sap_propagate_device_noalloc": (C.c_int, [_P, C.POINTER(sap_params), C.c_void_p, C.c_void_p,
+                                               C.POINTER(sap_stats)]),
     "sap_server_set_budget": (C.c_int, [_P, C.c_int64]),
     "sap_server_cold": (C.c_int, [_P, C.c_int32, _U8P]),
     "sap_server_set_slots": (C.c_int, [_P, C.c_int32, C.c_int32]),
@@ -453,6 +455,15 @@ class Matrix:
         p = params or default_params()
         _chk(lib().sap_propagate(self.h, C.byref(p), _dp(lb), _dp(ub), C.byref(st)), "sap_propagate")
         return lb, ub, st.as_dict()
+
+    def propagate_device_noalloc(self, dlb: int, dub: int, params: sap_params | None = None):
+        """sap_propagate_device_noalloc on caller-owned device arrays (addresses dlb, dub of
+        ncols doubles each, overwritten with the fixpoint unless cut off); returns the stats."""
+        st = sap_stats()
+        p = params or default_params()
+        _chk(lib().sap_propagate_device_noalloc(self.h, C.byref(p), C.c_void_p(dlb), C.c_void_p(dub), C.byref(st)),
+             "sap_propagate_device_noalloc")
+        return st.as_dict()
 
     def propagate_reasons(self, lb=None, ub=None, params: sap_params | None = None, histcap: int | None = None):
         """Propagate one bound vector with reasons: bounds, stats, the history of accepted

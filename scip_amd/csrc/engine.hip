@@ -3431,6 +3431,15 @@ __global__ __launch_bounds__(SAP_BLOCK) void k_row_activities(Mat M, Params P, c
 // ------------------------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------------------------
+// ---- scratch of the no-allocation entry point (sap_propagate_device_noalloc) ----
+// One dense single-vector bound set (bounds, candidate keys, round tags, row marks, redundancy
+// flags, per-unit counters and the status block) for matrices of up to SAP_NOALLOC_MAXCOLS
+// columns and SAP_NOALLOC_MAXROWS rows: 56 B per column, 5 B per row, 256-byte aligned pieces.
+#define SAP_NOALLOC_MAXCOLS ((int64_t)1 << 20)
+#define SAP_NOALLOC_MAXROWS ((int64_t)1 << 20)
+#define SAP_NOALLOC_BYTES ((size_t)SAP_NOALLOC_MAXCOLS * 56 + (size_t)SAP_NOALLOC_MAXROWS * 5 + ((size_t)1 << 16))
+static __device__ __align__(256) unsigned char g_noalloc_scratch[SAP_NOALLOC_BYTES];
+
 struct sap_context
 {
    int32_t device;
@@ -3442,6 +3451,7 @@ struct sap_context
                             // never begins on a stream with work of other bound sets in flight
    hipEvent_t ev[4];
    Status* hstatus;   // pinned
+   unsigned char* scratch;   // this device's address of the no-allocation scratch array
 };
 
 struct sap_matrix
@@ -3810,6 +3820,9 @@ int sap_context_create(int32_t device, sap_context** out)
    for( int i = 0; i < 4; ++i )
       HIPCHK(hipEventCreate(&c->ev[i]));
    HIPCHK(hipHostMalloc((void**)&c->hstatus, sizeof(Status), hipHostMallocDefault));
+   // the device address of the no-allocation entry point's scratch array on this device (each
+   // device holds its own copy; the symbol's name is not a device address in host code)
+   HIPCHK(hipGetSymbolAddress((void**)&c->scratch, HIP_SYMBOL(g_noalloc_scratch)));
    *out = c;
    return SAP_OKAY;
 }
@@ -6141,6 +6154,103 @@ int sap_slots_collect(sap_bounds* b, int32_t n, const int32_t* slots, int32_t* r
    RCCHK(collectSlots(b, n, slots, result, rounds, chgcap, chgptr, chgcol, chglb, chgub));
    for( int32_t i = 0; i < n; ++i )
       (*b->active)[(size_t)slots[i]] = 0;
+   return SAP_OKAY;
+}
+
+int sap_propagate_device_noalloc(sap_matrix* m, const sap_params* params, double* dlb, double* dub, sap_stats* stats)
+{
+   if( m == NULL || dlb == NULL || dub == NULL )
+      return SAP_INVALIDDATA;
+   RCCHK(checkParams(params));
+   if( m->nseg != 1 || m->ncols > SAP_NOALLOC_MAXCOLS || m->nrows > SAP_NOALLOC_MAXROWS )
+   {
+      fprintf(stderr, "[scip_amd] sap_propagate_device_noalloc: at most %lld columns and %lld rows of one instance "
+         "(got %lld x %lld, %d instances)\n", (long long)SAP_NOALLOC_MAXCOLS, (long long)SAP_NOALLOC_MAXROWS,
+         (long long)m->nrows, (long long)m->ncols, (int)m->nseg);
+      return SAP_INVALIDDATA;
+   }
+   if( (params->flags & SAP_FLAG_RANGEDROWS) && !(params->flags & SAP_FLAG_NOTIGHTEN) && m->nranged > 0 )
+   {
+      fprintf(stderr, "[scip_amd] sap_propagate_device_noalloc: ranged-row propagation needs per-nonzero "
+         "flags and is not available without allocation\n");
+      return SAP_INVALIDDATA;
+   }
+   sap_context* c = m->ctx;
+   HIPCHK(hipSetDevice(c->device));
+   hipStream_t s = c->stream;
+   const size_t nc = (size_t)m->ncols, nr = (size_t)m->nrows;
+   // a host-side bound set whose device arrays are pieces of this device's scratch array
+   sap_bounds b;
+   memset(&b, 0, sizeof(b));
+   b.mat = m;
+   b.nvec = 1;
+   b.graphbroken = 1;   // rounds from the host loop: capturing a graph would allocate
+   b.internal = 1;
+   Vec& V = b.V;
+   V.nvec = 1;
+   V.nseg = 1;
+   V.single = 1;
+   V.dense = 1;
+   V.hasforced = m->hasforced;
+   unsigned char* next = c->scratch;
+   auto take = [&next](size_t bytes) {
+      unsigned char* r = next;
+      next += (bytes + 255) & ~(size_t)255;
+      return (void*)r;
+   };
+   V.bnd = (double2*)take(16 * nc);
+   V.clb = (long long*)take(8 * nc);
+   V.cub = (long long*)take(8 * nc);
+   if( m->hasforced )
+   {
+      V.flb = (long long*)take(8 * nc);
+      V.fub = (long long*)take(8 * nc);
+   }
+   V.coltag = (int32_t*)take(4 * nc);
+   V.chgtag = (int32_t*)take(4 * nc);
+   V.rowmark = (int32_t*)take(4 * nr);
+   V.redundant = (uint8_t*)take(nr);
+   V.vcutoff = (int32_t*)take(4);
+   V.vrounds = (int32_t*)take(4);
+   V.vstart = (int32_t*)take(4);
+   V.vpend = (int32_t*)take(4);
+   V.vlogn = (int32_t*)take(4);
+   b.dstatus = (Status*)take(sizeof(Status));
+   V.st = b.dstatus;
+   if( (size_t)(next - c->scratch) > SAP_NOALLOC_BYTES )
+      return SAP_ERROR;   // cannot happen: the size covers the largest admitted matrix
+   // the array keeps the last call's contents: everything a call reads before writing is
+   // cleared with fill kernels (no memset nodes, so a caller may capture the call)
+   k_fill_i64<<<gridFor(nc), SAP_BLOCK, 0, s>>>(V.clb, KEY_NEG_INF, nc);
+   k_fill_i64<<<gridFor(nc), SAP_BLOCK, 0, s>>>(V.cub, KEY_POS_INF, nc);
+   if( m->hasforced )
+   {
+      k_fill_i64<<<gridFor(nc), SAP_BLOCK, 0, s>>>(V.flb, KEY_NEG_INF, nc);
+      k_fill_i64<<<gridFor(nc), SAP_BLOCK, 0, s>>>(V.fub, KEY_POS_INF, nc);
+   }
+   k_fill_i32<<<gridFor(nc), SAP_BLOCK, 0, s>>>(V.coltag, 0, nc);
+   k_fill_i32<<<gridFor(nc), SAP_BLOCK, 0, s>>>(V.chgtag, 0, nc);
+   k_fill_i32<<<gridFor(nr), SAP_BLOCK, 0, s>>>(V.rowmark, 0, nr);
+   k_fill_i32<<<1, SAP_BLOCK, 0, s>>>(V.vpend, 0, 1);
+   k_fill_i32<<<1, SAP_BLOCK, 0, s>>>(V.vstart, 0, 1);
+   k_fill_i32<<<1, SAP_BLOCK, 0, s>>>(V.vlogn, 0, 1);
+   k_fill_i32<<<gridFor(sizeof(Status) / 4), SAP_BLOCK, 0, s>>>((int32_t*)b.dstatus, 0, sizeof(Status) / 4);
+   k_interleave<<<gridFor(nc), SAP_BLOCK, 0, s>>>(V.bnd, dlb, dub, (int64_t)nc);
+   HIPCHK(hipGetLastError());
+   sap_stats local;
+   memset(&local, 0, sizeof(local));
+   const Params P = toParams(params);
+   RCCHK(beginCall(&b, 1, 1));   // generation 1: the redundancy flags are cleared too
+   RCCHK(runRoundsHost(&b, P, &local));
+   local.result = c->hstatus->cutoff ? SAP_CUTOFF : (c->hstatus->nchg > 0 ? SAP_REDUCEDDOM : SAP_DIDNOTFIND);
+   if( local.result != SAP_CUTOFF )
+   {
+      k_deinterleave<<<gridFor(nc), SAP_BLOCK, 0, s>>>(V.bnd, dlb, dub, (int64_t)nc);
+      HIPCHK(hipGetLastError());
+   }
+   HIPCHK(hipStreamSynchronize(s));
+   if( stats )
+      *stats = local;
    return SAP_OKAY;
 }
 
