@@ -70,6 +70,15 @@ struct Request
    bool cold = false;   // its problem was uploaded while it waited
 };
 
+// an evicted problem's device objects, freed by the loader thread (freeing waits for the
+// device, which the scheduling loop must not)
+struct Dead
+{
+   sap_matrix* mat;
+   sap_bounds* root;
+   sap_bounds* pool;
+};
+
 }   // namespace
 
 struct sap_server
@@ -92,6 +101,7 @@ struct sap_server
    std::condition_variable cv;
    std::deque<Problem*> jobs;
    std::deque<Problem*> loaded;
+   std::deque<Dead> frees;   // evicted problems' objects, freed by the loader
    bool stop = false;
    int32_t nloading = 0;   // submitted and not yet collected (scheduling thread only)
 };
@@ -117,20 +127,31 @@ double nowSeconds()
    return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
+void freeDead(const Dead& d)
+{
+   if( d.pool )
+      sap_bounds_free(d.pool);
+   if( d.root )
+      sap_bounds_free(d.root);
+   if( d.mat )
+      sap_matrix_free(d.mat);
+}
+
+// releases p's device objects: they go to the loader thread, which frees them
 void evict(sap_server* s, Problem* p)
 {
-   if( p->pool )
-      sap_bounds_free(p->pool);
-   if( p->root )
-      sap_bounds_free(p->root);
-   if( p->mat )
-      sap_matrix_free(p->mat);
+   Dead d = { p->mat, p->root, p->pool };
    p->pool = NULL;
    p->root = NULL;
    p->mat = NULL;
    s->resident -= p->bytes;
    p->bytes = 0;
    s->st.evictions++;
+   {
+      std::lock_guard<std::mutex> lock(s->mu);
+      s->frees.push_back(d);
+   }
+   s->cv.notify_one();
 }
 
 // the least recently used resident problem without running, queued or in-flight work
@@ -193,14 +214,24 @@ void loaderMain(sap_server* s)
    for( ;; )
    {
       Problem* p = NULL;
+      std::deque<Dead> dead;
       {
          std::unique_lock<std::mutex> lock(s->mu);
-         s->cv.wait(lock, [s] { return s->stop || !s->jobs.empty(); });
-         if( s->jobs.empty() )
+         s->cv.wait(lock, [s] { return s->stop || !s->jobs.empty() || !s->frees.empty(); });
+         dead.swap(s->frees);
+         if( s->jobs.empty() && dead.empty() )
             return;   // stop requested and nothing left
-         p = s->jobs.front();
-         s->jobs.pop_front();
+         if( !s->jobs.empty() )
+         {
+            p = s->jobs.front();
+            s->jobs.pop_front();
+         }
       }
+      // evicted problems first: their memory makes room for the next upload
+      for( const Dead& d : dead )
+         freeDead(d);
+      if( p == NULL )
+         continue;
       const int rc = buildProblem(p, s->lctx);
       {
          std::lock_guard<std::mutex> lock(s->mu);
@@ -507,11 +538,11 @@ int sap_server_free(sap_server* s)
    }
    s->cv.notify_one();
    if( s->loader.joinable() )
-      s->loader.join();
+      s->loader.join();   // it leaves after freeing every evicted problem
    for( Problem* p : s->probs )
    {
       if( p->mat )
-         evict(s, p);
+         freeDead(Dead{ p->mat, p->root, p->pool });
       delete p;
    }
    for( sap_context* c : s->ctx )

@@ -36,6 +36,7 @@
  * codes have SCIP_RETCODE values); nothing aborts.
  */
 #include <limits.h>
+#include <math.h>
 #include <string.h>
 
 #include "scip/scip.h"
@@ -87,6 +88,8 @@ struct SCIP_PropData
    double*               vals;
    int                   nnz;
    SCIP_Longint          nconflicts;         /**< conflict analyses started from device cutoffs */
+   sap_bounds*           diveset;            /**< dives: one node vector per dive */
+   int                   divevecs;           /**< vectors of diveset */
 };
 
 /*
@@ -208,6 +211,12 @@ SCIP_RETCODE freeMatrix(
    )
 {
    /* bound sets before their matrix */
+   if( propdata->diveset != NULL )
+   {
+      SCIP_CALL( engineCall(sap_bounds_free(propdata->diveset), "sap_bounds_free") );
+      propdata->diveset = NULL;
+      propdata->divevecs = 0;
+   }
    if( propdata->probeset != NULL )
    {
       SCIP_CALL( engineCall(sap_bounds_free(propdata->probeset), "sap_bounds_free") );
@@ -1223,6 +1232,266 @@ SCIP_RETCODE SCIPgpulinearStrongbranch(
    SCIPfreeBufferArray(scip, &ub);
    SCIPfreeBufferArray(scip, &lb);
    return SCIP_OKAY;
+}
+
+/** batched propagation dives (the fix-and-propagate loop of heur_fixandinfer.c:255-280 and of
+ *  SCIP's diving heuristics, SCIPfixVarProbing / SCIPpropagateProbing per step), many dives at
+ *  once on the device.  Dive k visits the candidates in its order order[k*ncands ...]; at each
+ *  step it fixes the first candidate still unfixed in its current bounds to fixvals[k*ncands +
+ *  c] (rounded to the nearest integer for an integral variable, then moved into the current
+ *  domain) and propagates; it ends on a cutoff, when no candidate is left unfixed or after
+ *  maxdepth fixings.  Every step of all live dives is one batched device call: each dive is a
+ *  node vector started from the node's propagated bounds with all of its fixings so far
+ *  (sap_propagate_nodes).  Outputs per dive: depth[k] fixings made, cutoff[k], and with
+ *  divelbs / diveubs (ndives x the matrix's columns, indexed by SCIPvarGetProbindex) its
+ *  final bounds. */
+SCIP_RETCODE SCIPgpulinearDive(
+   SCIP*                 scip,               /**< SCIP data structure */
+   int                   ndives,             /**< number of dives */
+   int                   ncands,             /**< number of candidates */
+   SCIP_VAR**            cands,              /**< candidate variables (columns of the matrix) */
+   const int*            order,              /**< ndives x ncands: the order in which dive k visits them */
+   const SCIP_Real*      fixvals,            /**< ndives x ncands: the value dive k fixes candidate c to */
+   int                   maxdepth,           /**< most fixings per dive (-1: no limit) */
+   int*                  depth,              /**< array of ndives: fixings made */
+   SCIP_Bool*            cutoff,             /**< array of ndives: the dive ended in a cutoff */
+   SCIP_Real*            divelbs,            /**< ndives x columns: final lower bounds, or NULL */
+   SCIP_Real*            diveubs             /**< ndives x columns: final upper bounds, or NULL */
+   )
+{
+   SCIP_PROP* prop = SCIPfindProp(scip, PROP_NAME);
+   SCIP_PROPDATA* propdata;
+   sap_params params;
+   sap_stats stats;
+   double* lb0;
+   double* ub0;
+   double* cur;        /* per dive: current lower bounds, then upper bounds (2 x nvars) */
+   int* pos;           /* per dive: next index into its order */
+   int* fixcol;        /* per dive: its fixings so far (ncands slots each) */
+   double* fixval;
+   SCIP_Bool* live;
+   int* vmap;          /* vector of the batched call -> dive */
+   int32_t* result;
+   int64_t* ovptr;
+   int32_t* ovcol;
+   double* ovval;
+   int64_t* chgptr;
+   int32_t* chgcol;
+   double* chglb;
+   double* chgub;
+   int64_t cap;
+   int nvars;
+   int nlive;
+   int rc = SAP_OKAY;
+   int j;
+   int k;
+   int q;
+
+   if( prop == NULL || ndives < 0 || ncands < 0 || (ndives > 0 && ncands > 0 && (cands == NULL || order == NULL
+            || fixvals == NULL)) || (ndives > 0 && (depth == NULL || cutoff == NULL))
+      || ((divelbs == NULL) != (diveubs == NULL)) )
+      return SCIP_INVALIDDATA;
+   propdata = SCIPpropGetData(prop);
+   if( propdata->mat == NULL )
+      return SCIP_INVALIDCALL;
+   nvars = propdata->nvars;
+   for( k = 0; k < ndives * ncands; ++k )
+      if( order[k] < 0 || order[k] >= ncands )
+         return SCIP_INVALIDDATA;
+   for( k = 0; k < ncands; ++k )
+   {
+      int col = SCIPvarGetProbindex(cands[k]);
+      if( col < 0 || col >= nvars || propdata->vars[col] != cands[k] )
+         return SCIP_INVALIDDATA;
+   }
+   for( k = 0; k < ndives; ++k )
+   {
+      depth[k] = 0;
+      cutoff[k] = FALSE;
+   }
+   if( ndives == 0 )
+      return SCIP_OKAY;
+
+   SCIP_CALL( SCIPallocBufferArray(scip, &lb0, nvars) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &ub0, nvars) );
+   for( j = 0; j < nvars; ++j )
+   {
+      lb0[j] = SCIPvarGetLbLocal(propdata->vars[j]);
+      ub0[j] = SCIPvarGetUbLocal(propdata->vars[j]);
+   }
+   SCIP_CALL( getEngineParams(scip, propdata, &params) );
+   if( propdata->probebase == NULL )
+   {
+      SCIP_CALL( engineCall(sap_bounds_create(propdata->mat, 1, &propdata->probebase), "sap_bounds_create") );
+   }
+   /* the node's bounds, propagated to their fixpoint: the base of every dive */
+   SCIP_CALL( engineCall(sap_bounds_upload(propdata->probebase, 0, lb0, ub0), "sap_bounds_upload") );
+   memset(&stats, 0, sizeof(stats));
+   SCIP_CALL( engineCall(sap_propagate_resident(propdata->probebase, &params, &stats), "sap_propagate_resident") );
+   if( stats.result == SAP_CUTOFF )
+   {
+      for( k = 0; k < ndives; ++k )
+         cutoff[k] = TRUE;
+      SCIPfreeBufferArray(scip, &ub0);
+      SCIPfreeBufferArray(scip, &lb0);
+      return SCIP_OKAY;
+   }
+   SCIP_CALL( engineCall(sap_bounds_download(propdata->probebase, 0, lb0, ub0), "sap_bounds_download") );
+   if( propdata->diveset != NULL && propdata->divevecs < ndives )
+   {
+      SCIP_CALL( engineCall(sap_bounds_free(propdata->diveset), "sap_bounds_free") );
+      propdata->diveset = NULL;
+   }
+   if( propdata->diveset == NULL )
+   {
+      SCIP_CALL( engineCall(sap_bounds_create(propdata->mat, ndives, &propdata->diveset), "sap_bounds_create") );
+      propdata->divevecs = ndives;
+   }
+
+   SCIP_CALL( SCIPallocBufferArray(scip, &cur, 2 * (size_t)ndives * (size_t)nvars) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &pos, ndives) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &fixcol, (size_t)ndives * (size_t)(ncands > 0 ? ncands : 1)) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &fixval, (size_t)ndives * (size_t)(ncands > 0 ? ncands : 1)) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &live, ndives) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &vmap, ndives) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &result, ndives) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &ovptr, ndives + 1) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &ovcol, (size_t)ndives * (size_t)(ncands > 0 ? ncands : 1)) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &ovval, (size_t)ndives * (size_t)(ncands > 0 ? ncands : 1)) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &chgptr, ndives + 1) );
+   cap = 64 * (int64_t)ndives + 1024;
+   SCIP_CALL( SCIPallocBufferArray(scip, &chgcol, cap) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &chglb, cap) );
+   SCIP_CALL( SCIPallocBufferArray(scip, &chgub, cap) );
+   for( k = 0; k < ndives; ++k )
+   {
+      memcpy(cur + 2 * (size_t)k * (size_t)nvars, lb0, sizeof(double) * (size_t)nvars);
+      memcpy(cur + (2 * (size_t)k + 1) * (size_t)nvars, ub0, sizeof(double) * (size_t)nvars);
+      pos[k] = 0;
+      live[k] = TRUE;
+   }
+   params.flags |= SAP_FLAG_BASE_IS_FIXPOINT;
+
+   for( ;; )
+   {
+      /* every live dive fixes its next unfixed candidate; its node is all of its fixings */
+      nlive = 0;
+      ovptr[0] = 0;
+      for( k = 0; k < ndives; ++k )
+      {
+         const double* klb = cur + 2 * (size_t)k * (size_t)nvars;
+         const double* kub = klb + nvars;
+         int c = -1;
+
+         if( !live[k] )
+            continue;
+         if( maxdepth >= 0 && depth[k] >= maxdepth )
+         {
+            live[k] = FALSE;
+            continue;
+         }
+         while( pos[k] < ncands && c < 0 )
+         {
+            int cand = order[(size_t)k * (size_t)ncands + (size_t)pos[k]];
+            int col = SCIPvarGetProbindex(cands[cand]);
+            ++pos[k];
+            if( klb[col] < kub[col] )
+               c = cand;
+         }
+         if( c < 0 )
+         {
+            live[k] = FALSE;
+            continue;
+         }
+         {
+            int col = SCIPvarGetProbindex(cands[c]);
+            double v = fixvals[(size_t)k * (size_t)ncands + (size_t)c];
+            if( SCIPvarIsIntegral(cands[c]) )
+               v = floor(v + 0.5);
+            v = v < klb[col] ? klb[col] : (v > kub[col] ? kub[col] : v);
+            fixcol[(size_t)k * (size_t)ncands + (size_t)depth[k]] = col;
+            fixval[(size_t)k * (size_t)ncands + (size_t)depth[k]] = v;
+            ++depth[k];
+         }
+         for( q = 0; q < depth[k]; ++q )
+         {
+            ovcol[ovptr[nlive] + q] = (int32_t)fixcol[(size_t)k * (size_t)ncands + (size_t)q];
+            ovval[ovptr[nlive] + q] = fixval[(size_t)k * (size_t)ncands + (size_t)q];
+         }
+         ovptr[nlive + 1] = ovptr[nlive] + depth[k];
+         vmap[nlive] = k;
+         ++nlive;
+      }
+      if( nlive == 0 )
+         break;
+      for( ;; )
+      {
+         memset(&stats, 0, sizeof(stats));
+         rc = sap_propagate_nodes(propdata->diveset, &params, nlive, propdata->probebase, 0, ovptr, ovcol, ovval,
+            ovval, result, NULL, cap, chgptr, chgcol, chglb, chgub, &stats);
+         if( rc != SAP_NOMEMORY || chgptr[nlive] <= cap )
+            break;
+         cap = chgptr[nlive];   /* room for every change: again (same results) */
+         SCIPfreeBufferArray(scip, &chgub);
+         SCIPfreeBufferArray(scip, &chglb);
+         SCIPfreeBufferArray(scip, &chgcol);
+         SCIP_CALL( SCIPallocBufferArray(scip, &chgcol, cap) );
+         SCIP_CALL( SCIPallocBufferArray(scip, &chglb, cap) );
+         SCIP_CALL( SCIPallocBufferArray(scip, &chgub, cap) );
+      }
+      if( rc != SAP_OKAY )
+         break;
+      for( j = 0; j < nlive; ++j )
+      {
+         double* klb;
+         double* kub;
+         int64_t e;
+         k = vmap[j];
+         if( result[j] == SAP_CUTOFF )
+         {
+            cutoff[k] = TRUE;
+            live[k] = FALSE;
+            continue;
+         }
+         /* the dive's bounds: the base, its fixings, then the propagated changes */
+         klb = cur + 2 * (size_t)k * (size_t)nvars;
+         kub = klb + nvars;
+         memcpy(klb, lb0, sizeof(double) * (size_t)nvars);
+         memcpy(kub, ub0, sizeof(double) * (size_t)nvars);
+         for( e = ovptr[j]; e < ovptr[j + 1]; ++e )
+            klb[ovcol[e]] = kub[ovcol[e]] = ovval[e];
+         for( e = chgptr[j]; e < chgptr[j + 1]; ++e )
+         {
+            klb[chgcol[e]] = chglb[e];
+            kub[chgcol[e]] = chgub[e];
+         }
+      }
+   }
+   if( rc == SAP_OKAY && divelbs != NULL )
+      for( k = 0; k < ndives; ++k )
+      {
+         memcpy(divelbs + (size_t)k * (size_t)nvars, cur + 2 * (size_t)k * (size_t)nvars,
+            sizeof(double) * (size_t)nvars);
+         memcpy(diveubs + (size_t)k * (size_t)nvars, cur + (2 * (size_t)k + 1) * (size_t)nvars,
+            sizeof(double) * (size_t)nvars);
+      }
+   SCIPfreeBufferArray(scip, &chgub);
+   SCIPfreeBufferArray(scip, &chglb);
+   SCIPfreeBufferArray(scip, &chgcol);
+   SCIPfreeBufferArray(scip, &chgptr);
+   SCIPfreeBufferArray(scip, &ovval);
+   SCIPfreeBufferArray(scip, &ovcol);
+   SCIPfreeBufferArray(scip, &ovptr);
+   SCIPfreeBufferArray(scip, &result);
+   SCIPfreeBufferArray(scip, &vmap);
+   SCIPfreeBufferArray(scip, &live);
+   SCIPfreeBufferArray(scip, &fixval);
+   SCIPfreeBufferArray(scip, &fixcol);
+   SCIPfreeBufferArray(scip, &pos);
+   SCIPfreeBufferArray(scip, &cur);
+   SCIPfreeBufferArray(scip, &ub0);
+   SCIPfreeBufferArray(scip, &lb0);
+   return engineCall(rc, "sap_propagate_nodes");
 }
 
 /** creates the gpulinear propagator and includes it in SCIP */

@@ -42,6 +42,29 @@ SCIP_RETCODE SCIPgpulinearStrongbranch(
    SCIP_Bool*            cutoff              /**< pointer to store whether the node is infeasible */
    );
 
+/** batched propagation dives (the fix-and-propagate loop of heur_fixandinfer.c:255-280 and of
+ *  SCIP's diving heuristics, SCIPfixVarProbing / SCIPpropagateProbing per step): ndives dives
+ *  from the node's propagated bounds advance together, one batched device call per step; dive k
+ *  fixes, in its order order[k*ncands ...], the next candidate still unfixed to
+ *  fixvals[k*ncands + c] (nearest integer for an integral variable, moved into the current
+ *  domain) and propagates, until a cutoff, no unfixed candidate or maxdepth fixings (-1: no
+ *  limit); per dive the fixings made, whether it ended in a cutoff and (divelbs / diveubs, or
+ *  NULL) its final bounds, indexed by SCIPvarGetProbindex */
+SCIP_EXPORT
+SCIP_RETCODE SCIPgpulinearDive(
+   SCIP*                 scip,               /**< SCIP data structure */
+   int                   ndives,             /**< number of dives */
+   int                   ncands,             /**< number of candidates */
+   SCIP_VAR**            cands,              /**< candidate variables */
+   const int*            order,              /**< ndives x ncands: the order in which dive k visits them */
+   const SCIP_Real*      fixvals,            /**< ndives x ncands: the value dive k fixes candidate c to */
+   int                   maxdepth,           /**< most fixings per dive (-1: no limit) */
+   int*                  depth,              /**< array of ndives: fixings made */
+   SCIP_Bool*            cutoff,             /**< array of ndives: the dive ended in a cutoff */
+   SCIP_Real*            divelbs,            /**< ndives x active variables: final lower bounds, or NULL */
+   SCIP_Real*            diveubs             /**< ndives x active variables: final upper bounds, or NULL */
+   );
+
 #ifdef __cplusplus
 }
 #endif

@@ -766,6 +766,27 @@ SCIP_EXPORT int th_strongbranch(SCIP* s, int ncands, const int* cand, const doub
    return rc;
 }
 
+/* batched dives on the device (SCIPgpulinearDive) over candidate columns cand, at the current
+ * local bounds */
+SCIP_EXPORT int th_dive(SCIP* s, int ndives, int ncands, const int* cand, const int* order, const double* fixvals,
+   int maxdepth, int* depth, int* cutoff, double* divelbs, double* diveubs)
+{
+   SCIP_VAR** vars = (SCIP_VAR**)malloc(sizeof(SCIP_VAR*) * (size_t)(ncands > 0 ? ncands : 1));
+   SCIP_Bool* cut = (SCIP_Bool*)malloc(sizeof(SCIP_Bool) * (size_t)(ndives > 0 ? ndives : 1));
+   SCIP_RETCODE rc;
+   int c;
+   if( vars == NULL || cut == NULL )
+      return SCIP_NOMEMORY;
+   for( c = 0; c < ncands; ++c )
+      vars[c] = s->vars[cand[c]];
+   rc = SCIPgpulinearDive(s, ndives, ncands, vars, order, fixvals, maxdepth, depth, cut, divelbs, diveubs);
+   for( c = 0; c < ndives; ++c )
+      cutoff[c] = (int)cut[c];
+   free(vars);
+   free(cut);
+   return rc;
+}
+
 /* the recorded local changes; with non-NULL arrays they are copied out */
 SCIP_EXPORT int th_get_changes(SCIP* s, int* var, int* bound, int* inferinfo, double* oldval, double* newval)
 {

@@ -57,6 +57,8 @@ def lib():
         IPn = _P(np.int32)
         L.th_strongbranch.argtypes = [C.c_void_p, C.c_int, IPn, _P(np.float64), IPn, IPn, _P(np.float64),
                                       _P(np.float64), C.POINTER(C.c_int)]
+        L.th_dive.argtypes = [C.c_void_p, C.c_int, C.c_int, IPn, IPn, _P(np.float64), C.c_int, IPn, IPn,
+                              _P(np.float64), _P(np.float64)]
         L.th_resprop_change.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
         L.th_get_changes.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         L.th_get_conflict.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
@@ -103,6 +105,18 @@ class Host:
         rc = lib().th_strongbranch(self.h, n, np.ascontiguousarray(cands, dtype=np.int32),
                                    np.ascontiguousarray(solvals, dtype=np.float64), di, ui, nl, nu, C.byref(cut))
         return di[:n].astype(bool), ui[:n].astype(bool), nl, nu, bool(cut.value), rc
+
+    def dive(self, cands, order, fixvals, maxdepth=-1):
+        """SCIPgpulinearDive at the current local bounds: (depth, cutoff, divelbs, diveubs, rc)."""
+        order = np.ascontiguousarray(order, dtype=np.int32)
+        nd = order.shape[0]
+        depth = np.zeros(max(nd, 1), dtype=np.int32)
+        cut = np.zeros(max(nd, 1), dtype=np.int32)
+        dl = np.zeros(max(nd, 1) * self.n)
+        du = np.zeros(max(nd, 1) * self.n)
+        rc = lib().th_dive(self.h, nd, len(cands), np.ascontiguousarray(cands, dtype=np.int32), order.ravel(),
+                           np.ascontiguousarray(fixvals, dtype=np.float64).ravel(), maxdepth, depth, cut, dl, du)
+        return depth[:nd], cut[:nd].astype(bool), dl.reshape(-1, self.n)[:nd], du.reshape(-1, self.n)[:nd], rc
 
     def propexec(self):
         r = C.c_int(0)

@@ -454,3 +454,93 @@ def test_strongbranch_children_match_the_oracle(gpu_ctx, seed):
         # the deductions are real: some candidate has an infeasible child or a tighter union
         assert edi.any() or eui.any() or (enl > rl).any() or (enu < ru).any()
     assert h.close() == ph.SCIP_OKAY
+
+
+def _dive_expectation(inst, lb, ub, cands, order, fixvals, maxdepth):
+    """Each dive emulated with the oracle's Jacobi twin: the node propagated, then every step
+    starts from it with all of the dive's fixings so far (only their rows marked first)."""
+    om = orc.OMatrix.from_instance(inst)
+    rl, ru, rr = orc.propagate_jacobi(om, lb, ub)
+    nd = order.shape[0]
+    if rr["cutoff"]:
+        return np.zeros(nd, dtype=int), np.ones(nd, dtype=bool), None, None
+    depth = np.zeros(nd, dtype=int)
+    cut = np.zeros(nd, dtype=bool)
+    out_l = np.tile(rl, (nd, 1))
+    out_u = np.tile(ru, (nd, 1))
+    for k in range(nd):
+        cl, cu = rl.copy(), ru.copy()
+        fix = []
+        pos = 0
+        while maxdepth < 0 or len(fix) < maxdepth:
+            c = None
+            while pos < len(cands) and c is None:
+                cand = order[k, pos]
+                pos += 1
+                if cl[cands[cand]] < cu[cands[cand]]:
+                    c = cand
+            if c is None:
+                break
+            col = cands[c]
+            v = fixvals[k, c]
+            if inst.integral[col]:
+                v = np.floor(v + 0.5)
+            v = min(max(v, cl[col]), cu[col])
+            fix.append((col, v))
+            sl, su = rl.copy(), ru.copy()
+            for fc, fv in fix:
+                sl[fc] = su[fc] = fv
+            nl, nu, r = orc.propagate_jacobi_marked(om, sl, su, orc.rows_of_columns(om, [f[0] for f in fix]))
+            if r["cutoff"]:
+                cut[k] = True
+                break
+            cl, cu = nl, nu
+        depth[k] = len(fix)
+        out_l[k], out_u[k] = cl, cu
+    return depth, cut, out_l, out_u
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["planted_binary", "covering", "integer"])
+def test_batched_dives_match_the_oracle(gpu_ctx, kind):
+    # fix-and-propagate dives (heur_fixandinfer.c) advanced together on the device, every dive
+    # against the oracle's propagation of its own fixings: on a system with a planted 0/1 point
+    # the dives that fix towards that point fix every candidate and the others mostly end in
+    # cutoffs, on covering/packing rows with random values every dive ends in a cutoff, on
+    # general integers they fix a few and propagation does the rest
+    rng = np.random.default_rng({"planted_binary": 7100, "covering": 7300, "integer": 7200}[kind])
+    if kind == "planted_binary":
+        inst, xs = random_binary_instance(rng, 500, 300, 12, point=True)
+    elif kind == "covering":
+        inst = sa.gen_covering_packing(300, 200, 10, seed=11)
+    else:
+        from tests.test_oracle_fixpoint import integer_instance
+        inst, _ = integer_instance(rng, nrows=300, ncols=200, maxlen=8, coefrange=20, boundrange=0, ubmax=6,
+                                   planted=True)
+    cands = np.nonzero(inst.integral)[0][:80].astype(np.int32)
+    nd = 24
+    order = np.array([rng.permutation(len(cands)) for _ in range(nd)], dtype=np.int32)
+    fixvals = inst.lb[cands] + (inst.ub[cands] - inst.lb[cands]) * rng.uniform(0, 1, size=(nd, len(cands)))
+    if kind == "planted_binary":
+        fixvals[::2] = xs[cands]
+    edepth, ecut, el, eu = _dive_expectation(inst, inst.lb, inst.ub, cands, order, fixvals, -1)
+    if kind == "covering":
+        assert ecut.all()
+    elif kind == "integer":
+        assert not ecut.any()
+    else:
+        assert not ecut[::2].any() and ecut[1::2].any()
+    h = ph.Host(inst)
+    assert h.call("th_initsol") == ph.SCIP_OKAY
+    depth, cut, dl, du, rc = h.dive(cands, order, fixvals)
+    assert rc == ph.SCIP_OKAY
+    assert np.array_equal(depth, edepth) and np.array_equal(cut, ecut)
+    for k in range(nd):
+        if not ecut[k]:
+            assert np.array_equal(dl[k], el[k]) and np.array_equal(du[k], eu[k]), k
+            assert np.all(dl[k][cands] == du[k][cands])   # every candidate ended fixed
+    # a depth limit stops every dive there
+    depth2, cut2, _, _, rc = h.dive(cands, order, fixvals, maxdepth=3)
+    assert rc == ph.SCIP_OKAY and depth2.max() <= 3
+    assert np.array_equal(depth2, np.minimum(edepth, 3)) or ecut.any()
+    assert h.close() == ph.SCIP_OKAY

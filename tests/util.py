@@ -44,9 +44,10 @@ def random_integer_instance(rng, nrows, ncols, maxlen, coefrange=5, boundrange=3
     return make_instance(rows, lb, ub, [1] * ncols)
 
 
-def random_binary_instance(rng, nrows, ncols, maxlen, coefmax=20, hidden=True):
+def random_binary_instance(rng, nrows, ncols, maxlen, coefmax=20, hidden=True, point=False):
     """Pure-binary rows with random sides tight enough to propagate; with hidden=True every
-    row is satisfied by a planted 0/1 point, so the system is feasible."""
+    row is satisfied by a planted 0/1 point, so the system is feasible.  point=True returns
+    (instance, planted point)."""
     rows = []
     xs = rng.integers(0, 2, size=ncols).astype(float)
     for _ in range(nrows):
@@ -66,7 +67,8 @@ def random_binary_instance(rng, nrows, ncols, maxlen, coefmax=20, hidden=True):
             rows.append(([(int(c), float(a)) for c, a in zip(cols, coefs)], -INF, side))
         else:
             rows.append(([(int(c), float(a)) for c, a in zip(cols, coefs)], side, INF))
-    return make_instance(rows, [0.0] * ncols, [1.0] * ncols, [1] * ncols)
+    inst = make_instance(rows, [0.0] * ncols, [1.0] * ncols, [1] * ncols)
+    return (inst, xs) if point else inst
 
 
 def random_fixings(rng, ncols, nvec, maxfix):
