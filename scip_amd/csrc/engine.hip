@@ -3361,6 +3361,34 @@ __global__ void k_fill_i32(int32_t* p, int32_t val, size_t n)
       p[i] = val;
 }
 
+// word copy between device memory and mapped pinned host memory.  The batched calls move
+// their small lists this way, as a kernel on the call's own stream, instead of through
+// hipMemcpyAsync, whose copies of pinned memory measured slower for these sizes.
+__global__ void k_copy_words(uint32_t* dst, const uint32_t* src, size_t n)
+{
+   for( size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x )
+      dst[i] = src[i];
+}
+
+__global__ void k_iota_i32(int32_t* p, size_t n)
+{
+   for( size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x )
+      p[i] = (int32_t)i;
+}
+
+// the per-vector outcome of a call on vectors 0..n-1 of a one-instance bound set, written
+// packed into mapped host memory: change-log length, cutoff flag, last round and start round
+__global__ void k_pack_meta(Vec V, int32_t n, int32_t* out)
+{
+   for( int32_t v = blockIdx.x * (int32_t)blockDim.x + (int32_t)threadIdx.x; v < n; v += (int32_t)(gridDim.x * blockDim.x) )
+   {
+      out[v] = V.vlogn[v];
+      out[(size_t)n + v] = V.vcutoff[v];
+      out[2 * (size_t)n + v] = V.vrounds[v];
+      out[3 * (size_t)n + v] = V.vstart[v];
+   }
+}
+
 __global__ void k_interleave(double2* dst, const double* lb, const double* ub, int64_t n)
 {
    for( int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x )
@@ -3562,6 +3590,18 @@ struct sap_bounds
    std::vector<const sap_bounds*>* srcb;
    std::vector<int32_t>* srcv;
    std::vector<int64_t>* srcgen;
+   // host side of the batched calls' small transfers: pinned memory, so that no copy is a
+   // synchronous pageable one (uploads are handed out in order, see upload(); downloads land
+   // in hdn, read after the stream is synchronised)
+   int32_t* ident;       // nvec: 0, 1, ..., nvec-1 (the slot list of a one-shot batch)
+   char* hup;            // pinned, mapped and coherent: uploads are staged here ...
+   char* hupd;           // ... and read by k_copy_words through this device address
+   size_t hupcap, hupused;
+   char* hdn;            // pinned landing area, written by k_copy_words through hdnd
+   char* hdnd;
+   size_t hdncap;
+   // vectors 0..logmax-1 may hold change-log bits that no collection has cleared yet
+   int32_t logmax;
 };
 
 // the fields of the vector state that one call narrows (the vectors it works on, the
@@ -4495,6 +4535,15 @@ static void boundsRelease(sap_bounds* b)
       (void)hipHostFree(b->hpend);
    b->hstat = NULL;
    b->hpend = NULL;
+   (void)hipFree(b->ident);
+   b->ident = NULL;
+   if( b->hup )
+      (void)hipHostFree(b->hup);
+   if( b->hdn )
+      (void)hipHostFree(b->hdn);
+   b->hup = b->hupd = NULL;
+   b->hdn = b->hdnd = NULL;
+   b->hupcap = b->hupused = b->hdncap = 0;
    for( int e = 0; e < 2; ++e )
       if( b->evs[e] != NULL )
          (void)hipEventDestroy(b->evs[e]);
@@ -4582,6 +4631,7 @@ int sap_bounds_create_mode(sap_matrix* m, int32_t nvec, int32_t mode, sap_bounds
    BALLOC(b->slotidx, nvec);
    BALLOC(b->slotidx2, nvec);
    BALLOC(b->outoff, (size_t)nvec + 1);
+   BALLOC(b->ident, nvec);
    if( !V.dense )
    {
       for( int p = 0; p < 2; ++p )
@@ -4614,6 +4664,7 @@ int sap_bounds_create_mode(sap_matrix* m, int32_t nvec, int32_t mode, sap_bounds
    k_fill_i32<<<gridFor((size_t)nvec), SAP_BLOCK, 0, s>>>(V.vpend, 0, (size_t)nvec);
    k_fill_i32<<<gridFor((size_t)nvec), SAP_BLOCK, 0, s>>>(V.vstart, 0, (size_t)nvec);
    k_fill_i32<<<gridFor((size_t)nvec), SAP_BLOCK, 0, s>>>(V.vlogn, 0, (size_t)nvec);
+   k_iota_i32<<<gridFor((size_t)nvec), SAP_BLOCK, 0, s>>>(b->ident, (size_t)nvec);
    HIPCHK(hipGetLastError());
    HIPCHK(hipMemsetAsync(b->dstatus, 0, sizeof(Status), s));
    HIPCHK(hipStreamSynchronize(s));
@@ -5299,6 +5350,120 @@ static int ensureLog(sap_bounds* b)
 }
 
 // validate overlays and copy them to the persistent device staging
+// pinned host memory that the device reads and writes directly (mapped, coherent), with its
+// device address
+static int pinnedArea(size_t bytes, char** h, char** d)
+{
+   *h = NULL;
+   *d = NULL;
+   HIPCHK(hipHostMalloc((void**)h, bytes, hipHostMallocMapped | hipHostMallocCoherent));
+   const hipError_t e = hipHostGetDevicePointer((void**)d, *h, 0);
+   if( e != hipSuccess || *d == NULL )
+   {
+      (void)hipHostFree(*h);
+      *h = NULL;
+      *d = NULL;
+      fprintf(stderr, "[scip_amd] hipHostGetDevicePointer failed: %s\n", hipGetErrorString(e));
+      return SAP_ERROR;
+   }
+   return SAP_OKAY;
+}
+
+static int copyWords(hipStream_t s, void* dst, const void* src, size_t bytes)
+{
+   if( bytes % 4 != 0 )
+      return SAP_ERROR;
+   const size_t n = bytes / 4;
+   k_copy_words<<<gridFor(n), SAP_BLOCK, 0, s>>>((uint32_t*)dst, (const uint32_t*)src, n);
+   HIPCHK(hipGetLastError());
+   return SAP_OKAY;
+}
+
+// A host-to-device copy for the batched calls: the bytes go to the bound set's pinned upload
+// area first, and a copy kernel on the call's stream moves them, so nothing waits (a pageable
+// source makes every copy synchronous) and the caller's array may be reused at once.  Pieces
+// of the area are handed out in order; when it is full, the stream is drained, after which
+// every earlier piece has been read, and the area starts over (grown if one piece needs more).
+static int upload(sap_bounds* b, void* ddst, const void* hsrc, size_t bytes)
+{
+   if( bytes == 0 )
+      return SAP_OKAY;
+   hipStream_t s = b->mat->ctx->stream;
+   const size_t need = (bytes + 255) & ~(size_t)255;
+   if( b->hupused + need > b->hupcap )
+   {
+      HIPCHK(hipStreamSynchronize(s));
+      b->hupused = 0;
+      if( need > b->hupcap )
+      {
+         const size_t cap = std::max(std::max(need, 2 * b->hupcap), (size_t)1 << 16);
+         if( b->hup != NULL )
+            (void)hipHostFree(b->hup);
+         b->hup = NULL;
+         b->hupd = NULL;
+         b->hupcap = 0;
+         RCCHK(pinnedArea(cap, &b->hup, &b->hupd));
+         b->hupcap = cap;
+      }
+   }
+   memcpy(b->hup + b->hupused, hsrc, bytes);
+   RCCHK(copyWords(s, ddst, b->hupd + b->hupused, bytes));
+   b->hupused += need;
+   return SAP_OKAY;
+}
+
+// pinned landing area of at least `bytes` for device-to-host copies of the batched calls
+// (download()); the caller synchronises the stream before reading it and has read it before
+// the next call here
+static int downloadArea(sap_bounds* b, size_t bytes, char** out)
+{
+   if( bytes > b->hdncap )
+   {
+      HIPCHK(hipStreamSynchronize(b->mat->ctx->stream));
+      const size_t cap = std::max(std::max(bytes, 2 * b->hdncap), (size_t)1 << 16);
+      if( b->hdn != NULL )
+         (void)hipHostFree(b->hdn);
+      b->hdn = NULL;
+      b->hdnd = NULL;
+      b->hdncap = 0;
+      RCCHK(pinnedArea(cap, &b->hdn, &b->hdnd));
+      b->hdncap = cap;
+   }
+   *out = b->hdn;
+   return SAP_OKAY;
+}
+
+// device-to-host copy into the landing area at h (inside b->hdn), by a copy kernel
+static int download(sap_bounds* b, char* h, const void* dsrc, size_t bytes)
+{
+   if( bytes == 0 )
+      return SAP_OKAY;
+   if( h < b->hdn || h + bytes > b->hdn + b->hdncap )
+      return SAP_ERROR;
+   return copyWords(b->mat->ctx->stream, b->hdnd + (h - b->hdn), dsrc, bytes);
+}
+
+// change-log length, cutoff flag, last round and start round of vectors 0..n-1 (one instance),
+// in one download
+static int fetchMeta(sap_bounds* b, int32_t n, std::vector<int32_t>& logn, std::vector<int32_t>& cut,
+   std::vector<int32_t>* vr, std::vector<int32_t>* vs)
+{
+   hipStream_t s = b->mat->ctx->stream;
+   char* h = NULL;
+   RCCHK(downloadArea(b, 16 * (size_t)n, &h));
+   k_pack_meta<<<gridFor((size_t)n), SAP_BLOCK, 0, s>>>(b->V, n, (int32_t*)b->hdnd);
+   HIPCHK(hipGetLastError());
+   HIPCHK(hipStreamSynchronize(s));
+   const int32_t* p = (const int32_t*)h;
+   logn.assign(p, p + n);
+   cut.assign(p + n, p + 2 * (size_t)n);
+   if( vr != NULL )
+      vr->assign(p + 2 * (size_t)n, p + 3 * (size_t)n);
+   if( vs != NULL )
+      vs->assign(p + 3 * (size_t)n, p + 4 * (size_t)n);
+   return SAP_OKAY;
+}
+
 static int stageOverlay(sap_bounds* b, int32_t n, const int64_t* ovptr, const int32_t* ovcol, const double* ovlb,
    const double* ovub, int64_t* novout)
 {
@@ -5315,7 +5480,7 @@ static int stageOverlay(sap_bounds* b, int32_t n, const int64_t* ovptr, const in
    RCCHK(ensureOverlay(b, nov));
    hipStream_t s = b->mat->ctx->stream;
    if( ovptr != NULL )
-      HIPCHK(hipMemcpyAsync(b->ovptr, ovptr, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyHostToDevice, s));
+      RCCHK(upload(b, b->ovptr, ovptr, sizeof(int64_t) * ((size_t)n + 1)));
    else
    {
       k_fill_i64<<<gridFor((size_t)n + 1), SAP_BLOCK, 0, s>>>((long long*)b->ovptr, 0, (size_t)n + 1);
@@ -5323,9 +5488,9 @@ static int stageOverlay(sap_bounds* b, int32_t n, const int64_t* ovptr, const in
    }
    if( nov > 0 )
    {
-      HIPCHK(hipMemcpyAsync(b->ovcol, ovcol, sizeof(int32_t) * (size_t)nov, hipMemcpyHostToDevice, s));
-      HIPCHK(hipMemcpyAsync(b->ovlb, ovlb, sizeof(double) * (size_t)nov, hipMemcpyHostToDevice, s));
-      HIPCHK(hipMemcpyAsync(b->ovub, ovub, sizeof(double) * (size_t)nov, hipMemcpyHostToDevice, s));
+      RCCHK(upload(b, b->ovcol, ovcol, sizeof(int32_t) * (size_t)nov));
+      RCCHK(upload(b, b->ovlb, ovlb, sizeof(double) * (size_t)nov));
+      RCCHK(upload(b, b->ovub, ovub, sizeof(double) * (size_t)nov));
    }
    *novout = nov;
    return SAP_OKAY;
@@ -5370,21 +5535,21 @@ static int initFromBase(sap_bounds* b, int32_t n, const int32_t* hslots, const s
       (*b->srcv)[v] = bv;
       (*b->srcgen)[v] = B != NULL ? B->gen : 0;
    }
+   // both lists use slotidx2: the second upload is queued behind the first list's kernel on
+   // the same stream, so it cannot overwrite the list that kernel reads
    if( !rest.empty() )
    {
-      HIPCHK(hipMemcpyAsync(b->slotidx2, rest.data(), sizeof(int32_t) * rest.size(), hipMemcpyHostToDevice, s));
+      RCCHK(upload(b, b->slotidx2, rest.data(), sizeof(int32_t) * rest.size()));
       k_slot_restore<<<slotGrid(1024, (int32_t)rest.size()), SAP_BLOCK, 0, s>>>(b->V, dbase, m->ncols, b->slotidx2,
          (int32_t)rest.size());
       HIPCHK(hipGetLastError());
-      HIPCHK(hipStreamSynchronize(s));
    }
    if( !full.empty() )
    {
-      HIPCHK(hipMemcpyAsync(b->slotidx2, full.data(), sizeof(int32_t) * full.size(), hipMemcpyHostToDevice, s));
+      RCCHK(upload(b, b->slotidx2, full.data(), sizeof(int32_t) * full.size()));
       k_slot_base<<<slotGrid((size_t)m->ncols, (int32_t)full.size()), SAP_BLOCK, 0, s>>>(b->V.bnd, dbase, m->ncols,
          b->slotidx2, (int32_t)full.size());
       HIPCHK(hipGetLastError());
-      HIPCHK(hipStreamSynchronize(s));
    }
    return SAP_OKAY;
 }
@@ -5415,13 +5580,8 @@ static int collectSlots(sap_bounds* b, int32_t n, const int32_t* hslots, int32_t
 {
    sap_matrix* m = b->mat;
    hipStream_t s = m->ctx->stream;
-   const size_t nv = (size_t)b->nvec;
-   std::vector<int32_t> logn(nv), cut(nv), vr(nv), vs(nv);
-   HIPCHK(hipMemcpyAsync(logn.data(), b->V.vlogn, sizeof(int32_t) * nv, hipMemcpyDeviceToHost, s));
-   HIPCHK(hipMemcpyAsync(cut.data(), b->V.vcutoff, sizeof(int32_t) * nv, hipMemcpyDeviceToHost, s));
-   HIPCHK(hipMemcpyAsync(vr.data(), b->V.vrounds, sizeof(int32_t) * nv, hipMemcpyDeviceToHost, s));
-   HIPCHK(hipMemcpyAsync(vs.data(), b->V.vstart, sizeof(int32_t) * nv, hipMemcpyDeviceToHost, s));
-   HIPCHK(hipStreamSynchronize(s));
+   std::vector<int32_t> logn, cut, vr, vs;
+   RCCHK(fetchMeta(b, b->nvec, logn, cut, &vr, &vs));
    // output offsets: a cut-off slot has an empty list
    std::vector<int64_t> off((size_t)n + 1, 0);
    int32_t maxlog = 0;
@@ -5441,8 +5601,25 @@ static int collectSlots(sap_bounds* b, int32_t n, const int32_t* hslots, int32_t
    const bool want = chgcol != NULL;
    if( want && tot > chgcap )
       return SAP_NOMEMORY;   // nothing cleared: the caller may collect again with room
+   // from here on every listed slot's log bits are cleared; when the slots cover every vector
+   // that may hold bits, no later call needs to clear them first
+   int32_t covered = 0;
+   {
+      std::vector<char> seen((size_t)std::max(b->logmax, 0), 0);
+      for( int32_t i = 0; i < n; ++i )
+         if( hslots[i] < b->logmax && !seen[(size_t)hslots[i]] )
+         {
+            seen[(size_t)hslots[i]] = 1;
+            ++covered;
+         }
+   }
+   const bool allclear = covered == b->logmax;
    if( maxlog == 0 )
+   {
+      if( allclear )
+         b->logmax = 0;
       return SAP_OKAY;
+   }
    // slots whose logs are only cleared, and slots whose changes are gathered
    std::vector<int32_t> clr, keep;
    std::vector<int64_t> koff;
@@ -5461,25 +5638,34 @@ static int collectSlots(sap_bounds* b, int32_t n, const int32_t* hslots, int32_t
    }
    if( !clr.empty() )
    {
-      HIPCHK(hipMemcpyAsync(b->slotidx, clr.data(), sizeof(int32_t) * clr.size(), hipMemcpyHostToDevice, s));
-      k_collect<<<slotGrid((size_t)maxlog, (int32_t)clr.size()), SAP_BLOCK, 0, s>>>(b->V, m->ncols, b->slotidx,
+      RCCHK(upload(b, b->slotidx2, clr.data(), sizeof(int32_t) * clr.size()));
+      k_collect<<<slotGrid((size_t)maxlog, (int32_t)clr.size()), SAP_BLOCK, 0, s>>>(b->V, m->ncols, b->slotidx2,
          (int32_t)clr.size(), b->outoff, NULL, NULL);
       HIPCHK(hipGetLastError());
-      HIPCHK(hipStreamSynchronize(s));
    }
    if( keep.empty() )
+   {
+      HIPCHK(hipStreamSynchronize(s));
+      if( allclear )
+         b->logmax = 0;
       return SAP_OKAY;
+   }
    RCCHK(ensureOut(b, tot));
-   HIPCHK(hipMemcpyAsync(b->slotidx, keep.data(), sizeof(int32_t) * keep.size(), hipMemcpyHostToDevice, s));
-   HIPCHK(hipMemcpyAsync(b->outoff, koff.data(), sizeof(int64_t) * koff.size(), hipMemcpyHostToDevice, s));
+   RCCHK(upload(b, b->slotidx, keep.data(), sizeof(int32_t) * keep.size()));
+   RCCHK(upload(b, b->outoff, koff.data(), sizeof(int64_t) * koff.size()));
    k_collect<<<slotGrid((size_t)maxlog, (int32_t)keep.size()), SAP_BLOCK, 0, s>>>(b->V, m->ncols, b->slotidx,
       (int32_t)keep.size(), b->outoff, b->outcol, b->outbnd);
    HIPCHK(hipGetLastError());
-   std::vector<int32_t> hc((size_t)tot);
-   std::vector<double2> hb((size_t)tot);
-   HIPCHK(hipMemcpyAsync(hc.data(), b->outcol, sizeof(int32_t) * (size_t)tot, hipMemcpyDeviceToHost, s));
-   HIPCHK(hipMemcpyAsync(hb.data(), b->outbnd, sizeof(double2) * (size_t)tot, hipMemcpyDeviceToHost, s));
+   const size_t cbytes = ((sizeof(int32_t) * (size_t)tot) + 255) & ~(size_t)255;
+   char* h = NULL;
+   RCCHK(downloadArea(b, cbytes + sizeof(double2) * (size_t)tot, &h));
+   const int32_t* hc = (const int32_t*)h;
+   const double2* hb = (const double2*)(h + cbytes);
+   RCCHK(download(b, h, b->outcol, sizeof(int32_t) * (size_t)tot));
+   RCCHK(download(b, h + cbytes, b->outbnd, sizeof(double2) * (size_t)tot));
    HIPCHK(hipStreamSynchronize(s));
+   if( allclear )
+      b->logmax = 0;
    // the log order depends on atomics: each slot's list is sorted by column
    std::vector<int64_t> idx;
    for( int32_t i = 0; i < n; ++i )
@@ -5505,24 +5691,23 @@ static int collectSlots(sap_bounds* b, int32_t n, const int32_t* hslots, int32_t
 // clear the change logs of all vectors (before a one-shot call reuses them)
 static int clearLogs(sap_bounds* b)
 {
+   if( b->logmax == 0 )
+      return SAP_OKAY;   // every log bit written so far has been collected
    sap_matrix* m = b->mat;
    hipStream_t s = m->ctx->stream;
-   std::vector<int32_t> all((size_t)b->nvec), logn((size_t)b->nvec);
-   HIPCHK(hipMemcpyAsync(logn.data(), b->V.vlogn, sizeof(int32_t) * (size_t)b->nvec, hipMemcpyDeviceToHost, s));
-   HIPCHK(hipStreamSynchronize(s));
+   std::vector<int32_t> logn, cut;
+   RCCHK(fetchMeta(b, b->nvec, logn, cut, NULL, NULL));
    int32_t maxlog = 0;
    for( int32_t v = 0; v < b->nvec; ++v )
-   {
-      all[(size_t)v] = v;
       maxlog = std::max(maxlog, logn[(size_t)v]);
+   if( maxlog > 0 )
+   {
+      k_collect<<<slotGrid((size_t)maxlog, b->nvec), SAP_BLOCK, 0, s>>>(b->V, m->ncols, b->ident, b->nvec, b->outoff,
+         NULL, NULL);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(s));
    }
-   if( maxlog == 0 )
-      return SAP_OKAY;
-   HIPCHK(hipMemcpyAsync(b->slotidx, all.data(), sizeof(int32_t) * (size_t)b->nvec, hipMemcpyHostToDevice, s));
-   k_collect<<<slotGrid((size_t)maxlog, b->nvec), SAP_BLOCK, 0, s>>>(b->V, m->ncols, b->slotidx, b->nvec, b->outoff,
-      NULL, NULL);
-   HIPCHK(hipGetLastError());
-   HIPCHK(hipStreamSynchronize(s));
+   b->logmax = 0;
    return SAP_OKAY;
 }
 
@@ -5543,21 +5728,21 @@ static int batchCall(sap_bounds* b, const sap_params* params, int32_t nvec, cons
       ident[(size_t)v] = v;
    RCCHK(initFromBase(b, nvec, ident.data(), B, bv, dbase));
    b->gen++;
-   HIPCHK(hipMemcpyAsync(b->slotidx, ident.data(), sizeof(int32_t) * (size_t)nvec, hipMemcpyHostToDevice, s));
-   // the call works on the first nvec vectors; the single-vector cutoff shortcut only
-   // applies when the batch is one vector
+   // the call works on the first nvec vectors (slot list b->ident); the single-vector cutoff
+   // shortcut only applies when the batch is one vector
    const CallScope saved(b->V);
    b->V.nvec = nvec;
    b->V.single = (nvec == 1 && m->nseg == 1) ? 1 : 0;
    b->V.logchg = 1;
+   b->logmax = std::max(b->logmax, nvec);
    int rc = beginCall(b, nvec, (!fromfix && b->V.dense) ? 1 : 0);
    if( rc == SAP_OKAY )
    {
       if( nov > 0 )
-         k_slot_overlay<<<slotGrid((size_t)(nov / nvec + 1), nvec), SAP_BLOCK, 0, s>>>(b->V, m->ncols, b->slotidx,
+         k_slot_overlay<<<slotGrid((size_t)(nov / nvec + 1), nvec), SAP_BLOCK, 0, s>>>(b->V, m->ncols, b->ident,
             nvec, b->ovptr, b->ovcol, b->ovlb, b->ovub);
       if( fromfix && nov > 0 )
-         k_slot_mark<<<dim3(slotMarkBlocks(nov)), SAP_BLOCK, 0, s>>>(toMat(m), b->V, b->slotidx, nvec,
+         k_slot_mark<<<dim3(slotMarkBlocks(nov)), SAP_BLOCK, 0, s>>>(toMat(m), b->V, b->ident, nvec,
             b->ovptr, b->ovcol, 0, slotMarkChunk(nov));
       rc = hipToRc(hipGetLastError());
    }
@@ -5703,24 +5888,29 @@ static int probeCore(sap_bounds* b, const sap_params* params, int32_t nprobe, co
    for( int32_t p0 = 0; p0 < nprobe; p0 += b->nvec )
    {
       const int32_t n = std::min<int32_t>(b->nvec, nprobe - p0);
-      HIPCHK(hipMemcpyAsync(b->pcol, probecol + p0, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, s));
+      RCCHK(upload(b, b->pcol, probecol + p0, sizeof(int32_t) * (size_t)n));
+      char* h = NULL;
       if( split )
       {
+         RCCHK(downloadArea(b, sizeof(double2) * (size_t)n, &h));
          k_gather_bounds<<<gridFor((size_t)n), SAP_BLOCK, 0, s>>>(dbase, nc, b->pcol, NULL, n, b->outbnd);
          HIPCHK(hipGetLastError());
-         HIPCHK(hipMemcpyAsync(pbase.data() + p0, b->outbnd, sizeof(double2) * (size_t)n, hipMemcpyDeviceToHost, s));
+         RCCHK(download(b, h, b->outbnd, sizeof(double2) * (size_t)n));
          HIPCHK(hipStreamSynchronize(s));
+         memcpy(pbase.data() + p0, h, sizeof(double2) * (size_t)n);
          for( int32_t q = p0; q < p0 + n; ++q )
             if( !(downub[q] >= pbase[(size_t)q].x && downub[q] < uplb[q] && uplb[q] <= pbase[(size_t)q].y) )
                return SAP_INVALIDDATA;
          continue;
       }
-      int32_t bad = INT32_MAX;
+      RCCHK(downloadArea(b, sizeof(int32_t), &h));
       k_fill_i32<<<1, 64, 0, s>>>(b->pcount, INT32_MAX, 1);
       k_probe_check<<<gridFor((size_t)n), SAP_BLOCK, 0, s>>>(toMat(m), dbase, b->pcol, n, b->pcount);
       HIPCHK(hipGetLastError());
-      HIPCHK(hipMemcpyAsync(&bad, b->pcount, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      RCCHK(download(b, h, b->pcount, sizeof(int32_t)));
       HIPCHK(hipStreamSynchronize(s));
+      int32_t bad = INT32_MAX;
+      memcpy(&bad, h, sizeof(int32_t));
       if( bad != INT32_MAX )
          return SAP_INVALIDDATA;
    }
@@ -5768,11 +5958,7 @@ static int probeCore(sap_bounds* b, const sap_params* params, int32_t nprobe, co
       acc.sweep_ms += cs.sweep_ms;
       acc.sweep_launches += cs.sweep_launches;
       // room for the columns changed in both branches: at most the shorter log per probe
-      logn.resize((size_t)nv);
-      cut.resize((size_t)nv);
-      HIPCHK(hipMemcpyAsync(logn.data(), b->V.vlogn, sizeof(int32_t) * (size_t)nv, hipMemcpyDeviceToHost, s));
-      HIPCHK(hipMemcpyAsync(cut.data(), b->V.vcutoff, sizeof(int32_t) * (size_t)nv, hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
+      RCCHK(fetchMeta(b, nv, logn, cut, NULL, NULL));
       off.assign((size_t)np + 1, 0);
       int32_t maxs = 0;
       for( int32_t p = 0; p < np; ++p )
@@ -5787,14 +5973,17 @@ static int probeCore(sap_bounds* b, const sap_params* params, int32_t nprobe, co
       if( room > 0 )
       {
          RCCHK(ensureProbe(b, room));
-         HIPCHK(hipMemcpyAsync(b->pcol, probecol + p0, sizeof(int32_t) * (size_t)np, hipMemcpyHostToDevice, s));
-         HIPCHK(hipMemcpyAsync(b->outoff, off.data(), sizeof(int64_t) * (size_t)np, hipMemcpyHostToDevice, s));
+         char* h = NULL;
+         RCCHK(downloadArea(b, sizeof(int32_t) * (size_t)np, &h));
+         RCCHK(upload(b, b->pcol, probecol + p0, sizeof(int32_t) * (size_t)np));
+         RCCHK(upload(b, b->outoff, off.data(), sizeof(int64_t) * (size_t)np));
          k_fill_i32<<<gridFor((size_t)np), SAP_BLOCK, 0, s>>>(b->pcount, 0, (size_t)np);
          k_probe_pairs<<<slotGrid((size_t)maxs, np), SAP_BLOCK, 0, s>>>(b->V, nc, b->pcol, np, b->outoff, b->pcount,
             b->outcol, b->outbnd, b->outbnd2);
          HIPCHK(hipGetLastError());
-         HIPCHK(hipMemcpyAsync(pc.data(), b->pcount, sizeof(int32_t) * (size_t)np, hipMemcpyDeviceToHost, s));
+         RCCHK(download(b, h, b->pcount, sizeof(int32_t) * (size_t)np));
          HIPCHK(hipStreamSynchronize(s));
+         memcpy(pc.data(), h, sizeof(int32_t) * (size_t)np);
       }
       int64_t got = 0;
       for( int32_t p = 0; p < np; ++p )
@@ -5803,10 +5992,17 @@ static int probeCore(sap_bounds* b, const sap_params* params, int32_t nprobe, co
       std::vector<double2> t0((size_t)room), t1((size_t)room);
       if( got > 0 )
       {
-         HIPCHK(hipMemcpyAsync(tc.data(), b->outcol, sizeof(int32_t) * (size_t)room, hipMemcpyDeviceToHost, s));
-         HIPCHK(hipMemcpyAsync(t0.data(), b->outbnd, sizeof(double2) * (size_t)room, hipMemcpyDeviceToHost, s));
-         HIPCHK(hipMemcpyAsync(t1.data(), b->outbnd2, sizeof(double2) * (size_t)room, hipMemcpyDeviceToHost, s));
+         const size_t cb = (sizeof(int32_t) * (size_t)room + 255) & ~(size_t)255;
+         const size_t bb = (sizeof(double2) * (size_t)room + 255) & ~(size_t)255;
+         char* h = NULL;
+         RCCHK(downloadArea(b, cb + 2 * bb, &h));
+         RCCHK(download(b, h, b->outcol, sizeof(int32_t) * (size_t)room));
+         RCCHK(download(b, h + cb, b->outbnd, sizeof(double2) * (size_t)room));
+         RCCHK(download(b, h + cb + bb, b->outbnd2, sizeof(double2) * (size_t)room));
          HIPCHK(hipStreamSynchronize(s));
+         memcpy(tc.data(), h, sizeof(int32_t) * (size_t)room);
+         memcpy(t0.data(), h + cb, sizeof(double2) * (size_t)room);
+         memcpy(t1.data(), h + cb + bb, sizeof(double2) * (size_t)room);
       }
       // split probing: the probe columns' own bounds in both branches
       std::vector<double2> x0, x1;
@@ -5819,12 +6015,15 @@ static int probeCore(sap_bounds* b, const sap_params* params, int32_t nprobe, co
             xc[(size_t)v] = probecol[p0 + v / 2];
          }
          x0.resize((size_t)nv);
-         HIPCHK(hipMemcpyAsync(b->pcol, xc.data(), sizeof(int32_t) * (size_t)nv, hipMemcpyHostToDevice, s));
-         HIPCHK(hipMemcpyAsync(b->pcount, xv.data(), sizeof(int32_t) * (size_t)nv, hipMemcpyHostToDevice, s));
+         char* h = NULL;
+         RCCHK(downloadArea(b, sizeof(double2) * (size_t)nv, &h));
+         RCCHK(upload(b, b->pcol, xc.data(), sizeof(int32_t) * (size_t)nv));
+         RCCHK(upload(b, b->pcount, xv.data(), sizeof(int32_t) * (size_t)nv));
          k_gather_bounds<<<gridFor((size_t)nv), SAP_BLOCK, 0, s>>>(b->V.bnd, nc, b->pcol, b->pcount, nv, b->outbnd);
          HIPCHK(hipGetLastError());
-         HIPCHK(hipMemcpyAsync(x0.data(), b->outbnd, sizeof(double2) * (size_t)nv, hipMemcpyDeviceToHost, s));
+         RCCHK(download(b, h, b->outbnd, sizeof(double2) * (size_t)nv));
          HIPCHK(hipStreamSynchronize(s));
+         memcpy(x0.data(), h, sizeof(double2) * (size_t)nv);
       }
       // verdicts and rounds; the change logs are cleared for the next chunk
       ident.resize((size_t)nv);
@@ -5980,6 +6179,7 @@ static int slotsInit(sap_bounds* b)
       return SAP_INVALIDDATA;
    RCCHK(ensureLog(b));
    RCCHK(clearLogs(b));
+   b->logmax = b->nvec;   // any slot may hold log bits from here on
    b->active = new std::vector<int32_t>((size_t)b->nvec, 0);
    if( b->hstat == NULL )
    {
@@ -6015,7 +6215,7 @@ int sap_slots_admit(sap_bounds* b, int32_t n, const int32_t* slots, const sap_bo
    }
    int64_t nov = 0;
    RCCHK(stageOverlay(b, n, ovptr, ovcol, ovlb, ovub, &nov));
-   HIPCHK(hipMemcpyAsync(b->slotidx, slots, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, m->ctx->stream));
+   RCCHK(upload(b, b->slotidx, slots, sizeof(int32_t) * (size_t)n));
    RCCHK(admitSlots(b, n, slots, b->slotidx, base, basevec, base->V.bnd + (size_t)basevec * (size_t)m->ncols, nov,
       (flags & SAP_FLAG_BASE_IS_FIXPOINT) ? 0 : 1));
    b->gen++;
