@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <memory>
 #include <thread>
 #include <vector>
 
@@ -3752,6 +3753,68 @@ static void parallelFor(int64_t n, int64_t grain, F fn)
       x.join();
 }
 
+// host array without value initialisation: the parallel passes write every element, and
+// zeroing hundreds of MB first would be a serial pass of its own
+template <typename T>
+struct HostArray
+{
+   std::unique_ptr<T[]> p;
+   explicit HostArray(size_t n) : p(new T[n > 0 ? n : 1]) {}
+   T* data() { return p.get(); }
+   T& operator[](size_t i) { return p[i]; }
+   const T& operator[](size_t i) const { return p[i]; }
+};
+
+// column-major copy (colptr, rowidx) of the stored rows, without atomics: thread t owns the
+// columns [ncols*t/T, ncols*(t+1)/T) and scans every nonzero twice, counting, then (after a
+// prefix sum) writing the rows of its own columns, so each column's rows come out ascending
+static void buildColumns(int64_t nrows, int64_t ncols, int64_t nnz, const int64_t* prowptr, const int32_t* pcol,
+   int64_t* colptr, int32_t* rowidx)
+{
+   int nt = (int)std::thread::hardware_concurrency();
+   nt = nt < 1 ? 1 : (nt > 16 ? 16 : nt);
+   if( nnz < (1 << 18) || ncols < nt )
+      nt = 1;
+   for( int64_t j = 0; j <= ncols; ++j )
+      colptr[j] = 0;
+   auto owned = [&](int t, int64_t& lo, int64_t& hi) {
+      lo = ncols * t / nt;
+      hi = ncols * (t + 1) / nt;
+   };
+   std::vector<std::thread> th;
+   for( int t = 0; t < nt; ++t )
+      th.emplace_back([&, t]() {
+         int64_t lo, hi;
+         owned(t, lo, hi);
+         for( int64_t k = 0; k < nnz; ++k )
+         {
+            const int64_t c = pcol[k];
+            if( c >= lo && c < hi )
+               colptr[c + 1]++;
+         }
+      });
+   for( auto& x : th )
+      x.join();
+   th.clear();
+   for( int64_t j = 0; j < ncols; ++j )
+      colptr[j + 1] += colptr[j];
+   for( int t = 0; t < nt; ++t )
+      th.emplace_back([&, t]() {
+         int64_t lo, hi;
+         owned(t, lo, hi);
+         std::vector<int64_t> cur(colptr + lo, colptr + hi);
+         for( int64_t r = 0; r < nrows; ++r )
+            for( int64_t k = prowptr[r]; k < prowptr[r + 1]; ++k )
+            {
+               const int64_t c = pcol[k];
+               if( c >= lo && c < hi )
+                  rowidx[cur[(size_t)(c - lo)]++] = (int32_t)r;
+            }
+      });
+   for( auto& x : th )
+      x.join();
+}
+
 static inline int lenBin(int64_t L, int32_t longrow)
 {
    for( int q = 0; q < 6; ++q )
@@ -4078,13 +4141,13 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
       const int64_t i = perm[(size_t)r];
       prowptr[(size_t)r + 1] = prowptr[(size_t)r] + (rowptr[i + 1] - rowptr[i]);
    }
-   std::vector<int32_t> pcol((size_t)nnz);
-   std::vector<double> pval((size_t)nnz);
-   std::vector<uint8_t> pflag((size_t)nnz);
+   HostArray<int32_t> pcol((size_t)nnz);
+   HostArray<double> pval((size_t)nnz);
+   HostArray<uint8_t> pflag((size_t)nnz);
    std::vector<double2> sides((size_t)nrows);
    std::vector<uint8_t> rowflags((size_t)nrows), rowbin((size_t)nrows);
-   std::vector<int64_t> colptr((size_t)ncols + 1, 0);
-   std::vector<int32_t> rowidx((size_t)nnz);
+   std::vector<int64_t> colptr((size_t)ncols + 1);
+   HostArray<int32_t> rowidx((size_t)nnz);
    parallelFor(nrows, 1 << 16, [&](int64_t r0, int64_t r1, int) {
       for( int64_t r = r0; r < r1; ++r )
       {
@@ -4099,7 +4162,6 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
             pflag[(size_t)(o + k)] = colflags[(size_t)c];
             if( !(colflags[(size_t)c] & 2) )
                allbin = false;
-            __atomic_fetch_add(&colptr[(size_t)c + 1], (int64_t)1, __ATOMIC_RELAXED);
          }
          sides[(size_t)r] = make_double2(lhs[i], rhs[i]);
          // bit 1: eligible for ranged-row propagation (two sides of magnitude below SCIP's
@@ -4109,18 +4171,7 @@ static int matrixCreate(sap_context* ctx, int64_t nrows, int64_t ncols, const in
          rowbin[(size_t)r] = obin[(size_t)i];
       }
    });
-   for( int64_t j = 0; j < ncols; ++j )
-      colptr[(size_t)j + 1] += colptr[(size_t)j];
-   {
-      // the row order within a column is irrelevant (rows of a changed column are marked)
-      std::vector<int64_t> fill(colptr.begin(), colptr.end() - 1);
-      parallelFor(nrows, 1 << 16, [&](int64_t r0, int64_t r1, int) {
-         for( int64_t r = r0; r < r1; ++r )
-            for( int64_t k = prowptr[(size_t)r]; k < prowptr[(size_t)r + 1]; ++k )
-               rowidx[(size_t)__atomic_fetch_add(&fill[(size_t)pcol[(size_t)k]], (int64_t)1, __ATOMIC_RELAXED)]
-                  = (int32_t)r;
-      });
-   }
+   buildColumns(nrows, ncols, nnz, prowptr.data(), pcol.data(), colptr.data(), rowidx.data());
 
    // tiles over the packed rows of the tiled bins (consecutive rows, at most 64 lanes)
    std::vector<int64_t> tstart;
