@@ -605,8 +605,9 @@ def run_serving(sa, args, parts):
     """Poisson arrivals (rate, seconds from the arguments).  Half the requests are nodes of the
     covering/packing 200,000 x 300,000 problem (1..50 binaries fixed); half are nodes (1..50
     integer columns fixed to 0/1) of one of the 512 uneven instances, chosen with Zipf(1.2)
-    popularity over a random order, so request sizes follow the instances' size law.  The HBM
-    budget holds the covering/packing problem and 85 % of the instances' footprint: instances
+    popularity over a random order, so request sizes follow the instances' size law; slot pools
+    are sized by each problem's share of the requests.  The HBM budget holds the
+    covering/packing problem and 85 % of the instances' footprint: instances
     are uploaded on demand and evicted least recently used during the trace.  Every replay
     starts from the same resident set (the most popular instances that fit).  Continuous
     batching against the plain one-request-at-a-time form: each at the offered rate (latency)
@@ -618,7 +619,14 @@ def run_serving(sa, args, parts):
     srv = sa.Server(0, 4, 1 << 50, 8)
     pids = [srv.add_problem(p) for p in probs]
     assert pids == list(range(len(probs)))
+    # slot pools sized by demand, as a deployment would size them from its request mix: the
+    # covering/packing problem 256 slots, an instance 8 to 128 (a slot per ~64 of its requests
+    # in the trace, rounded up to a power of two)
     srv.set_slots(0, 256)
+    demand = np.bincount(prob, minlength=len(probs))
+    for pid in range(1, len(probs)):
+        want = int(2 ** np.ceil(np.log2(max(demand[pid] / 64.0, 1.0))))
+        srv.set_slots(pid, int(min(128, max(8, want))))
     foot = np.array([srv.footprint(pid) for pid in pids], dtype=np.int64)
     budget = int(foot[0] + 0.85 * foot[1:].sum())
     srv.set_budget(budget)
