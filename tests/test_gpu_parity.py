@@ -329,3 +329,21 @@ def test_unique_integer_fixpoint_equals_scip_order(gpu_ctx, seed):
     sl, su, sr = orc.propagate_scip(orc.OMatrix.from_instance(inst), inst.lb, inst.ub, orc.params(boundstreps=0.0))
     assert not sr["cutoff"] and gs["result"] == sa.SAP_REDUCEDDOM
     assert np.array_equal(gl, sl) and np.array_equal(gu, su)
+
+
+@pytest.mark.parametrize("blocks", [0, 3])
+def test_gather_floor_entry_point(gpu_ctx, blocks):
+    # the memory-floor measurement (sap_bench_gather) on a matrix whose nonzero count is not a
+    # multiple of four (the kernel's vector loads and its one-by-one tail), with one lane per
+    # four nonzeros and with a grid-stride loop over three workgroups; it leaves the bounds alone
+    inst = sa.gen_mixed(3001, 2500, seed=8)
+    assert inst.nnz % 4 != 0
+    mat = sa.Matrix(gpu_ctx, inst)
+    b = sa.Bounds(mat, 1)
+    b.upload(inst.lb, inst.ub)
+    ms = b.bench_gather(3, blocks)
+    assert 0.0 < ms < 1000.0
+    gl, gu = b.download(0)
+    assert np.array_equal(gl, inst.lb) and np.array_equal(gu, inst.ub)
+    b.close()
+    mat.close()
