@@ -959,6 +959,10 @@ __device__ __forceinline__ Elem loadElem(const Mat& M, const double2* bnd, int64
 //     lower bounds.  Equality is left to the exact path.
 // ------------------------------------------------------------------------------------------
 #define SAP_QCAP 128
+// the queue is a ring indexed modulo SAP_QCAP: a drain leaves fewer than a wavefront's worth
+// of entries and an append adds at most one per lane, so it never holds more than
+// 2 * SAP_WAVE - 1 entries
+static_assert((SAP_QCAP & (SAP_QCAP - 1)) == 0 && SAP_QCAP >= 2 * SAP_WAVE, "queue capacity");
 
 struct WaveQueue
 {
