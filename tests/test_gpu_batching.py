@@ -432,3 +432,43 @@ def test_parents_outlive_children(gpu_ctx):
     b.close()
     mat.close()
     ctx.close()
+
+
+def test_log_clearing_after_uncollected_calls(gpu_ctx):
+    # the host keeps a bound on the vectors whose change-log bits may be uncollected: a call
+    # that stops for lack of room (SAP_NOMEMORY) leaves them, a smaller call must then still
+    # start from clean logs for every vector, and later calls of any size give the same
+    # answers as a fresh bound set
+    inst, om, rl, ru, ptr, cols, vals = node_problem(11, nvec=16)
+    mat = sa.Matrix(gpu_ctx, inst)
+    base = sa.Bounds(mat, 1)
+    base.upload(rl, ru)
+    p = sa.default_params(flags=sa.SAP_FLAG_BASE_IS_FIXPOINT)
+
+    def sub(n):
+        return ptr[:n + 1], cols[:ptr[n]], vals[:ptr[n]]
+
+    fresh = {}
+    for n in (16, 8, 3):
+        f = sa.Bounds(mat, 16)
+        sp, sc, sv = sub(n)
+        fresh[n] = f.propagate_nodes(base, 0, sp, sc, sv, sv, p)
+        f.close()
+    need = int(fresh[16]["chgptr"][-1])
+    assert need > 1
+    nodes = sa.Bounds(mat, 16)
+    for n in (16, 8, 16, 3, 8):
+        if n == 16:
+            sp, sc, sv = sub(16)
+            with pytest.raises(sa.SapError) as e:
+                nodes.propagate_nodes(base, 0, sp, sc, sv, sv, p, chgcap=1)
+            assert e.value.rc == sa.SAP_NOMEMORY
+        sp, sc, sv = sub(n)
+        out = nodes.propagate_nodes(base, 0, sp, sc, sv, sv, p)
+        for key in ("result", "rounds", "chgptr", "chgcol", "chglb", "chgub"):
+            assert np.array_equal(out[key], fresh[n][key]), (n, key)
+        for v in range(n):
+            check_node_out(om, rl, ru, sp, sc, sv, out, v, check_rounds=True)
+    nodes.close()
+    base.close()
+    mat.close()
