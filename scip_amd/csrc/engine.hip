@@ -3375,6 +3375,24 @@ __global__ void k_copy_words(uint32_t* dst, const uint32_t* src, size_t n)
       dst[i] = src[i];
 }
 
+// up to four word copies of one upload (piece y of the grid's y dimension)
+struct CopyPieces
+{
+   uint32_t* dst[4];
+   const uint32_t* src[4];
+   size_t n[4];
+};
+
+__global__ void k_copy_pieces(CopyPieces c)
+{
+   const int y = (int)blockIdx.y;
+   uint32_t* dst = c.dst[y];
+   const uint32_t* src = c.src[y];
+   const size_t n = c.n[y];
+   for( size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x )
+      dst[i] = src[i];
+}
+
 __global__ void k_iota_i32(int32_t* p, size_t n)
 {
    for( size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x )
@@ -5452,17 +5470,33 @@ static int copyWords(hipStream_t s, void* dst, const void* src, size_t bytes)
    return SAP_OKAY;
 }
 
-// A host-to-device copy for the batched calls: the bytes go to the bound set's pinned upload
-// area first, and a copy kernel on the call's stream moves them, so nothing waits (a pageable
-// source makes every copy synchronous) and the caller's array may be reused at once.  Pieces
-// of the area are handed out in order; when it is full, the stream is drained, after which
-// every earlier piece has been read, and the area starts over (grown if one piece needs more).
-static int upload(sap_bounds* b, void* ddst, const void* hsrc, size_t bytes)
+// Host-to-device copies for the batched calls (at most four at once): the bytes go to the bound
+// set's pinned upload area first, and one copy kernel on the call's stream moves them, so
+// nothing waits (a pageable source makes every copy synchronous) and the caller's arrays may
+// be reused at once.  Pieces of the area are handed out in order; when it is full, the stream
+// is drained, after which every earlier piece has been read, and the area starts over (grown
+// if the pieces need more).
+struct Upload
 {
-   if( bytes == 0 )
-      return SAP_OKAY;
+   void* dst;
+   const void* src;
+   size_t bytes;
+};
+
+static int uploadMany(sap_bounds* b, const Upload* u, int count)
+{
+   if( count < 1 || count > 4 )
+      return SAP_ERROR;
    hipStream_t s = b->mat->ctx->stream;
-   const size_t need = (bytes + 255) & ~(size_t)255;
+   size_t need = 0;
+   for( int i = 0; i < count; ++i )
+   {
+      if( u[i].bytes % 4 != 0 )
+         return SAP_ERROR;
+      need += (u[i].bytes + 255) & ~(size_t)255;
+   }
+   if( need == 0 )
+      return SAP_OKAY;
    if( b->hupused + need > b->hupcap )
    {
       HIPCHK(hipStreamSynchronize(s));
@@ -5479,10 +5513,28 @@ static int upload(sap_bounds* b, void* ddst, const void* hsrc, size_t bytes)
          b->hupcap = cap;
       }
    }
-   memcpy(b->hup + b->hupused, hsrc, bytes);
-   RCCHK(copyWords(s, ddst, b->hupd + b->hupused, bytes));
-   b->hupused += need;
+   CopyPieces c;
+   memset(&c, 0, sizeof(c));
+   size_t most = 0;
+   for( int i = 0; i < count; ++i )
+   {
+      if( u[i].bytes > 0 )
+         memcpy(b->hup + b->hupused, u[i].src, u[i].bytes);
+      c.dst[i] = (uint32_t*)u[i].dst;
+      c.src[i] = (const uint32_t*)(b->hupd + b->hupused);
+      c.n[i] = u[i].bytes / 4;
+      most = std::max(most, c.n[i]);
+      b->hupused += (u[i].bytes + 255) & ~(size_t)255;
+   }
+   k_copy_pieces<<<dim3(gridFor(most), (unsigned)count), SAP_BLOCK, 0, s>>>(c);
+   HIPCHK(hipGetLastError());
    return SAP_OKAY;
+}
+
+static int upload(sap_bounds* b, void* ddst, const void* hsrc, size_t bytes)
+{
+   const Upload u = { ddst, hsrc, bytes };
+   return uploadMany(b, &u, 1);
 }
 
 // pinned landing area of at least `bytes` for device-to-host copies of the batched calls
@@ -5547,24 +5599,31 @@ static int stageOverlay(sap_bounds* b, int32_t n, const int64_t* ovptr, const in
    for( int32_t v = 0; ovptr && v < n; ++v )
       if( ovptr[v + 1] < ovptr[v] || ovptr[v + 1] - ovptr[v] > nc )
          return SAP_INVALIDDATA;
+   // without early exits, so that the check vectorises
+   int bad = 0;
    for( int64_t q = 0; q < nov; ++q )
-      if( ovcol[q] < 0 || ovcol[q] >= nc || ovlb[q] != ovlb[q] || ovub[q] != ovub[q] )
-         return SAP_INVALIDDATA;
+      bad |= (ovcol[q] < 0) | (ovcol[q] >= nc) | (ovlb[q] != ovlb[q]) | (ovub[q] != ovub[q]);
+   if( bad )
+      return SAP_INVALIDDATA;
    RCCHK(ensureOverlay(b, nov));
    hipStream_t s = b->mat->ctx->stream;
-   if( ovptr != NULL )
-      RCCHK(upload(b, b->ovptr, ovptr, sizeof(int64_t) * ((size_t)n + 1)));
-   else
+   if( ovptr == NULL )
    {
       k_fill_i64<<<gridFor((size_t)n + 1), SAP_BLOCK, 0, s>>>((long long*)b->ovptr, 0, (size_t)n + 1);
       HIPCHK(hipGetLastError());
    }
+   Upload u[4];
+   int nu = 0;
+   if( ovptr != NULL )
+      u[nu++] = Upload{ b->ovptr, ovptr, sizeof(int64_t) * ((size_t)n + 1) };
    if( nov > 0 )
    {
-      RCCHK(upload(b, b->ovcol, ovcol, sizeof(int32_t) * (size_t)nov));
-      RCCHK(upload(b, b->ovlb, ovlb, sizeof(double) * (size_t)nov));
-      RCCHK(upload(b, b->ovub, ovub, sizeof(double) * (size_t)nov));
+      u[nu++] = Upload{ b->ovcol, ovcol, sizeof(int32_t) * (size_t)nov };
+      u[nu++] = Upload{ b->ovlb, ovlb, sizeof(double) * (size_t)nov };
+      u[nu++] = Upload{ b->ovub, ovub, sizeof(double) * (size_t)nov };
    }
+   if( nu > 0 )
+      RCCHK(uploadMany(b, u, nu));
    *novout = nov;
    return SAP_OKAY;
 }
@@ -5724,8 +5783,9 @@ static int collectSlots(sap_bounds* b, int32_t n, const int32_t* hslots, int32_t
       return SAP_OKAY;
    }
    RCCHK(ensureOut(b, tot));
-   RCCHK(upload(b, b->slotidx, keep.data(), sizeof(int32_t) * keep.size()));
-   RCCHK(upload(b, b->outoff, koff.data(), sizeof(int64_t) * koff.size()));
+   const Upload lists[2] = { { b->slotidx, keep.data(), sizeof(int32_t) * keep.size() },
+      { b->outoff, koff.data(), sizeof(int64_t) * koff.size() } };
+   RCCHK(uploadMany(b, lists, 2));
    k_collect<<<slotGrid((size_t)maxlog, (int32_t)keep.size()), SAP_BLOCK, 0, s>>>(b->V, m->ncols, b->slotidx,
       (int32_t)keep.size(), b->outoff, b->outcol, b->outbnd);
    HIPCHK(hipGetLastError());
@@ -5739,11 +5799,36 @@ static int collectSlots(sap_bounds* b, int32_t n, const int32_t* hslots, int32_t
    HIPCHK(hipStreamSynchronize(s));
    if( allclear )
       b->logmax = 0;
-   // the log order depends on atomics: each slot's list is sorted by column
+   // the log order depends on atomics: each slot's list is sorted by column (short lists, the
+   // common case, by insertion into the output)
    std::vector<int64_t> idx;
    for( int32_t i = 0; i < n; ++i )
    {
       const int64_t a = off[(size_t)i], e = off[(size_t)i + 1];
+      if( e - a <= 16 )
+      {
+         for( int64_t q = a; q < e; ++q )
+         {
+            const int32_t c = hc[(size_t)q];
+            const double2 bq = hb[(size_t)q];
+            int64_t w = q;
+            while( w > a && chgcol[w - 1] > c )
+            {
+               chgcol[w] = chgcol[w - 1];
+               if( chglb )
+                  chglb[w] = chglb[w - 1];
+               if( chgub )
+                  chgub[w] = chgub[w - 1];
+               --w;
+            }
+            chgcol[w] = c;
+            if( chglb )
+               chglb[w] = bq.x;
+            if( chgub )
+               chgub[w] = bq.y;
+         }
+         continue;
+      }
       idx.resize((size_t)(e - a));
       for( int64_t q = a; q < e; ++q )
          idx[(size_t)(q - a)] = q;
@@ -6048,8 +6133,9 @@ static int probeCore(sap_bounds* b, const sap_params* params, int32_t nprobe, co
          RCCHK(ensureProbe(b, room));
          char* h = NULL;
          RCCHK(downloadArea(b, sizeof(int32_t) * (size_t)np, &h));
-         RCCHK(upload(b, b->pcol, probecol + p0, sizeof(int32_t) * (size_t)np));
-         RCCHK(upload(b, b->outoff, off.data(), sizeof(int64_t) * (size_t)np));
+         const Upload lists[2] = { { b->pcol, probecol + p0, sizeof(int32_t) * (size_t)np },
+            { b->outoff, off.data(), sizeof(int64_t) * (size_t)np } };
+         RCCHK(uploadMany(b, lists, 2));
          k_fill_i32<<<gridFor((size_t)np), SAP_BLOCK, 0, s>>>(b->pcount, 0, (size_t)np);
          k_probe_pairs<<<slotGrid((size_t)maxs, np), SAP_BLOCK, 0, s>>>(b->V, nc, b->pcol, np, b->outoff, b->pcount,
             b->outcol, b->outbnd, b->outbnd2);
@@ -6090,8 +6176,9 @@ static int probeCore(sap_bounds* b, const sap_params* params, int32_t nprobe, co
          x0.resize((size_t)nv);
          char* h = NULL;
          RCCHK(downloadArea(b, sizeof(double2) * (size_t)nv, &h));
-         RCCHK(upload(b, b->pcol, xc.data(), sizeof(int32_t) * (size_t)nv));
-         RCCHK(upload(b, b->pcount, xv.data(), sizeof(int32_t) * (size_t)nv));
+         const Upload lists[2] = { { b->pcol, xc.data(), sizeof(int32_t) * (size_t)nv },
+            { b->pcount, xv.data(), sizeof(int32_t) * (size_t)nv } };
+         RCCHK(uploadMany(b, lists, 2));
          k_gather_bounds<<<gridFor((size_t)nv), SAP_BLOCK, 0, s>>>(b->V.bnd, nc, b->pcol, b->pcount, nv, b->outbnd);
          HIPCHK(hipGetLastError());
          RCCHK(download(b, h, b->outbnd, sizeof(double2) * (size_t)nv));
