@@ -2879,12 +2879,11 @@ __global__ void k_reason_reset(Vec V, int64_t ncols)
 // ------------------------------------------------------------------------------------------
 // one workgroup of SAP_STAT_SLOTS threads: folds the previous sweep's timestamps, then
 // (thread 0) advances the round or declares the call done
-__global__ __launch_bounds__(SAP_STAT_SLOTS) void k_round_begin(Vec V, int32_t maxrounds)
+// (thread 0) advance the round or declare the call done
+__device__ __forceinline__ void roundAdvance(const Vec& V, int32_t maxrounds)
 {
    Status* st = V.st;
-   const int t = (int)threadIdx.x;
-   foldStamps(st);
-   if( t != 0 || st->done )
+   if( threadIdx.x != 0 || st->done )
       return;
    if( stopsNext(*st, V.dense != 0, maxrounds) )
    {
@@ -2901,10 +2900,19 @@ __global__ __launch_bounds__(SAP_STAT_SLOTS) void k_round_begin(Vec V, int32_t m
    st->touched = 0;
 }
 
+__global__ __launch_bounds__(SAP_STAT_SLOTS) void k_round_begin(Vec V, int32_t maxrounds)
+{
+   foldStamps(V.st);
+   roundAdvance(V, maxrounds);
+}
+
 // per-call reset of the vector state; status counters for the call.  gen (1..255) is the
 // call's generation of the locally-redundant row flags (redGen); the call that starts
-// generation 1 clears the first nclear flags (those of every vector) first
-__global__ void k_call_reset(Vec V, size_t nclear, int32_t nvec, int32_t firstall, int32_t gen)
+// generation 1 clears the first nclear flags (those of every vector) first.  With begin set
+// (a launch of one workgroup, nothing enqueued between it and the first round) it also does
+// the first round's k_round_begin, whose stamps were just cleared, saving that launch.
+__global__ void k_call_reset(Vec V, size_t nclear, int32_t nvec, int32_t firstall, int32_t gen, int32_t begin,
+   int32_t maxrounds)
 {
    for( size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nclear; i += (size_t)gridDim.x * blockDim.x )
       V.redundant[i] = 0;
@@ -2962,6 +2970,11 @@ __global__ void k_call_reset(Vec V, size_t nclear, int32_t nvec, int32_t firstal
    {
       V.st->sweep_ticks = 0;
       V.st->sweep_count = 0;
+   }
+   if( begin )
+   {
+      __syncthreads();
+      roundAdvance(V, maxrounds);
    }
 }
 
@@ -3629,6 +3642,7 @@ struct sap_bounds
    size_t hdncap;
    // vectors 0..logmax-1 may hold change-log bits that no collection has cleared yet
    int32_t logmax;
+   int32_t begun;   // the last beginCall also began the call's first round
 };
 
 // the fields of the vector state that one call narrows (the vectors it works on, the
@@ -4910,14 +4924,16 @@ static void fillStats(sap_context* c, double tot, double sweepms, int64_t launch
    stats->nrecompute = (int64_t)h->nrecomp;
 }
 
-// enqueue one round on stream s: control, sweep (between two events when given), column step
+// enqueue one round on stream s: control (unless the call's reset already began this round),
+// sweep (between two events when given), column step
 static void enqueueRound(sap_bounds* b, const Mat& M, const Params& P, const Layout& L, hipStream_t s, hipEvent_t e0,
-   hipEvent_t e1)
+   hipEvent_t e1, bool begun = false)
 {
    sap_matrix* m = b->mat;
    const int nsweep = L.blockstart[SAP_NKINDS];
    const unsigned colgrid = gridFor((size_t)b->V.nvec * (size_t)m->ncols, 4096);
-   k_round_begin<<<1, SAP_STAT_SLOTS, 0, s>>>(b->V, P.maxrounds);
+   if( !begun )
+      k_round_begin<<<1, SAP_STAT_SLOTS, 0, s>>>(b->V, P.maxrounds);
    if( (P.flags & SAP_FLAG_RANGEDROWS) && m->nranged > 0 && b->V.reasons )
       k_ranged<true><<<gridFor((size_t)b->V.nvec * (size_t)m->nranged, 4096), SAP_BLOCK, 0, s>>>(M, b->V, P);
    else if( (P.flags & SAP_FLAG_RANGEDROWS) && m->nranged > 0 )
@@ -4970,7 +4986,8 @@ static int runRoundsHost(sap_bounds* b, const Params& P, sap_stats* stats)
          fprintf(stderr, "[scip_amd] round cap %d exceeded\n", SAP_ROUND_CAP);
          return SAP_ERROR;
       }
-      enqueueRound(b, M, P, L, s, c->ev[2], c->ev[3]);
+      enqueueRound(b, M, P, L, s, c->ev[2], c->ev[3], it == 0 && b->begun);
+      b->begun = 0;
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(c->hstatus, b->dstatus, sizeof(Status), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
@@ -5088,7 +5105,8 @@ static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats)
       {
          const Mat M0 = toMat(b->mat);
          const Layout L0 = makeLayout(b);
-         enqueueRound(b, M0, P, L0, s, NULL, NULL);
+         enqueueRound(b, M0, P, L0, s, NULL, NULL, b->begun != 0);
+         b->begun = 0;
          HIPCHK(hipGetLastError());
       }
       else
@@ -5126,8 +5144,11 @@ static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats)
 }
 
 // prepare a call on the first nvec vectors: reset per-call state; with firstall the first
-// round sweeps every row (dense mode), else rows are queued explicitly
-static int beginCall(sap_bounds* b, int32_t nvec, int32_t firstall)
+// round sweeps every row (dense mode), else rows are queued explicitly.  maxrounds >= -1:
+// the caller enqueues nothing before the first round, so when the reset is one workgroup it
+// also begins that round (b->begun), and the first round is then enqueued without its
+// control kernel
+static int beginCall(sap_bounds* b, int32_t nvec, int32_t firstall, int32_t maxrounds = -2)
 {
    sap_matrix* m = b->mat;
    hipStream_t s = m->ctx->stream;
@@ -5142,9 +5163,13 @@ static int beginCall(sap_bounds* b, int32_t nvec, int32_t firstall)
       work = (size_t)nvec;
    if( work < SAP_STAT_SLOTS )
       work = SAP_STAT_SLOTS;
-   k_call_reset<<<gridFor(work), SAP_BLOCK, 0, s>>>(b->V, nclear, nvec, firstall, gen);
+   const unsigned grid = gridFor(work);
+   const int32_t begin = (maxrounds >= -1 && grid == 1) ? 1 : 0;
+   static_assert(SAP_BLOCK == SAP_STAT_SLOTS, "one workgroup of k_call_reset covers the stat slots");
+   k_call_reset<<<grid, SAP_BLOCK, 0, s>>>(b->V, nclear, nvec, firstall, gen, begin, maxrounds);
    HIPCHK(hipGetLastError());
    b->redgen = gen;
+   b->begun = begin;
    return SAP_OKAY;
 }
 
@@ -5198,7 +5223,7 @@ int sap_propagate_resident(sap_bounds* b, const sap_params* params, sap_stats* s
    const CallScope saved(b->V);
    b->V.nvec = 1;
    b->V.single = (m->nseg == 1) ? 1 : 0;   // packed instances: a cutoff stops only its instance
-   int rc = beginCall(b, 1, b->V.dense);
+   int rc = beginCall(b, 1, b->V.dense, b->V.dense ? P.maxrounds : -2);
    if( rc == SAP_OKAY && !b->V.dense )
       rc = queueAllRows(b, 1);
    if( rc == SAP_OKAY )
