@@ -2,7 +2,8 @@
 //
 // Device algorithm (one propagation round, all bound vectors of a sap_bounds at once):
 //   k_round_begin : advance the round id, stop when the worklists are empty / cutoff /
-//                   maxrounds, reset the next round's counters.
+//                   maxrounds, reset the next round's counters (in a plain call the call's
+//                   reset kernel does this for the first round).
 //   k_sweep       : one launch over all row-length bins.  Every worklist item (row, vector)
 //                   computes the row's min/max activity (canonical pairwise double-double
 //                   sum over row positions), infinity/huge counters and maxactdelta from the

@@ -49,7 +49,7 @@ struct Problem
    std::vector<int32_t> freeslots;
    int32_t running = 0;
    bool inflight = false;
-   // upload by the loader thread: reserved and handed over, not yet collected
+   // upload by a loader thread: reserved and handed over, not yet collected
    bool loading = false;
    int loadrc = SAP_OKAY;
    sap_params lparams;            // parameters of the root propagation
@@ -71,7 +71,7 @@ struct Request
    bool cold = false;   // its problem was uploaded while it waited
 };
 
-// an evicted problem's device objects, freed by the loader thread (freeing waits for the
+// an evicted problem's device objects, freed by a loader thread (freeing waits for the
 // device, which the scheduling loop must not)
 struct Dead
 {
@@ -102,7 +102,7 @@ struct sap_server
    std::condition_variable cv;
    std::deque<Problem*> jobs;
    std::deque<Problem*> loaded;
-   std::deque<Dead> frees;   // evicted problems' objects, freed by the loader
+   std::deque<Dead> frees;   // evicted problems' objects, freed by the loaders
    bool stop = false;
    int32_t nloading = 0;   // submitted and not yet collected (scheduling thread only)
 };
@@ -142,7 +142,7 @@ void freeDead(const Dead& d)
       sap_matrix_free(d.mat);
 }
 
-// releases p's device objects: they go to the loader thread, which frees them
+// releases p's device objects: they go to the loader threads, which free them
 void evict(sap_server* s, Problem* p)
 {
    Dead d = { p->mat, p->root, p->pool };
@@ -170,7 +170,7 @@ Problem* lruVictim(sap_server* s)
    return victim;
 }
 
-// builds p's matrix, root fixpoint and slot pool on context c (the loader's); on failure
+// builds p's matrix, root fixpoint and slot pool on context c (a loader's); on failure
 // nothing is left allocated
 int buildProblem(Problem* p, sap_context* c)
 {
@@ -299,7 +299,7 @@ int handOver(sap_server* s, Problem* p)
    return SAP_OKAY;
 }
 
-// starts an upload of p on the loader thread: 0 submitted or waiting for room, < 0 error
+// starts an upload of p on a loader thread: 0 submitted or waiting for room, < 0 error
 int submitLoad(sap_server* s, Problem* p, const sap_params* params)
 {
    const int64_t need = footprint(p, p->slots);
@@ -320,7 +320,7 @@ int submitLoad(sap_server* s, Problem* p, const sap_params* params)
    return 0;
 }
 
-// hands over every upload the loader has finished; the first error is returned
+// hands over every upload the loaders have finished; the first error is returned
 int collectLoads(sap_server* s)
 {
    std::deque<Problem*> fin;
@@ -339,7 +339,7 @@ int collectLoads(sap_server* s)
    return rc;
 }
 
-// waits until the loader has finished every submitted upload and hands them over
+// waits until the loaders have finished every submitted upload and hands them over
 int drainLoads(sap_server* s)
 {
    int rc = SAP_OKAY;
@@ -355,7 +355,8 @@ int drainLoads(sap_server* s)
 }
 
 // 1: resident (uploaded now if it was not); 0: does not fit now; < 0 SAP error code.  The
-// upload runs on this thread (the loader has no work at the times this is called).
+// upload runs on this thread, on the first loader's context (the loaders have no work at the
+// times this is called).
 int makeResident(sap_server* s, Problem* p, const sap_params* params)
 {
    if( p->mat != NULL )
@@ -713,7 +714,7 @@ int sap_server_replay(sap_server* s, const sap_params* params, int32_t mode, int
       // between these checks, so no pool waits for the steps of the others
       while( s->ndone < nreq && rc == SAP_OKAY )
       {
-         // problems the loader has finished join their serving contexts
+         // problems the loaders have finished join their serving contexts
          rc = collectLoads(s);
          if( rc != SAP_OKAY )
             break;
