@@ -5100,7 +5100,9 @@ static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats)
          return SAP_ERROR;
       }
       // the first group's one round is enqueued directly (measured 0.5-1.5 % faster per call
-      // than launching a one-round graph); then graphs of 2, 4 and 8 rounds
+      // than launching a one-round graph); then graphs of 2, 4 and 8 rounds.  A call on several
+      // vectors rarely ends after its first round: its two-round graph follows without a wait
+      // (rounds after the end cost only their early exits)
       static const int kSchedule[3] = { 0, 2, 1 };   // 2, 4, 8 rounds
       if( it == 0 )
       {
@@ -5109,6 +5111,11 @@ static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats)
          enqueueRound(b, M0, P, L0, s, NULL, NULL, b->begun != 0);
          b->begun = 0;
          HIPCHK(hipGetLastError());
+         if( b->V.nvec > 1 )
+         {
+            HIPCHK(hipGraphLaunch(b->graph[kSchedule[0]].exec, s));
+            it = 1;
+         }
       }
       else
          HIPCHK(hipGraphLaunch(b->graph[kSchedule[it < 3 ? it - 1 : 2]].exec, s));
@@ -6016,6 +6023,9 @@ int sap_propagate_nodes(sap_bounds* b, const sap_params* params, int32_t nvec, c
    return SAP_OKAY;
 }
 
+// probing: pair lists up to this many entries are read back together with their counts
+#define SAP_PROBE_ONEREAD 4096
+
 // probing buffers: probe columns and counters (nvec each), pair outputs for n entries
 static int ensureProbe(sap_bounds* b, int64_t n)
 {
@@ -6106,7 +6116,7 @@ static int probeCore(sap_bounds* b, const sap_params* params, int32_t nprobe, co
    memset(&acc, 0, sizeof(acc));
    int32_t anyboth = 0, anyinfo = 0;
    std::vector<int64_t> ovptr, off;
-   std::vector<int32_t> ovcol, ident, res, rnd, logn, cut, pc;
+   std::vector<int32_t> ovcol, logn, cut, vr, vs, pc;
    std::vector<double> ovlb, ovub;
    for( int32_t p0 = 0; p0 < nprobe; p0 += per )
    {
@@ -6141,8 +6151,9 @@ static int probeCore(sap_bounds* b, const sap_params* params, int32_t nprobe, co
       acc.device_ms += cs.device_ms;
       acc.sweep_ms += cs.sweep_ms;
       acc.sweep_launches += cs.sweep_launches;
-      // room for the columns changed in both branches: at most the shorter log per probe
-      RCCHK(fetchMeta(b, nv, logn, cut, NULL, NULL));
+      // room for the columns changed in both branches: at most the shorter log per probe; the
+      // same read gives the verdicts and rounds
+      RCCHK(fetchMeta(b, nv, logn, cut, &vr, &vs));
       off.assign((size_t)np + 1, 0);
       int32_t maxs = 0;
       for( int32_t p = 0; p < np; ++p )
@@ -6154,11 +6165,19 @@ static int probeCore(sap_bounds* b, const sap_params* params, int32_t nprobe, co
       }
       const int64_t room = off[(size_t)np];
       pc.assign((size_t)np, 0);
+      std::vector<int32_t> tc((size_t)room);
+      std::vector<double2> t0((size_t)room), t1((size_t)room);
       if( room > 0 )
       {
          RCCHK(ensureProbe(b, room));
+         // pair lists of at most SAP_PROBE_ONEREAD entries come back with the counts in one
+         // read; longer ones only once the counts show that some pair was found
+         const bool one = room <= SAP_PROBE_ONEREAD;
+         const size_t pb = (sizeof(int32_t) * (size_t)np + 255) & ~(size_t)255;
+         const size_t cb = (sizeof(int32_t) * (size_t)room + 255) & ~(size_t)255;
+         const size_t bb = (sizeof(double2) * (size_t)room + 255) & ~(size_t)255;
          char* h = NULL;
-         RCCHK(downloadArea(b, sizeof(int32_t) * (size_t)np, &h));
+         RCCHK(downloadArea(b, pb + (one ? cb + 2 * bb : 0), &h));
          const Upload lists[2] = { { b->pcol, probecol + p0, sizeof(int32_t) * (size_t)np },
             { b->outoff, off.data(), sizeof(int64_t) * (size_t)np } };
          RCCHK(uploadMany(b, lists, 2));
@@ -6167,27 +6186,34 @@ static int probeCore(sap_bounds* b, const sap_params* params, int32_t nprobe, co
             b->outcol, b->outbnd, b->outbnd2);
          HIPCHK(hipGetLastError());
          RCCHK(download(b, h, b->pcount, sizeof(int32_t) * (size_t)np));
+         if( one )
+         {
+            RCCHK(download(b, h + pb, b->outcol, sizeof(int32_t) * (size_t)room));
+            RCCHK(download(b, h + pb + cb, b->outbnd, sizeof(double2) * (size_t)room));
+            RCCHK(download(b, h + pb + cb + bb, b->outbnd2, sizeof(double2) * (size_t)room));
+         }
          HIPCHK(hipStreamSynchronize(s));
          memcpy(pc.data(), h, sizeof(int32_t) * (size_t)np);
-      }
-      int64_t got = 0;
-      for( int32_t p = 0; p < np; ++p )
-         got += pc[(size_t)p];
-      std::vector<int32_t> tc((size_t)room);
-      std::vector<double2> t0((size_t)room), t1((size_t)room);
-      if( got > 0 )
-      {
-         const size_t cb = (sizeof(int32_t) * (size_t)room + 255) & ~(size_t)255;
-         const size_t bb = (sizeof(double2) * (size_t)room + 255) & ~(size_t)255;
-         char* h = NULL;
-         RCCHK(downloadArea(b, cb + 2 * bb, &h));
-         RCCHK(download(b, h, b->outcol, sizeof(int32_t) * (size_t)room));
-         RCCHK(download(b, h + cb, b->outbnd, sizeof(double2) * (size_t)room));
-         RCCHK(download(b, h + cb + bb, b->outbnd2, sizeof(double2) * (size_t)room));
-         HIPCHK(hipStreamSynchronize(s));
-         memcpy(tc.data(), h, sizeof(int32_t) * (size_t)room);
-         memcpy(t0.data(), h + cb, sizeof(double2) * (size_t)room);
-         memcpy(t1.data(), h + cb + bb, sizeof(double2) * (size_t)room);
+         int64_t got = 0;
+         for( int32_t p = 0; p < np; ++p )
+            got += pc[(size_t)p];
+         if( got > 0 && !one )
+         {
+            RCCHK(downloadArea(b, cb + 2 * bb, &h));
+            RCCHK(download(b, h, b->outcol, sizeof(int32_t) * (size_t)room));
+            RCCHK(download(b, h + cb, b->outbnd, sizeof(double2) * (size_t)room));
+            RCCHK(download(b, h + cb + bb, b->outbnd2, sizeof(double2) * (size_t)room));
+            HIPCHK(hipStreamSynchronize(s));
+            memcpy(tc.data(), h, sizeof(int32_t) * (size_t)room);
+            memcpy(t0.data(), h + cb, sizeof(double2) * (size_t)room);
+            memcpy(t1.data(), h + cb + bb, sizeof(double2) * (size_t)room);
+         }
+         else if( got > 0 )
+         {
+            memcpy(tc.data(), h + pb, sizeof(int32_t) * (size_t)room);
+            memcpy(t0.data(), h + pb + cb, sizeof(double2) * (size_t)room);
+            memcpy(t1.data(), h + pb + cb + bb, sizeof(double2) * (size_t)room);
+         }
       }
       // split probing: the probe columns' own bounds in both branches
       std::vector<double2> x0, x1;
@@ -6211,25 +6237,31 @@ static int probeCore(sap_bounds* b, const sap_params* params, int32_t nprobe, co
          HIPCHK(hipStreamSynchronize(s));
          memcpy(x0.data(), h, sizeof(double2) * (size_t)nv);
       }
-      // verdicts and rounds; the change logs are cleared for the next chunk
-      ident.resize((size_t)nv);
-      res.resize((size_t)nv);
-      rnd.resize((size_t)nv);
+      // the change logs of the chunk's vectors are cleared for the next chunk (after the pair
+      // kernel, in stream order; nothing to wait for)
+      int32_t maxlog = 0;
       for( int32_t v = 0; v < nv; ++v )
-         ident[(size_t)v] = v;
-      RCCHK(collectSlots(b, nv, ident.data(), res.data(), rnd.data(), 0, NULL, NULL, NULL, NULL));
+         maxlog = std::max(maxlog, logn[(size_t)v]);
+      if( maxlog > 0 )
+      {
+         k_collect<<<slotGrid((size_t)maxlog, nv), SAP_BLOCK, 0, s>>>(b->V, nc, b->ident, nv, b->outoff, NULL, NULL);
+         HIPCHK(hipGetLastError());
+      }
+      if( b->logmax <= nv )
+         b->logmax = 0;
+      // verdicts (a cut-off branch is infeasible) and rounds, as a collection would report them
       std::vector<int64_t> idx;
       for( int32_t p = 0; p < np; ++p )
       {
          const int32_t q = p0 + p;
-         verdict[q] = (res[(size_t)(2 * p)] == SAP_CUTOFF ? SAP_PROBE_DOWN_INFEASIBLE : 0)
-            | (res[(size_t)(2 * p + 1)] == SAP_CUTOFF ? SAP_PROBE_UP_INFEASIBLE : 0);
+         const size_t v0 = (size_t)(2 * p), v1 = v0 + 1;
+         verdict[q] = (cut[v0] ? SAP_PROBE_DOWN_INFEASIBLE : 0) | (cut[v1] ? SAP_PROBE_UP_INFEASIBLE : 0);
          anyboth |= verdict[q] == (SAP_PROBE_DOWN_INFEASIBLE | SAP_PROBE_UP_INFEASIBLE);
          anyinfo |= verdict[q] != 0 || pc[(size_t)p] > 0;
          if( rounds )
          {
-            rounds[2 * (size_t)q] = rnd[(size_t)(2 * p)];
-            rounds[2 * (size_t)q + 1] = rnd[(size_t)(2 * p + 1)];
+            rounds[2 * (size_t)q] = vr[v0] > 0 ? vr[v0] - vs[v0] + 1 : 0;
+            rounds[2 * (size_t)q + 1] = vr[v1] > 0 ? vr[v1] - vs[v1] + 1 : 0;
          }
          // the order within a probe depends on atomics: sort by column
          const int64_t a = off[(size_t)p];

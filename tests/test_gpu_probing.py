@@ -9,7 +9,7 @@ import pytest
 
 import scip_amd as sa
 from oracle import oracle as orc
-from tests.util import random_binary_instance
+from tests.util import INF, make_instance, random_binary_instance
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -89,6 +89,34 @@ def test_probe_base_inside_the_probe_set_and_capacity(gpu_ctx):
     for k in ("verdict", "dedptr", "dedcol", "dlb0", "dub0", "dlb1", "dub1"):
         assert np.array_equal(out[k], out2[k]), k
     b.close()
+    mat.close()
+
+
+def test_probe_long_pair_lists(gpu_ctx):
+    # x + y = 1 written as two rows for every column y of a probe's fan: each branch of probe 0
+    # fixes its 5,000 fan columns, of probe 1 its ten, so the chunk's pair lists (5,010 entries)
+    # are longer than those read back together with their counts
+    k0, k1 = 5000, 10
+    rows = []
+    for k in range(k0 + k1):
+        x = 0 if k < k0 else 1
+        rows.append(([(x, 1.0), (2 + k, 1.0)], 1.0, INF))
+        rows.append(([(x, 1.0), (2 + k, 1.0)], -INF, 1.0))
+    n = 2 + k0 + k1
+    inst = make_instance(rows, [0.0] * n, [1.0] * n, [1] * n)
+    om = orc.OMatrix.from_instance(inst)
+    probecols = np.array([0, 1], dtype=np.int32)
+    say(f"[case] probe long pair lists, fans {k0} and {k1}")
+    mat = sa.Matrix(gpu_ctx, inst)
+    base = sa.Bounds(mat, 1)
+    base.upload(inst.lb, inst.ub)
+    b = sa.Bounds(mat, 4)
+    out = b.probe(base, 0, probecols, sa.default_params())
+    ref = check_against_oracle(om, inst.lb, inst.ub, probecols, out, check_rounds=True)
+    assert [len(r["cols"]) for r in ref] == [k0, k1]
+    assert list(out["verdict"]) == [0, 0]
+    b.close()
+    base.close()
     mat.close()
 
 
