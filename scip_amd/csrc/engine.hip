@@ -5081,11 +5081,23 @@ static bool ensureGraphs(sap_bounds* b, const Params& P)
 // round already decides whether another round would run, so a call ends right after the
 // group of its last round; growing groups bound both the synchronisations (log2 of the rounds)
 // and the rounds launched after the end (fewer than the rounds run).
-static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats)
+// metan > 0: the outcomes of vectors 0..metan-1 (k_pack_meta) are also in b's landing area,
+// whose room the caller has made, when it returns, read by the call's last wait
+static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats, int32_t metan = 0)
 {
    RCCHK(ensureRanged(b, P));
    if( (P.flags & SAP_FLAG_HOSTLOOP) || b->graphbroken || !ensureGraphs(b, P) )
-      return runRoundsHost(b, P, stats);
+   {
+      RCCHK(runRoundsHost(b, P, stats));
+      if( metan > 0 )
+      {
+         hipStream_t s = b->mat->ctx->stream;
+         k_pack_meta<<<gridFor((size_t)metan), SAP_BLOCK, 0, s>>>(b->V, metan, (int32_t*)b->hdnd);
+         HIPCHK(hipGetLastError());
+         HIPCHK(hipStreamSynchronize(s));
+      }
+      return SAP_OKAY;
+   }
    sap_context* c = b->mat->ctx;
    hipStream_t s = c->stream;
    double sweepms = 0.0;
@@ -5122,6 +5134,8 @@ static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats)
       // the status goes to mapped host memory by a copy kernel on the stream
       k_copy_words<<<gridFor(sizeof(Status) / 4), SAP_BLOCK, 0, s>>>((uint32_t*)c->hstatusd, (const uint32_t*)b->dstatus,
          sizeof(Status) / 4);
+      if( metan > 0 )
+         k_pack_meta<<<gridFor((size_t)metan), SAP_BLOCK, 0, s>>>(b->V, metan, (int32_t*)b->hdnd);
       HIPCHK(hipGetLastError());
       // the end event of the call rides on every group's synchronisation, so that the last
       // one needs no further round trip
@@ -5603,6 +5617,19 @@ static int download(sap_bounds* b, char* h, const void* dsrc, size_t bytes)
 
 // change-log length, cutoff flag, last round and start round of vectors 0..n-1 (one instance),
 // in one download
+// the outcomes k_pack_meta left in the landing area (vectors 0..n-1)
+static void readMeta(const sap_bounds* b, int32_t n, std::vector<int32_t>& logn, std::vector<int32_t>& cut,
+   std::vector<int32_t>* vr, std::vector<int32_t>* vs)
+{
+   const int32_t* p = (const int32_t*)b->hdn;
+   logn.assign(p, p + n);
+   cut.assign(p + n, p + 2 * (size_t)n);
+   if( vr != NULL )
+      vr->assign(p + 2 * (size_t)n, p + 3 * (size_t)n);
+   if( vs != NULL )
+      vs->assign(p + 3 * (size_t)n, p + 4 * (size_t)n);
+}
+
 static int fetchMeta(sap_bounds* b, int32_t n, std::vector<int32_t>& logn, std::vector<int32_t>& cut,
    std::vector<int32_t>* vr, std::vector<int32_t>* vs)
 {
@@ -5612,13 +5639,7 @@ static int fetchMeta(sap_bounds* b, int32_t n, std::vector<int32_t>& logn, std::
    k_pack_meta<<<gridFor((size_t)n), SAP_BLOCK, 0, s>>>(b->V, n, (int32_t*)b->hdnd);
    HIPCHK(hipGetLastError());
    HIPCHK(hipStreamSynchronize(s));
-   const int32_t* p = (const int32_t*)h;
-   logn.assign(p, p + n);
-   cut.assign(p + n, p + 2 * (size_t)n);
-   if( vr != NULL )
-      vr->assign(p + 2 * (size_t)n, p + 3 * (size_t)n);
-   if( vs != NULL )
-      vs->assign(p + 3 * (size_t)n, p + 4 * (size_t)n);
+   readMeta(b, n, logn, cut, vr, vs);
    return SAP_OKAY;
 }
 
@@ -5740,13 +5761,21 @@ static int admitSlots(sap_bounds* b, int32_t n, const int32_t* hslots, const int
 
 // per-slot verdicts and sorted change lists of slots hslots[0..n); clears their change logs.
 // With chgcol == NULL only verdicts (and chgptr counts, when given) are produced.
+// premeta > 0: the outcomes of vectors 0..premeta-1, which hold every listed slot, are already
+// in the landing area (batchCall with meta)
 static int collectSlots(sap_bounds* b, int32_t n, const int32_t* hslots, int32_t* result, int32_t* rounds,
-   int64_t chgcap, int64_t* chgptr, int32_t* chgcol, double* chglb, double* chgub)
+   int64_t chgcap, int64_t* chgptr, int32_t* chgcol, double* chglb, double* chgub, int32_t premeta = 0)
 {
    sap_matrix* m = b->mat;
    hipStream_t s = m->ctx->stream;
    std::vector<int32_t> logn, cut, vr, vs;
-   RCCHK(fetchMeta(b, b->nvec, logn, cut, &vr, &vs));
+   bool have = premeta > 0;
+   for( int32_t i = 0; have && i < n; ++i )
+      have = hslots[i] >= 0 && hslots[i] < premeta;
+   if( have )
+      readMeta(b, premeta, logn, cut, &vr, &vs);
+   else
+      RCCHK(fetchMeta(b, b->nvec, logn, cut, &vr, &vs));
    // output offsets: a cut-off slot has an empty list
    std::vector<int64_t> off((size_t)n + 1, 0);
    int32_t maxlog = 0;
@@ -5904,7 +5933,7 @@ static int clearLogs(sap_bounds* b)
 
 // one-shot propagation of vectors 0..nvec-1 from a device base vector plus the staged overlays
 static int batchCall(sap_bounds* b, const sap_params* params, int32_t nvec, const sap_bounds* B, int32_t bv,
-   const double2* dbase, int64_t nov, sap_stats* stats)
+   const double2* dbase, int64_t nov, sap_stats* stats, bool meta = false)
 {
    sap_matrix* m = b->mat;
    hipStream_t s = m->ctx->stream;
@@ -5939,8 +5968,13 @@ static int batchCall(sap_bounds* b, const sap_params* params, int32_t nvec, cons
    }
    if( rc == SAP_OKAY && !fromfix && !b->V.dense )
       rc = queueAllRows(b, nvec);
+   // meta: the vectors' outcomes come back with the last round group (readMeta), saving the
+   // caller a wait
+   char* h = NULL;
+   if( rc == SAP_OKAY && meta )
+      rc = downloadArea(b, 16 * (size_t)nvec, &h);
    if( rc == SAP_OKAY )
-      rc = runRounds(b, P, stats);
+      rc = runRounds(b, P, stats, meta ? nvec : 0);
    saved.restore(b->V);
    return rc;
 }
@@ -5965,11 +5999,11 @@ int sap_propagate_batch(sap_bounds* b, const sap_params* params, int32_t nvec, c
    HIPCHK(hipMemcpyAsync(b->stage + nc, base_ub, sizeof(double) * (size_t)nc, hipMemcpyHostToDevice, s));
    k_interleave<<<gridFor((size_t)nc), SAP_BLOCK, 0, s>>>(b->base2, b->stage, b->stage + nc, nc);
    HIPCHK(hipGetLastError());
-   RCCHK(batchCall(b, params, nvec, NULL, 0, b->base2, nov, stats));
+   RCCHK(batchCall(b, params, nvec, NULL, 0, b->base2, nov, stats, true));
    std::vector<int32_t> ident((size_t)nvec), res((size_t)nvec);
    for( int32_t v = 0; v < nvec; ++v )
       ident[(size_t)v] = v;
-   RCCHK(collectSlots(b, nvec, ident.data(), res.data(), rounds, 0, NULL, NULL, NULL, NULL));
+   RCCHK(collectSlots(b, nvec, ident.data(), res.data(), rounds, 0, NULL, NULL, NULL, NULL, nvec));
    if( result )
       memcpy(result, res.data(), sizeof(int32_t) * (size_t)nvec);
    if( out_lb != NULL && out_ub != NULL )
@@ -6006,11 +6040,11 @@ int sap_propagate_nodes(sap_bounds* b, const sap_params* params, int32_t nvec, c
          m->ctx->stream));
       dbase = b->base2;
    }
-   RCCHK(batchCall(b, params, nvec, base == b ? NULL : base, basevec, dbase, nov, stats));
+   RCCHK(batchCall(b, params, nvec, base == b ? NULL : base, basevec, dbase, nov, stats, true));
    std::vector<int32_t> ident((size_t)nvec), res((size_t)nvec);
    for( int32_t v = 0; v < nvec; ++v )
       ident[(size_t)v] = v;
-   RCCHK(collectSlots(b, nvec, ident.data(), res.data(), rounds, chgcap, chgptr, chgcol, chglb, chgub));
+   RCCHK(collectSlots(b, nvec, ident.data(), res.data(), rounds, chgcap, chgptr, chgcol, chglb, chgub, nvec));
    if( result )
       memcpy(result, res.data(), sizeof(int32_t) * (size_t)nvec);
    if( stats )
@@ -6143,7 +6177,7 @@ static int probeCore(sap_bounds* b, const sap_params* params, int32_t nprobe, co
       RCCHK(stageOverlay(b, nv, ovptr.data(), ovcol.data(), ovlb.data(), ovub.data(), &nov));
       sap_stats cs;
       memset(&cs, 0, sizeof(cs));
-      RCCHK(batchCall(b, &prm, nv, base == b ? NULL : base, basevec, dbase, nov, &cs));
+      RCCHK(batchCall(b, &prm, nv, base == b ? NULL : base, basevec, dbase, nov, &cs, true));
       acc.rounds = std::max(acc.rounds, cs.rounds);
       acc.nchgbds += cs.nchgbds;
       acc.rows_swept += cs.rows_swept;
@@ -6152,8 +6186,8 @@ static int probeCore(sap_bounds* b, const sap_params* params, int32_t nprobe, co
       acc.sweep_ms += cs.sweep_ms;
       acc.sweep_launches += cs.sweep_launches;
       // room for the columns changed in both branches: at most the shorter log per probe; the
-      // same read gives the verdicts and rounds
-      RCCHK(fetchMeta(b, nv, logn, cut, &vr, &vs));
+      // same outcomes, read back with the last round group, give the verdicts and rounds
+      readMeta(b, nv, logn, cut, &vr, &vs);
       off.assign((size_t)np + 1, 0);
       int32_t maxs = 0;
       for( int32_t p = 0; p < np; ++p )
