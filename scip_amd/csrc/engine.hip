@@ -3274,6 +3274,73 @@ __global__ void k_step_begin(Vec V)
    }
 }
 
+// batched call whose change lists come back with its last wait: after every round group, once
+// the call has ended (its done flag, or the stop test the host applies after the group), the
+// lists of vectors 0..n-1 are numbered into outoff (a cut-off vector's list is empty) and
+// outoff[n] and hdr[0] (mapped host memory) receive their total; -1 while the call goes on.
+// One workgroup.
+__global__ __launch_bounds__(SAP_BLOCK) void k_list_offsets(Vec V, int32_t n, int32_t maxrounds, int64_t* outoff,
+   int64_t* hdr)
+{
+   const Status* st = V.st;
+   const bool ended = st->done || stopsNext(*st, V.dense != 0, maxrounds);
+   __shared__ int64_t part[SAP_BLOCK];
+   const int t = (int)threadIdx.x;
+   int64_t carry = 0;
+   for( int32_t c0 = 0; ended && c0 < n; c0 += SAP_BLOCK )
+   {
+      const int32_t v = c0 + t;
+      const int64_t len = (v < n && V.vcutoff[v] == 0) ? (int64_t)V.vlogn[v] : 0;
+      part[t] = len;
+      __syncthreads();
+      for( int d = 1; d < SAP_BLOCK; d <<= 1 )
+      {
+         const int64_t add = t >= d ? part[t - d] : 0;
+         __syncthreads();
+         part[t] += add;
+         __syncthreads();
+      }
+      if( v < n )
+         outoff[v] = carry + part[t] - len;
+      carry += part[SAP_BLOCK - 1];
+      __syncthreads();   // part is rewritten by the next chunk
+   }
+   if( t == 0 )
+   {
+      outoff[n] = ended ? carry : -1;
+      hdr[0] = ended ? carry : -1;
+   }
+}
+
+// the lists k_list_offsets numbered, written to mapped host memory when their total fits cap
+// entries, and every logged column's bit cleared (a cut-off vector's too); nothing while the
+// call goes on or when the lists do not fit (the logs stay for a collection with room)
+__global__ void k_collect_ended(Vec V, int64_t ncols, int32_t n, const int64_t* outoff, int64_t cap, int32_t* outcol,
+   double2* outbnd)
+{
+   const int64_t tot = outoff[n];
+   if( tot < 0 || tot > cap )
+      return;
+   for( int32_t v = blockIdx.y; v < n; v += gridDim.y )
+   {
+      const int32_t cnt = V.vlogn[v];
+      const bool keep = V.vcutoff[v] == 0;
+      const int64_t o = outoff[v];
+      const int32_t* log = V.log + (size_t)v * (size_t)ncols;
+      for( int32_t q = blockIdx.x * (int32_t)blockDim.x + (int32_t)threadIdx.x; q < cnt; q += (int32_t)(gridDim.x * blockDim.x) )
+      {
+         const int32_t j = log[q];
+         const size_t flat = (size_t)v * (size_t)ncols + (size_t)j;
+         if( keep )
+         {
+            outcol[o + q] = j;
+            outbnd[o + q] = V.bnd[flat];
+         }
+         atomicAnd(&V.logbits[flat >> 5], ~(1u << (flat & 31)));
+      }
+   }
+}
+
 // changed columns of the listed slots with their final bounds, at outoff[i] for slot i; the
 // log bits of the collected entries are cleared for the slot's next admission
 __global__ void k_collect(Vec V, int64_t ncols, const int32_t* slots, int32_t n, const int64_t* outoff,
@@ -5081,9 +5148,24 @@ static bool ensureGraphs(sap_bounds* b, const Params& P)
 // round already decides whether another round would run, so a call ends right after the
 // group of its last round; growing groups bound both the synchronisations (log2 of the rounds)
 // and the rounds launched after the end (fewer than the rounds run).
+// node calls: change lists of up to this many entries in all come back with the call's last
+// wait, in a landing area of 20 B per entry (others are collected after it)
+#define SAP_LISTS_ONEREAD ((int64_t)1 << 18)
+
+// change lists of a batched call brought back by its last wait (k_list_offsets,
+// k_collect_ended): device pointers into b's landing area
+struct ListSpec
+{
+   int64_t cap;       // entries that fit
+   int64_t* hdr;      // total, or -1 (call not ended)
+   int32_t* col;      // cap columns
+   double2* bnd;      // cap (lb, ub)
+};
+
 // metan > 0: the outcomes of vectors 0..metan-1 (k_pack_meta) are also in b's landing area,
-// whose room the caller has made, when it returns, read by the call's last wait
-static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats, int32_t metan = 0)
+// whose room the caller has made, when it returns, read by the call's last wait; with ls also
+// their change lists
+static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats, int32_t metan = 0, const ListSpec* ls = NULL)
 {
    RCCHK(ensureRanged(b, P));
    if( (P.flags & SAP_FLAG_HOSTLOOP) || b->graphbroken || !ensureGraphs(b, P) )
@@ -5140,6 +5222,13 @@ static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats, int32_t m
       // the end event of the call rides on every group's synchronisation, so that the last
       // one needs no further round trip
       HIPCHK(hipEventRecord(c->ev[1], s));
+      if( ls != NULL && metan > 0 )
+      {
+         k_list_offsets<<<1, SAP_BLOCK, 0, s>>>(b->V, metan, P.maxrounds, b->outoff, ls->hdr);
+         k_collect_ended<<<dim3(1, (unsigned)std::min(metan, 4096)), SAP_BLOCK, 0, s>>>(b->V, b->mat->ncols, metan,
+            b->outoff, ls->cap, ls->col, ls->bnd);
+         HIPCHK(hipGetLastError());
+      }
       HIPCHK(hipStreamSynchronize(s));
       prev = c->hstatus->rlocal;
       if( c->hstatus->done )
@@ -5761,6 +5850,56 @@ static int admitSlots(sap_bounds* b, int32_t n, const int32_t* hslots, const int
 
 // per-slot verdicts and sorted change lists of slots hslots[0..n); clears their change logs.
 // With chgcol == NULL only verdicts (and chgptr counts, when given) are produced.
+// change lists [off[i], off[i+1]) as gathered (hc, hb) into the caller's arrays; the log order
+// depends on atomics, so each list is sorted by column (short lists, the common case, by
+// insertion into the output)
+static void emitLists(int32_t n, const int64_t* off, const int32_t* hc, const double2* hb, int32_t* chgcol,
+   double* chglb, double* chgub)
+{
+   std::vector<int64_t> idx;
+   for( int32_t i = 0; i < n; ++i )
+   {
+      const int64_t a = off[i], e = off[i + 1];
+      if( e - a <= 16 )
+      {
+         for( int64_t q = a; q < e; ++q )
+         {
+            const int32_t c = hc[(size_t)q];
+            const double2 bq = hb[(size_t)q];
+            int64_t w = q;
+            while( w > a && chgcol[w - 1] > c )
+            {
+               chgcol[w] = chgcol[w - 1];
+               if( chglb )
+                  chglb[w] = chglb[w - 1];
+               if( chgub )
+                  chgub[w] = chgub[w - 1];
+               --w;
+            }
+            chgcol[w] = c;
+            if( chglb )
+               chglb[w] = bq.x;
+            if( chgub )
+               chgub[w] = bq.y;
+         }
+         continue;
+      }
+      idx.resize((size_t)(e - a));
+      for( int64_t q = a; q < e; ++q )
+         idx[(size_t)(q - a)] = q;
+      std::sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) { return hc[(size_t)x] < hc[(size_t)y]; });
+      for( int64_t q = a; q < e; ++q )
+      {
+         const int64_t src = idx[(size_t)(q - a)];
+         chgcol[q] = hc[(size_t)src];
+         if( chglb )
+            chglb[q] = hb[(size_t)src].x;
+         if( chgub )
+            chgub[q] = hb[(size_t)src].y;
+      }
+   }
+}
+
 // premeta > 0: the outcomes of vectors 0..premeta-1, which hold every listed slot, are already
 // in the landing area (batchCall with meta)
 static int collectSlots(sap_bounds* b, int32_t n, const int32_t* hslots, int32_t* result, int32_t* rounds,
@@ -5861,50 +6000,7 @@ static int collectSlots(sap_bounds* b, int32_t n, const int32_t* hslots, int32_t
    HIPCHK(hipStreamSynchronize(s));
    if( allclear )
       b->logmax = 0;
-   // the log order depends on atomics: each slot's list is sorted by column (short lists, the
-   // common case, by insertion into the output)
-   std::vector<int64_t> idx;
-   for( int32_t i = 0; i < n; ++i )
-   {
-      const int64_t a = off[(size_t)i], e = off[(size_t)i + 1];
-      if( e - a <= 16 )
-      {
-         for( int64_t q = a; q < e; ++q )
-         {
-            const int32_t c = hc[(size_t)q];
-            const double2 bq = hb[(size_t)q];
-            int64_t w = q;
-            while( w > a && chgcol[w - 1] > c )
-            {
-               chgcol[w] = chgcol[w - 1];
-               if( chglb )
-                  chglb[w] = chglb[w - 1];
-               if( chgub )
-                  chgub[w] = chgub[w - 1];
-               --w;
-            }
-            chgcol[w] = c;
-            if( chglb )
-               chglb[w] = bq.x;
-            if( chgub )
-               chgub[w] = bq.y;
-         }
-         continue;
-      }
-      idx.resize((size_t)(e - a));
-      for( int64_t q = a; q < e; ++q )
-         idx[(size_t)(q - a)] = q;
-      std::sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) { return hc[(size_t)x] < hc[(size_t)y]; });
-      for( int64_t q = a; q < e; ++q )
-      {
-         const int64_t src = idx[(size_t)(q - a)];
-         chgcol[q] = hc[(size_t)src];
-         if( chglb )
-            chglb[q] = hb[(size_t)src].x;
-         if( chgub )
-            chgub[q] = hb[(size_t)src].y;
-      }
-   }
+   emitLists(n, off.data(), hc, hb, chgcol, chglb, chgub);
    return SAP_OKAY;
 }
 
@@ -5933,7 +6029,7 @@ static int clearLogs(sap_bounds* b)
 
 // one-shot propagation of vectors 0..nvec-1 from a device base vector plus the staged overlays
 static int batchCall(sap_bounds* b, const sap_params* params, int32_t nvec, const sap_bounds* B, int32_t bv,
-   const double2* dbase, int64_t nov, sap_stats* stats, bool meta = false)
+   const double2* dbase, int64_t nov, sap_stats* stats, bool meta = false, const ListSpec* ls = NULL)
 {
    sap_matrix* m = b->mat;
    hipStream_t s = m->ctx->stream;
@@ -5969,12 +6065,12 @@ static int batchCall(sap_bounds* b, const sap_params* params, int32_t nvec, cons
    if( rc == SAP_OKAY && !fromfix && !b->V.dense )
       rc = queueAllRows(b, nvec);
    // meta: the vectors' outcomes come back with the last round group (readMeta), saving the
-   // caller a wait
+   // caller a wait; ls: their change lists too (the caller has made the landing area's room)
    char* h = NULL;
-   if( rc == SAP_OKAY && meta )
+   if( rc == SAP_OKAY && meta && ls == NULL )
       rc = downloadArea(b, 16 * (size_t)nvec, &h);
    if( rc == SAP_OKAY )
-      rc = runRounds(b, P, stats, meta ? nvec : 0);
+      rc = runRounds(b, P, stats, meta ? nvec : 0, meta ? ls : NULL);
    saved.restore(b->V);
    return rc;
 }
@@ -6040,11 +6136,55 @@ int sap_propagate_nodes(sap_bounds* b, const sap_params* params, int32_t nvec, c
          m->ctx->stream));
       dbase = b->base2;
    }
-   RCCHK(batchCall(b, params, nvec, base == b ? NULL : base, basevec, dbase, nov, stats, true));
-   std::vector<int32_t> ident((size_t)nvec), res((size_t)nvec);
-   for( int32_t v = 0; v < nvec; ++v )
-      ident[(size_t)v] = v;
-   RCCHK(collectSlots(b, nvec, ident.data(), res.data(), rounds, chgcap, chgptr, chgcol, chglb, chgub, nvec));
+   // change lists that fit the caller's room (up to SAP_LISTS_ONEREAD entries) come back with
+   // the call's last wait: landing area = outcomes of every vector, total, columns, bounds
+   const size_t mb = (16 * (size_t)b->nvec + 255) & ~(size_t)255;
+   const size_t cb = (sizeof(int32_t) * (size_t)chgcap + 255) & ~(size_t)255;
+   const bool oneread = chgcol != NULL && chgcap > 0 && chgcap <= SAP_LISTS_ONEREAD;
+   ListSpec ls;
+   char* h = NULL;
+   if( oneread )
+   {
+      RCCHK(downloadArea(b, mb + 256 + cb + sizeof(double2) * (size_t)chgcap, &h));
+      ls.cap = chgcap;
+      ls.hdr = (int64_t*)(b->hdnd + mb);
+      ls.col = (int32_t*)(b->hdnd + mb + 256);
+      ls.bnd = (double2*)(b->hdnd + mb + 256 + cb);
+      *(volatile int64_t*)(h + mb) = -1;   // stays -1 unless the device numbered the lists
+   }
+   RCCHK(batchCall(b, params, nvec, base == b ? NULL : base, basevec, dbase, nov, stats, true,
+      oneread ? &ls : NULL));
+   std::vector<int32_t> res((size_t)nvec);
+   const int64_t tot = oneread ? *(volatile int64_t*)(b->hdn + mb) : -1;
+   if( oneread && tot >= 0 && tot <= chgcap )
+   {
+      // the lists are in the landing area and the logs cleared: outcomes and offsets as a
+      // collection gives them
+      std::vector<int32_t> logn, cut, vr, vs;
+      readMeta(b, nvec, logn, cut, &vr, &vs);
+      std::vector<int64_t> off((size_t)nvec + 1, 0);
+      for( int32_t v = 0; v < nvec; ++v )
+      {
+         off[(size_t)v + 1] = off[(size_t)v] + (cut[(size_t)v] ? 0 : logn[(size_t)v]);
+         res[(size_t)v] = cut[(size_t)v] ? SAP_CUTOFF : (logn[(size_t)v] > 0 ? SAP_REDUCEDDOM : SAP_DIDNOTFIND);
+         if( rounds )
+            rounds[v] = vr[(size_t)v] > 0 ? vr[(size_t)v] - vs[(size_t)v] + 1 : 0;
+      }
+      if( off[(size_t)nvec] != tot )
+         return SAP_ERROR;
+      memcpy(chgptr, off.data(), sizeof(int64_t) * ((size_t)nvec + 1));
+      if( b->logmax <= nvec )
+         b->logmax = 0;
+      emitLists(nvec, off.data(), (const int32_t*)(b->hdn + mb + 256), (const double2*)(b->hdn + mb + 256 + cb),
+         chgcol, chglb, chgub);
+   }
+   else
+   {
+      std::vector<int32_t> ident((size_t)nvec);
+      for( int32_t v = 0; v < nvec; ++v )
+         ident[(size_t)v] = v;
+      RCCHK(collectSlots(b, nvec, ident.data(), res.data(), rounds, chgcap, chgptr, chgcol, chglb, chgub, nvec));
+   }
    if( result )
       memcpy(result, res.data(), sizeof(int32_t) * (size_t)nvec);
    if( stats )
