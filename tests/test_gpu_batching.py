@@ -39,9 +39,12 @@ def apply_changes(lb, ub, out, v):
     return lb, ub
 
 
-def check_node_out(om, rl, ru, ptr, cols, vals, out, v, check_rounds):
+def check_node_out(om, rl, ru, ptr, cols, vals, out, v, check_rounds, oparams=None):
     lb0, ub0 = apply_fixings(rl, ru, ptr, cols, vals, v)
-    ol, ou, orr = orc.propagate_jacobi(om, lb0, ub0)
+    if oparams is None:
+        ol, ou, orr = orc.propagate_jacobi(om, lb0, ub0)
+    else:
+        ol, ou, orr = orc.propagate_jacobi(om, lb0, ub0, oparams)
     assert (out["result"][v] == sa.SAP_CUTOFF) == bool(orr["cutoff"]), v
     if orr["cutoff"]:
         assert out["chgptr"][v + 1] == out["chgptr"][v]
@@ -92,6 +95,30 @@ def test_nodes_from_resident_base(gpu_ctx, mode, flags):
     out4 = nodes.propagate_nodes(base, 0, p2, c2, v2, v2, sa.default_params(flags=0))
     for v in range(nvec):
         check_node_out(om, inst.lb, inst.ub, p2, c2, v2, out4, v, check_rounds=True)
+    nodes.close()
+    base.close()
+    mat.close()
+
+
+@pytest.mark.parametrize("mode", [sa.SAP_MODE_DENSE, sa.SAP_MODE_SPARSE])
+@pytest.mark.parametrize("maxrounds", [1, 2, 3])
+def test_nodes_maxrounds(gpu_ctx, mode, maxrounds):
+    # a batched call launches its first round and a two-round group before it first waits:
+    # the round cap still ends every vector after maxrounds rounds
+    inst, om, rl, ru, ptr, cols, vals = node_problem(15)
+    nvec = len(ptr) - 1
+    say(f"[case] nodes maxrounds={maxrounds} mode={mode} nvec={nvec}")
+    mat = sa.Matrix(gpu_ctx, inst)
+    base = sa.Bounds(mat, 1)
+    base.upload(rl, ru)
+    nodes = sa.Bounds(mat, nvec, mode)
+    out = nodes.propagate_nodes(base, 0, ptr, cols, vals, vals, sa.default_params(maxrounds=maxrounds))
+    assert out["rounds"].max() <= maxrounds
+    for v in range(nvec):
+        check_node_out(om, rl, ru, ptr, cols, vals, out, v, check_rounds=True,
+                       oparams=orc.params(maxrounds=maxrounds))
+    full = nodes.propagate_nodes(base, 0, ptr, cols, vals, vals, sa.default_params())
+    assert full["rounds"].max() > maxrounds   # the cap binds for some node
     nodes.close()
     base.close()
     mat.close()
