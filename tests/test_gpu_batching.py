@@ -124,6 +124,37 @@ def test_nodes_maxrounds(gpu_ctx, mode, maxrounds):
     mat.close()
 
 
+@pytest.mark.parametrize("mode", [sa.SAP_MODE_DENSE, sa.SAP_MODE_SPARSE])
+def test_nodes_host_loop_and_probe_equal_graph_rounds(gpu_ctx, mode):
+    # the host-driven round loop reads the outcomes and change lists after its rounds, the
+    # graph-driven one with its last wait: same answers
+    inst, om, rl, ru, ptr, cols, vals = node_problem(16)
+    nvec = len(ptr) - 1
+    say(f"[case] nodes host loop mode={mode} nvec={nvec}")
+    mat = sa.Matrix(gpu_ctx, inst)
+    base = sa.Bounds(mat, 1)
+    base.upload(rl, ru)
+    nodes = sa.Bounds(mat, nvec, mode)
+    for flags in (0, sa.SAP_FLAG_BASE_IS_FIXPOINT):
+        a = nodes.propagate_nodes(base, 0, ptr, cols, vals, vals, sa.default_params(flags=flags))
+        b = nodes.propagate_nodes(base, 0, ptr, cols, vals, vals,
+                                  sa.default_params(flags=flags | sa.SAP_FLAG_HOSTLOOP))
+        for k in ("result", "chgptr", "chgcol", "chglb", "chgub"):
+            assert np.array_equal(a[k], b[k]), (flags, k)
+        if flags == 0:
+            assert np.array_equal(a["rounds"], b["rounds"])
+            for v in range(nvec):
+                check_node_out(om, rl, ru, ptr, cols, vals, b, v, check_rounds=True)
+    free = np.nonzero(rl < ru)[0].astype(np.int32)[:24]
+    pa = nodes.probe(base, 0, free, sa.default_params())
+    pb = nodes.probe(base, 0, free, sa.default_params(flags=sa.SAP_FLAG_HOSTLOOP))
+    for k in ("verdict", "rounds", "dedptr", "dedcol", "dlb0", "dub0", "dlb1", "dub1"):
+        assert np.array_equal(pa[k], pb[k]), k
+    nodes.close()
+    base.close()
+    mat.close()
+
+
 def tight_long_rows(rng, nrows, ncols):
     """Binary rows, three in four of 2..399 nonzeros, with sides at most 3 from the activity
     of a hidden 0/1 point: heavy fixings cut many nodes off, in different rounds."""
