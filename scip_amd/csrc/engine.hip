@@ -5811,19 +5811,36 @@ static int initFromBase(sap_bounds* b, int32_t n, const int32_t* hslots, const s
       (*b->srcgen)[v] = B != NULL ? B->gen : 0;
    }
    // both lists use slotidx2: the second upload is queued behind the first list's kernel on
-   // the same stream, so it cannot overwrite the list that kernel reads
+   // the same stream, so it cannot overwrite the list that kernel reads.  A list that is
+   // vectors 0..k-1 in order is b->ident's first k entries and needs no upload.
+   auto leading = [](const std::vector<int32_t>& l) {
+      for( size_t i = 0; i < l.size(); ++i )
+         if( l[i] != (int32_t)i )
+            return false;
+      return true;
+   };
    if( !rest.empty() )
    {
-      RCCHK(upload(b, b->slotidx2, rest.data(), sizeof(int32_t) * rest.size()));
-      k_slot_restore<<<slotGrid(1024, (int32_t)rest.size()), SAP_BLOCK, 0, s>>>(b->V, dbase, m->ncols, b->slotidx2,
+      const int32_t* d = b->ident;
+      if( !leading(rest) )
+      {
+         RCCHK(upload(b, b->slotidx2, rest.data(), sizeof(int32_t) * rest.size()));
+         d = b->slotidx2;
+      }
+      k_slot_restore<<<slotGrid(1024, (int32_t)rest.size()), SAP_BLOCK, 0, s>>>(b->V, dbase, m->ncols, d,
          (int32_t)rest.size());
       HIPCHK(hipGetLastError());
    }
    if( !full.empty() )
    {
-      RCCHK(upload(b, b->slotidx2, full.data(), sizeof(int32_t) * full.size()));
+      const int32_t* d = b->ident;
+      if( !leading(full) )
+      {
+         RCCHK(upload(b, b->slotidx2, full.data(), sizeof(int32_t) * full.size()));
+         d = b->slotidx2;
+      }
       k_slot_base<<<slotGrid((size_t)m->ncols, (int32_t)full.size()), SAP_BLOCK, 0, s>>>(b->V.bnd, dbase, m->ncols,
-         b->slotidx2, (int32_t)full.size());
+         d, (int32_t)full.size());
       HIPCHK(hipGetLastError());
    }
    return SAP_OKAY;
