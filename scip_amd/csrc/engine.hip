@@ -3588,6 +3588,7 @@ struct sap_context
    Status* hstatus;   // pinned, mapped and coherent: the status after a round group ...
    Status* hstatusd;  // ... written by a copy kernel through this device address
    unsigned char* scratch;   // this device's address of the no-allocation scratch array
+   int32_t resident;         // sweep workgroups the device holds at once (0: unknown)
 };
 
 struct sap_matrix
@@ -4035,6 +4036,11 @@ int sap_context_create(int32_t device, sap_context** out)
    // the device address of the no-allocation entry point's scratch array on this device (each
    // device holds its own copy; the symbol's name is not a device address in host code)
    HIPCHK(hipGetSymbolAddress((void**)&c->scratch, HIP_SYMBOL(g_noalloc_scratch)));
+   int ncu = 0, per = 0;
+   if( hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess
+      && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sweep<false>, SAP_BLOCK, 0) == hipSuccess && ncu > 0
+      && per > 0 )
+      c->resident = ncu * per;
    *out = c;
    return SAP_OKAY;
 }
@@ -4939,6 +4945,9 @@ static Layout makeLayout(const sap_bounds* b)
       const size_t cap = std::max<size_t>(64, (size_t)kCap[q] * (size_t)m->tuning.capscale / 100);
       if( blocks > cap )
          blocks = cap;
+      // the tiles' grid-stride loop runs in generations of resident workgroups: at most one
+      if( q == 0 && m->ctx->resident > 0 && blocks > (size_t)m->ctx->resident )
+         blocks = (size_t)m->ctx->resident;
       acc += (int)blocks;
    }
    L.blockstart[SAP_NKINDS] = acc;
