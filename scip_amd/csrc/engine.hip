@@ -3712,6 +3712,7 @@ struct sap_bounds
    // vectors 0..logmax-1 may hold change-log bits that no collection has cleared yet
    int32_t logmax;
    int32_t begun;   // the last beginCall also began the call's first round
+   int32_t lastrounds;   // rounds of the last graph-driven call on b (predicts the next one's)
 };
 
 // the fields of the vector state that one call narrows (the vectors it works on, the
@@ -5204,8 +5205,9 @@ static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats, int32_t m
       }
       // the first group's one round is enqueued directly (measured 0.5-1.5 % faster per call
       // than launching a one-round graph); then graphs of 2, 4 and 8 rounds.  A call on several
-      // vectors rarely ends after its first round: its two-round graph follows without a wait
-      // (rounds after the end cost only their early exits)
+      // vectors rarely ends after its first round, nor does one on a bound set whose last call
+      // took more: its two-round graph follows without a wait (rounds after the end cost only
+      // their early exits)
       static const int kSchedule[3] = { 0, 2, 1 };   // 2, 4, 8 rounds
       if( it == 0 )
       {
@@ -5214,7 +5216,7 @@ static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats, int32_t m
          enqueueRound(b, M0, P, L0, s, NULL, NULL, b->begun != 0);
          b->begun = 0;
          HIPCHK(hipGetLastError());
-         if( b->V.nvec > 1 )
+         if( b->V.nvec > 1 || b->lastrounds > 1 )
          {
             HIPCHK(hipGraphLaunch(b->graph[kSchedule[0]].exec, s));
             it = 1;
@@ -5255,6 +5257,7 @@ static int runRounds(sap_bounds* b, const Params& P, sap_stats* stats, int32_t m
    // not time on this runtime)
    sweepms = (double)c->hstatus->sweep_ticks / (double)c->wallclock_khz;
    launches = (int64_t)c->hstatus->sweep_count;
+   b->lastrounds = c->hstatus->rlocal;
    (void)prev;
    float tot = 0.0f;
    HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[1]));
